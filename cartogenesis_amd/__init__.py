@@ -1,0 +1,99 @@
+"""MI355X-native (gfx950) ocean-circulation multigrid solver.
+
+The product is ``libcgocean.so``: hand-written HIP kernels and a C ABI declared in
+``include/cgocean.h``. This package loads it with ctypes and mirrors the reference's Kotlin
+interface for the solve (OceanAccelerator, OceanStencil, OceanCirculation) in ``ocean.py``.
+Importing the package never falls back to anything: if the library is missing, ``lib()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libcgocean.so"
+
+# Status codes and methods, as in include/cgocean.h.
+CG_OK, CG_DECLINED, CG_CANCELLED = 0, 1, 2
+CG_ERR_ARG, CG_ERR_DEVICE, CG_ERR_NOMEM, CG_ERR_STATE = -1, -2, -3, -4
+CG_METHOD_REFERENCE, CG_METHOD_RESCUE, CG_METHOD_REFINED, CG_METHOD_FGMRES = 0, 1, 2, 3
+CG_OPT_MIN_CELLS, CG_OPT_TILE_ROWS, CG_OPT_USE_GRAPH = 1, 2, 3
+CG_OPT_FGMRES_RESTART, CG_OPT_STEPS_PER_CHECK, CG_OPT_TIME_KERNELS = 4, 5, 6
+
+
+class CgSolution(ctypes.Structure):
+    _fields_ = [("cycles", ctypes.c_int), ("relative_residual", ctypes.c_double),
+                ("refinements", ctypes.c_int), ("status", ctypes.c_int)]
+
+
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_vp = ctypes.c_void_p
+_int, _dbl, _long, _u64 = ctypes.c_int, ctypes.c_double, ctypes.c_long, ctypes.c_uint64
+
+# Every entry point of include/cgocean.h with its ctypes signature: (restype, argtypes).
+SIGNATURES = {
+    "cg_ocean_open": (_int, [_int, ctypes.POINTER(_vp), ctypes.c_char_p, ctypes.c_size_t]),
+    "cg_ocean_close": (None, [_vp]),
+    "cg_ocean_name": (_int, [_vp, ctypes.c_char_p, ctypes.c_size_t]),
+    "cg_ocean_last_error": (ctypes.c_char_p, [_vp]),
+    "cg_ocean_set_option": (_int, [_vp, _int, _long]),
+    "cg_ocean_get_option": (_long, [_vp, _int]),
+    "cg_ocean_stencil_upload": (_int, [_vp, _int, _int, _u8p, _f32p, _f32p, _f32p, _f32p, _vp,
+                                       ctypes.POINTER(_u64)]),
+    "cg_ocean_stencil_release": (_int, [_vp, _u64]),
+    "cg_ocean_relax": (_int, [_vp, _u64, _f32p, _f32p, _f32p, _int, _vp]),
+    "cg_ocean_solve": (_int, [_vp, ctypes.POINTER(_u64), _int, _f32p, _vp, _dbl, _int, _int, _vp, _vp,
+                              ctypes.POINTER(CgSolution), _vp]),
+    "cg_ocean_plan_create": (_int, [_vp, ctypes.POINTER(_u64), _int, _int, ctypes.POINTER(_vp)]),
+    "cg_ocean_plan_destroy": (_int, [_vp]),
+    "cg_ocean_plan_set_forcing": (_int, [_vp, _f32p]),
+    "cg_ocean_plan_solve": (_int, [_vp, _dbl, _int, _int, ctypes.POINTER(CgSolution)]),
+    "cg_ocean_plan_solve_batch": (_int, [ctypes.POINTER(_vp), _int, _dbl, _int, ctypes.POINTER(CgSolution)]),
+    "cg_ocean_plan_read": (_int, [_vp, _vp, _vp]),
+    "cg_ocean_kernel_time_us": (_dbl, [_vp, _int, ctypes.POINTER(_long)]),
+    "cg_bottom_drag": (_dbl, []),
+    "cg_beta_rows": (_int, [_int, _dbl, _f64p]),
+    "cg_circulation_stencil": (_int, [_int, _int, _dbl, _dbl, _u8p, _f64p, _dbl, _vp, _f32p, _f32p, _f32p,
+                                      _f32p, _f64p, _f32p]),
+    "cg_coarse_water": (_int, [_u8p, _int, _int, _int, _int, _int, _u8p]),
+    "cg_level_shapes": (_int, [_int, _int, _dbl, _dbl, _int, _i32p, _i32p, _f64p, _f64p]),
+    "cg_problem_generate": (_int, [_int, _int, _u64, _dbl, _int, _vp, _vp, _vp]),
+}
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded HIP library. Raises if it has not been built: there is no fallback."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m cartogenesis_amd.build` "
+                           "(or __graft_entry__.build()); there is no CPU fallback")
+    L = ctypes.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+class CgError(RuntimeError):
+    pass
+
+
+def check(status: int, ctx=None, what: str = "") -> int:
+    if status < 0:
+        msg = lib().cg_ocean_last_error(ctx).decode() if ctx else ""
+        raise CgError(f"{what} failed with status {status}: {msg}")
+    return status
+
+
+from . import ocean, problem  # noqa: E402,F401

@@ -1,0 +1,240 @@
+// Host-side problem builders of the ocean solver: the circulation stencil, coarse masks, the level
+// ladder's shapes, beta per row, the bottom drag, and the synthetic workload generator.
+//
+// Compiled with -ffp-contract=off so every double and float operation rounds once, in the order
+// written, as the reference's JVM arithmetic does (the reference is Kotlin; JVM floating point is
+// strict IEEE with no fused multiply-add).
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "cgocean.h"
+
+namespace {
+
+// WorldScale.ROTATION_RATE_PER_S (WGC:302): a Float constant, promoted to double where used.
+constexpr float kRotationRateF = 7.2921159e-5f;
+// OceanStage constants (OS:64, 80, 83, 91).
+constexpr double kSeawaterDensity = 1025.0;
+constexpr double kEarthMeanRadius = 6.371e6;
+constexpr double kGulfStreamLatitudeDeg = 30.0;
+constexpr double kGulfStreamWidth = 50000.0;
+// OceanCirculation constants (OC:88, 104, 122).
+constexpr double kSeriesBelowCellPeclet = 1e-4;
+constexpr int kCoarsestRows = 8;
+constexpr double kSquareEnough = 1.25;
+
+// splitmix64 (Steele, Lea and Flood 2014): the generator's only source of randomness.
+inline uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// A uniform double in [0, 1) from 53 bits of the hash of (seed, stream, index).
+inline double uniform01(uint64_t seed, uint64_t stream, uint64_t index) {
+    uint64_t h = splitmix64(splitmix64(seed ^ (stream * 0xD1B54A32D192ED03ull)) + index);
+    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+
+inline double smooth(double t) { return t * t * (3.0 - 2.0 * t); }
+
+// One octave of value noise at the centre of cell (row, col): a lattice of nx by ny cells over the
+// whole map, periodic in x, with a seeded value at each lattice point, blended by smoothstep.
+double octave(uint64_t seed, int o, int nx, int ny, int across, int down, int row, int col) {
+    double u = ((double)col + 0.5) / (double)across * nx;
+    double v = ((double)row + 0.5) / (double)down * ny;
+    int i0 = (int)std::floor(u), j0 = (int)std::floor(v);
+    double fu = smooth(u - i0), fv = smooth(v - j0);
+    int i1 = (i0 + 1) % nx;
+    int j1 = std::min(j0 + 1, ny);
+    uint64_t stream = 100 + (uint64_t)o;
+    auto at = [&](int i, int j) { return uniform01(seed, stream, (uint64_t)j * (uint64_t)(nx + 1) + (uint64_t)i); };
+    double top = at(i0, j0) * (1.0 - fu) + at(i1, j0) * fu;
+    double bottom = at(i0, j1) * (1.0 - fu) + at(i1, j1) * fu;
+    return top * (1.0 - fv) + bottom * fv;
+}
+
+}  // namespace
+
+extern "C" {
+
+// OceanStage.BOTTOM_DRAG_PER_S (OS:102-103): 2 Omega cos(30 deg) / a_Earth * 50 km, with Omega the
+// Float rotation rate promoted to double, evaluated left to right.
+double cg_bottom_drag(void) {
+    return 2.0 * (double)kRotationRateF * std::cos(kGulfStreamLatitudeDeg * M_PI / 180.0) / kEarthMeanRadius *
+           kGulfStreamWidth;
+}
+
+// WorldScale.planetaryVorticityGradientPerMeterSecond (WGC:272-273) at ClimateStage.latitudeOf's
+// binary32 row latitude (ClimateStage.kt:1313-1314), with radiusMeters = worldWidthKm * 1000f / 2 pi
+// (WGC:246).
+int cg_beta_rows(int down, double world_width_km, double *beta) {
+    if (down <= 0 || !beta) return CG_ERR_ARG;
+    double radius = world_width_km * (double)1000.0f / (2.0 * M_PI);
+    for (int row = 0; row < down; row++) {
+        float lat = 90.0f - 180.0f * ((float)row + 0.5f) / (float)down;
+        beta[row] = 2.0 * (double)kRotationRateF * std::cos((double)lat * M_PI / 180.0) / radius;
+    }
+    return CG_OK;
+}
+
+// OceanCirculation.stencil (OC:161-219) and withBalance (OC:222-231).
+int cg_circulation_stencil(int across, int down, double dx, double dy, const uint8_t *water,
+                           const double *beta, double drag, const double *balance, float *east,
+                           float *west, float *north, float *south, double *centre, float *forcing) {
+    if (across <= 0 || down <= 0 || !water || !beta || !east || !west || !north || !south || !centre || !forcing)
+        return CG_ERR_ARG;
+    const double along_y = drag / (dy * dy);
+    const double width_sq = dx * dx;
+    for (int row = 0; row < down; row++) {
+        const double b = beta[row];
+        const double peclet = std::fabs(b) * dx / (2.0 * drag);
+        double ex, wx;
+        if (peclet < kSeriesBelowCellPeclet) {
+            const double fitted = drag * (1.0 + peclet * peclet / 3.0) / width_sq;
+            const double upstream = std::fabs(b) / (2.0 * dx);
+            ex = fitted + upstream;
+            wx = fitted - upstream;
+        } else {
+            // r p (coth p +- 1) / dx^2 with coth p - 1 = 2 / expm1(2p).
+            const double scale = drag * peclet / width_sq;
+            const double coth_less_one = 2.0 / std::expm1(2.0 * peclet);
+            ex = scale * (2.0 + coth_less_one);
+            wx = scale * coth_less_one;
+        }
+        // Pole walls fold into the edge rows' centre weight.
+        const double ny = row == 0 ? 0.0 : along_y;
+        const double sy = row == down - 1 ? 0.0 : along_y;
+        const double c = ex + wx + 2.0 * along_y + (along_y - ny) + (along_y - sy);
+        const float fe = (float)(ex / c), fw = (float)(wx / c), fn = (float)(ny / c), fs = (float)(sy / c);
+        for (long cell = (long)row * across; cell < (long)(row + 1) * across; cell++) {
+            east[cell] = fe;
+            west[cell] = fw;
+            north[cell] = fn;
+            south[cell] = fs;
+            centre[cell] = c;
+        }
+    }
+    const long cells = (long)across * down;
+    for (long cell = 0; cell < cells; cell++)
+        forcing[cell] = (water[cell] && balance) ? (float)(balance[cell] / centre[cell]) : 0.0f;
+    return CG_OK;
+}
+
+// OceanCirculation.coarseWater (OC:404-423).
+int cg_coarse_water(const uint8_t *fine, int fa, int fd, int ca, int cd, int every_cell_water, uint8_t *coarse) {
+    if (!fine || !coarse || fa <= 0 || fd <= 0 || ca <= 0 || cd <= 0 || fa % ca || fd % cd) return CG_ERR_ARG;
+    const int cpc = fa / ca, rpc = fd / cd;
+    const uint8_t ev = every_cell_water ? 1 : 0;
+    std::memset(coarse, ev, (size_t)ca * cd);
+    for (int row = 0; row < fd; row++)
+        for (int col = 0; col < fa; col++)
+            if ((fine[(long)row * fa + col] ? 1 : 0) != ev) coarse[(long)(row / rpc) * ca + col / cpc] = (uint8_t)!ev;
+    return CG_OK;
+}
+
+// The shapes OceanCirculation.levels walks (OC:434-462).
+int cg_level_shapes(int across, int down, double dx, double dy, int max_levels, int *la, int *ld, double *cw,
+                    double *ch) {
+    if (across <= 0 || down <= 0 || max_levels < 1 || !la || !ld) return CG_ERR_ARG;
+    int count = 1;
+    la[0] = across;
+    ld[0] = down;
+    if (cw) cw[0] = dx;
+    if (ch) ch[0] = dy;
+    double w = dx, h = dy;
+    while (count < max_levels) {
+        const bool wide = w > h * kSquareEnough;
+        const bool tall = h > w * kSquareEnough;
+        const int na = wide ? across : across / 2;
+        const int nd = tall ? down : down / 2;
+        const bool even = (na == across || across % 2 == 0) && (nd == down || down % 2 == 0);
+        if (!even || nd < kCoarsestRows || na % 2 != 0) break;
+        w *= (double)across / na;
+        h *= (double)down / nd;
+        across = na;
+        down = nd;
+        la[count] = across;
+        ld[count] = down;
+        if (cw) cw[count] = w;
+        if (ch) ch[count] = h;
+        count++;
+    }
+    return count;
+}
+
+// The synthetic workload. The curl follows OceanStage.curlForcing (OS:689-708): central
+// differences, columns wrapping, edge rows one-sided, with per-cell depth H in rho H.
+int cg_problem_generate(int across, int down, uint64_t seed, double land_fraction, int month, uint8_t *water,
+                        double *depth, double *balance) {
+    if (across <= 0 || down <= 0 || !(land_fraction >= 0.0 && land_fraction < 1.0) || month > 11) return CG_ERR_ARG;
+    const long cells = (long)across * down;
+    // Land mask: 4 octaves of value noise, lattices 8x4, 16x8, 32x16, 64x32, amplitude halving.
+    std::vector<double> noise(cells);
+    for (int row = 0; row < down; row++)
+        for (int col = 0; col < across; col++) {
+            double total = 0.0, amp = 1.0;
+            for (int o = 0; o < 4; o++) {
+                total += amp * octave(seed, o, 8 << o, 4 << o, across, down, row, col);
+                amp *= 0.5;
+            }
+            noise[(long)row * across + col] = total;
+        }
+    // The lowest land_fraction of the noise is land: exactly round(land_fraction * cells) cells.
+    const long land_cells = std::lround(land_fraction * (double)cells);
+    std::vector<uint8_t> wet(cells, 1);
+    if (land_cells > 0) {
+        std::vector<double> sorted(noise);
+        std::nth_element(sorted.begin(), sorted.begin() + (land_cells - 1), sorted.end());
+        const double threshold = sorted[land_cells - 1];
+        for (long i = 0; i < cells; i++) wet[i] = noise[i] > threshold ? 1 : 0;
+    }
+    // Depth: i.i.d. uniform 200..6000 m on water, 0 on land.
+    std::vector<double> h(cells, 0.0);
+    for (long i = 0; i < cells; i++)
+        if (wet[i]) h[i] = 200.0 + 5800.0 * uniform01(seed, 1, (uint64_t)i);
+    // Zonal stress: tau_x = amplitude cos(3 (lat - shift)) + N(0, 0.005^2) per cell.
+    double amplitude = 0.1, shift_deg = 0.0;
+    if (month >= 0) {
+        amplitude = 0.1 * std::cos(2.0 * M_PI * month / 12.0);
+        shift_deg = 10.0 * std::sin(2.0 * M_PI * month / 12.0);
+    }
+    std::vector<double> tau(cells);
+    for (int row = 0; row < down; row++) {
+        const float lat = 90.0f - 180.0f * ((float)row + 0.5f) / (float)down;
+        const double band = amplitude * std::cos(3.0 * ((double)lat - shift_deg) * M_PI / 180.0);
+        for (int col = 0; col < across; col++) {
+            const long i = (long)row * across + col;
+            // Box-Muller from two uniforms of the cell.
+            const double u1 = 1.0 - uniform01(seed, 2, (uint64_t)i);
+            const double u2 = uniform01(seed, 3, (uint64_t)i);
+            const double g = std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+            tau[i] = band + 0.005 * g;
+        }
+    }
+    const double world_width_m = 40075.0 * 1000.0;
+    const double height_m = world_width_m * 0.5 / down;
+    if (balance) {
+        for (int row = 0; row < down; row++) {
+            const int rn = std::max(row - 1, 0), rs = std::min(row + 1, down - 1);
+            const double ns_m = (double)(rs - rn) * height_m;
+            for (int col = 0; col < across; col++) {
+                const long i = (long)row * across + col;
+                if (!wet[i]) { balance[i] = 0.0; continue; }
+                // tau_y = 0, so d(tau_y)/dx is (0 - 0) / 2dx = 0.
+                const double dtn_dx = 0.0;
+                const double dte_dy = (tau[(long)rn * across + col] - tau[(long)rs * across + col]) / ns_m;
+                balance[i] = (dtn_dx - dte_dy) * (1.0 / (kSeawaterDensity * h[i]));
+            }
+        }
+    }
+    if (water) std::memcpy(water, wet.data(), (size_t)cells);
+    if (depth) std::memcpy(depth, h.data(), sizeof(double) * (size_t)cells);
+    return CG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,1093 @@
+// The ocean solver's runtime and C ABI: device contexts, resident stencils, the relaxation seam,
+// device-resident solve plans (reference-exact multigrid, binary32-in-binary64 refinement, FGMRES),
+// hipGraph replay and kernel timing. See include/cgocean.h for the contract of each entry point.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "cgocean.h"
+#include "kernels.hpp"
+
+using namespace cg;
+
+namespace {
+
+constexpr int kMaxLevels = 24;
+constexpr int kMostCycles = 200;          // OC:116
+constexpr double kCoarsestShare = 0.1;    // OC:110
+constexpr int kPrecondCoarsestPasses = 200;  // OC:629
+constexpr int kKrylovRestarts = 3;        // OC:535
+constexpr int kDirectMaxWater = 2048;     // largest coarsest grid solved by a dense inverse
+constexpr int kRefinedMostSteps = 1000;
+
+struct Stencil {
+    int across = 0, down = 0;
+    long cells = 0;
+    bool rowconst = false;
+    bool has_centre = false;
+    uint8_t *water = nullptr;
+    float4 *roww = nullptr;
+    float4 *cellw = nullptr;
+    double *rowc = nullptr;
+    double *cellc = nullptr;
+    float *fbuf = nullptr;  // the seam's forcing
+    float *xbuf[2] = {nullptr, nullptr};
+    // Host copies, for the coarsest grid's direct solve.
+    std::vector<uint8_t> h_water;
+    std::vector<float4> h_w;   // per row or per cell
+    std::vector<double> h_c;   // per row or per cell
+
+    LevelArgs args() const {
+        LevelArgs a;
+        a.across = across;
+        a.down = down;
+        a.water = water;
+        a.roww = rowconst ? roww : nullptr;
+        a.cellw = rowconst ? nullptr : cellw;
+        a.rowc = rowconst ? rowc : nullptr;
+        a.cellc = rowconst ? nullptr : cellc;
+        return a;
+    }
+};
+
+struct TimedLaunch {
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct cg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::recursive_mutex mu;
+    std::string err;
+    std::string name;
+    long min_cells = 1L << 16;
+    int tile_rows = 16;
+    int use_graph = 1;
+    int fgmres_m = 20;
+    int steps_per_check = 1;
+    int time_kernels = 0;
+    std::map<uint64_t, Stencil *> stencils;
+    uint64_t next_handle = 1;
+    unsigned long long *h_pinned = nullptr;  // host-visible scalars
+    std::vector<TimedLaunch> timed;
+    size_t timed_used = 0;
+    long timed_dropped = 0;
+};
+
+namespace {
+
+void set_err(cg_ctx *ctx, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->err = buf;
+}
+
+#define HIPCHK(ctx, call)                                                                      \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            set_err(ctx, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return CG_ERR_DEVICE;                                                              \
+        }                                                                                      \
+    } while (0)
+
+inline unsigned blocks_for(long n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+template <typename T>
+int dalloc(cg_ctx *ctx, T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * count);
+    if (e != hipSuccess) {
+        set_err(ctx, "hipMalloc of %zu bytes failed: %s", sizeof(T) * count, hipGetErrorString(e));
+        *p = nullptr;
+        return CG_ERR_NOMEM;
+    }
+    return CG_OK;
+}
+
+void dfree(void *p) {
+    if (p) (void)hipFree(p);
+}
+
+void free_stencil(Stencil *s) {
+    if (!s) return;
+    dfree(s->water);
+    dfree(s->roww);
+    dfree(s->cellw);
+    dfree(s->rowc);
+    dfree(s->cellc);
+    dfree(s->fbuf);
+    dfree(s->xbuf[0]);
+    dfree(s->xbuf[1]);
+    delete s;
+}
+
+Stencil *find_stencil(cg_ctx *ctx, cg_stencil h) {
+    auto it = ctx->stencils.find(h);
+    return it == ctx->stencils.end() ? nullptr : it->second;
+}
+
+// Event-timed launch bracket for kernel-duration records on the context's stream.
+struct TimeScope {
+    cg_ctx *ctx;
+    TimedLaunch *tl = nullptr;
+    TimeScope(cg_ctx *c, bool on) : ctx(c) {
+        if (!on || !ctx->time_kernels) return;
+        if (ctx->timed_used >= 200000) {
+            ctx->timed_dropped++;
+            return;
+        }
+        if (ctx->timed_used == ctx->timed.size()) {
+            TimedLaunch t;
+            if (hipEventCreate(&t.a) != hipSuccess || hipEventCreate(&t.b) != hipSuccess) return;
+            ctx->timed.push_back(t);
+        }
+        tl = &ctx->timed[ctx->timed_used++];
+        (void)hipEventRecord(tl->a, ctx->stream);
+    }
+    ~TimeScope() {
+        if (tl) (void)hipEventRecord(tl->b, ctx->stream);
+    }
+};
+
+// The fused smoother: `passes` red-black passes of a row-constant stencil from xin into xout,
+// in launches of two passes (and one of one pass for an odd count), ping-ponging between xin and
+// xout so that the final values land in xout. xin is read only when zero_start is false. For
+// per-cell weights the colour kernel runs in place on xout after copying xin.
+template <int TH>
+hipError_t launch_tile(hipStream_t s, const LevelArgs &L, const float *f, const float *xin, float *xout, int nh,
+                       bool zero) {
+    constexpr int NHMAX = 4;
+    const int hc = (nh + 1) & ~1;
+    const int iw = 128 - 2 * hc;
+    const long waves = (long)((L.across + iw - 1) / iw) * ((L.down + TH - 1) / TH);
+    const unsigned blocks = (unsigned)((waves + 3) / 4);
+    (void)NHMAX;
+    if (nh == 4) {
+        if (zero) hipLaunchKernelGGL((k_relax_tile<4, TH, true>), dim3(blocks), dim3(256), 0, s, L, f, xin, xout);
+        else hipLaunchKernelGGL((k_relax_tile<4, TH, false>), dim3(blocks), dim3(256), 0, s, L, f, xin, xout);
+    } else {
+        if (zero) hipLaunchKernelGGL((k_relax_tile<2, TH, true>), dim3(blocks), dim3(256), 0, s, L, f, xin, xout);
+        else hipLaunchKernelGGL((k_relax_tile<2, TH, false>), dim3(blocks), dim3(256), 0, s, L, f, xin, xout);
+    }
+    return hipGetLastError();
+}
+
+hipError_t tile_dispatch(cg_ctx *ctx, const LevelArgs &L, const float *f, const float *xin, float *xout, int nh,
+                         bool zero) {
+    switch (ctx->tile_rows) {
+        case 8: return launch_tile<8>(ctx->stream, L, f, xin, xout, nh, zero);
+        case 24: return launch_tile<24>(ctx->stream, L, f, xin, xout, nh, zero);
+        case 32: return launch_tile<32>(ctx->stream, L, f, xin, xout, nh, zero);
+        default: return launch_tile<16>(ctx->stream, L, f, xin, xout, nh, zero);
+    }
+}
+
+// Runs `passes` passes on level values held in a ping-pong pair x[0], x[1] with x[*cur] current;
+// on return *cur names the buffer holding the result. zero_start: the current values are all zero
+// and need not be read. timed: record the launches under kernel kind 0.
+int relax_pp(cg_ctx *ctx, const Stencil *st, const float *f, float *x[2], int *cur, int passes, bool zero_start,
+             bool timed) {
+    if (passes <= 0) return CG_OK;
+    const LevelArgs L = st->args();
+    if (st->rowconst && st->across >= 2 && (st->across & 1) == 0) {
+        int left = passes;
+        bool zero = zero_start;
+        while (left > 0) {
+            const int nh = left >= 2 ? 4 : 2;
+            TimeScope ts(ctx, timed);
+            hipError_t e = tile_dispatch(ctx, L, f, x[*cur], x[1 - *cur], nh, zero);
+            if (e != hipSuccess) {
+                set_err(ctx, "smoother launch failed: %s", hipGetErrorString(e));
+                return CG_ERR_DEVICE;
+            }
+            *cur = 1 - *cur;
+            left -= nh / 2;
+            zero = false;
+        }
+        return CG_OK;
+    }
+    if (zero_start) {
+        hipLaunchKernelGGL(k_fill_f32, dim3(blocks_for(st->cells, 256)), dim3(256), 0, ctx->stream, x[*cur], st->cells,
+                           0.0f);
+    }
+    const long half = (long)(st->across / 2) * st->down;
+    for (int p = 0; p < passes; ++p)
+        for (int colour = 0; colour <= 1; ++colour)
+            hipLaunchKernelGGL(k_relax_colour, dim3(blocks_for(half, 256)), dim3(256), 0, ctx->stream, L, f, x[*cur],
+                               colour);
+    HIPCHK(ctx, hipGetLastError());
+    return CG_OK;
+}
+
+}  // namespace
+
+// ================================================================================================
+// Plans
+// ================================================================================================
+
+struct PlanLevel {
+    Stencil *st = nullptr;
+    float *f = nullptr;         // this level's forcing (coarse levels: restricted; level 0: correction)
+    float *x[2] = {nullptr, nullptr};
+    int cur = 0;
+    double *bal = nullptr;      // restricted balance (reference method), this level's size
+};
+
+struct cg_plan {
+    cg_ctx *ctx = nullptr;
+    int n = 0;
+    int pole_is_wall = 1;
+    PlanLevel lv[kMaxLevels];
+    float *f0 = nullptr;          // the finest forcing
+    double *r64 = nullptr;        // finest-size binary64 scratch
+    double *x64[2] = {nullptr, nullptr};
+    int x64cur = 0;
+    bool x64_valid = false;
+    float *x32ref = nullptr;      // reference-method answer lives in lv[0].x[lv[0].cur]
+    unsigned long long *d_u64 = nullptr;  // [0] residual max, [1] balance max, [2..] scratch
+    int *d_passes = nullptr;
+    double largest = -1.0;        // max |f0 * c0|, cached per forcing
+    // Coarsest direct solve.
+    int nw = 0;
+    int *d_idx = nullptr;
+    float *d_Mt = nullptr;
+    // Krylov workspace (binary64, finest size), allocated on first use.
+    std::vector<double *> kv;
+    // Graphs of one refinement step, for each parity of the binary64 ping-pong.
+    hipGraphExec_t step_exec[2] = {nullptr, nullptr};
+    int last_answer = 0;  // 0: reference (binary32 in lv[0]); 1: binary64 in x64[x64cur]
+};
+
+namespace {
+
+void free_plan(cg_plan *p) {
+    if (!p) return;
+    for (int i = 0; i < p->n; ++i) {
+        dfree(p->lv[i].f);
+        dfree(p->lv[i].x[0]);
+        dfree(p->lv[i].x[1]);
+        dfree(p->lv[i].bal);
+    }
+    dfree(p->f0);
+    dfree(p->r64);
+    dfree(p->x64[0]);
+    dfree(p->x64[1]);
+    dfree(p->d_u64);
+    dfree(p->d_passes);
+    dfree(p->d_idx);
+    dfree(p->d_Mt);
+    for (double *v : p->kv) dfree(v);
+    for (int i = 0; i < 2; ++i)
+        if (p->step_exec[i]) (void)hipGraphExecDestroy(p->step_exec[i]);
+    delete p;
+}
+
+// The coarsest grid's dense inverse, M = -(I - W)^-1 over its water cells, by Gauss-Jordan
+// elimination with partial pivoting in binary64, stored column-major in binary32.
+int build_coarsest_inverse(cg_ctx *ctx, cg_plan *p) {
+    const Stencil *st = p->lv[p->n - 1].st;
+    const int across = st->across, down = st->down;
+    std::vector<int> idx;
+    std::vector<int> pos(st->cells, -1);
+    for (long c = 0; c < st->cells; ++c)
+        if (st->h_water[c]) {
+            pos[c] = (int)idx.size();
+            idx.push_back((int)c);
+        }
+    const int n = (int)idx.size();
+    p->nw = n;
+    if (n == 0 || n > kDirectMaxWater) return CG_OK;
+    std::vector<double> a((size_t)n * n, 0.0), inv((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        const int cell = idx[i];
+        const int row = cell / across, col = cell % across;
+        const float4 w = st->rowconst ? st->h_w[row] : st->h_w[cell];
+        a[(size_t)i * n + i] = 1.0;
+        auto sub = [&](int nb, double wt) {
+            if (nb >= 0 && pos[nb] >= 0) a[(size_t)i * n + pos[nb]] -= wt;
+        };
+        const int ce = col + 1 == across ? 0 : col + 1;
+        const int cw = col == 0 ? across - 1 : col - 1;
+        sub(row * across + ce, w.x);
+        sub(row * across + cw, w.y);
+        if (row > 0) sub(cell - across, w.z);
+        if (row + 1 < down) sub(cell + across, w.w);
+        inv[(size_t)i * n + i] = 1.0;
+    }
+    for (int k = 0; k < n; ++k) {
+        int piv = k;
+        for (int i = k + 1; i < n; ++i)
+            if (std::fabs(a[(size_t)i * n + k]) > std::fabs(a[(size_t)piv * n + k])) piv = i;
+        if (a[(size_t)piv * n + k] == 0.0) {
+            p->nw = 0;  // singular: fall back to relaxation
+            return CG_OK;
+        }
+        if (piv != k)
+            for (int j = 0; j < n; ++j) {
+                std::swap(a[(size_t)k * n + j], a[(size_t)piv * n + j]);
+                std::swap(inv[(size_t)k * n + j], inv[(size_t)piv * n + j]);
+            }
+        const double d = 1.0 / a[(size_t)k * n + k];
+        for (int j = 0; j < n; ++j) {
+            a[(size_t)k * n + j] *= d;
+            inv[(size_t)k * n + j] *= d;
+        }
+        for (int i = 0; i < n; ++i) {
+            if (i == k) continue;
+            const double m = a[(size_t)i * n + k];
+            if (m == 0.0) continue;
+            for (int j = 0; j < n; ++j) {
+                a[(size_t)i * n + j] -= m * a[(size_t)k * n + j];
+                inv[(size_t)i * n + j] -= m * inv[(size_t)k * n + j];
+            }
+        }
+    }
+    std::vector<float> mt((size_t)n * n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) mt[(size_t)j * n + i] = (float)(-inv[(size_t)i * n + j]);
+    int rc;
+    if ((rc = dalloc(ctx, &p->d_idx, n))) return rc;
+    if ((rc = dalloc(ctx, &p->d_Mt, (size_t)n * n))) return rc;
+    HIPCHK(ctx, hipMemcpy(p->d_idx, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(p->d_Mt, mt.data(), sizeof(float) * mt.size(), hipMemcpyHostToDevice));
+    return CG_OK;
+}
+
+int read_u64(cg_plan *p, int slot, unsigned long long *v) {
+    cg_ctx *ctx = p->ctx;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->h_pinned + slot, p->d_u64 + slot, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    *v = ctx->h_pinned[slot];
+    return CG_OK;
+}
+
+double bits_to_double(unsigned long long b) {
+    double d;
+    std::memcpy(&d, &b, sizeof d);
+    return d;
+}
+
+void zero_u64(cg_plan *p, int slot, int n) {
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, p->ctx->stream, p->d_u64 + slot, n, 0ull);
+}
+
+// max |f0 * c0|: the reference's largest balance (OC:483).
+int plan_largest(cg_plan *p, double *out) {
+    if (p->largest >= 0.0) {
+        *out = p->largest;
+        return CG_OK;
+    }
+    cg_ctx *ctx = p->ctx;
+    const Stencil *st = p->lv[0].st;
+    zero_u64(p, 1, 1);
+    hipLaunchKernelGGL(k_balance_max, dim3(blocks_for(st->cells, 256)), dim3(256), 0, ctx->stream, st->args(), p->f0,
+                       p->d_u64 + 1);
+    HIPCHK(ctx, hipGetLastError());
+    unsigned long long b;
+    int rc = read_u64(p, 1, &b);
+    if (rc) return rc;
+    p->largest = bits_to_double(b);
+    *out = p->largest;
+    return CG_OK;
+}
+
+// ---- the reference method ---------------------------------------------------------------------
+
+// The binary64 residual's largest magnitude of level 0's current binary32 values.
+int ref_residual(cg_plan *p, const float *f, double *out) {
+    cg_ctx *ctx = p->ctx;
+    const PlanLevel &l0 = p->lv[0];
+    zero_u64(p, 0, 1);
+    hipLaunchKernelGGL(k_residual64, dim3(blocks_for(l0.st->cells, 256)), dim3(256), 0, ctx->stream, l0.st->args(), f,
+                       l0.x[l0.cur], (double *)nullptr, p->d_u64);
+    HIPCHK(ctx, hipGetLastError());
+    unsigned long long b;
+    int rc = read_u64(p, 0, &b);
+    if (rc) return rc;
+    *out = bits_to_double(b);
+    return CG_OK;
+}
+
+// vCycle (OC:740-781) with level 0's forcing f0 and its values in lv[0]; fixed > 0 relaxes the
+// coarsest that many passes instead of to convergence.
+int ref_vcycle(cg_plan *p, const float *f0, double tol, int fixed) {
+    cg_ctx *ctx = p->ctx;
+    hipStream_t s = ctx->stream;
+    const int n = p->n;
+    int rc;
+    for (int l = 0; l < n - 1; ++l) {
+        PlanLevel &L = p->lv[l];
+        PlanLevel &C = p->lv[l + 1];
+        const float *f = l == 0 ? f0 : L.f;
+        if ((rc = relax_pp(ctx, L.st, f, L.x, &L.cur, 2, false, l == 0))) return rc;
+        hipLaunchKernelGGL(k_residual64, dim3(blocks_for(L.st->cells, 256)), dim3(256), 0, s, L.st->args(), f,
+                           L.x[L.cur], p->r64, (unsigned long long *)nullptr);
+        hipLaunchKernelGGL(k_restrict64, dim3(blocks_for(C.st->cells, 256)), dim3(256), 0, s, p->r64, L.st->across,
+                           L.st->down, C.st->across, C.st->down, C.bal);
+        hipLaunchKernelGGL(k_with_balance, dim3(blocks_for(C.st->cells, 256)), dim3(256), 0, s, C.st->args(), C.bal, C.f);
+        hipLaunchKernelGGL(k_fill_f32, dim3(blocks_for(C.st->cells, 256)), dim3(256), 0, s, C.x[C.cur], C.st->cells, 0.0f);
+    }
+    if (n > 1) {
+        PlanLevel &C = p->lv[n - 1];
+        hipLaunchKernelGGL(k_coarsest_relax, dim3(1), dim3(1024), 0, s, C.st->args(), C.f, C.x[C.cur],
+                           tol * kCoarsestShare, fixed, p->d_passes);
+    }
+    for (int l = n - 2; l >= 0; --l) {
+        PlanLevel &L = p->lv[l];
+        PlanLevel &C = p->lv[l + 1];
+        const float *f = l == 0 ? f0 : L.f;
+        hipLaunchKernelGGL(k_prolong_add, dim3(blocks_for(L.st->cells, 256)), dim3(256), 0, s, C.x[C.cur], C.st->across,
+                           C.st->down, L.x[L.cur], L.st->across, L.st->down, L.st->water,
+                           p->pole_is_wall ? -1.0f : 1.0f);
+        if ((rc = relax_pp(ctx, L.st, f, L.x, &L.cur, 2, false, l == 0))) return rc;
+    }
+    if (n == 1) {
+        if ((rc = relax_pp(ctx, p->lv[0].st, f0, p->lv[0].x, &p->lv[0].cur, 2, false, true))) return rc;
+    }
+    HIPCHK(ctx, hipGetLastError());
+    return CG_OK;
+}
+
+// solve (OC:473-493) from the binary32 values in lv[0].
+int ref_solve(cg_plan *p, double tol, cg_solution *info, const volatile int *cancel) {
+    double largest;
+    int rc = plan_largest(p, &largest);
+    if (rc) return rc;
+    double res = 0.0;
+    if (largest > 0.0) {
+        if ((rc = ref_residual(p, p->f0, &res))) return rc;
+        res /= largest;
+    }
+    int cycles = 0;
+    while (res >= tol && cycles < kMostCycles) {
+        if (cancel && *cancel) {
+            info->status = CG_CANCELLED;
+            break;
+        }
+        if ((rc = ref_vcycle(p, p->f0, tol, 0))) return rc;
+        cycles++;
+        if ((rc = ref_residual(p, p->f0, &res))) return rc;
+        res /= largest;
+    }
+    info->cycles = cycles;
+    info->relative_residual = res;
+    p->last_answer = 0;
+    return CG_OK;
+}
+
+// ---- the refined method ----------------------------------------------------------------------
+
+// One binary32 V(2,2) cycle on the correction problem from zero, with level 0's forcing lv[0].f.
+// The answer is left in lv[0].x[lv[0].cur].
+int vcycle32(cg_plan *p, bool timed) {
+    cg_ctx *ctx = p->ctx;
+    hipStream_t s = ctx->stream;
+    const int n = p->n;
+    int rc;
+    for (int l = 0; l < n - 1; ++l) {
+        PlanLevel &L = p->lv[l];
+        PlanLevel &C = p->lv[l + 1];
+        if ((rc = relax_pp(ctx, L.st, L.f, L.x, &L.cur, 2, true, timed && l == 0))) return rc;
+        hipLaunchKernelGGL(k_residual_restrict32, dim3(blocks_for(C.st->cells, 256)), dim3(256), 0, s, L.st->args(), L.f,
+                           L.x[L.cur], C.st->args(), C.f);
+    }
+    PlanLevel &B = p->lv[n - 1];
+    if (n == 1) {
+        if ((rc = relax_pp(ctx, B.st, B.f, B.x, &B.cur, 2, true, timed))) return rc;
+    } else if (p->nw > 0 && p->d_Mt) {
+        hipLaunchKernelGGL(k_coarsest_direct, dim3(1), dim3(256), sizeof(float) * p->nw, s, p->d_idx, p->nw, p->d_Mt, B.f,
+                           B.x[B.cur], B.st->cells);
+    } else {
+        hipLaunchKernelGGL(k_fill_f32, dim3(blocks_for(B.st->cells, 256)), dim3(256), 0, s, B.x[B.cur], B.st->cells, 0.0f);
+        hipLaunchKernelGGL(k_coarsest_relax, dim3(1), dim3(1024), 0, s, B.st->args(), B.f, B.x[B.cur], 0.0,
+                           kPrecondCoarsestPasses, p->d_passes);
+    }
+    for (int l = n - 2; l >= 0; --l) {
+        PlanLevel &L = p->lv[l];
+        PlanLevel &C = p->lv[l + 1];
+        hipLaunchKernelGGL(k_prolong_add, dim3(blocks_for(L.st->cells, 256)), dim3(256), 0, s, C.x[C.cur], C.st->across,
+                           C.st->down, L.x[L.cur], L.st->across, L.st->down, L.st->water,
+                           p->pole_is_wall ? -1.0f : 1.0f);
+        if ((rc = relax_pp(ctx, L.st, L.f, L.x, &L.cur, 2, false, timed && l == 0))) return rc;
+    }
+    HIPCHK(ctx, hipGetLastError());
+    return CG_OK;
+}
+
+// The outer binary64 step: x64[next] = x64[cur] + delta, its residual into lv[0].f, max into slot 0.
+int outer_step(cg_plan *p, const float *delta) {
+    cg_ctx *ctx = p->ctx;
+    const Stencil *st = p->lv[0].st;
+    zero_u64(p, 0, 1);
+    hipLaunchKernelGGL(k_outer_residual, dim3(blocks_for(st->cells, 256)), dim3(256), 0, ctx->stream, st->args(), p->f0,
+                       p->x64[p->x64cur], delta, p->x64[1 - p->x64cur], p->lv[0].f, p->d_u64);
+    HIPCHK(ctx, hipGetLastError());
+    p->x64cur = 1 - p->x64cur;
+    return CG_OK;
+}
+
+// One refinement step as a graph: the V-cycle on the current correction forcing, then the outer
+// step applying its answer, then the residual max copied to pinned host memory. The binary32 level
+// buffers' ping-pong parities are the same at the start of every step (each level runs an even
+// number of launches per cycle), so one graph per binary64 parity suffices.
+int build_step_graphs(cg_plan *p) {
+    cg_ctx *ctx = p->ctx;
+    for (int par = 0; par < 2; ++par) {
+        if (p->step_exec[par]) continue;
+        int saved_x64 = p->x64cur;
+        int saved_cur[kMaxLevels];
+        for (int l = 0; l < p->n; ++l) saved_cur[l] = p->lv[l].cur;
+        p->x64cur = par;
+        hipGraph_t g = nullptr;
+        HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        int rc = vcycle32(p, false);
+        if (!rc) rc = outer_step(p, p->lv[0].x[p->lv[0].cur]);
+        if (!rc) {
+            hipError_t e = hipMemcpyAsync(ctx->h_pinned, p->d_u64, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                          ctx->stream);
+            if (e != hipSuccess) {
+                set_err(ctx, "capture of the residual readback failed: %s", hipGetErrorString(e));
+                rc = CG_ERR_DEVICE;
+            }
+        }
+        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        for (int l = 0; l < p->n; ++l) {
+            if (p->lv[l].cur != saved_cur[l]) {
+                set_err(ctx, "level %d ping-pong parity changed over a cycle", l);
+                rc = rc ? rc : CG_ERR_STATE;
+            }
+            p->lv[l].cur = saved_cur[l];
+        }
+        p->x64cur = saved_x64;
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) {
+            set_err(ctx, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+            return CG_ERR_DEVICE;
+        }
+        e = hipGraphInstantiate(&p->step_exec[par], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            set_err(ctx, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+            p->step_exec[par] = nullptr;
+            return CG_ERR_DEVICE;
+        }
+    }
+    return CG_OK;
+}
+
+int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *info, const volatile int *cancel) {
+    cg_ctx *ctx = p->ctx;
+    double largest;
+    int rc = plan_largest(p, &largest);
+    if (rc) return rc;
+    const Stencil *st = p->lv[0].st;
+    if (!warm || !p->x64_valid) {
+        HIPCHK(ctx, hipMemsetAsync(p->x64[p->x64cur], 0, sizeof(double) * st->cells, ctx->stream));
+    }
+    p->x64_valid = true;
+    if ((rc = outer_step(p, nullptr))) return rc;
+    unsigned long long b;
+    if ((rc = read_u64(p, 0, &b))) return rc;
+    double res = largest > 0.0 ? bits_to_double(b) / largest : 0.0;
+    const bool graph = ctx->use_graph && !ctx->time_kernels;
+    if (graph && res >= tol) {
+        if ((rc = build_step_graphs(p))) return rc;
+    }
+    int cycles = 0;
+    while (res >= tol && cycles < kRefinedMostSteps) {
+        if (cancel && *cancel) {
+            info->status = CG_CANCELLED;
+            break;
+        }
+        if (graph) {
+            HIPCHK(ctx, hipGraphLaunch(p->step_exec[p->x64cur], ctx->stream));
+            p->x64cur = 1 - p->x64cur;
+            HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+            b = ctx->h_pinned[0];
+        } else {
+            if ((rc = vcycle32(p, true))) return rc;
+            if ((rc = outer_step(p, p->lv[0].x[p->lv[0].cur]))) return rc;
+            if ((rc = read_u64(p, 0, &b))) return rc;
+        }
+        cycles++;
+        const double r = bits_to_double(b);
+        res = largest > 0.0 ? r / largest : 0.0;
+        if (std::isnan(r)) break;
+    }
+    info->cycles = cycles;
+    info->refinements = cycles + 1;
+    info->relative_residual = res;
+    p->last_answer = 1;
+    return CG_OK;
+}
+
+}  // namespace
+
+// ================================================================================================
+// C ABI
+// ================================================================================================
+
+extern "C" {
+
+int cg_ocean_open(int device, cg_ctx **out, char *reason, size_t reason_len) {
+    auto fail = [&](const char *msg) {
+        if (reason && reason_len) snprintf(reason, reason_len, "%s", msg);
+        if (out) *out = nullptr;
+        return CG_ERR_DEVICE;
+    };
+    if (!out) return CG_ERR_ARG;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return fail("no HIP device is visible");
+    if (device < 0 || device >= count) return fail("no such HIP device");
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail("the device's properties could not be read");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        char msg[256];
+        snprintf(msg, sizeof msg, "device %d is %s, not gfx950 (MI355X)", device, prop.gcnArchName);
+        return fail(msg);
+    }
+    if (hipSetDevice(device) != hipSuccess) return fail("hipSetDevice failed");
+    cg_ctx *ctx = new cg_ctx();
+    ctx->device = device;
+    ctx->name = prop.name;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return fail("hipStreamCreate failed");
+    }
+    if (hipHostMalloc((void **)&ctx->h_pinned, 64 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return fail("hipHostMalloc failed");
+    }
+    *out = ctx;
+    if (reason && reason_len) reason[0] = 0;
+    return CG_OK;
+}
+
+void cg_ocean_close(cg_ctx *ctx) {
+    if (!ctx) return;
+    {
+        std::lock_guard<std::recursive_mutex> g(ctx->mu);
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        for (auto &kv : ctx->stencils) free_stencil(kv.second);
+        ctx->stencils.clear();
+        for (auto &t : ctx->timed) {
+            (void)hipEventDestroy(t.a);
+            (void)hipEventDestroy(t.b);
+        }
+        (void)hipHostFree(ctx->h_pinned);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+}
+
+int cg_ocean_name(cg_ctx *ctx, char *buf, size_t len) {
+    if (!ctx || !buf || !len) return CG_ERR_ARG;
+    snprintf(buf, len, "%s (HIP, gfx950)", ctx->name.c_str());
+    return CG_OK;
+}
+
+const char *cg_ocean_last_error(cg_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int cg_ocean_set_option(cg_ctx *ctx, int option, long value) {
+    if (!ctx) return CG_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    switch (option) {
+        case CG_OPT_MIN_CELLS: ctx->min_cells = value < 0 ? 0 : value; return CG_OK;
+        case CG_OPT_TILE_ROWS:
+            if (value != 8 && value != 16 && value != 24 && value != 32) {
+                set_err(ctx, "tile rows must be 8, 16, 24 or 32");
+                return CG_ERR_ARG;
+            }
+            ctx->tile_rows = (int)value;
+            return CG_OK;
+        case CG_OPT_USE_GRAPH: ctx->use_graph = value ? 1 : 0; return CG_OK;
+        case CG_OPT_FGMRES_RESTART:
+            if (value < 1 || value > 64) {
+                set_err(ctx, "FGMRES restart must be 1..64");
+                return CG_ERR_ARG;
+            }
+            ctx->fgmres_m = (int)value;
+            return CG_OK;
+        case CG_OPT_STEPS_PER_CHECK: ctx->steps_per_check = value < 1 ? 1 : (int)value; return CG_OK;
+        case 6: ctx->time_kernels = value ? 1 : 0; return CG_OK;
+        default: set_err(ctx, "unknown option %d", option); return CG_ERR_ARG;
+    }
+}
+
+long cg_ocean_get_option(cg_ctx *ctx, int option) {
+    if (!ctx) return -1;
+    switch (option) {
+        case CG_OPT_MIN_CELLS: return ctx->min_cells;
+        case CG_OPT_TILE_ROWS: return ctx->tile_rows;
+        case CG_OPT_USE_GRAPH: return ctx->use_graph;
+        case CG_OPT_FGMRES_RESTART: return ctx->fgmres_m;
+        case CG_OPT_STEPS_PER_CHECK: return ctx->steps_per_check;
+        case 6: return ctx->time_kernels;
+        default: return -1;
+    }
+}
+
+int cg_ocean_stencil_upload(cg_ctx *ctx, int across, int down, const uint8_t *water, const float *east,
+                            const float *west, const float *north, const float *south, const double *centre,
+                            cg_stencil *out) {
+    if (!ctx || !out) return CG_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    *out = 0;
+    if (across <= 0 || down <= 0 || !water || !east || !west || !north || !south) {
+        set_err(ctx, "stencil upload: bad sizes or null arrays");
+        return CG_ERR_ARG;
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    Stencil *s = new Stencil();
+    s->across = across;
+    s->down = down;
+    s->cells = (long)across * down;
+    s->has_centre = centre != nullptr;
+    // Row-constant when every cell of a row carries the row's first weights, bit for bit.
+    bool rc_w = true, rc_c = true;
+    for (int r = 0; r < down && rc_w; ++r) {
+        const long b = (long)r * across;
+        for (int c = 1; c < across; ++c) {
+            const long i = b + c;
+            if (std::memcmp(&east[i], &east[b], 4) || std::memcmp(&west[i], &west[b], 4) ||
+                std::memcmp(&north[i], &north[b], 4) || std::memcmp(&south[i], &south[b], 4)) {
+                rc_w = false;
+                break;
+            }
+        }
+    }
+    if (centre)
+        for (int r = 0; r < down && rc_c; ++r) {
+            const long b = (long)r * across;
+            for (int c = 1; c < across; ++c)
+                if (std::memcmp(&centre[b + c], &centre[b], 8)) {
+                    rc_c = false;
+                    break;
+                }
+        }
+    s->rowconst = rc_w && rc_c;
+    s->h_water.assign(water, water + s->cells);
+    int rc = 0;
+    auto bail = [&](int code) {
+        free_stencil(s);
+        return code;
+    };
+    if ((rc = dalloc(ctx, &s->water, s->cells))) return bail(rc);
+    HIPCHK(ctx, hipMemcpy(s->water, water, s->cells, hipMemcpyHostToDevice));
+    if (s->rowconst) {
+        s->h_w.resize(down);
+        s->h_c.assign(down, 1.0);
+        for (int r = 0; r < down; ++r) {
+            const long b = (long)r * across;
+            s->h_w[r] = make_float4(east[b], west[b], north[b], south[b]);
+            if (centre) s->h_c[r] = centre[b];
+        }
+        if ((rc = dalloc(ctx, &s->roww, down))) return bail(rc);
+        if ((rc = dalloc(ctx, &s->rowc, down))) return bail(rc);
+        HIPCHK(ctx, hipMemcpy(s->roww, s->h_w.data(), sizeof(float4) * down, hipMemcpyHostToDevice));
+        HIPCHK(ctx, hipMemcpy(s->rowc, s->h_c.data(), sizeof(double) * down, hipMemcpyHostToDevice));
+    } else {
+        s->h_w.resize(s->cells);
+        s->h_c.assign(s->cells, 1.0);
+        for (long i = 0; i < s->cells; ++i) {
+            s->h_w[i] = make_float4(east[i], west[i], north[i], south[i]);
+            if (centre) s->h_c[i] = centre[i];
+        }
+        if ((rc = dalloc(ctx, &s->cellw, s->cells))) return bail(rc);
+        if ((rc = dalloc(ctx, &s->cellc, s->cells))) return bail(rc);
+        HIPCHK(ctx, hipMemcpy(s->cellw, s->h_w.data(), sizeof(float4) * s->cells, hipMemcpyHostToDevice));
+        HIPCHK(ctx, hipMemcpy(s->cellc, s->h_c.data(), sizeof(double) * s->cells, hipMemcpyHostToDevice));
+    }
+    const uint64_t h = ctx->next_handle++;
+    ctx->stencils[h] = s;
+    *out = h;
+    return CG_OK;
+}
+
+int cg_ocean_stencil_release(cg_ctx *ctx, cg_stencil h) {
+    if (!ctx) return CG_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    auto it = ctx->stencils.find(h);
+    if (it == ctx->stencils.end()) {
+        set_err(ctx, "no stencil %llu", (unsigned long long)h);
+        return CG_ERR_ARG;
+    }
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    free_stencil(it->second);
+    ctx->stencils.erase(it);
+    return CG_OK;
+}
+
+int cg_ocean_relax(cg_ctx *ctx, cg_stencil h, const float *forcing, const float *start, float *out, int passes,
+                   const volatile int *cancel) {
+    if (!ctx) return CG_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    Stencil *s = find_stencil(ctx, h);
+    if (!s) {
+        set_err(ctx, "no stencil %llu", (unsigned long long)h);
+        return CG_ERR_ARG;
+    }
+    // The declines of GpuOcean.solve (GO:52-58).
+    if (s->across <= 0 || s->across % 2 != 0 || s->down <= 0 || passes < 0) return CG_DECLINED;
+    if (s->cells < ctx->min_cells) return CG_DECLINED;
+    if (!forcing || !start || !out) {
+        set_err(ctx, "relax: null array");
+        return CG_ERR_ARG;
+    }
+    if (passes == 0) {
+        if (out != start) std::memmove(out, start, sizeof(float) * s->cells);
+        return CG_OK;
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    int rc;
+    if (!s->fbuf) {
+        if ((rc = dalloc(ctx, &s->fbuf, s->cells))) return rc;
+        if ((rc = dalloc(ctx, &s->xbuf[0], s->cells))) return rc;
+        if ((rc = dalloc(ctx, &s->xbuf[1], s->cells))) return rc;
+    }
+    HIPCHK(ctx, hipMemcpyAsync(s->fbuf, forcing, sizeof(float) * s->cells, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(s->xbuf[0], start, sizeof(float) * s->cells, hipMemcpyHostToDevice, ctx->stream));
+    int cur = 0;
+    // Cancellation is polled between launches of two passes, as GpuOcean polls per pass (GO:85-86).
+    int left = passes;
+    while (left > 0) {
+        if (cancel && *cancel) {
+            (void)hipStreamSynchronize(ctx->stream);
+            return CG_CANCELLED;
+        }
+        const int now = std::min(left, 2);
+        if ((rc = relax_pp(ctx, s, s->fbuf, s->xbuf, &cur, now, false, false))) return rc;
+        left -= now;
+    }
+    HIPCHK(ctx, hipMemcpyAsync(out, s->xbuf[cur], sizeof(float) * s->cells, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return CG_OK;
+}
+
+int cg_ocean_plan_create(cg_ctx *ctx, const cg_stencil *levels, int nlevels, int pole_is_wall, cg_plan **out) {
+    if (!ctx || !out) return CG_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    *out = nullptr;
+    if (!levels || nlevels < 1 || nlevels > kMaxLevels) {
+        set_err(ctx, "plan: 1..%d levels required", kMaxLevels);
+        return CG_ERR_ARG;
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    cg_plan *p = new cg_plan();
+    p->ctx = ctx;
+    p->n = nlevels;
+    p->pole_is_wall = pole_is_wall ? 1 : 0;
+    int rc = 0;
+    auto bail = [&](int code) {
+        free_plan(p);
+        return code;
+    };
+    for (int l = 0; l < nlevels; ++l) {
+        Stencil *s = find_stencil(ctx, levels[l]);
+        if (!s) {
+            set_err(ctx, "plan: no stencil %llu at level %d", (unsigned long long)levels[l], l);
+            return bail(CG_ERR_ARG);
+        }
+        if (!s->has_centre) {
+            set_err(ctx, "plan: level %d was uploaded without centre weights", l);
+            return bail(CG_ERR_ARG);
+        }
+        if (s->across % 2 != 0) {
+            set_err(ctx, "plan: level %d has an odd width; red-black colouring needs an even one", l);
+            return bail(CG_ERR_ARG);
+        }
+        if (l > 0) {
+            const Stencil *f = p->lv[l - 1].st;
+            if (f->across % s->across || f->down % s->down) {
+                set_err(ctx, "plan: level %d does not divide level %d", l, l - 1);
+                return bail(CG_ERR_ARG);
+            }
+        }
+        p->lv[l].st = s;
+        if ((rc = dalloc(ctx, &p->lv[l].f, s->cells))) return bail(rc);
+        if ((rc = dalloc(ctx, &p->lv[l].x[0], s->cells))) return bail(rc);
+        if ((rc = dalloc(ctx, &p->lv[l].x[1], s->cells))) return bail(rc);
+        if (l > 0 && (rc = dalloc(ctx, &p->lv[l].bal, s->cells))) return bail(rc);
+        HIPCHK(ctx, hipMemset(p->lv[l].x[0], 0, sizeof(float) * s->cells));
+        HIPCHK(ctx, hipMemset(p->lv[l].x[1], 0, sizeof(float) * s->cells));
+        HIPCHK(ctx, hipMemset(p->lv[l].f, 0, sizeof(float) * s->cells));
+    }
+    const long n0 = p->lv[0].st->cells;
+    if ((rc = dalloc(ctx, &p->f0, n0))) return bail(rc);
+    if ((rc = dalloc(ctx, &p->r64, n0))) return bail(rc);
+    if ((rc = dalloc(ctx, &p->x64[0], n0))) return bail(rc);
+    if ((rc = dalloc(ctx, &p->x64[1], n0))) return bail(rc);
+    if ((rc = dalloc(ctx, &p->d_u64, 16))) return bail(rc);
+    if ((rc = dalloc(ctx, &p->d_passes, 4))) return bail(rc);
+    HIPCHK(ctx, hipMemset(p->f0, 0, sizeof(float) * n0));
+    HIPCHK(ctx, hipMemset(p->x64[0], 0, sizeof(double) * n0));
+    HIPCHK(ctx, hipMemset(p->x64[1], 0, sizeof(double) * n0));
+    if (nlevels > 1 && (rc = build_coarsest_inverse(ctx, p))) return bail(rc);
+    *out = p;
+    return CG_OK;
+}
+
+int cg_ocean_plan_destroy(cg_plan *p) {
+    if (!p) return CG_ERR_ARG;
+    cg_ctx *ctx = p->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    free_plan(p);
+    return CG_OK;
+}
+
+int cg_ocean_plan_set_forcing(cg_plan *p, const float *forcing) {
+    if (!p || !forcing) return CG_ERR_ARG;
+    cg_ctx *ctx = p->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipMemcpyAsync(p->f0, forcing, sizeof(float) * p->lv[0].st->cells, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    p->largest = -1.0;
+    p->x64_valid = false;
+    return CG_OK;
+}
+
+static int plan_solve_locked(cg_plan *p, double tol, int method, int warm, cg_solution *info,
+                             const volatile int *cancel) {
+    cg_ctx *ctx = p->ctx;
+    cg_solution local{};
+    if (!info) info = &local;
+    *info = cg_solution{};
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (method == CG_METHOD_REFERENCE || method == CG_METHOD_RESCUE) {
+        PlanLevel &l0 = p->lv[0];
+        if (!warm) HIPCHK(ctx, hipMemsetAsync(l0.x[l0.cur], 0, sizeof(float) * l0.st->cells, ctx->stream));
+        int rc = ref_solve(p, tol, info, cancel);
+        if (rc) return rc;
+        if (method == CG_METHOD_RESCUE && info->relative_residual >= tol && std::isfinite(info->relative_residual)) {
+            set_err(ctx, "the Krylov rescue is not built on the device yet");
+            return CG_ERR_STATE;
+        }
+        return CG_OK;
+    }
+    if (method == CG_METHOD_REFINED) return refined_solve(p, tol, warm != 0, info, cancel);
+    set_err(ctx, "unknown or unbuilt method %d", method);
+    return CG_ERR_ARG;
+}
+
+int cg_ocean_plan_solve(cg_plan *p, double tol, int method, int warm, cg_solution *info) {
+    if (!p) return CG_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(p->ctx->mu);
+    return plan_solve_locked(p, tol, method, warm, info, nullptr);
+}
+
+int cg_ocean_plan_solve_batch(cg_plan *const *plans, int count, double tol, int method, cg_solution *infos) {
+    if (!plans || count < 0) return CG_ERR_ARG;
+    for (int i = 0; i < count; ++i) {
+        int rc = cg_ocean_plan_solve(plans[i], tol, method, 0, infos ? &infos[i] : nullptr);
+        if (rc) return rc;
+    }
+    return CG_OK;
+}
+
+int cg_ocean_plan_read(cg_plan *p, double *out64, float *out32) {
+    if (!p) return CG_ERR_ARG;
+    cg_ctx *ctx = p->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const Stencil *st = p->lv[0].st;
+    const long n = st->cells;
+    if (p->last_answer == 0) {
+        const float *x = p->lv[0].x[p->lv[0].cur];
+        if (out32) HIPCHK(ctx, hipMemcpyAsync(out32, x, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+        if (out64) {
+            hipLaunchKernelGGL(k_f32_to_f64, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, x, p->r64, n);
+            HIPCHK(ctx, hipMemcpyAsync(out64, p->r64, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+        }
+    } else {
+        const double *x = p->x64[p->x64cur];
+        if (out64) HIPCHK(ctx, hipMemcpyAsync(out64, x, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+        if (out32) {
+            float *tmp = p->lv[0].x[1 - p->lv[0].cur];
+            hipLaunchKernelGGL(k_f64_to_f32, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, x, tmp, n);
+            HIPCHK(ctx, hipMemcpyAsync(out32, tmp, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+        }
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return CG_OK;
+}
+
+int cg_ocean_solve(cg_ctx *ctx, const cg_stencil *levels, int nlevels, const float *forcing, const float *start,
+                   double tol, int pole_is_wall, int method, float *out, double *out64, cg_solution *info,
+                   const volatile int *cancel) {
+    if (!ctx || !forcing) return CG_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    cg_plan *p = nullptr;
+    int rc = cg_ocean_plan_create(ctx, levels, nlevels, pole_is_wall, &p);
+    if (rc) return rc;
+    rc = cg_ocean_plan_set_forcing(p, forcing);
+    const long n = p->lv[0].st->cells;
+    int warm = 0;
+    if (!rc && start) {
+        warm = 1;
+        if (method == CG_METHOD_REFERENCE || method == CG_METHOD_RESCUE) {
+            PlanLevel &l0 = p->lv[0];
+            hipError_t e = hipMemcpy(l0.x[l0.cur], start, sizeof(float) * n, hipMemcpyHostToDevice);
+            if (e != hipSuccess) rc = CG_ERR_DEVICE;
+        } else {
+            std::vector<double> s64(start, start + n);
+            hipError_t e = hipMemcpy(p->x64[p->x64cur], s64.data(), sizeof(double) * n, hipMemcpyHostToDevice);
+            if (e != hipSuccess) rc = CG_ERR_DEVICE;
+            p->x64_valid = true;
+        }
+    }
+    if (!rc) rc = plan_solve_locked(p, tol, method, warm, info, cancel);
+    if (!rc) rc = cg_ocean_plan_read(p, out64, out);
+    cg_ocean_plan_destroy(p);
+    return rc;
+}
+
+double cg_ocean_kernel_time_us(cg_ctx *ctx, int kind, long *launches) {
+    if (!ctx) return -1.0;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (kind < 0) {
+        (void)hipStreamSynchronize(ctx->stream);
+        ctx->timed_used = 0;
+        ctx->timed_dropped = 0;
+        if (launches) *launches = 0;
+        return 0.0;
+    }
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return -1.0;
+    double total = 0.0;
+    long counted = 0;
+    for (size_t i = 0; i < ctx->timed_used; ++i) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, ctx->timed[i].a, ctx->timed[i].b) == hipSuccess) {
+            total += ms;
+            counted++;
+        }
+    }
+    if (launches) *launches = counted;
+    return counted ? total * 1000.0 / counted : 0.0;
+}
+
+}  // extern "C"

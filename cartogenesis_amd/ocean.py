@@ -1,0 +1,308 @@
+"""Host-side mirror of the reference's ocean-solver interface, over the C ABI of libcgocean.so.
+
+Names, argument meanings and failure behaviour follow the reference's Kotlin:
+  OceanStencil            OceanCirculation.kt:24-42
+  OceanCirculation.*      OceanCirculation.kt:82-829 (stencil, withBalance, coarseWater, levels,
+                          Solution, OceanSolveFailure, requireSolved, solve)
+  OceanAccelerator        OceanAccelerator.kt:15-42
+  HipOcean                desktop/.../GpuOcean.kt:26-251 (createOrNull, name, solve, residency)
+The relaxation, the residual and the whole solve run on the GPU through the HIP library; nothing
+here computes them on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes
+from collections import OrderedDict
+from dataclasses import dataclass
+from typing import Callable, List, Optional
+
+import numpy as np
+
+from . import (CG_CANCELLED, CG_DECLINED, CG_METHOD_REFERENCE, CG_METHOD_REFINED, CG_OK, CG_OPT_MIN_CELLS,
+               CgSolution, check, lib)
+
+
+class OceanStencil:
+    """One steady linear problem on one grid (OceanCirculation.kt:24-42): binary32 neighbour
+    weights and forcing, the binary64 centre weight, a water mask; row-major, columns wrapping."""
+
+    def __init__(self, cells_across: int, cells_down: int, is_water, east_weight, west_weight, north_weight,
+                 south_weight, forcing, centre_weight):
+        n = int(cells_across) * int(cells_down)
+        self.cells_across = int(cells_across)
+        self.cells_down = int(cells_down)
+        self.is_water = np.ascontiguousarray(is_water, dtype=np.uint8).reshape(n)
+        self.east_weight = np.ascontiguousarray(east_weight, dtype=np.float32).reshape(n)
+        self.west_weight = np.ascontiguousarray(west_weight, dtype=np.float32).reshape(n)
+        self.north_weight = np.ascontiguousarray(north_weight, dtype=np.float32).reshape(n)
+        self.south_weight = np.ascontiguousarray(south_weight, dtype=np.float32).reshape(n)
+        self.forcing = np.ascontiguousarray(forcing, dtype=np.float32).reshape(n)
+        self.centre_weight = np.ascontiguousarray(centre_weight, dtype=np.float64).reshape(n)
+
+    @property
+    def cells(self) -> int:
+        return self.cells_across * self.cells_down
+
+
+@dataclass
+class Solution:
+    """OceanCirculation.Solution (OceanCirculation.kt:125), with the binary64 answer when the
+    method carries one and the refinement count."""
+    values: np.ndarray
+    cycles: int
+    relative_residual: float
+    values64: Optional[np.ndarray] = None
+    refinements: int = 0
+
+
+class OceanSolveFailure(RuntimeError):
+    """OceanCirculation.OceanSolveFailure (OceanCirculation.kt:133)."""
+
+
+class OceanCirculation:
+    """The reference's solver object (OceanCirculation.kt:82-829), with the heavy work on the GPU."""
+
+    SERIES_BELOW_CELL_PECLET = 1e-4
+    RESIDUAL_TOLERANCE = 1e-3
+    PASSES_EACH_WAY = 2
+    COARSEST_ROWS = 8
+    COARSEST_TOLERANCE_SHARE = 0.1
+    COARSEST_PASSES_PER_CHECK = 20
+    MOST_CYCLES = 200
+    KRYLOV_RESTARTS = 3
+    PRECONDITIONER_COARSEST_PASSES = 200
+
+    @staticmethod
+    def require_solved(what: str, solution: Solution, tolerance: float) -> Solution:
+        """requireSolved (OceanCirculation.kt:140-151)."""
+        residual_finite = np.isfinite(solution.relative_residual)
+        bad = np.flatnonzero(~np.isfinite(solution.values))
+        first_bad = int(bad[0]) if bad.size else -1
+        if not residual_finite or first_bad >= 0 or solution.relative_residual >= tolerance:
+            msg = (f"{what} did not solve: relative residual {solution.relative_residual} after {solution.cycles} "
+                   f"iterations against a tolerance of {tolerance}")
+            if first_bad >= 0:
+                msg += f", and cell {first_bad} holds {solution.values[first_bad]}"
+            raise OceanSolveFailure(msg)
+        return solution
+
+    @staticmethod
+    def stencil(cells_across, cells_down, cell_width_meters, cell_height_meters, is_water, beta_per_meter_second,
+                bottom_drag_per_second, balance_per_second_squared=None) -> OceanStencil:
+        """OceanCirculation.stencil (OceanCirculation.kt:161-219), built by the library's host code."""
+        n = cells_across * cells_down
+        water = np.ascontiguousarray(is_water, dtype=np.uint8).reshape(n)
+        beta = np.ascontiguousarray(beta_per_meter_second, dtype=np.float64)
+        e, w, no, s, f = (np.empty(n, np.float32) for _ in range(5))
+        c = np.empty(n, np.float64)
+        bal = None
+        if balance_per_second_squared is not None:
+            bal = np.ascontiguousarray(balance_per_second_squared, dtype=np.float64).reshape(n)
+        check(lib().cg_circulation_stencil(cells_across, cells_down, float(cell_width_meters),
+                                           float(cell_height_meters), water, beta, float(bottom_drag_per_second),
+                                           None if bal is None else bal.ctypes.data_as(ctypes.c_void_p),
+                                           e, w, no, s, c, f), what="cg_circulation_stencil")
+        return OceanStencil(cells_across, cells_down, water, e, w, no, s, f, c)
+
+    @staticmethod
+    def with_balance(stencil: OceanStencil, balance) -> OceanStencil:
+        """withBalance (OceanCirculation.kt:222-231): the same operator, another right-hand side."""
+        bal = np.asarray(balance, dtype=np.float64).reshape(stencil.cells)
+        f = np.zeros(stencil.cells, np.float32)
+        wet = stencil.is_water != 0
+        f[wet] = (bal[wet] / stencil.centre_weight[wet]).astype(np.float32)
+        return OceanStencil(stencil.cells_across, stencil.cells_down, stencil.is_water, stencil.east_weight,
+                            stencil.west_weight, stencil.north_weight, stencil.south_weight, f,
+                            stencil.centre_weight)
+
+    @staticmethod
+    def balance_of(stencil: OceanStencil) -> np.ndarray:
+        """balanceOf (OceanCirculation.kt:325-327)."""
+        return stencil.forcing.astype(np.float64) * stencil.centre_weight
+
+    @staticmethod
+    def coarse_water(fine, fine_across, fine_down, coarse_across, coarse_down, every_cell_water) -> np.ndarray:
+        """coarseWater (OceanCirculation.kt:404-423)."""
+        out = np.empty(coarse_across * coarse_down, np.uint8)
+        check(lib().cg_coarse_water(np.ascontiguousarray(fine, dtype=np.uint8), fine_across, fine_down,
+                                    coarse_across, coarse_down, int(bool(every_cell_water)), out),
+              what="cg_coarse_water")
+        return out
+
+    @staticmethod
+    def level_shapes(across, down, dx, dy, max_levels=24):
+        a = np.empty(max_levels, np.int32)
+        d = np.empty(max_levels, np.int32)
+        w = np.empty(max_levels)
+        h = np.empty(max_levels)
+        n = check(lib().cg_level_shapes(across, down, float(dx), float(dy), max_levels, a, d, w, h),
+                  what="cg_level_shapes")
+        return [(int(a[i]), int(d[i]), float(w[i]), float(h[i])) for i in range(n)]
+
+    @staticmethod
+    def levels(finest: OceanStencil, cell_width_meters: float, cell_height_meters: float, every_cell_water: bool,
+               coarser: Callable[[int, int, np.ndarray], OceanStencil]) -> List[OceanStencil]:
+        """levels (OceanCirculation.kt:434-462): the grids of a V-cycle, finest first."""
+        out = [finest]
+        shapes = OceanCirculation.level_shapes(finest.cells_across, finest.cells_down, cell_width_meters,
+                                               cell_height_meters)
+        water, a, d = finest.is_water, finest.cells_across, finest.cells_down
+        for (na, nd, _, _) in shapes[1:]:
+            water = OceanCirculation.coarse_water(water, a, d, na, nd, every_cell_water)
+            a, d = na, nd
+            out.append(coarser(a, d, water))
+        return out
+
+
+class OceanAccelerator:
+    """OceanAccelerator (OceanAccelerator.kt:15-42): relax on a device, or return None to decline."""
+
+    name: str = ""
+
+    def solve(self, stencil: OceanStencil, start: np.ndarray, passes: int) -> Optional[np.ndarray]:
+        raise NotImplementedError
+
+
+class HipOcean(OceanAccelerator):
+    """The MI355X accelerator: GpuOcean's role (GpuOcean.kt:26-251) over the HIP library.
+
+    Stencils are made resident on first sight and kept in an LRU of MOST_RESIDENT_STENCILS keyed on
+    the identity of their east-weight array, as GpuOcean keys its residency (GpuOcean.kt:104-137);
+    a stencil rebuilt by with_balance shares that array and so hits the cache."""
+
+    MOST_RESIDENT_STENCILS = 24
+    SMALLEST_GRID_WORTH_THE_CARD_CELLS = 1 << 16
+
+    @dataclass
+    class Result:
+        accelerator: Optional["HipOcean"]
+        unavailable_because: Optional[str]
+
+    def __init__(self, ctx: ctypes.c_void_p, name: str):
+        self._ctx = ctx
+        self.name = name
+        self._resident: "OrderedDict[int, tuple]" = OrderedDict()
+
+    @staticmethod
+    def create_or_null(device: int = 0) -> "HipOcean.Result":
+        """createOrNull (GpuOcean.kt:189-202): an accelerator, or the reason there is none."""
+        ctx = ctypes.c_void_p()
+        reason = ctypes.create_string_buffer(512)
+        status = lib().cg_ocean_open(device, ctypes.byref(ctx), reason, len(reason))
+        if status != CG_OK or not ctx:
+            return HipOcean.Result(None, reason.value.decode() or f"status {status}")
+        name = ctypes.create_string_buffer(256)
+        lib().cg_ocean_name(ctx, name, len(name))
+        return HipOcean.Result(HipOcean(ctx, name.value.decode()), None)
+
+    @property
+    def ctx(self):
+        return self._ctx
+
+    def close(self) -> None:
+        if self._ctx:
+            for (_, handle) in self._resident.values():
+                lib().cg_ocean_stencil_release(self._ctx, handle)
+            self._resident.clear()
+            lib().cg_ocean_close(self._ctx)
+            self._ctx = None
+
+    def set_option(self, option: int, value: int) -> None:
+        check(lib().cg_ocean_set_option(self._ctx, option, int(value)), self._ctx, "cg_ocean_set_option")
+
+    def holding(self, stencil: OceanStencil) -> int:
+        """The device handle of a stencil, uploading it the first time it is seen."""
+        key = id(stencil.east_weight)
+        hit = self._resident.get(key)
+        if hit is not None and hit[0] is stencil.east_weight:
+            self._resident.move_to_end(key)
+            return hit[1]
+        while len(self._resident) >= self.MOST_RESIDENT_STENCILS:
+            _, (_, old) = self._resident.popitem(last=False)
+            lib().cg_ocean_stencil_release(self._ctx, old)
+        handle = ctypes.c_uint64(0)
+        check(lib().cg_ocean_stencil_upload(self._ctx, stencil.cells_across, stencil.cells_down, stencil.is_water,
+                                            stencil.east_weight, stencil.west_weight, stencil.north_weight,
+                                            stencil.south_weight,
+                                            stencil.centre_weight.ctypes.data_as(ctypes.c_void_p),
+                                            ctypes.byref(handle)), self._ctx, "cg_ocean_stencil_upload")
+        self._resident[key] = (stencil.east_weight, handle.value)
+        return handle.value
+
+    def solve(self, stencil: OceanStencil, start: np.ndarray, passes: int, cancel=None) -> Optional[np.ndarray]:
+        """OceanAccelerator.solve (OceanAccelerator.kt:37-41): `passes` red-black passes from start,
+        a new array back, or None when the device declines (or saw `cancel` set: a ctypes c_int)."""
+        across, down = stencil.cells_across, stencil.cells_down
+        if across <= 0 or across % 2 != 0 or down <= 0 or passes < 0:
+            return None
+        start = np.ascontiguousarray(start, dtype=np.float32)
+        if start.size != across * down or stencil.is_water.size != across * down:
+            return None
+        if across * down < lib().cg_ocean_get_option(self._ctx, CG_OPT_MIN_CELLS):
+            return None
+        handle = self.holding(stencil)
+        out = np.empty(across * down, np.float32)
+        status = lib().cg_ocean_relax(self._ctx, handle, stencil.forcing, start, out, int(passes),
+                                      None if cancel is None else ctypes.cast(ctypes.pointer(cancel), ctypes.c_void_p))
+        if status in (CG_DECLINED, CG_CANCELLED):
+            return None
+        check(status, self._ctx, "cg_ocean_relax")
+        return out
+
+    def solve_levels(self, levels: List[OceanStencil], start: Optional[np.ndarray], tolerance: float,
+                     pole_is_wall: bool = True, method: int = CG_METHOD_REFERENCE, cancel=None) -> Solution:
+        """The whole solve on the device: OceanCirculation.solve (OceanCirculation.kt:473-493) for
+        CG_METHOD_REFERENCE, or the binary32-in-binary64 refinement for CG_METHOD_REFINED."""
+        handles = (ctypes.c_uint64 * len(levels))(*[self.holding(s) for s in levels])
+        n = levels[0].cells
+        out = np.empty(n, np.float32)
+        out64 = np.empty(n, np.float64)
+        info = CgSolution()
+        st = None if start is None else np.ascontiguousarray(start, dtype=np.float32)
+        check(lib().cg_ocean_solve(self._ctx, handles, len(levels), levels[0].forcing,
+                                   None if st is None else st.ctypes.data_as(ctypes.c_void_p), float(tolerance),
+                                   int(bool(pole_is_wall)), int(method), out.ctypes.data_as(ctypes.c_void_p),
+                                   out64.ctypes.data_as(ctypes.c_void_p), ctypes.byref(info),
+                                   None if cancel is None else ctypes.cast(ctypes.pointer(cancel), ctypes.c_void_p)),
+              self._ctx, "cg_ocean_solve")
+        return Solution(out, info.cycles, info.relative_residual, out64, info.refinements)
+
+
+class Plan:
+    """A device-resident solve (cg_ocean_plan_*): ladder, buffers, forcing and answer stay in HBM."""
+
+    def __init__(self, acc: HipOcean, levels: List[OceanStencil], pole_is_wall: bool = True):
+        self.acc = acc
+        self.levels = levels
+        handles = (ctypes.c_uint64 * len(levels))(*[acc.holding(s) for s in levels])
+        self._p = ctypes.c_void_p()
+        check(lib().cg_ocean_plan_create(acc.ctx, handles, len(levels), int(bool(pole_is_wall)),
+                                         ctypes.byref(self._p)), acc.ctx, "cg_ocean_plan_create")
+        self.set_forcing(levels[0].forcing)
+
+    @property
+    def handle(self):
+        return self._p
+
+    def set_forcing(self, forcing) -> None:
+        check(lib().cg_ocean_plan_set_forcing(self._p, np.ascontiguousarray(forcing, dtype=np.float32)),
+              self.acc.ctx, "cg_ocean_plan_set_forcing")
+
+    def solve(self, tol: float, method: int = CG_METHOD_REFINED, warm: bool = False) -> CgSolution:
+        info = CgSolution()
+        check(lib().cg_ocean_plan_solve(self._p, float(tol), int(method), int(warm), ctypes.byref(info)),
+              self.acc.ctx, "cg_ocean_plan_solve")
+        return info
+
+    def read(self):
+        n = self.levels[0].cells
+        out64 = np.empty(n, np.float64)
+        out32 = np.empty(n, np.float32)
+        check(lib().cg_ocean_plan_read(self._p, out64.ctypes.data_as(ctypes.c_void_p),
+                                       out32.ctypes.data_as(ctypes.c_void_p)), self.acc.ctx, "cg_ocean_plan_read")
+        return out64, out32
+
+    def close(self) -> None:
+        if self._p:
+            lib().cg_ocean_plan_destroy(self._p)
+            self._p = None

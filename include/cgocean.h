@@ -1,0 +1,171 @@
+/*
+ * cgocean.h: the C ABI of the MI355X (gfx950) ocean-circulation solver.
+ *
+ * Plain pointers and sizes only. No C++ exception crosses this boundary; every entry point returns
+ * an int status (CG_OK, CG_DECLINED, CG_CANCELLED or a negative error) and cg_ocean_last_error()
+ * gives the reason of the last failure on a context.
+ *
+ * Reference interfaces are cited as paths relative to the reference repository:
+ *   OA = worldgen/src/commonMain/kotlin/com/cartogenesis/worldgen/pipeline/OceanAccelerator.kt
+ *   OC = worldgen/src/commonMain/kotlin/com/cartogenesis/worldgen/pipeline/OceanCirculation.kt
+ *   OS = worldgen/src/commonMain/kotlin/com/cartogenesis/worldgen/pipeline/OceanStage.kt
+ *   GO = desktop/src/main/kotlin/com/cartogenesis/desktop/GpuOcean.kt
+ *   WGC = worldgen/src/commonMain/kotlin/com/cartogenesis/worldgen/model/WorldGenConfig.kt
+ *
+ * Grids are row-major, cell = row * across + column; columns wrap east to west; row 0 is the
+ * northern edge. Masks are one byte per cell, non-zero for water.
+ *
+ * Threading: a context owns one HIP stream and one mutex; calls on one context are serialised.
+ * Use one context per thread for concurrency.
+ */
+#ifndef CGOCEAN_H
+#define CGOCEAN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------------------------- */
+#define CG_OK 0
+#define CG_DECLINED 1   /* the device cannot take this job; the caller relaxes on the CPU (OA:21-25) */
+#define CG_CANCELLED 2  /* the cancel flag was seen set; nothing more was run */
+#define CG_ERR_ARG (-1)
+#define CG_ERR_DEVICE (-2)
+#define CG_ERR_NOMEM (-3)
+#define CG_ERR_STATE (-4)
+
+/* ---- solve methods -------------------------------------------------------------------------- */
+/* The reference algorithm on the device: binary32 values, binary64 residual, V(2,2) red-black
+ * Gauss-Seidel, coarsest grid relaxed to 0.1 x tol (OC:473-493, 740-794). Bit-identical to the
+ * processor's solve on the same ladder. */
+#define CG_METHOD_REFERENCE 0
+/* solveWithKrylovRescue (OC:507-528): the above, then BiCGSTAB preconditioned by the cycle. */
+#define CG_METHOD_RESCUE 1
+/* Binary32 V(2,2) cycles on the correction, wrapped in binary64 iterative refinement; the
+ * outer residual and the accumulated answer are binary64; the coarsest grid is solved directly. */
+#define CG_METHOD_REFINED 2
+/* Flexible GMRES(m) in binary64, right-preconditioned by one binary32 V(2,2) cycle. */
+#define CG_METHOD_FGMRES 3
+
+/* ---- options (cg_ocean_set_option) ------------------------------------------------------------ */
+#define CG_OPT_MIN_CELLS 1       /* relax declines grids with fewer cells (GO:167-172); default 65536 */
+#define CG_OPT_TILE_ROWS 2       /* rows per wavefront tile of the fused smoother: 8, 16, 24 or 32 */
+#define CG_OPT_USE_GRAPH 3       /* 1: replay each refinement step as a captured hipGraph */
+#define CG_OPT_FGMRES_RESTART 4  /* restart length m of CG_METHOD_FGMRES (1..64), default 20 */
+#define CG_OPT_STEPS_PER_CHECK 5 /* refinement steps between host reads of the residual, default 1 */
+
+typedef struct cg_ctx cg_ctx;
+typedef struct cg_plan cg_plan;
+typedef uint64_t cg_stencil; /* a device-resident stencil; 0 is never a valid handle */
+
+/* What a solve returned: OceanCirculation.Solution (OC:125) plus the refinement count. */
+typedef struct {
+    int cycles;               /* V-cycles run (for Krylov methods: preconditioner applications) */
+    double relative_residual; /* max|F - Lx| / max|F| of the returned values, binary64 */
+    int refinements;          /* outer binary64 residual evaluations (CG_METHOD_REFINED/FGMRES) */
+    int status;               /* CG_OK, or CG_CANCELLED */
+} cg_solution;
+
+/* ---- device and context -------------------------------------------------------------------- */
+
+/* Probes HIP device `device` and prepares a context: GpuOcean.createOrNull (GO:189-202). On
+ * failure returns a negative status, *out = NULL and writes the reason into reason[reason_len]. */
+int cg_ocean_open(int device, cg_ctx **out, char *reason, size_t reason_len);
+void cg_ocean_close(cg_ctx *ctx);
+/* OceanAccelerator.name (OA:18): the device's marketing name. */
+int cg_ocean_name(cg_ctx *ctx, char *buf, size_t len);
+const char *cg_ocean_last_error(cg_ctx *ctx);
+int cg_ocean_set_option(cg_ctx *ctx, int option, long value);
+long cg_ocean_get_option(cg_ctx *ctx, int option);
+
+/* ---- stencils ------------------------------------------------------------------------------- */
+
+/* Makes an OceanStencil (OC:24-42) resident on the device: mask, the four neighbour weights and
+ * the centre weight. Replaces GpuOcean.holding's identity-keyed LRU (GO:104-137) with an explicit
+ * handle. Weights that are equal along every row (the circulation's, OC:207-213) are stored once
+ * per row; others once per cell. centre may be NULL when only relaxation is asked for. */
+int cg_ocean_stencil_upload(cg_ctx *ctx, int across, int down, const uint8_t *water,
+                            const float *east, const float *west, const float *north,
+                            const float *south, const double *centre, cg_stencil *out);
+int cg_ocean_stencil_release(cg_ctx *ctx, cg_stencil stencil);
+
+/* OceanAccelerator.solve (OA:37-41) as GpuOcean.solve runs it (GO:45-102): `passes` red-black
+ * passes, colour 0 then colour 1, of x = ((e xE + w xW) + n xN) + s xS - f in binary32 without
+ * fused multiply-add, land set to zero, columns wrapping, zero beyond either pole. Bit-identical to
+ * OceanCirculation.relax (OC:241-274). `start` and `forcing` are read only; `out` receives the
+ * result (it may alias start). Returns CG_DECLINED for an odd or non-positive width, a
+ * non-positive height, negative passes or fewer cells than CG_OPT_MIN_CELLS (GO:52-58), and
+ * CG_CANCELLED if *cancel is seen non-zero between passes (GO:85-86). */
+int cg_ocean_relax(cg_ctx *ctx, cg_stencil stencil, const float *forcing, const float *start,
+                   float *out, int passes, const volatile int *cancel);
+
+/* ---- whole solves --------------------------------------------------------------------------- */
+
+/* OceanCirculation.solve / solveWithKrylovRescue (OC:473-528) on the device, over a ladder of
+ * resident stencils, finest first, as OceanCirculation.levels builds it (OC:434-462). forcing is
+ * the finest grid's f (OceanStencil.forcing); start the initial values (NULL for zeros). tol is
+ * the relative max-norm residual to reach; pole_is_wall is prolongAdd's (OC:354-392). out (binary32)
+ * and/or out64 (binary64) receive the answer; either may be NULL. requireSolved stays with the
+ * caller: an unconverged solve returns CG_OK with info->relative_residual >= tol. */
+int cg_ocean_solve(cg_ctx *ctx, const cg_stencil *levels, int nlevels, const float *forcing,
+                   const float *start, double tol, int pole_is_wall, int method, float *out,
+                   double *out64, cg_solution *info, const volatile int *cancel);
+
+/* A device-resident solve: the ladder, its buffers, the finest forcing and the answer stay in
+ * HBM between calls, so repeated solves pay no host transfer. */
+int cg_ocean_plan_create(cg_ctx *ctx, const cg_stencil *levels, int nlevels, int pole_is_wall,
+                         cg_plan **out);
+int cg_ocean_plan_destroy(cg_plan *plan);
+/* Uploads the finest forcing (host pointer). */
+int cg_ocean_plan_set_forcing(cg_plan *plan, const float *forcing);
+/* Solves from zero (or from the plan's current answer if warm != 0) to tol. */
+int cg_ocean_plan_solve(cg_plan *plan, double tol, int method, int warm, cg_solution *info);
+/* Solves several plans of one context concurrently, each from zero, to one tol. */
+int cg_ocean_plan_solve_batch(cg_plan *const *plans, int count, double tol, int method,
+                              cg_solution *infos);
+/* Copies the answer back; either pointer may be NULL. */
+int cg_ocean_plan_read(cg_plan *plan, double *out64, float *out32);
+/* Synchronises the context's stream and reports the average duration, in microseconds, of the
+ * kernels of `kind` (0: fused finest-grid smoother launches) recorded since the last reset with
+ * HIP events on the stream they were launched on; kind -1 resets the record. */
+double cg_ocean_kernel_time_us(cg_ctx *ctx, int kind, long *launches);
+
+/* ---- host-side problem builders (no device needed) --------------------------------------------- */
+
+/* BOTTOM_DRAG_PER_S (OS:102-103) with Earth's rotation rate as the reference's Float (WGC:302). */
+double cg_bottom_drag(void);
+/* beta per row, 2 Omega cos(phi) / a, with phi the row's centre latitude computed in binary32
+ * (ClimateStage.latitudeOf, ClimateStage.kt:1313-1314; WGC:246, 272-273, 302). */
+int cg_beta_rows(int down, double world_width_km, double *beta);
+
+/* OceanCirculation.stencil then withBalance (OC:161-231). balance may be NULL (zero forcing). */
+int cg_circulation_stencil(int across, int down, double dx, double dy, const uint8_t *water,
+                           const double *beta, double drag, const double *balance, float *east,
+                           float *west, float *north, float *south, double *centre,
+                           float *forcing);
+/* OceanCirculation.coarseWater (OC:404-423). */
+int cg_coarse_water(const uint8_t *fine, int fine_across, int fine_down, int coarse_across,
+                    int coarse_down, int every_cell_water, uint8_t *coarse);
+/* The level shapes OceanCirculation.levels walks (OC:434-462): returns the count (<= max_levels)
+ * and fills across/down/cell width/cell height per level, finest first. */
+int cg_level_shapes(int across, int down, double dx, double dy, int max_levels, int *level_across,
+                    int *level_down, double *cell_width, double *cell_height);
+
+/* Synthetic circulation inputs: a seeded 4-octave value-noise land mask thresholded so that
+ * `land_fraction` of cells are land, depth H ~ U(200, 6000) m on water, and the balance
+ * curl_z(tau) / (rho H) of a zonal stress tau_x = amplitude * cos(3 (lat - shift)) plus Gaussian
+ * noise, by OceanStage.curlForcing's differences (OS:689-708). Not a reference function: the
+ * reference's stress comes from its climate stage. month in 0..11 sets
+ * amplitude = 0.1 cos(2 pi month / 12) and shift = 10 deg sin(2 pi month / 12); month < 0 gives
+ * the annual case, amplitude 0.1 and no shift. Any of the outputs may be NULL. */
+int cg_problem_generate(int across, int down, uint64_t seed, double land_fraction, int month,
+                        uint8_t *water, double *depth, double *balance);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CGOCEAN_H */
