@@ -1,0 +1,190 @@
+"""Benchmark: circulation solves per second at 1024 x 2048 to relative residual 1e-8 on one MI355X.
+
+One step = one solve of the synthetic 1024 x 2048 world (seed 0, 30% land, annual forcing) from
+psi = 0 to max|F - L psi| / max|F| < 1e-8, with the ladder, forcing and answer resident in HBM
+(a device-resident plan). Prints one JSON line.
+
+Plain GPU multigrid (binary32 V(2,2) cycles of the reference's ladder inside binary64 iterative
+refinement) is the method measured; the learned preconditioner is not built yet.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+import numpy as np  # noqa: E402
+
+METRIC = "solves/s at 1024×2048 to 1e-8 residual: learned-precond vs plain GPU MG"
+HBM_PEAK_GBS = 8000.0
+ROWS, COLS = 1024, 2048
+# Algorithmic bytes per cell of the finest grid's fused two-pass smoother launched from given
+# values (the cycle's up leg): x read 4 + forcing 4 + mask 1 + x write 4.
+SMOOTH_BYTES_PER_CELL = 13
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(levels, budget_s: float, threads: int):
+    """The oracle's binary64 twin of the reference multigrid (C, OpenMP row bands) on the host,
+    solving the same 1024 x 2048 problem to 1e-8, repeated until `budget_s` of CPU time."""
+    import oracle
+
+    oracle.set_threads(threads)
+    ladder = [oracle.Stencil(s.cells_across, s.cells_down, s.is_water, s.east_weight, s.west_weight, s.north_weight,
+                             s.south_weight, s.forcing, s.centre_weight) for s in levels]
+    times = []
+    t_all = time.perf_counter()
+    while True:
+        t0 = time.perf_counter()
+        _, cycles, rr = oracle.solve64(ladder, 1e-8)
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() - t_all > budget_s or len(times) >= 20:
+            break
+    value = len(times) / sum(times)
+    return {"value": round(value, 4), "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"{len(times)} full 1024x2048 solves to 1e-8 ({cycles} V(2,2) cycles each) by the oracle's "
+                      f"binary64 C restatement of OceanCirculation.solve, {threads} OpenMP threads"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--tile-rows", type=int, default=16)
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-oracle-check", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the circulation solve is a single-GPU path here "
+              "(instances would shard one per GPU with no exchange); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import cartogenesis_amd as cg
+    from cartogenesis_amd.ocean import HipOcean, Plan
+    from cartogenesis_amd.problem import circulation_problem
+
+    log("bench: building the 1024x2048 problem")
+    prob = circulation_problem(ROWS, COLS, seed=0, land_fraction=0.3)
+    res = HipOcean.create_or_null(0)
+    if res.accelerator is None:
+        raise RuntimeError(f"no MI355X accelerator: {res.unavailable_because}")
+    acc = res.accelerator
+    acc.set_option(cg.CG_OPT_TILE_ROWS, args.tile_rows)
+    acc.set_option(cg.CG_OPT_USE_GRAPH, 0 if args.no_graph else 1)
+    plan = Plan(acc, prob.levels)
+    lib = cg.lib()
+    import ctypes
+
+    log(f"bench: warmup {args.warmup}")
+    for _ in range(args.warmup):
+        info = plan.solve(args.tol, cg.CG_METHOD_REFINED)
+    lib.cg_ocean_kernel_time_us(acc.ctx, -1, None)
+    log(f"bench: timing {args.steps} solves")
+    cycles = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        info = plan.solve(args.tol, cg.CG_METHOD_REFINED)
+        cycles.append(info.cycles)
+    elapsed = time.perf_counter() - t0
+    value = args.steps / elapsed
+    rel_res = info.relative_residual
+    x64, _ = plan.read()
+    log(f"bench: {value:.1f} solves/s, {np.mean(cycles):.1f} cycles, residual {rel_res:.3e}")
+
+    # Kernel durations of the finest smoother, with HIP events on the library's stream, over a
+    # second pass of the same solves with graph replay off (events bracket single launches).
+    acc.set_option(cg.CG_OPT_TIME_KERNELS, 1)
+    lib.cg_ocean_kernel_time_us(acc.ctx, -1, None)
+    n_timed = max(5, min(args.steps, 20))
+    t1 = time.perf_counter()
+    for _ in range(n_timed):
+        plan.solve(args.tol, cg.CG_METHOD_REFINED)
+    eager = n_timed / (time.perf_counter() - t1)
+    cnt = ctypes.c_long(0)
+    us_up = lib.cg_ocean_kernel_time_us(acc.ctx, 1, ctypes.byref(cnt))
+    n_up = cnt.value
+    us_zero = lib.cg_ocean_kernel_time_us(acc.ctx, 0, ctypes.byref(cnt))
+    acc.set_option(cg.CG_OPT_TIME_KERNELS, 0)
+    cells = ROWS * COLS
+    traffic = SMOOTH_BYTES_PER_CELL * cells
+    achieved = traffic / (us_up * 1e-6) / 1e9 if us_up > 0 else None
+    log(f"bench: smoother from values {us_up:.2f} us ({n_up} launches), from zero {us_zero:.2f} us; "
+        f"eager {eager:.1f} solves/s")
+
+    err = None
+    cpu = None
+    if not args.no_oracle_check or not args.no_cpu:
+        import oracle
+        threads = min(16, os.cpu_count() or 1)
+        oracle.set_threads(threads)
+        ladder = [oracle.Stencil(s.cells_across, s.cells_down, s.is_water, s.east_weight, s.west_weight,
+                                 s.north_weight, s.south_weight, s.forcing, s.centre_weight) for s in prob.levels]
+        if not args.no_oracle_check:
+            log("bench: binary64 oracle to 1e-12 for the error report")
+            ref, _, _ = oracle.solve64(ladder, 1e-12)
+            err = float(np.linalg.norm(x64 - ref) / np.linalg.norm(ref))
+        if not args.no_cpu:
+            log("bench: CPU baseline")
+            cpu = cpu_baseline(prob.levels, args.cpu_budget, threads)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "circulation 1024x2048 (seed 0, 30% land, annual forcing) from zero to 1e-8: plain GPU "
+                        "multigrid, binary32 V(2,2) red-black cycles of the reference ladder inside binary64 "
+                        "iterative refinement; learned preconditioner not built",
+            "rows": ROWS, "cols": COLS, "tol": args.tol, "method": "refined",
+            "graph": not args.no_graph, "tile_rows": args.tile_rows,
+            "cycles_per_solve": float(np.mean(cycles)),
+            "relative_residual": rel_res,
+            "rel_l2_vs_fp64_oracle": err,
+            "eager_solves_per_s": round(eager, 3),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": f"k_relax_tile<4,{args.tile_rows},false> (finest grid, 2 fused red-black passes)",
+            "achieved": None if achieved is None else round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "avg_launch_us": round(us_up, 3),
+            "launches": n_up,
+            "algorithmic_bytes_per_launch": traffic,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    plan.close()
+    acc.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -32,6 +32,25 @@ __device__ __forceinline__ double centre_at(const LevelArgs &L, int row, long ce
     return L.rowc ? L.rowc[row] : L.cellc[cell];
 }
 
+// Lane i receives lane i-1's value (wave_shr:1) / lane i+1's (wave_shl:1); the edge lane keeps
+// its own, which only a tile's halo ever reads.
+__device__ __forceinline__ float lane_from_below(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, v), __builtin_bit_cast(int, v),
+                                                                 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_from_above(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, v), __builtin_bit_cast(int, v),
+                                                                 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_value(float v, int lane) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+// v where bit `bit` of m is set, else +0 (one sign-extending bit extract and one AND).
+__device__ __forceinline__ float keep_if(float v, uint32_t m, int bit) {
+    const int sel = __builtin_amdgcn_sbfe((int)m, bit, 1);
+    return __builtin_bit_cast(float, __builtin_bit_cast(int, v) & sel);
+}
+
 // Binary32 update of one cell in the reference's order: ((e xE + w xW) + n xN) + s xS - f.
 __device__ __forceinline__ float rb_update(float4 wt, float xe, float xw, float xn, float xs, float f) {
     return wt.x * xe + wt.y * xw + wt.z * xn + wt.w * xs - f;
@@ -40,25 +59,57 @@ __device__ __forceinline__ float rb_update(float4 wt, float xe, float xw, float 
 // ------------------------------------------------------------------------------------------------
 // Fused red-black smoother, row-constant weights.
 //
-// One wavefront owns a tile of TH rows by (128 - 2 HC) columns plus a halo of HR = NH rows and
-// HC (NH rounded up to even) columns. Lane l holds the two adjacent columns 2l and 2l+1 of every
-// tile row in registers (A = even column, B = odd column), so in each half-sweep every lane updates
-// exactly one of its two cells per row: east/west neighbours are the lane's other register or its
-// neighbour lane's (one cross-lane shuffle per row), north/south neighbours are the same lane's
-// registers of the adjacent rows. NH half-sweeps (colour 0 first) run without any barrier or LDS;
-// each half-sweep invalidates one more cell of the halo, so after NH the interior is exact and is
-// written back. Rows beyond a pole are zero and never change, like the reference's zero beyond the
-// pole; columns wrap by modular loading. Reads xin, writes xout (the two must differ: neighbouring
-// tiles read the halo this tile would otherwise overwrite).
+// One wavefront owns a tile of TH rows by IW = 128 - 2 HC columns plus a halo of HR rows and HC
+// columns. Lane l holds the two adjacent columns 2l and 2l+1 of every tile row in registers
+// (A = even column, B = odd column), so in each half-sweep every lane updates exactly one of its
+// two cells per row: east/west neighbours are the lane's other register or its neighbour lane's
+// (one cross-lane shuffle per row), north/south neighbours are the same lane's registers of the
+// adjacent rows. NH half-sweeps (colour 0 first) run without any barrier or LDS; each invalidates
+// one more cell of the halo, so after NH the interior is exact and is written back. Rows beyond a
+// pole are zero and never change, like the reference's zero beyond the pole (each array the
+// smoother reads carries one all-zero row after its last, which such rows load); columns wrap by
+// modular loading. Reads xin, writes xout (the two must differ: neighbouring tiles read the halo
+// this tile would otherwise overwrite).
+//
+// MODE selects what is fused around the smoothing, for a ladder whose next grid halves both axes:
+//   TILE_PLAIN    : xout = NH half-sweeps of xin.
+//   TILE_DOWN     : the cycle's down leg: from zero (xin unread), then the binary32 residual
+//                   u = x - (((e xE + w xW) + n xN) + s xS - f) of the interior, restricted as
+//                   sum u * g(row) over each 2x2 block into the next grid's forcing fc (zero on its
+//                   land), g(row) = centre(row) * 1/4 / centre_coarse(row / 2) in binary32.
+//   TILE_UP       : the cycle's up leg: before smoothing, the coarse correction xc is added to the
+//                   water cells bilinearly in prolongAdd's binary32 arithmetic (OC:354-392).
 // ------------------------------------------------------------------------------------------------
-template <int NH, int TH, bool ZERO_START>
+enum TileMode { TILE_PLAIN = 0, TILE_DOWN = 1, TILE_UP = 2 };
+
+struct TileExtra {
+    // TILE_DOWN
+    const uint8_t *cwater;  // next grid's mask
+    const float *g;         // per fine row restriction factor
+    float *fc;              // next grid's forcing (out)
+    // TILE_UP
+    const float *xc;        // next grid's values
+    int ca, cd;             // next grid's shape
+    float beyond;           // -1 for a wall at the poles
+};
+
+template <int NH, int MODE>
+struct TileGeom {
+    static constexpr int HR = NH + (MODE == TILE_DOWN ? 1 : 0);
+    static constexpr int HC = MODE == TILE_PLAIN ? ((NH + 1) & ~1) : ((NH + 2) & ~1);
+    static constexpr int IW = 128 - 2 * HC;
+};
+
+template <int NH, int TH, bool ZERO_START, int MODE>
 __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__restrict__ f,
-                                                    const float *__restrict__ xin, float *__restrict__ xout) {
-    constexpr int HR = NH;
-    constexpr int HC = (NH + 1) & ~1;
+                                                    const float *__restrict__ xin, float *__restrict__ xout,
+                                                    TileExtra ex) {
+    constexpr int HR = TileGeom<NH, MODE>::HR;
+    constexpr int HC = TileGeom<NH, MODE>::HC;
+    constexpr int IW = TileGeom<NH, MODE>::IW;
     constexpr int R = TH + 2 * HR;
-    constexpr int IW = 128 - 2 * HC;
     static_assert(R <= 64, "tile rows must fit a 64-bit row mask");
+    static_assert(TH % 2 == 0, "tiles start on even rows");
     const int across = L.across, down = L.down;
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -70,29 +121,81 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
     int ca = (col0 - HC + 2 * lane) % across;
     if (ca < 0) ca += across;
 
+    // Lane j holds the weights of tile row j (rows beyond a pole take the edge row's; their cells
+    // are never water, so the weights only multiply zeros); readlane hands them out per row.
+    float4 wrow = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (lane < R) {
+        int rr = row0 - HR + lane;
+        rr = rr < 0 ? 0 : (rr >= down ? down - 1 : rr);
+        wrow = L.roww[rr];
+    }
     float A[R], B[R], fA[R], fB[R];
     uint64_t wA = 0, wB = 0;
+    // Branch-free loads (a row beyond a pole loads the edge row and is then zeroed), so that all
+    // of the tile's loads are in flight together.
 #pragma unroll
     for (int i = 0; i < R; ++i) {
+        // A row beyond a pole reads row `down`, the all-zero row every array the smoother reads
+        // carries past its last row: zero values, zero forcing, no water. The device mask holds
+        // 0 or 1 per cell.
         const int r = row0 - HR + i;
-        if (r >= 0 && r < down) {
-            const long idx = (long)r * across + ca;
-            const float2 fv = *reinterpret_cast<const float2 *>(f + idx);
-            const uint16_t m = *reinterpret_cast<const uint16_t *>(L.water + idx);
-            fA[i] = fv.x;
-            fB[i] = fv.y;
-            wA |= (uint64_t)((m & 0xff) != 0) << i;
-            wB |= (uint64_t)((m >> 8) != 0) << i;
-            if (ZERO_START) {
-                A[i] = 0.0f;
-                B[i] = 0.0f;
-            } else {
-                const float2 xv = *reinterpret_cast<const float2 *>(xin + idx);
-                A[i] = xv.x;
-                B[i] = xv.y;
-            }
+        const int rc = (r >= 0 && r < down) ? r : down;
+        const long idx = (long)rc * across + ca;
+        const float2 fv = *reinterpret_cast<const float2 *>(f + idx);
+        const uint32_t m = *reinterpret_cast<const uint16_t *>(L.water + idx);
+        fA[i] = fv.x;
+        fB[i] = fv.y;
+        wA |= (uint64_t)(m & 1u) << i;
+        wB |= (uint64_t)((m >> 8) & 1u) << i;
+        if (ZERO_START) {
+            A[i] = 0.0f;
+            B[i] = 0.0f;
         } else {
-            fA[i] = fB[i] = A[i] = B[i] = 0.0f;
+            const float2 xv = *reinterpret_cast<const float2 *>(xin + idx);
+            A[i] = xv.x;
+            B[i] = xv.y;
+        }
+    }
+
+    if constexpr (MODE == TILE_UP) {
+        // Fine row r = row0 - HR + i reads coarse rows above = floor((r - 1) / 2) and above + 1,
+        // i.e. row0 / 2 + q with q = floor((i - HR - 1) / 2): a compile-time offset, since row0 is
+        // even. Lane l's pair (2m, 2m+1) reads coarse columns m-1, m, m+1 (from lanes l-1, l, l+1).
+        constexpr int QMIN = -((HR + 2) / 2);  // floor((0 - HR - 1) / 2)
+        constexpr int QMAX = (R - HR - 2) >= 0 ? (R - HR - 2) / 2 + 1 : 0;
+        constexpr int NQ = QMAX - QMIN + 1;
+        const int m = ca >> 1;
+        const int kc0 = row0 >> 1;
+        float cm[NQ], cl[NQ], cr[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            // prolongAdd's `at` (OC:380-385): beyond a pole, the edge row times `beyond`. In range
+            // the factor is 1, and 1 * v is v exactly.
+            const int kr = kc0 + QMIN + q;
+            const int kc = kr < 0 ? 0 : (kr >= ex.cd ? ex.cd - 1 : kr);
+            const float fac = (kr < 0 || kr >= ex.cd) ? ex.beyond : 1.0f;
+            const float v = fac * ex.xc[(long)kc * ex.ca + m];
+            cm[q] = v;
+            cl[q] = lane_from_below(v);
+            cr[q] = lane_from_above(v);
+        }
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            // q index of `above` for this row, relative to QMIN.
+            constexpr int dummy = 0;
+            (void)dummy;
+            const int qa = ((i - HR - 1) >= 0 ? (i - HR - 1) / 2 : -((HR + 1 - i + 1) / 2)) - QMIN;
+            const bool r_even = ((i + HR) & 1) == 0;
+            const float db = r_even ? 0.75f : 0.25f;
+            // Column 2m: west m-1, east m, blend 0.75; column 2m+1: west m, east m+1, blend 0.25.
+            const float topA = cl[qa] * (1.0f - 0.75f) + cm[qa] * 0.75f;
+            const float botA = cl[qa + 1] * (1.0f - 0.75f) + cm[qa + 1] * 0.75f;
+            const float topB = cm[qa] * (1.0f - 0.25f) + cr[qa] * 0.25f;
+            const float botB = cm[qa + 1] * (1.0f - 0.25f) + cr[qa + 1] * 0.25f;
+            const uint32_t mA = i < 32 ? (uint32_t)wA : (uint32_t)(wA >> 32);
+            const uint32_t mB = i < 32 ? (uint32_t)wB : (uint32_t)(wB >> 32);
+            if ((mA >> (i & 31)) & 1) A[i] += topA * (1.0f - db) + botA * db;
+            if ((mB >> (i & 31)) & 1) B[i] += topB * (1.0f - db) + botB * db;
         }
     }
 
@@ -108,30 +211,67 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
             const int i = t - h;
             if (i < 1 || i > R - 2) continue;
             const int colour = h & 1;
-            int r = row0 - HR + i;
-            r = r < 0 ? 0 : (r >= down ? down - 1 : r);
-            const float4 wt = L.roww[r];
+            const float4 wt = make_float4(lane_value(wrow.x, i), lane_value(wrow.y, i), lane_value(wrow.z, i),
+                                          lane_value(wrow.w, i));
+            const uint32_t mA = i < 32 ? (uint32_t)wA : (uint32_t)(wA >> 32);
+            const uint32_t mB = i < 32 ? (uint32_t)wB : (uint32_t)(wB >> 32);
             // Row (row0 - HR + i) has the parity of (i + HR) since row0 is even; the even column
             // of the pair has colour (row & 1).
             if (((i + HR) & 1) == colour) {
-                const float xw = __shfl_up(B[i], 1, 64);
-                const float v = rb_update(wt, B[i], xw, A[i - 1], A[i + 1], fA[i]);
-                A[i] = ((wA >> i) & 1) ? v : 0.0f;
+                const float xw = lane_from_below(B[i]);
+                A[i] = keep_if(rb_update(wt, B[i], xw, A[i - 1], A[i + 1], fA[i]), mA, i & 31);
             } else {
-                const float xe = __shfl_down(A[i], 1, 64);
-                const float v = rb_update(wt, xe, A[i], B[i - 1], B[i + 1], fB[i]);
-                B[i] = ((wB >> i) & 1) ? v : 0.0f;
+                const float xe = lane_from_above(A[i]);
+                B[i] = keep_if(rb_update(wt, xe, A[i], B[i - 1], B[i + 1], fB[i]), mB, i & 31);
             }
         }
+        // Keep each step's weights local to the step: without this the scheduler hoists every
+        // row's readlanes to the top and spills the scalars.
+        __builtin_amdgcn_sched_barrier(0);
     }
 
     const int lo = HC / 2;
     const int cabs = col0 + 2 * (lane - lo);
-    if (lane >= lo && lane < lo + IW / 2 && cabs < across) {
+    const bool writer = lane >= lo && lane < lo + IW / 2 && cabs < across;
+    if (writer) {
 #pragma unroll
         for (int i = HR; i < HR + TH; ++i) {
             const int r = row0 - HR + i;
             if (r < down) *reinterpret_cast<float2 *>(xout + (long)r * across + cabs) = make_float2(A[i], B[i]);
+        }
+    }
+
+    if constexpr (MODE == TILE_DOWN) {
+        const int cacross = across >> 1;
+#pragma unroll
+        for (int i = HR; i < HR + TH; i += 2) {
+            const int r = row0 - HR + i;  // even
+            const bool in = r < down;
+            float u[4];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int ii = i + k;
+                const float4 wt = make_float4(lane_value(wrow.x, ii), lane_value(wrow.y, ii), lane_value(wrow.z, ii),
+                                              lane_value(wrow.w, ii));
+                const float xw = lane_from_below(B[ii]);
+                const float xe = lane_from_above(A[ii]);
+                const float ua = A[ii] - rb_update(wt, B[ii], xw, A[ii - 1], A[ii + 1], fA[ii]);
+                const float ub = B[ii] - rb_update(wt, xe, A[ii], B[ii - 1], B[ii + 1], fB[ii]);
+                const uint32_t mA = ii < 32 ? (uint32_t)wA : (uint32_t)(wA >> 32);
+                const uint32_t mB = ii < 32 ? (uint32_t)wB : (uint32_t)(wB >> 32);
+                u[2 * k] = keep_if(ua, mA, ii & 31);
+                u[2 * k + 1] = keep_if(ub, mB, ii & 31);
+            }
+            if (writer && in) {
+                const long cidx = (long)(r >> 1) * cacross + (cabs >> 1);
+                const float g0 = ex.g[r], g1 = ex.g[r + 1];
+                float acc = 0.0f;
+                acc += u[0] * g0;
+                acc += u[1] * g0;
+                acc += u[2] * g1;
+                acc += u[3] * g1;
+                ex.fc[cidx] = ex.cwater[cidx] ? acc : 0.0f;
+            }
         }
     }
 }
@@ -165,25 +305,47 @@ __device__ __forceinline__ unsigned long long abs_bits(double v) {
     return (unsigned long long)__double_as_longlong(v) & 0x7fffffffffffffffull;
 }
 
-__device__ __forceinline__ void block_max_atomic(unsigned long long v, unsigned long long *out) {
-    // Wave reduction then one atomic per wave.
+// A grid-wide maximum: each block reduces in registers and LDS, then its thread 0 does one relaxed
+// device-scope atomicMax into *out (no fences, so no L2 write-back per block). *out must be zero
+// when the kernel starts; block 0 zeroes *clear (the slot the next reduction will use), which a
+// later launch on the same stream then sees, so graph replays need no memset.
+struct MaxReduce {
+    unsigned long long *out;
+    unsigned long long *clear;
+};
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         const unsigned long long o = __shfl_xor(v, off, 64);
         v = o > v ? o : v;
     }
-    if ((threadIdx.x & 63) == 0 && v) atomicMax(out, v);
+    return v;
+}
+
+__device__ void grid_max(unsigned long long v, const MaxReduce &red) {
+    __shared__ unsigned long long sh[16];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && red.clear) *red.clear = 0ull;
+    v = wave_max_u64(v);
+    const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = 0;
+        for (int i = 0; i < nw; ++i) m = sh[i] > m ? sh[i] : m;
+        if (m) atomicMax(red.out, m);
+    }
 }
 
 // Binary64 residual (OC:292-315): r = (x - (sum w x - f)) * centre on water, 0 on land; and its
-// largest magnitude into *rmax when given.
-__global__ void k_residual64(LevelArgs L, const float *__restrict__ f, const float *__restrict__ x,
-                             double *__restrict__ r, unsigned long long *rmax) {
+// largest magnitude into red.out when red.out is given. Grid-stride over cells.
+__global__ __launch_bounds__(256) void k_residual64(LevelArgs L, const float *__restrict__ f,
+                                                    const float *__restrict__ x, double *__restrict__ r,
+                                                    MaxReduce red) {
     const int across = L.across;
     const long cells = (long)across * L.down;
-    const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long m = 0;
-    if (cell < cells) {
+    for (long cell = (long)blockIdx.x * blockDim.x + threadIdx.x; cell < cells; cell += (long)gridDim.x * blockDim.x) {
         double out = 0.0;
         if (L.water[cell]) {
             const int row = (int)(cell / across);
@@ -199,21 +361,22 @@ __global__ void k_residual64(LevelArgs L, const float *__restrict__ f, const flo
             out = ((double)x[cell] - bal) * centre_at(L, row, cell);
         }
         if (r) r[cell] = out;
-        m = abs_bits(out);
+        const unsigned long long b = abs_bits(out);
+        m = b > m ? b : m;
     }
-    if (rmax) block_max_atomic(m, rmax);
+    if (red.out) grid_max(m, red);
 }
 
 // The largest magnitude of f * centre (balanceOf then largest, OC:325-327, 318-322).
-__global__ void k_balance_max(LevelArgs L, const float *__restrict__ f, unsigned long long *out) {
+__global__ __launch_bounds__(256) void k_balance_max(LevelArgs L, const float *__restrict__ f, MaxReduce red) {
     const long cells = (long)L.across * L.down;
-    const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long m = 0;
-    if (cell < cells) {
+    for (long cell = (long)blockIdx.x * blockDim.x + threadIdx.x; cell < cells; cell += (long)gridDim.x * blockDim.x) {
         const int row = (int)(cell / L.across);
-        m = abs_bits((double)f[cell] * centre_at(L, row, cell));
+        const unsigned long long b = abs_bits((double)f[cell] * centre_at(L, row, cell));
+        m = b > m ? b : m;
     }
-    block_max_atomic(m, out);
+    grid_max(m, red);
 }
 
 // restrict (OC:334-346): each coarse cell sums fine * share over the fine cells it covers in the
@@ -380,15 +543,15 @@ __global__ __launch_bounds__(1024) void k_coarsest_relax(LevelArgs L, const floa
 
 // The outer step: x_new = x + delta (binary64; delta may be null), u = x_new - (sum w x_new - f)
 // in binary64, the correction forcing fc = (float)u on water (0 on land), and the largest
-// |u * centre| into *rmax. Land values are zero in x_new.
-__global__ void k_outer_residual(LevelArgs L, const float *__restrict__ f, const double *__restrict__ x,
-                                 const float *__restrict__ delta, double *__restrict__ xnew, float *__restrict__ fc,
-                                 unsigned long long *rmax) {
+// |u * centre| into red.out. Land values are zero in x_new. Grid-stride over cells.
+__global__ __launch_bounds__(256) void k_outer_residual(LevelArgs L, const float *__restrict__ f,
+                                                        const double *__restrict__ x, const float *__restrict__ delta,
+                                                        double *__restrict__ xnew, float *__restrict__ fc,
+                                                        MaxReduce red) {
     const int across = L.across;
     const long cells = (long)across * L.down;
-    const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long m = 0;
-    if (cell < cells) {
+    for (long cell = (long)blockIdx.x * blockDim.x + threadIdx.x; cell < cells; cell += (long)gridDim.x * blockDim.x) {
         const int row = (int)(cell / across), col = (int)(cell - (long)row * across);
         double v = 0.0, u = 0.0;
         if (L.water[cell]) {
@@ -403,12 +566,13 @@ __global__ void k_outer_residual(LevelArgs L, const float *__restrict__ f, const
             const double bal = (double)wt.x * val(ie) + (double)wt.y * val(iw) + (double)wt.z * vn + (double)wt.w * vs -
                                (double)f[cell];
             u = v - bal;
-            m = abs_bits(u * centre_at(L, row, cell));
+            const unsigned long long b = abs_bits(u * centre_at(L, row, cell));
+            m = b > m ? b : m;
         }
         xnew[cell] = v;
         fc[cell] = (float)u;
     }
-    block_max_atomic(m, rmax);
+    grid_max(m, red);
 }
 
 // Binary32 residual of the correction problem restricted to the next grid, fused: each thread owns
@@ -460,6 +624,185 @@ __global__ __launch_bounds__(256) void k_coarsest_direct(const int *__restrict__
         float acc = 0.0f;
         for (int j = 0; j < n; ++j) acc = fmaf(Mt[(long)j * n + i], fs[j], acc);
         x[idx[i]] = acc;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// The bottom of the binary32 correction cycle in one workgroup: every level from `first` down to
+// the coarsest (each small enough that all of them fit in LDS together), relaxed, restricted,
+// solved directly at the coarsest, prolonged and relaxed again, with __syncthreads between
+// phases. Same arithmetic as the multi-workgroup kernels: binary32 updates in the reference's
+// order, binary32 residual restricted with the per-row factor g, prolongAdd's bilinear weights.
+// ------------------------------------------------------------------------------------------------
+constexpr int kBottomMaxLevels = 8;
+
+struct BottomLevel {
+    int across, down;
+    const uint8_t *water;
+    const float4 *roww;
+    const float *g;  // restriction factor per row to the next level (null at the coarsest)
+    int off_x, off_f, off_w;  // LDS byte offsets
+};
+
+struct BottomArgs {
+    int nlev;
+    BottomLevel lv[kBottomMaxLevels];
+    const float *f_in;  // forcing of the first level (global)
+    float *x_out;       // values of the first level (global)
+    const int *idx;     // coarsest water cells, and M = -(I - W)^-1 column-major (nw x nw)
+    const float *Mt;
+    int nw;
+    int coarsest_passes;  // relaxation passes at the coarsest when there is no inverse
+    float beyond;
+};
+
+__device__ __forceinline__ void lds_relax_pass(const BottomLevel &L, float *x, const float *f, const uint8_t *w) {
+    const int across = L.across, half = across >> 1;
+    const int n = half * L.down;
+    for (int colour = 0; colour <= 1; ++colour) {
+        for (int t = threadIdx.x; t < n; t += blockDim.x) {
+            const int row = t / half;
+            const int col = 2 * (t - row * half) + ((colour + row) & 1);
+            const int cell = row * across + col;
+            if (!w[cell]) {
+                x[cell] = 0.0f;
+                continue;
+            }
+            const int ce = col + 1 == across ? 0 : col + 1;
+            const int cw = col == 0 ? across - 1 : col - 1;
+            const float xn = row > 0 ? x[cell - across] : 0.0f;
+            const float xs = row + 1 < L.down ? x[cell + across] : 0.0f;
+            x[cell] = rb_update(L.roww[row], x[row * across + ce], x[row * across + cw], xn, xs, f[cell]);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_bottom(BottomArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    auto X = [&](int l) { return reinterpret_cast<float *>(smem + a.lv[l].off_x); };
+    auto F = [&](int l) { return reinterpret_cast<float *>(smem + a.lv[l].off_f); };
+    auto W = [&](int l) { return reinterpret_cast<uint8_t *>(smem + a.lv[l].off_w); };
+    for (int l = 0; l < a.nlev; ++l) {
+        const int cells = a.lv[l].across * a.lv[l].down;
+        uint8_t *w = W(l);
+        float *x = X(l);
+        for (int c = threadIdx.x; c < cells; c += blockDim.x) {
+            w[c] = a.lv[l].water[c];
+            x[c] = 0.0f;
+        }
+    }
+    {
+        const int cells = a.lv[0].across * a.lv[0].down;
+        float *f = F(0);
+        for (int c = threadIdx.x; c < cells; c += blockDim.x) f[c] = a.f_in[c];
+    }
+    __syncthreads();
+    // Down.
+    for (int l = 0; l + 1 < a.nlev; ++l) {
+        const BottomLevel &L = a.lv[l];
+        const BottomLevel &C = a.lv[l + 1];
+        float *x = X(l);
+        const float *f = F(l);
+        const uint8_t *w = W(l);
+        lds_relax_pass(L, x, f, w);
+        lds_relax_pass(L, x, f, w);
+        const int cpc = L.across / C.across, rpc = L.down / C.down;
+        float *fc = F(l + 1);
+        const uint8_t *wc = W(l + 1);
+        for (int t = threadIdx.x; t < C.across * C.down; t += blockDim.x) {
+            if (!wc[t]) {
+                fc[t] = 0.0f;
+                continue;
+            }
+            const int crow = t / C.across, ccol = t - crow * C.across;
+            float acc = 0.0f;
+            for (int dr = 0; dr < rpc; ++dr) {
+                const int row = crow * rpc + dr;
+                const float4 wt = L.roww[row];
+                const float g = L.g[row];
+                for (int dc = 0; dc < cpc; ++dc) {
+                    const int col = ccol * cpc + dc;
+                    const int cell = row * L.across + col;
+                    if (!w[cell]) continue;
+                    const int ce = col + 1 == L.across ? 0 : col + 1;
+                    const int cw = col == 0 ? L.across - 1 : col - 1;
+                    const float xn = row > 0 ? x[cell - L.across] : 0.0f;
+                    const float xs = row + 1 < L.down ? x[cell + L.across] : 0.0f;
+                    const float u = x[cell] - rb_update(wt, x[row * L.across + ce], x[row * L.across + cw], xn, xs, f[cell]);
+                    acc += u * g;
+                }
+            }
+            fc[t] = acc;
+        }
+        __syncthreads();
+    }
+    // Coarsest.
+    {
+        const int l = a.nlev - 1;
+        const BottomLevel &C = a.lv[l];
+        float *x = X(l);
+        const float *f = F(l);
+        if (a.nw > 0) {
+            const int nw = a.nw;
+            const int parts = nw <= (int)blockDim.x ? (int)blockDim.x / nw : 1;
+            __shared__ float partial[1024];
+            for (int base = 0; base < nw; base += (int)blockDim.x / parts) {
+                const int i = base + (int)threadIdx.x % ((int)blockDim.x / parts);
+                const int p = (int)threadIdx.x / ((int)blockDim.x / parts);
+                float acc = 0.0f;
+                if (i < nw && p < parts)
+                    for (int j = p; j < nw; j += parts) acc = fmaf(a.Mt[(long)j * nw + i], f[a.idx[j]], acc);
+                partial[threadIdx.x] = acc;
+                __syncthreads();
+                if (p == 0 && i < nw) {
+                    float s = 0.0f;
+                    for (int q = 0; q < parts; ++q) s += partial[q * ((int)blockDim.x / parts) + (i - base)];
+                    x[a.idx[i]] = s;
+                }
+                __syncthreads();
+            }
+        } else {
+            for (int p = 0; p < a.coarsest_passes; ++p) lds_relax_pass(C, x, f, W(l));
+        }
+        __syncthreads();
+    }
+    // Up.
+    for (int l = a.nlev - 2; l >= 0; --l) {
+        const BottomLevel &L = a.lv[l];
+        const BottomLevel &C = a.lv[l + 1];
+        float *x = X(l);
+        const float *xc = X(l + 1);
+        const uint8_t *w = W(l);
+        const int cpc = L.across / C.across, rpc = L.down / C.down;
+        const int ca = C.across, cd = C.down;
+        for (int cell = threadIdx.x; cell < L.across * L.down; cell += blockDim.x) {
+            if (!w[cell]) continue;
+            const int row = cell / L.across, col = cell - row * L.across;
+            const float crow = ((float)row + 0.5f) / (float)rpc - 0.5f;
+            const int above = (int)floorf(crow);
+            const float db = crow - (float)above;
+            const float ccol = ((float)col + 0.5f) / (float)cpc - 0.5f;
+            const int left = (int)floorf(ccol);
+            const float ab = ccol - (float)left;
+            const int wc = (left + ca) % ca, ec = (left + 1) % ca;
+            auto at = [&](int rr, int cc) -> float {
+                if (rr < 0) return a.beyond * xc[cc];
+                if (rr >= cd) return a.beyond * xc[(cd - 1) * ca + cc];
+                return xc[rr * ca + cc];
+            };
+            const float top = at(above, wc) * (1.0f - ab) + at(above, ec) * ab;
+            const float bottom = at(above + 1, wc) * (1.0f - ab) + at(above + 1, ec) * ab;
+            x[cell] += top * (1.0f - db) + bottom * db;
+        }
+        __syncthreads();
+        lds_relax_pass(L, x, F(l), w);
+        lds_relax_pass(L, x, F(l), w);
+    }
+    {
+        const int cells = a.lv[0].across * a.lv[0].down;
+        const float *x = X(0);
+        for (int c = threadIdx.x; c < cells; c += blockDim.x) a.x_out[c] = x[c];
     }
 }
 

@@ -60,7 +60,13 @@ struct Stencil {
 
 struct TimedLaunch {
     hipEvent_t a, b;
+    int kind;
 };
+
+// Kernel kinds of the timing record: the finest grid's fused smoother from zero (the cycle's
+// down leg) and from given values (its up leg).
+constexpr int kKindSmoothFromZero = 0;
+constexpr int kKindSmooth = 1;
 
 }  // namespace
 
@@ -76,6 +82,9 @@ struct cg_ctx {
     int fgmres_m = 20;
     int steps_per_check = 1;
     int time_kernels = 0;
+    int fused = 1;
+    size_t lds_limit = 0;       // dynamic LDS a workgroup may use (k_bottom)
+    bool bottom_attr_set = false;
     std::map<uint64_t, Stencil *> stencils;
     uint64_t next_handle = 1;
     unsigned long long *h_pinned = nullptr;  // host-visible scalars
@@ -119,6 +128,21 @@ int dalloc(cg_ctx *ctx, T **p, size_t count) {
     return CG_OK;
 }
 
+// A grid array the smoother reads: cells plus one all-zero row after the last (row `down`),
+// which rows beyond a pole load. Zero-filled.
+template <typename T>
+int dalloc_grid(cg_ctx *ctx, T **p, int across, int down) {
+    const size_t n = (size_t)across * (down + 1);
+    int rc = dalloc(ctx, p, n);
+    if (rc) return rc;
+    hipError_t e = hipMemset(*p, 0, sizeof(T) * n);
+    if (e != hipSuccess) {
+        set_err(ctx, "hipMemset failed: %s", hipGetErrorString(e));
+        return CG_ERR_DEVICE;
+    }
+    return CG_OK;
+}
+
 void dfree(void *p) {
     if (p) (void)hipFree(p);
 }
@@ -145,7 +169,7 @@ Stencil *find_stencil(cg_ctx *ctx, cg_stencil h) {
 struct TimeScope {
     cg_ctx *ctx;
     TimedLaunch *tl = nullptr;
-    TimeScope(cg_ctx *c, bool on) : ctx(c) {
+    TimeScope(cg_ctx *c, bool on, int kind) : ctx(c) {
         if (!on || !ctx->time_kernels) return;
         if (ctx->timed_used >= 200000) {
             ctx->timed_dropped++;
@@ -157,6 +181,7 @@ struct TimeScope {
             ctx->timed.push_back(t);
         }
         tl = &ctx->timed[ctx->timed_used++];
+        tl->kind = kind;
         (void)hipEventRecord(tl->a, ctx->stream);
     }
     ~TimeScope() {
@@ -168,32 +193,39 @@ struct TimeScope {
 // in launches of two passes (and one of one pass for an odd count), ping-ponging between xin and
 // xout so that the final values land in xout. xin is read only when zero_start is false. For
 // per-cell weights the colour kernel runs in place on xout after copying xin.
-template <int TH>
-hipError_t launch_tile(hipStream_t s, const LevelArgs &L, const float *f, const float *xin, float *xout, int nh,
-                       bool zero) {
-    constexpr int NHMAX = 4;
-    const int hc = (nh + 1) & ~1;
-    const int iw = 128 - 2 * hc;
+template <int TH, int MODE>
+hipError_t launch_tile_mode(hipStream_t s, const LevelArgs &L, const float *f, const float *xin, float *xout, int nh,
+                            bool zero, const TileExtra &ex) {
+    const int iw = nh == 4 ? TileGeom<4, MODE>::IW : TileGeom<2, MODE>::IW;
     const long waves = (long)((L.across + iw - 1) / iw) * ((L.down + TH - 1) / TH);
     const unsigned blocks = (unsigned)((waves + 3) / 4);
-    (void)NHMAX;
     if (nh == 4) {
-        if (zero) hipLaunchKernelGGL((k_relax_tile<4, TH, true>), dim3(blocks), dim3(256), 0, s, L, f, xin, xout);
-        else hipLaunchKernelGGL((k_relax_tile<4, TH, false>), dim3(blocks), dim3(256), 0, s, L, f, xin, xout);
+        if (zero) hipLaunchKernelGGL((k_relax_tile<4, TH, true, MODE>), dim3(blocks), dim3(256), 0, s, L, f, xin, xout, ex);
+        else hipLaunchKernelGGL((k_relax_tile<4, TH, false, MODE>), dim3(blocks), dim3(256), 0, s, L, f, xin, xout, ex);
     } else {
-        if (zero) hipLaunchKernelGGL((k_relax_tile<2, TH, true>), dim3(blocks), dim3(256), 0, s, L, f, xin, xout);
-        else hipLaunchKernelGGL((k_relax_tile<2, TH, false>), dim3(blocks), dim3(256), 0, s, L, f, xin, xout);
+        if (zero) hipLaunchKernelGGL((k_relax_tile<2, TH, true, MODE>), dim3(blocks), dim3(256), 0, s, L, f, xin, xout, ex);
+        else hipLaunchKernelGGL((k_relax_tile<2, TH, false, MODE>), dim3(blocks), dim3(256), 0, s, L, f, xin, xout, ex);
     }
     return hipGetLastError();
 }
 
+template <int TH>
+hipError_t launch_tile(hipStream_t s, const LevelArgs &L, const float *f, const float *xin, float *xout, int nh,
+                       bool zero, int mode, const TileExtra &ex) {
+    switch (mode) {
+        case TILE_DOWN: return launch_tile_mode<TH, TILE_DOWN>(s, L, f, xin, xout, 4, true, ex);
+        case TILE_UP: return launch_tile_mode<TH, TILE_UP>(s, L, f, xin, xout, 4, false, ex);
+        default: return launch_tile_mode<TH, TILE_PLAIN>(s, L, f, xin, xout, nh, zero, ex);
+    }
+}
+
 hipError_t tile_dispatch(cg_ctx *ctx, const LevelArgs &L, const float *f, const float *xin, float *xout, int nh,
-                         bool zero) {
+                         bool zero, int mode = TILE_PLAIN, const TileExtra &ex = TileExtra{}) {
     switch (ctx->tile_rows) {
-        case 8: return launch_tile<8>(ctx->stream, L, f, xin, xout, nh, zero);
-        case 24: return launch_tile<24>(ctx->stream, L, f, xin, xout, nh, zero);
-        case 32: return launch_tile<32>(ctx->stream, L, f, xin, xout, nh, zero);
-        default: return launch_tile<16>(ctx->stream, L, f, xin, xout, nh, zero);
+        case 8: return launch_tile<8>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex);
+        case 24: return launch_tile<24>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex);
+        case 32: return launch_tile<32>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex);
+        default: return launch_tile<16>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex);
     }
 }
 
@@ -209,7 +241,7 @@ int relax_pp(cg_ctx *ctx, const Stencil *st, const float *f, float *x[2], int *c
         bool zero = zero_start;
         while (left > 0) {
             const int nh = left >= 2 ? 4 : 2;
-            TimeScope ts(ctx, timed);
+            TimeScope ts(ctx, timed && nh == 4, zero ? kKindSmoothFromZero : kKindSmooth);
             hipError_t e = tile_dispatch(ctx, L, f, x[*cur], x[1 - *cur], nh, zero);
             if (e != hipSuccess) {
                 set_err(ctx, "smoother launch failed: %s", hipGetErrorString(e));
@@ -246,6 +278,7 @@ struct PlanLevel {
     float *x[2] = {nullptr, nullptr};
     int cur = 0;
     double *bal = nullptr;      // restricted balance (reference method), this level's size
+    float *g = nullptr;         // per row: centre * share / centre of the next level (binary32)
 };
 
 struct cg_plan {
@@ -259,7 +292,9 @@ struct cg_plan {
     int x64cur = 0;
     bool x64_valid = false;
     float *x32ref = nullptr;      // reference-method answer lives in lv[0].x[lv[0].cur]
-    unsigned long long *d_u64 = nullptr;  // [0] residual max, [1] balance max, [2..] scratch
+    // Maxima: [0], [1] the refined method's residual (by binary64 parity), [2] the largest
+    // balance, [3] the reference method's residual.
+    unsigned long long *d_u64 = nullptr;
     int *d_passes = nullptr;
     double largest = -1.0;        // max |f0 * c0|, cached per forcing
     // Coarsest direct solve.
@@ -270,6 +305,12 @@ struct cg_plan {
     std::vector<double *> kv;
     // Graphs of one refinement step, for each parity of the binary64 ping-pong.
     hipGraphExec_t step_exec[2] = {nullptr, nullptr};
+    // The fused cycle: tile levels 0..bottom-1 run fused down/up kernels, levels bottom..n-1 run
+    // in one workgroup (k_bottom) with bottom_lds bytes of LDS; fusable is false when a level
+    // has per-cell weights or does not halve both axes, and the unfused cycle runs instead.
+    bool fusable = false;
+    int bottom = 0;
+    size_t bottom_lds = 0;
     int last_answer = 0;  // 0: reference (binary32 in lv[0]); 1: binary64 in x64[x64cur]
 };
 
@@ -282,6 +323,7 @@ void free_plan(cg_plan *p) {
         dfree(p->lv[i].x[0]);
         dfree(p->lv[i].x[1]);
         dfree(p->lv[i].bal);
+        dfree(p->lv[i].g);
     }
     dfree(p->f0);
     dfree(p->r64);
@@ -368,6 +410,64 @@ int build_coarsest_inverse(cg_ctx *ctx, cg_plan *p) {
     return CG_OK;
 }
 
+// The dynamic LDS k_bottom may use on this device (its static LDS is 4 KiB of partial sums).
+size_t bottom_lds_limit(cg_ctx *ctx) {
+    if (ctx->bottom_attr_set) return ctx->lds_limit;
+    ctx->bottom_attr_set = true;
+    int optin = 0, plain = 0;
+    (void)hipDeviceGetAttribute(&plain, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device);
+    (void)hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, ctx->device);
+    size_t total = (size_t)std::max(optin, plain);
+    if (total < 16384) total = 65536;
+    size_t dyn = total - 4096 - 256;
+    if (hipFuncSetAttribute((const void *)k_bottom, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) != hipSuccess) {
+        (void)hipGetLastError();
+        dyn = (size_t)plain > 4096 + 256 ? (size_t)plain - 4096 - 256 : 0;
+    }
+    ctx->lds_limit = dyn;
+    return dyn;
+}
+
+// Restriction factors per row, and which levels the fused cycle runs in one workgroup.
+int plan_fusion_setup(cg_ctx *ctx, cg_plan *p) {
+    const int n = p->n;
+    bool rowconst = true;
+    for (int l = 0; l < n; ++l) rowconst = rowconst && p->lv[l].st->rowconst;
+    for (int l = 0; l + 1 < n; ++l) {
+        const Stencil *F = p->lv[l].st, *C = p->lv[l + 1].st;
+        if (!F->rowconst || !C->rowconst) continue;
+        const int rpc = F->down / C->down, cpc = F->across / C->across;
+        const double share = 1.0 / (double)(cpc * rpc);
+        std::vector<float> g(F->down);
+        for (int row = 0; row < F->down; ++row) g[row] = (float)(F->h_c[row] * share / C->h_c[row / rpc]);
+        int rc;
+        if ((rc = dalloc(ctx, &p->lv[l].g, F->down))) return rc;
+        HIPCHK(ctx, hipMemcpy(p->lv[l].g, g.data(), sizeof(float) * F->down, hipMemcpyHostToDevice));
+    }
+    p->fusable = false;
+    if (!rowconst) return CG_OK;
+    const size_t limit = bottom_lds_limit(ctx);
+    auto level_bytes = [&](int l) {
+        const size_t cells = (size_t)p->lv[l].st->cells;
+        return 2 * ((cells * 4 + 15) & ~(size_t)15) + ((cells + 15) & ~(size_t)15);
+    };
+    int b = n - 1;
+    size_t bytes = level_bytes(b);
+    if (bytes > limit || n - b > kBottomMaxLevels) return CG_OK;
+    while (b > 0 && n - (b - 1) <= kBottomMaxLevels && bytes + level_bytes(b - 1) <= limit) {
+        --b;
+        bytes += level_bytes(b);
+    }
+    for (int l = 0; l < b; ++l) {
+        const Stencil *F = p->lv[l].st, *C = p->lv[l + 1].st;
+        if (F->across != 2 * C->across || F->down != 2 * C->down || (F->down & 1)) return CG_OK;
+    }
+    p->bottom = b;
+    p->bottom_lds = bytes;
+    p->fusable = true;
+    return CG_OK;
+}
+
 int read_u64(cg_plan *p, int slot, unsigned long long *v) {
     cg_ctx *ctx = p->ctx;
     HIPCHK(ctx, hipMemcpyAsync(ctx->h_pinned + slot, p->d_u64 + slot, sizeof(unsigned long long), hipMemcpyDeviceToHost,
@@ -383,8 +483,17 @@ double bits_to_double(unsigned long long b) {
     return d;
 }
 
-void zero_u64(cg_plan *p, int slot, int n) {
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, p->ctx->stream, p->d_u64 + slot, n, 0ull);
+constexpr int kReduceBlocks = 2048;
+
+inline unsigned reduce_grid(long cells) { return (unsigned)std::min<long>(blocks_for(cells, 256), kReduceBlocks); }
+
+// A reduction into slot `slot`, zeroed first by a fill kernel.
+MaxReduce max_into(cg_plan *p, int slot) {
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, p->ctx->stream, p->d_u64 + slot, 1, 0ull);
+    MaxReduce r;
+    r.out = p->d_u64 + slot;
+    r.clear = nullptr;
+    return r;
 }
 
 // max |f0 * c0|: the reference's largest balance (OC:483).
@@ -395,12 +504,11 @@ int plan_largest(cg_plan *p, double *out) {
     }
     cg_ctx *ctx = p->ctx;
     const Stencil *st = p->lv[0].st;
-    zero_u64(p, 1, 1);
-    hipLaunchKernelGGL(k_balance_max, dim3(blocks_for(st->cells, 256)), dim3(256), 0, ctx->stream, st->args(), p->f0,
-                       p->d_u64 + 1);
+    const MaxReduce red = max_into(p, 2);
+    hipLaunchKernelGGL(k_balance_max, dim3(reduce_grid(st->cells)), dim3(256), 0, ctx->stream, st->args(), p->f0, red);
     HIPCHK(ctx, hipGetLastError());
     unsigned long long b;
-    int rc = read_u64(p, 1, &b);
+    int rc = read_u64(p, 2, &b);
     if (rc) return rc;
     p->largest = bits_to_double(b);
     *out = p->largest;
@@ -413,12 +521,12 @@ int plan_largest(cg_plan *p, double *out) {
 int ref_residual(cg_plan *p, const float *f, double *out) {
     cg_ctx *ctx = p->ctx;
     const PlanLevel &l0 = p->lv[0];
-    zero_u64(p, 0, 1);
-    hipLaunchKernelGGL(k_residual64, dim3(blocks_for(l0.st->cells, 256)), dim3(256), 0, ctx->stream, l0.st->args(), f,
-                       l0.x[l0.cur], (double *)nullptr, p->d_u64);
+    const MaxReduce red = max_into(p, 3);
+    hipLaunchKernelGGL(k_residual64, dim3(reduce_grid(l0.st->cells)), dim3(256), 0, ctx->stream, l0.st->args(), f,
+                       l0.x[l0.cur], (double *)nullptr, red);
     HIPCHK(ctx, hipGetLastError());
     unsigned long long b;
-    int rc = read_u64(p, 0, &b);
+    int rc = read_u64(p, 3, &b);
     if (rc) return rc;
     *out = bits_to_double(b);
     return CG_OK;
@@ -436,8 +544,9 @@ int ref_vcycle(cg_plan *p, const float *f0, double tol, int fixed) {
         PlanLevel &C = p->lv[l + 1];
         const float *f = l == 0 ? f0 : L.f;
         if ((rc = relax_pp(ctx, L.st, f, L.x, &L.cur, 2, false, l == 0))) return rc;
-        hipLaunchKernelGGL(k_residual64, dim3(blocks_for(L.st->cells, 256)), dim3(256), 0, s, L.st->args(), f,
-                           L.x[L.cur], p->r64, (unsigned long long *)nullptr);
+        MaxReduce none{};
+        hipLaunchKernelGGL(k_residual64, dim3(reduce_grid(L.st->cells)), dim3(256), 0, s, L.st->args(), f, L.x[L.cur],
+                           p->r64, none);
         hipLaunchKernelGGL(k_restrict64, dim3(blocks_for(C.st->cells, 256)), dim3(256), 0, s, p->r64, L.st->across,
                            L.st->down, C.st->across, C.st->down, C.bal);
         hipLaunchKernelGGL(k_with_balance, dim3(blocks_for(C.st->cells, 256)), dim3(256), 0, s, C.st->args(), C.bal, C.f);
@@ -493,9 +602,82 @@ int ref_solve(cg_plan *p, double tol, cg_solution *info, const volatile int *can
 
 // ---- the refined method ----------------------------------------------------------------------
 
+// The fused cycle: per tile level one down launch (smooth from zero, residual, restriction) and
+// one up launch (prolongation, smooth); the bottom levels in one k_bottom launch.
+int vcycle32_fused(cg_plan *p, bool timed) {
+    cg_ctx *ctx = p->ctx;
+    hipStream_t s = ctx->stream;
+    const int n = p->n, b = p->bottom;
+    const float beyond = p->pole_is_wall ? -1.0f : 1.0f;
+    for (int l = 0; l < b; ++l) {
+        PlanLevel &L = p->lv[l];
+        PlanLevel &C = p->lv[l + 1];
+        TileExtra ex{};
+        ex.cwater = C.st->water;
+        ex.g = L.g;
+        ex.fc = C.f;
+        TimeScope ts(ctx, timed && l == 0, kKindSmoothFromZero);
+        hipError_t e = tile_dispatch(ctx, L.st->args(), L.f, L.x[L.cur], L.x[1 - L.cur], 4, true, TILE_DOWN, ex);
+        if (e != hipSuccess) {
+            set_err(ctx, "down-leg launch failed: %s", hipGetErrorString(e));
+            return CG_ERR_DEVICE;
+        }
+        L.cur = 1 - L.cur;
+    }
+    BottomArgs ba{};
+    ba.nlev = n - b;
+    for (int l = b; l < n; ++l) {
+        const PlanLevel &L = p->lv[l];
+        BottomLevel &bl = ba.lv[l - b];
+        bl.across = L.st->across;
+        bl.down = L.st->down;
+        bl.water = L.st->water;
+        bl.roww = L.st->roww;
+        bl.g = L.g;
+    }
+    size_t off = 0;
+    for (int i = 0; i < ba.nlev; ++i) {
+        const size_t cells = (size_t)ba.lv[i].across * ba.lv[i].down;
+        ba.lv[i].off_x = (int)off;
+        off += (cells * 4 + 15) & ~(size_t)15;
+        ba.lv[i].off_f = (int)off;
+        off += (cells * 4 + 15) & ~(size_t)15;
+        ba.lv[i].off_w = (int)off;
+        off += (cells + 15) & ~(size_t)15;
+    }
+    PlanLevel &B0 = p->lv[b];
+    ba.f_in = B0.f;
+    ba.x_out = B0.x[B0.cur];
+    ba.idx = p->d_idx;
+    ba.Mt = p->d_Mt;
+    ba.nw = p->d_Mt ? p->nw : 0;
+    ba.coarsest_passes = kPrecondCoarsestPasses;
+    ba.beyond = beyond;
+    hipLaunchKernelGGL(k_bottom, dim3(1), dim3(1024), p->bottom_lds, s, ba);
+    for (int l = b - 1; l >= 0; --l) {
+        PlanLevel &L = p->lv[l];
+        PlanLevel &C = p->lv[l + 1];
+        TileExtra ex{};
+        ex.xc = C.x[C.cur];
+        ex.ca = C.st->across;
+        ex.cd = C.st->down;
+        ex.beyond = beyond;
+        TimeScope ts(ctx, timed && l == 0, kKindSmooth);
+        hipError_t e = tile_dispatch(ctx, L.st->args(), L.f, L.x[L.cur], L.x[1 - L.cur], 4, false, TILE_UP, ex);
+        if (e != hipSuccess) {
+            set_err(ctx, "up-leg launch failed: %s", hipGetErrorString(e));
+            return CG_ERR_DEVICE;
+        }
+        L.cur = 1 - L.cur;
+    }
+    HIPCHK(ctx, hipGetLastError());
+    return CG_OK;
+}
+
 // One binary32 V(2,2) cycle on the correction problem from zero, with level 0's forcing lv[0].f.
 // The answer is left in lv[0].x[lv[0].cur].
 int vcycle32(cg_plan *p, bool timed) {
+    if (p->ctx->fused && p->fusable) return vcycle32_fused(p, timed);
     cg_ctx *ctx = p->ctx;
     hipStream_t s = ctx->stream;
     const int n = p->n;
@@ -534,9 +716,13 @@ int vcycle32(cg_plan *p, bool timed) {
 int outer_step(cg_plan *p, const float *delta) {
     cg_ctx *ctx = p->ctx;
     const Stencil *st = p->lv[0].st;
-    zero_u64(p, 0, 1);
-    hipLaunchKernelGGL(k_outer_residual, dim3(blocks_for(st->cells, 256)), dim3(256), 0, ctx->stream, st->args(), p->f0,
-                       p->x64[p->x64cur], delta, p->x64[1 - p->x64cur], p->lv[0].f, p->d_u64);
+    // Slot x64cur receives the maximum (zeroed by the previous step); the other slot is zeroed for
+    // the next step.
+    MaxReduce red;
+    red.out = p->d_u64 + p->x64cur;
+    red.clear = p->d_u64 + (1 - p->x64cur);
+    hipLaunchKernelGGL(k_outer_residual, dim3(reduce_grid(st->cells)), dim3(256), 0, ctx->stream, st->args(), p->f0,
+                       p->x64[p->x64cur], delta, p->x64[1 - p->x64cur], p->lv[0].f, red);
     HIPCHK(ctx, hipGetLastError());
     p->x64cur = 1 - p->x64cur;
     return CG_OK;
@@ -559,8 +745,8 @@ int build_step_graphs(cg_plan *p) {
         int rc = vcycle32(p, false);
         if (!rc) rc = outer_step(p, p->lv[0].x[p->lv[0].cur]);
         if (!rc) {
-            hipError_t e = hipMemcpyAsync(ctx->h_pinned, p->d_u64, sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                                          ctx->stream);
+            hipError_t e = hipMemcpyAsync(ctx->h_pinned, p->d_u64, 2 * sizeof(unsigned long long),
+                                          hipMemcpyDeviceToHost, ctx->stream);
             if (e != hipSuccess) {
                 set_err(ctx, "capture of the residual readback failed: %s", hipGetErrorString(e));
                 rc = CG_ERR_DEVICE;
@@ -604,9 +790,11 @@ int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *info, const vo
         HIPCHK(ctx, hipMemsetAsync(p->x64[p->x64cur], 0, sizeof(double) * st->cells, ctx->stream));
     }
     p->x64_valid = true;
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, p->d_u64, 2, 0ull);
+    const int first_slot = p->x64cur;
     if ((rc = outer_step(p, nullptr))) return rc;
     unsigned long long b;
-    if ((rc = read_u64(p, 0, &b))) return rc;
+    if ((rc = read_u64(p, first_slot, &b))) return rc;
     double res = largest > 0.0 ? bits_to_double(b) / largest : 0.0;
     const bool graph = ctx->use_graph && !ctx->time_kernels;
     if (graph && res >= tol) {
@@ -618,15 +806,16 @@ int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *info, const vo
             info->status = CG_CANCELLED;
             break;
         }
+        const int slot = p->x64cur;
         if (graph) {
             HIPCHK(ctx, hipGraphLaunch(p->step_exec[p->x64cur], ctx->stream));
             p->x64cur = 1 - p->x64cur;
             HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-            b = ctx->h_pinned[0];
+            b = ctx->h_pinned[slot];
         } else {
             if ((rc = vcycle32(p, true))) return rc;
             if ((rc = outer_step(p, p->lv[0].x[p->lv[0].cur]))) return rc;
-            if ((rc = read_u64(p, 0, &b))) return rc;
+            if ((rc = read_u64(p, slot, &b))) return rc;
         }
         cycles++;
         const double r = bits_to_double(b);
@@ -731,6 +920,7 @@ int cg_ocean_set_option(cg_ctx *ctx, int option, long value) {
             return CG_OK;
         case CG_OPT_STEPS_PER_CHECK: ctx->steps_per_check = value < 1 ? 1 : (int)value; return CG_OK;
         case 6: ctx->time_kernels = value ? 1 : 0; return CG_OK;
+        case 7: ctx->fused = value ? 1 : 0; return CG_OK;
         default: set_err(ctx, "unknown option %d", option); return CG_ERR_ARG;
     }
 }
@@ -744,6 +934,7 @@ long cg_ocean_get_option(cg_ctx *ctx, int option) {
         case CG_OPT_FGMRES_RESTART: return ctx->fgmres_m;
         case CG_OPT_STEPS_PER_CHECK: return ctx->steps_per_check;
         case 6: return ctx->time_kernels;
+        case 7: return ctx->fused;
         default: return -1;
     }
 }
@@ -793,8 +984,10 @@ int cg_ocean_stencil_upload(cg_ctx *ctx, int across, int down, const uint8_t *wa
         free_stencil(s);
         return code;
     };
-    if ((rc = dalloc(ctx, &s->water, s->cells))) return bail(rc);
-    HIPCHK(ctx, hipMemcpy(s->water, water, s->cells, hipMemcpyHostToDevice));
+    // The device mask holds exactly 0 or 1 per cell (the smoother extracts bits from it).
+    for (long i = 0; i < s->cells; ++i) s->h_water[i] = water[i] ? 1 : 0;
+    if ((rc = dalloc_grid(ctx, &s->water, across, down))) return bail(rc);
+    HIPCHK(ctx, hipMemcpy(s->water, s->h_water.data(), s->cells, hipMemcpyHostToDevice));
     if (s->rowconst) {
         s->h_w.resize(down);
         s->h_c.assign(down, 1.0);
@@ -863,9 +1056,9 @@ int cg_ocean_relax(cg_ctx *ctx, cg_stencil h, const float *forcing, const float 
     HIPCHK(ctx, hipSetDevice(ctx->device));
     int rc;
     if (!s->fbuf) {
-        if ((rc = dalloc(ctx, &s->fbuf, s->cells))) return rc;
-        if ((rc = dalloc(ctx, &s->xbuf[0], s->cells))) return rc;
-        if ((rc = dalloc(ctx, &s->xbuf[1], s->cells))) return rc;
+        if ((rc = dalloc_grid(ctx, &s->fbuf, s->across, s->down))) return rc;
+        if ((rc = dalloc_grid(ctx, &s->xbuf[0], s->across, s->down))) return rc;
+        if ((rc = dalloc_grid(ctx, &s->xbuf[1], s->across, s->down))) return rc;
     }
     HIPCHK(ctx, hipMemcpyAsync(s->fbuf, forcing, sizeof(float) * s->cells, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipMemcpyAsync(s->xbuf[0], start, sizeof(float) * s->cells, hipMemcpyHostToDevice, ctx->stream));
@@ -926,25 +1119,27 @@ int cg_ocean_plan_create(cg_ctx *ctx, const cg_stencil *levels, int nlevels, int
             }
         }
         p->lv[l].st = s;
-        if ((rc = dalloc(ctx, &p->lv[l].f, s->cells))) return bail(rc);
-        if ((rc = dalloc(ctx, &p->lv[l].x[0], s->cells))) return bail(rc);
-        if ((rc = dalloc(ctx, &p->lv[l].x[1], s->cells))) return bail(rc);
+        if ((rc = dalloc_grid(ctx, &p->lv[l].f, s->across, s->down))) return bail(rc);
+        if ((rc = dalloc_grid(ctx, &p->lv[l].x[0], s->across, s->down))) return bail(rc);
+        if ((rc = dalloc_grid(ctx, &p->lv[l].x[1], s->across, s->down))) return bail(rc);
         if (l > 0 && (rc = dalloc(ctx, &p->lv[l].bal, s->cells))) return bail(rc);
         HIPCHK(ctx, hipMemset(p->lv[l].x[0], 0, sizeof(float) * s->cells));
         HIPCHK(ctx, hipMemset(p->lv[l].x[1], 0, sizeof(float) * s->cells));
         HIPCHK(ctx, hipMemset(p->lv[l].f, 0, sizeof(float) * s->cells));
     }
     const long n0 = p->lv[0].st->cells;
-    if ((rc = dalloc(ctx, &p->f0, n0))) return bail(rc);
+    if ((rc = dalloc_grid(ctx, &p->f0, p->lv[0].st->across, p->lv[0].st->down))) return bail(rc);
     if ((rc = dalloc(ctx, &p->r64, n0))) return bail(rc);
     if ((rc = dalloc(ctx, &p->x64[0], n0))) return bail(rc);
     if ((rc = dalloc(ctx, &p->x64[1], n0))) return bail(rc);
     if ((rc = dalloc(ctx, &p->d_u64, 16))) return bail(rc);
+    HIPCHK(ctx, hipMemset(p->d_u64, 0, 16 * sizeof(unsigned long long)));
     if ((rc = dalloc(ctx, &p->d_passes, 4))) return bail(rc);
     HIPCHK(ctx, hipMemset(p->f0, 0, sizeof(float) * n0));
     HIPCHK(ctx, hipMemset(p->x64[0], 0, sizeof(double) * n0));
     HIPCHK(ctx, hipMemset(p->x64[1], 0, sizeof(double) * n0));
     if (nlevels > 1 && (rc = build_coarsest_inverse(ctx, p))) return bail(rc);
+    if ((rc = plan_fusion_setup(ctx, p))) return bail(rc);
     *out = p;
     return CG_OK;
 }
@@ -1080,6 +1275,7 @@ double cg_ocean_kernel_time_us(cg_ctx *ctx, int kind, long *launches) {
     double total = 0.0;
     long counted = 0;
     for (size_t i = 0; i < ctx->timed_used; ++i) {
+        if (ctx->timed[i].kind != kind) continue;
         float ms = 0.0f;
         if (hipEventElapsedTime(&ms, ctx->timed[i].a, ctx->timed[i].b) == hipSuccess) {
             total += ms;

@@ -22,6 +22,14 @@ from . import (CG_CANCELLED, CG_DECLINED, CG_METHOD_REFERENCE, CG_METHOD_REFINED
                CgSolution, check, lib)
 
 
+def _flat(a, dtype, n: int) -> np.ndarray:
+    """a itself when it already is a flat contiguous array of dtype and size n (so that arrays
+    shared between stencils keep their identity), else a converted copy."""
+    if isinstance(a, np.ndarray) and a.dtype == dtype and a.ndim == 1 and a.size == n and a.flags.c_contiguous:
+        return a
+    return np.ascontiguousarray(a, dtype=dtype).reshape(n)
+
+
 class OceanStencil:
     """One steady linear problem on one grid (OceanCirculation.kt:24-42): binary32 neighbour
     weights and forcing, the binary64 centre weight, a water mask; row-major, columns wrapping."""
@@ -31,13 +39,13 @@ class OceanStencil:
         n = int(cells_across) * int(cells_down)
         self.cells_across = int(cells_across)
         self.cells_down = int(cells_down)
-        self.is_water = np.ascontiguousarray(is_water, dtype=np.uint8).reshape(n)
-        self.east_weight = np.ascontiguousarray(east_weight, dtype=np.float32).reshape(n)
-        self.west_weight = np.ascontiguousarray(west_weight, dtype=np.float32).reshape(n)
-        self.north_weight = np.ascontiguousarray(north_weight, dtype=np.float32).reshape(n)
-        self.south_weight = np.ascontiguousarray(south_weight, dtype=np.float32).reshape(n)
-        self.forcing = np.ascontiguousarray(forcing, dtype=np.float32).reshape(n)
-        self.centre_weight = np.ascontiguousarray(centre_weight, dtype=np.float64).reshape(n)
+        self.is_water = _flat(is_water, np.uint8, n)
+        self.east_weight = _flat(east_weight, np.float32, n)
+        self.west_weight = _flat(west_weight, np.float32, n)
+        self.north_weight = _flat(north_weight, np.float32, n)
+        self.south_weight = _flat(south_weight, np.float32, n)
+        self.forcing = _flat(forcing, np.float32, n)
+        self.centre_weight = _flat(centre_weight, np.float64, n)
 
     @property
     def cells(self) -> int:

@@ -129,8 +129,11 @@ int cg_ocean_plan_solve_batch(cg_plan *const *plans, int count, double tol, int 
 /* Copies the answer back; either pointer may be NULL. */
 int cg_ocean_plan_read(cg_plan *plan, double *out64, float *out32);
 /* Synchronises the context's stream and reports the average duration, in microseconds, of the
- * kernels of `kind` (0: fused finest-grid smoother launches) recorded since the last reset with
- * HIP events on the stream they were launched on; kind -1 resets the record. */
+ * finest grid's two-pass fused smoother launches of `kind` (0: from zero, the cycle's down leg;
+ * 1: from given values, its up leg and the relaxation seam) recorded with HIP events on the stream
+ * they were launched on while option CG_OPT_TIME_KERNELS was 1 (graph replay is then off); kind -1
+ * resets the record. *launches receives the count averaged. */
+#define CG_OPT_TIME_KERNELS 6
 double cg_ocean_kernel_time_us(cg_ctx *ctx, int kind, long *launches);
 
 /* ---- host-side problem builders (no device needed) --------------------------------------------- */

@@ -1,0 +1,144 @@
+"""Whole solves on the device against the oracle.
+
+- The reference method (binary32 values, binary64 residual, V(2,2), coarsest to 0.1 tol) is held
+  to the oracle's restatement of OceanCirculation.solve (OceanCirculation.kt:473-493) to the bit:
+  same values, same cycle count, same residual.
+- The refined method (binary32 cycles inside binary64 refinement) is held to the binary64 oracle
+  (the exact solution of the same operator, pinned against scipy direct solves in
+  tests/test_oracle.py): relative L2 <= 1e-10 at tolerance 1e-12.
+- At the full 1024 x 2048 size, where the oracle is slow, by properties that hold at any size: the
+  residual recomputed on the CPU, exact linearity under a power-of-two scaling of the forcing, and
+  bit-identical repeats.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import stommel_analytic, stommel_basin, to_oracle
+from cartogenesis_amd import CG_METHOD_REFERENCE, CG_METHOD_REFINED, CG_OPT_USE_GRAPH
+from cartogenesis_amd.ocean import OceanCirculation as OC, Plan
+from cartogenesis_amd.problem import circulation_problem
+
+pytestmark = pytest.mark.gpu
+
+PARITY_L2 = 1e-10  # relative L2 against the binary64 oracle
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def oracle_residual(levels, x64):
+    st = to_oracle(levels[0])
+    return float(np.abs(oracle.residual64(st, x64)).max() / np.abs(oracle.balance_of(st)).max())
+
+
+@pytest.mark.parametrize("rows,tol", [(64, 1e-3), (128, 1e-5), (256, 1e-3), (256, 1e-5)])
+def test_reference_method_is_bit_identical_to_the_processor_solve(acc, rows, tol):
+    p = circulation_problem(rows, seed=0)
+    sol = acc.solve_levels(p.levels, None, tol, True, CG_METHOD_REFERENCE)
+    x, cycles, rr = oracle.solve([to_oracle(s) for s in p.levels], np.zeros(p.levels[0].cells, np.float32), tol)
+    print(f"OCEAN reference method {rows}x{2 * rows} tol {tol}: {sol.cycles} cycles, residual {sol.relative_residual}")
+    assert sol.cycles == cycles
+    assert sol.relative_residual == rr
+    assert np.array_equal(sol.values.view(np.uint32), x.view(np.uint32))
+    OC.require_solved("the circulation", sol, tol)
+
+
+def test_reference_method_from_a_warm_start(acc):
+    p = circulation_problem(128, seed=3, land_fraction=0.4)
+    rng = np.random.default_rng(1)
+    start = (rng.normal(size=p.levels[0].cells) * 100).astype(np.float32)
+    sol = acc.solve_levels(p.levels, start, 1e-4, True, CG_METHOD_REFERENCE)
+    x, cycles, rr = oracle.solve([to_oracle(s) for s in p.levels], start, 1e-4)
+    assert sol.cycles == cycles and sol.relative_residual == rr
+    assert np.array_equal(sol.values.view(np.uint32), x.view(np.uint32))
+
+
+@pytest.mark.parametrize("rows,seed,land", [(64, 0, 0.3), (128, 0, 0.3), (256, 0, 0.3), (128, 7, 0.2), (128, 11, 0.4)])
+def test_refined_method_matches_the_binary64_oracle(acc, rows, seed, land):
+    p = circulation_problem(rows, seed=seed, land_fraction=land)
+    sol = acc.solve_levels(p.levels, None, 1e-12, True, CG_METHOD_REFINED)
+    ref64, _, ref_rr = oracle.solve64([to_oracle(s) for s in p.levels], 1e-13)
+    err = rel_l2(sol.values64, ref64)
+    print(f"OCEAN refined {rows}x{2 * rows} seed {seed}: {sol.cycles} cycles, residual {sol.relative_residual}, "
+          f"rel L2 vs binary64 oracle {err}")
+    assert ref_rr < 1e-13
+    assert sol.relative_residual < 1e-12
+    assert oracle_residual(p.levels, sol.values64) < 1e-12
+    assert err <= PARITY_L2
+
+
+def test_refined_graph_and_eager_agree_to_the_bit(acc):
+    p = circulation_problem(128, seed=2)
+    acc.set_option(CG_OPT_USE_GRAPH, 0)
+    try:
+        eager = acc.solve_levels(p.levels, None, 1e-10, True, CG_METHOD_REFINED)
+    finally:
+        acc.set_option(CG_OPT_USE_GRAPH, 1)
+    graphed = acc.solve_levels(p.levels, None, 1e-10, True, CG_METHOD_REFINED)
+    assert eager.cycles == graphed.cycles
+    assert np.array_equal(eager.values64.view(np.uint64), graphed.values64.view(np.uint64))
+
+
+def test_full_size_refined_solve_properties(acc):
+    """1024 x 2048 (the benchmark's workload): converged by the CPU's own residual, linear to the
+    bit under a forcing doubled, and bit-identical on repeat; replays of the step graph between
+    eager work agree with each other."""
+    p = circulation_problem(1024, seed=0)
+    plan = Plan(acc, p.levels)
+    try:
+        info = plan.solve(1e-8, CG_METHOD_REFINED)
+        x1, _ = plan.read()
+        assert info.relative_residual < 1e-8
+        assert oracle_residual(p.levels, x1) < 1e-8
+        info2 = plan.solve(1e-8, CG_METHOD_REFINED)
+        x2, _ = plan.read()
+        assert info2.cycles == info.cycles
+        assert np.array_equal(x1.view(np.uint64), x2.view(np.uint64))
+        plan.set_forcing(p.levels[0].forcing * np.float32(2.0))
+        info3 = plan.solve(1e-8, CG_METHOD_REFINED)
+        x3, _ = plan.read()
+        assert info3.cycles == info.cycles
+        assert np.array_equal((2.0 * x1).view(np.uint64), x3.view(np.uint64))
+        print(f"OCEAN full size: {info.cycles} cycles to {info.relative_residual}")
+    finally:
+        plan.close()
+
+
+@pytest.mark.parametrize("aspect", [1.0, 0.5])
+def test_stommel_basin_on_the_device(acc, aspect):
+    """OceanCirculationTest.kt:89-106: the discrete basin within 1% of Stommel's analytic one,
+    solved on the device at the reference test's tolerance (1e-5)."""
+    levels, across, down, dx, dy = stommel_basin(6_250.0, aspect)
+    for method in (CG_METHOD_REFERENCE, CG_METHOD_REFINED):
+        sol = acc.solve_levels(levels, None, 1e-5, True, method)
+        stream = sol.values.reshape(down, across)
+        cols = np.arange(1, across - 1)
+        rows = np.arange(down)
+        X, Y = np.meshgrid(cols * dx, (down - 0.5 - rows) * dy)
+        exact = stommel_analytic(across, down, dx, dy, -1e-12, X, Y)
+        worst = np.abs(stream[:, 1:-1] - exact).max()
+        largest = np.abs(exact).max()
+        print(f"OCEAN Stommel basin on device, aspect {aspect}, method {method}: {worst / largest}")
+        assert worst / largest < 0.01
+
+
+@pytest.mark.parametrize("rows", [64, 256, 512])
+def test_fused_and_unfused_cycles_reach_the_oracle(acc, rows):
+    """The fused cycle (one down and one up launch per grid, the bottom grids in one workgroup) and
+    the unfused one (separate smoothing, residual, restriction and prolongation kernels) both reach
+    the binary64 oracle; they differ only in the coarsest grid's summation order."""
+    p = circulation_problem(rows, seed=5)
+    ref64 = oracle.solve64([to_oracle(s) for s in p.levels], 1e-13)[0]
+    out = {}
+    for fused in (1, 0):
+        acc.set_option(7, fused)
+        try:
+            sol = acc.solve_levels(p.levels, None, 1e-11, True, CG_METHOD_REFINED)
+        finally:
+            acc.set_option(7, 1)
+        out[fused] = sol
+        assert sol.relative_residual < 1e-11
+        assert rel_l2(sol.values64, ref64) <= PARITY_L2
+    assert abs(out[1].cycles - out[0].cycles) <= 1
