@@ -306,9 +306,12 @@ __device__ __forceinline__ unsigned long long abs_bits(double v) {
 }
 
 // A grid-wide maximum: each block reduces in registers and LDS, then its thread 0 does one relaxed
-// device-scope atomicMax into *out (no fences, so no L2 write-back per block). *out must be zero
-// when the kernel starts; block 0 zeroes *clear (the slot the next reduction will use), which a
-// later launch on the same stream then sees, so graph replays need no memset.
+// atomicMax into out[block % kMaxSlots] (no fences, so no L2 write-back per block; 64 slots keep
+// the atomics uncontended). The reader takes the largest slot. The slots must be zero when the
+// kernel starts; the blocks together zero all of `clear` (the slots the next reduction will use), which a later
+// launch on the same stream then sees, so graph replays need no memset.
+constexpr int kMaxSlots = 64;
+
 struct MaxReduce {
     unsigned long long *out;
     unsigned long long *clear;
@@ -325,7 +328,10 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
 
 __device__ void grid_max(unsigned long long v, const MaxReduce &red) {
     __shared__ unsigned long long sh[16];
-    if (blockIdx.x == 0 && threadIdx.x == 0 && red.clear) *red.clear = 0ull;
+    const int b = blockIdx.x + blockIdx.y * gridDim.x;
+    const int nb = gridDim.x * gridDim.y;
+    if (threadIdx.x == 0 && red.clear)
+        for (int i = b; i < kMaxSlots; i += nb) red.clear[i] = 0ull;
     v = wave_max_u64(v);
     const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
     if ((threadIdx.x & 63) == 0) sh[w] = v;
@@ -333,48 +339,62 @@ __device__ void grid_max(unsigned long long v, const MaxReduce &red) {
     if (threadIdx.x == 0) {
         unsigned long long m = 0;
         for (int i = 0; i < nw; ++i) m = sh[i] > m ? sh[i] : m;
-        if (m) atomicMax(red.out, m);
+        if (m) atomicMax(red.out + (b & (kMaxSlots - 1)), m);
     }
 }
 
+// The 2D launch of the per-cell binary64 kernels: blockIdx.x covers 256 columns, blockIdx.y
+// covers kRowsPerBlock rows, so no thread divides.
+constexpr int kRowsPerBlock = 4;
+
 // Binary64 residual (OC:292-315): r = (x - (sum w x - f)) * centre on water, 0 on land; and its
-// largest magnitude into red.out when red.out is given. Grid-stride over cells.
+// largest magnitude into red.out when red.out is given.
 __global__ __launch_bounds__(256) void k_residual64(LevelArgs L, const float *__restrict__ f,
                                                     const float *__restrict__ x, double *__restrict__ r,
                                                     MaxReduce red) {
     const int across = L.across;
-    const long cells = (long)across * L.down;
+    const int col = blockIdx.x * 256 + threadIdx.x;
     unsigned long long m = 0;
-    for (long cell = (long)blockIdx.x * blockDim.x + threadIdx.x; cell < cells; cell += (long)gridDim.x * blockDim.x) {
-        double out = 0.0;
-        if (L.water[cell]) {
-            const int row = (int)(cell / across);
-            const int col = (int)(cell - (long)row * across);
-            const int ce = col + 1 == across ? 0 : col + 1;
-            const int cw = col == 0 ? across - 1 : col - 1;
-            const double vn = row > 0 ? (double)x[cell - across] : 0.0;
-            const double vs = row + 1 < L.down ? (double)x[cell + across] : 0.0;
-            const float4 wt = weights_at(L, row, cell);
-            const double bal = (double)wt.x * (double)x[(long)row * across + ce] +
-                               (double)wt.y * (double)x[(long)row * across + cw] + (double)wt.z * vn +
-                               (double)wt.w * vs - (double)f[cell];
-            out = ((double)x[cell] - bal) * centre_at(L, row, cell);
+    if (col < across) {
+        const int ce = col + 1 == across ? 0 : col + 1;
+        const int cw = col == 0 ? across - 1 : col - 1;
+#pragma unroll
+        for (int k = 0; k < kRowsPerBlock; ++k) {
+            const int row = blockIdx.y * kRowsPerBlock + k;
+            if (row >= L.down) break;
+            const long cell = (long)row * across + col;
+            double out = 0.0;
+            if (L.water[cell]) {
+                const double vn = row > 0 ? (double)x[cell - across] : 0.0;
+                const double vs = row + 1 < L.down ? (double)x[cell + across] : 0.0;
+                const float4 wt = weights_at(L, row, cell);
+                const double bal = (double)wt.x * (double)x[(long)row * across + ce] +
+                                   (double)wt.y * (double)x[(long)row * across + cw] + (double)wt.z * vn +
+                                   (double)wt.w * vs - (double)f[cell];
+                out = ((double)x[cell] - bal) * centre_at(L, row, cell);
+            }
+            if (r) r[cell] = out;
+            const unsigned long long b = abs_bits(out);
+            m = b > m ? b : m;
         }
-        if (r) r[cell] = out;
-        const unsigned long long b = abs_bits(out);
-        m = b > m ? b : m;
     }
     if (red.out) grid_max(m, red);
 }
 
 // The largest magnitude of f * centre (balanceOf then largest, OC:325-327, 318-322).
 __global__ __launch_bounds__(256) void k_balance_max(LevelArgs L, const float *__restrict__ f, MaxReduce red) {
-    const long cells = (long)L.across * L.down;
+    const int across = L.across;
+    const int col = blockIdx.x * 256 + threadIdx.x;
     unsigned long long m = 0;
-    for (long cell = (long)blockIdx.x * blockDim.x + threadIdx.x; cell < cells; cell += (long)gridDim.x * blockDim.x) {
-        const int row = (int)(cell / L.across);
-        const unsigned long long b = abs_bits((double)f[cell] * centre_at(L, row, cell));
-        m = b > m ? b : m;
+    if (col < across) {
+#pragma unroll
+        for (int k = 0; k < kRowsPerBlock; ++k) {
+            const int row = blockIdx.y * kRowsPerBlock + k;
+            if (row >= L.down) break;
+            const long cell = (long)row * across + col;
+            const unsigned long long b = abs_bits((double)f[cell] * centre_at(L, row, cell));
+            m = b > m ? b : m;
+        }
     }
     grid_max(m, red);
 }
@@ -543,34 +563,39 @@ __global__ __launch_bounds__(1024) void k_coarsest_relax(LevelArgs L, const floa
 
 // The outer step: x_new = x + delta (binary64; delta may be null), u = x_new - (sum w x_new - f)
 // in binary64, the correction forcing fc = (float)u on water (0 on land), and the largest
-// |u * centre| into red.out. Land values are zero in x_new. Grid-stride over cells.
+// |u * centre| into red. Land values are zero in x_new. 2D launch as k_residual64.
 __global__ __launch_bounds__(256) void k_outer_residual(LevelArgs L, const float *__restrict__ f,
                                                         const double *__restrict__ x, const float *__restrict__ delta,
                                                         double *__restrict__ xnew, float *__restrict__ fc,
                                                         MaxReduce red) {
     const int across = L.across;
-    const long cells = (long)across * L.down;
+    const int col = blockIdx.x * 256 + threadIdx.x;
     unsigned long long m = 0;
-    for (long cell = (long)blockIdx.x * blockDim.x + threadIdx.x; cell < cells; cell += (long)gridDim.x * blockDim.x) {
-        const int row = (int)(cell / across), col = (int)(cell - (long)row * across);
-        double v = 0.0, u = 0.0;
-        if (L.water[cell]) {
-            const int ce = col + 1 == across ? 0 : col + 1;
-            const int cw = col == 0 ? across - 1 : col - 1;
-            const long ie = (long)row * across + ce, iw = (long)row * across + cw;
-            auto val = [&](long i) -> double { return delta ? x[i] + (double)delta[i] : x[i]; };
-            v = val(cell);
-            const double vn = row > 0 ? val(cell - across) : 0.0;
-            const double vs = row + 1 < L.down ? val(cell + across) : 0.0;
-            const float4 wt = weights_at(L, row, cell);
-            const double bal = (double)wt.x * val(ie) + (double)wt.y * val(iw) + (double)wt.z * vn + (double)wt.w * vs -
-                               (double)f[cell];
-            u = v - bal;
-            const unsigned long long b = abs_bits(u * centre_at(L, row, cell));
-            m = b > m ? b : m;
+    if (col < across) {
+        const int ce = col + 1 == across ? 0 : col + 1;
+        const int cw = col == 0 ? across - 1 : col - 1;
+#pragma unroll
+        for (int k = 0; k < kRowsPerBlock; ++k) {
+            const int row = blockIdx.y * kRowsPerBlock + k;
+            if (row >= L.down) break;
+            const long cell = (long)row * across + col;
+            double v = 0.0, u = 0.0;
+            if (L.water[cell]) {
+                const long ie = (long)row * across + ce, iw = (long)row * across + cw;
+                auto val = [&](long i) -> double { return delta ? x[i] + (double)delta[i] : x[i]; };
+                v = val(cell);
+                const double vn = row > 0 ? val(cell - across) : 0.0;
+                const double vs = row + 1 < L.down ? val(cell + across) : 0.0;
+                const float4 wt = weights_at(L, row, cell);
+                const double bal = (double)wt.x * val(ie) + (double)wt.y * val(iw) + (double)wt.z * vn +
+                                   (double)wt.w * vs - (double)f[cell];
+                u = v - bal;
+                const unsigned long long b = abs_bits(u * centre_at(L, row, cell));
+                m = b > m ? b : m;
+            }
+            xnew[cell] = v;
+            fc[cell] = (float)u;
         }
-        xnew[cell] = v;
-        fc[cell] = (float)u;
     }
     grid_max(m, red);
 }
@@ -641,7 +666,7 @@ struct BottomLevel {
     const uint8_t *water;
     const float4 *roww;
     const float *g;  // restriction factor per row to the next level (null at the coarsest)
-    int off_x, off_f, off_w;  // LDS byte offsets
+    int off_x, off_f, off_w, off_rw, off_g;  // LDS byte offsets
 };
 
 struct BottomArgs {
@@ -656,152 +681,199 @@ struct BottomArgs {
     float beyond;
 };
 
-__device__ __forceinline__ void lds_relax_pass(const BottomLevel &L, float *x, const float *f, const uint8_t *w) {
+// The bottom's phases: each of the workgroup's 1024 threads handles at most kBottomCellsPerThread
+// cells of a phase, and reads everything it needs for all of them before it writes any, so that
+// its LDS loads overlap instead of each waiting behind the previous cell's store. Within one colour
+// of a red-black sweep no cell reads a cell of its own colour, so the deferred writes change nothing.
+constexpr int kBottomCellsPerThread = 8;
+
+struct BottomView {
+    int across, down;
+    float *x, *f;
+    uint8_t *w;
+    const float4 *rw;
+    const float *g;
+};
+
+__device__ __forceinline__ void lds_relax_pass(const BottomView &L) {
     const int across = L.across, half = across >> 1;
     const int n = half * L.down;
     for (int colour = 0; colour <= 1; ++colour) {
-        for (int t = threadIdx.x; t < n; t += blockDim.x) {
-            const int row = t / half;
-            const int col = 2 * (t - row * half) + ((colour + row) & 1);
-            const int cell = row * across + col;
-            if (!w[cell]) {
-                x[cell] = 0.0f;
-                continue;
+        float nv[kBottomCellsPerThread];
+        int nc[kBottomCellsPerThread];
+#pragma unroll
+        for (int k = 0; k < kBottomCellsPerThread; ++k) {
+            const int t = (int)threadIdx.x + k * (int)blockDim.x;
+            nc[k] = -1;
+            nv[k] = 0.0f;
+            if (t < n) {
+                const int row = t / half;
+                const int col = 2 * (t - row * half) + ((colour + row) & 1);
+                const int cell = row * across + col;
+                nc[k] = cell;
+                if (L.w[cell]) {
+                    const int ce = col + 1 == across ? 0 : col + 1;
+                    const int cw = col == 0 ? across - 1 : col - 1;
+                    const float xn = row > 0 ? L.x[cell - across] : 0.0f;
+                    const float xs = row + 1 < L.down ? L.x[cell + across] : 0.0f;
+                    nv[k] = rb_update(L.rw[row], L.x[row * across + ce], L.x[row * across + cw], xn, xs, L.f[cell]);
+                }
             }
-            const int ce = col + 1 == across ? 0 : col + 1;
-            const int cw = col == 0 ? across - 1 : col - 1;
-            const float xn = row > 0 ? x[cell - across] : 0.0f;
-            const float xs = row + 1 < L.down ? x[cell + across] : 0.0f;
-            x[cell] = rb_update(L.roww[row], x[row * across + ce], x[row * across + cw], xn, xs, f[cell]);
         }
+#pragma unroll
+        for (int k = 0; k < kBottomCellsPerThread; ++k)
+            if (nc[k] >= 0) L.x[nc[k]] = nv[k];
         __syncthreads();
     }
 }
 
 __global__ __launch_bounds__(1024) void k_bottom(BottomArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    auto X = [&](int l) { return reinterpret_cast<float *>(smem + a.lv[l].off_x); };
-    auto F = [&](int l) { return reinterpret_cast<float *>(smem + a.lv[l].off_f); };
-    auto W = [&](int l) { return reinterpret_cast<uint8_t *>(smem + a.lv[l].off_w); };
+    __shared__ float partial[1024];
+    auto view = [&](int l) {
+        BottomView v;
+        v.across = a.lv[l].across;
+        v.down = a.lv[l].down;
+        v.x = reinterpret_cast<float *>(smem + a.lv[l].off_x);
+        v.f = reinterpret_cast<float *>(smem + a.lv[l].off_f);
+        v.w = reinterpret_cast<uint8_t *>(smem + a.lv[l].off_w);
+        v.rw = reinterpret_cast<const float4 *>(smem + a.lv[l].off_rw);
+        v.g = reinterpret_cast<const float *>(smem + a.lv[l].off_g);
+        return v;
+    };
     for (int l = 0; l < a.nlev; ++l) {
-        const int cells = a.lv[l].across * a.lv[l].down;
-        uint8_t *w = W(l);
-        float *x = X(l);
+        const BottomLevel &B = a.lv[l];
+        const int cells = B.across * B.down;
+        uint8_t *w = reinterpret_cast<uint8_t *>(smem + B.off_w);
+        float *x = reinterpret_cast<float *>(smem + B.off_x);
         for (int c = threadIdx.x; c < cells; c += blockDim.x) {
-            w[c] = a.lv[l].water[c];
+            w[c] = B.water[c];
             x[c] = 0.0f;
+        }
+        float4 *rw = reinterpret_cast<float4 *>(smem + B.off_rw);
+        float *g = reinterpret_cast<float *>(smem + B.off_g);
+        for (int r = threadIdx.x; r < B.down; r += blockDim.x) {
+            rw[r] = B.roww[r];
+            g[r] = B.g ? B.g[r] : 0.0f;
         }
     }
     {
         const int cells = a.lv[0].across * a.lv[0].down;
-        float *f = F(0);
+        float *f = reinterpret_cast<float *>(smem + a.lv[0].off_f);
         for (int c = threadIdx.x; c < cells; c += blockDim.x) f[c] = a.f_in[c];
     }
     __syncthreads();
-    // Down.
+    // Down: relax twice, then the binary32 residual restricted with g (k_residual_restrict32's sums).
     for (int l = 0; l + 1 < a.nlev; ++l) {
-        const BottomLevel &L = a.lv[l];
-        const BottomLevel &C = a.lv[l + 1];
-        float *x = X(l);
-        const float *f = F(l);
-        const uint8_t *w = W(l);
-        lds_relax_pass(L, x, f, w);
-        lds_relax_pass(L, x, f, w);
+        const BottomView L = view(l), C = view(l + 1);
+        lds_relax_pass(L);
+        lds_relax_pass(L);
         const int cpc = L.across / C.across, rpc = L.down / C.down;
-        float *fc = F(l + 1);
-        const uint8_t *wc = W(l + 1);
-        for (int t = threadIdx.x; t < C.across * C.down; t += blockDim.x) {
-            if (!wc[t]) {
-                fc[t] = 0.0f;
-                continue;
-            }
-            const int crow = t / C.across, ccol = t - crow * C.across;
-            float acc = 0.0f;
-            for (int dr = 0; dr < rpc; ++dr) {
-                const int row = crow * rpc + dr;
-                const float4 wt = L.roww[row];
-                const float g = L.g[row];
-                for (int dc = 0; dc < cpc; ++dc) {
-                    const int col = ccol * cpc + dc;
-                    const int cell = row * L.across + col;
-                    if (!w[cell]) continue;
-                    const int ce = col + 1 == L.across ? 0 : col + 1;
-                    const int cw = col == 0 ? L.across - 1 : col - 1;
-                    const float xn = row > 0 ? x[cell - L.across] : 0.0f;
-                    const float xs = row + 1 < L.down ? x[cell + L.across] : 0.0f;
-                    const float u = x[cell] - rb_update(wt, x[row * L.across + ce], x[row * L.across + cw], xn, xs, f[cell]);
-                    acc += u * g;
+        const int nco = C.across * C.down;
+        float cv[kBottomCellsPerThread];
+#pragma unroll
+        for (int k = 0; k < kBottomCellsPerThread; ++k) {
+            const int t = (int)threadIdx.x + k * (int)blockDim.x;
+            cv[k] = 0.0f;
+            if (t < nco && C.w[t]) {
+                const int crow = t / C.across, ccol = t - crow * C.across;
+                float acc = 0.0f;
+                for (int dr = 0; dr < rpc; ++dr) {
+                    const int row = crow * rpc + dr;
+                    const float4 wt = L.rw[row];
+                    const float g = L.g[row];
+                    for (int dc = 0; dc < cpc; ++dc) {
+                        const int col = ccol * cpc + dc;
+                        const int cell = row * L.across + col;
+                        if (!L.w[cell]) continue;
+                        const int ce = col + 1 == L.across ? 0 : col + 1;
+                        const int cw = col == 0 ? L.across - 1 : col - 1;
+                        const float xn = row > 0 ? L.x[cell - L.across] : 0.0f;
+                        const float xs = row + 1 < L.down ? L.x[cell + L.across] : 0.0f;
+                        const float u = L.x[cell] - rb_update(wt, L.x[row * L.across + ce], L.x[row * L.across + cw],
+                                                              xn, xs, L.f[cell]);
+                        acc += u * g;
+                    }
                 }
+                cv[k] = acc;
             }
-            fc[t] = acc;
+        }
+#pragma unroll
+        for (int k = 0; k < kBottomCellsPerThread; ++k) {
+            const int t = (int)threadIdx.x + k * (int)blockDim.x;
+            if (t < nco) C.f[t] = cv[k];
         }
         __syncthreads();
     }
-    // Coarsest.
+    // Coarsest: x = M f with the j-sum split over `parts` thread groups, then summed in LDS.
     {
-        const int l = a.nlev - 1;
-        const BottomLevel &C = a.lv[l];
-        float *x = X(l);
-        const float *f = F(l);
+        const BottomView C = view(a.nlev - 1);
         if (a.nw > 0) {
             const int nw = a.nw;
             const int parts = nw <= (int)blockDim.x ? (int)blockDim.x / nw : 1;
-            __shared__ float partial[1024];
-            for (int base = 0; base < nw; base += (int)blockDim.x / parts) {
-                const int i = base + (int)threadIdx.x % ((int)blockDim.x / parts);
-                const int p = (int)threadIdx.x / ((int)blockDim.x / parts);
+            const int chunk = (int)blockDim.x / parts;
+            for (int base = 0; base < nw; base += chunk) {
+                const int i = base + (int)threadIdx.x % chunk;
+                const int p = (int)threadIdx.x / chunk;
                 float acc = 0.0f;
                 if (i < nw && p < parts)
-                    for (int j = p; j < nw; j += parts) acc = fmaf(a.Mt[(long)j * nw + i], f[a.idx[j]], acc);
+                    for (int j = p; j < nw; j += parts) acc = fmaf(a.Mt[(long)j * nw + i], C.f[a.idx[j]], acc);
                 partial[threadIdx.x] = acc;
                 __syncthreads();
                 if (p == 0 && i < nw) {
-                    float s = 0.0f;
-                    for (int q = 0; q < parts; ++q) s += partial[q * ((int)blockDim.x / parts) + (i - base)];
-                    x[a.idx[i]] = s;
+                    float sum = 0.0f;
+                    for (int q = 0; q < parts; ++q) sum += partial[q * chunk + (i - base)];
+                    C.x[a.idx[i]] = sum;
                 }
                 __syncthreads();
             }
         } else {
-            for (int p = 0; p < a.coarsest_passes; ++p) lds_relax_pass(C, x, f, W(l));
+            for (int pass = 0; pass < a.coarsest_passes; ++pass) lds_relax_pass(C);
         }
         __syncthreads();
     }
-    // Up.
+    // Up: prolongAdd's bilinear correction (OC:354-392) on water, then relax twice.
     for (int l = a.nlev - 2; l >= 0; --l) {
-        const BottomLevel &L = a.lv[l];
-        const BottomLevel &C = a.lv[l + 1];
-        float *x = X(l);
-        const float *xc = X(l + 1);
-        const uint8_t *w = W(l);
+        const BottomView L = view(l), C = view(l + 1);
         const int cpc = L.across / C.across, rpc = L.down / C.down;
         const int ca = C.across, cd = C.down;
-        for (int cell = threadIdx.x; cell < L.across * L.down; cell += blockDim.x) {
-            if (!w[cell]) continue;
-            const int row = cell / L.across, col = cell - row * L.across;
-            const float crow = ((float)row + 0.5f) / (float)rpc - 0.5f;
-            const int above = (int)floorf(crow);
-            const float db = crow - (float)above;
-            const float ccol = ((float)col + 0.5f) / (float)cpc - 0.5f;
-            const int left = (int)floorf(ccol);
-            const float ab = ccol - (float)left;
-            const int wc = (left + ca) % ca, ec = (left + 1) % ca;
-            auto at = [&](int rr, int cc) -> float {
-                if (rr < 0) return a.beyond * xc[cc];
-                if (rr >= cd) return a.beyond * xc[(cd - 1) * ca + cc];
-                return xc[rr * ca + cc];
-            };
-            const float top = at(above, wc) * (1.0f - ab) + at(above, ec) * ab;
-            const float bottom = at(above + 1, wc) * (1.0f - ab) + at(above + 1, ec) * ab;
-            x[cell] += top * (1.0f - db) + bottom * db;
+        const int nfi = L.across * L.down;
+        float pv[kBottomCellsPerThread];
+#pragma unroll
+        for (int k = 0; k < kBottomCellsPerThread; ++k) {
+            const int cell = (int)threadIdx.x + k * (int)blockDim.x;
+            pv[k] = 0.0f;
+            if (cell < nfi && L.w[cell]) {
+                const int row = cell / L.across, col = cell - row * L.across;
+                const float crow = ((float)row + 0.5f) / (float)rpc - 0.5f;
+                const int above = (int)floorf(crow);
+                const float db = crow - (float)above;
+                const float ccol = ((float)col + 0.5f) / (float)cpc - 0.5f;
+                const int left = (int)floorf(ccol);
+                const float ab = ccol - (float)left;
+                const int wc = (left + ca) % ca, ec = (left + 1) % ca;
+                auto at = [&](int rr, int cc) -> float {
+                    if (rr < 0) return a.beyond * C.x[cc];
+                    if (rr >= cd) return a.beyond * C.x[(cd - 1) * ca + cc];
+                    return C.x[rr * ca + cc];
+                };
+                const float top = at(above, wc) * (1.0f - ab) + at(above, ec) * ab;
+                const float bottom = at(above + 1, wc) * (1.0f - ab) + at(above + 1, ec) * ab;
+                pv[k] = L.x[cell] + (top * (1.0f - db) + bottom * db);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kBottomCellsPerThread; ++k) {
+            const int cell = (int)threadIdx.x + k * (int)blockDim.x;
+            if (cell < nfi && L.w[cell]) L.x[cell] = pv[k];
         }
         __syncthreads();
-        lds_relax_pass(L, x, F(l), w);
-        lds_relax_pass(L, x, F(l), w);
+        lds_relax_pass(L);
+        lds_relax_pass(L);
     }
     {
         const int cells = a.lv[0].across * a.lv[0].down;
-        const float *x = X(0);
+        const float *x = reinterpret_cast<const float *>(smem + a.lv[0].off_x);
         for (int c = threadIdx.x; c < cells; c += blockDim.x) a.x_out[c] = x[c];
     }
 }

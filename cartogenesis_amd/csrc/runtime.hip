@@ -220,8 +220,9 @@ hipError_t launch_tile(hipStream_t s, const LevelArgs &L, const float *f, const 
 }
 
 hipError_t tile_dispatch(cg_ctx *ctx, const LevelArgs &L, const float *f, const float *xin, float *xout, int nh,
-                         bool zero, int mode = TILE_PLAIN, const TileExtra &ex = TileExtra{}) {
-    switch (ctx->tile_rows) {
+                         bool zero, int mode = TILE_PLAIN, const TileExtra &ex = TileExtra{}, int rows = 0) {
+    switch (rows ? rows : ctx->tile_rows) {
+        case 4: return launch_tile<4>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex);
         case 8: return launch_tile<8>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex);
         case 24: return launch_tile<24>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex);
         case 32: return launch_tile<32>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex);
@@ -292,8 +293,8 @@ struct cg_plan {
     int x64cur = 0;
     bool x64_valid = false;
     float *x32ref = nullptr;      // reference-method answer lives in lv[0].x[lv[0].cur]
-    // Maxima: [0], [1] the refined method's residual (by binary64 parity), [2] the largest
-    // balance, [3] the reference method's residual.
+    // Maxima, in groups of kMaxSlots: group 0 and 1 the refined method's residual (by binary64
+    // parity), 2 the largest balance, 3 the reference method's residual.
     unsigned long long *d_u64 = nullptr;
     int *d_passes = nullptr;
     double largest = -1.0;        // max |f0 * c0|, cached per forcing
@@ -449,12 +450,18 @@ int plan_fusion_setup(cg_ctx *ctx, cg_plan *p) {
     const size_t limit = bottom_lds_limit(ctx);
     auto level_bytes = [&](int l) {
         const size_t cells = (size_t)p->lv[l].st->cells;
-        return 2 * ((cells * 4 + 15) & ~(size_t)15) + ((cells + 15) & ~(size_t)15);
+        const size_t down = (size_t)p->lv[l].st->down;
+        return 2 * ((cells * 4 + 15) & ~(size_t)15) + ((cells + 15) & ~(size_t)15) + down * 16 +
+               ((down * 4 + 15) & ~(size_t)15);
     };
     int b = n - 1;
     size_t bytes = level_bytes(b);
     if (bytes > limit || n - b > kBottomMaxLevels) return CG_OK;
-    while (b > 0 && n - (b - 1) <= kBottomMaxLevels && bytes + level_bytes(b - 1) <= limit) {
+    // Each of k_bottom's 1024 threads handles at most kBottomCellsPerThread cells of a phase.
+    const long most_cells = 1024L * kBottomCellsPerThread;
+    if (p->lv[b].st->cells > most_cells) return CG_OK;
+    while (b > 0 && n - (b - 1) <= kBottomMaxLevels && bytes + level_bytes(b - 1) <= limit &&
+           p->lv[b - 1].st->cells <= most_cells) {
         --b;
         bytes += level_bytes(b);
     }
@@ -468,12 +475,20 @@ int plan_fusion_setup(cg_ctx *ctx, cg_plan *p) {
     return CG_OK;
 }
 
-int read_u64(cg_plan *p, int slot, unsigned long long *v) {
+// The largest of a group's kMaxSlots maxima, once copied to host memory.
+unsigned long long group_max(const unsigned long long *slots) {
+    unsigned long long m = 0;
+    for (int i = 0; i < kMaxSlots; ++i) m = slots[i] > m ? slots[i] : m;
+    return m;
+}
+
+// Reads the maximum held in slot group `group` (kMaxSlots slots at d_u64 + group * kMaxSlots).
+int read_u64(cg_plan *p, int group, unsigned long long *v) {
     cg_ctx *ctx = p->ctx;
-    HIPCHK(ctx, hipMemcpyAsync(ctx->h_pinned + slot, p->d_u64 + slot, sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                               ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->h_pinned + group * kMaxSlots, p->d_u64 + group * kMaxSlots,
+                               kMaxSlots * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    *v = ctx->h_pinned[slot];
+    *v = group_max(ctx->h_pinned + group * kMaxSlots);
     return CG_OK;
 }
 
@@ -483,15 +498,17 @@ double bits_to_double(unsigned long long b) {
     return d;
 }
 
-constexpr int kReduceBlocks = 2048;
+// The 2D launch grid of the per-cell binary64 kernels.
+inline dim3 grid2d(const Stencil *st) {
+    return dim3(blocks_for(st->across, 256), blocks_for(st->down, kRowsPerBlock));
+}
 
-inline unsigned reduce_grid(long cells) { return (unsigned)std::min<long>(blocks_for(cells, 256), kReduceBlocks); }
-
-// A reduction into slot `slot`, zeroed first by a fill kernel.
-MaxReduce max_into(cg_plan *p, int slot) {
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, p->ctx->stream, p->d_u64 + slot, 1, 0ull);
+// A reduction into slot group `group`, zeroed first by a fill kernel.
+MaxReduce max_into(cg_plan *p, int group) {
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(kMaxSlots), 0, p->ctx->stream, p->d_u64 + group * kMaxSlots, kMaxSlots,
+                       0ull);
     MaxReduce r;
-    r.out = p->d_u64 + slot;
+    r.out = p->d_u64 + group * kMaxSlots;
     r.clear = nullptr;
     return r;
 }
@@ -505,7 +522,7 @@ int plan_largest(cg_plan *p, double *out) {
     cg_ctx *ctx = p->ctx;
     const Stencil *st = p->lv[0].st;
     const MaxReduce red = max_into(p, 2);
-    hipLaunchKernelGGL(k_balance_max, dim3(reduce_grid(st->cells)), dim3(256), 0, ctx->stream, st->args(), p->f0, red);
+    hipLaunchKernelGGL(k_balance_max, grid2d(st), dim3(256), 0, ctx->stream, st->args(), p->f0, red);
     HIPCHK(ctx, hipGetLastError());
     unsigned long long b;
     int rc = read_u64(p, 2, &b);
@@ -522,7 +539,7 @@ int ref_residual(cg_plan *p, const float *f, double *out) {
     cg_ctx *ctx = p->ctx;
     const PlanLevel &l0 = p->lv[0];
     const MaxReduce red = max_into(p, 3);
-    hipLaunchKernelGGL(k_residual64, dim3(reduce_grid(l0.st->cells)), dim3(256), 0, ctx->stream, l0.st->args(), f,
+    hipLaunchKernelGGL(k_residual64, grid2d(l0.st), dim3(256), 0, ctx->stream, l0.st->args(), f,
                        l0.x[l0.cur], (double *)nullptr, red);
     HIPCHK(ctx, hipGetLastError());
     unsigned long long b;
@@ -545,7 +562,7 @@ int ref_vcycle(cg_plan *p, const float *f0, double tol, int fixed) {
         const float *f = l == 0 ? f0 : L.f;
         if ((rc = relax_pp(ctx, L.st, f, L.x, &L.cur, 2, false, l == 0))) return rc;
         MaxReduce none{};
-        hipLaunchKernelGGL(k_residual64, dim3(reduce_grid(L.st->cells)), dim3(256), 0, s, L.st->args(), f, L.x[L.cur],
+        hipLaunchKernelGGL(k_residual64, grid2d(L.st), dim3(256), 0, s, L.st->args(), f, L.x[L.cur],
                            p->r64, none);
         hipLaunchKernelGGL(k_restrict64, dim3(blocks_for(C.st->cells, 256)), dim3(256), 0, s, p->r64, L.st->across,
                            L.st->down, C.st->across, C.st->down, C.bal);
@@ -602,6 +619,15 @@ int ref_solve(cg_plan *p, double tol, cg_solution *info, const volatile int *can
 
 // ---- the refined method ----------------------------------------------------------------------
 
+// Tile rows for a grid of the fused cycle: the context's choice where it gives at least one
+// wavefront per SIMD (1024 on the 256-CU MI355X), else 4, so that a small grid's launch, bound by
+// one wavefront's serial path, runs the shortest path.
+int level_tile_rows(const cg_ctx *ctx, const Stencil *st) {
+    const int th = ctx->tile_rows;
+    const long waves = (long)((st->across + 115) / 116) * ((st->down + th - 1) / th);
+    return waves >= 1024 ? th : 4;
+}
+
 // The fused cycle: per tile level one down launch (smooth from zero, residual, restriction) and
 // one up launch (prolongation, smooth); the bottom levels in one k_bottom launch.
 int vcycle32_fused(cg_plan *p, bool timed) {
@@ -617,7 +643,8 @@ int vcycle32_fused(cg_plan *p, bool timed) {
         ex.g = L.g;
         ex.fc = C.f;
         TimeScope ts(ctx, timed && l == 0, kKindSmoothFromZero);
-        hipError_t e = tile_dispatch(ctx, L.st->args(), L.f, L.x[L.cur], L.x[1 - L.cur], 4, true, TILE_DOWN, ex);
+        hipError_t e = tile_dispatch(ctx, L.st->args(), L.f, L.x[L.cur], L.x[1 - L.cur], 4, true, TILE_DOWN, ex,
+                                     level_tile_rows(ctx, L.st));
         if (e != hipSuccess) {
             set_err(ctx, "down-leg launch failed: %s", hipGetErrorString(e));
             return CG_ERR_DEVICE;
@@ -644,14 +671,20 @@ int vcycle32_fused(cg_plan *p, bool timed) {
         off += (cells * 4 + 15) & ~(size_t)15;
         ba.lv[i].off_w = (int)off;
         off += (cells + 15) & ~(size_t)15;
+        ba.lv[i].off_rw = (int)off;
+        off += (size_t)ba.lv[i].down * 16;
+        ba.lv[i].off_g = (int)off;
+        off += ((size_t)ba.lv[i].down * 4 + 15) & ~(size_t)15;
     }
     PlanLevel &B0 = p->lv[b];
     ba.f_in = B0.f;
     ba.x_out = B0.x[B0.cur];
     ba.idx = p->d_idx;
     ba.Mt = p->d_Mt;
+    // A coarsest grid with no water has the answer zero (its values start at zero); one with an
+    // inverse is solved directly; otherwise it is relaxed a fixed number of passes.
     ba.nw = p->d_Mt ? p->nw : 0;
-    ba.coarsest_passes = kPrecondCoarsestPasses;
+    ba.coarsest_passes = (p->nw == 0) ? 0 : kPrecondCoarsestPasses;
     ba.beyond = beyond;
     hipLaunchKernelGGL(k_bottom, dim3(1), dim3(1024), p->bottom_lds, s, ba);
     for (int l = b - 1; l >= 0; --l) {
@@ -663,7 +696,8 @@ int vcycle32_fused(cg_plan *p, bool timed) {
         ex.cd = C.st->down;
         ex.beyond = beyond;
         TimeScope ts(ctx, timed && l == 0, kKindSmooth);
-        hipError_t e = tile_dispatch(ctx, L.st->args(), L.f, L.x[L.cur], L.x[1 - L.cur], 4, false, TILE_UP, ex);
+        hipError_t e = tile_dispatch(ctx, L.st->args(), L.f, L.x[L.cur], L.x[1 - L.cur], 4, false, TILE_UP, ex,
+                                     level_tile_rows(ctx, L.st));
         if (e != hipSuccess) {
             set_err(ctx, "up-leg launch failed: %s", hipGetErrorString(e));
             return CG_ERR_DEVICE;
@@ -692,7 +726,9 @@ int vcycle32(cg_plan *p, bool timed) {
     PlanLevel &B = p->lv[n - 1];
     if (n == 1) {
         if ((rc = relax_pp(ctx, B.st, B.f, B.x, &B.cur, 2, true, timed))) return rc;
-    } else if (p->nw > 0 && p->d_Mt) {
+    } else if (p->nw == 0) {
+        hipLaunchKernelGGL(k_fill_f32, dim3(blocks_for(B.st->cells, 256)), dim3(256), 0, s, B.x[B.cur], B.st->cells, 0.0f);
+    } else if (p->d_Mt) {
         hipLaunchKernelGGL(k_coarsest_direct, dim3(1), dim3(256), sizeof(float) * p->nw, s, p->d_idx, p->nw, p->d_Mt, B.f,
                            B.x[B.cur], B.st->cells);
     } else {
@@ -719,9 +755,9 @@ int outer_step(cg_plan *p, const float *delta) {
     // Slot x64cur receives the maximum (zeroed by the previous step); the other slot is zeroed for
     // the next step.
     MaxReduce red;
-    red.out = p->d_u64 + p->x64cur;
-    red.clear = p->d_u64 + (1 - p->x64cur);
-    hipLaunchKernelGGL(k_outer_residual, dim3(reduce_grid(st->cells)), dim3(256), 0, ctx->stream, st->args(), p->f0,
+    red.out = p->d_u64 + p->x64cur * kMaxSlots;
+    red.clear = p->d_u64 + (1 - p->x64cur) * kMaxSlots;
+    hipLaunchKernelGGL(k_outer_residual, grid2d(st), dim3(256), 0, ctx->stream, st->args(), p->f0,
                        p->x64[p->x64cur], delta, p->x64[1 - p->x64cur], p->lv[0].f, red);
     HIPCHK(ctx, hipGetLastError());
     p->x64cur = 1 - p->x64cur;
@@ -745,7 +781,7 @@ int build_step_graphs(cg_plan *p) {
         int rc = vcycle32(p, false);
         if (!rc) rc = outer_step(p, p->lv[0].x[p->lv[0].cur]);
         if (!rc) {
-            hipError_t e = hipMemcpyAsync(ctx->h_pinned, p->d_u64, 2 * sizeof(unsigned long long),
+            hipError_t e = hipMemcpyAsync(ctx->h_pinned, p->d_u64, 2 * kMaxSlots * sizeof(unsigned long long),
                                           hipMemcpyDeviceToHost, ctx->stream);
             if (e != hipSuccess) {
                 set_err(ctx, "capture of the residual readback failed: %s", hipGetErrorString(e));
@@ -790,7 +826,7 @@ int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *info, const vo
         HIPCHK(ctx, hipMemsetAsync(p->x64[p->x64cur], 0, sizeof(double) * st->cells, ctx->stream));
     }
     p->x64_valid = true;
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, p->d_u64, 2, 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(2), dim3(kMaxSlots), 0, ctx->stream, p->d_u64, 2 * kMaxSlots, 0ull);
     const int first_slot = p->x64cur;
     if ((rc = outer_step(p, nullptr))) return rc;
     unsigned long long b;
@@ -800,27 +836,46 @@ int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *info, const vo
     if (graph && res >= tol) {
         if ((rc = build_step_graphs(p))) return rc;
     }
+    // Steps run in batches without a host read in between: the batch is the number of steps the
+    // last observed convergence factor predicts are still needed, less one (at least one, at most
+    // 8), so the host waits on the device about twice per solve rather than once per step. The
+    // policy is the same with and without graphs, so both give the same steps and bits.
     int cycles = 0;
+    double rate = -1.0;  // observed residual reduction per step
     while (res >= tol && cycles < kRefinedMostSteps) {
         if (cancel && *cancel) {
             info->status = CG_CANCELLED;
             break;
         }
-        const int slot = p->x64cur;
+        int batch = 1;
+        if (rate > 0.0 && rate < 0.95 && res > 0.0) {
+            const double need = std::ceil(std::log(tol / res) / std::log(rate));
+            batch = (int)std::max(1.0, std::min(8.0, need - 1.0));
+        }
+        batch = std::min(batch, kRefinedMostSteps - cycles);
+        int slot = p->x64cur;
+        for (int k = 0; k < batch; ++k) {
+            slot = p->x64cur;
+            if (graph) {
+                HIPCHK(ctx, hipGraphLaunch(p->step_exec[p->x64cur], ctx->stream));
+                p->x64cur = 1 - p->x64cur;
+            } else {
+                if ((rc = vcycle32(p, true))) return rc;
+                if ((rc = outer_step(p, p->lv[0].x[p->lv[0].cur]))) return rc;
+            }
+        }
         if (graph) {
-            HIPCHK(ctx, hipGraphLaunch(p->step_exec[p->x64cur], ctx->stream));
-            p->x64cur = 1 - p->x64cur;
             HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-            b = ctx->h_pinned[slot];
+            b = group_max(ctx->h_pinned + slot * kMaxSlots);
         } else {
-            if ((rc = vcycle32(p, true))) return rc;
-            if ((rc = outer_step(p, p->lv[0].x[p->lv[0].cur]))) return rc;
             if ((rc = read_u64(p, slot, &b))) return rc;
         }
-        cycles++;
+        cycles += batch;
         const double r = bits_to_double(b);
+        const double prev = res;
         res = largest > 0.0 ? r / largest : 0.0;
         if (std::isnan(r)) break;
+        if (prev > 0.0 && res > 0.0) rate = std::pow(res / prev, 1.0 / batch);
     }
     info->cycles = cycles;
     info->refinements = cycles + 1;
@@ -862,7 +917,8 @@ int cg_ocean_open(int device, cg_ctx **out, char *reason, size_t reason_len) {
         delete ctx;
         return fail("hipStreamCreate failed");
     }
-    if (hipHostMalloc((void **)&ctx->h_pinned, 64 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void **)&ctx->h_pinned, 4 * kMaxSlots * sizeof(unsigned long long), hipHostMallocDefault) !=
+        hipSuccess) {
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return fail("hipHostMalloc failed");
@@ -1132,8 +1188,8 @@ int cg_ocean_plan_create(cg_ctx *ctx, const cg_stencil *levels, int nlevels, int
     if ((rc = dalloc(ctx, &p->r64, n0))) return bail(rc);
     if ((rc = dalloc(ctx, &p->x64[0], n0))) return bail(rc);
     if ((rc = dalloc(ctx, &p->x64[1], n0))) return bail(rc);
-    if ((rc = dalloc(ctx, &p->d_u64, 16))) return bail(rc);
-    HIPCHK(ctx, hipMemset(p->d_u64, 0, 16 * sizeof(unsigned long long)));
+    if ((rc = dalloc(ctx, &p->d_u64, 4 * kMaxSlots))) return bail(rc);
+    HIPCHK(ctx, hipMemset(p->d_u64, 0, 4 * kMaxSlots * sizeof(unsigned long long)));
     if ((rc = dalloc(ctx, &p->d_passes, 4))) return bail(rc);
     HIPCHK(ctx, hipMemset(p->f0, 0, sizeof(float) * n0));
     HIPCHK(ctx, hipMemset(p->x64[0], 0, sizeof(double) * n0));
