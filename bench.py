@@ -62,17 +62,39 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--tol", type=float, default=1e-8)
-    ap.add_argument("--tile-rows", type=int, default=16)
+    ap.add_argument("--tile-rows", type=int, default=8)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-oracle-check", action="store_true")
+    ap.add_argument("--method", choices=["refined", "fgmres"], default="refined")
+    ap.add_argument("--restart", type=int, default=8, help="FGMRES restart length")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy six times (for scaling HBM counters) and exit")
+    ap.add_argument("--level-passes", default="22222222",
+                    help="passes per level of the binary32 cycle, one hex digit per level, finest last")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the circulation solve is a single-GPU path here "
               "(instances would shard one per GPU with no exchange); run with --gpus 1", file=sys.stderr)
         return 2
+
+    if args.copy_calibration:
+        # A plain copy of known size, far beyond L2 and the infinity cache: the reference point
+        # that scales rocprofv3's FETCH_SIZE and WRITE_SIZE to bytes.
+        import torch
+
+        n = 256 * 1024 * 1024
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        c = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+            c.copy_(b)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_per_copy": n * 4, "copies": 6}))
+        return 0
 
     import cartogenesis_amd as cg
     from cartogenesis_amd.ocean import HipOcean, Plan
@@ -86,19 +108,22 @@ def main() -> int:
     acc = res.accelerator
     acc.set_option(cg.CG_OPT_TILE_ROWS, args.tile_rows)
     acc.set_option(cg.CG_OPT_USE_GRAPH, 0 if args.no_graph else 1)
+    acc.set_option(cg.CG_OPT_FGMRES_RESTART, args.restart)
+    acc.set_option(cg.CG_OPT_LEVEL_PASSES, int(args.level_passes, 16))
+    method = cg.CG_METHOD_FGMRES if args.method == "fgmres" else cg.CG_METHOD_REFINED
     plan = Plan(acc, prob.levels)
     lib = cg.lib()
     import ctypes
 
     log(f"bench: warmup {args.warmup}")
     for _ in range(args.warmup):
-        info = plan.solve(args.tol, cg.CG_METHOD_REFINED)
+        info = plan.solve(args.tol, method)
     lib.cg_ocean_kernel_time_us(acc.ctx, -1, None)
     log(f"bench: timing {args.steps} solves")
     cycles = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        info = plan.solve(args.tol, cg.CG_METHOD_REFINED)
+        info = plan.solve(args.tol, method)
         cycles.append(info.cycles)
     elapsed = time.perf_counter() - t0
     value = args.steps / elapsed
@@ -113,7 +138,7 @@ def main() -> int:
     n_timed = max(5, min(args.steps, 20))
     t1 = time.perf_counter()
     for _ in range(n_timed):
-        plan.solve(args.tol, cg.CG_METHOD_REFINED)
+        plan.solve(args.tol, method)
     eager = n_timed / (time.perf_counter() - t1)
     cnt = ctypes.c_long(0)
     us_up = lib.cg_ocean_kernel_time_us(acc.ctx, 1, ctypes.byref(cnt))
@@ -159,8 +184,8 @@ def main() -> int:
             "workload": "circulation 1024x2048 (seed 0, 30% land, annual forcing) from zero to 1e-8: plain GPU "
                         "multigrid, binary32 V(2,2) red-black cycles of the reference ladder inside binary64 "
                         "iterative refinement; learned preconditioner not built",
-            "rows": ROWS, "cols": COLS, "tol": args.tol, "method": "refined",
-            "graph": not args.no_graph, "tile_rows": args.tile_rows,
+            "rows": ROWS, "cols": COLS, "tol": args.tol, "method": args.method,
+            "graph": not args.no_graph, "tile_rows": args.tile_rows, "level_passes": args.level_passes,
             "cycles_per_solve": float(np.mean(cycles)),
             "relative_residual": rel_res,
             "rel_l2_vs_fp64_oracle": err,

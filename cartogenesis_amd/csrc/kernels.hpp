@@ -82,6 +82,18 @@ __device__ __forceinline__ float rb_update(float4 wt, float xe, float xw, float 
 // ------------------------------------------------------------------------------------------------
 enum TileMode { TILE_PLAIN = 0, TILE_DOWN = 1, TILE_UP = 2 };
 
+// Workgroups are dealt to the MI355X's 8 XCDs round-robin by workgroup id, and each XCD has its own
+// L2. Renumbering so that XCD k runs the k-th contiguous eighth of the logical blocks keeps
+// neighbouring tiles, which read each other's halo rows, on one L2. A bijection on [0, gridDim.x):
+// the last gridDim.x mod 8 blocks keep their ids.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block() {
+    const int b = blockIdx.x, nb = gridDim.x;
+    const int full = nb - nb % kXcds;
+    if (b >= full) return b;
+    return (b % kXcds) * (full / kXcds) + b / kXcds;
+}
+
 struct TileExtra {
     // TILE_DOWN
     const uint8_t *cwater;  // next grid's mask
@@ -112,7 +124,7 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
     static_assert(TH % 2 == 0, "tiles start on even rows");
     const int across = L.across, down = L.down;
     const int lane = threadIdx.x & 63;
-    const int wave = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wave = xcd_block() * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int tiles_x = (across + IW - 1) / IW;
     const int tiles_y = (down + TH - 1) / TH;
     if (wave >= tiles_x * tiles_y) return;
@@ -561,41 +573,101 @@ __global__ __launch_bounds__(1024) void k_coarsest_relax(LevelArgs L, const floa
 // Kernels of the refined (binary64 outer, binary32 inner) solve.
 // ------------------------------------------------------------------------------------------------
 
+// A double moved between lanes by one DPP control on each 32-bit half (0x138 wave_shr:1: lane i
+// receives lane i-1's; 0x130 wave_shl:1: lane i+1's). The edge lane receives 0.
+template <int CTRL>
+__device__ __forceinline__ double dpp_double(double v) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xf, 0xf, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 // The outer step: x_new = x + delta (binary64; delta may be null), u = x_new - (sum w x_new - f)
 // in binary64, the correction forcing fc = (float)u on water (0 on land), and the largest
-// |u * centre| into red. Land values are zero in x_new. 2D launch as k_residual64.
+// |u * centre| into red. Land values are zero in x_new.
+//
+// Thread t of a block owns the column pair (2c, 2c+1), c = blockIdx.x * 256 + t, over the
+// kRowsPerBlock rows of blockIdx.y, and loads a window of kRowsPerBlock + 2 rows once (north and
+// south neighbours come from the window); east/west neighbours come from the adjacent lanes by DPP,
+// and only a wave's edge lanes (and the pair at the wrap) load theirs. Needs an even width.
 __global__ __launch_bounds__(256) void k_outer_residual(LevelArgs L, const float *__restrict__ f,
                                                         const double *__restrict__ x, const float *__restrict__ delta,
                                                         double *__restrict__ xnew, float *__restrict__ fc,
                                                         MaxReduce red) {
-    const int across = L.across;
-    const int col = blockIdx.x * 256 + threadIdx.x;
-    unsigned long long m = 0;
-    if (col < across) {
-        const int ce = col + 1 == across ? 0 : col + 1;
-        const int cw = col == 0 ? across - 1 : col - 1;
+    const int across = L.across, down = L.down, half = across >> 1;
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    const bool mine = c < half;
+    const int cc = mine ? c : half - 1;  // idle lanes shadow the last pair, so DPP sources stay in range
+    const int col = 2 * cc;
+    const int r0 = blockIdx.y * kRowsPerBlock;
+    constexpr int W = kRowsPerBlock + 2;
+    double vx[W], vy[W];  // x + delta of the pair, rows r0-1 .. r0+kRowsPerBlock
 #pragma unroll
-        for (int k = 0; k < kRowsPerBlock; ++k) {
-            const int row = blockIdx.y * kRowsPerBlock + k;
-            if (row >= L.down) break;
-            const long cell = (long)row * across + col;
-            double v = 0.0, u = 0.0;
-            if (L.water[cell]) {
-                const long ie = (long)row * across + ce, iw = (long)row * across + cw;
-                auto val = [&](long i) -> double { return delta ? x[i] + (double)delta[i] : x[i]; };
-                v = val(cell);
-                const double vn = row > 0 ? val(cell - across) : 0.0;
-                const double vs = row + 1 < L.down ? val(cell + across) : 0.0;
-                const float4 wt = weights_at(L, row, cell);
-                const double bal = (double)wt.x * val(ie) + (double)wt.y * val(iw) + (double)wt.z * vn +
-                                   (double)wt.w * vs - (double)f[cell];
-                u = v - bal;
-                const unsigned long long b = abs_bits(u * centre_at(L, row, cell));
-                m = b > m ? b : m;
+    for (int i = 0; i < W; ++i) {
+        const int r = r0 - 1 + i;
+        vx[i] = vy[i] = 0.0;
+        if (r >= 0 && r < down) {
+            const long idx = (long)r * across + col;
+            const double2 xv = *reinterpret_cast<const double2 *>(x + idx);
+            vx[i] = xv.x;
+            vy[i] = xv.y;
+            if (delta) {
+                const float2 dv = *reinterpret_cast<const float2 *>(delta + idx);
+                vx[i] += (double)dv.x;
+                vy[i] += (double)dv.y;
             }
-            xnew[cell] = v;
-            fc[cell] = (float)u;
         }
+    }
+    // Columns of the pair's west and east neighbours, and whether the adjacent lane holds them.
+    const int cw = col == 0 ? across - 1 : col - 1;
+    const int ce = col + 2 == across ? 0 : col + 2;
+    const bool lane_w = lane > 0 && col > 0;
+    const bool lane_e = lane < 63 && c + 1 < half;
+    unsigned long long m = 0;
+#pragma unroll
+    for (int k = 0; k < kRowsPerBlock; ++k) {
+        const int i = k + 1;
+        const int row = r0 + k;
+        // Every lane runs the shifts (DPP reads the neighbour lane's registers).
+        double west = dpp_double<0x138>(vy[i]);  // lane - 1's odd column
+        double east = dpp_double<0x130>(vx[i]);  // lane + 1's even column
+        if (!mine || row >= down) continue;
+        const long idx = (long)row * across + col;
+        if (!lane_w) {
+            west = x[(long)row * across + cw];
+            if (delta) west += (double)delta[(long)row * across + cw];
+        }
+        if (!lane_e) {
+            east = x[(long)row * across + ce];
+            if (delta) east += (double)delta[(long)row * across + ce];
+        }
+        const float2 fv = *reinterpret_cast<const float2 *>(f + idx);
+        const uint16_t wm = *reinterpret_cast<const uint16_t *>(L.water + idx);
+        const float4 wt = weights_at(L, row, idx);
+        const double cen = centre_at(L, row, idx);
+        const double vn_x = row > 0 ? vx[i - 1] : 0.0, vn_y = row > 0 ? vy[i - 1] : 0.0;
+        const double vs_x = row + 1 < down ? vx[i + 1] : 0.0, vs_y = row + 1 < down ? vy[i + 1] : 0.0;
+        double ux = 0.0, uy = 0.0, ox = 0.0, oy = 0.0;
+        if (wm & 0xff) {
+            const double bal = (double)wt.x * vy[i] + (double)wt.y * west + (double)wt.z * vn_x + (double)wt.w * vs_x -
+                               (double)fv.x;
+            ox = vx[i];
+            ux = ox - bal;
+            const unsigned long long b = abs_bits(ux * cen);
+            m = b > m ? b : m;
+        }
+        if (wm >> 8) {
+            const double bal = (double)wt.x * east + (double)wt.y * vx[i] + (double)wt.z * vn_y + (double)wt.w * vs_y -
+                               (double)fv.y;
+            oy = vy[i];
+            uy = oy - bal;
+            const unsigned long long b = abs_bits(uy * cen);
+            m = b > m ? b : m;
+        }
+        *reinterpret_cast<double2 *>(xnew + idx) = make_double2(ox, oy);
+        *reinterpret_cast<float2 *>(fc + idx) = make_float2((float)ux, (float)uy);
     }
     grid_max(m, red);
 }
@@ -681,11 +753,14 @@ struct BottomArgs {
     float beyond;
 };
 
-// The bottom's phases: each of the workgroup's 1024 threads handles at most kBottomCellsPerThread
-// cells of a phase, and reads everything it needs for all of them before it writes any, so that
-// its LDS loads overlap instead of each waiting behind the previous cell's store. Within one colour
-// of a red-black sweep no cell reads a cell of its own colour, so the deferred writes change nothing.
-constexpr int kBottomCellsPerThread = 8;
+// The bottom's levels are at most kBottomRows rows by 2 * kBottomPairs columns, so that the
+// workgroup's 1024 threads, seen as 32 x 32 (tx = column pair, ty = row), give every phase at most
+// one cell per thread per colour (two for the prolongation): no loops, no divisions, every LDS
+// read of a phase issued before its one write. The workgroup has one CU's four SIMDs, so the
+// bottom is bound by its instruction count; keeping each phase to a few instructions per cell is
+// what keeps it short.
+constexpr int kBottomRows = 32;
+constexpr int kBottomPairs = 32;
 
 struct BottomView {
     int across, down;
@@ -696,33 +771,23 @@ struct BottomView {
 };
 
 __device__ __forceinline__ void lds_relax_pass(const BottomView &L) {
-    const int across = L.across, half = across >> 1;
-    const int n = half * L.down;
+    const int tx = threadIdx.x & 31, row = threadIdx.x >> 5;
+    const int across = L.across;
+    const bool mine = row < L.down && tx < (across >> 1);
     for (int colour = 0; colour <= 1; ++colour) {
-        float nv[kBottomCellsPerThread];
-        int nc[kBottomCellsPerThread];
-#pragma unroll
-        for (int k = 0; k < kBottomCellsPerThread; ++k) {
-            const int t = (int)threadIdx.x + k * (int)blockDim.x;
-            nc[k] = -1;
-            nv[k] = 0.0f;
-            if (t < n) {
-                const int row = t / half;
-                const int col = 2 * (t - row * half) + ((colour + row) & 1);
-                const int cell = row * across + col;
-                nc[k] = cell;
-                if (L.w[cell]) {
-                    const int ce = col + 1 == across ? 0 : col + 1;
-                    const int cw = col == 0 ? across - 1 : col - 1;
-                    const float xn = row > 0 ? L.x[cell - across] : 0.0f;
-                    const float xs = row + 1 < L.down ? L.x[cell + across] : 0.0f;
-                    nv[k] = rb_update(L.rw[row], L.x[row * across + ce], L.x[row * across + cw], xn, xs, L.f[cell]);
-                }
+        if (mine) {
+            const int col = 2 * tx + ((colour + row) & 1);
+            const int cell = row * across + col;
+            float v = 0.0f;
+            if (L.w[cell]) {
+                const int ce = col + 1 == across ? 0 : col + 1;
+                const int cw = col == 0 ? across - 1 : col - 1;
+                const float xn = row > 0 ? L.x[cell - across] : 0.0f;
+                const float xs = row + 1 < L.down ? L.x[cell + across] : 0.0f;
+                v = rb_update(L.rw[row], L.x[row * across + ce], L.x[row * across + cw], xn, xs, L.f[cell]);
             }
+            L.x[cell] = v;
         }
-#pragma unroll
-        for (int k = 0; k < kBottomCellsPerThread; ++k)
-            if (nc[k] >= 0) L.x[nc[k]] = nv[k];
         __syncthreads();
     }
 }
@@ -730,6 +795,7 @@ __device__ __forceinline__ void lds_relax_pass(const BottomView &L) {
 __global__ __launch_bounds__(1024) void k_bottom(BottomArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ float partial[1024];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     auto view = [&](int l) {
         BottomView v;
         v.across = a.lv[l].across;
@@ -752,9 +818,9 @@ __global__ __launch_bounds__(1024) void k_bottom(BottomArgs a) {
         }
         float4 *rw = reinterpret_cast<float4 *>(smem + B.off_rw);
         float *g = reinterpret_cast<float *>(smem + B.off_g);
-        for (int r = threadIdx.x; r < B.down; r += blockDim.x) {
-            rw[r] = B.roww[r];
-            g[r] = B.g ? B.g[r] : 0.0f;
+        if ((int)threadIdx.x < B.down) {
+            rw[threadIdx.x] = B.roww[threadIdx.x];
+            g[threadIdx.x] = B.g ? B.g[threadIdx.x] : 0.0f;
         }
     }
     {
@@ -769,21 +835,16 @@ __global__ __launch_bounds__(1024) void k_bottom(BottomArgs a) {
         lds_relax_pass(L);
         lds_relax_pass(L);
         const int cpc = L.across / C.across, rpc = L.down / C.down;
-        const int nco = C.across * C.down;
-        float cv[kBottomCellsPerThread];
-#pragma unroll
-        for (int k = 0; k < kBottomCellsPerThread; ++k) {
-            const int t = (int)threadIdx.x + k * (int)blockDim.x;
-            cv[k] = 0.0f;
-            if (t < nco && C.w[t]) {
-                const int crow = t / C.across, ccol = t - crow * C.across;
-                float acc = 0.0f;
+        if (ty < C.down && tx < C.across) {
+            const int t = ty * C.across + tx;
+            float acc = 0.0f;
+            if (C.w[t]) {
                 for (int dr = 0; dr < rpc; ++dr) {
-                    const int row = crow * rpc + dr;
+                    const int row = ty * rpc + dr;
                     const float4 wt = L.rw[row];
                     const float g = L.g[row];
                     for (int dc = 0; dc < cpc; ++dc) {
-                        const int col = ccol * cpc + dc;
+                        const int col = tx * cpc + dc;
                         const int cell = row * L.across + col;
                         if (!L.w[cell]) continue;
                         const int ce = col + 1 == L.across ? 0 : col + 1;
@@ -795,13 +856,8 @@ __global__ __launch_bounds__(1024) void k_bottom(BottomArgs a) {
                         acc += u * g;
                     }
                 }
-                cv[k] = acc;
             }
-        }
-#pragma unroll
-        for (int k = 0; k < kBottomCellsPerThread; ++k) {
-            const int t = (int)threadIdx.x + k * (int)blockDim.x;
-            if (t < nco) C.f[t] = cv[k];
+            C.f[t] = acc;
         }
         __syncthreads();
     }
@@ -832,22 +888,23 @@ __global__ __launch_bounds__(1024) void k_bottom(BottomArgs a) {
         }
         __syncthreads();
     }
-    // Up: prolongAdd's bilinear correction (OC:354-392) on water, then relax twice.
+    // Up: prolongAdd's bilinear correction (OC:354-392) on water, then relax twice. Thread (tx, ty)
+    // owns fine cells (ty, 2 tx) and (ty, 2 tx + 1).
     for (int l = a.nlev - 2; l >= 0; --l) {
         const BottomView L = view(l), C = view(l + 1);
         const int cpc = L.across / C.across, rpc = L.down / C.down;
         const int ca = C.across, cd = C.down;
-        const int nfi = L.across * L.down;
-        float pv[kBottomCellsPerThread];
-#pragma unroll
-        for (int k = 0; k < kBottomCellsPerThread; ++k) {
-            const int cell = (int)threadIdx.x + k * (int)blockDim.x;
-            pv[k] = 0.0f;
-            if (cell < nfi && L.w[cell]) {
-                const int row = cell / L.across, col = cell - row * L.across;
-                const float crow = ((float)row + 0.5f) / (float)rpc - 0.5f;
-                const int above = (int)floorf(crow);
-                const float db = crow - (float)above;
+        float pv[2] = {0.0f, 0.0f};
+        const int row = ty;
+        const bool mine = row < L.down && tx < (L.across >> 1);
+        if (mine) {
+            const float crow = ((float)row + 0.5f) / (float)rpc - 0.5f;
+            const int above = (int)floorf(crow);
+            const float db = crow - (float)above;
+            for (int k = 0; k < 2; ++k) {
+                const int col = 2 * tx + k;
+                const int cell = row * L.across + col;
+                if (!L.w[cell]) continue;
                 const float ccol = ((float)col + 0.5f) / (float)cpc - 0.5f;
                 const int left = (int)floorf(ccol);
                 const float ab = ccol - (float)left;
@@ -862,11 +919,12 @@ __global__ __launch_bounds__(1024) void k_bottom(BottomArgs a) {
                 pv[k] = L.x[cell] + (top * (1.0f - db) + bottom * db);
             }
         }
-#pragma unroll
-        for (int k = 0; k < kBottomCellsPerThread; ++k) {
-            const int cell = (int)threadIdx.x + k * (int)blockDim.x;
-            if (cell < nfi && L.w[cell]) L.x[cell] = pv[k];
-        }
+        __syncthreads();
+        if (mine)
+            for (int k = 0; k < 2; ++k) {
+                const int cell = row * L.across + 2 * tx + k;
+                if (L.w[cell]) L.x[cell] = pv[k];
+            }
         __syncthreads();
         lds_relax_pass(L);
         lds_relax_pass(L);
@@ -876,6 +934,211 @@ __global__ __launch_bounds__(1024) void k_bottom(BottomArgs a) {
         const float *x = reinterpret_cast<const float *>(smem + a.lv[0].off_x);
         for (int c = threadIdx.x; c < cells; c += blockDim.x) a.x_out[c] = x[c];
     }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Flexible GMRES (Saad 1993, SIAM J. Sci. Comput. 14, 461-469) on the binary32 correction problem,
+// right-preconditioned by one binary32 V-cycle, inside the binary64 refinement. Unnormalised
+// Arnoldi: basis vectors q_j are kept with their norms nu_j, so that no pass only rescales. The
+// small dense work (Hessenberg, Givens rotations, back-substitution) runs in one-block kernels on
+// device scalars, so an inner solve needs no host round trip.
+// ------------------------------------------------------------------------------------------------
+constexpr int kFgmresMax = 16;
+
+struct FgmresState {
+    double H[(kFgmresMax + 1) * kFgmresMax];  // column-major, ld = kFgmresMax + 1
+    double cs[kFgmresMax], sn[kFgmresMax];
+    double g[kFgmresMax + 1];
+    double nu[kFgmresMax + 1];
+    double coef[kFgmresMax + 1];  // k_orth's: coef[i] = h_ij / nu_i; scale = 1 / nu_j
+    double scale;
+    double y[kFgmresMax];
+    double resid;                 // |g_{j+1}|: the least-squares residual estimate
+};
+
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int i = 0; i < nw; ++i) t += sh[i];
+    return t;
+}
+
+// partials[b] = sum over the block's cells of a * b (binary64), 2D launch as k_residual64.
+__global__ __launch_bounds__(256) void k_dot_partials(int across, int down, const float *__restrict__ a,
+                                                      const float *__restrict__ b, double *__restrict__ partials) {
+    __shared__ double sh[16];
+    const int col = blockIdx.x * 256 + threadIdx.x;
+    double acc = 0.0;
+    if (col < across) {
+#pragma unroll
+        for (int k = 0; k < kRowsPerBlock; ++k) {
+            const int row = blockIdx.y * kRowsPerBlock + k;
+            if (row >= down) break;
+            const long cell = (long)row * across + col;
+            acc += (double)a[cell] * (double)b[cell];
+        }
+    }
+    const double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x + blockIdx.y * gridDim.x] = t;
+}
+
+// a = (W - I) z on water (0 on land), and partials[i * nb + b] = sum a * q_i for i <= j.
+__global__ __launch_bounds__(256) void k_apply_dots(LevelArgs L, const float *__restrict__ z, float *__restrict__ a,
+                                                    const float *__restrict__ Q, long qstride, int j,
+                                                    double *__restrict__ partials) {
+    __shared__ double sh[16];
+    const int across = L.across;
+    const int col = blockIdx.x * 256 + threadIdx.x;
+    double acc[kFgmresMax + 1];
+#pragma unroll
+    for (int i = 0; i <= kFgmresMax; ++i) acc[i] = 0.0;
+    if (col < across) {
+        const int ce = col + 1 == across ? 0 : col + 1;
+        const int cw = col == 0 ? across - 1 : col - 1;
+#pragma unroll
+        for (int k = 0; k < kRowsPerBlock; ++k) {
+            const int row = blockIdx.y * kRowsPerBlock + k;
+            if (row >= L.down) break;
+            const long cell = (long)row * across + col;
+            float v = 0.0f;
+            if (L.water[cell]) {
+                const float zn = row > 0 ? z[cell - across] : 0.0f;
+                const float zs = row + 1 < L.down ? z[cell + across] : 0.0f;
+                v = rb_update(weights_at(L, row, cell), z[(long)row * across + ce], z[(long)row * across + cw], zn, zs,
+                              z[cell]);
+            }
+            a[cell] = v;
+#pragma unroll
+            for (int i = 0; i <= kFgmresMax; ++i)
+                if (i <= j) acc[i] += (double)v * (double)Q[i * qstride + cell];
+        }
+    }
+    const int nb = gridDim.x * gridDim.y, b = blockIdx.x + blockIdx.y * gridDim.x;
+#pragma unroll
+    for (int i = 0; i <= kFgmresMax; ++i) {
+        if (i > j) break;
+        const double t = block_sum(acc[i], sh);
+        if (threadIdx.x == 0) partials[(long)i * nb + b] = t;
+    }
+}
+
+// q_{j+1} = a * scale - sum_{i<=j} coef_i q_i, and partials[b] = sum q_{j+1}^2.
+__global__ __launch_bounds__(256) void k_orth(int across, int down, const float *__restrict__ a, float *Q, long qstride,
+                                              int j, const FgmresState *__restrict__ st, double *__restrict__ partials) {
+    __shared__ double sh[16];
+    const int col = blockIdx.x * 256 + threadIdx.x;
+    double acc = 0.0;
+    if (col < across) {
+        const float scale = (float)st->scale;
+        float c[kFgmresMax + 1];
+#pragma unroll
+        for (int i = 0; i <= kFgmresMax; ++i) c[i] = i <= j ? (float)st->coef[i] : 0.0f;
+        float *out = Q + (long)(j + 1) * qstride;
+#pragma unroll
+        for (int k = 0; k < kRowsPerBlock; ++k) {
+            const int row = blockIdx.y * kRowsPerBlock + k;
+            if (row >= down) break;
+            const long cell = (long)row * across + col;
+            float v = a[cell] * scale;
+#pragma unroll
+            for (int i = 0; i <= kFgmresMax; ++i)
+                if (i <= j) v -= c[i] * Q[i * qstride + cell];
+            out[cell] = v;
+            acc += (double)v * (double)v;
+        }
+    }
+    const double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x + blockIdx.y * gridDim.x] = t;
+}
+
+// The inner solve's start: nu_0 = beta = |q_0| from the partials, g = (beta, 0, ...).
+__global__ __launch_bounds__(256) void k_fgmres_start(const double *__restrict__ partials, int nb, FgmresState *st) {
+    __shared__ double sh[16];
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) v += partials[b];
+    const double t = block_sum(v, sh);
+    if (threadIdx.x == 0) {
+        const double beta = sqrt(t);
+        st->nu[0] = beta;
+        for (int i = 0; i <= kFgmresMax; ++i) st->g[i] = 0.0;
+        st->g[0] = beta;
+        st->resid = beta;
+    }
+}
+
+// h_ij = <a, q_i> / (nu_j nu_i); k_orth's coefficients h_ij / nu_i and scale 1 / nu_j.
+__global__ __launch_bounds__(256) void k_fgmres_dots(const double *__restrict__ partials, int nb, int j,
+                                                     FgmresState *st) {
+    __shared__ double sh[16];
+    for (int i = 0; i <= j; ++i) {
+        double v = 0.0;
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) v += partials[(long)i * nb + b];
+        const double t = block_sum(v, sh);
+        if (threadIdx.x == 0) {
+            const double nuj = st->nu[j];
+            const double h = nuj > 0.0 && st->nu[i] > 0.0 ? t / (nuj * st->nu[i]) : 0.0;
+            st->H[j * (kFgmresMax + 1) + i] = h;
+            st->coef[i] = st->nu[i] > 0.0 ? h / st->nu[i] : 0.0;
+        }
+    }
+    if (threadIdx.x == 0) st->scale = st->nu[j] > 0.0 ? 1.0 / st->nu[j] : 0.0;
+}
+
+// nu_{j+1} = |q_{j+1}| = h_{j+1,j}; the previous rotations applied to column j, a new one chosen
+// to annihilate h_{j+1,j}, and g rotated with it.
+__global__ __launch_bounds__(256) void k_fgmres_norm(const double *__restrict__ partials, int nb, int j,
+                                                     FgmresState *st) {
+    __shared__ double sh[16];
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) v += partials[b];
+    const double t = block_sum(v, sh);
+    if (threadIdx.x != 0) return;
+    const double hn = sqrt(t);
+    st->nu[j + 1] = hn;
+    double *h = st->H + j * (kFgmresMax + 1);
+    h[j + 1] = hn;
+    for (int i = 0; i < j; ++i) {
+        const double a = st->cs[i] * h[i] + st->sn[i] * h[i + 1];
+        const double b = -st->sn[i] * h[i] + st->cs[i] * h[i + 1];
+        h[i] = a;
+        h[i + 1] = b;
+    }
+    const double r = hypot(h[j], h[j + 1]);
+    const double c = r > 0.0 ? h[j] / r : 1.0, s = r > 0.0 ? h[j + 1] / r : 0.0;
+    st->cs[j] = c;
+    st->sn[j] = s;
+    h[j] = r;
+    h[j + 1] = 0.0;
+    st->g[j + 1] = -s * st->g[j];
+    st->g[j] = c * st->g[j];
+    st->resid = fabs(st->g[j + 1]);
+}
+
+// y = R^-1 g over the first m columns, and the combination's coefficients y_j / nu_j.
+__global__ void k_fgmres_solve(int m, FgmresState *st) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int i = m - 1; i >= 0; --i) {
+        double v = st->g[i];
+        for (int k = i + 1; k < m; ++k) v -= st->H[k * (kFgmresMax + 1) + i] * st->y[k];
+        const double d = st->H[i * (kFgmresMax + 1) + i];
+        st->y[i] = d != 0.0 ? v / d : 0.0;
+    }
+    for (int i = 0; i < m; ++i) st->coef[i] = st->nu[i] > 0.0 ? st->y[i] / st->nu[i] : 0.0;
+}
+
+// delta = sum_{i<m} coef_i z_i.
+__global__ __launch_bounds__(256) void k_combine(long cells, const float *__restrict__ Z, long zstride, int m,
+                                                 const FgmresState *__restrict__ st, float *__restrict__ delta) {
+    const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= cells) return;
+    double v = 0.0;
+    for (int i = 0; i < m; ++i) v += st->coef[i] * (double)Z[i * zstride + cell];
+    delta[cell] = (float)v;
 }
 
 // x64 <- (double) x32, and the reverse, elementwise.

@@ -83,6 +83,9 @@ struct cg_ctx {
     int steps_per_check = 1;
     int time_kernels = 0;
     int fused = 1;
+    // Passes per level of the fused binary32 cycle, 4 bits per level from level 0 up (2, 4 or 6;
+    // levels past the eighth and the bottom's run 2).
+    long level_passes = 0x22222222;
     size_t lds_limit = 0;       // dynamic LDS a workgroup may use (k_bottom)
     bool bottom_attr_set = false;
     std::map<uint64_t, Stencil *> stencils;
@@ -193,29 +196,33 @@ struct TimeScope {
 // in launches of two passes (and one of one pass for an odd count), ping-ponging between xin and
 // xout so that the final values land in xout. xin is read only when zero_start is false. For
 // per-cell weights the colour kernel runs in place on xout after copying xin.
-template <int TH, int MODE>
-hipError_t launch_tile_mode(hipStream_t s, const LevelArgs &L, const float *f, const float *xin, float *xout, int nh,
-                            bool zero, const TileExtra &ex) {
-    const int iw = nh == 4 ? TileGeom<4, MODE>::IW : TileGeom<2, MODE>::IW;
+template <int TH, int MODE, int NH>
+hipError_t launch_tile_mode(hipStream_t s, const LevelArgs &L, const float *f, const float *xin, float *xout, bool zero,
+                            const TileExtra &ex) {
+    const int iw = TileGeom<NH, MODE>::IW;
     const long waves = (long)((L.across + iw - 1) / iw) * ((L.down + TH - 1) / TH);
     const unsigned blocks = (unsigned)((waves + 3) / 4);
-    if (nh == 4) {
-        if (zero) hipLaunchKernelGGL((k_relax_tile<4, TH, true, MODE>), dim3(blocks), dim3(256), 0, s, L, f, xin, xout, ex);
-        else hipLaunchKernelGGL((k_relax_tile<4, TH, false, MODE>), dim3(blocks), dim3(256), 0, s, L, f, xin, xout, ex);
-    } else {
-        if (zero) hipLaunchKernelGGL((k_relax_tile<2, TH, true, MODE>), dim3(blocks), dim3(256), 0, s, L, f, xin, xout, ex);
-        else hipLaunchKernelGGL((k_relax_tile<2, TH, false, MODE>), dim3(blocks), dim3(256), 0, s, L, f, xin, xout, ex);
-    }
+    if (zero) hipLaunchKernelGGL((k_relax_tile<NH, TH, true, MODE>), dim3(blocks), dim3(256), 0, s, L, f, xin, xout, ex);
+    else hipLaunchKernelGGL((k_relax_tile<NH, TH, false, MODE>), dim3(blocks), dim3(256), 0, s, L, f, xin, xout, ex);
     return hipGetLastError();
 }
 
 template <int TH>
 hipError_t launch_tile(hipStream_t s, const LevelArgs &L, const float *f, const float *xin, float *xout, int nh,
                        bool zero, int mode, const TileExtra &ex) {
+    // The fused down and up legs run 2, 4 or 6 passes (4, 8 or 12 half-sweeps) in their one launch.
     switch (mode) {
-        case TILE_DOWN: return launch_tile_mode<TH, TILE_DOWN>(s, L, f, xin, xout, 4, true, ex);
-        case TILE_UP: return launch_tile_mode<TH, TILE_UP>(s, L, f, xin, xout, 4, false, ex);
-        default: return launch_tile_mode<TH, TILE_PLAIN>(s, L, f, xin, xout, nh, zero, ex);
+        case TILE_DOWN:
+            if (nh == 12) return launch_tile_mode<TH, TILE_DOWN, 12>(s, L, f, xin, xout, true, ex);
+            if (nh == 8) return launch_tile_mode<TH, TILE_DOWN, 8>(s, L, f, xin, xout, true, ex);
+            return launch_tile_mode<TH, TILE_DOWN, 4>(s, L, f, xin, xout, true, ex);
+        case TILE_UP:
+            if (nh == 12) return launch_tile_mode<TH, TILE_UP, 12>(s, L, f, xin, xout, false, ex);
+            if (nh == 8) return launch_tile_mode<TH, TILE_UP, 8>(s, L, f, xin, xout, false, ex);
+            return launch_tile_mode<TH, TILE_UP, 4>(s, L, f, xin, xout, false, ex);
+        default:
+            if (nh == 4) return launch_tile_mode<TH, TILE_PLAIN, 4>(s, L, f, xin, xout, zero, ex);
+            return launch_tile_mode<TH, TILE_PLAIN, 2>(s, L, f, xin, xout, zero, ex);
     }
 }
 
@@ -312,6 +319,12 @@ struct cg_plan {
     bool fusable = false;
     int bottom = 0;
     size_t bottom_lds = 0;
+    // FGMRES workspace (allocated on first use): basis Q (kFgmresMax + 1 vectors) and
+    // preconditioned vectors Z (kFgmresMax), each `vstride` floats (a grid with its zero row).
+    float *fq = nullptr, *fz = nullptr, *fa = nullptr, *fdelta = nullptr;
+    long vstride = 0;
+    double *fpart = nullptr;
+    FgmresState *fst = nullptr;
     int last_answer = 0;  // 0: reference (binary32 in lv[0]); 1: binary64 in x64[x64cur]
 };
 
@@ -328,6 +341,12 @@ void free_plan(cg_plan *p) {
     }
     dfree(p->f0);
     dfree(p->r64);
+    dfree(p->fq);
+    dfree(p->fz);
+    dfree(p->fa);
+    dfree(p->fdelta);
+    dfree(p->fpart);
+    dfree(p->fst);
     dfree(p->x64[0]);
     dfree(p->x64[1]);
     dfree(p->d_u64);
@@ -457,11 +476,12 @@ int plan_fusion_setup(cg_ctx *ctx, cg_plan *p) {
     int b = n - 1;
     size_t bytes = level_bytes(b);
     if (bytes > limit || n - b > kBottomMaxLevels) return CG_OK;
-    // Each of k_bottom's 1024 threads handles at most kBottomCellsPerThread cells of a phase.
-    const long most_cells = 1024L * kBottomCellsPerThread;
-    if (p->lv[b].st->cells > most_cells) return CG_OK;
-    while (b > 0 && n - (b - 1) <= kBottomMaxLevels && bytes + level_bytes(b - 1) <= limit &&
-           p->lv[b - 1].st->cells <= most_cells) {
+    // k_bottom's levels are at most kBottomRows rows by 2 kBottomPairs columns.
+    auto fits = [&](int l) {
+        return p->lv[l].st->down <= kBottomRows && p->lv[l].st->across <= 2 * kBottomPairs;
+    };
+    if (!fits(b)) return CG_OK;
+    while (b > 0 && n - (b - 1) <= kBottomMaxLevels && bytes + level_bytes(b - 1) <= limit && fits(b - 1)) {
         --b;
         bytes += level_bytes(b);
     }
@@ -628,9 +648,15 @@ int level_tile_rows(const cg_ctx *ctx, const Stencil *st) {
     return waves >= 1024 ? th : 4;
 }
 
+// Half-sweeps of level l's fused legs: twice its passes in the context's schedule.
+int level_half_sweeps(const cg_ctx *ctx, int l) {
+    if (l >= 8) return 4;
+    return 2 * (int)((ctx->level_passes >> (4 * l)) & 0xf);
+}
+
 // The fused cycle: per tile level one down launch (smooth from zero, residual, restriction) and
 // one up launch (prolongation, smooth); the bottom levels in one k_bottom launch.
-int vcycle32_fused(cg_plan *p, bool timed) {
+int vcycle32_fused(cg_plan *p, bool timed, const float *in0, float *out0) {
     cg_ctx *ctx = p->ctx;
     hipStream_t s = ctx->stream;
     const int n = p->n, b = p->bottom;
@@ -643,8 +669,9 @@ int vcycle32_fused(cg_plan *p, bool timed) {
         ex.g = L.g;
         ex.fc = C.f;
         TimeScope ts(ctx, timed && l == 0, kKindSmoothFromZero);
-        hipError_t e = tile_dispatch(ctx, L.st->args(), L.f, L.x[L.cur], L.x[1 - L.cur], 4, true, TILE_DOWN, ex,
-                                     level_tile_rows(ctx, L.st));
+        const float *fl = (l == 0 && in0) ? in0 : L.f;
+        hipError_t e = tile_dispatch(ctx, L.st->args(), fl, L.x[L.cur], L.x[1 - L.cur], level_half_sweeps(ctx, l), true,
+                                     TILE_DOWN, ex, level_tile_rows(ctx, L.st));
         if (e != hipSuccess) {
             set_err(ctx, "down-leg launch failed: %s", hipGetErrorString(e));
             return CG_ERR_DEVICE;
@@ -677,8 +704,8 @@ int vcycle32_fused(cg_plan *p, bool timed) {
         off += ((size_t)ba.lv[i].down * 4 + 15) & ~(size_t)15;
     }
     PlanLevel &B0 = p->lv[b];
-    ba.f_in = B0.f;
-    ba.x_out = B0.x[B0.cur];
+    ba.f_in = (b == 0 && in0) ? in0 : B0.f;
+    ba.x_out = (b == 0 && out0) ? out0 : B0.x[B0.cur];
     ba.idx = p->d_idx;
     ba.Mt = p->d_Mt;
     // A coarsest grid with no water has the answer zero (its values start at zero); one with an
@@ -696,8 +723,10 @@ int vcycle32_fused(cg_plan *p, bool timed) {
         ex.cd = C.st->down;
         ex.beyond = beyond;
         TimeScope ts(ctx, timed && l == 0, kKindSmooth);
-        hipError_t e = tile_dispatch(ctx, L.st->args(), L.f, L.x[L.cur], L.x[1 - L.cur], 4, false, TILE_UP, ex,
-                                     level_tile_rows(ctx, L.st));
+        const float *fl = (l == 0 && in0) ? in0 : L.f;
+        float *xo = (l == 0 && out0) ? out0 : L.x[1 - L.cur];
+        hipError_t e = tile_dispatch(ctx, L.st->args(), fl, L.x[L.cur], xo, level_half_sweeps(ctx, l), false, TILE_UP,
+                                     ex, level_tile_rows(ctx, L.st));
         if (e != hipSuccess) {
             set_err(ctx, "up-leg launch failed: %s", hipGetErrorString(e));
             return CG_ERR_DEVICE;
@@ -710,8 +739,24 @@ int vcycle32_fused(cg_plan *p, bool timed) {
 
 // One binary32 V(2,2) cycle on the correction problem from zero, with level 0's forcing lv[0].f.
 // The answer is left in lv[0].x[lv[0].cur].
-int vcycle32(cg_plan *p, bool timed) {
-    if (p->ctx->fused && p->fusable) return vcycle32_fused(p, timed);
+int vcycle32_unfused(cg_plan *p, bool timed);
+
+// One binary32 V(2,2) cycle on the correction problem from zero. Level 0's forcing is in0 (or
+// lv[0].f) and its answer goes to out0 (or stays in lv[0].x[lv[0].cur]).
+int vcycle32(cg_plan *p, bool timed, const float *in0 = nullptr, float *out0 = nullptr) {
+    if (p->ctx->fused && p->fusable) return vcycle32_fused(p, timed, in0, out0);
+    cg_ctx *ctx = p->ctx;
+    PlanLevel &L0 = p->lv[0];
+    if (in0) HIPCHK(ctx, hipMemcpyAsync(L0.f, in0, sizeof(float) * L0.st->cells, hipMemcpyDeviceToDevice, ctx->stream));
+    int rc = vcycle32_unfused(p, timed);
+    if (rc) return rc;
+    if (out0)
+        HIPCHK(ctx, hipMemcpyAsync(out0, L0.x[L0.cur], sizeof(float) * L0.st->cells, hipMemcpyDeviceToDevice,
+                                   ctx->stream));
+    return CG_OK;
+}
+
+int vcycle32_unfused(cg_plan *p, bool timed) {
     cg_ctx *ctx = p->ctx;
     hipStream_t s = ctx->stream;
     const int n = p->n;
@@ -749,7 +794,7 @@ int vcycle32(cg_plan *p, bool timed) {
 }
 
 // The outer binary64 step: x64[next] = x64[cur] + delta, its residual into lv[0].f, max into slot 0.
-int outer_step(cg_plan *p, const float *delta) {
+int outer_step(cg_plan *p, const float *delta, float *fc = nullptr) {
     cg_ctx *ctx = p->ctx;
     const Stencil *st = p->lv[0].st;
     // Slot x64cur receives the maximum (zeroed by the previous step); the other slot is zeroed for
@@ -757,8 +802,9 @@ int outer_step(cg_plan *p, const float *delta) {
     MaxReduce red;
     red.out = p->d_u64 + p->x64cur * kMaxSlots;
     red.clear = p->d_u64 + (1 - p->x64cur) * kMaxSlots;
-    hipLaunchKernelGGL(k_outer_residual, grid2d(st), dim3(256), 0, ctx->stream, st->args(), p->f0,
-                       p->x64[p->x64cur], delta, p->x64[1 - p->x64cur], p->lv[0].f, red);
+    const dim3 grid(blocks_for(st->across / 2, 256), blocks_for(st->down, kRowsPerBlock));
+    hipLaunchKernelGGL(k_outer_residual, grid, dim3(256), 0, ctx->stream, st->args(), p->f0,
+                       p->x64[p->x64cur], delta, p->x64[1 - p->x64cur], fc ? fc : p->lv[0].f, red);
     HIPCHK(ctx, hipGetLastError());
     p->x64cur = 1 - p->x64cur;
     return CG_OK;
@@ -884,6 +930,86 @@ int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *info, const vo
     return CG_OK;
 }
 
+// The binary64 refinement whose inner solve is FGMRES(m) right-preconditioned by the binary32
+// V-cycle. Each outer step: m inner iterations from the current residual (q_0 = u, the correction
+// forcing the outer kernel writes), delta from the least-squares combination, then the outer
+// binary64 residual. m is the context's restart length for the first step, then the number of
+// iterations the observed rate predicts are still needed (plus one), within 2..restart.
+int fgmres_solve(cg_plan *p, double tol, bool warm, cg_solution *info, const volatile int *cancel) {
+    cg_ctx *ctx = p->ctx;
+    hipStream_t s = ctx->stream;
+    const Stencil *st = p->lv[0].st;
+    const int M = std::min(ctx->fgmres_m, kFgmresMax);
+    int rc;
+    if (!p->fq) {
+        p->vstride = (long)st->across * (st->down + 1);
+        if ((rc = dalloc(ctx, &p->fq, (size_t)(kFgmresMax + 1) * p->vstride))) return rc;
+        if ((rc = dalloc(ctx, &p->fz, (size_t)kFgmresMax * p->vstride))) return rc;
+        if ((rc = dalloc(ctx, &p->fa, st->cells))) return rc;
+        if ((rc = dalloc(ctx, &p->fdelta, st->cells))) return rc;
+        const dim3 g = grid2d(st);
+        if ((rc = dalloc(ctx, &p->fpart, (size_t)(kFgmresMax + 1) * g.x * g.y))) return rc;
+        if ((rc = dalloc(ctx, &p->fst, 1))) return rc;
+        HIPCHK(ctx, hipMemset(p->fq, 0, sizeof(float) * (kFgmresMax + 1) * p->vstride));
+        HIPCHK(ctx, hipMemset(p->fz, 0, sizeof(float) * kFgmresMax * p->vstride));
+        HIPCHK(ctx, hipMemset(p->fst, 0, sizeof(FgmresState)));
+    }
+    double largest;
+    if ((rc = plan_largest(p, &largest))) return rc;
+    if (!warm || !p->x64_valid) HIPCHK(ctx, hipMemsetAsync(p->x64[p->x64cur], 0, sizeof(double) * st->cells, s));
+    p->x64_valid = true;
+    hipLaunchKernelGGL(k_fill_u64, dim3(2), dim3(kMaxSlots), 0, s, p->d_u64, 2 * kMaxSlots, 0ull);
+    int slot = p->x64cur;
+    if ((rc = outer_step(p, nullptr, p->fq))) return rc;
+    unsigned long long bits;
+    if ((rc = read_u64(p, slot, &bits))) return rc;
+    double res = largest > 0.0 ? bits_to_double(bits) / largest : 0.0;
+    const dim3 g2 = grid2d(st);
+    const int nb = (int)(g2.x * g2.y);
+    int cycles = 0, outer = 0;
+    double rate = -1.0;
+    while (res >= tol && cycles < kRefinedMostSteps) {
+        if (cancel && *cancel) {
+            info->status = CG_CANCELLED;
+            break;
+        }
+        int m = M;
+        if (rate > 0.0 && rate < 0.95 && res > 0.0)
+            m = (int)std::max(2.0, std::min((double)M, std::ceil(std::log(tol / res) / std::log(rate)) + 1.0));
+        hipLaunchKernelGGL(k_dot_partials, g2, dim3(256), 0, s, st->across, st->down, p->fq, p->fq, p->fpart);
+        hipLaunchKernelGGL(k_fgmres_start, dim3(1), dim3(256), 0, s, p->fpart, nb, p->fst);
+        for (int j = 0; j < m; ++j) {
+            float *qj = p->fq + (long)j * p->vstride;
+            float *zj = p->fz + (long)j * p->vstride;
+            if ((rc = vcycle32(p, false, qj, zj))) return rc;
+            hipLaunchKernelGGL(k_apply_dots, g2, dim3(256), 0, s, st->args(), zj, p->fa, p->fq, p->vstride, j, p->fpart);
+            hipLaunchKernelGGL(k_fgmres_dots, dim3(1), dim3(256), 0, s, p->fpart, nb, j, p->fst);
+            hipLaunchKernelGGL(k_orth, g2, dim3(256), 0, s, st->across, st->down, p->fa, p->fq, p->vstride, j, p->fst,
+                               p->fpart);
+            hipLaunchKernelGGL(k_fgmres_norm, dim3(1), dim3(256), 0, s, p->fpart, nb, j, p->fst);
+        }
+        hipLaunchKernelGGL(k_fgmres_solve, dim3(1), dim3(64), 0, s, m, p->fst);
+        hipLaunchKernelGGL(k_combine, dim3(blocks_for(st->cells, 256)), dim3(256), 0, s, st->cells, p->fz, p->vstride,
+                           m, p->fst, p->fdelta);
+        HIPCHK(ctx, hipGetLastError());
+        slot = p->x64cur;
+        if ((rc = outer_step(p, p->fdelta, p->fq))) return rc;
+        if ((rc = read_u64(p, slot, &bits))) return rc;
+        cycles += m;
+        outer++;
+        const double prev = res;
+        const double r = bits_to_double(bits);
+        res = largest > 0.0 ? r / largest : 0.0;
+        if (std::isnan(r)) break;
+        if (prev > 0.0 && res > 0.0) rate = std::pow(res / prev, 1.0 / m);
+    }
+    info->cycles = cycles;
+    info->refinements = outer + 1;
+    info->relative_residual = res;
+    p->last_answer = 1;
+    return CG_OK;
+}
+
 }  // namespace
 
 // ================================================================================================
@@ -977,6 +1103,16 @@ int cg_ocean_set_option(cg_ctx *ctx, int option, long value) {
         case CG_OPT_STEPS_PER_CHECK: ctx->steps_per_check = value < 1 ? 1 : (int)value; return CG_OK;
         case 6: ctx->time_kernels = value ? 1 : 0; return CG_OK;
         case 7: ctx->fused = value ? 1 : 0; return CG_OK;
+        case 8:
+            for (int l = 0; l < 8; ++l) {
+                const long v = (value >> (4 * l)) & 0xf;
+                if (v != 2 && v != 4 && v != 6) {
+                    set_err(ctx, "level passes must be 2, 4 or 6 (level %d has %ld)", l, v);
+                    return CG_ERR_ARG;
+                }
+            }
+            ctx->level_passes = value;
+            return CG_OK;
         default: set_err(ctx, "unknown option %d", option); return CG_ERR_ARG;
     }
 }
@@ -991,6 +1127,7 @@ long cg_ocean_get_option(cg_ctx *ctx, int option) {
         case CG_OPT_STEPS_PER_CHECK: return ctx->steps_per_check;
         case 6: return ctx->time_kernels;
         case 7: return ctx->fused;
+        case 8: return ctx->level_passes;
         default: return -1;
     }
 }
@@ -1241,6 +1378,7 @@ static int plan_solve_locked(cg_plan *p, double tol, int method, int warm, cg_so
         return CG_OK;
     }
     if (method == CG_METHOD_REFINED) return refined_solve(p, tol, warm != 0, info, cancel);
+    if (method == CG_METHOD_FGMRES) return fgmres_solve(p, tol, warm != 0, info, cancel);
     set_err(ctx, "unknown or unbuilt method %d", method);
     return CG_ERR_ARG;
 }

@@ -142,3 +142,58 @@ def test_fused_and_unfused_cycles_reach_the_oracle(acc, rows):
         assert sol.relative_residual < 1e-11
         assert rel_l2(sol.values64, ref64) <= PARITY_L2
     assert abs(out[1].cycles - out[0].cycles) <= 1
+
+
+@pytest.mark.parametrize("rows,seed", [(64, 0), (256, 0), (128, 4)])
+def test_fgmres_method_matches_the_binary64_oracle(acc, rows, seed):
+    """FGMRES(m) right-preconditioned by the binary32 cycle, inside binary64 refinement: the same
+    fixed point as the refined method, in fewer V-cycles."""
+    from cartogenesis_amd import CG_METHOD_FGMRES
+    p = circulation_problem(rows, seed=seed)
+    ref64 = oracle.solve64([to_oracle(s) for s in p.levels], 1e-13)[0]
+    sol = acc.solve_levels(p.levels, None, 1e-12, True, CG_METHOD_FGMRES)
+    plain = acc.solve_levels(p.levels, None, 1e-12, True, CG_METHOD_REFINED)
+    err = rel_l2(sol.values64, ref64)
+    print(f"OCEAN fgmres {rows}x{2 * rows} seed {seed}: {sol.cycles} V-cycles in {sol.refinements} refinements "
+          f"(plain {plain.cycles}), residual {sol.relative_residual}, rel L2 {err}")
+    assert sol.relative_residual < 1e-12
+    assert oracle_residual(p.levels, sol.values64) < 1e-12
+    assert err <= PARITY_L2
+
+
+def test_fgmres_full_size_converges_and_repeats(acc):
+    from cartogenesis_amd import CG_METHOD_FGMRES
+    p = circulation_problem(1024, seed=0)
+    plan = Plan(acc, p.levels)
+    try:
+        info = plan.solve(1e-8, CG_METHOD_FGMRES)
+        x1, _ = plan.read()
+        assert info.relative_residual < 1e-8
+        assert oracle_residual(p.levels, x1) < 1e-8
+        info2 = plan.solve(1e-8, CG_METHOD_FGMRES)
+        x2, _ = plan.read()
+        assert info2.cycles == info.cycles
+        assert np.array_equal(x1.view(np.uint64), x2.view(np.uint64))
+        plain = plan.solve(1e-8, CG_METHOD_REFINED)
+        print(f"OCEAN full size fgmres: {info.cycles} V-cycles, {info.refinements} refinements; plain {plain.cycles}")
+    finally:
+        plan.close()
+
+
+@pytest.mark.parametrize("schedule", [0x22222462, 0x22224662, 0x22222242])
+def test_level_pass_schedules_reach_the_oracle(acc, schedule):
+    """More passes on the coarser grids of the binary32 cycle change its convergence rate, never
+    the fixed point."""
+    from cartogenesis_amd import CG_OPT_LEVEL_PASSES
+    p = circulation_problem(256, seed=0)
+    ref64 = oracle.solve64([to_oracle(s) for s in p.levels], 1e-13)[0]
+    acc.set_option(CG_OPT_LEVEL_PASSES, schedule)
+    try:
+        sol = acc.solve_levels(p.levels, None, 1e-12, True, CG_METHOD_REFINED)
+    finally:
+        acc.set_option(CG_OPT_LEVEL_PASSES, 0x22222222)
+    base = acc.solve_levels(p.levels, None, 1e-12, True, CG_METHOD_REFINED)
+    print(f"OCEAN schedule {schedule:#x}: {sol.cycles} cycles (V(2,2): {base.cycles})")
+    assert sol.relative_residual < 1e-12
+    assert rel_l2(sol.values64, ref64) <= PARITY_L2
+    assert sol.cycles <= base.cycles
