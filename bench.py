@@ -25,9 +25,18 @@ import numpy as np  # noqa: E402
 METRIC = "solves/s at 1024×2048 to 1e-8 residual: learned-precond vs plain GPU MG"
 HBM_PEAK_GBS = 8000.0
 ROWS, COLS = 1024, 2048
-# Algorithmic bytes per cell of the finest grid's fused two-pass smoother launched from given
-# values (the cycle's up leg): x read 4 + forcing 4 + mask 1 + x write 4.
-SMOOTH_BYTES_PER_CELL = 13
+# Algorithmic bytes per launch at 1024 x 2048 (N = 2,097,152 cells), each array read or written once:
+#  - the binary64 outer step k_outer_residual: x64 read 8 + delta 4 + f 4 + mask 1, x64 written 8 +
+#    correction forcing 4 = 29 B/cell (the per-cell FP64 residual of the byte model, 25 B, plus the
+#    x += delta fused into it);
+#  - the finest up leg k_relax_tile<4,8,false,2>: x 4 + f 4 + mask 1 + coarse x 1 read, x 4 written
+#    = 14 B/cell (the 13 B two-pass smoothing figure plus the fused prolongation's coarse values).
+CELLS = 1024 * 2048
+OUTER_BYTES = 29 * CELLS
+UP_BYTES = 14 * CELLS
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE and WRITE_SIZE (scaled by a 1 GiB copy),
+# for the default settings (tile rows 8, two passes per grid); see profiles/hbm_traffic_*.md.
+MEASURED_TRAFFIC = {"outer": 78181417, "up": 30654638}
 
 
 def log(msg: str) -> None:
@@ -113,12 +122,10 @@ def main() -> int:
     method = cg.CG_METHOD_FGMRES if args.method == "fgmres" else cg.CG_METHOD_REFINED
     plan = Plan(acc, prob.levels)
     lib = cg.lib()
-    import ctypes
 
     log(f"bench: warmup {args.warmup}")
     for _ in range(args.warmup):
         info = plan.solve(args.tol, method)
-    lib.cg_ocean_kernel_time_us(acc.ctx, -1, None)
     log(f"bench: timing {args.steps} solves")
     cycles = []
     t0 = time.perf_counter()
@@ -131,25 +138,17 @@ def main() -> int:
     x64, _ = plan.read()
     log(f"bench: {value:.1f} solves/s, {np.mean(cycles):.1f} cycles, residual {rel_res:.3e}")
 
-    # Kernel durations of the finest smoother, with HIP events on the library's stream, over a
-    # second pass of the same solves with graph replay off (events bracket single launches).
-    acc.set_option(cg.CG_OPT_TIME_KERNELS, 1)
-    lib.cg_ocean_kernel_time_us(acc.ctx, -1, None)
-    n_timed = max(5, min(args.steps, 20))
-    t1 = time.perf_counter()
-    for _ in range(n_timed):
-        plan.solve(args.tol, method)
-    eager = n_timed / (time.perf_counter() - t1)
-    cnt = ctypes.c_long(0)
-    us_up = lib.cg_ocean_kernel_time_us(acc.ctx, 1, ctypes.byref(cnt))
-    n_up = cnt.value
-    us_zero = lib.cg_ocean_kernel_time_us(acc.ctx, 0, ctypes.byref(cnt))
-    acc.set_option(cg.CG_OPT_TIME_KERNELS, 0)
-    cells = ROWS * COLS
-    traffic = SMOOTH_BYTES_PER_CELL * cells
-    achieved = traffic / (us_up * 1e-6) / 1e9 if us_up > 0 else None
-    log(f"bench: smoother from values {us_up:.2f} us ({n_up} launches), from zero {us_zero:.2f} us; "
-        f"eager {eager:.1f} solves/s")
+    # Kernel durations with HIP events on the library's stream: 200 back-to-back launches of each
+    # kernel on the plan's buffers as the timed solves left them, the average per launch.
+    reps = 200
+    us_outer = lib.cg_ocean_plan_time_kernel(plan.handle, 2, reps)
+    us_up = lib.cg_ocean_plan_time_kernel(plan.handle, 1, reps)
+    us_down = lib.cg_ocean_plan_time_kernel(plan.handle, 0, reps)
+    defaults = args.tile_rows == 8 and args.level_passes == "22222222"
+    outer_gbs = OUTER_BYTES / (us_outer * 1e-6) / 1e9 if us_outer > 0 else None
+    up_gbs = UP_BYTES / (us_up * 1e-6) / 1e9 if us_up > 0 else None
+    log(f"bench: outer residual {us_outer:.2f} us ({outer_gbs:.0f} GB/s algorithmic), finest up leg {us_up:.2f} us "
+        f"({up_gbs:.0f} GB/s), finest down leg {us_down:.2f} us")
 
     err = None
     cpu = None
@@ -189,19 +188,28 @@ def main() -> int:
             "cycles_per_solve": float(np.mean(cycles)),
             "relative_residual": rel_res,
             "rel_l2_vs_fp64_oracle": err,
-            "eager_solves_per_s": round(eager, 3),
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"k_relax_tile<4,{args.tile_rows},false> (finest grid, 2 fused red-black passes)",
-            "achieved": None if achieved is None else round(achieved, 1),
+            "kernel": "k_outer_residual (binary64 x += delta, residual, correction forcing, max), the largest "
+                      "HBM-bound kernel by trace time; the one-workgroup bottom of the cycle, equal in time, is "
+                      "latency-bound",
+            "achieved": None if outer_gbs is None else round(outer_gbs, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "avg_launch_us": round(us_up, 3),
-            "launches": n_up,
-            "algorithmic_bytes_per_launch": traffic,
+            "frac": None if outer_gbs is None else round(outer_gbs / HBM_PEAK_GBS, 4),
+            "traffic": MEASURED_TRAFFIC["outer"] if defaults else None,
+            "avg_launch_us": round(us_outer, 3),
+            "algorithmic_bytes_per_launch": OUTER_BYTES,
+            "smoother": {
+                "kernel": "k_relax_tile<4,8,false,2> (finest up leg: prolongation + 2 fused red-black passes)",
+                "achieved": None if up_gbs is None else round(up_gbs, 1),
+                "frac": None if up_gbs is None else round(up_gbs / HBM_PEAK_GBS, 4),
+                "avg_launch_us": round(us_up, 3),
+                "algorithmic_bytes_per_launch": UP_BYTES,
+                "traffic": MEASURED_TRAFFIC["up"] if defaults else None,
+                "down_leg_avg_launch_us": round(us_down, 3),
+            },
         },
         "cpu_baseline": cpu,
     }

@@ -57,6 +57,7 @@ SIGNATURES = {
     "cg_ocean_plan_solve_batch": (_int, [ctypes.POINTER(_vp), _int, _dbl, _int, ctypes.POINTER(CgSolution)]),
     "cg_ocean_plan_read": (_int, [_vp, _vp, _vp]),
     "cg_ocean_kernel_time_us": (_dbl, [_vp, _int, ctypes.POINTER(_long)]),
+    "cg_ocean_plan_time_kernel": (_dbl, [_vp, _int, _int]),
     "cg_bottom_drag": (_dbl, []),
     "cg_beta_rows": (_int, [_int, _dbl, _f64p]),
     "cg_circulation_stencil": (_int, [_int, _int, _dbl, _dbl, _u8p, _f64p, _dbl, _vp, _f32p, _f32p, _f32p,

@@ -1141,6 +1141,93 @@ __global__ __launch_bounds__(256) void k_combine(long cells, const float *__rest
     delta[cell] = (float)v;
 }
 
+// ------------------------------------------------------------------------------------------------
+// The Krylov rescue's binary64 vector kernels (solveByKrylov, OC:548-607; applyOperator OC:637-659).
+// ------------------------------------------------------------------------------------------------
+
+// applyOperator: into = ((e zE + w zW + n zN + s zS) - z) * centre on water, 0 on land.
+__global__ __launch_bounds__(256) void k_apply64(LevelArgs L, const double *__restrict__ z, double *__restrict__ into) {
+    const int across = L.across;
+    const int col = blockIdx.x * 256 + threadIdx.x;
+    if (col >= across) return;
+    const int ce = col + 1 == across ? 0 : col + 1;
+    const int cw = col == 0 ? across - 1 : col - 1;
+#pragma unroll
+    for (int k = 0; k < kRowsPerBlock; ++k) {
+        const int row = blockIdx.y * kRowsPerBlock + k;
+        if (row >= L.down) break;
+        const long cell = (long)row * across + col;
+        double out = 0.0;
+        if (L.water[cell]) {
+            const double zn = row > 0 ? z[cell - across] : 0.0;
+            const double zs = row + 1 < L.down ? z[cell + across] : 0.0;
+            const float4 wt = weights_at(L, row, cell);
+            const double nb = (double)wt.x * z[(long)row * across + ce] + (double)wt.y * z[(long)row * across + cw] +
+                              (double)wt.z * zn + (double)wt.w * zs;
+            out = (nb - z[cell]) * centre_at(L, row, cell);
+        }
+        into[cell] = out;
+    }
+}
+
+// balanceOf (OC:325-327) into a binary64 array: f * centre.
+__global__ void k_balance64(LevelArgs L, const float *__restrict__ f, double *__restrict__ out) {
+    const long cells = (long)L.across * L.down;
+    const long cell = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= cells) return;
+    out[cell] = (double)f[cell] * centre_at(L, (int)(cell / L.across), cell);
+}
+
+// r = water ? balance - r : 0 (OC:565).
+__global__ void k_rinit64(const uint8_t *__restrict__ water, const double *__restrict__ balance, double *r, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) r[i] = water[i] ? balance[i] - r[i] : 0.0;
+}
+
+// Binary64 dot partials, one per block (the host adds them in block order).
+__global__ __launch_bounds__(256) void k_dot64(const double *__restrict__ a, const double *__restrict__ b, long n,
+                                               double *__restrict__ partials) {
+    __shared__ double sh[16];
+    double acc = 0.0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        acc += a[i] * b[i];
+    const double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// Largest magnitude of a binary64 vector into red.
+__global__ __launch_bounds__(256) void k_maxabs64(const double *__restrict__ a, long n, MaxReduce red) {
+    unsigned long long m = 0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const unsigned long long b = abs_bits(a[i]);
+        m = b > m ? b : m;
+    }
+    grid_max(m, red);
+}
+
+// p = r + beta (p - omega v)  (OC:578).
+__global__ void k_bicg_p(double *p, const double *__restrict__ r, const double *__restrict__ v, double beta,
+                         double omega, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+}
+
+// y += a * x.
+__global__ void k_axpy64(double *y, const double *__restrict__ x, double a, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += a * x[i];
+}
+
+// x += omega shat; r -= omega t  (OC:595-598, one loop as the reference's).
+__global__ void k_bicg_xr(double *x, double *r, const double *__restrict__ shat, const double *__restrict__ t,
+                          double omega, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        x[i] += omega * shat[i];
+        r[i] -= omega * t[i];
+    }
+}
+
 // x64 <- (double) x32, and the reverse, elementwise.
 __global__ void k_f32_to_f64(const float *__restrict__ a, double *__restrict__ b, long n) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;

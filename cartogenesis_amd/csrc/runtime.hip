@@ -309,8 +309,6 @@ struct cg_plan {
     int nw = 0;
     int *d_idx = nullptr;
     float *d_Mt = nullptr;
-    // Krylov workspace (binary64, finest size), allocated on first use.
-    std::vector<double *> kv;
     // Graphs of one refinement step, for each parity of the binary64 ping-pong.
     hipGraphExec_t step_exec[2] = {nullptr, nullptr};
     // The fused cycle: tile levels 0..bottom-1 run fused down/up kernels, levels bottom..n-1 run
@@ -321,6 +319,11 @@ struct cg_plan {
     size_t bottom_lds = 0;
     // FGMRES workspace (allocated on first use): basis Q (kFgmresMax + 1 vectors) and
     // preconditioned vectors Z (kFgmresMax), each `vstride` floats (a grid with its zero row).
+    // Krylov rescue workspace (binary64 vectors, allocated on first use).
+    double *kx = nullptr, *kr = nullptr, *ksh = nullptr, *kp = nullptr, *kv = nullptr, *kph = nullptr, *kshat = nullptr,
+           *kt = nullptr, *kbal = nullptr, *kpart = nullptr;
+    float *kf32 = nullptr;
+    std::vector<double> h_part;
     float *fq = nullptr, *fz = nullptr, *fa = nullptr, *fdelta = nullptr;
     long vstride = 0;
     double *fpart = nullptr;
@@ -341,6 +344,8 @@ void free_plan(cg_plan *p) {
     }
     dfree(p->f0);
     dfree(p->r64);
+    for (double *v : {p->kx, p->kr, p->ksh, p->kp, p->kv, p->kph, p->kshat, p->kt, p->kbal, p->kpart}) dfree(v);
+    dfree(p->kf32);
     dfree(p->fq);
     dfree(p->fz);
     dfree(p->fa);
@@ -353,7 +358,6 @@ void free_plan(cg_plan *p) {
     dfree(p->d_passes);
     dfree(p->d_idx);
     dfree(p->d_Mt);
-    for (double *v : p->kv) dfree(v);
     for (int i = 0; i < 2; ++i)
         if (p->step_exec[i]) (void)hipGraphExecDestroy(p->step_exec[i]);
     delete p;
@@ -634,6 +638,149 @@ int ref_solve(cg_plan *p, double tol, cg_solution *info, const volatile int *can
     info->cycles = cycles;
     info->relative_residual = res;
     p->last_answer = 0;
+    return CG_OK;
+}
+
+// ---- the Krylov rescue (solveWithKrylovRescue / solveByKrylov, OC:507-607) --------------------
+
+constexpr int kDotBlocks = 1024;
+
+// dot (OC:661-665) on the device: per-block partial sums added on the host in block order.
+int krylov_dot(cg_plan *p, const double *a, const double *b, double *out) {
+    cg_ctx *ctx = p->ctx;
+    const long n = p->lv[0].st->cells;
+    hipLaunchKernelGGL(k_dot64, dim3(kDotBlocks), dim3(256), 0, ctx->stream, a, b, n, p->kpart);
+    HIPCHK(ctx, hipMemcpyAsync(p->h_part.data(), p->kpart, sizeof(double) * kDotBlocks, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    double sum = 0.0;
+    for (int i = 0; i < kDotBlocks; ++i) sum += p->h_part[i];
+    *out = sum;
+    return CG_OK;
+}
+
+int krylov_largest(cg_plan *p, const double *a, double *out) {
+    cg_ctx *ctx = p->ctx;
+    const MaxReduce red = max_into(p, 3);
+    hipLaunchKernelGGL(k_maxabs64, dim3(kDotBlocks), dim3(256), 0, ctx->stream, a, p->lv[0].st->cells, red);
+    unsigned long long b;
+    int rc = read_u64(p, 3, &b);
+    if (rc) return rc;
+    *out = bits_to_double(b);
+    return CG_OK;
+}
+
+// precondition (OC:610-622): one reference V-cycle from zero on L z = balance, the coarsest relaxed
+// a fixed 200 passes, answer widened to binary64.
+int krylov_precondition(cg_plan *p, const double *balance, double *out, double tol) {
+    cg_ctx *ctx = p->ctx;
+    hipStream_t s = ctx->stream;
+    PlanLevel &l0 = p->lv[0];
+    const long n = l0.st->cells;
+    hipLaunchKernelGGL(k_with_balance, dim3(blocks_for(n, 256)), dim3(256), 0, s, l0.st->args(), balance, p->kf32);
+    HIPCHK(ctx, hipMemsetAsync(l0.x[l0.cur], 0, sizeof(float) * n, s));
+    int rc = ref_vcycle(p, p->kf32, tol, kPrecondCoarsestPasses);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_f32_to_f64, dim3(blocks_for(n, 256)), dim3(256), 0, s, l0.x[l0.cur], out, n);
+    HIPCHK(ctx, hipGetLastError());
+    return CG_OK;
+}
+
+// solveByKrylov from the binary32 values in lv[0] (in and out), binary64 throughout.
+int krylov_solve(cg_plan *p, double tol, int most, int *iterations, double *relative) {
+    cg_ctx *ctx = p->ctx;
+    hipStream_t s = ctx->stream;
+    PlanLevel &l0 = p->lv[0];
+    const Stencil *st = l0.st;
+    const long n = st->cells;
+    const unsigned nb = blocks_for(n, 256);
+    int rc;
+    if (!p->kx) {
+        for (double **v : {&p->kx, &p->kr, &p->ksh, &p->kp, &p->kv, &p->kph, &p->kshat, &p->kt, &p->kbal})
+            if ((rc = dalloc(ctx, v, n))) return rc;
+        if ((rc = dalloc(ctx, &p->kpart, kDotBlocks))) return rc;
+        if ((rc = dalloc_grid(ctx, &p->kf32, st->across, st->down))) return rc;
+        p->h_part.assign(kDotBlocks, 0.0);
+    }
+    double largest;
+    if ((rc = plan_largest(p, &largest))) return rc;
+    hipLaunchKernelGGL(k_f32_to_f64, dim3(nb), dim3(256), 0, s, l0.x[l0.cur], p->kx, n);
+    hipLaunchKernelGGL(k_balance64, dim3(nb), dim3(256), 0, s, st->args(), p->f0, p->kbal);
+    hipLaunchKernelGGL(k_apply64, grid2d(st), dim3(256), 0, s, st->args(), p->kx, p->kr);
+    hipLaunchKernelGGL(k_rinit64, dim3(nb), dim3(256), 0, s, st->water, p->kbal, p->kr, n);
+    double r_max;
+    if ((rc = krylov_largest(p, p->kr, &r_max))) return rc;
+    double res = largest > 0.0 ? r_max / largest : 0.0;
+    HIPCHK(ctx, hipMemcpyAsync(p->ksh, p->kr, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    HIPCHK(ctx, hipMemsetAsync(p->kp, 0, sizeof(double) * n, s));
+    HIPCHK(ctx, hipMemsetAsync(p->kv, 0, sizeof(double) * n, s));
+    double rho = 1.0, alpha = 1.0, omega = 1.0;
+    int it = 0;
+    while (res >= tol && it < most) {
+        double rho_next;
+        if ((rc = krylov_dot(p, p->ksh, p->kr, &rho_next))) return rc;
+        if (rho_next == 0.0 || omega == 0.0) break;
+        const double beta = (rho_next / rho) * (alpha / omega);
+        hipLaunchKernelGGL(k_bicg_p, dim3(nb), dim3(256), 0, s, p->kp, p->kr, p->kv, beta, omega, n);
+        if ((rc = krylov_precondition(p, p->kp, p->kph, tol))) return rc;
+        hipLaunchKernelGGL(k_apply64, grid2d(st), dim3(256), 0, s, st->args(), p->kph, p->kv);
+        double sv;
+        if ((rc = krylov_dot(p, p->ksh, p->kv, &sv))) return rc;
+        if (sv == 0.0) break;
+        alpha = rho_next / sv;
+        hipLaunchKernelGGL(k_axpy64, dim3(nb), dim3(256), 0, s, p->kr, p->kv, -alpha, n);
+        hipLaunchKernelGGL(k_axpy64, dim3(nb), dim3(256), 0, s, p->kx, p->kph, alpha, n);
+        rho = rho_next;
+        it++;
+        if ((rc = krylov_largest(p, p->kr, &r_max))) return rc;
+        res = r_max / largest;
+        if (res < tol) break;
+        if ((rc = krylov_precondition(p, p->kr, p->kshat, tol))) return rc;
+        hipLaunchKernelGGL(k_apply64, grid2d(st), dim3(256), 0, s, st->args(), p->kshat, p->kt);
+        double tt, tr;
+        if ((rc = krylov_dot(p, p->kt, p->kt, &tt))) return rc;
+        if (tt == 0.0) omega = 0.0;
+        else {
+            if ((rc = krylov_dot(p, p->kt, p->kr, &tr))) return rc;
+            omega = tr / tt;
+        }
+        hipLaunchKernelGGL(k_bicg_xr, dim3(nb), dim3(256), 0, s, p->kx, p->kr, p->kshat, p->kt, omega, n);
+        if ((rc = krylov_largest(p, p->kr, &r_max))) return rc;
+        res = r_max / largest;
+    }
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(nb), dim3(256), 0, s, p->kx, l0.x[l0.cur], n);
+    HIPCHK(ctx, hipGetLastError());
+    *iterations = it;
+    *relative = res;
+    return CG_OK;
+}
+
+// solveWithKrylovRescue (OC:507-528) from the binary32 values in lv[0].
+int rescue_solve(cg_plan *p, double tol, cg_solution *info, const volatile int *cancel) {
+    int rc = ref_solve(p, tol, info, cancel);
+    if (rc || info->status == CG_CANCELLED) return rc;
+    if (info->relative_residual < tol || !std::isfinite(info->relative_residual)) return CG_OK;
+    // Values not all finite also return as the cycles left them (OC:516).
+    const long n = p->lv[0].st->cells;
+    std::vector<float> v(n);
+    HIPCHK(p->ctx, hipMemcpy(v.data(), p->lv[0].x[p->lv[0].cur], sizeof(float) * n, hipMemcpyDeviceToHost));
+    for (float a : v)
+        if (!std::isfinite(a)) return CG_OK;
+    int its = 0, total = 0, restarts = 0;
+    double rr = info->relative_residual;
+    if ((rc = krylov_solve(p, tol, kMostCycles, &its, &rr))) return rc;
+    total = its;
+    while (rr >= tol && restarts < kKrylovRestarts) {
+        HIPCHK(p->ctx, hipMemcpy(v.data(), p->lv[0].x[p->lv[0].cur], sizeof(float) * n, hipMemcpyDeviceToHost));
+        bool finite = true;
+        for (float a : v) finite = finite && std::isfinite(a);
+        if (!finite) break;
+        if ((rc = krylov_solve(p, tol, kMostCycles, &its, &rr))) return rc;
+        total += its;
+        restarts++;
+    }
+    info->cycles += total;
+    info->relative_residual = rr;
     return CG_OK;
 }
 
@@ -1369,13 +1516,8 @@ static int plan_solve_locked(cg_plan *p, double tol, int method, int warm, cg_so
     if (method == CG_METHOD_REFERENCE || method == CG_METHOD_RESCUE) {
         PlanLevel &l0 = p->lv[0];
         if (!warm) HIPCHK(ctx, hipMemsetAsync(l0.x[l0.cur], 0, sizeof(float) * l0.st->cells, ctx->stream));
-        int rc = ref_solve(p, tol, info, cancel);
-        if (rc) return rc;
-        if (method == CG_METHOD_RESCUE && info->relative_residual >= tol && std::isfinite(info->relative_residual)) {
-            set_err(ctx, "the Krylov rescue is not built on the device yet");
-            return CG_ERR_STATE;
-        }
-        return CG_OK;
+        if (method == CG_METHOD_RESCUE) return rescue_solve(p, tol, info, cancel);
+        return ref_solve(p, tol, info, cancel);
     }
     if (method == CG_METHOD_REFINED) return refined_solve(p, tol, warm != 0, info, cancel);
     if (method == CG_METHOD_FGMRES) return fgmres_solve(p, tol, warm != 0, info, cancel);
@@ -1453,6 +1595,56 @@ int cg_ocean_solve(cg_ctx *ctx, const cg_stencil *levels, int nlevels, const flo
     if (!rc) rc = cg_ocean_plan_read(p, out64, out);
     cg_ocean_plan_destroy(p);
     return rc;
+}
+
+double cg_ocean_plan_time_kernel(cg_plan *p, int kind, int reps) {
+    if (!p || reps < 1 || kind < 0 || kind > 2) return -1.0;
+    cg_ctx *ctx = p->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (hipSetDevice(ctx->device) != hipSuccess) return -1.0;
+    if (kind < 2 && (!p->fusable || p->bottom == 0)) {
+        set_err(ctx, "plan has no fused finest tile level to time");
+        return -1.0;
+    }
+    PlanLevel &L = p->lv[0];
+    const int nh = level_half_sweeps(ctx, 0), th = level_tile_rows(ctx, L.st);
+    auto launch = [&]() -> hipError_t {
+        if (kind == 2) {
+            MaxReduce red;
+            red.out = p->d_u64 + 3 * kMaxSlots;
+            red.clear = nullptr;
+            const Stencil *st = L.st;
+            const dim3 grid(blocks_for(st->across / 2, 256), blocks_for(st->down, kRowsPerBlock));
+            hipLaunchKernelGGL(k_outer_residual, grid, dim3(256), 0, ctx->stream, st->args(), p->f0, p->x64[p->x64cur],
+                               L.x[L.cur], p->x64[1 - p->x64cur], L.f, red);
+            return hipGetLastError();
+        }
+        TileExtra ex{};
+        PlanLevel &C = p->lv[1];
+        if (kind == 0) {
+            ex.cwater = C.st->water;
+            ex.g = L.g;
+            ex.fc = C.f;
+            return tile_dispatch(ctx, L.st->args(), L.f, L.x[L.cur], L.x[1 - L.cur], nh, true, TILE_DOWN, ex, th);
+        }
+        ex.xc = C.x[C.cur];
+        ex.ca = C.st->across;
+        ex.cd = C.st->down;
+        ex.beyond = p->pole_is_wall ? -1.0f : 1.0f;
+        return tile_dispatch(ctx, L.st->args(), L.f, L.x[L.cur], L.x[1 - L.cur], nh, false, TILE_UP, ex, th);
+    };
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return -1.0;
+    if (launch() != hipSuccess) return -1.0;  // warm
+    (void)hipEventRecord(a, ctx->stream);
+    for (int i = 0; i < reps; ++i)
+        if (launch() != hipSuccess) return -1.0;
+    (void)hipEventRecord(b, ctx->stream);
+    float ms = 0.0f;
+    const bool ok = hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ok ? 1000.0 * ms / reps : -1.0;
 }
 
 double cg_ocean_kernel_time_us(cg_ctx *ctx, int kind, long *launches) {

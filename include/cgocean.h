@@ -47,7 +47,8 @@ extern "C" {
 /* Binary32 V(2,2) cycles on the correction, wrapped in binary64 iterative refinement; the
  * outer residual and the accumulated answer are binary64; the coarsest grid is solved directly. */
 #define CG_METHOD_REFINED 2
-/* Flexible GMRES(m) in binary64, right-preconditioned by one binary32 V(2,2) cycle. */
+/* Binary64 iterative refinement whose inner solve is flexible GMRES(m) over a binary32 basis,
+ * right-preconditioned by one binary32 V(2,2) cycle per iteration. */
 #define CG_METHOD_FGMRES 3
 
 /* ---- options (cg_ocean_set_option) ------------------------------------------------------------ */
@@ -139,6 +140,13 @@ int cg_ocean_plan_read(cg_plan *plan, double *out64, float *out32);
  * resets the record. *launches receives the count averaged. */
 #define CG_OPT_TIME_KERNELS 6
 double cg_ocean_kernel_time_us(cg_ctx *ctx, int kind, long *launches);
+
+/* Times `reps` back-to-back launches of one of the plan's kernels between two HIP events on the
+ * context's stream and returns the average in microseconds (-1 on failure): kind 0 the finest
+ * grid's fused down leg, 1 its fused up leg, 2 the binary64 outer residual. Each launch reads the
+ * plan's current buffers and writes the same outputs, so repeats leave the plan as one launch
+ * does; call it between solves, not during one. */
+double cg_ocean_plan_time_kernel(cg_plan *plan, int kind, int reps);
 
 /* ---- host-side problem builders (no device needed) --------------------------------------------- */
 

@@ -197,3 +197,24 @@ def test_level_pass_schedules_reach_the_oracle(acc, schedule):
     assert sol.relative_residual < 1e-12
     assert rel_l2(sol.values64, ref64) <= PARITY_L2
     assert sol.cycles <= base.cycles
+
+
+@pytest.mark.parametrize("rows,tol", [(64, 1e-6), (32, 1e-7)])
+def test_krylov_rescue_matches_the_processor(acc, rows, tol):
+    """solveWithKrylovRescue (OceanCirculation.kt:507-528): where binary32 cycles stall on their
+    rounding floor (200 cycles), BiCGSTAB preconditioned by the reference cycle takes over. The
+    cycles are bit-identical; BiCGSTAB's dot products are summed in another order (block partials),
+    so its values agree to binary32 rounding, with the same iteration count."""
+    from cartogenesis_amd import CG_METHOD_RESCUE
+    p = circulation_problem(rows, seed=0)
+    sol = acc.solve_levels(p.levels, None, tol, True, CG_METHOD_RESCUE)
+    x, cycles, rr = oracle.solve([to_oracle(s) for s in p.levels], np.zeros(p.levels[0].cells, np.float32), tol,
+                                 mode=oracle.RESCUE)
+    print(f"OCEAN rescue {rows}x{2 * rows} tol {tol}: device {sol.cycles} / {sol.relative_residual}, "
+          f"processor {cycles} / {rr}")
+    assert cycles > 200, "the case must exercise the rescue"
+    assert sol.cycles == cycles
+    assert sol.relative_residual < tol and rr < tol
+    assert sol.relative_residual == pytest.approx(rr, rel=1e-3)
+    assert np.allclose(sol.values, x, rtol=1e-5, atol=1e-6 * np.abs(x).max())
+    OC.require_solved("the circulation", sol, tol)
