@@ -58,6 +58,10 @@ SIGNATURES = {
     "cg_ocean_plan_read": (_int, [_vp, _vp, _vp]),
     "cg_ocean_kernel_time_us": (_dbl, [_vp, _int, ctypes.POINTER(_long)]),
     "cg_ocean_plan_time_kernel": (_dbl, [_vp, _int, _int]),
+    "cg_ocean_water_on": (_int, [_vp, _u8p, _int, _int, _int, _int, _u8p]),
+    "cg_ocean_curl_forcing": (_int, [_vp, _int, _int, _dbl, _dbl, _f64p, _f64p, _vp, _f64p]),
+    "cg_ocean_plan_set_stress": (_int, [_vp, _dbl, _dbl, _f64p, _f64p, _vp]),
+    "cg_ocean_plan_velocities": (_int, [_vp, _dbl, _dbl, _vp, _vp]),
     "cg_bottom_drag": (_dbl, []),
     "cg_beta_rows": (_int, [_int, _dbl, _f64p]),
     "cg_circulation_stencil": (_int, [_int, _int, _dbl, _dbl, _u8p, _f64p, _dbl, _vp, _f32p, _f32p, _f32p,
@@ -65,6 +69,7 @@ SIGNATURES = {
     "cg_coarse_water": (_int, [_u8p, _int, _int, _int, _int, _int, _u8p]),
     "cg_level_shapes": (_int, [_int, _int, _dbl, _dbl, _int, _i32p, _i32p, _f64p, _f64p]),
     "cg_problem_generate": (_int, [_int, _int, _u64, _dbl, _int, _vp, _vp, _vp]),
+    "cg_problem_stress": (_int, [_int, _int, _u64, _int, _vp, _vp]),
 }
 
 _lib = None

@@ -167,6 +167,27 @@ int cg_level_shapes(int across, int down, double dx, double dy, int max_levels, 
     return count;
 }
 
+// The synthetic workload's zonal stress: tau_x = amplitude cos(3 (lat - shift)) + N(0, 0.005^2) per
+// cell (Box-Muller from two uniforms of the cell), lat the row's binary32 centre latitude.
+static void zonal_stress(int across, int down, uint64_t seed, int month, double *tau) {
+    double amplitude = 0.1, shift_deg = 0.0;
+    if (month >= 0) {
+        amplitude = 0.1 * std::cos(2.0 * M_PI * month / 12.0);
+        shift_deg = 10.0 * std::sin(2.0 * M_PI * month / 12.0);
+    }
+    for (int row = 0; row < down; row++) {
+        const float lat = 90.0f - 180.0f * ((float)row + 0.5f) / (float)down;
+        const double band = amplitude * std::cos(3.0 * ((double)lat - shift_deg) * M_PI / 180.0);
+        for (int col = 0; col < across; col++) {
+            const long i = (long)row * across + col;
+            const double u1 = 1.0 - uniform01(seed, 2, (uint64_t)i);
+            const double u2 = uniform01(seed, 3, (uint64_t)i);
+            const double g = std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+            tau[i] = band + 0.005 * g;
+        }
+    }
+}
+
 // The synthetic workload. The curl follows OceanStage.curlForcing (OS:689-708): central
 // differences, columns wrapping, edge rows one-sided, with per-cell depth H in rho H.
 int cg_problem_generate(int across, int down, uint64_t seed, double land_fraction, int month, uint8_t *water,
@@ -197,25 +218,8 @@ int cg_problem_generate(int across, int down, uint64_t seed, double land_fractio
     std::vector<double> h(cells, 0.0);
     for (long i = 0; i < cells; i++)
         if (wet[i]) h[i] = 200.0 + 5800.0 * uniform01(seed, 1, (uint64_t)i);
-    // Zonal stress: tau_x = amplitude cos(3 (lat - shift)) + N(0, 0.005^2) per cell.
-    double amplitude = 0.1, shift_deg = 0.0;
-    if (month >= 0) {
-        amplitude = 0.1 * std::cos(2.0 * M_PI * month / 12.0);
-        shift_deg = 10.0 * std::sin(2.0 * M_PI * month / 12.0);
-    }
     std::vector<double> tau(cells);
-    for (int row = 0; row < down; row++) {
-        const float lat = 90.0f - 180.0f * ((float)row + 0.5f) / (float)down;
-        const double band = amplitude * std::cos(3.0 * ((double)lat - shift_deg) * M_PI / 180.0);
-        for (int col = 0; col < across; col++) {
-            const long i = (long)row * across + col;
-            // Box-Muller from two uniforms of the cell.
-            const double u1 = 1.0 - uniform01(seed, 2, (uint64_t)i);
-            const double u2 = uniform01(seed, 3, (uint64_t)i);
-            const double g = std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
-            tau[i] = band + 0.005 * g;
-        }
-    }
+    zonal_stress(across, down, seed, month, tau.data());
     const double world_width_m = 40075.0 * 1000.0;
     const double height_m = world_width_m * 0.5 / down;
     if (balance) {
@@ -234,6 +238,14 @@ int cg_problem_generate(int across, int down, uint64_t seed, double land_fractio
     }
     if (water) std::memcpy(water, wet.data(), (size_t)cells);
     if (depth) std::memcpy(depth, h.data(), sizeof(double) * (size_t)cells);
+    return CG_OK;
+}
+
+int cg_problem_stress(int across, int down, uint64_t seed, int month, double *eastward, double *northward) {
+    if (across <= 0 || down <= 0 || month > 11) return CG_ERR_ARG;
+    const long cells = (long)across * down;
+    if (eastward) zonal_stress(across, down, seed, month, eastward);
+    if (northward) std::fill(northward, northward + cells, 0.0);
     return CG_OK;
 }
 

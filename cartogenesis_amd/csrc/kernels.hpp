@@ -1238,4 +1238,83 @@ __global__ void k_f64_to_f32(const double *__restrict__ a, float *__restrict__ b
     if (i < n) b[i] = (float)a[i];
 }
 
+// ---- the round's input and output steps ------------------------------------------------------
+
+// OceanStage.waterOn (OS:551-573): solve cell (row, col) is water when every map cell it overlaps by
+// more than 1e-6 of a map cell (TOUCHING_SHARE, OS:129) is water. One thread per solve cell, grid
+// (across / 256, down); the map is read straight from HBM (the overlap is a few cells).
+__global__ __launch_bounds__(256) void k_water_on(const uint8_t *__restrict__ land, int map_across, int map_down,
+                                                  int across, int down, uint8_t *__restrict__ water) {
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = blockIdx.y;
+    if (col >= across) return;
+    const double touching = 1e-6;
+    const double cols_per = (double)map_across / across;
+    const double rows_per = (double)map_down / down;
+    const int first_row = (int)floor(row * rows_per + touching);
+    const int last_row = min((int)ceil((row + 1) * rows_per - touching) - 1, map_down - 1);
+    const int first_col = (int)floor(col * cols_per + touching);
+    const int last_col = min((int)ceil((col + 1) * cols_per - touching) - 1, map_across - 1);
+    int any_land = 0;
+    for (int mr = first_row; mr <= last_row; ++mr)
+        for (int mc = first_col; mc <= last_col; ++mc) any_land |= land[(long)mr * map_across + mc];
+    water[(long)row * across + col] = any_land ? 0 : 1;
+}
+
+// OceanStage.curlForcing (OS:689-708): F = (d tau_n/dx - d tau_e/dy) / (rho H), binary64, central
+// differences, columns wrapping, edge rows one-sided, the row above the northern one. rho = 1025
+// (OS:64); H the 500 m wind-driven layer (OS:74) when depth is null, else depth per cell. F is
+// written when non-null; f, when non-null, receives withBalance (OC:222-231): (float)(F / c) on
+// water, 0 on land, c the stencil L's centre weight.
+__global__ __launch_bounds__(256) void k_curl_forcing(int across, int down, double width_m, double height_m,
+                                                      const double *__restrict__ east,
+                                                      const double *__restrict__ north,
+                                                      const double *__restrict__ depth, double *__restrict__ F,
+                                                      LevelArgs L, float *__restrict__ f) {
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = blockIdx.y;
+    if (col >= across) return;
+    const int rn = row > 0 ? row - 1 : 0;
+    const int rs = row + 1 < down ? row + 1 : down - 1;
+    const double ns_m = (double)(rs - rn) * height_m;
+    const int ce = col + 1 == across ? 0 : col + 1;
+    const int cw = col == 0 ? across - 1 : col - 1;
+    const long base = (long)row * across;
+    const long cell = base + col;
+    const double dtn_dx = (north[base + ce] - north[base + cw]) / (2.0 * width_m);
+    const double dte_dy = (east[(long)rn * across + col] - east[(long)rs * across + col]) / ns_m;
+    const double scale = depth ? 1.0 / (1025.0 * depth[cell]) : 1.0 / (1025.0 * 500.0);
+    const double v = (dtn_dx - dte_dy) * scale;
+    if (F) F[cell] = v;
+    if (f) f[cell] = L.water[cell] ? (float)(v / centre_at(L, row, cell)) : 0.0f;
+}
+
+// OceanCirculation.velocities (OC:801-828): eastward -(psi_N - psi_S) / 2dy and northward
+// (psi_E - psi_W) / 2dx at cell centres, the differences in binary32, the quotients in binary64 then
+// rounded; zero on land; beyond a pole psi is minus the edge row's. psi is s32, or s64 rounded to
+// binary32 per load when s32 is null.
+__global__ __launch_bounds__(256) void k_velocities(int across, int down, const uint8_t *__restrict__ water,
+                                                    const float *__restrict__ s32, const double *__restrict__ s64,
+                                                    double width_m, double height_m, float *__restrict__ east,
+                                                    float *__restrict__ north) {
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = blockIdx.y;
+    if (col >= across) return;
+    const long base = (long)row * across;
+    const long cell = base + col;
+    if (!water[cell]) {
+        east[cell] = 0.0f;
+        north[cell] = 0.0f;
+        return;
+    }
+    auto psi = [&](long i) -> float { return s32 ? s32[i] : (float)s64[i]; };
+    const int ce = col + 1 == across ? 0 : col + 1;
+    const int cw = col == 0 ? across - 1 : col - 1;
+    const float here = psi(cell);
+    const float sn = row > 0 ? psi(cell - across) : -here;
+    const float ss = row + 1 < down ? psi(cell + across) : -here;
+    east[cell] = (float)(-(double)(sn - ss) / (2.0 * height_m));
+    north[cell] = (float)((double)(psi(base + ce) - psi(base + cw)) / (2.0 * width_m));
+}
+
 }  // namespace cg

@@ -329,6 +329,10 @@ struct cg_plan {
     double *fpart = nullptr;
     FgmresState *fst = nullptr;
     int last_answer = 0;  // 0: reference (binary32 in lv[0]); 1: binary64 in x64[x64cur]
+    // The round's input and output steps (allocated on first use): the wind stress and depth on the
+    // finest grid, and the velocities of the answer.
+    double *tau_e = nullptr, *tau_n = nullptr, *depth = nullptr;
+    float *vel = nullptr;
 };
 
 namespace {
@@ -358,6 +362,10 @@ void free_plan(cg_plan *p) {
     dfree(p->d_passes);
     dfree(p->d_idx);
     dfree(p->d_Mt);
+    dfree(p->tau_e);
+    dfree(p->tau_n);
+    dfree(p->depth);
+    dfree(p->vel);
     for (int i = 0; i < 2; ++i)
         if (p->step_exec[i]) (void)hipGraphExecDestroy(p->step_exec[i]);
     delete p;
@@ -1537,6 +1545,117 @@ int cg_ocean_plan_solve_batch(cg_plan *const *plans, int count, double tol, int 
         int rc = cg_ocean_plan_solve(plans[i], tol, method, 0, infos ? &infos[i] : nullptr);
         if (rc) return rc;
     }
+    return CG_OK;
+}
+
+// ---- the round's input and output steps ---------------------------------------------------------
+
+int cg_ocean_water_on(cg_ctx *ctx, const uint8_t *map_land, int map_across, int map_down, int across, int down,
+                      uint8_t *water) {
+    if (!ctx || !map_land || !water || map_across <= 0 || map_down <= 0 || across <= 0 || down <= 0)
+        return CG_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    uint8_t *dland = nullptr, *dwater = nullptr;
+    const size_t nmap = (size_t)map_across * map_down, n = (size_t)across * down;
+    int rc = dalloc(ctx, &dland, nmap);
+    if (!rc) rc = dalloc(ctx, &dwater, n);
+    if (!rc && hipMemcpyAsync(dland, map_land, nmap, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) rc = CG_ERR_DEVICE;
+    if (!rc) {
+        hipLaunchKernelGGL(k_water_on, dim3(blocks_for(across, 256), down), dim3(256), 0, ctx->stream, dland, map_across,
+                           map_down, across, down, dwater);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(water, dwater, n, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess) {
+            set_err(ctx, "water_on on the device failed");
+            rc = CG_ERR_DEVICE;
+        }
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    dfree(dland);
+    dfree(dwater);
+    return rc;
+}
+
+int cg_ocean_curl_forcing(cg_ctx *ctx, int across, int down, double width_m, double height_m, const double *eastward,
+                          const double *northward, const double *depth, double *forcing) {
+    if (!ctx || !eastward || !northward || !forcing || across <= 0 || down <= 0 || !(width_m > 0.0) ||
+        !(height_m > 0.0))
+        return CG_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const size_t n = (size_t)across * down;
+    double *de = nullptr, *dn = nullptr, *dd = nullptr, *dF = nullptr;
+    int rc = dalloc(ctx, &de, n);
+    if (!rc) rc = dalloc(ctx, &dn, n);
+    if (!rc && depth) rc = dalloc(ctx, &dd, n);
+    if (!rc) rc = dalloc(ctx, &dF, n);
+    if (!rc) {
+        bool ok = hipMemcpyAsync(de, eastward, n * 8, hipMemcpyHostToDevice, ctx->stream) == hipSuccess &&
+                  hipMemcpyAsync(dn, northward, n * 8, hipMemcpyHostToDevice, ctx->stream) == hipSuccess &&
+                  (!depth || hipMemcpyAsync(dd, depth, n * 8, hipMemcpyHostToDevice, ctx->stream) == hipSuccess);
+        if (ok) {
+            hipLaunchKernelGGL(k_curl_forcing, dim3(blocks_for(across, 256), down), dim3(256), 0, ctx->stream, across,
+                               down, width_m, height_m, de, dn, dd, dF, LevelArgs{}, (float *)nullptr);
+            ok = hipGetLastError() == hipSuccess &&
+                 hipMemcpyAsync(forcing, dF, n * 8, hipMemcpyDeviceToHost, ctx->stream) == hipSuccess &&
+                 hipStreamSynchronize(ctx->stream) == hipSuccess;
+        }
+        if (!ok) {
+            set_err(ctx, "curl_forcing on the device failed");
+            rc = CG_ERR_DEVICE;
+        }
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    for (double *v : {de, dn, dd, dF}) dfree(v);
+    return rc;
+}
+
+int cg_ocean_plan_set_stress(cg_plan *p, double width_m, double height_m, const double *eastward,
+                             const double *northward, const double *depth) {
+    if (!p || !eastward || !northward || !(width_m > 0.0) || !(height_m > 0.0)) return CG_ERR_ARG;
+    cg_ctx *ctx = p->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const Stencil *st = p->lv[0].st;
+    if (!st->has_centre) {
+        set_err(ctx, "the finest stencil was uploaded without centre weights");
+        return CG_ERR_STATE;
+    }
+    const size_t n = st->cells;
+    int rc;
+    if (!p->tau_e && ((rc = dalloc(ctx, &p->tau_e, n)) || (rc = dalloc(ctx, &p->tau_n, n)))) return rc;
+    if (depth && !p->depth && (rc = dalloc(ctx, &p->depth, n))) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(p->tau_e, eastward, n * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(p->tau_n, northward, n * 8, hipMemcpyHostToDevice, ctx->stream));
+    if (depth) HIPCHK(ctx, hipMemcpyAsync(p->depth, depth, n * 8, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_curl_forcing, dim3(blocks_for(st->across, 256), st->down), dim3(256), 0, ctx->stream,
+                       st->across, st->down, width_m, height_m, p->tau_e, p->tau_n, depth ? p->depth : nullptr,
+                       (double *)nullptr, st->args(), p->f0);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    p->largest = -1.0;
+    p->x64_valid = false;
+    return CG_OK;
+}
+
+int cg_ocean_plan_velocities(cg_plan *p, double width_m, double height_m, float *eastward, float *northward) {
+    if (!p || !(width_m > 0.0) || !(height_m > 0.0)) return CG_ERR_ARG;
+    cg_ctx *ctx = p->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const Stencil *st = p->lv[0].st;
+    const size_t n = st->cells;
+    int rc;
+    if (!p->vel && (rc = dalloc(ctx, &p->vel, 2 * n))) return rc;
+    const float *s32 = p->last_answer == 0 ? p->lv[0].x[p->lv[0].cur] : nullptr;
+    const double *s64 = p->last_answer == 0 ? nullptr : p->x64[p->x64cur];
+    hipLaunchKernelGGL(k_velocities, dim3(blocks_for(st->across, 256), st->down), dim3(256), 0, ctx->stream,
+                       st->across, st->down, st->water, s32, s64, width_m, height_m, p->vel, p->vel + n);
+    HIPCHK(ctx, hipGetLastError());
+    if (eastward) HIPCHK(ctx, hipMemcpyAsync(eastward, p->vel, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (northward) HIPCHK(ctx, hipMemcpyAsync(northward, p->vel + n, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return CG_OK;
 }
 

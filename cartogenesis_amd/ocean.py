@@ -276,6 +276,30 @@ class HipOcean(OceanAccelerator):
         return Solution(out, info.cycles, info.relative_residual, out64, info.refinements)
 
 
+    def water_on(self, map_land, map_across: int, map_down: int, across: int, down: int) -> np.ndarray:
+        """OceanStage.waterOn (OceanStage.kt:551-573) on the device: the solve grid's water mask from
+        a map's land mask (non-zero for land)."""
+        out = np.empty(across * down, np.uint8)
+        land = np.ascontiguousarray(map_land, dtype=np.uint8).reshape(map_across * map_down)
+        check(lib().cg_ocean_water_on(self._ctx, land, map_across, map_down, across, down, out),
+              self._ctx, "cg_ocean_water_on")
+        return out
+
+    def curl_forcing(self, across: int, down: int, width_m: float, height_m: float, eastward, northward,
+                     depth=None) -> np.ndarray:
+        """OceanStage.curlForcing (OceanStage.kt:689-708) on the device; depth None takes the
+        reference's 500 m wind-driven layer."""
+        n = across * down
+        out = np.empty(n, np.float64)
+        d = None if depth is None else np.ascontiguousarray(depth, dtype=np.float64).reshape(n)
+        check(lib().cg_ocean_curl_forcing(self._ctx, across, down, float(width_m), float(height_m),
+                                          np.ascontiguousarray(eastward, dtype=np.float64).reshape(n),
+                                          np.ascontiguousarray(northward, dtype=np.float64).reshape(n),
+                                          None if d is None else d.ctypes.data_as(ctypes.c_void_p), out),
+              self._ctx, "cg_ocean_curl_forcing")
+        return out
+
+
 class Plan:
     """A device-resident solve (cg_ocean_plan_*): ladder, buffers, forcing and answer stay in HBM."""
 
@@ -301,6 +325,28 @@ class Plan:
         check(lib().cg_ocean_plan_solve(self._p, float(tol), int(method), int(warm), ctypes.byref(info)),
               self.acc.ctx, "cg_ocean_plan_solve")
         return info
+
+    def set_stress(self, width_m: float, height_m: float, eastward, northward, depth=None) -> None:
+        """The finest forcing from a wind stress, on the device: curlForcing then withBalance
+        (OceanStage.kt:689-708, OceanCirculation.kt:222-231)."""
+        n = self.levels[0].cells
+        d = None if depth is None else np.ascontiguousarray(depth, dtype=np.float64).reshape(n)
+        check(lib().cg_ocean_plan_set_stress(self._p, float(width_m), float(height_m),
+                                             np.ascontiguousarray(eastward, dtype=np.float64).reshape(n),
+                                             np.ascontiguousarray(northward, dtype=np.float64).reshape(n),
+                                             None if d is None else d.ctypes.data_as(ctypes.c_void_p)),
+              self.acc.ctx, "cg_ocean_plan_set_stress")
+
+    def velocities(self, width_m: float, height_m: float):
+        """OceanCirculation.velocities (OceanCirculation.kt:801-828) of the last answer, on the device:
+        (eastward, northward) in m/s, binary32."""
+        n = self.levels[0].cells
+        e = np.empty(n, np.float32)
+        nn = np.empty(n, np.float32)
+        check(lib().cg_ocean_plan_velocities(self._p, float(width_m), float(height_m),
+                                             e.ctypes.data_as(ctypes.c_void_p), nn.ctypes.data_as(ctypes.c_void_p)),
+              self.acc.ctx, "cg_ocean_plan_velocities")
+        return e, nn
 
     def read(self):
         n = self.levels[0].cells

@@ -41,6 +41,17 @@ def generate(across: int, down: int, seed: int, land_fraction: float = 0.3, mont
     return water, depth, balance
 
 
+def stress(across: int, down: int, seed: int, month: int = -1):
+    """(eastward, northward) wind stress of the synthetic world, N/m^2: generate's balance is its curl
+    over rho H."""
+    n = across * down
+    e = np.empty(n, np.float64)
+    nn = np.empty(n, np.float64)
+    check(lib().cg_problem_stress(across, down, int(seed), int(month), e.ctypes.data_as(ctypes.c_void_p),
+                                  nn.ctypes.data_as(ctypes.c_void_p)), what="cg_problem_stress")
+    return e, nn
+
+
 @dataclass
 class Problem:
     rows: int

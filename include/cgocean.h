@@ -148,6 +148,30 @@ double cg_ocean_kernel_time_us(cg_ctx *ctx, int kind, long *launches);
  * does; call it between solves, not during one. */
 double cg_ocean_plan_time_kernel(cg_plan *plan, int kind, int reps);
 
+/* ---- the round's input and output steps, on the device ------------------------------------- */
+
+/* OceanStage.waterOn (OS:551-573): the solve grid's water mask (across x down) from a map's land mask
+ * (map_land, map_across x map_down, non-zero for land): a solve cell is water when every map cell it
+ * overlaps by more than 1e-6 of a map cell is water. Host buffers in and out. */
+int cg_ocean_water_on(cg_ctx *ctx, const uint8_t *map_land, int map_across, int map_down, int across,
+                      int down, uint8_t *water);
+/* OceanStage.curlForcing (OS:689-708): forcing = curl_z(tau) / (rho H) per cell in binary64, tau
+ * given eastward and northward in N/m^2, central differences over cells of width_m x height_m,
+ * columns wrapping, edge rows one-sided. depth NULL takes the reference's 500 m wind-driven layer
+ * (OS:74); otherwise H = depth[cell]. Host buffers in and out. */
+int cg_ocean_curl_forcing(cg_ctx *ctx, int across, int down, double width_m, double height_m,
+                          const double *eastward, const double *northward, const double *depth,
+                          double *forcing);
+/* The same curl made the plan's finest forcing on the device, f = (float)(F / c) on water and 0 on
+ * land (withBalance, OC:222-231): the circulation half of OceanStage.circulateUnder (OS:427-432)
+ * with only the stress crossing the bus. */
+int cg_ocean_plan_set_stress(cg_plan *plan, double width_m, double height_m, const double *eastward,
+                             const double *northward, const double *depth);
+/* OceanCirculation.velocities (OC:801-828) of the plan's last answer rounded to binary32, computed
+ * on the device; either output may be NULL. */
+int cg_ocean_plan_velocities(cg_plan *plan, double width_m, double height_m, float *eastward,
+                             float *northward);
+
 /* ---- host-side problem builders (no device needed) --------------------------------------------- */
 
 /* BOTTOM_DRAG_PER_S (OS:102-103) with Earth's rotation rate as the reference's Float (WGC:302). */
@@ -178,6 +202,10 @@ int cg_level_shapes(int across, int down, double dx, double dy, int max_levels, 
  * the annual case, amplitude 0.1 and no shift. Any of the outputs may be NULL. */
 int cg_problem_generate(int across, int down, uint64_t seed, double land_fraction, int month,
                         uint8_t *water, double *depth, double *balance);
+/* The synthetic workload's wind stress that cg_problem_generate's balance is the curl of: eastward
+ * tau_x per cell as described there, northward zero. Either output may be NULL. */
+int cg_problem_stress(int across, int down, uint64_t seed, int month, double *eastward,
+                      double *northward);
 
 #ifdef __cplusplus
 }

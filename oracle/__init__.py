@@ -66,6 +66,8 @@ def lib():
                              _c_double, _c_int, _c_int, _c_int, ctypes.c_void_p, ctypes.POINTER(_c_double)]
     L.or_solve64.restype = _c_int
     L.or_velocities.argtypes = [_c_int, _c_int, _u8p, _f32p, _c_double, _c_double, _f32p, _f32p]
+    L.or_water_on.argtypes = [_u8p, _c_int, _c_int, _c_int, _c_int, _u8p]
+    L.or_curl_forcing.argtypes = [_c_int, _c_int, _c_double, _c_double, _f64p, _f64p, ctypes.c_void_p, _f64p]
     _lib = L
     return L
 
@@ -268,3 +270,23 @@ def velocities(st: Stencil, stream, dx, dy):
     n = np.empty(st.cells, np.float32)
     lib().or_velocities(st.across, st.down, st.water, np.ascontiguousarray(stream, dtype=np.float32), dx, dy, e, n)
     return e, n
+
+
+def water_on(map_land, map_across, map_down, across, down):
+    """OceanStage.waterOn (OS:551-573): the solve grid's mask from a map's land mask."""
+    out = np.empty(across * down, np.uint8)
+    land = np.ascontiguousarray(map_land, dtype=np.uint8).reshape(map_across * map_down)
+    lib().or_water_on(land, map_across, map_down, across, down, out)
+    return out
+
+
+def curl_forcing(across, down, width_m, height_m, eastward, northward, depth=None):
+    """OceanStage.curlForcing (OS:689-708); depth None takes the 500 m wind-driven layer."""
+    n = across * down
+    e = np.ascontiguousarray(eastward, dtype=np.float64).reshape(n)
+    nn = np.ascontiguousarray(northward, dtype=np.float64).reshape(n)
+    d = None if depth is None else np.ascontiguousarray(depth, dtype=np.float64).reshape(n)
+    out = np.empty(n)
+    lib().or_curl_forcing(across, down, float(width_m), float(height_m), e, nn,
+                          None if d is None else d.ctypes.data_as(ctypes.c_void_p), out)
+    return out

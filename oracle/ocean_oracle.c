@@ -749,3 +749,53 @@ void or_velocities(int across, int down, const uint8_t *water, const float *stre
         }
     }
 }
+
+/* OceanStage.waterOn (OS:551-573): a solve cell is water when every map cell it overlaps is water.
+ * The overlap of solve row `row` is map rows floor(row * rpc + 1e-6) .. ceil((row + 1) * rpc - 1e-6) - 1,
+ * capped at the last map row (TOUCHING_SHARE, OS:129), likewise for columns; no wrap. */
+void or_water_on(const uint8_t *map_land, int map_across, int map_down, int across, int down, uint8_t *water)
+{
+    const double touching = 1e-6;
+    const double cols_per = (double)map_across / across;
+    const double rows_per = (double)map_down / down;
+    for (int row = 0; row < down; row++) {
+        int first_row = (int)floor(row * rows_per + touching);
+        int last_row = (int)ceil((row + 1) * rows_per - touching) - 1;
+        if (last_row > map_down - 1) last_row = map_down - 1;
+        for (int col = 0; col < across; col++) {
+            int first_col = (int)floor(col * cols_per + touching);
+            int last_col = (int)ceil((col + 1) * cols_per - touching) - 1;
+            if (last_col > map_across - 1) last_col = map_across - 1;
+            int all_water = 1;
+            for (int mr = first_row; mr <= last_row; mr++)
+                for (int mc = first_col; mc <= last_col; mc++)
+                    if (map_land[(long)mr * map_across + mc]) all_water = 0;
+            water[(long)row * across + col] = (uint8_t)all_water;
+        }
+    }
+}
+
+/* OceanStage.curlForcing (OS:689-708): F = (d tau_n/dx - d tau_e/dy) / (rho H) per cell, binary64,
+ * central differences in meters, columns wrapping, the edge rows one-sided, y running north (the row
+ * above is the northern one). rho = 1025 kg/m^3 (OS:64); H the wind-driven layer's 500 m (OS:74)
+ * when depth is NULL, else depth[cell] (the per-cell reading of the synthetic workload). */
+void or_curl_forcing(int across, int down, double width_m, double height_m, const double *eastward,
+                     const double *northward, const double *depth, double *forcing)
+{
+    const double rho = 1025.0, layer = 500.0;
+    const double per_density_depth = 1.0 / (rho * layer);
+    for (int row = 0; row < down; row++) {
+        int rn = row - 1 < 0 ? 0 : row - 1;
+        int rs = row + 1 > down - 1 ? down - 1 : row + 1;
+        double ns_m = (rs - rn) * height_m;
+        for (int col = 0; col < across; col++) {
+            long cell = (long)row * across + col;
+            int ce = col + 1 == across ? 0 : col + 1;
+            int cw = col == 0 ? across - 1 : col - 1;
+            double dtn_dx = (northward[(long)row * across + ce] - northward[(long)row * across + cw]) / (2.0 * width_m);
+            double dte_dy = (eastward[(long)rn * across + col] - eastward[(long)rs * across + col]) / ns_m;
+            double scale = depth ? 1.0 / (rho * depth[cell]) : per_density_depth;
+            forcing[cell] = (dtn_dx - dte_dy) * scale;
+        }
+    }
+}
