@@ -18,7 +18,7 @@ LIB_PATH = PKG / "libcgocean.so"
 # Status codes and methods, as in include/cgocean.h.
 CG_OK, CG_DECLINED, CG_CANCELLED = 0, 1, 2
 CG_ERR_ARG, CG_ERR_DEVICE, CG_ERR_NOMEM, CG_ERR_STATE = -1, -2, -3, -4
-CG_METHOD_REFERENCE, CG_METHOD_RESCUE, CG_METHOD_REFINED, CG_METHOD_FGMRES = 0, 1, 2, 3
+CG_METHOD_REFERENCE, CG_METHOD_RESCUE, CG_METHOD_REFINED, CG_METHOD_FGMRES, CG_METHOD_KRYLOV = 0, 1, 2, 3, 4
 CG_OPT_MIN_CELLS, CG_OPT_TILE_ROWS, CG_OPT_USE_GRAPH = 1, 2, 3
 CG_OPT_FGMRES_RESTART, CG_OPT_STEPS_PER_CHECK, CG_OPT_TIME_KERNELS = 4, 5, 6
 CG_OPT_FUSED, CG_OPT_LEVEL_PASSES = 7, 8
@@ -55,6 +55,9 @@ SIGNATURES = {
     "cg_ocean_plan_set_forcing": (_int, [_vp, _f32p]),
     "cg_ocean_plan_solve": (_int, [_vp, _dbl, _int, _int, ctypes.POINTER(CgSolution)]),
     "cg_ocean_plan_solve_batch": (_int, [ctypes.POINTER(_vp), _int, _dbl, _int, ctypes.POINTER(CgSolution)]),
+    "cg_ocean_plan_set_bodies": (_int, [_vp, _vp, _int]),
+    "cg_ocean_plan_correct_bodies": (_int, [_vp, _f32p, _f32p]),
+    "cg_ocean_plan_set_start": (_int, [_vp, _f32p]),
     "cg_ocean_plan_read": (_int, [_vp, _vp, _vp]),
     "cg_ocean_kernel_time_us": (_dbl, [_vp, _int, ctypes.POINTER(_long)]),
     "cg_ocean_plan_time_kernel": (_dbl, [_vp, _int, _int]),
@@ -68,6 +71,12 @@ SIGNATURES = {
                                       _f32p, _f64p, _f32p]),
     "cg_coarse_water": (_int, [_u8p, _int, _int, _int, _int, _int, _u8p]),
     "cg_level_shapes": (_int, [_int, _int, _dbl, _dbl, _int, _i32p, _i32p, _f64p, _f64p]),
+    "cg_heat_diffusivity": (_dbl, [_dbl, _dbl]),
+    "cg_heat_stencil": (_int, [_int, _int, _dbl, _dbl, _u8p, _f32p, _vp, _dbl, _int, _dbl, _dbl, _vp, _vp,
+                               _f32p, _f32p, _f32p, _f32p, _f64p, _f32p]),
+    "cg_water_bodies": (_int, [_u8p, _int, _int, _i32p]),
+    "cg_resample": (_int, [_f32p, _int, _int, _int, _int, _f32p]),
+    "cg_coarse_mean": (_int, [_f32p, _int, _int, _int, _int, _f32p]),
     "cg_problem_generate": (_int, [_int, _int, _u64, _dbl, _int, _vp, _vp, _vp]),
     "cg_problem_stress": (_int, [_int, _int, _u64, _int, _vp, _vp]),
 }

@@ -1,5 +1,6 @@
-// Host-side problem builders of the ocean solver: the circulation stencil, coarse masks, the level
-// ladder's shapes, beta per row, the bottom drag, and the synthetic workload generator.
+// Host-side problem builders of the ocean solver: the circulation and heat stencils, coarse masks,
+// the level ladder's shapes, beta per row, the bottom drag, water bodies, the heat's coarse-grid
+// resampling, and the synthetic workload generator.
 //
 // Compiled with -ffp-contract=off so every double and float operation rounds once, in the order
 // written, as the reference's JVM arithmetic does (the reference is Kotlin; JVM floating point is
@@ -56,6 +57,39 @@ double octave(uint64_t seed, int o, int nx, int ny, int across, int down, int ro
     double top = at(i0, j0) * (1.0 - fu) + at(i1, j0) * fu;
     double bottom = at(i0, j1) * (1.0 - fu) + at(i1, j1) * fu;
     return top * (1.0 - fv) + bottom * fv;
+}
+
+// OceanHeat's deformation radius and eddy diffusivity (OceanHeat.kt:86-129).
+struct EddyModel {
+    double wave_speed;    // BAROCLINIC_WAVE_SPEED_M_PER_S: 2 beta_eq R^2, beta_eq = 2 Omega / a_Earth, R = 240 km
+    double eddy_velocity; // EDDY_VELOCITY_MPS: K = 2,500 m^2/s at 45 degrees on any planet
+    EddyModel() {
+        const double omega = (double)kRotationRateF;
+        wave_speed = 2.0 * (2.0 * omega / kEarthMeanRadius) * 240000.0 * 240000.0;
+        eddy_velocity = 2500.0 * 2.0 * omega * std::sin(45.0 * M_PI / 180.0) / wave_speed;
+    }
+    double radius(double latitude_deg, double planet_radius_m) const {
+        const double omega = (double)kRotationRateF;
+        const double phi = latitude_deg * M_PI / 180.0;
+        const double f = std::fabs(2.0 * omega * std::sin(phi));
+        const double beta = 2.0 * omega * std::cos(phi) / planet_radius_m;
+        const double away = f == 0.0 ? INFINITY : wave_speed / f;
+        const double near = beta <= 0.0 ? INFINITY : std::sqrt(wave_speed / (2.0 * beta));
+        return std::min(away, near);
+    }
+    double diffusivity(double latitude_deg, double planet_radius_m) const {
+        return eddy_velocity * radius(latitude_deg, planet_radius_m);
+    }
+};
+
+// The Scharfetter-Gummel face weight K B(P) / d^2, B(x) = x / (e^x - 1), P = w d / K (OH:231-244).
+inline double sg_face(double k, double outward, double spacing) {
+    const double p = outward * spacing / k;
+    double b;
+    if (std::fabs(p) < 1e-3) b = 1.0 - p / 2.0;
+    else if (p > 700.0) b = 0.0;
+    else b = p / std::expm1(p);
+    return k / (spacing * spacing) * b;
 }
 
 }  // namespace
@@ -238,6 +272,139 @@ int cg_problem_generate(int across, int down, uint64_t seed, double land_fractio
     }
     if (water) std::memcpy(water, wet.data(), (size_t)cells);
     if (depth) std::memcpy(depth, h.data(), sizeof(double) * (size_t)cells);
+    return CG_OK;
+}
+
+// OceanHeat.diffusivity (OH:128-129).
+double cg_heat_diffusivity(double latitude_deg, double planet_radius_m) {
+    static const EddyModel model;
+    return model.diffusivity(latitude_deg, planet_radius_m);
+}
+
+// OceanHeat.stencil then withBalance (OH:151-228). Face velocities come from psi at the cell's
+// corners; a corner touching land is the coast's psi, zero, else the mean of its four cells, a cell
+// beyond a pole reading minus its own column's value in the cell's row.
+int cg_heat_stencil(int across, int down, double dx, double dy, const uint8_t *water, const float *stream,
+                    const float *target, double relaxation_s, int with_target, double planet_radius_m,
+                    double constant_k, const float *entrainment, const float *subsurface, float *east, float *west,
+                    float *north, float *south, double *centre, float *forcing) {
+    if (across <= 0 || down <= 0 || !water || !stream || (with_target && !target) || !(relaxation_s > 0.0) ||
+        !east || !west || !north || !south || !centre || !forcing || (constant_k <= 0.0 && !(planet_radius_m > 0.0)))
+        return CG_ERR_ARG;
+    static const EddyModel model;
+    auto k_at = [&](double lat) { return constant_k > 0.0 ? constant_k : model.diffusivity(lat, planet_radius_m); };
+    std::vector<double> along(down), across_edge(down + 1);
+    for (int row = 0; row < down; ++row) {
+        const float lat = 90.0f - 180.0f * ((float)row + 0.5f) / (float)down;  // ClimateStage.latitudeOf
+        along[row] = k_at((double)lat);
+    }
+    for (int b = 0; b <= down; ++b) across_edge[b] = k_at(90.0 - 180.0 * b / down);
+    const double rate = 1.0 / relaxation_s;
+    auto wet = [&](int r, int c) { return water[(long)r * across + c] != 0; };
+    auto corner = [&](int row, int wc, int ec, int other) -> double {
+        for (int r : {row, other})
+            if (r >= 0 && r < down && (!wet(r, wc) || !wet(r, ec))) return 0.0;
+        auto psi = [&](int r, int c) -> double {
+            return (r < 0 || r >= down) ? -(double)stream[(long)row * across + c] : (double)stream[(long)r * across + c];
+        };
+        return (psi(row, wc) + psi(row, ec) + psi(other, wc) + psi(other, ec)) / 4.0;
+    };
+    for (int row = 0; row < down; ++row) {
+        for (int col = 0; col < across; ++col) {
+            const long cell = (long)row * across + col;
+            east[cell] = west[cell] = north[cell] = south[cell] = 0.0f;
+            centre[cell] = 1.0;
+            forcing[cell] = 0.0f;
+            if (!water[cell]) continue;
+            const int ce = col + 1 == across ? 0 : col + 1;
+            const int cw = col == 0 ? across - 1 : col - 1;
+            const double ne = corner(row, col, ce, row - 1), se = corner(row, col, ce, row + 1);
+            const double nw = corner(row, cw, col, row - 1), sw = corner(row, cw, col, row + 1);
+            const double ae = wet(row, ce) ? sg_face(along[row], -(ne - se) / dy, dx) : 0.0;
+            const double aw = wet(row, cw) ? sg_face(along[row], (nw - sw) / dy, dx) : 0.0;
+            const double an = (row > 0 && wet(row - 1, col)) ? sg_face(across_edge[row], (ne - nw) / dx, dy) : 0.0;
+            const double as = (row + 1 < down && wet(row + 1, col)) ? sg_face(across_edge[row + 1], -(se - sw) / dx, dy)
+                                                                    : 0.0;
+            const double ent = entrainment ? (double)entrainment[cell] : 0.0;
+            const double c = ae + aw + an + as + rate + ent;
+            east[cell] = (float)(ae / c);
+            west[cell] = (float)(aw / c);
+            north[cell] = (float)(an / c);
+            south[cell] = (float)(as / c);
+            centre[cell] = c;
+            if (with_target) {
+                const double balance = (double)(-target[cell]) * rate - ent * (subsurface ? (double)subsurface[cell] : 0.0);
+                forcing[cell] = (float)(balance / c);
+            }
+        }
+    }
+    return CG_OK;
+}
+
+// OceanCirculation.waterBodies (OC:670-697): four-connected, columns wrapping, numbered in the order
+// of each body's first cell (a flood fill from each unlabelled water cell in cell order).
+int cg_water_bodies(const uint8_t *water, int across, int down, int32_t *body_of) {
+    if (!water || !body_of || across <= 0 || down <= 0) return CG_ERR_ARG;
+    const long cells = (long)across * down;
+    std::fill(body_of, body_of + cells, -1);
+    std::vector<int32_t> stack;
+    int count = 0;
+    for (long seed = 0; seed < cells; ++seed) {
+        if (!water[seed] || body_of[seed] >= 0) continue;
+        body_of[seed] = count;
+        stack.assign(1, (int32_t)seed);
+        while (!stack.empty()) {
+            const long cell = stack.back();
+            stack.pop_back();
+            const int row = (int)(cell / across), col = (int)(cell % across);
+            const long nbs[4] = {(long)row * across + (col + 1) % across, (long)row * across + (col + across - 1) % across,
+                                 row > 0 ? cell - across : -1, row + 1 < down ? cell + across : -1};
+            for (long nb : nbs)
+                if (nb >= 0 && water[nb] && body_of[nb] < 0) {
+                    body_of[nb] = count;
+                    stack.push_back((int32_t)nb);
+                }
+        }
+        ++count;
+    }
+    return count;
+}
+
+// OceanStage.resample / sample (OS:1199-1223): bilinear in binary32 at each target cell centre,
+// columns wrapping, rows clamped.
+int cg_resample(const float *field, int fa, int fd, int ta, int td, float *out) {
+    if (!field || !out || fa <= 0 || fd <= 0 || ta <= 0 || td <= 0) return CG_ERR_ARG;
+    for (int row = 0; row < td; ++row) {
+        const float fr = ((float)row + 0.5f) * (float)fd / (float)td - 0.5f;
+        const float clamped = std::min(std::max(fr, 0.0f), (float)(fd - 1));
+        const int above = std::min((int)clamped, fd - 1);
+        const int below = std::min(above + 1, fd - 1);
+        const float db = clamped - (float)above;
+        for (int col = 0; col < ta; ++col) {
+            const float fc = ((float)col + 0.5f) * (float)fa / (float)ta - 0.5f;
+            const int left = (int)std::floor(fc);
+            const float ab = fc - (float)left;
+            const int wc = ((left % fa) + fa) % fa;
+            const int ec = wc + 1 == fa ? 0 : wc + 1;
+            const float top = field[(long)above * fa + wc] * (1.0f - ab) + field[(long)above * fa + ec] * ab;
+            const float bottom = field[(long)below * fa + wc] * (1.0f - ab) + field[(long)below * fa + ec] * ab;
+            out[(long)row * ta + col] = top * (1.0f - db) + bottom * db;
+        }
+    }
+    return CG_OK;
+}
+
+// OceanStage.coarseMean (OS:857-860): restrict (OC:334-346) of the binary64 promotion, each coarse
+// cell accumulating fine * (1 / (cx cy)) in row-major fine order, then rounded to binary32.
+int cg_coarse_mean(const float *fine, int fa, int fd, int ca, int cd, float *coarse) {
+    if (!fine || !coarse || fa <= 0 || fd <= 0 || ca <= 0 || cd <= 0 || fa % ca || fd % cd) return CG_ERR_ARG;
+    const int cpc = fa / ca, rpc = fd / cd;
+    const double share = 1.0 / (double)(cpc * rpc);
+    std::vector<double> acc((size_t)ca * cd, 0.0);
+    for (int row = 0; row < fd; ++row)
+        for (int col = 0; col < fa; ++col)
+            acc[(long)(row / rpc) * ca + col / cpc] += (double)fine[(long)row * fa + col] * share;
+    for (long i = 0; i < (long)ca * cd; ++i) coarse[i] = (float)acc[i];
     return CG_OK;
 }
 

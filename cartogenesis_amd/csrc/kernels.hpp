@@ -1317,4 +1317,74 @@ __global__ __launch_bounds__(256) void k_velocities(int across, int down, const 
     north[cell] = (float)((double)(psi(base + ce) - psi(base + cw)) / (2.0 * width_m));
 }
 
+// ---- correctBodyMeans (OC:709-731) --------------------------------------------------------------
+
+constexpr int kBodySegment = 4096;  // most cells of one body one workgroup sums
+
+// The binary64 residual of one water cell, in k_residual64's order.
+__device__ __forceinline__ double residual64_at(const LevelArgs &L, const float *__restrict__ f,
+                                                const float *__restrict__ x, int row, int col) {
+    const int across = L.across;
+    const long cell = (long)row * across + col;
+    const int ce = col + 1 == across ? 0 : col + 1;
+    const int cw = col == 0 ? across - 1 : col - 1;
+    const double vn = row > 0 ? (double)x[cell - across] : 0.0;
+    const double vs = row + 1 < L.down ? (double)x[cell + across] : 0.0;
+    const float4 wt = weights_at(L, row, cell);
+    const double bal = (double)wt.x * (double)x[(long)row * across + ce] + (double)wt.y * (double)x[(long)row * across + cw] +
+                       (double)wt.z * vn + (double)wt.w * vs - (double)f[cell];
+    return ((double)x[cell] - bal) * centre_at(L, row, cell);
+}
+
+// One workgroup per segment of the body-grouped cell list: the sum of the residuals of its cells,
+// each thread summing a strided share in order, then a fixed tree: deterministic.
+__global__ __launch_bounds__(256) void k_body_segment_sums(LevelArgs L, const float *__restrict__ f,
+                                                           const float *__restrict__ x,
+                                                           const int *__restrict__ order,
+                                                           const int *__restrict__ seg_start,
+                                                           const int *__restrict__ seg_len,
+                                                           double *__restrict__ seg_sum) {
+    __shared__ double part[256];
+    const int seg = blockIdx.x;
+    const int start = seg_start[seg], len = seg_len[seg];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < len; i += 256) {
+        const int cell = order[start + i];
+        const int row = cell / L.across;
+        acc += residual64_at(L, f, x, row, cell - row * L.across);
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) seg_sum[seg] = part[0];
+}
+
+// One thread per body: its segments' sums in order, over its margin sum, rounded to binary32; a body
+// with no margin is left alone (shift 0).
+__global__ __launch_bounds__(256) void k_body_shift(const double *__restrict__ seg_sum,
+                                                    const int *__restrict__ body_seg0,
+                                                    const int *__restrict__ body_nseg,
+                                                    const double *__restrict__ margin, int bodies,
+                                                    float *__restrict__ shift) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= bodies) return;
+    double sum = 0.0;
+    const int s0 = body_seg0[b], ns = body_nseg[b];
+    for (int k = 0; k < ns; ++k) sum += seg_sum[s0 + k];
+    const double m = margin[b];
+    shift[b] = m <= 0.0 ? 0.0f : (float)(sum / m);
+}
+
+// x -= shift[body] on every cell of a body.
+__global__ __launch_bounds__(256) void k_body_apply(const int *__restrict__ body_of, const float *__restrict__ shift,
+                                                    float *__restrict__ x, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b = body_of[i];
+    if (b >= 0) x[i] -= shift[b];
+}
+
 }  // namespace cg

@@ -333,6 +333,15 @@ struct cg_plan {
     // finest grid, and the velocities of the answer.
     double *tau_e = nullptr, *tau_n = nullptr, *depth = nullptr;
     float *vel = nullptr;
+    // The finest grid's water bodies (correctBodyMeans, OC:709-731): body per cell; the cells grouped
+    // by body in cell order, cut into segments of at most kBodySegment cells; per body its first
+    // segment, its segment count and its margin sum c (1 - (e + w + n + s)), summed on the host in
+    // cell order as the reference does.
+    int nbodies = 0, nseg = 0;
+    int *d_body_of = nullptr, *d_order = nullptr, *d_seg_start = nullptr, *d_seg_len = nullptr;
+    int *d_body_seg0 = nullptr, *d_body_nseg = nullptr;
+    double *d_margin = nullptr, *d_seg_sum = nullptr;
+    float *d_shift = nullptr;
 };
 
 namespace {
@@ -366,6 +375,10 @@ void free_plan(cg_plan *p) {
     dfree(p->tau_n);
     dfree(p->depth);
     dfree(p->vel);
+    for (void *v : {(void *)p->d_body_of, (void *)p->d_order, (void *)p->d_seg_start, (void *)p->d_seg_len,
+                    (void *)p->d_body_seg0, (void *)p->d_body_nseg, (void *)p->d_margin, (void *)p->d_seg_sum,
+                    (void *)p->d_shift})
+        dfree(v);
     for (int i = 0; i < 2; ++i)
         if (p->step_exec[i]) (void)hipGraphExecDestroy(p->step_exec[i]);
     delete p;
@@ -622,7 +635,25 @@ int ref_vcycle(cg_plan *p, const float *f0, double tol, int fixed) {
     return CG_OK;
 }
 
-// solve (OC:473-493) from the binary32 values in lv[0].
+// correctBodyMeans (OC:709-731) of x against forcing f on the finest grid, when the plan has bodies.
+int correct_bodies(cg_plan *p, const float *f, float *x) {
+    if (p->nbodies == 0) return CG_OK;
+    cg_ctx *ctx = p->ctx;
+    hipStream_t s = ctx->stream;
+    const Stencil *st = p->lv[0].st;
+    if (p->nseg == 0) return CG_OK;  // bodies without cells: nothing to shift
+    hipLaunchKernelGGL(k_body_segment_sums, dim3(p->nseg), dim3(256), 0, s, st->args(), f, x, p->d_order,
+                       p->d_seg_start, p->d_seg_len, p->d_seg_sum);
+    hipLaunchKernelGGL(k_body_shift, dim3(blocks_for(p->nbodies, 256)), dim3(256), 0, s, p->d_seg_sum, p->d_body_seg0,
+                       p->d_body_nseg, p->d_margin, p->nbodies, p->d_shift);
+    hipLaunchKernelGGL(k_body_apply, dim3(blocks_for(st->cells, 256)), dim3(256), 0, s, p->d_body_of, p->d_shift, x,
+                       st->cells);
+    HIPCHK(ctx, hipGetLastError());
+    return CG_OK;
+}
+
+// solve (OC:473-493) from the binary32 values in lv[0], with correctBodyMeans after each cycle when
+// the plan has bodies.
 int ref_solve(cg_plan *p, double tol, cg_solution *info, const volatile int *cancel) {
     double largest;
     int rc = plan_largest(p, &largest);
@@ -639,6 +670,7 @@ int ref_solve(cg_plan *p, double tol, cg_solution *info, const volatile int *can
             break;
         }
         if ((rc = ref_vcycle(p, p->f0, tol, 0))) return rc;
+        if ((rc = correct_bodies(p, p->f0, p->lv[0].x[p->lv[0].cur]))) return rc;
         cycles++;
         if ((rc = ref_residual(p, p->f0, &res))) return rc;
         res /= largest;
@@ -679,7 +711,7 @@ int krylov_largest(cg_plan *p, const double *a, double *out) {
 }
 
 // precondition (OC:610-622): one reference V-cycle from zero on L z = balance, the coarsest relaxed
-// a fixed 200 passes, answer widened to binary64.
+// a fixed 200 passes, then the bodies' means; answer widened to binary64.
 int krylov_precondition(cg_plan *p, const double *balance, double *out, double tol) {
     cg_ctx *ctx = p->ctx;
     hipStream_t s = ctx->stream;
@@ -689,6 +721,7 @@ int krylov_precondition(cg_plan *p, const double *balance, double *out, double t
     HIPCHK(ctx, hipMemsetAsync(l0.x[l0.cur], 0, sizeof(float) * n, s));
     int rc = ref_vcycle(p, p->kf32, tol, kPrecondCoarsestPasses);
     if (rc) return rc;
+    if ((rc = correct_bodies(p, p->kf32, l0.x[l0.cur]))) return rc;
     hipLaunchKernelGGL(k_f32_to_f64, dim3(blocks_for(n, 256)), dim3(256), 0, s, l0.x[l0.cur], out, n);
     HIPCHK(ctx, hipGetLastError());
     return CG_OK;
@@ -789,6 +822,36 @@ int rescue_solve(cg_plan *p, double tol, cg_solution *info, const volatile int *
     }
     info->cycles += total;
     info->relative_residual = rr;
+    return CG_OK;
+}
+
+// solveByKrylov from the binary32 values in lv[0], started afresh up to kKrylovRestarts times while
+// it stops short of tol with finite values: the heat's loop (OS:486-497). cycles counts the
+// iterations of every start, refinements the starts.
+int krylov_restarted(cg_plan *p, double tol, cg_solution *info, const volatile int *cancel) {
+    const long n = p->lv[0].st->cells;
+    std::vector<float> v(n);
+    int its = 0, total = 0, starts = 0;
+    double rr = 0.0;
+    int rc;
+    while (true) {
+        if (cancel && *cancel) {
+            info->status = CG_CANCELLED;
+            break;
+        }
+        if ((rc = krylov_solve(p, tol, kMostCycles, &its, &rr))) return rc;
+        total += its;
+        starts++;
+        if (rr < tol || starts > kKrylovRestarts) break;
+        HIPCHK(p->ctx, hipMemcpy(v.data(), p->lv[0].x[p->lv[0].cur], sizeof(float) * n, hipMemcpyDeviceToHost));
+        bool finite = true;
+        for (float a : v) finite = finite && std::isfinite(a);
+        if (!finite) break;
+    }
+    info->cycles = total;
+    info->refinements = starts;
+    info->relative_residual = rr;
+    p->last_answer = 0;
     return CG_OK;
 }
 
@@ -1521,10 +1584,11 @@ static int plan_solve_locked(cg_plan *p, double tol, int method, int warm, cg_so
     if (!info) info = &local;
     *info = cg_solution{};
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    if (method == CG_METHOD_REFERENCE || method == CG_METHOD_RESCUE) {
+    if (method == CG_METHOD_REFERENCE || method == CG_METHOD_RESCUE || method == CG_METHOD_KRYLOV) {
         PlanLevel &l0 = p->lv[0];
         if (!warm) HIPCHK(ctx, hipMemsetAsync(l0.x[l0.cur], 0, sizeof(float) * l0.st->cells, ctx->stream));
         if (method == CG_METHOD_RESCUE) return rescue_solve(p, tol, info, cancel);
+        if (method == CG_METHOD_KRYLOV) return krylov_restarted(p, tol, info, cancel);
         return ref_solve(p, tol, info, cancel);
     }
     if (method == CG_METHOD_REFINED) return refined_solve(p, tol, warm != 0, info, cancel);
@@ -1545,6 +1609,116 @@ int cg_ocean_plan_solve_batch(cg_plan *const *plans, int count, double tol, int 
         int rc = cg_ocean_plan_solve(plans[i], tol, method, 0, infos ? &infos[i] : nullptr);
         if (rc) return rc;
     }
+    return CG_OK;
+}
+
+int cg_ocean_plan_set_bodies(cg_plan *p, const int32_t *body_of, int count) {
+    if (!p || count < 0 || (count > 0 && !body_of)) return CG_ERR_ARG;
+    cg_ctx *ctx = p->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const Stencil *st = p->lv[0].st;
+    const long n = st->cells;
+    for (void *v : {(void *)p->d_body_of, (void *)p->d_order, (void *)p->d_seg_start, (void *)p->d_seg_len,
+                    (void *)p->d_body_seg0, (void *)p->d_body_nseg, (void *)p->d_margin, (void *)p->d_seg_sum,
+                    (void *)p->d_shift})
+        dfree(v);
+    p->d_body_of = p->d_order = p->d_seg_start = p->d_seg_len = p->d_body_seg0 = p->d_body_nseg = nullptr;
+    p->d_margin = p->d_seg_sum = nullptr;
+    p->d_shift = nullptr;
+    p->nbodies = p->nseg = 0;
+    if (count == 0) return CG_OK;
+    if (!st->has_centre) {
+        set_err(ctx, "bodies need the finest stencil's centre weights");
+        return CG_ERR_STATE;
+    }
+    std::vector<int> per(count, 0);
+    for (long c = 0; c < n; ++c) {
+        const int b = body_of[c];
+        if (b < -1 || b >= count) {
+            set_err(ctx, "body %d of cell %ld is outside -1..%d", b, c, count - 1);
+            return CG_ERR_ARG;
+        }
+        if (b >= 0) per[b]++;
+    }
+    // Cells grouped by body, each body's in cell order; the margins summed in cell order.
+    std::vector<int> first(count + 1, 0), order, fill;
+    for (int b = 0; b < count; ++b) first[b + 1] = first[b] + per[b];
+    order.resize(first[count]);
+    fill.assign(first.begin(), first.end() - 1);
+    std::vector<double> margin(count, 0.0);
+    for (long c = 0; c < n; ++c) {
+        const int b = body_of[c];
+        if (b < 0) continue;
+        order[fill[b]++] = (int)c;
+        const int row = (int)(c / st->across);
+        const float4 w = st->rowconst ? st->h_w[row] : st->h_w[c];
+        const double cc = st->rowconst ? st->h_c[row] : st->h_c[c];
+        const double weights = (double)w.x + (double)w.y + (double)w.z + (double)w.w;
+        margin[b] += cc * (1.0 - weights);
+    }
+    std::vector<int> seg_start, seg_len, body_seg0(count), body_nseg(count);
+    for (int b = 0; b < count; ++b) {
+        body_seg0[b] = (int)seg_start.size();
+        for (int at = first[b]; at < first[b + 1]; at += kBodySegment) {
+            seg_start.push_back(at);
+            seg_len.push_back(std::min(kBodySegment, first[b + 1] - at));
+        }
+        body_nseg[b] = (int)seg_start.size() - body_seg0[b];
+    }
+    const int nseg = (int)seg_start.size();
+    int rc;
+    if ((rc = dalloc(ctx, &p->d_body_of, n)) || (rc = dalloc(ctx, &p->d_order, order.size())) ||
+        (rc = dalloc(ctx, &p->d_seg_start, nseg)) || (rc = dalloc(ctx, &p->d_seg_len, nseg)) ||
+        (rc = dalloc(ctx, &p->d_body_seg0, count)) || (rc = dalloc(ctx, &p->d_body_nseg, count)) ||
+        (rc = dalloc(ctx, &p->d_margin, count)) || (rc = dalloc(ctx, &p->d_seg_sum, nseg)) ||
+        (rc = dalloc(ctx, &p->d_shift, count)))
+        return rc;
+    HIPCHK(ctx, hipMemcpy(p->d_body_of, body_of, sizeof(int) * n, hipMemcpyHostToDevice));
+    if (!order.empty()) HIPCHK(ctx, hipMemcpy(p->d_order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
+    if (nseg) {
+        HIPCHK(ctx, hipMemcpy(p->d_seg_start, seg_start.data(), sizeof(int) * nseg, hipMemcpyHostToDevice));
+        HIPCHK(ctx, hipMemcpy(p->d_seg_len, seg_len.data(), sizeof(int) * nseg, hipMemcpyHostToDevice));
+    }
+    HIPCHK(ctx, hipMemcpy(p->d_body_seg0, body_seg0.data(), sizeof(int) * count, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(p->d_body_nseg, body_nseg.data(), sizeof(int) * count, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(p->d_margin, margin.data(), sizeof(double) * count, hipMemcpyHostToDevice));
+    p->nbodies = count;
+    p->nseg = nseg;
+    return CG_OK;
+}
+
+int cg_ocean_plan_correct_bodies(cg_plan *p, const float *values, float *out) {
+    if (!p || !values || !out) return CG_ERR_ARG;
+    cg_ctx *ctx = p->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    PlanLevel &l0 = p->lv[0];
+    const long n = l0.st->cells;
+    float *x = l0.x[l0.cur];
+    HIPCHK(ctx, hipMemcpyAsync(x, values, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    int rc = correct_bodies(p, p->f0, x);
+    if (rc) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(out, x, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    p->last_answer = 0;
+    return CG_OK;
+}
+
+int cg_ocean_plan_set_start(cg_plan *p, const float *start) {
+    if (!p || !start) return CG_ERR_ARG;
+    cg_ctx *ctx = p->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    PlanLevel &l0 = p->lv[0];
+    const long n = l0.st->cells;
+    HIPCHK(ctx, hipMemcpyAsync(l0.x[l0.cur], start, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_f32_to_f64, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, l0.x[l0.cur],
+                       p->x64[p->x64cur], n);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    p->x64_valid = true;
+    p->last_answer = 0;
     return CG_OK;
 }
 

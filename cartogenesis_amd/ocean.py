@@ -4,6 +4,8 @@ Names, argument meanings and failure behaviour follow the reference's Kotlin:
   OceanStencil            OceanCirculation.kt:24-42
   OceanCirculation.*      OceanCirculation.kt:82-829 (stencil, withBalance, coarseWater, levels,
                           Solution, OceanSolveFailure, requireSolved, solve)
+  OceanHeat               OceanHeat.kt:61-281 (diffusivity, stencil)
+  OceanStage (heat grids) OceanStage.kt:449-498, 857-860, 1199-1223 (heat_levels, resample, coarse_mean)
   OceanAccelerator        OceanAccelerator.kt:15-42
   HipOcean                desktop/.../GpuOcean.kt:26-251 (createOrNull, name, solve, residency)
 The relaxation, the residual and the whole solve run on the GPU through the HIP library; nothing
@@ -61,6 +63,13 @@ class Solution:
     relative_residual: float
     values64: Optional[np.ndarray] = None
     refinements: int = 0
+
+
+@dataclass
+class WaterBodies:
+    """OceanCirculation.WaterBodies (OceanCirculation.kt:667-668): body per cell, -1 on land."""
+    body_of: np.ndarray
+    count: int
 
 
 class OceanSolveFailure(RuntimeError):
@@ -160,6 +169,93 @@ class OceanCirculation:
             a, d = na, nd
             out.append(coarser(a, d, water))
         return out
+
+
+    @staticmethod
+    def water_bodies(is_water, cells_across: int, cells_down: int) -> WaterBodies:
+        """waterBodies (OceanCirculation.kt:670-697)."""
+        out = np.empty(cells_across * cells_down, np.int32)
+        count = check(lib().cg_water_bodies(np.ascontiguousarray(is_water, dtype=np.uint8).reshape(-1), cells_across,
+                                            cells_down, out), what="cg_water_bodies")
+        return WaterBodies(out, int(count))
+
+
+def resample(field, from_across: int, from_down: int, to_across: int, to_down: int) -> np.ndarray:
+    """OceanStage.resample (OceanStage.kt:1199-1223)."""
+    out = np.empty(to_across * to_down, np.float32)
+    check(lib().cg_resample(np.ascontiguousarray(field, dtype=np.float32).reshape(-1), from_across, from_down,
+                            to_across, to_down, out), what="cg_resample")
+    return out
+
+
+def coarse_mean(fine, fine_across: int, fine_down: int, coarse_across: int, coarse_down: int) -> np.ndarray:
+    """OceanStage.coarseMean (OceanStage.kt:857-860)."""
+    out = np.empty(coarse_across * coarse_down, np.float32)
+    check(lib().cg_coarse_mean(np.ascontiguousarray(fine, dtype=np.float32).reshape(-1), fine_across, fine_down,
+                               coarse_across, coarse_down, out), what="cg_coarse_mean")
+    return out
+
+
+class OceanHeat:
+    """The sea's heat balance (OceanHeat.kt:61-281): u.grad T - K lap T + (T - T_lat)/tau +
+    (w/h)(T - T_sub) = 0, exponentially fitted per face, solved by the same multigrid."""
+
+    MIDLATITUDE_DIFFUSIVITY_M2_PER_S = 2500.0
+    MIDLATITUDE_DEGREES = 45.0
+    # OceanStage.RELAXATION_SECONDS (OceanStage.kt:113-114): a 50 m mixed layer's heat capacity,
+    # 4e6 J/m^3/C, over the 25 W/m^2/C surface exchange (EnergyBalance.kt:688, 738, 754, 832).
+    RELAXATION_SECONDS = 50.0 * 4.0e6 / 25.0
+    HEAT_SOLVE_RESTARTS = 3
+
+    @staticmethod
+    def diffusivity(latitude_degrees: float, radius_meters: float) -> float:
+        """diffusivity (OceanHeat.kt:128-129)."""
+        return float(lib().cg_heat_diffusivity(float(latitude_degrees), float(radius_meters)))
+
+    @staticmethod
+    def stencil(cells_across, cells_down, cell_width_meters, cell_height_meters, is_water, stream, target_c,
+                relaxation_seconds, with_target, radius_meters=None, constant_diffusivity=None,
+                entrainment_per_s=None, subsurface_c=None) -> OceanStencil:
+        """stencil (OceanHeat.kt:151-228). The diffusivity is diffusivity(latitude, radius_meters),
+        or constant_diffusivity where given (the reference's diffusivityAt lambda)."""
+        n = cells_across * cells_down
+        water = np.ascontiguousarray(is_water, dtype=np.uint8).reshape(n)
+        e, w, no, s, f = (np.empty(n, np.float32) for _ in range(5))
+        c = np.empty(n, np.float64)
+
+        def opt(a):
+            return None if a is None else np.ascontiguousarray(a, dtype=np.float32).reshape(n)
+
+        tgt, ent, sub = opt(target_c), opt(entrainment_per_s), opt(subsurface_c)
+        keep = [a for a in (tgt, ent, sub) if a is not None]
+        ptr = lambda a: None if a is None else a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+        check(lib().cg_heat_stencil(cells_across, cells_down, float(cell_width_meters), float(cell_height_meters),
+                                    water, np.ascontiguousarray(stream, dtype=np.float32).reshape(n), ptr(tgt),
+                                    float(relaxation_seconds), int(bool(with_target)),
+                                    float(radius_meters or 0.0), float(constant_diffusivity or 0.0), ptr(ent),
+                                    ptr(sub), e, w, no, s, c, f), what="cg_heat_stencil")
+        del keep
+        return OceanStencil(cells_across, cells_down, water, e, w, no, s, f, c)
+
+    @staticmethod
+    def levels(across, down, width_m, height_m, is_water, stream, target_c, radius_meters=None,
+               constant_diffusivity=None, entrainment_per_s=None, subsurface_c=None,
+               relaxation_seconds=None) -> List[OceanStencil]:
+        """The heat's grids as OceanStage.circulateUnder builds them (OceanStage.kt:463-479): the
+        finest with its target, the coarse ones with the stream resampled, the entrainment's block
+        means and no target; coarse masks keep any water (everyCellWater = false)."""
+        tau = OceanHeat.RELAXATION_SECONDS if relaxation_seconds is None else relaxation_seconds
+        stream = np.ascontiguousarray(stream, dtype=np.float32).reshape(-1)
+        finest = OceanHeat.stencil(across, down, width_m, height_m, is_water, stream, target_c, tau, True,
+                                   radius_meters, constant_diffusivity, entrainment_per_s, subsurface_c)
+
+        def coarser(a, d, w):
+            ent = None if entrainment_per_s is None else coarse_mean(entrainment_per_s, across, down, a, d)
+            return OceanHeat.stencil(a, d, width_m * across / a, height_m * down / d, w,
+                                     resample(stream, across, down, a, d), np.zeros(a * d, np.float32), tau, False,
+                                     radius_meters, constant_diffusivity, ent, None)
+
+        return OceanCirculation.levels(finest, width_m, height_m, False, coarser)
 
 
 class OceanAccelerator:
@@ -347,6 +443,28 @@ class Plan:
                                              e.ctypes.data_as(ctypes.c_void_p), nn.ctypes.data_as(ctypes.c_void_p)),
               self.acc.ctx, "cg_ocean_plan_velocities")
         return e, nn
+
+    def set_bodies(self, bodies: Optional[WaterBodies]) -> None:
+        """The finest grid's water bodies for correctBodyMeans (OceanCirculation.kt:709-731); None
+        removes them."""
+        if bodies is None or bodies.count == 0:
+            check(lib().cg_ocean_plan_set_bodies(self._p, None, 0), self.acc.ctx, "cg_ocean_plan_set_bodies")
+            return
+        b = np.ascontiguousarray(bodies.body_of, dtype=np.int32).reshape(-1)
+        check(lib().cg_ocean_plan_set_bodies(self._p, b.ctypes.data_as(ctypes.c_void_p), int(bodies.count)),
+              self.acc.ctx, "cg_ocean_plan_set_bodies")
+
+    def correct_bodies(self, values) -> np.ndarray:
+        """correctBodyMeans on the device against the plan's finest forcing."""
+        out = np.empty(self.levels[0].cells, np.float32)
+        check(lib().cg_ocean_plan_correct_bodies(self._p, np.ascontiguousarray(values, dtype=np.float32), out),
+              self.acc.ctx, "cg_ocean_plan_correct_bodies")
+        return out
+
+    def set_start(self, start) -> None:
+        """The values a warm solve starts from."""
+        check(lib().cg_ocean_plan_set_start(self._p, np.ascontiguousarray(start, dtype=np.float32)),
+              self.acc.ctx, "cg_ocean_plan_set_start")
 
     def read(self):
         n = self.levels[0].cells

@@ -50,6 +50,10 @@ extern "C" {
 /* Binary64 iterative refinement whose inner solve is flexible GMRES(m) over a binary32 basis,
  * right-preconditioned by one binary32 V(2,2) cycle per iteration. */
 #define CG_METHOD_FGMRES 3
+/* solveByKrylov (OC:548-607) from the plan's start, with the plan's water bodies, started afresh up
+ * to 3 times while it stops short of tol: the heat's solve (OS:486-497). cg_solution.cycles counts
+ * the iterations of every start, refinements the starts. */
+#define CG_METHOD_KRYLOV 4
 
 /* ---- options (cg_ocean_set_option) ------------------------------------------------------------ */
 #define CG_OPT_MIN_CELLS 1       /* relax declines grids with fewer cells (GO:167-172); default 65536 */
@@ -131,6 +135,15 @@ int cg_ocean_plan_solve(cg_plan *plan, double tol, int method, int warm, cg_solu
 /* Solves several plans of one context concurrently, each from zero, to one tol. */
 int cg_ocean_plan_solve_batch(cg_plan *const *plans, int count, double tol, int method,
                               cg_solution *infos);
+/* The finest grid's water bodies (OceanCirculation.WaterBodies, OC:670-697: body per cell, -1 on
+ * land, `count` bodies) for correctBodyMeans (OC:709-731), which then follows every cycle of
+ * CG_METHOD_REFERENCE/RESCUE and every preconditioning cycle of RESCUE/KRYLOV. count 0 removes them.
+ * The per-body residual sums are deterministic tree sums, not the reference's sequential sums. */
+int cg_ocean_plan_set_bodies(cg_plan *plan, const int32_t *body_of, int count);
+/* correctBodyMeans on the device alone: values (host) against the plan's finest forcing, into out. */
+int cg_ocean_plan_correct_bodies(cg_plan *plan, const float *values, float *out);
+/* Sets the answer a warm solve starts from (binary32; the binary64 answer becomes its widening). */
+int cg_ocean_plan_set_start(cg_plan *plan, const float *start);
 /* Copies the answer back; either pointer may be NULL. */
 int cg_ocean_plan_read(cg_plan *plan, double *out64, float *out32);
 /* Synchronises the context's stream and reports the average duration, in microseconds, of the
@@ -192,6 +205,29 @@ int cg_coarse_water(const uint8_t *fine, int fine_across, int fine_down, int coa
  * and fills across/down/cell width/cell height per level, finest first. */
 int cg_level_shapes(int across, int down, double dx, double dy, int max_levels, int *level_across,
                     int *level_down, double *cell_width, double *cell_height);
+
+/* OceanHeat.diffusivity (OceanHeat.kt:128-129): K = V R_d in m^2/s at a latitude on a planet. */
+double cg_heat_diffusivity(double latitude_deg, double planet_radius_m);
+/* OceanHeat.stencil then withBalance (OceanHeat.kt:151-228): per-cell exponentially fitted
+ * (Scharfetter-Gummel) weights of the heat's advection-diffusion-relaxation balance on one grid.
+ * stream is psi (binary32); target the temperature each cell relaxes to over relaxation_s; with_target
+ * 0 gives a zero forcing (a coarse grid's). The diffusivity is constant_k where that is positive,
+ * else cg_heat_diffusivity at each row's latitude on a planet of planet_radius_m. entrainment (w/h per
+ * second) and subsurface (its temperature) may be NULL. */
+int cg_heat_stencil(int across, int down, double dx, double dy, const uint8_t *water, const float *stream,
+                    const float *target, double relaxation_s, int with_target, double planet_radius_m,
+                    double constant_k, const float *entrainment, const float *subsurface, float *east,
+                    float *west, float *north, float *south, double *centre, float *forcing);
+/* OceanCirculation.waterBodies (OC:670-697): returns the count, fills body_of (-1 on land). */
+int cg_water_bodies(const uint8_t *water, int across, int down, int32_t *body_of);
+/* OceanStage.resample (OS:1199-1223): a field read bilinearly at every cell centre of another grid
+ * covering the same map (the heat's coarse streams). */
+int cg_resample(const float *field, int from_across, int from_down, int to_across, int to_down,
+                float *out);
+/* OceanStage.coarseMean (OS:857-860): block means of a rate per unit area (the heat's coarse
+ * entrainment). */
+int cg_coarse_mean(const float *fine, int fine_across, int fine_down, int coarse_across,
+                   int coarse_down, float *coarse);
 
 /* Synthetic circulation inputs: a seeded 4-octave value-noise land mask thresholded so that
  * `land_fraction` of cells are land, depth H ~ U(200, 6000) m on water, and the balance

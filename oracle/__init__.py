@@ -67,6 +67,23 @@ def lib():
     L.or_solve64.restype = _c_int
     L.or_velocities.argtypes = [_c_int, _c_int, _u8p, _f32p, _c_double, _c_double, _f32p, _f32p]
     L.or_water_on.argtypes = [_u8p, _c_int, _c_int, _c_int, _c_int, _u8p]
+    L.or_water_bodies.argtypes = [_u8p, _c_int, _c_int, _i32p]
+    L.or_water_bodies.restype = _c_int
+    L.or_correct_body_means_flat.argtypes = [_c_int, _c_int, _u8p, _f32p, _f32p, _f32p, _f32p, _f32p, _f64p,
+                                             _f32p, _i32p, _c_int]
+    L.or_solve_bodies.argtypes = [_c_int, _i32p, _i32p, _vpp, _vpp, _vpp, _vpp, _vpp, _vpp, _vpp, _f32p,
+                                  _c_double, _c_int, _c_int, ctypes.c_void_p, _c_int, _c_int,
+                                  ctypes.POINTER(_c_double)]
+    L.or_solve_bodies.restype = _c_int
+    L.or_deformation_radius.argtypes = [_c_double, _c_double]
+    L.or_deformation_radius.restype = _c_double
+    L.or_heat_diffusivity.argtypes = [_c_double, _c_double]
+    L.or_heat_diffusivity.restype = _c_double
+    _vp = ctypes.c_void_p
+    L.or_heat_stencil.argtypes = [_c_int, _c_int, _c_double, _c_double, _u8p, _f32p, _f32p, _c_double, _c_int,
+                                  _c_double, _c_double, _vp, _vp, _f32p, _f32p, _f32p, _f32p, _f64p, _f32p]
+    L.or_resample.argtypes = [_f32p, _c_int, _c_int, _c_int, _c_int, _f32p]
+    L.or_coarse_mean.argtypes = [_f32p, _c_int, _c_int, _c_int, _c_int, _f32p]
     L.or_curl_forcing.argtypes = [_c_int, _c_int, _c_double, _c_double, _f64p, _f64p, ctypes.c_void_p, _f64p]
     _lib = L
     return L
@@ -234,6 +251,22 @@ def _ladder_args(ladder):
 SOLVE, RESCUE, VCYCLE, KRYLOV = 0, 1, 2, 3
 
 
+def solve_bodies(ladder, start, tol, pole_is_wall, mode, body_of=None, body_count=0, most_iterations=200):
+    """solve / solveWithKrylovRescue / solveByKrylov with the finest grid's water bodies
+    (OC:473-607, correctBodyMeans OC:709-731). Returns (values, cycles, relative_residual)."""
+    args, keep = _ladder_args(ladder)
+    x = np.array(start, dtype=np.float32).reshape(ladder[0].cells).copy()
+    rr = ctypes.c_double(0.0)
+    b = None if body_of is None else np.ascontiguousarray(body_of, dtype=np.int32)
+    cycles = lib().or_solve_bodies(*args, x, float(tol), int(pole_is_wall), int(mode),
+                                   None if b is None else b.ctypes.data_as(ctypes.c_void_p), int(body_count),
+                                   int(most_iterations), ctypes.byref(rr))
+    del keep
+    if cycles < 0:
+        raise ValueError("oracle solve rejected its arguments")
+    return x, int(cycles), float(rr.value)
+
+
 def solve(ladder, start, tol, pole_is_wall=True, mode=SOLVE):
     """solve / solveWithKrylovRescue / one vCycle / solveByKrylov (OC:473-607), processor relax.
 
@@ -289,4 +322,63 @@ def curl_forcing(across, down, width_m, height_m, eastward, northward, depth=Non
     out = np.empty(n)
     lib().or_curl_forcing(across, down, float(width_m), float(height_m), e, nn,
                           None if d is None else d.ctypes.data_as(ctypes.c_void_p), out)
+    return out
+
+
+def water_bodies(water, across, down):
+    """waterBodies (OC:670-697): (body_of int32, count)."""
+    out = np.empty(across * down, np.int32)
+    count = lib().or_water_bodies(np.ascontiguousarray(water, dtype=np.uint8).reshape(-1), across, down, out)
+    return out, int(count)
+
+
+def correct_body_means(st: Stencil, values, body_of, count):
+    """correctBodyMeans (OC:709-731) on a copy of values."""
+    x = np.array(values, dtype=np.float32).reshape(st.cells).copy()
+    lib().or_correct_body_means_flat(st.across, st.down, st.water, st.east, st.west, st.north, st.south,
+                                     st.forcing, st.centre, x, np.ascontiguousarray(body_of, dtype=np.int32),
+                                     int(count))
+    return x
+
+
+def deformation_radius(latitude_deg, radius_m):
+    """OceanHeat.deformationRadiusMeters (OceanHeat.kt:98-105)."""
+    return float(lib().or_deformation_radius(float(latitude_deg), float(radius_m)))
+
+
+def heat_diffusivity(latitude_deg, radius_m):
+    """OceanHeat.diffusivity (OceanHeat.kt:128-129)."""
+    return float(lib().or_heat_diffusivity(float(latitude_deg), float(radius_m)))
+
+
+def heat_stencil(across, down, dx, dy, water, stream, target, relaxation_s, with_target, radius_m=0.0,
+                 constant_k=0.0, entrainment=None, subsurface=None) -> Stencil:
+    """OceanHeat.stencil (OceanHeat.kt:151-228); constant_k > 0 holds the diffusivity constant."""
+    n = across * down
+    e, w, nn, s = (np.empty(n, np.float32) for _ in range(4))
+    c = np.empty(n, np.float64)
+    f = np.empty(n, np.float32)
+    ent = None if entrainment is None else np.ascontiguousarray(entrainment, dtype=np.float32).reshape(n)
+    sub = None if subsurface is None else np.ascontiguousarray(subsurface, dtype=np.float32).reshape(n)
+    water = np.ascontiguousarray(water, dtype=np.uint8).reshape(n)
+    lib().or_heat_stencil(across, down, float(dx), float(dy), water,
+                          np.ascontiguousarray(stream, dtype=np.float32).reshape(n),
+                          np.ascontiguousarray(target, dtype=np.float32).reshape(n), float(relaxation_s),
+                          int(bool(with_target)), float(radius_m), float(constant_k),
+                          None if ent is None else ent.ctypes.data_as(ctypes.c_void_p),
+                          None if sub is None else sub.ctypes.data_as(ctypes.c_void_p), e, w, nn, s, c, f)
+    return Stencil(across, down, water, e, w, nn, s, f, c)
+
+
+def resample(field, fa, fd, ta, td):
+    """OceanStage.resample (OceanStage.kt:1199-1223)."""
+    out = np.empty(ta * td, np.float32)
+    lib().or_resample(np.ascontiguousarray(field, dtype=np.float32).reshape(-1), fa, fd, ta, td, out)
+    return out
+
+
+def coarse_mean(fine, fa, fd, ca, cd):
+    """OceanStage.coarseMean (OceanStage.kt:857-860)."""
+    out = np.empty(ca * cd, np.float32)
+    lib().or_coarse_mean(np.ascontiguousarray(fine, dtype=np.float32).reshape(-1), fa, fd, ca, cd, out)
     return out

@@ -287,6 +287,8 @@ int or_level_shapes(int across, int down, double dx, double dy, int max_levels, 
 typedef struct {
     int count;
     or_st st[OR_MAX_LEVELS];
+    const int32_t *body_of; /* waterBodies of the finest grid, or NULL (OC:670-697) */
+    int body_count;
 } or_ladder;
 
 static void or_ladder_from(or_ladder *L, int count, const int *acr, const int *dwn,
@@ -295,6 +297,8 @@ static void or_ladder_from(or_ladder *L, int count, const int *acr, const int *d
                            const double *const *c)
 {
     L->count = count;
+    L->body_of = NULL;
+    L->body_count = 0;
     for (int i = 0; i < count; i++) {
         or_st st = {acr[i], dwn[i], water[i], e[i], w[i], n[i], s[i], f[i], c[i]};
         L->st[i] = st;
@@ -325,6 +329,75 @@ void or_relax_to_convergence(int across, int down, const uint8_t *water, const f
     double *scratch = malloc(sizeof(double) * (size_t)across * down);
     or_relax_to_convergence_st(&st, x, tol, scratch);
     free(scratch);
+}
+
+/* correctBodyMeans (OC:709-731): shift each body by minus its summed residual over its summed margin
+ * c (1 - (e + w + n + s)), both summed sequentially in cell order in binary64; the shift is rounded to
+ * binary32 and subtracted in binary32. */
+static void or_correct_body_means(const or_st *st, float *x, const int32_t *body_of, int count)
+{
+    long cells = (long)st->across * st->down;
+    double *r = malloc(sizeof(double) * (size_t)cells);
+    double *rs = calloc((size_t)(count > 0 ? count : 1), sizeof(double));
+    double *ms = calloc((size_t)(count > 0 ? count : 1), sizeof(double));
+    or_residual_st(st, x, r);
+    for (long cell = 0; cell < cells; cell++) {
+        int b = body_of[cell];
+        if (b < 0) continue;
+        double weights = (double)st->e[cell] + (double)st->w[cell] + (double)st->n[cell] + (double)st->s[cell];
+        rs[b] += r[cell];
+        ms[b] += st->c[cell] * (1.0 - weights);
+    }
+    for (long cell = 0; cell < cells; cell++) {
+        int b = body_of[cell];
+        if (b < 0 || ms[b] <= 0.0) continue;
+        x[cell] -= (float)(rs[b] / ms[b]);
+    }
+    free(r);
+    free(rs);
+    free(ms);
+}
+
+void or_correct_body_means_flat(int across, int down, const uint8_t *water, const float *e, const float *w,
+                                const float *n, const float *s, const float *f, const double *c, float *x,
+                                const int32_t *body_of, int count)
+{
+    or_st st = {across, down, water, e, w, n, s, f, c};
+    or_correct_body_means(&st, x, body_of, count);
+}
+
+/* waterBodies (OC:670-697): four-connected labels, columns wrapping, numbered in the order of each
+ * body's first cell; -1 on land. Returns the count. */
+int or_water_bodies(const uint8_t *water, int across, int down, int32_t *body_of)
+{
+    long cells = (long)across * down;
+    int32_t *pending = malloc(sizeof(int32_t) * (size_t)(cells > 0 ? cells : 1));
+    int count = 0;
+    for (long i = 0; i < cells; i++) body_of[i] = -1;
+    for (long seed = 0; seed < cells; seed++) {
+        if (!water[seed] || body_of[seed] >= 0) continue;
+        long top = 0;
+        pending[top++] = (int32_t)seed;
+        body_of[seed] = count;
+        while (top > 0) {
+            long cell = pending[--top];
+            int row = (int)(cell / across);
+            int col = (int)(cell - (long)row * across);
+            long nb[4] = {(long)row * across + (col + 1 == across ? 0 : col + 1),
+                          (long)row * across + (col == 0 ? across - 1 : col - 1),
+                          row > 0 ? cell - across : -1, row + 1 < down ? cell + across : -1};
+            for (int k = 0; k < 4; k++) {
+                long next = nb[k];
+                if (next >= 0 && water[next] && body_of[next] < 0) {
+                    body_of[next] = count;
+                    pending[top++] = (int32_t)next;
+                }
+            }
+        }
+        count++;
+    }
+    free(pending);
+    return count;
 }
 
 /* vCycle (OC:740-781). levels[0]'s forcing is the cycle's right-hand side; x is level 0's values,
@@ -373,7 +446,7 @@ typedef struct {
     double relative_residual;
 } or_solution;
 
-/* solve (OC:473-493) with bodies = null and the processor's relax. x is start in, answer out. */
+/* solve (OC:473-493) with the processor's relax; with bodies, correctBodyMeans after each cycle. x is start in, answer out. */
 static or_solution or_solve_l(const or_ladder *L, float *x, double tol, int pole_is_wall)
 {
     const or_st *fin = &L->st[0];
@@ -387,6 +460,7 @@ static or_solution or_solve_l(const or_ladder *L, float *x, double tol, int pole
     else residual = 0.0;
     while (residual >= tol && cycles < OR_MOST_CYCLES) {
         or_vcycle_l(L, x, tol, pole_is_wall, 0);
+        if (L->body_of) or_correct_body_means(fin, x, L->body_of, L->body_count);
         cycles++;
         or_residual_st(fin, x, tmp);
         residual = or_largest(cells, tmp) / largest;
@@ -424,7 +498,8 @@ static double or_dot(long n, const double *a, const double *b)
     return s;
 }
 
-/* precondition (OC:610-622): one V-cycle on L z = balance from zero, coarsest 200 passes. */
+/* precondition (OC:610-622): one V-cycle on L z = balance from zero, coarsest 200 passes, then the
+ * bodies' means where there are bodies. */
 static void or_precondition(const or_ladder *L, const double *balance, double tol, int pole_is_wall,
                             float *zf, double *out)
 {
@@ -436,11 +511,13 @@ static void or_precondition(const or_ladder *L, const double *balance, double to
     corrected.st[0].f = forcing;
     memset(zf, 0, sizeof(float) * (size_t)cells);
     or_vcycle_l(&corrected, zf, tol, pole_is_wall, OR_PRECONDITIONER_COARSEST_PASSES);
+    if (L->body_of) or_correct_body_means(&corrected.st[0], zf, L->body_of, L->body_count);
     for (long i = 0; i < cells; i++) out[i] = (double)zf[i];
     free(forcing);
 }
 
-/* solveByKrylov (OC:548-607): right-preconditioned BiCGSTAB in binary64, bodies = null. */
+/* solveByKrylov (OC:548-607): right-preconditioned BiCGSTAB in binary64, the ladder's bodies in
+ * the preconditioner. */
 static or_solution or_solve_by_krylov_l(const or_ladder *L, float *x_io, double tol, int pole_is_wall,
                                         int most_iterations)
 {
@@ -538,6 +615,29 @@ int or_solve(int count, const int *acr, const int *dwn, const uint8_t *const *wa
     else if (mode == 1) sol = or_solve_with_rescue_l(&L, x, tol, pole_is_wall);
     else if (mode == 2) { or_vcycle_l(&L, x, tol, pole_is_wall, 0); sol.cycles = 1; }
     else if (mode == 3) sol = or_solve_by_krylov_l(&L, x, tol, pole_is_wall, OR_MOST_CYCLES);
+    else return -1;
+    if (relative_residual) *relative_residual = sol.relative_residual;
+    return sol.cycles;
+}
+
+/* or_solve with the finest grid's water bodies (body_of may be NULL) and, for solveByKrylov, its
+ * iteration limit (mostIterations, OC:548-556). */
+int or_solve_bodies(int count, const int *acr, const int *dwn, const uint8_t *const *water,
+                    const float *const *e, const float *const *w, const float *const *n,
+                    const float *const *s, const float *const *f, const double *const *c, float *x,
+                    double tol, int pole_is_wall, int mode, const int32_t *body_of, int body_count,
+                    int most_iterations, double *relative_residual)
+{
+    if (count < 1 || count > OR_MAX_LEVELS) return -1;
+    or_ladder L;
+    or_ladder_from(&L, count, acr, dwn, water, e, w, n, s, f, c);
+    L.body_of = body_of;
+    L.body_count = body_count;
+    or_solution sol = {0, 0.0};
+    if (mode == 0) sol = or_solve_l(&L, x, tol, pole_is_wall);
+    else if (mode == 1) sol = or_solve_with_rescue_l(&L, x, tol, pole_is_wall);
+    else if (mode == 2) { or_vcycle_l(&L, x, tol, pole_is_wall, 0); sol.cycles = 1; }
+    else if (mode == 3) sol = or_solve_by_krylov_l(&L, x, tol, pole_is_wall, most_iterations);
     else return -1;
     if (relative_residual) *relative_residual = sol.relative_residual;
     return sol.cycles;
@@ -798,4 +898,157 @@ void or_curl_forcing(int across, int down, double width_m, double height_m, cons
             forcing[cell] = (dtn_dx - dte_dy) * scale;
         }
     }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* The heat problem (OceanHeat.kt, "OH")                                                      */
+/* ------------------------------------------------------------------------------------------ */
+
+#define OR_ROTATION_RATE ((double)7.2921159e-5f) /* WorldScale.ROTATION_RATE_PER_S, a Float (WGC:302) */
+#define OR_EARTH_MEAN_RADIUS_M 6.371e6            /* OS:80 */
+
+/* BAROCLINIC_WAVE_SPEED_M_PER_S (OH:86-88): 2 beta_eq R^2 with Earth's equatorial beta, R = 240 km. */
+static double or_wave_speed(void)
+{
+    return 2.0 * (2.0 * OR_ROTATION_RATE / OR_EARTH_MEAN_RADIUS_M) * 240000.0 * 240000.0;
+}
+
+/* deformationRadiusMeters (OH:98-105). */
+double or_deformation_radius(double latitude_deg, double radius_m)
+{
+    double latitude = latitude_deg * M_PI / 180.0;
+    double coriolis = fabs(2.0 * OR_ROTATION_RATE * sin(latitude));
+    double beta = 2.0 * OR_ROTATION_RATE * cos(latitude) / radius_m;
+    double c = or_wave_speed();
+    double off_equator = coriolis == 0.0 ? INFINITY : c / coriolis;
+    double equatorial = beta <= 0.0 ? INFINITY : sqrt(c / (2.0 * beta));
+    return off_equator < equatorial ? off_equator : equatorial;
+}
+
+/* diffusivity (OH:128-129) with EDDY_VELOCITY_MPS (OH:112-114): K = V R_d, V fixed so that K is
+ * 2,500 m^2/s at 45 degrees. */
+double or_heat_diffusivity(double latitude_deg, double radius_m)
+{
+    double v = 2500.0 * 2.0 * OR_ROTATION_RATE * sin(45.0 * M_PI / 180.0) / or_wave_speed();
+    return v * or_deformation_radius(latitude_deg, radius_m);
+}
+
+/* bernoulli (OH:240-244): x / (e^x - 1), its series 1 - x/2 below 1e-3, zero above 700. */
+static double or_bernoulli(double x)
+{
+    if (fabs(x) < 1e-3) return 1.0 - x / 2.0;
+    if (x > 700.0) return 0.0;
+    return x / expm1(x);
+}
+
+/* faceWeight (OH:231-234). */
+static double or_face_weight(double k, double outward, double spacing)
+{
+    double peclet = outward * spacing / k;
+    return k / (spacing * spacing) * or_bernoulli(peclet);
+}
+
+/* corner (OH:262-281): psi at the corner between two columns on the north or south side of row:
+ * zero when any of the four cells around it is land (beyond a pole counts as not land), otherwise
+ * their mean, a cell beyond a pole taking minus the same column's value in row. */
+static double or_corner(const float *stream, const uint8_t *water, int across, int down, int row, int wc, int ec,
+                        int north)
+{
+    int other = north ? row - 1 : row + 1;
+#define OR_ON_LAND(r, c) ((r) >= 0 && (r) < down && !water[(long)(r) * across + (c)])
+    if (OR_ON_LAND(row, wc) || OR_ON_LAND(row, ec) || OR_ON_LAND(other, wc) || OR_ON_LAND(other, ec)) return 0.0;
+#undef OR_ON_LAND
+#define OR_AT(r, c) (((r) < 0 || (r) >= down) ? -(double)stream[(long)row * across + (c)] : (double)stream[(long)(r) * across + (c)])
+    return (OR_AT(row, wc) + OR_AT(row, ec) + OR_AT(other, wc) + OR_AT(other, ec)) / 4.0;
+#undef OR_AT
+}
+
+/* OceanHeat.stencil then withBalance (OH:151-228). The diffusivity is `constant_k` where that is
+ * positive, else diffusivity(latitude, radius_m). entrainment and subsurface may be NULL. Land keeps
+ * zero weights, centre 1 and forcing 0. */
+void or_heat_stencil(int across, int down, double dx, double dy, const uint8_t *water, const float *stream,
+                     const float *target, double relaxation_s, int with_target, double radius_m, double constant_k,
+                     const float *entrainment, const float *subsurface, float *e, float *w, float *n, float *s,
+                     double *c, float *f)
+{
+    long cells = (long)across * down;
+    double rate = 1.0 / relaxation_s;
+    double *along = malloc(sizeof(double) * (size_t)down);
+    double *boundary = malloc(sizeof(double) * (size_t)(down + 1));
+    for (int row = 0; row < down; row++) {
+        float lat = 90.0f - 180.0f * ((float)row + 0.5f) / (float)down; /* ClimateStage.latitudeOf */
+        along[row] = constant_k > 0.0 ? constant_k : or_heat_diffusivity((double)lat, radius_m);
+    }
+    for (int b = 0; b <= down; b++)
+        boundary[b] = constant_k > 0.0 ? constant_k : or_heat_diffusivity(90.0 - 180.0 * b / down, radius_m);
+    for (long i = 0; i < cells; i++) { e[i] = w[i] = n[i] = s[i] = 0.0f; c[i] = 1.0; f[i] = 0.0f; }
+    for (int row = 0; row < down; row++) {
+        for (int col = 0; col < across; col++) {
+            long cell = (long)row * across + col;
+            if (!water[cell]) continue;
+            int ce = col + 1 == across ? 0 : col + 1;
+            int cw = col == 0 ? across - 1 : col - 1;
+            double ne = or_corner(stream, water, across, down, row, col, ce, 1);
+            double se = or_corner(stream, water, across, down, row, col, ce, 0);
+            double nw = or_corner(stream, water, across, down, row, cw, col, 1);
+            double sw = or_corner(stream, water, across, down, row, cw, col, 0);
+            double out_e = -(ne - se) / dy;
+            double out_w = (nw - sw) / dy;
+            double out_n = (ne - nw) / dx;
+            double out_s = -(se - sw) / dx;
+            double we = water[(long)row * across + ce] ? or_face_weight(along[row], out_e, dx) : 0.0;
+            double ww = water[(long)row * across + cw] ? or_face_weight(along[row], out_w, dx) : 0.0;
+            double wn = (row > 0 && water[cell - across]) ? or_face_weight(boundary[row], out_n, dy) : 0.0;
+            double ws = (row + 1 < down && water[cell + across]) ? or_face_weight(boundary[row + 1], out_s, dy) : 0.0;
+            double ent = entrainment ? (double)entrainment[cell] : 0.0;
+            double centre = we + ww + wn + ws + rate + ent;
+            e[cell] = (float)(we / centre);
+            w[cell] = (float)(ww / centre);
+            n[cell] = (float)(wn / centre);
+            s[cell] = (float)(ws / centre);
+            c[cell] = centre;
+            if (with_target) {
+                double balance = (double)(-target[cell]) * rate - ent * (subsurface ? (double)subsurface[cell] : 0.0);
+                f[cell] = (float)(balance / centre);
+            }
+        }
+    }
+    free(along);
+    free(boundary);
+}
+
+/* resample and sample (OS:1199-1223): bilinear at each target cell centre, columns wrapping, rows
+ * clamped, all in binary32. */
+void or_resample(const float *field, int fa, int fd, int ta, int td, float *out)
+{
+    for (int row = 0; row < td; row++) {
+        float from_row = ((float)row + 0.5f) * (float)fd / (float)td - 0.5f;
+        for (int col = 0; col < ta; col++) {
+            float from_col = ((float)col + 0.5f) * (float)fa / (float)ta - 0.5f;
+            int left = (int)floorf(from_col);
+            float ab = from_col - (float)left;
+            int wc = ((left % fa) + fa) % fa;
+            int ec = wc + 1 == fa ? 0 : wc + 1;
+            float cr = from_row < 0.0f ? 0.0f : (from_row > (float)(fd - 1) ? (float)(fd - 1) : from_row);
+            int above = (int)cr < fd - 1 ? (int)cr : fd - 1;
+            int below = above + 1 < fd - 1 ? above + 1 : fd - 1;
+            float db = cr - (float)above;
+            float top = field[(long)above * fa + wc] * (1.0f - ab) + field[(long)above * fa + ec] * ab;
+            float bottom = field[(long)below * fa + wc] * (1.0f - ab) + field[(long)below * fa + ec] * ab;
+            out[(long)row * ta + col] = top * (1.0f - db) + bottom * db;
+        }
+    }
+}
+
+/* coarseMean (OS:857-860): restrict of the binary64 promotion, rounded to binary32. */
+void or_coarse_mean(const float *fine, int fa, int fd, int ca, int cd, float *coarse)
+{
+    long nf = (long)fa * fd, nc = (long)ca * cd;
+    double *f64 = malloc(sizeof(double) * (size_t)nf);
+    double *c64 = malloc(sizeof(double) * (size_t)nc);
+    for (long i = 0; i < nf; i++) f64[i] = (double)fine[i];
+    or_restrict(f64, fa, fd, ca, cd, c64);
+    for (long i = 0; i < nc; i++) coarse[i] = (float)c64[i];
+    free(f64);
+    free(c64);
 }
