@@ -32,6 +32,21 @@ def test_library_exports_every_declared_entry_point():
     assert set(names) == set(cg.SIGNATURES), "the ctypes table and the header disagree"
 
 
+def test_jni_shim_and_kotlin_class_agree_with_each_other_and_the_header():
+    """The JNI binding cannot be compiled here (no JDK): its C natives must be exactly the Kotlin
+    class's `external fun`s, and every library call it makes must be declared in the header."""
+    jni = Path(__file__).resolve().parents[1] / "cartogenesis_amd" / "jni"
+    c_text = (jni / "cgocean_jni.c").read_text()
+    kt_text = (jni / "HipOcean.kt").read_text()
+    natives = set(re.findall(r"Java_com_cartogenesis_desktop_HipOcean_(\w+)\(", c_text))
+    externals = set(re.findall(r"external fun (\w+)\(", kt_text))
+    assert natives and natives == externals
+    assert "package com.cartogenesis.desktop" in kt_text and "class HipOcean" in kt_text
+    body = re.sub(r"/\*.*?\*/", "", c_text, flags=re.S)
+    called = set(re.findall(r"\b(cg_\w+)\s*\(", body)) - {"cg_solution"}
+    assert called and called <= set(declared_functions())
+
+
 def test_open_without_a_device_declines_with_a_reason():
     res = HipOcean.create_or_null(0)
     if res.accelerator is not None:
