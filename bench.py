@@ -65,6 +65,73 @@ def cpu_baseline(levels, budget_s: float, threads: int):
                       f"binary64 C restatement of OceanCirculation.solve, {threads} OpenMP threads"}
 
 
+def batched_year(acc, prob, tol: float, reps: int, warmup: int):
+    """The 12 monthly forcings of the benchmark world solved together in one plan (one launch per
+    kernel for all twelve), from zero to tol each: solves/s = 12 x reps / wall time."""
+    from cartogenesis_amd.ocean import BatchPlan
+    from cartogenesis_amd.problem import monthly_forcings
+
+    log("bench: batched year (12 monthly forcings of the 1024x2048 world)")
+    forcings = monthly_forcings(prob)
+    plan = BatchPlan(acc, prob.levels, len(forcings))
+    try:
+        for i, f in enumerate(forcings):
+            plan.set_forcing(i, f)
+        for _ in range(max(1, warmup // 2)):
+            infos = plan.solve_all(tol)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            infos = plan.solve_all(tol)
+        elapsed = time.perf_counter() - t0
+    finally:
+        plan.close()
+    value = len(forcings) * reps / elapsed
+    log(f"bench: batched year {value:.1f} solves/s, {1000 * elapsed / reps:.2f} ms per year")
+    return {
+        "workload": "12 monthly forcings (band amplitude 0.1 cos(2 pi m / 12), shifted 10 deg sin(2 pi m / 12)) of "
+                    "the 1024x2048 world, solved together in one plan from zero to tol each",
+        "instances": len(forcings), "reps": reps,
+        "solves_per_s": round(value, 3),
+        "ms_per_batch": round(1000.0 * elapsed / reps, 4),
+        "cycles": [int(i.cycles) for i in infos],
+        "max_relative_residual": max(float(i.relative_residual) for i in infos),
+    }
+
+
+def batched_worlds(acc, tol: float):
+    """Four worlds of uneven size, twelve months each: one plan of twelve instances per world, the
+    worlds one after another; solves/s = 48 / wall time."""
+    from cartogenesis_amd.ocean import BatchPlan
+    from cartogenesis_amd.problem import circulation_problem, monthly_forcings
+
+    shapes = [(512, 1, 0.20), (768, 2, 0.27), (1024, 3, 0.33), (1024, 4, 0.40)]
+    plans = []
+    log("bench: building the four-world year")
+    for rows, seed, land in shapes:
+        prob = circulation_problem(rows, seed=seed, land_fraction=land)
+        plan = BatchPlan(acc, prob.levels, 12)
+        for i, f in enumerate(monthly_forcings(prob)):
+            plan.set_forcing(i, f)
+        plans.append(plan)
+    try:
+        for plan in plans:
+            plan.solve_all(tol)
+        t0 = time.perf_counter()
+        per = []
+        for plan in plans:
+            t1 = time.perf_counter()
+            infos = plan.solve_all(tol)
+            per.append(round(1000.0 * (time.perf_counter() - t1), 3))
+        elapsed = time.perf_counter() - t0
+    finally:
+        for plan in plans:
+            plan.close()
+    value = 48 / elapsed
+    log(f"bench: four-world year {value:.1f} solves/s")
+    return {"workload": "12 months x worlds 512x1024, 768x1536, 1024x2048, 1024x2048 (seeds 1-4, land 20/27/33/40%)",
+            "solves_per_s": round(value, 3), "ms_per_world": per}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +149,11 @@ def main() -> int:
                     help="only run a 1 GiB device-to-device copy six times (for scaling HBM counters) and exit")
     ap.add_argument("--level-passes", default="22222222",
                     help="passes per level of the binary32 cycle, one hex digit per level, finest last")
+    ap.add_argument("--batch-reps", type=int, default=5,
+                    help="timed repetitions of the batched year (12 monthly forcings solved together); 0 skips it")
+    ap.add_argument("--worlds", action="store_true",
+                    help="also time the four-world year: 12 months on each of 512x1024, 768x1536, 1024x2048 and "
+                         "1024x2048 (seeds 1-4, land 20/27/33/40%%)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -150,6 +222,11 @@ def main() -> int:
     log(f"bench: outer residual {us_outer:.2f} us ({outer_gbs:.0f} GB/s algorithmic), finest up leg {us_up:.2f} us "
         f"({up_gbs:.0f} GB/s), finest down leg {us_down:.2f} us")
 
+    batched = None
+    if args.batch_reps > 0:
+        batched = batched_year(acc, prob, args.tol, args.batch_reps, args.warmup)
+    worlds = batched_worlds(acc, args.tol) if args.worlds else None
+
     err = None
     cpu = None
     if not args.no_oracle_check or not args.no_cpu:
@@ -213,6 +290,10 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
     }
+    if batched:
+        out["batched"] = batched
+    if worlds:
+        out["batched_worlds"] = worlds
     print(json.dumps(out), flush=True)
     plan.close()
     acc.close()

@@ -52,6 +52,10 @@ SIGNATURES = {
                               ctypes.POINTER(CgSolution), _vp]),
     "cg_ocean_plan_create": (_int, [_vp, ctypes.POINTER(_u64), _int, _int, ctypes.POINTER(_vp)]),
     "cg_ocean_plan_destroy": (_int, [_vp]),
+    "cg_ocean_plan_create_batch": (_int, [_vp, ctypes.POINTER(_u64), _int, _int, _int, ctypes.POINTER(_vp)]),
+    "cg_ocean_plan_set_forcing_at": (_int, [_vp, _int, _f32p]),
+    "cg_ocean_plan_solve_all": (_int, [_vp, _dbl, _int, ctypes.POINTER(CgSolution)]),
+    "cg_ocean_plan_read_at": (_int, [_vp, _int, _vp, _vp]),
     "cg_ocean_plan_set_forcing": (_int, [_vp, _f32p]),
     "cg_ocean_plan_solve": (_int, [_vp, _dbl, _int, _int, ctypes.POINTER(CgSolution)]),
     "cg_ocean_plan_solve_batch": (_int, [ctypes.POINTER(_vp), _int, _dbl, _int, ctypes.POINTER(CgSolution)]),

@@ -103,6 +103,10 @@ struct TileExtra {
     const float *xc;        // next grid's values
     int ca, cd;             // next grid's shape
     float beyond;           // -1 for a wall at the poles
+    // Instances: blockIdx.y selects one; each instance's grids sit fs (this grid) and cs (the next
+    // grid) floats after the previous instance's; an instance whose active flag is 0 is skipped.
+    long fs, cs;
+    const int *active;
 };
 
 template <int NH, int MODE>
@@ -124,6 +128,15 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
     static_assert(TH % 2 == 0, "tiles start on even rows");
     const int across = L.across, down = L.down;
     const int lane = threadIdx.x & 63;
+    const int inst = blockIdx.y;
+    if (ex.active && !ex.active[inst]) return;
+    if (inst) {
+        f += inst * ex.fs;
+        if (xin) xin += inst * ex.fs;
+        xout += inst * ex.fs;
+        if (MODE == TILE_DOWN) ex.fc += inst * ex.cs;
+        if (MODE == TILE_UP) ex.xc += inst * ex.cs;
+    }
     const int wave = xcd_block() * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int tiles_x = (across + IW - 1) / IW;
     const int tiles_y = (down + TH - 1) / TH;
@@ -591,10 +604,30 @@ __device__ __forceinline__ double dpp_double(double v) {
 // kRowsPerBlock rows of blockIdx.y, and loads a window of kRowsPerBlock + 2 rows once (north and
 // south neighbours come from the window); east/west neighbours come from the adjacent lanes by DPP,
 // and only a wave's edge lanes (and the pair at the wrap) load theirs. Needs an even width.
+// Instances of the outer step: blockIdx.z selects one; its binary32 grids sit s32 floats, its
+// binary64 grids s64 doubles and its reduction slots red_stride slots after the previous one's; an
+// instance whose active flag is 0 is skipped.
+struct OuterBatch {
+    long s32, s64;
+    int red_stride;
+    const int *active;
+};
+
 __global__ __launch_bounds__(256) void k_outer_residual(LevelArgs L, const float *__restrict__ f,
                                                         const double *__restrict__ x, const float *__restrict__ delta,
                                                         double *__restrict__ xnew, float *__restrict__ fc,
-                                                        MaxReduce red) {
+                                                        MaxReduce red, OuterBatch ob) {
+    const int inst = blockIdx.z;
+    if (ob.active && !ob.active[inst]) return;
+    if (inst) {
+        f += inst * ob.s32;
+        x += inst * ob.s64;
+        if (delta) delta += inst * ob.s32;
+        xnew += inst * ob.s64;
+        fc += inst * ob.s32;
+        red.out += inst * ob.red_stride;
+        if (red.clear) red.clear += inst * ob.red_stride;
+    }
     const int across = L.across, down = L.down, half = across >> 1;
     const int lane = threadIdx.x & 63;
     const int c = blockIdx.x * 256 + threadIdx.x;
@@ -746,6 +779,10 @@ struct BottomArgs {
     BottomLevel lv[kBottomMaxLevels];
     const float *f_in;  // forcing of the first level (global)
     float *x_out;       // values of the first level (global)
+    // Instances: workgroup b solves instance b, whose f_in and x_out sit b strides further; an
+    // instance whose active flag is 0 is skipped.
+    long f_stride, x_stride;
+    const int *active;
     const int *idx;     // coarsest water cells, and M = -(I - W)^-1 column-major (nw x nw)
     const float *Mt;
     int nw;
@@ -793,6 +830,10 @@ __device__ __forceinline__ void lds_relax_pass(const BottomView &L) {
 }
 
 __global__ __launch_bounds__(1024) void k_bottom(BottomArgs a) {
+    const int inst = blockIdx.x;
+    if (a.active && !a.active[inst]) return;
+    const float *f_in = a.f_in + inst * a.f_stride;
+    float *x_out = a.x_out + inst * a.x_stride;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ float partial[1024];
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
@@ -826,7 +867,7 @@ __global__ __launch_bounds__(1024) void k_bottom(BottomArgs a) {
     {
         const int cells = a.lv[0].across * a.lv[0].down;
         float *f = reinterpret_cast<float *>(smem + a.lv[0].off_f);
-        for (int c = threadIdx.x; c < cells; c += blockDim.x) f[c] = a.f_in[c];
+        for (int c = threadIdx.x; c < cells; c += blockDim.x) f[c] = f_in[c];
     }
     __syncthreads();
     // Down: relax twice, then the binary32 residual restricted with g (k_residual_restrict32's sums).
@@ -932,7 +973,7 @@ __global__ __launch_bounds__(1024) void k_bottom(BottomArgs a) {
     {
         const int cells = a.lv[0].across * a.lv[0].down;
         const float *x = reinterpret_cast<const float *>(smem + a.lv[0].off_x);
-        for (int c = threadIdx.x; c < cells; c += blockDim.x) a.x_out[c] = x[c];
+        for (int c = threadIdx.x; c < cells; c += blockDim.x) x_out[c] = x[c];
     }
 }
 

@@ -25,6 +25,7 @@ constexpr int kMostCycles = 200;          // OC:116
 constexpr double kCoarsestShare = 0.1;    // OC:110
 constexpr int kPrecondCoarsestPasses = 200;  // OC:629
 constexpr int kKrylovRestarts = 3;        // OC:535
+constexpr int kMaxInstances = 64;          // right-hand sides one plan solves together
 constexpr int kDirectMaxWater = 2048;     // largest coarsest grid solved by a dense inverse
 constexpr int kRefinedMostSteps = 1000;
 
@@ -198,42 +199,47 @@ struct TimeScope {
 // per-cell weights the colour kernel runs in place on xout after copying xin.
 template <int TH, int MODE, int NH>
 hipError_t launch_tile_mode(hipStream_t s, const LevelArgs &L, const float *f, const float *xin, float *xout, bool zero,
-                            const TileExtra &ex) {
+                            const TileExtra &ex, int instances) {
     const int iw = TileGeom<NH, MODE>::IW;
     const long waves = (long)((L.across + iw - 1) / iw) * ((L.down + TH - 1) / TH);
-    const unsigned blocks = (unsigned)((waves + 3) / 4);
-    if (zero) hipLaunchKernelGGL((k_relax_tile<NH, TH, true, MODE>), dim3(blocks), dim3(256), 0, s, L, f, xin, xout, ex);
-    else hipLaunchKernelGGL((k_relax_tile<NH, TH, false, MODE>), dim3(blocks), dim3(256), 0, s, L, f, xin, xout, ex);
+    unsigned blocks = (unsigned)((waves + 3) / 4);
+    // With several instances (grid y), a row of blocks a multiple of the XCD count keeps each
+    // block's XCD its x index modulo 8, which xcd_block's renumbering assumes.
+    if (instances > 1) blocks = (blocks + kXcds - 1) / kXcds * kXcds;
+    const dim3 grid(blocks, instances > 1 ? instances : 1);
+    if (zero) hipLaunchKernelGGL((k_relax_tile<NH, TH, true, MODE>), grid, dim3(256), 0, s, L, f, xin, xout, ex);
+    else hipLaunchKernelGGL((k_relax_tile<NH, TH, false, MODE>), grid, dim3(256), 0, s, L, f, xin, xout, ex);
     return hipGetLastError();
 }
 
 template <int TH>
 hipError_t launch_tile(hipStream_t s, const LevelArgs &L, const float *f, const float *xin, float *xout, int nh,
-                       bool zero, int mode, const TileExtra &ex) {
+                       bool zero, int mode, const TileExtra &ex, int k) {
     // The fused down and up legs run 2, 4 or 6 passes (4, 8 or 12 half-sweeps) in their one launch.
     switch (mode) {
         case TILE_DOWN:
-            if (nh == 12) return launch_tile_mode<TH, TILE_DOWN, 12>(s, L, f, xin, xout, true, ex);
-            if (nh == 8) return launch_tile_mode<TH, TILE_DOWN, 8>(s, L, f, xin, xout, true, ex);
-            return launch_tile_mode<TH, TILE_DOWN, 4>(s, L, f, xin, xout, true, ex);
+            if (nh == 12) return launch_tile_mode<TH, TILE_DOWN, 12>(s, L, f, xin, xout, true, ex, k);
+            if (nh == 8) return launch_tile_mode<TH, TILE_DOWN, 8>(s, L, f, xin, xout, true, ex, k);
+            return launch_tile_mode<TH, TILE_DOWN, 4>(s, L, f, xin, xout, true, ex, k);
         case TILE_UP:
-            if (nh == 12) return launch_tile_mode<TH, TILE_UP, 12>(s, L, f, xin, xout, false, ex);
-            if (nh == 8) return launch_tile_mode<TH, TILE_UP, 8>(s, L, f, xin, xout, false, ex);
-            return launch_tile_mode<TH, TILE_UP, 4>(s, L, f, xin, xout, false, ex);
+            if (nh == 12) return launch_tile_mode<TH, TILE_UP, 12>(s, L, f, xin, xout, false, ex, k);
+            if (nh == 8) return launch_tile_mode<TH, TILE_UP, 8>(s, L, f, xin, xout, false, ex, k);
+            return launch_tile_mode<TH, TILE_UP, 4>(s, L, f, xin, xout, false, ex, k);
         default:
-            if (nh == 4) return launch_tile_mode<TH, TILE_PLAIN, 4>(s, L, f, xin, xout, zero, ex);
-            return launch_tile_mode<TH, TILE_PLAIN, 2>(s, L, f, xin, xout, zero, ex);
+            if (nh == 4) return launch_tile_mode<TH, TILE_PLAIN, 4>(s, L, f, xin, xout, zero, ex, k);
+            return launch_tile_mode<TH, TILE_PLAIN, 2>(s, L, f, xin, xout, zero, ex, k);
     }
 }
 
 hipError_t tile_dispatch(cg_ctx *ctx, const LevelArgs &L, const float *f, const float *xin, float *xout, int nh,
-                         bool zero, int mode = TILE_PLAIN, const TileExtra &ex = TileExtra{}, int rows = 0) {
+                         bool zero, int mode = TILE_PLAIN, const TileExtra &ex = TileExtra{}, int rows = 0,
+                         int instances = 1) {
     switch (rows ? rows : ctx->tile_rows) {
-        case 4: return launch_tile<4>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex);
-        case 8: return launch_tile<8>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex);
-        case 24: return launch_tile<24>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex);
-        case 32: return launch_tile<32>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex);
-        default: return launch_tile<16>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex);
+        case 4: return launch_tile<4>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex, instances);
+        case 8: return launch_tile<8>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex, instances);
+        case 24: return launch_tile<24>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex, instances);
+        case 32: return launch_tile<32>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex, instances);
+        default: return launch_tile<16>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex, instances);
     }
 }
 
@@ -293,6 +299,16 @@ struct cg_plan {
     cg_ctx *ctx = nullptr;
     int n = 0;
     int pole_is_wall = 1;
+    // Instances: K right-hand sides on one ladder. Every per-instance grid array holds K copies,
+    // each a grid with its zero row (vstride(l) floats); the binary64 answers K copies of the finest
+    // cells. The refined method's reduction slots of instance i and binary64 parity q are group
+    // red_base + 2 i + q of d_u64 (red_base 0 for one instance, 4 otherwise).
+    int K = 1;
+    int red_base = 0;
+    unsigned long long *h_red = nullptr;  // pinned copy of d_u64
+    int *d_active = nullptr, *h_active = nullptr;  // per instance: still solving (K > 1)
+    std::vector<double> largest_k;        // per instance max |f c|, < 0 when unknown
+    std::vector<int> ans_par;             // per instance: the binary64 parity holding its answer
     PlanLevel lv[kMaxLevels];
     float *f0 = nullptr;          // the finest forcing
     double *r64 = nullptr;        // finest-size binary64 scratch
@@ -368,6 +384,9 @@ void free_plan(cg_plan *p) {
     dfree(p->x64[0]);
     dfree(p->x64[1]);
     dfree(p->d_u64);
+    dfree(p->d_active);
+    if (p->h_red) (void)hipHostFree(p->h_red);
+    if (p->h_active) (void)hipHostFree(p->h_active);
     dfree(p->d_passes);
     dfree(p->d_idx);
     dfree(p->d_Mt);
@@ -530,10 +549,10 @@ unsigned long long group_max(const unsigned long long *slots) {
 // Reads the maximum held in slot group `group` (kMaxSlots slots at d_u64 + group * kMaxSlots).
 int read_u64(cg_plan *p, int group, unsigned long long *v) {
     cg_ctx *ctx = p->ctx;
-    HIPCHK(ctx, hipMemcpyAsync(ctx->h_pinned + group * kMaxSlots, p->d_u64 + group * kMaxSlots,
+    HIPCHK(ctx, hipMemcpyAsync(p->h_red + group * kMaxSlots, p->d_u64 + group * kMaxSlots,
                                kMaxSlots * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    *v = group_max(ctx->h_pinned + group * kMaxSlots);
+    *v = group_max(p->h_red + group * kMaxSlots);
     return CG_OK;
 }
 
@@ -547,6 +566,9 @@ double bits_to_double(unsigned long long b) {
 inline dim3 grid2d(const Stencil *st) {
     return dim3(blocks_for(st->across, 256), blocks_for(st->down, kRowsPerBlock));
 }
+
+// Floats between two instances' copies of a grid: the grid and its zero row.
+inline long vstride(const Stencil *st) { return (long)st->across * (st->down + 1); }
 
 // A reduction into slot group `group`, zeroed first by a fill kernel.
 MaxReduce max_into(cg_plan *p, int group) {
@@ -576,6 +598,27 @@ int plan_largest(cg_plan *p, double *out) {
     *out = p->largest;
     return CG_OK;
 }
+
+// max |f c| of instance i's finest forcing, cached until its forcing changes.
+int plan_largest_at(cg_plan *p, int i, double *out) {
+    if (p->K == 1) return plan_largest(p, out);
+    if (p->largest_k[i] >= 0.0) {
+        *out = p->largest_k[i];
+        return CG_OK;
+    }
+    cg_ctx *ctx = p->ctx;
+    const Stencil *st = p->lv[0].st;
+    const MaxReduce red = max_into(p, 2);
+    hipLaunchKernelGGL(k_balance_max, grid2d(st), dim3(256), 0, ctx->stream, st->args(), p->f0 + i * vstride(st), red);
+    HIPCHK(ctx, hipGetLastError());
+    unsigned long long b;
+    int rc = read_u64(p, 2, &b);
+    if (rc) return rc;
+    p->largest_k[i] = bits_to_double(b);
+    *out = p->largest_k[i];
+    return CG_OK;
+}
+
 
 // ---- the reference method ---------------------------------------------------------------------
 
@@ -886,10 +929,13 @@ int vcycle32_fused(cg_plan *p, bool timed, const float *in0, float *out0) {
         ex.cwater = C.st->water;
         ex.g = L.g;
         ex.fc = C.f;
+        ex.fs = vstride(L.st);
+        ex.cs = vstride(C.st);
+        ex.active = p->K > 1 ? p->d_active : nullptr;
         TimeScope ts(ctx, timed && l == 0, kKindSmoothFromZero);
         const float *fl = (l == 0 && in0) ? in0 : L.f;
         hipError_t e = tile_dispatch(ctx, L.st->args(), fl, L.x[L.cur], L.x[1 - L.cur], level_half_sweeps(ctx, l), true,
-                                     TILE_DOWN, ex, level_tile_rows(ctx, L.st));
+                                     TILE_DOWN, ex, level_tile_rows(ctx, L.st), p->K);
         if (e != hipSuccess) {
             set_err(ctx, "down-leg launch failed: %s", hipGetErrorString(e));
             return CG_ERR_DEVICE;
@@ -931,7 +977,10 @@ int vcycle32_fused(cg_plan *p, bool timed, const float *in0, float *out0) {
     ba.nw = p->d_Mt ? p->nw : 0;
     ba.coarsest_passes = (p->nw == 0) ? 0 : kPrecondCoarsestPasses;
     ba.beyond = beyond;
-    hipLaunchKernelGGL(k_bottom, dim3(1), dim3(1024), p->bottom_lds, s, ba);
+    ba.f_stride = vstride(B0.st);
+    ba.x_stride = vstride(B0.st);
+    ba.active = p->K > 1 ? p->d_active : nullptr;
+    hipLaunchKernelGGL(k_bottom, dim3(p->K), dim3(1024), p->bottom_lds, s, ba);
     for (int l = b - 1; l >= 0; --l) {
         PlanLevel &L = p->lv[l];
         PlanLevel &C = p->lv[l + 1];
@@ -940,11 +989,14 @@ int vcycle32_fused(cg_plan *p, bool timed, const float *in0, float *out0) {
         ex.ca = C.st->across;
         ex.cd = C.st->down;
         ex.beyond = beyond;
+        ex.fs = vstride(L.st);
+        ex.cs = vstride(C.st);
+        ex.active = p->K > 1 ? p->d_active : nullptr;
         TimeScope ts(ctx, timed && l == 0, kKindSmooth);
         const float *fl = (l == 0 && in0) ? in0 : L.f;
         float *xo = (l == 0 && out0) ? out0 : L.x[1 - L.cur];
         hipError_t e = tile_dispatch(ctx, L.st->args(), fl, L.x[L.cur], xo, level_half_sweeps(ctx, l), false, TILE_UP,
-                                     ex, level_tile_rows(ctx, L.st));
+                                     ex, level_tile_rows(ctx, L.st), p->K);
         if (e != hipSuccess) {
             set_err(ctx, "up-leg launch failed: %s", hipGetErrorString(e));
             return CG_ERR_DEVICE;
@@ -1018,11 +1070,16 @@ int outer_step(cg_plan *p, const float *delta, float *fc = nullptr) {
     // Slot x64cur receives the maximum (zeroed by the previous step); the other slot is zeroed for
     // the next step.
     MaxReduce red;
-    red.out = p->d_u64 + p->x64cur * kMaxSlots;
-    red.clear = p->d_u64 + (1 - p->x64cur) * kMaxSlots;
-    const dim3 grid(blocks_for(st->across / 2, 256), blocks_for(st->down, kRowsPerBlock));
+    red.out = p->d_u64 + (p->red_base + p->x64cur) * kMaxSlots;
+    red.clear = p->d_u64 + (p->red_base + 1 - p->x64cur) * kMaxSlots;
+    OuterBatch ob{};
+    ob.s32 = vstride(st);
+    ob.s64 = st->cells;
+    ob.red_stride = 2 * kMaxSlots;
+    ob.active = p->K > 1 ? p->d_active : nullptr;
+    const dim3 grid(blocks_for(st->across / 2, 256), blocks_for(st->down, kRowsPerBlock), p->K);
     hipLaunchKernelGGL(k_outer_residual, grid, dim3(256), 0, ctx->stream, st->args(), p->f0,
-                       p->x64[p->x64cur], delta, p->x64[1 - p->x64cur], fc ? fc : p->lv[0].f, red);
+                       p->x64[p->x64cur], delta, p->x64[1 - p->x64cur], fc ? fc : p->lv[0].f, red, ob);
     HIPCHK(ctx, hipGetLastError());
     p->x64cur = 1 - p->x64cur;
     return CG_OK;
@@ -1045,8 +1102,9 @@ int build_step_graphs(cg_plan *p) {
         int rc = vcycle32(p, false);
         if (!rc) rc = outer_step(p, p->lv[0].x[p->lv[0].cur]);
         if (!rc) {
-            hipError_t e = hipMemcpyAsync(ctx->h_pinned, p->d_u64, 2 * kMaxSlots * sizeof(unsigned long long),
-                                          hipMemcpyDeviceToHost, ctx->stream);
+            hipError_t e = hipMemcpyAsync(p->h_red + p->red_base * kMaxSlots, p->d_u64 + p->red_base * kMaxSlots,
+                                          2 * p->K * kMaxSlots * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                          ctx->stream);
             if (e != hipSuccess) {
                 set_err(ctx, "capture of the residual readback failed: %s", hipGetErrorString(e));
                 rc = CG_ERR_DEVICE;
@@ -1080,43 +1138,79 @@ int build_step_graphs(cg_plan *p) {
     return CG_OK;
 }
 
-int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *info, const volatile int *cancel) {
+// The refined method over every instance of the plan; infos[i] receives instance i's result.
+// Instance i stops at the first check where its residual is under tol (or not a number); with
+// several instances its kernels then skip it (the active flags) while the others go on.
+int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *infos, const volatile int *cancel) {
     cg_ctx *ctx = p->ctx;
-    double largest;
-    int rc = plan_largest(p, &largest);
-    if (rc) return rc;
+    const int K = p->K;
     const Stencil *st = p->lv[0].st;
+    std::vector<double> largest(K), res(K), rate(K, -1.0);
+    std::vector<int> cycles(K, 0);
+    std::vector<char> on(K, 1);
+    int rc;
+    for (int i = 0; i < K; ++i)
+        if ((rc = plan_largest_at(p, i, &largest[i]))) return rc;
     if (!warm || !p->x64_valid) {
-        HIPCHK(ctx, hipMemsetAsync(p->x64[p->x64cur], 0, sizeof(double) * st->cells, ctx->stream));
+        HIPCHK(ctx, hipMemsetAsync(p->x64[p->x64cur], 0, sizeof(double) * st->cells * K, ctx->stream));
     }
     p->x64_valid = true;
-    hipLaunchKernelGGL(k_fill_u64, dim3(2), dim3(kMaxSlots), 0, ctx->stream, p->d_u64, 2 * kMaxSlots, 0ull);
+    if (K > 1) {
+        for (int i = 0; i < K; ++i) p->h_active[i] = 1;
+        HIPCHK(ctx, hipMemcpyAsync(p->d_active, p->h_active, sizeof(int) * K, hipMemcpyHostToDevice, ctx->stream));
+    }
+    unsigned long long *const groups = p->d_u64 + p->red_base * kMaxSlots;
+    hipLaunchKernelGGL(k_fill_u64, dim3(2 * K), dim3(kMaxSlots), 0, ctx->stream, groups, 2 * K * kMaxSlots, 0ull);
+    auto slot_max = [&](int i, int par) { return group_max(p->h_red + (p->red_base + 2 * i + par) * kMaxSlots); };
+    auto read_all = [&]() -> int {
+        HIPCHK(ctx, hipMemcpyAsync(p->h_red + p->red_base * kMaxSlots, groups, 2 * K * kMaxSlots * sizeof(unsigned long long),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        return CG_OK;
+    };
     const int first_slot = p->x64cur;
     if ((rc = outer_step(p, nullptr))) return rc;
-    unsigned long long b;
-    if ((rc = read_u64(p, first_slot, &b))) return rc;
-    double res = largest > 0.0 ? bits_to_double(b) / largest : 0.0;
+    if ((rc = read_all())) return rc;
+    p->ans_par.assign(K, p->x64cur);
+    int left = 0;
+    for (int i = 0; i < K; ++i) {
+        res[i] = largest[i] > 0.0 ? bits_to_double(slot_max(i, first_slot)) / largest[i] : 0.0;
+        on[i] = res[i] >= tol;
+        left += on[i];
+    }
     const bool graph = ctx->use_graph && !ctx->time_kernels;
-    if (graph && res >= tol) {
+    if (graph && left > 0) {
         if ((rc = build_step_graphs(p))) return rc;
     }
+    auto publish_active = [&]() -> int {
+        if (K == 1) return CG_OK;
+        for (int i = 0; i < K; ++i) p->h_active[i] = on[i];
+        HIPCHK(ctx, hipMemcpyAsync(p->d_active, p->h_active, sizeof(int) * K, hipMemcpyHostToDevice, ctx->stream));
+        return CG_OK;
+    };
+    if ((rc = publish_active())) return rc;
     // Steps run in batches without a host read in between: the batch is the number of steps the
-    // last observed convergence factor predicts are still needed, less one (at least one, at most
-    // 8), so the host waits on the device about twice per solve rather than once per step. The
-    // policy is the same with and without graphs, so both give the same steps and bits.
-    int cycles = 0;
-    double rate = -1.0;  // observed residual reduction per step
-    while (res >= tol && cycles < kRefinedMostSteps) {
+    // slowest instance's last observed convergence factor predicts are still needed, less one (at
+    // least one, at most 8), so the host waits on the device about twice per solve rather than once
+    // per step. The policy is the same with and without graphs, so both give the same steps and bits.
+    int steps = 0;
+    int status = CG_OK;
+    while (left > 0 && steps < kRefinedMostSteps) {
         if (cancel && *cancel) {
-            info->status = CG_CANCELLED;
+            status = CG_CANCELLED;
             break;
         }
         int batch = 1;
-        if (rate > 0.0 && rate < 0.95 && res > 0.0) {
-            const double need = std::ceil(std::log(tol / res) / std::log(rate));
-            batch = (int)std::max(1.0, std::min(8.0, need - 1.0));
+        for (int i = 0; i < K; ++i) {
+            if (!on[i]) continue;
+            int want = 1;
+            if (rate[i] > 0.0 && rate[i] < 0.95 && res[i] > 0.0) {
+                const double need = std::ceil(std::log(tol / res[i]) / std::log(rate[i]));
+                want = (int)std::max(1.0, std::min(8.0, need - 1.0));
+            }
+            batch = i == 0 || want > batch ? want : batch;
         }
-        batch = std::min(batch, kRefinedMostSteps - cycles);
+        batch = std::min(batch, kRefinedMostSteps - steps);
         int slot = p->x64cur;
         for (int k = 0; k < batch; ++k) {
             slot = p->x64cur;
@@ -1128,22 +1222,34 @@ int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *info, const vo
                 if ((rc = outer_step(p, p->lv[0].x[p->lv[0].cur]))) return rc;
             }
         }
-        if (graph) {
-            HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-            b = group_max(ctx->h_pinned + slot * kMaxSlots);
-        } else {
-            if ((rc = read_u64(p, slot, &b))) return rc;
+        if (graph) HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        else if ((rc = read_all())) return rc;
+        steps += batch;
+        bool changed = false;
+        for (int i = 0; i < K; ++i) {
+            if (!on[i]) continue;
+            cycles[i] += batch;
+            const double r = bits_to_double(slot_max(i, slot));
+            const double prev = res[i];
+            res[i] = largest[i] > 0.0 ? r / largest[i] : 0.0;
+            p->ans_par[i] = p->x64cur;
+            if (std::isnan(r) || res[i] < tol) {
+                on[i] = 0;
+                --left;
+                changed = true;
+                continue;
+            }
+            if (prev > 0.0 && res[i] > 0.0) rate[i] = std::pow(res[i] / prev, 1.0 / batch);
         }
-        cycles += batch;
-        const double r = bits_to_double(b);
-        const double prev = res;
-        res = largest > 0.0 ? r / largest : 0.0;
-        if (std::isnan(r)) break;
-        if (prev > 0.0 && res > 0.0) rate = std::pow(res / prev, 1.0 / batch);
+        if (changed && left > 0 && (rc = publish_active())) return rc;
     }
-    info->cycles = cycles;
-    info->refinements = cycles + 1;
-    info->relative_residual = res;
+    for (int i = 0; i < K; ++i) {
+        infos[i] = cg_solution{};
+        infos[i].cycles = cycles[i];
+        infos[i].refinements = cycles[i] + 1;
+        infos[i].relative_residual = res[i];
+        infos[i].status = status;
+    }
     p->last_answer = 1;
     return CG_OK;
 }
@@ -1491,6 +1597,11 @@ int cg_ocean_relax(cg_ctx *ctx, cg_stencil h, const float *forcing, const float 
 }
 
 int cg_ocean_plan_create(cg_ctx *ctx, const cg_stencil *levels, int nlevels, int pole_is_wall, cg_plan **out) {
+    return cg_ocean_plan_create_batch(ctx, levels, nlevels, pole_is_wall, 1, out);
+}
+
+int cg_ocean_plan_create_batch(cg_ctx *ctx, const cg_stencil *levels, int nlevels, int pole_is_wall, int instances,
+                               cg_plan **out) {
     if (!ctx || !out) return CG_ERR_ARG;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     *out = nullptr;
@@ -1498,11 +1609,20 @@ int cg_ocean_plan_create(cg_ctx *ctx, const cg_stencil *levels, int nlevels, int
         set_err(ctx, "plan: 1..%d levels required", kMaxLevels);
         return CG_ERR_ARG;
     }
+    if (instances < 1 || instances > kMaxInstances) {
+        set_err(ctx, "plan: 1..%d instances", kMaxInstances);
+        return CG_ERR_ARG;
+    }
     HIPCHK(ctx, hipSetDevice(ctx->device));
     cg_plan *p = new cg_plan();
     p->ctx = ctx;
     p->n = nlevels;
     p->pole_is_wall = pole_is_wall ? 1 : 0;
+    const int K = instances;
+    p->K = K;
+    p->red_base = K == 1 ? 0 : 4;
+    p->largest_k.assign(K, -1.0);
+    p->ans_par.assign(K, 0);
     int rc = 0;
     auto bail = [&](int code) {
         free_plan(p);
@@ -1530,25 +1650,38 @@ int cg_ocean_plan_create(cg_ctx *ctx, const cg_stencil *levels, int nlevels, int
             }
         }
         p->lv[l].st = s;
-        if ((rc = dalloc_grid(ctx, &p->lv[l].f, s->across, s->down))) return bail(rc);
-        if ((rc = dalloc_grid(ctx, &p->lv[l].x[0], s->across, s->down))) return bail(rc);
-        if ((rc = dalloc_grid(ctx, &p->lv[l].x[1], s->across, s->down))) return bail(rc);
+        // K copies of the grid, each with its zero row, all zero.
+        if ((rc = dalloc_grid(ctx, &p->lv[l].f, s->across, (s->down + 1) * K - 1))) return bail(rc);
+        if ((rc = dalloc_grid(ctx, &p->lv[l].x[0], s->across, (s->down + 1) * K - 1))) return bail(rc);
+        if ((rc = dalloc_grid(ctx, &p->lv[l].x[1], s->across, (s->down + 1) * K - 1))) return bail(rc);
         if (l > 0 && (rc = dalloc(ctx, &p->lv[l].bal, s->cells))) return bail(rc);
-        HIPCHK(ctx, hipMemset(p->lv[l].x[0], 0, sizeof(float) * s->cells));
-        HIPCHK(ctx, hipMemset(p->lv[l].x[1], 0, sizeof(float) * s->cells));
-        HIPCHK(ctx, hipMemset(p->lv[l].f, 0, sizeof(float) * s->cells));
     }
     const long n0 = p->lv[0].st->cells;
-    if ((rc = dalloc_grid(ctx, &p->f0, p->lv[0].st->across, p->lv[0].st->down))) return bail(rc);
+    if ((rc = dalloc_grid(ctx, &p->f0, p->lv[0].st->across, (p->lv[0].st->down + 1) * K - 1))) return bail(rc);
     if ((rc = dalloc(ctx, &p->r64, n0))) return bail(rc);
-    if ((rc = dalloc(ctx, &p->x64[0], n0))) return bail(rc);
-    if ((rc = dalloc(ctx, &p->x64[1], n0))) return bail(rc);
-    if ((rc = dalloc(ctx, &p->d_u64, 4 * kMaxSlots))) return bail(rc);
-    HIPCHK(ctx, hipMemset(p->d_u64, 0, 4 * kMaxSlots * sizeof(unsigned long long)));
+    if ((rc = dalloc(ctx, &p->x64[0], n0 * K))) return bail(rc);
+    if ((rc = dalloc(ctx, &p->x64[1], n0 * K))) return bail(rc);
+    const int groups = p->red_base + 2 * K;
+    const int alloc_groups = groups > 4 ? groups : 4;
+    if ((rc = dalloc(ctx, &p->d_u64, (size_t)alloc_groups * kMaxSlots))) return bail(rc);
+    HIPCHK(ctx, hipMemset(p->d_u64, 0, (size_t)alloc_groups * kMaxSlots * sizeof(unsigned long long)));
+    if (hipHostMalloc((void **)&p->h_red, (size_t)alloc_groups * kMaxSlots * sizeof(unsigned long long),
+                      hipHostMallocDefault) != hipSuccess) {
+        p->h_red = nullptr;
+        set_err(ctx, "pinned host memory for the plan's maxima was refused");
+        return bail(CG_ERR_NOMEM);
+    }
+    if (K > 1) {
+        if ((rc = dalloc(ctx, &p->d_active, K))) return bail(rc);
+        if (hipHostMalloc((void **)&p->h_active, sizeof(int) * K, hipHostMallocDefault) != hipSuccess) {
+            p->h_active = nullptr;
+            set_err(ctx, "pinned host memory for the plan's flags was refused");
+            return bail(CG_ERR_NOMEM);
+        }
+    }
     if ((rc = dalloc(ctx, &p->d_passes, 4))) return bail(rc);
-    HIPCHK(ctx, hipMemset(p->f0, 0, sizeof(float) * n0));
-    HIPCHK(ctx, hipMemset(p->x64[0], 0, sizeof(double) * n0));
-    HIPCHK(ctx, hipMemset(p->x64[1], 0, sizeof(double) * n0));
+    HIPCHK(ctx, hipMemset(p->x64[0], 0, sizeof(double) * n0 * K));
+    HIPCHK(ctx, hipMemset(p->x64[1], 0, sizeof(double) * n0 * K));
     if (nlevels > 1 && (rc = build_coarsest_inverse(ctx, p))) return bail(rc);
     if ((rc = plan_fusion_setup(ctx, p))) return bail(rc);
     *out = p;
@@ -1565,15 +1698,58 @@ int cg_ocean_plan_destroy(cg_plan *p) {
     return CG_OK;
 }
 
-int cg_ocean_plan_set_forcing(cg_plan *p, const float *forcing) {
-    if (!p || !forcing) return CG_ERR_ARG;
+int cg_ocean_plan_set_forcing(cg_plan *p, const float *forcing) { return cg_ocean_plan_set_forcing_at(p, 0, forcing); }
+
+int cg_ocean_plan_set_forcing_at(cg_plan *p, int instance, const float *forcing) {
+    if (!p || !forcing || instance < 0 || instance >= p->K) return CG_ERR_ARG;
     cg_ctx *ctx = p->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    HIPCHK(ctx, hipMemcpyAsync(p->f0, forcing, sizeof(float) * p->lv[0].st->cells, hipMemcpyHostToDevice, ctx->stream));
+    const Stencil *st = p->lv[0].st;
+    HIPCHK(ctx, hipMemcpyAsync(p->f0 + instance * vstride(st), forcing, sizeof(float) * st->cells, hipMemcpyHostToDevice,
+                               ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    p->largest = -1.0;
+    if (instance == 0) p->largest = -1.0;
+    p->largest_k[instance] = -1.0;
     p->x64_valid = false;
+    return CG_OK;
+}
+
+int cg_ocean_plan_solve_all(cg_plan *p, double tol, int method, cg_solution *infos) {
+    if (!p || !infos) return CG_ERR_ARG;
+    cg_ctx *ctx = p->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (method != CG_METHOD_REFINED) {
+        set_err(ctx, "a plan's instances solve together by CG_METHOD_REFINED only");
+        return CG_ERR_ARG;
+    }
+    if (p->K > 1 && !(ctx->fused && p->fusable)) {
+        set_err(ctx, "several instances need the fused cycle (row-constant weights, halving grids)");
+        return CG_ERR_STATE;
+    }
+    return refined_solve(p, tol, false, infos, nullptr);
+}
+
+int cg_ocean_plan_read_at(cg_plan *p, int instance, double *out64, float *out32) {
+    if (!p || instance < 0 || instance >= p->K) return CG_ERR_ARG;
+    if (p->K == 1) return cg_ocean_plan_read(p, out64, out32);
+    cg_ctx *ctx = p->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (p->last_answer != 1) {
+        set_err(ctx, "the instances have no answer yet");
+        return CG_ERR_STATE;
+    }
+    const long n = p->lv[0].st->cells;
+    const double *x = p->x64[p->ans_par[instance]] + instance * n;
+    if (out64) HIPCHK(ctx, hipMemcpyAsync(out64, x, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    if (out32) {
+        float *tmp = p->lv[0].x[1 - p->lv[0].cur];
+        hipLaunchKernelGGL(k_f64_to_f32, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, x, tmp, n);
+        HIPCHK(ctx, hipMemcpyAsync(out32, tmp, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return CG_OK;
 }
 
@@ -1600,6 +1776,10 @@ static int plan_solve_locked(cg_plan *p, double tol, int method, int warm, cg_so
 int cg_ocean_plan_solve(cg_plan *p, double tol, int method, int warm, cg_solution *info) {
     if (!p) return CG_ERR_ARG;
     std::lock_guard<std::recursive_mutex> g(p->ctx->mu);
+    if (p->K > 1) {
+        set_err(p->ctx, "a plan of several instances solves with cg_ocean_plan_solve_all");
+        return CG_ERR_STATE;
+    }
     return plan_solve_locked(p, tol, method, warm, info, nullptr);
 }
 
@@ -1616,6 +1796,10 @@ int cg_ocean_plan_set_bodies(cg_plan *p, const int32_t *body_of, int count) {
     if (!p || count < 0 || (count > 0 && !body_of)) return CG_ERR_ARG;
     cg_ctx *ctx = p->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (p->K > 1) {
+        set_err(ctx, "this call takes a plan of one instance");
+        return CG_ERR_STATE;
+    }
     HIPCHK(ctx, hipSetDevice(ctx->device));
     const Stencil *st = p->lv[0].st;
     const long n = st->cells;
@@ -1692,6 +1876,10 @@ int cg_ocean_plan_correct_bodies(cg_plan *p, const float *values, float *out) {
     if (!p || !values || !out) return CG_ERR_ARG;
     cg_ctx *ctx = p->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (p->K > 1) {
+        set_err(ctx, "this call takes a plan of one instance");
+        return CG_ERR_STATE;
+    }
     HIPCHK(ctx, hipSetDevice(ctx->device));
     PlanLevel &l0 = p->lv[0];
     const long n = l0.st->cells;
@@ -1709,6 +1897,10 @@ int cg_ocean_plan_set_start(cg_plan *p, const float *start) {
     if (!p || !start) return CG_ERR_ARG;
     cg_ctx *ctx = p->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (p->K > 1) {
+        set_err(ctx, "this call takes a plan of one instance");
+        return CG_ERR_STATE;
+    }
     HIPCHK(ctx, hipSetDevice(ctx->device));
     PlanLevel &l0 = p->lv[0];
     const long n = l0.st->cells;
@@ -1790,6 +1982,10 @@ int cg_ocean_plan_set_stress(cg_plan *p, double width_m, double height_m, const 
     if (!p || !eastward || !northward || !(width_m > 0.0) || !(height_m > 0.0)) return CG_ERR_ARG;
     cg_ctx *ctx = p->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (p->K > 1) {
+        set_err(ctx, "this call takes a plan of one instance");
+        return CG_ERR_STATE;
+    }
     HIPCHK(ctx, hipSetDevice(ctx->device));
     const Stencil *st = p->lv[0].st;
     if (!st->has_centre) {
@@ -1817,6 +2013,10 @@ int cg_ocean_plan_velocities(cg_plan *p, double width_m, double height_m, float 
     if (!p || !(width_m > 0.0) || !(height_m > 0.0)) return CG_ERR_ARG;
     cg_ctx *ctx = p->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (p->K > 1) {
+        set_err(ctx, "this call takes a plan of one instance");
+        return CG_ERR_STATE;
+    }
     HIPCHK(ctx, hipSetDevice(ctx->device));
     const Stencil *st = p->lv[0].st;
     const size_t n = st->cells;
@@ -1909,7 +2109,7 @@ double cg_ocean_plan_time_kernel(cg_plan *p, int kind, int reps) {
             const Stencil *st = L.st;
             const dim3 grid(blocks_for(st->across / 2, 256), blocks_for(st->down, kRowsPerBlock));
             hipLaunchKernelGGL(k_outer_residual, grid, dim3(256), 0, ctx->stream, st->args(), p->f0, p->x64[p->x64cur],
-                               L.x[L.cur], p->x64[1 - p->x64cur], L.f, red);
+                               L.x[L.cur], p->x64[1 - p->x64cur], L.f, red, OuterBatch{});
             return hipGetLastError();
         }
         TileExtra ex{};

@@ -478,3 +478,45 @@ class Plan:
         if self._p:
             lib().cg_ocean_plan_destroy(self._p)
             self._p = None
+
+
+class BatchPlan:
+    """Several right-hand sides on one resident ladder, solved together (cg_ocean_plan_create_batch):
+    each kernel of the cycle covers every instance in one launch. Forcings are per instance;
+    solve_all runs the refined method from zero for all of them."""
+
+    def __init__(self, acc: HipOcean, levels: List[OceanStencil], instances: int, pole_is_wall: bool = True):
+        self.acc = acc
+        self.levels = levels
+        self.instances = int(instances)
+        handles = (ctypes.c_uint64 * len(levels))(*[acc.holding(s) for s in levels])
+        self._p = ctypes.c_void_p()
+        check(lib().cg_ocean_plan_create_batch(acc.ctx, handles, len(levels), int(bool(pole_is_wall)), self.instances,
+                                               ctypes.byref(self._p)), acc.ctx, "cg_ocean_plan_create_batch")
+
+    @property
+    def handle(self):
+        return self._p
+
+    def set_forcing(self, instance: int, forcing) -> None:
+        check(lib().cg_ocean_plan_set_forcing_at(self._p, int(instance), np.ascontiguousarray(forcing, dtype=np.float32)),
+              self.acc.ctx, "cg_ocean_plan_set_forcing_at")
+
+    def solve_all(self, tol: float) -> List[CgSolution]:
+        infos = (CgSolution * self.instances)()
+        check(lib().cg_ocean_plan_solve_all(self._p, float(tol), CG_METHOD_REFINED, infos), self.acc.ctx,
+              "cg_ocean_plan_solve_all")
+        return list(infos)
+
+    def read(self, instance: int):
+        n = self.levels[0].cells
+        out64 = np.empty(n, np.float64)
+        out32 = np.empty(n, np.float32)
+        check(lib().cg_ocean_plan_read_at(self._p, int(instance), out64.ctypes.data_as(ctypes.c_void_p),
+                                          out32.ctypes.data_as(ctypes.c_void_p)), self.acc.ctx, "cg_ocean_plan_read_at")
+        return out64, out32
+
+    def close(self) -> None:
+        if self._p:
+            lib().cg_ocean_plan_destroy(self._p)
+            self._p = None

@@ -87,3 +87,13 @@ def circulation_problem(rows: int, cols: int = 0, seed: int = 0, land_fraction: 
     water, depth, balance = generate(cols, rows, seed, land_fraction, month)
     levels, dx, dy = circulation_levels(cols, rows, water, balance)
     return Problem(rows, cols, seed, land_fraction, month, water, depth, balance, levels, dx, dy)
+
+
+def monthly_forcings(problem: Problem, months=range(12)) -> List[np.ndarray]:
+    """The finest binary32 forcing (withBalance of the month's curl) of each month of one synthetic
+    world: its mask and depth are the year's, the stress band's amplitude and latitude the month's."""
+    out = []
+    for m in months:
+        _, _, balance = generate(problem.cols, problem.rows, problem.seed, problem.land_fraction, int(m))
+        out.append(OceanCirculation.with_balance(problem.levels[0], balance).forcing)
+    return out

@@ -128,6 +128,19 @@ int cg_ocean_solve(cg_ctx *ctx, const cg_stencil *levels, int nlevels, const flo
 int cg_ocean_plan_create(cg_ctx *ctx, const cg_stencil *levels, int nlevels, int pole_is_wall,
                          cg_plan **out);
 int cg_ocean_plan_destroy(cg_plan *plan);
+/* A plan of `instances` right-hand sides (1..64) on one ladder, solved together: every kernel of the
+ * binary32 cycle and the binary64 outer step covers all instances in one launch, so the grids whose
+ * launches are latency-bound are paid once for all of them (the months of one world, for instance).
+ * Forcings are set per instance; cg_ocean_plan_solve_all solves them all from zero, each instance
+ * stopping at its own tolerance check. Calls that name no instance act on instance 0; the
+ * single-instance calls (solve, set_stress, set_bodies, set_start, velocities, correct_bodies)
+ * return CG_ERR_STATE on a plan of several. */
+int cg_ocean_plan_create_batch(cg_ctx *ctx, const cg_stencil *levels, int nlevels, int pole_is_wall,
+                               int instances, cg_plan **out);
+int cg_ocean_plan_set_forcing_at(cg_plan *plan, int instance, const float *forcing);
+/* CG_METHOD_REFINED over every instance; infos[i] receives instance i's result. */
+int cg_ocean_plan_solve_all(cg_plan *plan, double tol, int method, cg_solution *infos);
+int cg_ocean_plan_read_at(cg_plan *plan, int instance, double *out64, float *out32);
 /* Uploads the finest forcing (host pointer). */
 int cg_ocean_plan_set_forcing(cg_plan *plan, const float *forcing);
 /* Solves from zero (or from the plan's current answer if warm != 0) to tol. */
