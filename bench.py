@@ -132,6 +132,42 @@ def batched_worlds(acc, tol: float):
             "solves_per_s": round(value, 3), "ms_per_world": per}
 
 
+def batched_combined(acc, tol: float):
+    """64 instances of uneven size: 16 each at 256x512, 512x1024, 1024x2048 and 2048x4096 (months
+    0..11 then 0..3), one plan of 16 per size, the sizes one after another; solves/s = 64 / wall
+    time. Each instance's residual is the device's binary64 one."""
+    from cartogenesis_amd.ocean import BatchPlan
+    from cartogenesis_amd.problem import circulation_problem, monthly_forcings
+
+    plans = []
+    log("bench: building the combined 64-instance workload")
+    for rows, seed in ((256, 5), (512, 6), (1024, 7), (2048, 8)):
+        prob = circulation_problem(rows, seed=seed)
+        forcings = monthly_forcings(prob)
+        plan = BatchPlan(acc, prob.levels, 16)
+        for i in range(16):
+            plan.set_forcing(i, forcings[i % 12])
+        plans.append((rows, plan))
+    try:
+        for _, plan in plans:
+            plan.solve_all(tol)
+        t0 = time.perf_counter()
+        per, worst = {}, 0.0
+        for rows, plan in plans:
+            t1 = time.perf_counter()
+            infos = plan.solve_all(tol)
+            per[f"{rows}x{2 * rows}"] = round(1000.0 * (time.perf_counter() - t1), 3)
+            worst = max(worst, max(float(i.relative_residual) for i in infos))
+        elapsed = time.perf_counter() - t0
+    finally:
+        for _, plan in plans:
+            plan.close()
+    value = 64 / elapsed
+    log(f"bench: combined 64 instances {value:.1f} solves/s")
+    return {"workload": "64 instances: 16 monthly forcings each at 256x512, 512x1024, 1024x2048, 2048x4096",
+            "solves_per_s": round(value, 3), "ms_per_size": per, "max_relative_residual": worst}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +187,9 @@ def main() -> int:
                     help="passes per level of the binary32 cycle, one hex digit per level, finest last")
     ap.add_argument("--batch-reps", type=int, default=5,
                     help="timed repetitions of the batched year (12 monthly forcings solved together); 0 skips it")
+    ap.add_argument("--combined", action="store_true",
+                    help="also time 64 instances: 16 monthly forcings each at 256x512, 512x1024, 1024x2048 and "
+                         "2048x4096, one plan of 16 per size")
     ap.add_argument("--worlds", action="store_true",
                     help="also time the four-world year: 12 months on each of 512x1024, 768x1536, 1024x2048 and "
                          "1024x2048 (seeds 1-4, land 20/27/33/40%%)")
@@ -226,6 +265,7 @@ def main() -> int:
     if args.batch_reps > 0:
         batched = batched_year(acc, prob, args.tol, args.batch_reps, args.warmup)
     worlds = batched_worlds(acc, args.tol) if args.worlds else None
+    combined = batched_combined(acc, args.tol) if args.combined else None
 
     err = None
     cpu = None
@@ -294,6 +334,8 @@ def main() -> int:
         out["batched"] = batched
     if worlds:
         out["batched_worlds"] = worlds
+    if combined:
+        out["batched_combined"] = combined
     print(json.dumps(out), flush=True)
     plan.close()
     acc.close()

@@ -1086,9 +1086,10 @@ int outer_step(cg_plan *p, const float *delta, float *fc = nullptr) {
 }
 
 // One refinement step as a graph: the V-cycle on the current correction forcing, then the outer
-// step applying its answer, then the residual max copied to pinned host memory. The binary32 level
-// buffers' ping-pong parities are the same at the start of every step (each level runs an even
-// number of launches per cycle), so one graph per binary64 parity suffices.
+// step applying its answer. The residual maxima are copied to the host once per batch of steps,
+// outside the graph. The binary32 level buffers' ping-pong parities are the same at the start of
+// every step (each level runs an even number of launches per cycle), so one graph per binary64
+// parity suffices.
 int build_step_graphs(cg_plan *p) {
     cg_ctx *ctx = p->ctx;
     for (int par = 0; par < 2; ++par) {
@@ -1101,15 +1102,6 @@ int build_step_graphs(cg_plan *p) {
         HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         int rc = vcycle32(p, false);
         if (!rc) rc = outer_step(p, p->lv[0].x[p->lv[0].cur]);
-        if (!rc) {
-            hipError_t e = hipMemcpyAsync(p->h_red + p->red_base * kMaxSlots, p->d_u64 + p->red_base * kMaxSlots,
-                                          2 * p->K * kMaxSlots * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                                          ctx->stream);
-            if (e != hipSuccess) {
-                set_err(ctx, "capture of the residual readback failed: %s", hipGetErrorString(e));
-                rc = CG_ERR_DEVICE;
-            }
-        }
         hipError_t e = hipStreamEndCapture(ctx->stream, &g);
         for (int l = 0; l < p->n; ++l) {
             if (p->lv[l].cur != saved_cur[l]) {
@@ -1222,8 +1214,7 @@ int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *infos, const v
                 if ((rc = outer_step(p, p->lv[0].x[p->lv[0].cur]))) return rc;
             }
         }
-        if (graph) HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-        else if ((rc = read_all())) return rc;
+        if ((rc = read_all())) return rc;
         steps += batch;
         bool changed = false;
         for (int i = 0; i < K; ++i) {
