@@ -31,12 +31,17 @@ ROWS, COLS = 1024, 2048
 #    x += delta fused into it);
 #  - the finest up leg k_relax_tile<4,8,false,2>: x 4 + f 4 + mask 1 + coarse x 1 read, x 4 written
 #    = 14 B/cell (the 13 B two-pass smoothing figure plus the fused prolongation's coarse values).
+#  - the finest up leg with the outer step fused in, k_relax_tile<4,8,false,3>: x 4 + correction
+#    forcing 4 + mask 1 + coarse x 1 + x64 8 + f0 4 read, x64 8 + next forcing 4 written = 34 B/cell
+#    (the 14 + 29 of the two launches less the correction written and read back and the mask read
+#    twice).
 CELLS = 1024 * 2048
 OUTER_BYTES = 29 * CELLS
 UP_BYTES = 14 * CELLS
+FUSED_BYTES = 34 * CELLS
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE and WRITE_SIZE (scaled by a 1 GiB copy),
 # for the default settings (tile rows 8, two passes per grid); see profiles/hbm_traffic_*.md.
-MEASURED_TRAFFIC = {"outer": 78181417, "up": 30654638}
+MEASURED_TRAFFIC = {"outer": 78181417, "up": 30654638, "fused": None}
 
 
 def log(msg: str) -> None:
@@ -255,11 +260,15 @@ def main() -> int:
     us_outer = lib.cg_ocean_plan_time_kernel(plan.handle, 2, reps)
     us_up = lib.cg_ocean_plan_time_kernel(plan.handle, 1, reps)
     us_down = lib.cg_ocean_plan_time_kernel(plan.handle, 0, reps)
+    us_fused = lib.cg_ocean_plan_time_kernel(plan.handle, 3, reps)
+    fused = us_fused > 0
     defaults = args.tile_rows == 8 and args.level_passes == "22222222"
     outer_gbs = OUTER_BYTES / (us_outer * 1e-6) / 1e9 if us_outer > 0 else None
     up_gbs = UP_BYTES / (us_up * 1e-6) / 1e9 if us_up > 0 else None
+    fused_gbs = FUSED_BYTES / (us_fused * 1e-6) / 1e9 if fused else None
     log(f"bench: outer residual {us_outer:.2f} us ({outer_gbs:.0f} GB/s algorithmic), finest up leg {us_up:.2f} us "
-        f"({up_gbs:.0f} GB/s), finest down leg {us_down:.2f} us")
+        f"({up_gbs:.0f} GB/s), finest down leg {us_down:.2f} us" +
+        (f", up leg with the outer step fused {us_fused:.2f} us ({fused_gbs:.0f} GB/s)" if fused else ""))
 
     batched = None
     if args.batch_reps > 0:
@@ -306,7 +315,23 @@ def main() -> int:
             "relative_residual": rel_res,
             "rel_l2_vs_fp64_oracle": err,
         },
-        "roofline": {
+        "roofline": ({
+            "bound": "hbm",
+            "kernel": "k_relax_tile<4,8,false,3> (the finest up leg with the binary64 outer step fused in: "
+                      "prolongation, 2 red-black passes, x64 += delta, binary64 residual, next forcing, max), the "
+                      "largest HBM-bound kernel of a step",
+            "achieved": round(fused_gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(fused_gbs / HBM_PEAK_GBS, 4),
+            "traffic": MEASURED_TRAFFIC["fused"] if defaults else None,
+            "avg_launch_us": round(us_fused, 3),
+            "algorithmic_bytes_per_launch": FUSED_BYTES,
+            "separate_outer_residual": {"kernel": "k_outer_residual", "avg_launch_us": round(us_outer, 3),
+                                        "achieved": None if outer_gbs is None else round(outer_gbs, 1),
+                                        "algorithmic_bytes_per_launch": OUTER_BYTES,
+                                        "traffic": MEASURED_TRAFFIC["outer"] if defaults else None},
+        } if fused else {
             "bound": "hbm",
             "kernel": "k_outer_residual (binary64 x += delta, residual, correction forcing, max), the largest "
                       "HBM-bound kernel by trace time; the one-workgroup bottom of the cycle, equal in time, is "
@@ -318,6 +343,7 @@ def main() -> int:
             "traffic": MEASURED_TRAFFIC["outer"] if defaults else None,
             "avg_launch_us": round(us_outer, 3),
             "algorithmic_bytes_per_launch": OUTER_BYTES,
+        }) | {
             "smoother": {
                 "kernel": "k_relax_tile<4,8,false,2> (finest up leg: prolongation + 2 fused red-black passes)",
                 "achieved": None if up_gbs is None else round(up_gbs, 1),

@@ -51,6 +51,40 @@ __device__ __forceinline__ float keep_if(float v, uint32_t m, int bit) {
     return __builtin_bit_cast(float, __builtin_bit_cast(int, v) & sel);
 }
 
+// Largest magnitudes are reduced as the uint64 patterns of |v|: for non-negative doubles the
+// order of the patterns is the order of the values, and a NaN's pattern exceeds all of them.
+__device__ __forceinline__ unsigned long long abs_bits(double v) {
+    return (unsigned long long)__double_as_longlong(v) & 0x7fffffffffffffffull;
+}
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// A double moved between lanes by one DPP control on each 32-bit half (0x138 wave_shr:1: lane i
+// receives lane i-1's; 0x130 wave_shl:1: lane i+1's). The edge lane receives 0.
+template <int CTRL>
+__device__ __forceinline__ double dpp_double(double v) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xf, 0xf, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Reduction slots of a largest magnitude (see grid_max): kMaxSlots relaxed atomicMax targets, and
+// the slots the next reduction will use, which the launch zeroes as it goes.
+constexpr int kMaxSlots = 64;
+
+struct MaxReduce {
+    unsigned long long *out;
+    unsigned long long *clear;
+};
+
 // Binary32 update of one cell in the reference's order: ((e xE + w xW) + n xN) + s xS - f.
 __device__ __forceinline__ float rb_update(float4 wt, float xe, float xw, float xn, float xs, float f) {
     return wt.x * xe + wt.y * xw + wt.z * xn + wt.w * xs - f;
@@ -79,8 +113,14 @@ __device__ __forceinline__ float rb_update(float4 wt, float xe, float xw, float 
 //                   land), g(row) = centre(row) * 1/4 / centre_coarse(row / 2) in binary32.
 //   TILE_UP       : the cycle's up leg: before smoothing, the coarse correction xc is added to the
 //                   water cells bilinearly in prolongAdd's binary32 arithmetic (OC:354-392).
+//   TILE_UP_OUTER : TILE_UP on the finest grid with the binary64 outer step fused after it: one more
+//                   halo row each side, so the smoothed correction delta is exact on the interior and
+//                   its ring; then, as k_outer_residual does, x64n = x64 + delta and
+//                   u = x64n - (sum w x64n - f0) in binary64, written as the next cycle's binary32
+//                   forcing fcn, with the largest |u c| reduced per wave into red_out. delta itself is
+//                   never written.
 // ------------------------------------------------------------------------------------------------
-enum TileMode { TILE_PLAIN = 0, TILE_DOWN = 1, TILE_UP = 2 };
+enum TileMode { TILE_PLAIN = 0, TILE_DOWN = 1, TILE_UP = 2, TILE_UP_OUTER = 3 };
 
 // Workgroups are dealt to the MI355X's 8 XCDs round-robin by workgroup id, and each XCD has its own
 // L2. Renumbering so that XCD k runs the k-th contiguous eighth of the logical blocks keeps
@@ -107,11 +147,21 @@ struct TileExtra {
     // grid) floats after the previous instance's; an instance whose active flag is 0 is skipped.
     long fs, cs;
     const int *active;
+    // TILE_UP_OUTER: the binary64 answer in and out (instances s64 doubles apart), the finest forcing
+    // of the binary64 problem and the next cycle's forcing out (fs apart), the maxima's slots
+    // (red_stride apart).
+    const double *x64;
+    double *x64n;
+    const float *f0;
+    float *fcn;
+    MaxReduce red;
+    long s64;
+    int red_stride;
 };
 
 template <int NH, int MODE>
 struct TileGeom {
-    static constexpr int HR = NH + (MODE == TILE_DOWN ? 1 : 0);
+    static constexpr int HR = NH + ((MODE == TILE_DOWN || MODE == TILE_UP_OUTER) ? 1 : 0);
     static constexpr int HC = MODE == TILE_PLAIN ? ((NH + 1) & ~1) : ((NH + 2) & ~1);
     static constexpr int IW = 128 - 2 * HC;
 };
@@ -135,7 +185,15 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
         if (xin) xin += inst * ex.fs;
         xout += inst * ex.fs;
         if (MODE == TILE_DOWN) ex.fc += inst * ex.cs;
-        if (MODE == TILE_UP) ex.xc += inst * ex.cs;
+        if (MODE == TILE_UP || MODE == TILE_UP_OUTER) ex.xc += inst * ex.cs;
+        if (MODE == TILE_UP_OUTER) {
+            ex.x64 += inst * ex.s64;
+            ex.x64n += inst * ex.s64;
+            ex.f0 += inst * ex.fs;
+            ex.fcn += inst * ex.fs;
+            ex.red.out += inst * ex.red_stride;
+            ex.red.clear += inst * ex.red_stride;
+        }
     }
     const int wave = xcd_block() * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int tiles_x = (across + IW - 1) / IW;
@@ -182,7 +240,7 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
         }
     }
 
-    if constexpr (MODE == TILE_UP) {
+    if constexpr (MODE == TILE_UP || MODE == TILE_UP_OUTER) {
         // Fine row r = row0 - HR + i reads coarse rows above = floor((r - 1) / 2) and above + 1,
         // i.e. row0 / 2 + q with q = floor((i - HR - 1) / 2): a compile-time offset, since row0 is
         // even. Lane l's pair (2m, 2m+1) reads coarse columns m-1, m, m+1 (from lanes l-1, l, l+1).
@@ -258,11 +316,74 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
     const int lo = HC / 2;
     const int cabs = col0 + 2 * (lane - lo);
     const bool writer = lane >= lo && lane < lo + IW / 2 && cabs < across;
-    if (writer) {
+    if (writer && MODE != TILE_UP_OUTER) {
 #pragma unroll
         for (int i = HR; i < HR + TH; ++i) {
             const int r = row0 - HR + i;
             if (r < down) *reinterpret_cast<float2 *>(xout + (long)r * across + cabs) = make_float2(A[i], B[i]);
+        }
+    }
+
+    if constexpr (MODE == TILE_UP_OUTER) {
+        // Rows HR-1 .. HR+TH of the tile: x64n = x64 + delta, zero beyond a pole (k_outer_residual's
+        // neighbours there). Every lane loads its own pair, so the writers' neighbour lanes hold
+        // theirs for the shifts; the correction is exact on those lanes (the column halo keeps two
+        // columns of slack after NH half-sweeps).
+        double vx[TH + 2], vy[TH + 2];
+#pragma unroll
+        for (int j = 0; j < TH + 2; ++j) {
+            const int r = row0 - 1 + j;
+            vx[j] = vy[j] = 0.0;
+            if (r >= 0 && r < down) {
+                const double2 xv = *reinterpret_cast<const double2 *>(ex.x64 + (long)r * across + ca);
+                vx[j] = xv.x + (double)A[HR - 1 + j];
+                vy[j] = xv.y + (double)B[HR - 1 + j];
+            }
+        }
+        unsigned long long mx = 0;
+#pragma unroll
+        for (int k = 0; k < TH; ++k) {
+            const int i = HR + k;
+            const int row = row0 + k;
+            const double west = dpp_double<0x138>(vy[k + 1]);  // lane - 1's odd column
+            const double east = dpp_double<0x130>(vx[k + 1]);  // lane + 1's even column
+            const float4 wt = make_float4(lane_value(wrow.x, i), lane_value(wrow.y, i), lane_value(wrow.z, i),
+                                          lane_value(wrow.w, i));
+            if (!writer || row >= down) continue;
+            const long idx = (long)row * across + cabs;
+            const float2 fv = *reinterpret_cast<const float2 *>(ex.f0 + idx);
+            const double cen = L.rowc[row];
+            const uint32_t mA = i < 32 ? (uint32_t)wA : (uint32_t)(wA >> 32);
+            const uint32_t mB = i < 32 ? (uint32_t)wB : (uint32_t)(wB >> 32);
+            const double vn_x = row > 0 ? vx[k] : 0.0, vn_y = row > 0 ? vy[k] : 0.0;
+            const double vs_x = row + 1 < down ? vx[k + 2] : 0.0, vs_y = row + 1 < down ? vy[k + 2] : 0.0;
+            double ux = 0.0, uy = 0.0, ox = 0.0, oy = 0.0;
+            if ((mA >> (i & 31)) & 1) {
+                const double bal = (double)wt.x * vy[k + 1] + (double)wt.y * west + (double)wt.z * vn_x +
+                                   (double)wt.w * vs_x - (double)fv.x;
+                ox = vx[k + 1];
+                ux = ox - bal;
+                const unsigned long long b = abs_bits(ux * cen);
+                mx = b > mx ? b : mx;
+            }
+            if ((mB >> (i & 31)) & 1) {
+                const double bal = (double)wt.x * east + (double)wt.y * vx[k + 1] + (double)wt.z * vn_y +
+                                   (double)wt.w * vs_y - (double)fv.y;
+                oy = vy[k + 1];
+                uy = oy - bal;
+                const unsigned long long b = abs_bits(uy * cen);
+                mx = b > mx ? b : mx;
+            }
+            *reinterpret_cast<double2 *>(ex.x64n + idx) = make_double2(ox, oy);
+            *reinterpret_cast<float2 *>(ex.fcn + idx) = make_float2((float)ux, (float)uy);
+        }
+        // One relaxed atomicMax per wave into slot wave % kMaxSlots; the waves together zero the
+        // slots the next step reduces into.
+        mx = wave_max_u64(mx);
+        if (lane == 0) {
+            const int nw = tiles_x * tiles_y;
+            for (int j = wave; j < kMaxSlots; j += nw) ex.red.clear[j] = 0ull;
+            if (mx) atomicMax(ex.red.out + (wave & (kMaxSlots - 1)), mx);
         }
     }
 
@@ -326,31 +447,12 @@ __global__ void k_relax_colour(LevelArgs L, const float *__restrict__ f, float *
 // pattern exceeds every finite and infinite magnitude, so a NaN anywhere wins, as Kotlin's max
 // (the reference's `largest`, OC:318-322) returns NaN when either side is NaN.
 // ------------------------------------------------------------------------------------------------
-__device__ __forceinline__ unsigned long long abs_bits(double v) {
-    return (unsigned long long)__double_as_longlong(v) & 0x7fffffffffffffffull;
-}
 
 // A grid-wide maximum: each block reduces in registers and LDS, then its thread 0 does one relaxed
 // atomicMax into out[block % kMaxSlots] (no fences, so no L2 write-back per block; 64 slots keep
 // the atomics uncontended). The reader takes the largest slot. The slots must be zero when the
-// kernel starts; the blocks together zero all of `clear` (the slots the next reduction will use), which a later
-// launch on the same stream then sees, so graph replays need no memset.
-constexpr int kMaxSlots = 64;
-
-struct MaxReduce {
-    unsigned long long *out;
-    unsigned long long *clear;
-};
-
-__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long o = __shfl_xor(v, off, 64);
-        v = o > v ? o : v;
-    }
-    return v;
-}
-
+// kernel starts; the blocks together zero all of `clear` (the slots the next reduction will use),
+// which a later launch on the same stream then sees, so graph replays need no memset.
 __device__ void grid_max(unsigned long long v, const MaxReduce &red) {
     __shared__ unsigned long long sh[16];
     const int b = blockIdx.x + blockIdx.y * gridDim.x;
@@ -585,16 +687,6 @@ __global__ __launch_bounds__(1024) void k_coarsest_relax(LevelArgs L, const floa
 // ------------------------------------------------------------------------------------------------
 // Kernels of the refined (binary64 outer, binary32 inner) solve.
 // ------------------------------------------------------------------------------------------------
-
-// A double moved between lanes by one DPP control on each 32-bit half (0x138 wave_shr:1: lane i
-// receives lane i-1's; 0x130 wave_shl:1: lane i+1's). The edge lane receives 0.
-template <int CTRL>
-__device__ __forceinline__ double dpp_double(double v) {
-    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xf, 0xf, false);
-    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
-    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
-}
 
 // The outer step: x_new = x + delta (binary64; delta may be null), u = x_new - (sum w x_new - f)
 // in binary64, the correction forcing fc = (float)u on water (0 on land), and the largest

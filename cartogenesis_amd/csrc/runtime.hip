@@ -84,6 +84,7 @@ struct cg_ctx {
     int steps_per_check = 1;
     int time_kernels = 0;
     int fused = 1;
+    int fuse_outer = 1;  // the finest up leg and the binary64 outer step in one launch
     // Passes per level of the fused binary32 cycle, 4 bits per level from level 0 up (2, 4 or 6;
     // levels past the eighth and the bottom's run 2).
     long level_passes = 0x22222222;
@@ -225,6 +226,8 @@ hipError_t launch_tile(hipStream_t s, const LevelArgs &L, const float *f, const 
             if (nh == 12) return launch_tile_mode<TH, TILE_UP, 12>(s, L, f, xin, xout, false, ex, k);
             if (nh == 8) return launch_tile_mode<TH, TILE_UP, 8>(s, L, f, xin, xout, false, ex, k);
             return launch_tile_mode<TH, TILE_UP, 4>(s, L, f, xin, xout, false, ex, k);
+        case TILE_UP_OUTER:
+            return launch_tile_mode<TH, TILE_UP_OUTER, 4>(s, L, f, xin, xout, false, ex, k);
         default:
             if (nh == 4) return launch_tile_mode<TH, TILE_PLAIN, 4>(s, L, f, xin, xout, zero, ex, k);
             return launch_tile_mode<TH, TILE_PLAIN, 2>(s, L, f, xin, xout, zero, ex, k);
@@ -311,6 +314,7 @@ struct cg_plan {
     std::vector<int> ans_par;             // per instance: the binary64 parity holding its answer
     PlanLevel lv[kMaxLevels];
     float *f0 = nullptr;          // the finest forcing
+    float *f_alt = nullptr;       // with the fused outer step: level 0's other correction forcing
     double *r64 = nullptr;        // finest-size binary64 scratch
     double *x64[2] = {nullptr, nullptr};
     int x64cur = 0;
@@ -372,6 +376,7 @@ void free_plan(cg_plan *p) {
         dfree(p->lv[i].g);
     }
     dfree(p->f0);
+    dfree(p->f_alt);
     dfree(p->r64);
     for (double *v : {p->kx, p->kr, p->ksh, p->kp, p->kv, p->kph, p->kshat, p->kt, p->kbal, p->kpart}) dfree(v);
     dfree(p->kf32);
@@ -915,9 +920,23 @@ int level_half_sweeps(const cg_ctx *ctx, int l) {
     return 2 * (int)((ctx->level_passes >> (4 * l)) & 0xf);
 }
 
+// Level 0's correction forcing for binary64 parity q when the outer step is fused into the finest
+// up leg: the two buffers alternate, since that launch writes the next forcing while its
+// neighbouring waves still read this one.
+inline float *level0_forcing(cg_plan *p, int q) { return q == 0 ? p->lv[0].f : p->f_alt; }
+
+// Whether the refined method's steps fuse the outer step into the finest up leg.
+inline bool outer_fused(const cg_plan *p) {
+    const cg_ctx *ctx = p->ctx;
+    return ctx->fuse_outer && ctx->fused && p->fusable && p->bottom > 0 && level_half_sweeps(ctx, 0) == 4;
+}
+
 // The fused cycle: per tile level one down launch (smooth from zero, residual, restriction) and
 // one up launch (prolongation, smooth); the bottom levels in one k_bottom launch.
-int vcycle32_fused(cg_plan *p, bool timed, const float *in0, float *out0) {
+// With fuse_outer the finest up leg also runs the binary64 outer step (TILE_UP_OUTER): it reads
+// x64[x64cur] and writes x64[1 - x64cur], the next cycle's forcing (level0_forcing(p, 1 - x64cur)) and
+// the maxima of parity x64cur; the caller flips x64cur.
+int vcycle32_fused(cg_plan *p, bool timed, const float *in0, float *out0, bool fuse_outer) {
     cg_ctx *ctx = p->ctx;
     hipStream_t s = ctx->stream;
     const int n = p->n, b = p->bottom;
@@ -995,7 +1014,20 @@ int vcycle32_fused(cg_plan *p, bool timed, const float *in0, float *out0) {
         TimeScope ts(ctx, timed && l == 0, kKindSmooth);
         const float *fl = (l == 0 && in0) ? in0 : L.f;
         float *xo = (l == 0 && out0) ? out0 : L.x[1 - L.cur];
-        hipError_t e = tile_dispatch(ctx, L.st->args(), fl, L.x[L.cur], xo, level_half_sweeps(ctx, l), false, TILE_UP,
+        int mode = TILE_UP;
+        if (l == 0 && fuse_outer) {
+            const int q = p->x64cur;
+            mode = TILE_UP_OUTER;
+            ex.x64 = p->x64[q];
+            ex.x64n = p->x64[1 - q];
+            ex.f0 = p->f0;
+            ex.fcn = level0_forcing(p, 1 - q);
+            ex.red.out = p->d_u64 + (p->red_base + q) * kMaxSlots;
+            ex.red.clear = p->d_u64 + (p->red_base + 1 - q) * kMaxSlots;
+            ex.s64 = L.st->cells;
+            ex.red_stride = 2 * kMaxSlots;
+        }
+        hipError_t e = tile_dispatch(ctx, L.st->args(), fl, L.x[L.cur], xo, level_half_sweeps(ctx, l), false, mode,
                                      ex, level_tile_rows(ctx, L.st), p->K);
         if (e != hipSuccess) {
             set_err(ctx, "up-leg launch failed: %s", hipGetErrorString(e));
@@ -1014,7 +1046,7 @@ int vcycle32_unfused(cg_plan *p, bool timed);
 // One binary32 V(2,2) cycle on the correction problem from zero. Level 0's forcing is in0 (or
 // lv[0].f) and its answer goes to out0 (or stays in lv[0].x[lv[0].cur]).
 int vcycle32(cg_plan *p, bool timed, const float *in0 = nullptr, float *out0 = nullptr) {
-    if (p->ctx->fused && p->fusable) return vcycle32_fused(p, timed, in0, out0);
+    if (p->ctx->fused && p->fusable) return vcycle32_fused(p, timed, in0, out0, false);
     cg_ctx *ctx = p->ctx;
     PlanLevel &L0 = p->lv[0];
     if (in0) HIPCHK(ctx, hipMemcpyAsync(L0.f, in0, sizeof(float) * L0.st->cells, hipMemcpyDeviceToDevice, ctx->stream));
@@ -1085,6 +1117,20 @@ int outer_step(cg_plan *p, const float *delta, float *fc = nullptr) {
     return CG_OK;
 }
 
+// One refinement step: a binary32 cycle on the current correction forcing and the binary64 outer
+// step, fused into the finest up leg where outer_fused, else as its own launch.
+int refine_step(cg_plan *p, bool timed) {
+    if (outer_fused(p)) {
+        const int rc = vcycle32_fused(p, timed, level0_forcing(p, p->x64cur), nullptr, true);
+        if (rc) return rc;
+        p->x64cur = 1 - p->x64cur;
+        return CG_OK;
+    }
+    const int rc = vcycle32(p, timed);
+    if (rc) return rc;
+    return outer_step(p, p->lv[0].x[p->lv[0].cur]);
+}
+
 // One refinement step as a graph: the V-cycle on the current correction forcing, then the outer
 // step applying its answer. The residual maxima are copied to the host once per batch of steps,
 // outside the graph. The binary32 level buffers' ping-pong parities are the same at the start of
@@ -1100,8 +1146,7 @@ int build_step_graphs(cg_plan *p) {
         p->x64cur = par;
         hipGraph_t g = nullptr;
         HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        int rc = vcycle32(p, false);
-        if (!rc) rc = outer_step(p, p->lv[0].x[p->lv[0].cur]);
+        int rc = refine_step(p, false);
         hipError_t e = hipStreamEndCapture(ctx->stream, &g);
         for (int l = 0; l < p->n; ++l) {
             if (p->lv[l].cur != saved_cur[l]) {
@@ -1161,7 +1206,7 @@ int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *infos, const v
         return CG_OK;
     };
     const int first_slot = p->x64cur;
-    if ((rc = outer_step(p, nullptr))) return rc;
+    if ((rc = outer_step(p, nullptr, outer_fused(p) ? level0_forcing(p, 1 - p->x64cur) : nullptr))) return rc;
     if ((rc = read_all())) return rc;
     p->ans_par.assign(K, p->x64cur);
     int left = 0;
@@ -1210,8 +1255,7 @@ int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *infos, const v
                 HIPCHK(ctx, hipGraphLaunch(p->step_exec[p->x64cur], ctx->stream));
                 p->x64cur = 1 - p->x64cur;
             } else {
-                if ((rc = vcycle32(p, true))) return rc;
-                if ((rc = outer_step(p, p->lv[0].x[p->lv[0].cur]))) return rc;
+                if ((rc = refine_step(p, true))) return rc;
             }
         }
         if ((rc = read_all())) return rc;
@@ -1418,6 +1462,7 @@ int cg_ocean_set_option(cg_ctx *ctx, int option, long value) {
         case CG_OPT_STEPS_PER_CHECK: ctx->steps_per_check = value < 1 ? 1 : (int)value; return CG_OK;
         case 6: ctx->time_kernels = value ? 1 : 0; return CG_OK;
         case 7: ctx->fused = value ? 1 : 0; return CG_OK;
+        case 9: ctx->fuse_outer = value ? 1 : 0; return CG_OK;
         case 8:
             for (int l = 0; l < 8; ++l) {
                 const long v = (value >> (4 * l)) & 0xf;
@@ -1442,6 +1487,7 @@ long cg_ocean_get_option(cg_ctx *ctx, int option) {
         case CG_OPT_STEPS_PER_CHECK: return ctx->steps_per_check;
         case 6: return ctx->time_kernels;
         case 7: return ctx->fused;
+        case 9: return ctx->fuse_outer;
         case 8: return ctx->level_passes;
         default: return -1;
     }
@@ -1649,6 +1695,7 @@ int cg_ocean_plan_create_batch(cg_ctx *ctx, const cg_stencil *levels, int nlevel
     }
     const long n0 = p->lv[0].st->cells;
     if ((rc = dalloc_grid(ctx, &p->f0, p->lv[0].st->across, (p->lv[0].st->down + 1) * K - 1))) return bail(rc);
+    if ((rc = dalloc_grid(ctx, &p->f_alt, p->lv[0].st->across, (p->lv[0].st->down + 1) * K - 1))) return bail(rc);
     if ((rc = dalloc(ctx, &p->r64, n0))) return bail(rc);
     if ((rc = dalloc(ctx, &p->x64[0], n0 * K))) return bail(rc);
     if ((rc = dalloc(ctx, &p->x64[1], n0 * K))) return bail(rc);
@@ -2082,11 +2129,15 @@ int cg_ocean_solve(cg_ctx *ctx, const cg_stencil *levels, int nlevels, const flo
 }
 
 double cg_ocean_plan_time_kernel(cg_plan *p, int kind, int reps) {
-    if (!p || reps < 1 || kind < 0 || kind > 2) return -1.0;
+    if (!p || reps < 1 || kind < 0 || kind > 3) return -1.0;
     cg_ctx *ctx = p->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     if (hipSetDevice(ctx->device) != hipSuccess) return -1.0;
-    if (kind < 2 && (!p->fusable || p->bottom == 0)) {
+    if (kind == 3 && !outer_fused(p)) {
+        set_err(ctx, "the plan's steps do not fuse the outer step");
+        return -1.0;
+    }
+    if (kind != 2 && (!p->fusable || p->bottom == 0)) {
         set_err(ctx, "plan has no fused finest tile level to time");
         return -1.0;
     }
@@ -2115,6 +2166,21 @@ double cg_ocean_plan_time_kernel(cg_plan *p, int kind, int reps) {
         ex.ca = C.st->across;
         ex.cd = C.st->down;
         ex.beyond = p->pole_is_wall ? -1.0f : 1.0f;
+        if (kind == 3) {
+            // The fused finest up leg and outer step: reads parity q, writes parity 1 - q (not the
+            // answer's), its maxima into the scratch slot groups 3 (out) and 2 (cleared).
+            const int q = p->x64cur;
+            ex.x64 = p->x64[q];
+            ex.x64n = p->x64[1 - q];
+            ex.f0 = p->f0;
+            ex.fcn = level0_forcing(p, 1 - q);
+            ex.red.out = p->d_u64 + 3 * kMaxSlots;
+            ex.red.clear = p->d_u64 + 2 * kMaxSlots;
+            ex.s64 = L.st->cells;
+            ex.red_stride = 2 * kMaxSlots;
+            return tile_dispatch(ctx, L.st->args(), level0_forcing(p, q), L.x[L.cur], L.x[1 - L.cur], nh, false,
+                                 TILE_UP_OUTER, ex, th);
+        }
         return tile_dispatch(ctx, L.st->args(), L.f, L.x[L.cur], L.x[1 - L.cur], nh, false, TILE_UP, ex, th);
     };
     hipEvent_t a, b;

@@ -65,6 +65,9 @@ extern "C" {
                                     one workgroup); 0: separate smoothing/residual/transfer kernels */
 #define CG_OPT_LEVEL_PASSES 8    /* passes per grid of the binary32 cycle, one hex digit per grid from the
                                     finest (lowest digit) up, each 2, 4 or 6; default 0x22222222 */
+#define CG_OPT_FUSE_OUTER 9      /* 1 (default): CG_METHOD_REFINED runs the binary64 outer step inside the
+                                    finest grid's up-leg launch (fused cycle, two passes there); 0: as a
+                                    launch of its own. Both give the same bits. */
 
 typedef struct cg_ctx cg_ctx;
 typedef struct cg_plan cg_plan;
@@ -169,9 +172,10 @@ double cg_ocean_kernel_time_us(cg_ctx *ctx, int kind, long *launches);
 
 /* Times `reps` back-to-back launches of one of the plan's kernels between two HIP events on the
  * context's stream and returns the average in microseconds (-1 on failure): kind 0 the finest
- * grid's fused down leg, 1 its fused up leg, 2 the binary64 outer residual. Each launch reads the
- * plan's current buffers and writes the same outputs, so repeats leave the plan as one launch
- * does; call it between solves, not during one. */
+ * grid's fused down leg, 1 its fused up leg, 2 the binary64 outer residual, 3 the finest up leg
+ * with the outer step fused in. Each launch reads the plan's current buffers and writes the same
+ * outputs (never the answer), so repeats leave the plan as one launch does; call it between
+ * solves, not during one. */
 double cg_ocean_plan_time_kernel(cg_plan *plan, int kind, int reps);
 
 /* ---- the round's input and output steps, on the device ------------------------------------- */

@@ -144,6 +144,32 @@ def test_fused_and_unfused_cycles_reach_the_oracle(acc, rows):
     assert abs(out[1].cycles - out[0].cycles) <= 1
 
 
+@pytest.mark.parametrize("rows,graph", [(128, 1), (256, 0), (1024, 1)])
+def test_outer_step_fused_into_the_up_leg_gives_the_same_bits(acc, rows, graph):
+    """The finest up leg with the binary64 outer step fused in (CG_OPT_FUSE_OUTER, the default)
+    computes exactly what the up leg and k_outer_residual compute one after the other: the same
+    steps, residual and binary64 answer to the bit, with and without graph replay."""
+    from cartogenesis_amd import CG_OPT_FUSE_OUTER
+    p = circulation_problem(rows, seed=9)
+    out = {}
+    acc.set_option(CG_OPT_USE_GRAPH, graph)
+    try:
+        for fuse in (1, 0):
+            acc.set_option(CG_OPT_FUSE_OUTER, fuse)
+            plan = Plan(acc, p.levels)
+            try:
+                info = plan.solve(1e-10, CG_METHOD_REFINED)
+                out[fuse] = (info.cycles, info.relative_residual, plan.read()[0])
+            finally:
+                plan.close()
+    finally:
+        acc.set_option(CG_OPT_FUSE_OUTER, 1)
+        acc.set_option(CG_OPT_USE_GRAPH, 1)
+    print(f"OCEAN fused outer {2 * rows}x{rows}: {out[1][0]} cycles, residual {out[1][1]}")
+    assert out[1][0] == out[0][0] and out[1][1] == out[0][1]
+    assert np.array_equal(out[1][2].view(np.uint64), out[0][2].view(np.uint64))
+
+
 @pytest.mark.parametrize("rows,seed", [(64, 0), (256, 0), (128, 4)])
 def test_fgmres_method_matches_the_binary64_oracle(acc, rows, seed):
     """FGMRES(m) right-preconditioned by the binary32 cycle, inside binary64 refinement: the same
