@@ -282,6 +282,23 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
         }
     }
 
+    // TILE_UP_OUTER: the binary64 answer of rows HR-1 .. HR+TH (this lane's pair), loaded before the
+    // sweeps so that its latency overlaps them; zero beyond a pole.
+    constexpr int NX = MODE == TILE_UP_OUTER ? TH + 2 : 1;
+    double px[NX], py[NX];
+    if constexpr (MODE == TILE_UP_OUTER) {
+#pragma unroll
+        for (int j = 0; j < NX; ++j) {
+            const int r = row0 - 1 + j;
+            px[j] = py[j] = 0.0;
+            if (r >= 0 && r < down) {
+                const double2 xv = *reinterpret_cast<const double2 *>(ex.x64 + (long)r * across + ca);
+                px[j] = xv.x;
+                py[j] = xv.y;
+            }
+        }
+    }
+
     // Skewed schedule: half-sweep h reaches row i at step t = i + h. A colour-k update reads only
     // other-colour cells (its row's pair partner and the same column in the rows above and below),
     // which half-sweep h leaves untouched, so row i-1 being one half-sweep ahead is harmless; and
@@ -326,7 +343,7 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
 
     if constexpr (MODE == TILE_UP_OUTER) {
         // Rows HR-1 .. HR+TH of the tile: x64n = x64 + delta, zero beyond a pole (k_outer_residual's
-        // neighbours there). Every lane loads its own pair, so the writers' neighbour lanes hold
+        // neighbours there). Every lane holds its own pair, so the writers' neighbour lanes hold
         // theirs for the shifts; the correction is exact on those lanes (the column halo keeps two
         // columns of slack after NH half-sweeps).
         double vx[TH + 2], vy[TH + 2];
@@ -335,9 +352,8 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
             const int r = row0 - 1 + j;
             vx[j] = vy[j] = 0.0;
             if (r >= 0 && r < down) {
-                const double2 xv = *reinterpret_cast<const double2 *>(ex.x64 + (long)r * across + ca);
-                vx[j] = xv.x + (double)A[HR - 1 + j];
-                vy[j] = xv.y + (double)B[HR - 1 + j];
+                vx[j] = px[j] + (double)A[HR - 1 + j];
+                vy[j] = py[j] + (double)B[HR - 1 + j];
             }
         }
         unsigned long long mx = 0;
