@@ -261,6 +261,7 @@ def main() -> int:
     us_up = lib.cg_ocean_plan_time_kernel(plan.handle, 1, reps)
     us_down = lib.cg_ocean_plan_time_kernel(plan.handle, 0, reps)
     us_fused = lib.cg_ocean_plan_time_kernel(plan.handle, 3, reps)
+    us_bottom = lib.cg_ocean_plan_time_kernel(plan.handle, 4, reps)
     fused = us_fused > 0
     defaults = args.tile_rows == 8 and args.level_passes == "22222222"
     outer_gbs = OUTER_BYTES / (us_outer * 1e-6) / 1e9 if us_outer > 0 else None
@@ -268,7 +269,8 @@ def main() -> int:
     fused_gbs = FUSED_BYTES / (us_fused * 1e-6) / 1e9 if fused else None
     log(f"bench: outer residual {us_outer:.2f} us ({outer_gbs:.0f} GB/s algorithmic), finest up leg {us_up:.2f} us "
         f"({up_gbs:.0f} GB/s), finest down leg {us_down:.2f} us" +
-        (f", up leg with the outer step fused {us_fused:.2f} us ({fused_gbs:.0f} GB/s)" if fused else ""))
+        (f", up leg with the outer step fused {us_fused:.2f} us ({fused_gbs:.0f} GB/s)" if fused else "") +
+        f", bottom {us_bottom:.2f} us")
 
     batched = None
     if args.batch_reps > 0:
@@ -353,6 +355,8 @@ def main() -> int:
                 "traffic": MEASURED_TRAFFIC["up"] if defaults else None,
                 "down_leg_avg_launch_us": round(us_down, 3),
             },
+            "bottom": {"kernel": "k_bottom (the grids of 32x64 and smaller in one workgroup, latency-bound)",
+                       "avg_launch_us": round(us_bottom, 3)},
         },
         "cpu_baseline": cpu,
     }

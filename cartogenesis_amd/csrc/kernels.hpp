@@ -32,15 +32,13 @@ __device__ __forceinline__ double centre_at(const LevelArgs &L, int row, long ce
     return L.rowc ? L.rowc[row] : L.cellc[cell];
 }
 
-// Lane i receives lane i-1's value (wave_shr:1) / lane i+1's (wave_shl:1); the edge lane keeps
-// its own, which only a tile's halo ever reads.
+// Lane i receives lane i-1's value (wave_shr:1) / lane i+1's (wave_shl:1); the edge lane receives
+// 0 (bound_ctrl), which only a tile's halo ever reads. One DPP move, with no copy of an old value.
 __device__ __forceinline__ float lane_from_below(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, v), __builtin_bit_cast(int, v),
-                                                                 0x138, 0xf, 0xf, false));
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x138, 0xf, 0xf, true));
 }
 __device__ __forceinline__ float lane_from_above(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, v), __builtin_bit_cast(int, v),
-                                                                 0x130, 0xf, 0xf, false));
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x130, 0xf, 0xf, true));
 }
 __device__ __forceinline__ float lane_value(float v, int lane) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
@@ -223,7 +221,7 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
         // 0 or 1 per cell.
         const int r = row0 - HR + i;
         const int rc = (r >= 0 && r < down) ? r : down;
-        const long idx = (long)rc * across + ca;
+        const unsigned idx = (unsigned)rc * (unsigned)across + (unsigned)ca;  // cells < 2^32
         const float2 fv = *reinterpret_cast<const float2 *>(f + idx);
         const uint32_t m = *reinterpret_cast<const uint16_t *>(L.water + idx);
         fA[i] = fv.x;
@@ -277,8 +275,9 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
             const float botB = cm[qa + 1] * (1.0f - 0.25f) + cr[qa + 1] * 0.25f;
             const uint32_t mA = i < 32 ? (uint32_t)wA : (uint32_t)(wA >> 32);
             const uint32_t mB = i < 32 ? (uint32_t)wB : (uint32_t)(wB >> 32);
-            if ((mA >> (i & 31)) & 1) A[i] += topA * (1.0f - db) + botA * db;
-            if ((mB >> (i & 31)) & 1) B[i] += topB * (1.0f - db) + botB * db;
+            // Branch-free: on land the addend is +0, and a land cell holds +0, so it stays +0.
+            A[i] += keep_if(topA * (1.0f - db) + botA * db, mA, i & 31);
+            B[i] += keep_if(topB * (1.0f - db) + botB * db, mB, i & 31);
         }
     }
 
@@ -292,7 +291,7 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
             const int r = row0 - 1 + j;
             px[j] = py[j] = 0.0;
             if (r >= 0 && r < down) {
-                const double2 xv = *reinterpret_cast<const double2 *>(ex.x64 + (long)r * across + ca);
+                const double2 xv = *reinterpret_cast<const double2 *>(ex.x64 + ((unsigned)r * (unsigned)across + (unsigned)ca));
                 px[j] = xv.x;
                 py[j] = xv.y;
             }
@@ -337,7 +336,7 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
 #pragma unroll
         for (int i = HR; i < HR + TH; ++i) {
             const int r = row0 - HR + i;
-            if (r < down) *reinterpret_cast<float2 *>(xout + (long)r * across + cabs) = make_float2(A[i], B[i]);
+            if (r < down) *reinterpret_cast<float2 *>(xout + ((unsigned)r * (unsigned)across + (unsigned)cabs)) = make_float2(A[i], B[i]);
         }
     }
 
@@ -366,7 +365,7 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
             const float4 wt = make_float4(lane_value(wrow.x, i), lane_value(wrow.y, i), lane_value(wrow.z, i),
                                           lane_value(wrow.w, i));
             if (!writer || row >= down) continue;
-            const long idx = (long)row * across + cabs;
+            const unsigned idx = (unsigned)row * (unsigned)across + (unsigned)cabs;
             const float2 fv = *reinterpret_cast<const float2 *>(ex.f0 + idx);
             const double cen = L.rowc[row];
             const uint32_t mA = i < 32 ? (uint32_t)wA : (uint32_t)(wA >> 32);
@@ -983,17 +982,19 @@ __global__ __launch_bounds__(1024) void k_bottom(BottomArgs a) {
         const BottomView L = view(l), C = view(l + 1);
         lds_relax_pass(L);
         lds_relax_pass(L);
-        const int cpc = L.across / C.across, rpc = L.down / C.down;
+        // Each grid halves both axes into the next (the plan checks it): 2 x 2 blocks.
         if (ty < C.down && tx < C.across) {
             const int t = ty * C.across + tx;
             float acc = 0.0f;
             if (C.w[t]) {
-                for (int dr = 0; dr < rpc; ++dr) {
-                    const int row = ty * rpc + dr;
+#pragma unroll
+                for (int dr = 0; dr < 2; ++dr) {
+                    const int row = 2 * ty + dr;
                     const float4 wt = L.rw[row];
                     const float g = L.g[row];
-                    for (int dc = 0; dc < cpc; ++dc) {
-                        const int col = tx * cpc + dc;
+#pragma unroll
+                    for (int dc = 0; dc < 2; ++dc) {
+                        const int col = 2 * tx + dc;
                         const int cell = row * L.across + col;
                         if (!L.w[cell]) continue;
                         const int ce = col + 1 == L.across ? 0 : col + 1;
@@ -1041,23 +1042,26 @@ __global__ __launch_bounds__(1024) void k_bottom(BottomArgs a) {
     // owns fine cells (ty, 2 tx) and (ty, 2 tx + 1).
     for (int l = a.nlev - 2; l >= 0; --l) {
         const BottomView L = view(l), C = view(l + 1);
-        const int cpc = L.across / C.across, rpc = L.down / C.down;
         const int ca = C.across, cd = C.down;
         float pv[2] = {0.0f, 0.0f};
         const int row = ty;
         const bool mine = row < L.down && tx < (L.across >> 1);
         if (mine) {
-            const float crow = ((float)row + 0.5f) / (float)rpc - 0.5f;
+            // prolongAdd's positions with 2 fine cells per coarse one along each axis: dividing by
+            // 2.0f is exact, so the arithmetic is the reference's (OC:369-388).
+            const float crow = ((float)row + 0.5f) / 2.0f - 0.5f;
             const int above = (int)floorf(crow);
             const float db = crow - (float)above;
+#pragma unroll
             for (int k = 0; k < 2; ++k) {
                 const int col = 2 * tx + k;
                 const int cell = row * L.across + col;
                 if (!L.w[cell]) continue;
-                const float ccol = ((float)col + 0.5f) / (float)cpc - 0.5f;
+                const float ccol = ((float)col + 0.5f) / 2.0f - 0.5f;
                 const int left = (int)floorf(ccol);
                 const float ab = ccol - (float)left;
-                const int wc = (left + ca) % ca, ec = (left + 1) % ca;
+                // left is in -1 .. ca - 1.
+                const int wc = left < 0 ? left + ca : left, ec = left + 1 == ca ? 0 : left + 1;
                 auto at = [&](int rr, int cc) -> float {
                     if (rr < 0) return a.beyond * C.x[cc];
                     if (rr >= cd) return a.beyond * C.x[(cd - 1) * ca + cc];

@@ -534,7 +534,9 @@ int plan_fusion_setup(cg_ctx *ctx, cg_plan *p) {
         --b;
         bytes += level_bytes(b);
     }
-    for (int l = 0; l < b; ++l) {
+    // Every grid of the fused cycle halves both axes into the next (the bottom's transfers and the
+    // tiles' restriction assume 2 x 2 blocks).
+    for (int l = 0; l + 1 < n; ++l) {
         const Stencil *F = p->lv[l].st, *C = p->lv[l + 1].st;
         if (F->across != 2 * C->across || F->down != 2 * C->down || (F->down & 1)) return CG_OK;
     }
@@ -920,6 +922,52 @@ int level_half_sweeps(const cg_ctx *ctx, int l) {
     return 2 * (int)((ctx->level_passes >> (4 * l)) & 0xf);
 }
 
+// The bottom grids (levels bottom .. n-1) in one workgroup per instance: k_bottom.
+void bottom_launch(cg_plan *p, const float *in0, float *out0) {
+    hipStream_t s = p->ctx->stream;
+    const int n = p->n, b = p->bottom;
+    const float beyond = p->pole_is_wall ? -1.0f : 1.0f;
+    BottomArgs ba{};
+    ba.nlev = n - b;
+    for (int l = b; l < n; ++l) {
+        const PlanLevel &L = p->lv[l];
+        BottomLevel &bl = ba.lv[l - b];
+        bl.across = L.st->across;
+        bl.down = L.st->down;
+        bl.water = L.st->water;
+        bl.roww = L.st->roww;
+        bl.g = L.g;
+    }
+    size_t off = 0;
+    for (int i = 0; i < ba.nlev; ++i) {
+        const size_t cells = (size_t)ba.lv[i].across * ba.lv[i].down;
+        ba.lv[i].off_x = (int)off;
+        off += (cells * 4 + 15) & ~(size_t)15;
+        ba.lv[i].off_f = (int)off;
+        off += (cells * 4 + 15) & ~(size_t)15;
+        ba.lv[i].off_w = (int)off;
+        off += (cells + 15) & ~(size_t)15;
+        ba.lv[i].off_rw = (int)off;
+        off += (size_t)ba.lv[i].down * 16;
+        ba.lv[i].off_g = (int)off;
+        off += ((size_t)ba.lv[i].down * 4 + 15) & ~(size_t)15;
+    }
+    PlanLevel &B0 = p->lv[b];
+    ba.f_in = (b == 0 && in0) ? in0 : B0.f;
+    ba.x_out = (b == 0 && out0) ? out0 : B0.x[B0.cur];
+    ba.idx = p->d_idx;
+    ba.Mt = p->d_Mt;
+    // A coarsest grid with no water has the answer zero (its values start at zero); one with an
+    // inverse is solved directly; otherwise it is relaxed a fixed number of passes.
+    ba.nw = p->d_Mt ? p->nw : 0;
+    ba.coarsest_passes = (p->nw == 0) ? 0 : kPrecondCoarsestPasses;
+    ba.beyond = beyond;
+    ba.f_stride = vstride(B0.st);
+    ba.x_stride = vstride(B0.st);
+    ba.active = p->K > 1 ? p->d_active : nullptr;
+    hipLaunchKernelGGL(k_bottom, dim3(p->K), dim3(1024), p->bottom_lds, s, ba);
+}
+
 // Level 0's correction forcing for binary64 parity q when the outer step is fused into the finest
 // up leg: the two buffers alternate, since that launch writes the next forcing while its
 // neighbouring waves still read this one.
@@ -961,45 +1009,7 @@ int vcycle32_fused(cg_plan *p, bool timed, const float *in0, float *out0, bool f
         }
         L.cur = 1 - L.cur;
     }
-    BottomArgs ba{};
-    ba.nlev = n - b;
-    for (int l = b; l < n; ++l) {
-        const PlanLevel &L = p->lv[l];
-        BottomLevel &bl = ba.lv[l - b];
-        bl.across = L.st->across;
-        bl.down = L.st->down;
-        bl.water = L.st->water;
-        bl.roww = L.st->roww;
-        bl.g = L.g;
-    }
-    size_t off = 0;
-    for (int i = 0; i < ba.nlev; ++i) {
-        const size_t cells = (size_t)ba.lv[i].across * ba.lv[i].down;
-        ba.lv[i].off_x = (int)off;
-        off += (cells * 4 + 15) & ~(size_t)15;
-        ba.lv[i].off_f = (int)off;
-        off += (cells * 4 + 15) & ~(size_t)15;
-        ba.lv[i].off_w = (int)off;
-        off += (cells + 15) & ~(size_t)15;
-        ba.lv[i].off_rw = (int)off;
-        off += (size_t)ba.lv[i].down * 16;
-        ba.lv[i].off_g = (int)off;
-        off += ((size_t)ba.lv[i].down * 4 + 15) & ~(size_t)15;
-    }
-    PlanLevel &B0 = p->lv[b];
-    ba.f_in = (b == 0 && in0) ? in0 : B0.f;
-    ba.x_out = (b == 0 && out0) ? out0 : B0.x[B0.cur];
-    ba.idx = p->d_idx;
-    ba.Mt = p->d_Mt;
-    // A coarsest grid with no water has the answer zero (its values start at zero); one with an
-    // inverse is solved directly; otherwise it is relaxed a fixed number of passes.
-    ba.nw = p->d_Mt ? p->nw : 0;
-    ba.coarsest_passes = (p->nw == 0) ? 0 : kPrecondCoarsestPasses;
-    ba.beyond = beyond;
-    ba.f_stride = vstride(B0.st);
-    ba.x_stride = vstride(B0.st);
-    ba.active = p->K > 1 ? p->d_active : nullptr;
-    hipLaunchKernelGGL(k_bottom, dim3(p->K), dim3(1024), p->bottom_lds, s, ba);
+    bottom_launch(p, in0, out0);
     for (int l = b - 1; l >= 0; --l) {
         PlanLevel &L = p->lv[l];
         PlanLevel &C = p->lv[l + 1];
@@ -2129,7 +2139,7 @@ int cg_ocean_solve(cg_ctx *ctx, const cg_stencil *levels, int nlevels, const flo
 }
 
 double cg_ocean_plan_time_kernel(cg_plan *p, int kind, int reps) {
-    if (!p || reps < 1 || kind < 0 || kind > 3) return -1.0;
+    if (!p || reps < 1 || kind < 0 || kind > 4) return -1.0;
     cg_ctx *ctx = p->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     if (hipSetDevice(ctx->device) != hipSuccess) return -1.0;
@@ -2137,13 +2147,17 @@ double cg_ocean_plan_time_kernel(cg_plan *p, int kind, int reps) {
         set_err(ctx, "the plan's steps do not fuse the outer step");
         return -1.0;
     }
-    if (kind != 2 && (!p->fusable || p->bottom == 0)) {
+    if (kind != 2 && (!p->fusable || (kind != 4 && p->bottom == 0))) {
         set_err(ctx, "plan has no fused finest tile level to time");
         return -1.0;
     }
     PlanLevel &L = p->lv[0];
     const int nh = level_half_sweeps(ctx, 0), th = level_tile_rows(ctx, L.st);
     auto launch = [&]() -> hipError_t {
+        if (kind == 4) {
+            bottom_launch(p, nullptr, nullptr);
+            return hipGetLastError();
+        }
         if (kind == 2) {
             MaxReduce red;
             red.out = p->d_u64 + 3 * kMaxSlots;

@@ -173,7 +173,7 @@ double cg_ocean_kernel_time_us(cg_ctx *ctx, int kind, long *launches);
 /* Times `reps` back-to-back launches of one of the plan's kernels between two HIP events on the
  * context's stream and returns the average in microseconds (-1 on failure): kind 0 the finest
  * grid's fused down leg, 1 its fused up leg, 2 the binary64 outer residual, 3 the finest up leg
- * with the outer step fused in. Each launch reads the plan's current buffers and writes the same
+ * with the outer step fused in, 4 the one-workgroup bottom of the cycle (its coarsest grids). Each launch reads the plan's current buffers and writes the same
  * outputs (never the answer), so repeats leave the plan as one launch does; call it between
  * solves, not during one. */
 double cg_ocean_plan_time_kernel(cg_plan *plan, int kind, int reps);
