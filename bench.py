@@ -70,7 +70,7 @@ def cpu_baseline(levels, budget_s: float, threads: int):
                       f"binary64 C restatement of OceanCirculation.solve, {threads} OpenMP threads"}
 
 
-def batched_year(acc, prob, tol: float, reps: int, warmup: int):
+def batched_year(acc, prob, tol: float, reps: int, warmup: int, autotune: bool = True):
     """The 12 monthly forcings of the benchmark world solved together in one plan (one launch per
     kernel for all twelve), from zero to tol each: solves/s = 12 x reps / wall time."""
     from cartogenesis_amd.ocean import BatchPlan
@@ -82,6 +82,7 @@ def batched_year(acc, prob, tol: float, reps: int, warmup: int):
     try:
         for i, f in enumerate(forcings):
             plan.set_forcing(i, f)
+        tuned = plan.autotune(5) if autotune else None
         for _ in range(max(1, warmup // 2)):
             infos = plan.solve_all(tol)
         t0 = time.perf_counter()
@@ -100,6 +101,7 @@ def batched_year(acc, prob, tol: float, reps: int, warmup: int):
         "ms_per_batch": round(1000.0 * elapsed / reps, 4),
         "cycles": [int(i.cycles) for i in infos],
         "max_relative_residual": max(float(i.relative_residual) for i in infos),
+        "autotuned_tile_rows": tuned,
     }
 
 
@@ -181,6 +183,8 @@ def main() -> int:
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--tile-rows", type=int, default=8)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default tile heights instead of timing each grid's candidates at plan creation")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-oracle-check", action="store_true")
@@ -238,6 +242,10 @@ def main() -> int:
     method = cg.CG_METHOD_FGMRES if args.method == "fgmres" else cg.CG_METHOD_REFINED
     plan = Plan(acc, prob.levels)
     lib = cg.lib()
+    tuned = None
+    if not args.no_autotune:
+        tuned = plan.autotune(20)
+        log(f"bench: autotuned tile rows per grid {tuned}")
 
     log(f"bench: warmup {args.warmup}")
     for _ in range(args.warmup):
@@ -263,7 +271,7 @@ def main() -> int:
     us_fused = lib.cg_ocean_plan_time_kernel(plan.handle, 3, reps)
     us_bottom = lib.cg_ocean_plan_time_kernel(plan.handle, 4, reps)
     fused = us_fused > 0
-    defaults = args.tile_rows == 8 and args.level_passes == "22222222"
+    defaults = args.tile_rows == 8 and args.level_passes == "22222222" and (tuned is None or tuned[0] == 8)
     outer_gbs = OUTER_BYTES / (us_outer * 1e-6) / 1e9 if us_outer > 0 else None
     up_gbs = UP_BYTES / (us_up * 1e-6) / 1e9 if us_up > 0 else None
     fused_gbs = FUSED_BYTES / (us_fused * 1e-6) / 1e9 if fused else None
@@ -274,7 +282,7 @@ def main() -> int:
 
     batched = None
     if args.batch_reps > 0:
-        batched = batched_year(acc, prob, args.tol, args.batch_reps, args.warmup)
+        batched = batched_year(acc, prob, args.tol, args.batch_reps, args.warmup, not args.no_autotune)
     worlds = batched_worlds(acc, args.tol) if args.worlds else None
     combined = batched_combined(acc, args.tol) if args.combined else None
 
@@ -313,6 +321,7 @@ def main() -> int:
                         "iterative refinement; learned preconditioner not built",
             "rows": ROWS, "cols": COLS, "tol": args.tol, "method": args.method,
             "graph": not args.no_graph, "tile_rows": args.tile_rows, "level_passes": args.level_passes,
+            "autotuned_tile_rows": tuned,
             "cycles_per_solve": float(np.mean(cycles)),
             "relative_residual": rel_res,
             "rel_l2_vs_fp64_oracle": err,

@@ -875,10 +875,9 @@ constexpr int kBottomMaxLevels = 8;
 
 struct BottomLevel {
     int across, down;
-    const uint8_t *water;
-    const float4 *roww;
-    const float *g;  // restriction factor per row to the next level (null at the coarsest)
-    int off_x, off_f, off_w, off_rw, off_g;  // LDS byte offsets
+    // LDS byte offsets: values and forcing, then (in the static region) the mask, the row weights
+    // and the restriction factor per row to the next grid (0 at the coarsest).
+    int off_x, off_f, off_w, off_rw, off_g;
 };
 
 struct BottomArgs {
@@ -890,6 +889,10 @@ struct BottomArgs {
     // instance whose active flag is 0 is skipped.
     long f_stride, x_stride;
     const int *active;
+    // The static LDS region (masks, row weights, restriction factors of every grid) as uint32 words,
+    // and where it starts in LDS.
+    const uint32_t *blob;
+    int blob_words, static_base;
     const int *idx;     // coarsest water cells, and M = -(I - W)^-1 column-major (nw x nw)
     const float *Mt;
     int nw;
@@ -955,26 +958,23 @@ __global__ __launch_bounds__(1024) void k_bottom(BottomArgs a) {
         v.g = reinterpret_cast<const float *>(smem + a.lv[l].off_g);
         return v;
     };
+    // The grids' masks, row weights and restriction factors never change: the plan keeps them as
+    // one blob laid out as this LDS region, copied in one coalesced pass (one load per thread).
+    {
+        uint32_t *dst = reinterpret_cast<uint32_t *>(smem + a.static_base);
+        for (int c = threadIdx.x; c < a.blob_words; c += blockDim.x) dst[c] = a.blob[c];
+    }
     for (int l = 0; l < a.nlev; ++l) {
-        const BottomLevel &B = a.lv[l];
-        const int cells = B.across * B.down;
-        uint8_t *w = reinterpret_cast<uint8_t *>(smem + B.off_w);
-        float *x = reinterpret_cast<float *>(smem + B.off_x);
-        for (int c = threadIdx.x; c < cells; c += blockDim.x) {
-            w[c] = B.water[c];
-            x[c] = 0.0f;
-        }
-        float4 *rw = reinterpret_cast<float4 *>(smem + B.off_rw);
-        float *g = reinterpret_cast<float *>(smem + B.off_g);
-        if ((int)threadIdx.x < B.down) {
-            rw[threadIdx.x] = B.roww[threadIdx.x];
-            g[threadIdx.x] = B.g ? B.g[threadIdx.x] : 0.0f;
-        }
+        const int cells = a.lv[l].across * a.lv[l].down;
+        float *x = reinterpret_cast<float *>(smem + a.lv[l].off_x);
+        for (int c = threadIdx.x; c < cells; c += blockDim.x) x[c] = 0.0f;
     }
     {
+        // Pairs (the width is even), so a 32 x 64 grid arrives in one load per thread.
         const int cells = a.lv[0].across * a.lv[0].down;
-        float *f = reinterpret_cast<float *>(smem + a.lv[0].off_f);
-        for (int c = threadIdx.x; c < cells; c += blockDim.x) f[c] = f_in[c];
+        float2 *f = reinterpret_cast<float2 *>(smem + a.lv[0].off_f);
+        const float2 *src = reinterpret_cast<const float2 *>(f_in);
+        for (int c = threadIdx.x; 2 * c < cells; c += blockDim.x) f[c] = src[c];
     }
     __syncthreads();
     // Down: relax twice, then the binary32 residual restricted with g (k_residual_restrict32's sums).
@@ -1084,8 +1084,9 @@ __global__ __launch_bounds__(1024) void k_bottom(BottomArgs a) {
     }
     {
         const int cells = a.lv[0].across * a.lv[0].down;
-        const float *x = reinterpret_cast<const float *>(smem + a.lv[0].off_x);
-        for (int c = threadIdx.x; c < cells; c += blockDim.x) x_out[c] = x[c];
+        const float2 *x = reinterpret_cast<const float2 *>(smem + a.lv[0].off_x);
+        float2 *dst = reinterpret_cast<float2 *>(x_out);
+        for (int c = threadIdx.x; 2 * c < cells; c += blockDim.x) dst[c] = x[c];
     }
 }
 

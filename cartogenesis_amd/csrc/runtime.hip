@@ -238,6 +238,7 @@ hipError_t tile_dispatch(cg_ctx *ctx, const LevelArgs &L, const float *f, const 
                          bool zero, int mode = TILE_PLAIN, const TileExtra &ex = TileExtra{}, int rows = 0,
                          int instances = 1) {
     switch (rows ? rows : ctx->tile_rows) {
+        case 2: return launch_tile<2>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex, instances);
         case 4: return launch_tile<4>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex, instances);
         case 8: return launch_tile<8>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex, instances);
         case 24: return launch_tile<24>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex, instances);
@@ -315,6 +316,8 @@ struct cg_plan {
     PlanLevel lv[kMaxLevels];
     float *f0 = nullptr;          // the finest forcing
     float *f_alt = nullptr;       // with the fused outer step: level 0's other correction forcing
+    uint32_t *d_bblob = nullptr;  // k_bottom's static LDS region
+    int tuned_rows[kMaxLevels] = {0};  // per level: tile rows the autotuner chose (0: the default rule)
     double *r64 = nullptr;        // finest-size binary64 scratch
     double *x64[2] = {nullptr, nullptr};
     int x64cur = 0;
@@ -377,6 +380,7 @@ void free_plan(cg_plan *p) {
     }
     dfree(p->f0);
     dfree(p->f_alt);
+    dfree(p->d_bblob);
     dfree(p->r64);
     for (double *v : {p->kx, p->kr, p->ksh, p->kp, p->kv, p->kph, p->kshat, p->kt, p->kbal, p->kpart}) dfree(v);
     dfree(p->kf32);
@@ -498,6 +502,60 @@ size_t bottom_lds_limit(cg_ctx *ctx) {
 }
 
 // Restriction factors per row, and which levels the fused cycle runs in one workgroup.
+// k_bottom's LDS layout for levels bottom .. n-1: every grid's values and forcing, then the static
+// region (each grid's mask, row weights, restriction factors), 16-byte aligned pieces.
+void bottom_layout(const cg_plan *p, BottomArgs *ba) {
+    const int n = p->n, b = p->bottom;
+    ba->nlev = n - b;
+    for (int l = b; l < n; ++l) {
+        BottomLevel &bl = ba->lv[l - b];
+        bl.across = p->lv[l].st->across;
+        bl.down = p->lv[l].st->down;
+    }
+    size_t off = 0;
+    for (int i = 0; i < ba->nlev; ++i) {
+        const size_t cells = (size_t)ba->lv[i].across * ba->lv[i].down;
+        ba->lv[i].off_x = (int)off;
+        off += (cells * 4 + 15) & ~(size_t)15;
+        ba->lv[i].off_f = (int)off;
+        off += (cells * 4 + 15) & ~(size_t)15;
+    }
+    ba->static_base = (int)off;
+    for (int i = 0; i < ba->nlev; ++i) {
+        const size_t cells = (size_t)ba->lv[i].across * ba->lv[i].down;
+        ba->lv[i].off_w = (int)off;
+        off += (cells + 15) & ~(size_t)15;
+        ba->lv[i].off_rw = (int)off;
+        off += (size_t)ba->lv[i].down * 16;
+        ba->lv[i].off_g = (int)off;
+        off += ((size_t)ba->lv[i].down * 4 + 15) & ~(size_t)15;
+    }
+    ba->blob_words = (int)((off - (size_t)ba->static_base) / 4);
+}
+
+// The static region's bytes, from the stencils' host copies: mask (0/1), row weights, and
+// g(row) = centre(row) / 4 / centre_next(row / 2) in binary32 (the tiles' restriction factor).
+int build_bottom_blob(cg_ctx *ctx, cg_plan *p) {
+    BottomArgs ba{};
+    bottom_layout(p, &ba);
+    std::vector<uint32_t> words((size_t)ba.blob_words, 0u);
+    unsigned char *base = reinterpret_cast<unsigned char *>(words.data()) - ba.static_base;
+    for (int i = 0; i < ba.nlev; ++i) {
+        const Stencil *st = p->lv[p->bottom + i].st;
+        for (long c = 0; c < st->cells; ++c) base[ba.lv[i].off_w + c] = st->h_water[c] ? 1 : 0;
+        std::memcpy(base + ba.lv[i].off_rw, st->h_w.data(), sizeof(float4) * (size_t)st->down);
+        float *g = reinterpret_cast<float *>(base + ba.lv[i].off_g);
+        if (i + 1 < ba.nlev) {
+            const Stencil *C = p->lv[p->bottom + i + 1].st;
+            for (int row = 0; row < st->down; ++row) g[row] = (float)(st->h_c[row] * 0.25 / C->h_c[row / 2]);
+        }
+    }
+    int rc;
+    if ((rc = dalloc(ctx, &p->d_bblob, words.size()))) return rc;
+    HIPCHK(ctx, hipMemcpy(p->d_bblob, words.data(), sizeof(uint32_t) * words.size(), hipMemcpyHostToDevice));
+    return CG_OK;
+}
+
 int plan_fusion_setup(cg_ctx *ctx, cg_plan *p) {
     const int n = p->n;
     bool rowconst = true;
@@ -542,6 +600,8 @@ int plan_fusion_setup(cg_ctx *ctx, cg_plan *p) {
     }
     p->bottom = b;
     p->bottom_lds = bytes;
+    int rc;
+    if ((rc = build_bottom_blob(ctx, p))) return rc;
     p->fusable = true;
     return CG_OK;
 }
@@ -925,33 +985,11 @@ int level_half_sweeps(const cg_ctx *ctx, int l) {
 // The bottom grids (levels bottom .. n-1) in one workgroup per instance: k_bottom.
 void bottom_launch(cg_plan *p, const float *in0, float *out0) {
     hipStream_t s = p->ctx->stream;
-    const int n = p->n, b = p->bottom;
+    const int b = p->bottom;
     const float beyond = p->pole_is_wall ? -1.0f : 1.0f;
     BottomArgs ba{};
-    ba.nlev = n - b;
-    for (int l = b; l < n; ++l) {
-        const PlanLevel &L = p->lv[l];
-        BottomLevel &bl = ba.lv[l - b];
-        bl.across = L.st->across;
-        bl.down = L.st->down;
-        bl.water = L.st->water;
-        bl.roww = L.st->roww;
-        bl.g = L.g;
-    }
-    size_t off = 0;
-    for (int i = 0; i < ba.nlev; ++i) {
-        const size_t cells = (size_t)ba.lv[i].across * ba.lv[i].down;
-        ba.lv[i].off_x = (int)off;
-        off += (cells * 4 + 15) & ~(size_t)15;
-        ba.lv[i].off_f = (int)off;
-        off += (cells * 4 + 15) & ~(size_t)15;
-        ba.lv[i].off_w = (int)off;
-        off += (cells + 15) & ~(size_t)15;
-        ba.lv[i].off_rw = (int)off;
-        off += (size_t)ba.lv[i].down * 16;
-        ba.lv[i].off_g = (int)off;
-        off += ((size_t)ba.lv[i].down * 4 + 15) & ~(size_t)15;
-    }
+    bottom_layout(p, &ba);
+    ba.blob = p->d_bblob;
     PlanLevel &B0 = p->lv[b];
     ba.f_in = (b == 0 && in0) ? in0 : B0.f;
     ba.x_out = (b == 0 && out0) ? out0 : B0.x[B0.cur];
@@ -984,66 +1022,81 @@ inline bool outer_fused(const cg_plan *p) {
 // With fuse_outer the finest up leg also runs the binary64 outer step (TILE_UP_OUTER): it reads
 // x64[x64cur] and writes x64[1 - x64cur], the next cycle's forcing (level0_forcing(p, 1 - x64cur)) and
 // the maxima of parity x64cur; the caller flips x64cur.
+// Level l's tile rows: the autotuner's choice for this plan, else the context's rule.
+int plan_tile_rows(const cg_plan *p, int l) {
+    return p->tuned_rows[l] ? p->tuned_rows[l] : level_tile_rows(p->ctx, p->lv[l].st);
+}
+
+// Level l's down leg (smooth from zero, residual, restriction into level l + 1's forcing) with
+// `rows` tile rows; level 0 reads in0 when given. Leaves the result in L.x[1 - L.cur].
+hipError_t down_leg(cg_plan *p, int l, const float *in0, int rows) {
+    PlanLevel &L = p->lv[l];
+    PlanLevel &C = p->lv[l + 1];
+    TileExtra ex{};
+    ex.cwater = C.st->water;
+    ex.g = L.g;
+    ex.fc = C.f;
+    ex.fs = vstride(L.st);
+    ex.cs = vstride(C.st);
+    ex.active = p->K > 1 ? p->d_active : nullptr;
+    const float *fl = (l == 0 && in0) ? in0 : L.f;
+    return tile_dispatch(p->ctx, L.st->args(), fl, L.x[L.cur], L.x[1 - L.cur], level_half_sweeps(p->ctx, l), true,
+                         TILE_DOWN, ex, rows, p->K);
+}
+
+// Level l's up leg (prolongation of level l + 1's values, smoothing) with `rows` tile rows; at
+// level 0 with fuse_outer, the outer step too (TILE_UP_OUTER, see vcycle32_fused).
+hipError_t up_leg(cg_plan *p, int l, const float *in0, float *out0, bool fuse_outer, int rows) {
+    PlanLevel &L = p->lv[l];
+    PlanLevel &C = p->lv[l + 1];
+    TileExtra ex{};
+    ex.xc = C.x[C.cur];
+    ex.ca = C.st->across;
+    ex.cd = C.st->down;
+    ex.beyond = p->pole_is_wall ? -1.0f : 1.0f;
+    ex.fs = vstride(L.st);
+    ex.cs = vstride(C.st);
+    ex.active = p->K > 1 ? p->d_active : nullptr;
+    const float *fl = (l == 0 && in0) ? in0 : L.f;
+    float *xo = (l == 0 && out0) ? out0 : L.x[1 - L.cur];
+    int mode = TILE_UP;
+    if (l == 0 && fuse_outer) {
+        const int q = p->x64cur;
+        mode = TILE_UP_OUTER;
+        ex.x64 = p->x64[q];
+        ex.x64n = p->x64[1 - q];
+        ex.f0 = p->f0;
+        ex.fcn = level0_forcing(p, 1 - q);
+        ex.red.out = p->d_u64 + (p->red_base + q) * kMaxSlots;
+        ex.red.clear = p->d_u64 + (p->red_base + 1 - q) * kMaxSlots;
+        ex.s64 = L.st->cells;
+        ex.red_stride = 2 * kMaxSlots;
+    }
+    return tile_dispatch(p->ctx, L.st->args(), fl, L.x[L.cur], xo, level_half_sweeps(p->ctx, l), false, mode, ex, rows,
+                         p->K);
+}
+
 int vcycle32_fused(cg_plan *p, bool timed, const float *in0, float *out0, bool fuse_outer) {
     cg_ctx *ctx = p->ctx;
-    hipStream_t s = ctx->stream;
-    const int n = p->n, b = p->bottom;
-    const float beyond = p->pole_is_wall ? -1.0f : 1.0f;
+    const int b = p->bottom;
     for (int l = 0; l < b; ++l) {
-        PlanLevel &L = p->lv[l];
-        PlanLevel &C = p->lv[l + 1];
-        TileExtra ex{};
-        ex.cwater = C.st->water;
-        ex.g = L.g;
-        ex.fc = C.f;
-        ex.fs = vstride(L.st);
-        ex.cs = vstride(C.st);
-        ex.active = p->K > 1 ? p->d_active : nullptr;
         TimeScope ts(ctx, timed && l == 0, kKindSmoothFromZero);
-        const float *fl = (l == 0 && in0) ? in0 : L.f;
-        hipError_t e = tile_dispatch(ctx, L.st->args(), fl, L.x[L.cur], L.x[1 - L.cur], level_half_sweeps(ctx, l), true,
-                                     TILE_DOWN, ex, level_tile_rows(ctx, L.st), p->K);
+        hipError_t e = down_leg(p, l, in0, plan_tile_rows(p, l));
         if (e != hipSuccess) {
             set_err(ctx, "down-leg launch failed: %s", hipGetErrorString(e));
             return CG_ERR_DEVICE;
         }
-        L.cur = 1 - L.cur;
+        p->lv[l].cur = 1 - p->lv[l].cur;
     }
     bottom_launch(p, in0, out0);
     for (int l = b - 1; l >= 0; --l) {
-        PlanLevel &L = p->lv[l];
-        PlanLevel &C = p->lv[l + 1];
-        TileExtra ex{};
-        ex.xc = C.x[C.cur];
-        ex.ca = C.st->across;
-        ex.cd = C.st->down;
-        ex.beyond = beyond;
-        ex.fs = vstride(L.st);
-        ex.cs = vstride(C.st);
-        ex.active = p->K > 1 ? p->d_active : nullptr;
         TimeScope ts(ctx, timed && l == 0, kKindSmooth);
-        const float *fl = (l == 0 && in0) ? in0 : L.f;
-        float *xo = (l == 0 && out0) ? out0 : L.x[1 - L.cur];
-        int mode = TILE_UP;
-        if (l == 0 && fuse_outer) {
-            const int q = p->x64cur;
-            mode = TILE_UP_OUTER;
-            ex.x64 = p->x64[q];
-            ex.x64n = p->x64[1 - q];
-            ex.f0 = p->f0;
-            ex.fcn = level0_forcing(p, 1 - q);
-            ex.red.out = p->d_u64 + (p->red_base + q) * kMaxSlots;
-            ex.red.clear = p->d_u64 + (p->red_base + 1 - q) * kMaxSlots;
-            ex.s64 = L.st->cells;
-            ex.red_stride = 2 * kMaxSlots;
-        }
-        hipError_t e = tile_dispatch(ctx, L.st->args(), fl, L.x[L.cur], xo, level_half_sweeps(ctx, l), false, mode,
-                                     ex, level_tile_rows(ctx, L.st), p->K);
+        hipError_t e = up_leg(p, l, in0, out0, fuse_outer, plan_tile_rows(p, l));
         if (e != hipSuccess) {
             set_err(ctx, "up-leg launch failed: %s", hipGetErrorString(e));
             return CG_ERR_DEVICE;
         }
-        L.cur = 1 - L.cur;
+        p->lv[l].cur = 1 - p->lv[l].cur;
     }
     HIPCHK(ctx, hipGetLastError());
     return CG_OK;
@@ -2152,7 +2205,7 @@ double cg_ocean_plan_time_kernel(cg_plan *p, int kind, int reps) {
         return -1.0;
     }
     PlanLevel &L = p->lv[0];
-    const int nh = level_half_sweeps(ctx, 0), th = level_tile_rows(ctx, L.st);
+    const int nh = level_half_sweeps(ctx, 0), th = plan_tile_rows(p, 0);
     auto launch = [&]() -> hipError_t {
         if (kind == 4) {
             bottom_launch(p, nullptr, nullptr);
@@ -2209,6 +2262,91 @@ double cg_ocean_plan_time_kernel(cg_plan *p, int kind, int reps) {
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     return ok ? 1000.0 * ms / reps : -1.0;
+}
+
+int cg_ocean_plan_autotune(cg_plan *p, int reps, int *rows_out) {
+    if (!p || reps < 1) return CG_ERR_ARG;
+    cg_ctx *ctx = p->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (!(ctx->fused && p->fusable)) {
+        set_err(ctx, "only the fused cycle's tiles are tuned");
+        return CG_ERR_STATE;
+    }
+    static const int kCandidates[] = {2, 4, 8, 16, 24, 32};
+    const bool graph = ctx->use_graph && !ctx->time_kernels;
+    auto drop_graphs = [&]() {
+        for (int i = 0; i < 2; ++i)
+            if (p->step_exec[i]) {
+                (void)hipGraphExecDestroy(p->step_exec[i]);
+                p->step_exec[i] = nullptr;
+            }
+    };
+    // A refinement in progress from zero, as a solve starts one, so the steps timed are the solve's.
+    const int K = p->K;
+    const Stencil *st = p->lv[0].st;
+    HIPCHK(ctx, hipMemsetAsync(p->x64[p->x64cur], 0, sizeof(double) * st->cells * K, ctx->stream));
+    if (K > 1) {
+        for (int i = 0; i < K; ++i) p->h_active[i] = 1;
+        HIPCHK(ctx, hipMemcpyAsync(p->d_active, p->h_active, sizeof(int) * K, hipMemcpyHostToDevice, ctx->stream));
+    }
+    hipLaunchKernelGGL(k_fill_u64, dim3(2 * K), dim3(kMaxSlots), 0, ctx->stream, p->d_u64 + p->red_base * kMaxSlots,
+                       2 * K * kMaxSlots, 0ull);
+    int rc = outer_step(p, nullptr, outer_fused(p) ? level0_forcing(p, 1 - p->x64cur) : nullptr);
+    if (rc) return rc;
+    hipEvent_t a, b;
+    HIPCHK(ctx, hipEventCreate(&a));
+    HIPCHK(ctx, hipEventCreate(&b));
+    // Coordinate descent over the grids, finest first: each candidate height timed over `reps`
+    // whole refinement steps (graph replays where graphs are on), the other grids as chosen so far.
+    auto time_steps = [&](double *ms_out) -> int {
+        drop_graphs();
+        int r;
+        if (graph && (r = build_step_graphs(p))) return r;
+        auto step = [&]() -> int {
+            if (graph) {
+                HIPCHK(ctx, hipGraphLaunch(p->step_exec[p->x64cur], ctx->stream));
+                p->x64cur = 1 - p->x64cur;
+                return CG_OK;
+            }
+            return refine_step(p, false);
+        };
+        for (int i = 0; i < 2; ++i)
+            if ((r = step())) return r;
+        HIPCHK(ctx, hipEventRecord(a, ctx->stream));
+        for (int i = 0; i < reps; ++i)
+            if ((r = step())) return r;
+        HIPCHK(ctx, hipEventRecord(b, ctx->stream));
+        HIPCHK(ctx, hipEventSynchronize(b));
+        float ms = 0.0f;
+        HIPCHK(ctx, hipEventElapsedTime(&ms, a, b));
+        *ms_out = ms;
+        return CG_OK;
+    };
+    for (int l = 0; l < p->bottom && rc == CG_OK; ++l) {
+        const int keep = plan_tile_rows(p, l);
+        double best = -1.0;
+        int best_rows = keep;
+        for (int rows : kCandidates) {
+            if (rows > p->lv[l].st->down) continue;
+            p->tuned_rows[l] = rows;
+            double ms = 0.0;
+            if ((rc = time_steps(&ms))) break;
+            if (best < 0.0 || ms < best) {
+                best = ms;
+                best_rows = rows;
+            }
+        }
+        p->tuned_rows[l] = rc ? keep : best_rows;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    drop_graphs();
+    p->x64_valid = false;
+    if (rc) return rc;
+    if (rows_out)
+        for (int l = 0; l < p->n; ++l) rows_out[l] = l < p->bottom ? p->tuned_rows[l] : 0;
+    return CG_OK;
 }
 
 double cg_ocean_kernel_time_us(cg_ctx *ctx, int kind, long *launches) {

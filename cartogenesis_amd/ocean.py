@@ -461,6 +461,13 @@ class Plan:
               self.acc.ctx, "cg_ocean_plan_correct_bodies")
         return out
 
+    def autotune(self, reps: int = 20) -> List[int]:
+        """The plan's tile heights per grid, chosen by timing (cg_ocean_plan_autotune); 0 where a grid
+        is not tiled."""
+        rows = (ctypes.c_int * len(self.levels))()
+        check(lib().cg_ocean_plan_autotune(self._p, int(reps), rows), self.acc.ctx, "cg_ocean_plan_autotune")
+        return list(rows)
+
     def set_start(self, start) -> None:
         """The values a warm solve starts from."""
         check(lib().cg_ocean_plan_set_start(self._p, np.ascontiguousarray(start, dtype=np.float32)),
@@ -501,6 +508,12 @@ class BatchPlan:
     def set_forcing(self, instance: int, forcing) -> None:
         check(lib().cg_ocean_plan_set_forcing_at(self._p, int(instance), np.ascontiguousarray(forcing, dtype=np.float32)),
               self.acc.ctx, "cg_ocean_plan_set_forcing_at")
+
+    def autotune(self, reps: int = 20) -> List[int]:
+        """As Plan.autotune, timing the launches of all instances."""
+        rows = (ctypes.c_int * len(self.levels))()
+        check(lib().cg_ocean_plan_autotune(self._p, int(reps), rows), self.acc.ctx, "cg_ocean_plan_autotune")
+        return list(rows)
 
     def solve_all(self, tol: float) -> List[CgSolution]:
         infos = (CgSolution * self.instances)()

@@ -148,7 +148,8 @@ int cg_ocean_plan_read_at(cg_plan *plan, int instance, double *out64, float *out
 int cg_ocean_plan_set_forcing(cg_plan *plan, const float *forcing);
 /* Solves from zero (or from the plan's current answer if warm != 0) to tol. */
 int cg_ocean_plan_solve(cg_plan *plan, double tol, int method, int warm, cg_solution *info);
-/* Solves several plans of one context concurrently, each from zero, to one tol. */
+/* Solves several plans of one context one after another, each from zero, to one tol (plans of one
+ * instance; right-hand sides on one ladder go faster as one batched plan, below). */
 int cg_ocean_plan_solve_batch(cg_plan *const *plans, int count, double tol, int method,
                               cg_solution *infos);
 /* The finest grid's water bodies (OceanCirculation.WaterBodies, OC:670-697: body per cell, -1 on
@@ -201,6 +202,14 @@ int cg_ocean_plan_set_stress(cg_plan *plan, double width_m, double height_m, con
  * on the device; either output may be NULL. */
 int cg_ocean_plan_velocities(cg_plan *plan, double width_m, double height_m, float *eastward,
                              float *northward);
+
+/* The plan's tile autotuner: for every grid of the fused cycle above the bottom, finest first, times
+ * `reps` whole refinement steps (graph replays, HIP events on the context's stream) with that grid's
+ * tiles 2, 4, 8, 16, 24 and 32 rows high and the other grids as chosen so far, and keeps the fastest;
+ * the plan's later solves use the choice (results do not depend on the tiles). rows_out (NULL or
+ * nlevels ints) receives the height per grid, 0 where a grid is not tiled. Call it between solves;
+ * it leaves the plan without an answer to start a warm solve from. */
+int cg_ocean_plan_autotune(cg_plan *plan, int reps, int *rows_out);
 
 /* ---- host-side problem builders (no device needed) --------------------------------------------- */
 

@@ -102,3 +102,36 @@ def test_a_year_at_full_size(acc):
             assert infos[i].relative_residual < 1e-8 and res < 1.01e-8, f"month {i}: {res}"
     finally:
         batch.close()
+
+
+def test_autotuned_tiles_change_no_bits(acc):
+    """The autotuner picks tile heights per grid by timing; the tiles only partition the work, so
+    the answer, steps and residual are those of the default tiles, for one instance and for a batch."""
+    p = circulation_problem(512, seed=4)
+    base = Plan(acc, p.levels)
+    tuned = Plan(acc, p.levels)
+    try:
+        a = base.solve(1e-10, CG_METHOD_REFINED)
+        rows = tuned.autotune(5)
+        print(f"OCEAN autotuned tile rows 1024x512: {rows}", flush=True)
+        assert rows[0] in (2, 4, 8, 16, 24, 32)
+        b = tuned.solve(1e-10, CG_METHOD_REFINED)
+        assert a.cycles == b.cycles and a.relative_residual == b.relative_residual
+        assert np.array_equal(base.read()[0].view(np.uint64), tuned.read()[0].view(np.uint64))
+    finally:
+        base.close()
+        tuned.close()
+    forcings = monthly_forcings(p, months=(1, 7))
+    batch = BatchPlan(acc, p.levels, 2)
+    try:
+        for i, f in enumerate(forcings):
+            batch.set_forcing(i, f)
+        first = batch.solve_all(1e-9)
+        x0 = [batch.read(i)[0] for i in range(2)]
+        batch.autotune(3)
+        again = batch.solve_all(1e-9)
+        for i in range(2):
+            assert first[i].cycles == again[i].cycles
+            assert np.array_equal(batch.read(i)[0].view(np.uint64), x0[i].view(np.uint64))
+    finally:
+        batch.close()
