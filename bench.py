@@ -41,7 +41,7 @@ UP_BYTES = 14 * CELLS
 FUSED_BYTES = 34 * CELLS
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE and WRITE_SIZE (scaled by a 1 GiB copy),
 # for the default settings (tile rows 8, two passes per grid); see profiles/hbm_traffic_*.md.
-MEASURED_TRAFFIC = {"outer": 78181417, "up": 30654638, "fused": None}
+MEASURED_TRAFFIC = {"outer": 78691287, "up": 30635786, "fused": {4: 73281985, 8: 75150788}}
 
 
 def log(msg: str) -> None:
@@ -185,6 +185,8 @@ def main() -> int:
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default tile heights instead of timing each grid's candidates at plan creation")
+    ap.add_argument("--rows", default=None,
+                    help="comma-separated tile heights per grid (0: default rule) instead of autotuning, e.g. 4,4,2,2,2")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-oracle-check", action="store_true")
@@ -243,7 +245,12 @@ def main() -> int:
     plan = Plan(acc, prob.levels)
     lib = cg.lib()
     tuned = None
-    if not args.no_autotune:
+    if args.rows:
+        tuned = [int(v) for v in args.rows.split(",")]
+        tuned += [0] * (len(prob.levels) - len(tuned))
+        plan.set_tile_rows(tuned)
+        log(f"bench: tile rows per grid {tuned}")
+    elif not args.no_autotune:
         tuned = plan.autotune(20)
         log(f"bench: autotuned tile rows per grid {tuned}")
 
@@ -271,7 +278,8 @@ def main() -> int:
     us_fused = lib.cg_ocean_plan_time_kernel(plan.handle, 3, reps)
     us_bottom = lib.cg_ocean_plan_time_kernel(plan.handle, 4, reps)
     fused = us_fused > 0
-    defaults = args.tile_rows == 8 and args.level_passes == "22222222" and (tuned is None or tuned[0] == 8)
+    defaults = args.tile_rows == 8 and args.level_passes == "22222222"
+    rows0 = tuned[0] if tuned and tuned[0] else 8  # the finest grid's tile rows
     outer_gbs = OUTER_BYTES / (us_outer * 1e-6) / 1e9 if us_outer > 0 else None
     up_gbs = UP_BYTES / (us_up * 1e-6) / 1e9 if us_up > 0 else None
     fused_gbs = FUSED_BYTES / (us_fused * 1e-6) / 1e9 if fused else None
@@ -335,7 +343,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(fused_gbs / HBM_PEAK_GBS, 4),
-            "traffic": MEASURED_TRAFFIC["fused"] if defaults else None,
+            "traffic": MEASURED_TRAFFIC["fused"].get(rows0) if defaults else None,
+            "tile_rows": rows0,
             "avg_launch_us": round(us_fused, 3),
             "algorithmic_bytes_per_launch": FUSED_BYTES,
             "separate_outer_residual": {"kernel": "k_outer_residual", "avg_launch_us": round(us_outer, 3),

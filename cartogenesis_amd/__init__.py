@@ -66,6 +66,7 @@ SIGNATURES = {
     "cg_ocean_kernel_time_us": (_dbl, [_vp, _int, ctypes.POINTER(_long)]),
     "cg_ocean_plan_time_kernel": (_dbl, [_vp, _int, _int]),
     "cg_ocean_plan_autotune": (_int, [_vp, _int, _vp]),
+    "cg_ocean_plan_set_tile_rows": (_int, [_vp, _vp]),
     "cg_ocean_water_on": (_int, [_vp, _u8p, _int, _int, _int, _int, _u8p]),
     "cg_ocean_curl_forcing": (_int, [_vp, _int, _int, _dbl, _dbl, _f64p, _f64p, _vp, _f64p]),
     "cg_ocean_plan_set_stress": (_int, [_vp, _dbl, _dbl, _f64p, _f64p, _vp]),

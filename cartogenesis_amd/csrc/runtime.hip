@@ -2264,6 +2264,25 @@ double cg_ocean_plan_time_kernel(cg_plan *p, int kind, int reps) {
     return ok ? 1000.0 * ms / reps : -1.0;
 }
 
+int cg_ocean_plan_set_tile_rows(cg_plan *p, const int *rows) {
+    if (!p || !rows) return CG_ERR_ARG;
+    cg_ctx *ctx = p->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    for (int l = 0; l < p->n; ++l)
+        if (rows[l] != 0 && rows[l] != 2 && rows[l] != 4 && rows[l] != 8 && rows[l] != 16 && rows[l] != 24 &&
+            rows[l] != 32) {
+            set_err(ctx, "tile rows of grid %d must be 0, 2, 4, 8, 16, 24 or 32", l);
+            return CG_ERR_ARG;
+        }
+    for (int l = 0; l < p->n; ++l) p->tuned_rows[l] = rows[l];
+    for (int i = 0; i < 2; ++i)
+        if (p->step_exec[i]) {
+            (void)hipGraphExecDestroy(p->step_exec[i]);
+            p->step_exec[i] = nullptr;
+        }
+    return CG_OK;
+}
+
 int cg_ocean_plan_autotune(cg_plan *p, int reps, int *rows_out) {
     if (!p || reps < 1) return CG_ERR_ARG;
     cg_ctx *ctx = p->ctx;
@@ -2323,20 +2342,34 @@ int cg_ocean_plan_autotune(cg_plan *p, int reps, int *rows_out) {
         *ms_out = ms;
         return CG_OK;
     };
+    // Steps before any timing, so the clocks have left their idle state; then each candidate is
+    // timed twice, in forward and in reverse order, and its faster time counts, so a clock still
+    // ramping favours no end of the list.
+    {
+        double warm = 0.0;
+        for (int i = 0; i < 4 && rc == CG_OK; ++i) rc = time_steps(&warm);
+    }
+    constexpr int kNumCandidates = (int)(sizeof(kCandidates) / sizeof(kCandidates[0]));
     for (int l = 0; l < p->bottom && rc == CG_OK; ++l) {
         const int keep = plan_tile_rows(p, l);
-        double best = -1.0;
-        int best_rows = keep;
-        for (int rows : kCandidates) {
-            if (rows > p->lv[l].st->down) continue;
-            p->tuned_rows[l] = rows;
-            double ms = 0.0;
-            if ((rc = time_steps(&ms))) break;
-            if (best < 0.0 || ms < best) {
-                best = ms;
-                best_rows = rows;
+        double best_ms[kNumCandidates];
+        for (int c = 0; c < kNumCandidates; ++c) best_ms[c] = -1.0;
+        for (int round = 0; round < 2 && rc == CG_OK; ++round)
+            for (int k = 0; k < kNumCandidates; ++k) {
+                const int c = round == 0 ? k : kNumCandidates - 1 - k;
+                if (kCandidates[c] > p->lv[l].st->down) continue;
+                p->tuned_rows[l] = kCandidates[c];
+                double ms = 0.0;
+                if ((rc = time_steps(&ms))) break;
+                if (best_ms[c] < 0.0 || ms < best_ms[c]) best_ms[c] = ms;
             }
-        }
+        int best_rows = keep;
+        double best = -1.0;
+        for (int c = 0; c < kNumCandidates; ++c)
+            if (best_ms[c] >= 0.0 && (best < 0.0 || best_ms[c] < best)) {
+                best = best_ms[c];
+                best_rows = kCandidates[c];
+            }
         p->tuned_rows[l] = rc ? keep : best_rows;
     }
     (void)hipEventDestroy(a);

@@ -468,6 +468,11 @@ class Plan:
         check(lib().cg_ocean_plan_autotune(self._p, int(reps), rows), self.acc.ctx, "cg_ocean_plan_autotune")
         return list(rows)
 
+    def set_tile_rows(self, rows) -> None:
+        """Tile heights per grid (0 for the default rule), as autotune returns them."""
+        arr = (ctypes.c_int * len(self.levels))(*[int(r) for r in rows])
+        check(lib().cg_ocean_plan_set_tile_rows(self._p, arr), self.acc.ctx, "cg_ocean_plan_set_tile_rows")
+
     def set_start(self, start) -> None:
         """The values a warm solve starts from."""
         check(lib().cg_ocean_plan_set_start(self._p, np.ascontiguousarray(start, dtype=np.float32)),

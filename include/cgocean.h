@@ -210,6 +210,9 @@ int cg_ocean_plan_velocities(cg_plan *plan, double width_m, double height_m, flo
  * nlevels ints) receives the height per grid, 0 where a grid is not tiled. Call it between solves;
  * it leaves the plan without an answer to start a warm solve from. */
 int cg_ocean_plan_autotune(cg_plan *plan, int reps, int *rows_out);
+/* Sets the tile height of every grid of the plan (nlevels ints, each 0 for the context's rule, or 2,
+ * 4, 8, 16, 24 or 32), as the autotuner would: to repeat a tuned configuration. */
+int cg_ocean_plan_set_tile_rows(cg_plan *plan, const int *rows);
 
 /* ---- host-side problem builders (no device needed) --------------------------------------------- */
 
