@@ -211,9 +211,10 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
         wrow = L.roww[rr];
     }
     float A[R], B[R], fA[R], fB[R];
+    uint32_t mraw[R];
     uint64_t wA = 0, wB = 0;
-    // Branch-free loads (a row beyond a pole loads the edge row and is then zeroed), so that all
-    // of the tile's loads are in flight together.
+    // Branch-free loads (a row beyond a pole loads the edge row and is then zeroed), all issued
+    // before any is used, so that the tile's loads are in flight together.
 #pragma unroll
     for (int i = 0; i < R; ++i) {
         // A row beyond a pole reads row `down`, the all-zero row every array the smoother reads
@@ -223,11 +224,9 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
         const int rc = (r >= 0 && r < down) ? r : down;
         const unsigned idx = (unsigned)rc * (unsigned)across + (unsigned)ca;  // cells < 2^32
         const float2 fv = *reinterpret_cast<const float2 *>(f + idx);
-        const uint32_t m = *reinterpret_cast<const uint16_t *>(L.water + idx);
+        mraw[i] = *reinterpret_cast<const uint16_t *>(L.water + idx);
         fA[i] = fv.x;
         fB[i] = fv.y;
-        wA |= (uint64_t)(m & 1u) << i;
-        wB |= (uint64_t)((m >> 8) & 1u) << i;
         if (ZERO_START) {
             A[i] = 0.0f;
             B[i] = 0.0f;
@@ -236,6 +235,12 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
             A[i] = xv.x;
             B[i] = xv.y;
         }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        wA |= (uint64_t)(mraw[i] & 1u) << i;
+        wB |= (uint64_t)((mraw[i] >> 8) & 1u) << i;
     }
 
     if constexpr (MODE == TILE_UP || MODE == TILE_UP_OUTER) {
