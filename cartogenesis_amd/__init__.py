@@ -20,7 +20,7 @@ CG_OK, CG_DECLINED, CG_CANCELLED = 0, 1, 2
 CG_ERR_ARG, CG_ERR_DEVICE, CG_ERR_NOMEM, CG_ERR_STATE = -1, -2, -3, -4
 CG_METHOD_REFERENCE, CG_METHOD_RESCUE, CG_METHOD_REFINED, CG_METHOD_FGMRES, CG_METHOD_KRYLOV = 0, 1, 2, 3, 4
 CG_OPT_MIN_CELLS, CG_OPT_TILE_ROWS, CG_OPT_USE_GRAPH = 1, 2, 3
-CG_OPT_FGMRES_RESTART, CG_OPT_STEPS_PER_CHECK, CG_OPT_TIME_KERNELS = 4, 5, 6
+CG_OPT_FGMRES_RESTART, CG_OPT_TIME_KERNELS = 4, 6
 CG_OPT_FUSED, CG_OPT_LEVEL_PASSES, CG_OPT_FUSE_OUTER = 7, 8, 9
 
 

@@ -81,7 +81,6 @@ struct cg_ctx {
     int tile_rows = 16;
     int use_graph = 1;
     int fgmres_m = 20;
-    int steps_per_check = 1;
     int time_kernels = 0;
     int fused = 1;
     int fuse_outer = 1;  // the finest up leg and the binary64 outer step in one launch
@@ -1522,11 +1521,10 @@ int cg_ocean_set_option(cg_ctx *ctx, int option, long value) {
             }
             ctx->fgmres_m = (int)value;
             return CG_OK;
-        case CG_OPT_STEPS_PER_CHECK: ctx->steps_per_check = value < 1 ? 1 : (int)value; return CG_OK;
-        case 6: ctx->time_kernels = value ? 1 : 0; return CG_OK;
-        case 7: ctx->fused = value ? 1 : 0; return CG_OK;
-        case 9: ctx->fuse_outer = value ? 1 : 0; return CG_OK;
-        case 8:
+        case CG_OPT_TIME_KERNELS: ctx->time_kernels = value ? 1 : 0; return CG_OK;
+        case CG_OPT_FUSED: ctx->fused = value ? 1 : 0; return CG_OK;
+        case CG_OPT_FUSE_OUTER: ctx->fuse_outer = value ? 1 : 0; return CG_OK;
+        case CG_OPT_LEVEL_PASSES:
             for (int l = 0; l < 8; ++l) {
                 const long v = (value >> (4 * l)) & 0xf;
                 if (v != 2 && v != 4 && v != 6) {
@@ -1547,11 +1545,10 @@ long cg_ocean_get_option(cg_ctx *ctx, int option) {
         case CG_OPT_TILE_ROWS: return ctx->tile_rows;
         case CG_OPT_USE_GRAPH: return ctx->use_graph;
         case CG_OPT_FGMRES_RESTART: return ctx->fgmres_m;
-        case CG_OPT_STEPS_PER_CHECK: return ctx->steps_per_check;
-        case 6: return ctx->time_kernels;
-        case 7: return ctx->fused;
-        case 9: return ctx->fuse_outer;
-        case 8: return ctx->level_passes;
+        case CG_OPT_TIME_KERNELS: return ctx->time_kernels;
+        case CG_OPT_FUSED: return ctx->fused;
+        case CG_OPT_FUSE_OUTER: return ctx->fuse_outer;
+        case CG_OPT_LEVEL_PASSES: return ctx->level_passes;
         default: return -1;
     }
 }

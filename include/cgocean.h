@@ -60,7 +60,6 @@ extern "C" {
 #define CG_OPT_TILE_ROWS 2       /* rows per wavefront tile of the fused smoother: 8, 16, 24 or 32 */
 #define CG_OPT_USE_GRAPH 3       /* 1: replay each refinement step as a captured hipGraph */
 #define CG_OPT_FGMRES_RESTART 4  /* restart length m of CG_METHOD_FGMRES (1..64), default 20 */
-#define CG_OPT_STEPS_PER_CHECK 5 /* reserved */
 #define CG_OPT_FUSED 7           /* 1 (default): fused cycle (one launch per leg per grid, small grids in
                                     one workgroup); 0: separate smoothing/residual/transfer kernels */
 #define CG_OPT_LEVEL_PASSES 8    /* passes per grid of the binary32 cycle, one hex digit per grid from the
