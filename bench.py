@@ -105,74 +105,101 @@ def batched_year(acc, prob, tol: float, reps: int, warmup: int, autotune: bool =
     }
 
 
-def batched_worlds(acc, tol: float):
-    """Four worlds of uneven size, twelve months each: one plan of twelve instances per world, the
-    worlds one after another; solves/s = 48 / wall time."""
-    from cartogenesis_amd.ocean import BatchPlan
+def uneven_batches(acc, specs, instances: int, tol: float):
+    """Batched plans of several worlds of uneven size, each plan holding `instances` monthly
+    forcings of its world, timed two ways: the plans one after another on one context, and each
+    plan on a context (a HIP stream) of its own, solved by its own host thread, so that the worlds'
+    latency-bound launches overlap on the GPU. Returns (sequential, concurrent) solves/s, per-plan
+    milliseconds of the sequential run, and the largest residual."""
+    import threading
+    from cartogenesis_amd.ocean import BatchPlan, HipOcean
     from cartogenesis_amd.problem import circulation_problem, monthly_forcings
 
-    shapes = [(512, 1, 0.20), (768, 2, 0.27), (1024, 3, 0.33), (1024, 4, 0.40)]
-    plans = []
-    log("bench: building the four-world year")
-    for rows, seed, land in shapes:
+    worlds = []
+    for rows, seed, land in specs:
         prob = circulation_problem(rows, seed=seed, land_fraction=land)
-        plan = BatchPlan(acc, prob.levels, 12)
-        for i, f in enumerate(monthly_forcings(prob)):
+        forcings = monthly_forcings(prob)
+        worlds.append((rows, prob, [forcings[i % len(forcings)] for i in range(instances)]))
+
+    def make(ctx_acc, prob, forcings):
+        plan = BatchPlan(ctx_acc, prob.levels, instances)
+        for i, f in enumerate(forcings):
             plan.set_forcing(i, f)
-        plans.append(plan)
+        plan.autotune(3)
+        plan.solve_all(tol)  # warm
+        return plan
+
+    total = instances * len(worlds)
+    plans = [make(acc, prob, forcings) for _, prob, forcings in worlds]
     try:
-        for plan in plans:
-            plan.solve_all(tol)
         t0 = time.perf_counter()
-        per = []
-        for plan in plans:
+        per, cycles, worst = {}, {}, 0.0
+        for k, ((rows, _, _), plan) in enumerate(zip(worlds, plans)):
             t1 = time.perf_counter()
             infos = plan.solve_all(tol)
-            per.append(round(1000.0 * (time.perf_counter() - t1), 3))
-        elapsed = time.perf_counter() - t0
+            per[f"{k}:{rows}x{2 * rows}"] = round(1000.0 * (time.perf_counter() - t1), 3)
+            worst = worst_of(worst, infos)
+            cycles[f"{k}:{rows}x{2 * rows}"] = [min(i.cycles for i in infos), max(i.cycles for i in infos)]
+        sequential = total / (time.perf_counter() - t0)
     finally:
         for plan in plans:
             plan.close()
-    value = 48 / elapsed
-    log(f"bench: four-world year {value:.1f} solves/s")
+    accs = [HipOcean.create_or_null(0).accelerator for _ in worlds]
+    plans = [make(a, prob, forcings) for a, (_, prob, forcings) in zip(accs, worlds)]
+    try:
+        results = [None] * len(plans)
+
+        def run(k):
+            results[k] = plans[k].solve_all(tol)
+
+        threads = [threading.Thread(target=run, args=(k,)) for k in range(len(plans))]
+        t0 = time.perf_counter()
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        concurrent = total / (time.perf_counter() - t0)
+        for infos in results:
+            worst = worst_of(worst, infos)
+    finally:
+        for plan in plans:
+            plan.close()
+        for a in accs:
+            a.close()
+    return sequential, concurrent, per, cycles, worst
+
+
+def worst_of(worst: float, infos) -> float:
+    """The largest relative residual so far; NaN if any instance's residual is NaN."""
+    for i in infos:
+        r = float(i.relative_residual)
+        if r != r or worst != worst:
+            return float("nan")
+        worst = max(worst, r)
+    return worst
+
+
+def batched_worlds(acc, tol: float):
+    """Four worlds of uneven size, twelve months each (one batched plan of twelve per world)."""
+    log("bench: four-world year")
+    specs = [(512, 1, 0.20), (768, 2, 0.27), (1024, 3, 0.33), (1024, 4, 0.40)]
+    seq, conc, per, cycles, worst = uneven_batches(acc, specs, 12, tol)
+    log(f"bench: four-world year {seq:.1f} solves/s one world after another, {conc:.1f} with the worlds concurrent")
     return {"workload": "12 months x worlds 512x1024, 768x1536, 1024x2048, 1024x2048 (seeds 1-4, land 20/27/33/40%)",
-            "solves_per_s": round(value, 3), "ms_per_world": per}
+            "solves_per_s": round(conc, 3), "solves_per_s_one_world_at_a_time": round(seq, 3),
+            "ms_per_world_one_at_a_time": per, "cycles_min_max": cycles, "max_relative_residual": worst}
 
 
 def batched_combined(acc, tol: float):
     """64 instances of uneven size: 16 each at 256x512, 512x1024, 1024x2048 and 2048x4096 (months
-    0..11 then 0..3), one plan of 16 per size, the sizes one after another; solves/s = 64 / wall
-    time. Each instance's residual is the device's binary64 one."""
-    from cartogenesis_amd.ocean import BatchPlan
-    from cartogenesis_amd.problem import circulation_problem, monthly_forcings
-
-    plans = []
-    log("bench: building the combined 64-instance workload")
-    for rows, seed in ((256, 5), (512, 6), (1024, 7), (2048, 8)):
-        prob = circulation_problem(rows, seed=seed)
-        forcings = monthly_forcings(prob)
-        plan = BatchPlan(acc, prob.levels, 16)
-        for i in range(16):
-            plan.set_forcing(i, forcings[i % 12])
-        plans.append((rows, plan))
-    try:
-        for _, plan in plans:
-            plan.solve_all(tol)
-        t0 = time.perf_counter()
-        per, worst = {}, 0.0
-        for rows, plan in plans:
-            t1 = time.perf_counter()
-            infos = plan.solve_all(tol)
-            per[f"{rows}x{2 * rows}"] = round(1000.0 * (time.perf_counter() - t1), 3)
-            worst = max(worst, max(float(i.relative_residual) for i in infos))
-        elapsed = time.perf_counter() - t0
-    finally:
-        for _, plan in plans:
-            plan.close()
-    value = 64 / elapsed
-    log(f"bench: combined 64 instances {value:.1f} solves/s")
+    0..11 then 0..3), one batched plan of 16 per size."""
+    log("bench: combined 64 instances")
+    specs = [(256, 5, 0.3), (512, 6, 0.3), (1024, 7, 0.3), (2048, 8, 0.3)]
+    seq, conc, per, cycles, worst = uneven_batches(acc, specs, 16, tol)
+    log(f"bench: combined 64 instances {seq:.1f} solves/s one size after another, {conc:.1f} concurrent")
     return {"workload": "64 instances: 16 monthly forcings each at 256x512, 512x1024, 1024x2048, 2048x4096",
-            "solves_per_s": round(value, 3), "ms_per_size": per, "max_relative_residual": worst}
+            "solves_per_s": round(conc, 3), "solves_per_s_one_size_at_a_time": round(seq, 3),
+            "ms_per_size_one_at_a_time": per, "cycles_min_max": cycles, "max_relative_residual": worst}
 
 
 def main() -> int:

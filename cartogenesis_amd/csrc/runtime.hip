@@ -10,6 +10,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -41,6 +42,10 @@ struct Stencil {
     double *cellc = nullptr;
     float *fbuf = nullptr;  // the seam's forcing
     float *xbuf[2] = {nullptr, nullptr};
+    // Plans built on this stencil; a stencil released while plans still use it is freed when the
+    // last of them is destroyed, so a release never pulls memory out from under a live plan.
+    int plan_refs = 0;
+    bool released = false;
     // Host copies, for the coarsest grid's direct solve.
     std::vector<uint8_t> h_water;
     std::vector<float4> h_w;   // per row or per cell
@@ -90,6 +95,7 @@ struct cg_ctx {
     size_t lds_limit = 0;       // dynamic LDS a workgroup may use (k_bottom)
     bool bottom_attr_set = false;
     std::map<uint64_t, Stencil *> stencils;
+    std::set<Stencil *> orphans;  // released stencils that plans still use
     uint64_t next_handle = 1;
     unsigned long long *h_pinned = nullptr;  // host-visible scalars
     std::vector<TimedLaunch> timed;
@@ -371,6 +377,12 @@ namespace {
 void free_plan(cg_plan *p) {
     if (!p) return;
     for (int i = 0; i < p->n; ++i) {
+        Stencil *st = p->lv[i].st;
+        if (st && --st->plan_refs == 0 && st->released) {
+            p->ctx->orphans.erase(st);
+            free_stencil(st);
+        }
+        p->lv[i].st = nullptr;
         dfree(p->lv[i].f);
         dfree(p->lv[i].x[0]);
         dfree(p->lv[i].x[1]);
@@ -1483,6 +1495,8 @@ void cg_ocean_close(cg_ctx *ctx) {
         (void)hipStreamSynchronize(ctx->stream);
         for (auto &kv : ctx->stencils) free_stencil(kv.second);
         ctx->stencils.clear();
+        for (Stencil *st : ctx->orphans) free_stencil(st);
+        ctx->orphans.clear();
         for (auto &t : ctx->timed) {
             (void)hipEventDestroy(t.a);
             (void)hipEventDestroy(t.b);
@@ -1642,8 +1656,14 @@ int cg_ocean_stencil_release(cg_ctx *ctx, cg_stencil h) {
     }
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    free_stencil(it->second);
+    Stencil *st = it->second;
     ctx->stencils.erase(it);
+    if (st->plan_refs > 0) {
+        st->released = true;
+        ctx->orphans.insert(st);
+    } else {
+        free_stencil(st);
+    }
     return CG_OK;
 }
 
@@ -1747,6 +1767,7 @@ int cg_ocean_plan_create_batch(cg_ctx *ctx, const cg_stencil *levels, int nlevel
             }
         }
         p->lv[l].st = s;
+        ++s->plan_refs;
         // K copies of the grid, each with its zero row, all zero.
         if ((rc = dalloc_grid(ctx, &p->lv[l].f, s->across, (s->down + 1) * K - 1))) return bail(rc);
         if ((rc = dalloc_grid(ctx, &p->lv[l].x[0], s->across, (s->down + 1) * K - 1))) return bail(rc);

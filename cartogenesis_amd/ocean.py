@@ -272,7 +272,8 @@ class HipOcean(OceanAccelerator):
 
     Stencils are made resident on first sight and kept in an LRU of MOST_RESIDENT_STENCILS keyed on
     the identity of their east-weight array, as GpuOcean keys its residency (GpuOcean.kt:104-137);
-    a stencil rebuilt by with_balance shares that array and so hits the cache."""
+    a stencil rebuilt by with_balance shares that array and so hits the cache. Evicting a stencil
+    that a live Plan was built on is safe: the library keeps its memory until that plan is closed."""
 
     MOST_RESIDENT_STENCILS = 24
     SMALLEST_GRID_WORTH_THE_CARD_CELLS = 1 << 16

@@ -85,6 +85,7 @@ typedef struct {
 /* Probes HIP device `device` and prepares a context: GpuOcean.createOrNull (GO:189-202). On
  * failure returns a negative status, *out = NULL and writes the reason into reason[reason_len]. */
 int cg_ocean_open(int device, cg_ctx **out, char *reason, size_t reason_len);
+/* Frees the context and every stencil it still holds; destroy its plans first. */
 void cg_ocean_close(cg_ctx *ctx);
 /* OceanAccelerator.name (OA:18): the device's marketing name. */
 int cg_ocean_name(cg_ctx *ctx, char *buf, size_t len);
@@ -101,6 +102,9 @@ long cg_ocean_get_option(cg_ctx *ctx, int option);
 int cg_ocean_stencil_upload(cg_ctx *ctx, int across, int down, const uint8_t *water,
                             const float *east, const float *west, const float *north,
                             const float *south, const double *centre, cg_stencil *out);
+/* The handle is invalid once released. The stencil's device memory goes at once unless plans
+ * built on it are alive (an LRU like GpuOcean's, GO:109-115, may evict a plan's level); then
+ * it goes when the last of those plans is destroyed. */
 int cg_ocean_stencil_release(cg_ctx *ctx, cg_stencil stencil);
 
 /* OceanAccelerator.solve (OA:37-41) as GpuOcean.solve runs it (GO:45-102): `passes` red-black
