@@ -308,12 +308,15 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
     // which half-sweep h leaves untouched, so row i-1 being one half-sweep ahead is harmless; and
     // row i+1 has finished half-sweep h-1 earlier in the same step. The result equals NH full
     // half-sweeps in order, and each row's weights are live for only NH consecutive steps.
+    // Half-sweep h updates rows 1+h .. R-2-h only (a trapezoid): those read rows h .. R-1-h, which
+    // are exactly the rows half-sweep h-1 updated, and the last half-sweep still covers every row
+    // the tile writes or reads afterwards (the interior, plus its ring where HR = NH + 1).
 #pragma unroll
-    for (int t = 1; t < R - 2 + NH; ++t) {
+    for (int t = 1; t < R - 1; ++t) {
 #pragma unroll
         for (int h = 0; h < NH; ++h) {
             const int i = t - h;
-            if (i < 1 || i > R - 2) continue;
+            if (i < 1 + h || i > R - 2 - h) continue;
             const int colour = h & 1;
             const float4 wt = make_float4(lane_value(wrow.x, i), lane_value(wrow.y, i), lane_value(wrow.z, i),
                                           lane_value(wrow.w, i));
