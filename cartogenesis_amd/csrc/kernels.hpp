@@ -213,6 +213,30 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
     float A[R], B[R], fA[R], fB[R];
     uint32_t mraw[R];
     uint64_t wA = 0, wB = 0;
+    // What the modes read besides the tile's rows, loaded together with them so that a wave waits
+    // on memory once: TILE_UP's coarse values, TILE_UP_OUTER's binary64 answer and forcing, and
+    // TILE_DOWN's coarse mask.
+    //  - Coarse values: fine row r = row0 - HR + i reads coarse rows above = floor((r - 1) / 2) and
+    //    above + 1, i.e. row0 / 2 + q with q = floor((i - HR - 1) / 2), a compile-time offset since
+    //    row0 is even; lane l's pair (2m, 2m+1) reads coarse columns m-1, m, m+1 (lanes l-1, l, l+1).
+    //  - The binary64 answer of rows HR-1 .. HR+TH (this lane's pair), zero beyond a pole, and the
+    //    binary64 problem's forcing of rows HR .. HR+TH-1.
+    //  - The coarse mask of each row pair's coarse cell. Writer lanes' columns cabs equal ca.
+    constexpr bool kUp = MODE == TILE_UP || MODE == TILE_UP_OUTER;
+    constexpr int QMIN = -((HR + 2) / 2);  // floor((0 - HR - 1) / 2)
+    constexpr int QMAX = (R - HR - 2) >= 0 ? (R - HR - 2) / 2 + 1 : 0;
+    constexpr int NQ = kUp ? QMAX - QMIN + 1 : 1;
+    constexpr int NX = MODE == TILE_UP_OUTER ? TH + 2 : 1;
+    constexpr int NF = MODE == TILE_UP_OUTER ? TH : 1;
+    constexpr int NW = MODE == TILE_DOWN ? TH / 2 : 1;
+    float cv[NQ];
+    // TILE_UP_OUTER's binary64 answer goes straight from memory to LDS (global_load_lds, 16 bytes a
+    // lane: lane l's pair lands in slot l of its wave's row), so its rows hold no registers while
+    // the sweeps run. Four waves a workgroup.
+    constexpr bool kOuter = MODE == TILE_UP_OUTER;
+    __shared__ double2 x64s[kOuter ? 4 : 1][kOuter ? NX : 1][kOuter ? 64 : 1];
+    float2 f0v[NF];
+    uint32_t cwv[NW];
     // Branch-free loads (a row beyond a pole loads the edge row and is then zeroed), all issued
     // before any is used, so that the tile's loads are in flight together.
 #pragma unroll
@@ -236,6 +260,42 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
             B[i] = xv.y;
         }
     }
+    if constexpr (kUp) {
+        const int m = ca >> 1;
+        const int kc0 = row0 >> 1;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int kr = kc0 + QMIN + q;
+            const int kc = kr < 0 ? 0 : (kr >= ex.cd ? ex.cd - 1 : kr);
+            cv[q] = ex.xc[(unsigned)kc * (unsigned)ex.ca + (unsigned)m];
+        }
+    }
+    if constexpr (MODE == TILE_UP_OUTER) {
+#pragma unroll
+        for (int j = 0; j < NX; ++j) {
+            const int r = row0 - 1 + j;
+            const int rc = r < 0 ? 0 : (r >= down ? down - 1 : r);
+            const double *src = ex.x64 + ((unsigned)rc * (unsigned)across + (unsigned)ca);
+            __builtin_amdgcn_global_load_lds(
+                (const void *)src,
+                (__attribute__((address_space(3))) void *)&x64s[threadIdx.x >> 6][j][0], 16, 0, 0);
+        }
+#pragma unroll
+        for (int k = 0; k < TH; ++k) {
+            const int r = row0 + k;
+            const int rc = r < down ? r : down;
+            f0v[k] = *reinterpret_cast<const float2 *>(ex.f0 + ((unsigned)rc * (unsigned)across + (unsigned)ca));
+        }
+    }
+    if constexpr (MODE == TILE_DOWN) {
+        const unsigned cacross = (unsigned)across >> 1;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+            const int r = row0 + 2 * k;
+            const unsigned kc = r < down ? (unsigned)(r >> 1) : 0u;
+            cwv[k] = ex.cwater[kc * cacross + ((unsigned)ca >> 1)];
+        }
+    }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int i = 0; i < R; ++i) {
@@ -243,14 +303,7 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
         wB |= (uint64_t)((mraw[i] >> 8) & 1u) << i;
     }
 
-    if constexpr (MODE == TILE_UP || MODE == TILE_UP_OUTER) {
-        // Fine row r = row0 - HR + i reads coarse rows above = floor((r - 1) / 2) and above + 1,
-        // i.e. row0 / 2 + q with q = floor((i - HR - 1) / 2): a compile-time offset, since row0 is
-        // even. Lane l's pair (2m, 2m+1) reads coarse columns m-1, m, m+1 (from lanes l-1, l, l+1).
-        constexpr int QMIN = -((HR + 2) / 2);  // floor((0 - HR - 1) / 2)
-        constexpr int QMAX = (R - HR - 2) >= 0 ? (R - HR - 2) / 2 + 1 : 0;
-        constexpr int NQ = QMAX - QMIN + 1;
-        const int m = ca >> 1;
+    if constexpr (kUp) {
         const int kc0 = row0 >> 1;
         float cm[NQ], cl[NQ], cr[NQ];
 #pragma unroll
@@ -258,9 +311,8 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
             // prolongAdd's `at` (OC:380-385): beyond a pole, the edge row times `beyond`. In range
             // the factor is 1, and 1 * v is v exactly.
             const int kr = kc0 + QMIN + q;
-            const int kc = kr < 0 ? 0 : (kr >= ex.cd ? ex.cd - 1 : kr);
             const float fac = (kr < 0 || kr >= ex.cd) ? ex.beyond : 1.0f;
-            const float v = fac * ex.xc[(long)kc * ex.ca + m];
+            const float v = fac * cv[q];
             cm[q] = v;
             cl[q] = lane_from_below(v);
             cr[q] = lane_from_above(v);
@@ -268,8 +320,6 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
 #pragma unroll
         for (int i = 0; i < R; ++i) {
             // q index of `above` for this row, relative to QMIN.
-            constexpr int dummy = 0;
-            (void)dummy;
             const int qa = ((i - HR - 1) >= 0 ? (i - HR - 1) / 2 : -((HR + 1 - i + 1) / 2)) - QMIN;
             const bool r_even = ((i + HR) & 1) == 0;
             const float db = r_even ? 0.75f : 0.25f;
@@ -283,23 +333,6 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
             // Branch-free: on land the addend is +0, and a land cell holds +0, so it stays +0.
             A[i] += keep_if(topA * (1.0f - db) + botA * db, mA, i & 31);
             B[i] += keep_if(topB * (1.0f - db) + botB * db, mB, i & 31);
-        }
-    }
-
-    // TILE_UP_OUTER: the binary64 answer of rows HR-1 .. HR+TH (this lane's pair), loaded before the
-    // sweeps so that its latency overlaps them; zero beyond a pole.
-    constexpr int NX = MODE == TILE_UP_OUTER ? TH + 2 : 1;
-    double px[NX], py[NX];
-    if constexpr (MODE == TILE_UP_OUTER) {
-#pragma unroll
-        for (int j = 0; j < NX; ++j) {
-            const int r = row0 - 1 + j;
-            px[j] = py[j] = 0.0;
-            if (r >= 0 && r < down) {
-                const double2 xv = *reinterpret_cast<const double2 *>(ex.x64 + ((unsigned)r * (unsigned)across + (unsigned)ca));
-                px[j] = xv.x;
-                py[j] = xv.y;
-            }
         }
     }
 
@@ -324,7 +357,14 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
             const uint32_t mB = i < 32 ? (uint32_t)wB : (uint32_t)(wB >> 32);
             // Row (row0 - HR + i) has the parity of (i + HR) since row0 is even; the even column
             // of the pair has colour (row & 1).
-            if (((i + HR) & 1) == colour) {
+            if (ZERO_START && h == 0) {
+                // From zero every neighbour is +0 and the weights are non-negative, so each product
+                // and partial sum is +0 and the update is +0 - f exactly.
+                if (((i + HR) & 1) == colour)
+                    A[i] = keep_if(0.0f - fA[i], mA, i & 31);
+                else
+                    B[i] = keep_if(0.0f - fB[i], mB, i & 31);
+            } else if (((i + HR) & 1) == colour) {
                 const float xw = lane_from_below(B[i]);
                 A[i] = keep_if(rb_update(wt, B[i], xw, A[i - 1], A[i + 1], fA[i]), mA, i & 31);
             } else {
@@ -359,8 +399,9 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
             const int r = row0 - 1 + j;
             vx[j] = vy[j] = 0.0;
             if (r >= 0 && r < down) {
-                vx[j] = px[j] + (double)A[HR - 1 + j];
-                vy[j] = py[j] + (double)B[HR - 1 + j];
+                const double2 xv = x64s[threadIdx.x >> 6][j][lane];
+                vx[j] = xv.x + (double)A[HR - 1 + j];
+                vy[j] = xv.y + (double)B[HR - 1 + j];
             }
         }
         unsigned long long mx = 0;
@@ -374,7 +415,7 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
                                           lane_value(wrow.w, i));
             if (!writer || row >= down) continue;
             const unsigned idx = (unsigned)row * (unsigned)across + (unsigned)cabs;
-            const float2 fv = *reinterpret_cast<const float2 *>(ex.f0 + idx);
+            const float2 fv = f0v[k];
             const double cen = L.rowc[row];
             const uint32_t mA = i < 32 ? (uint32_t)wA : (uint32_t)(wA >> 32);
             const uint32_t mB = i < 32 ? (uint32_t)wB : (uint32_t)(wB >> 32);
@@ -439,7 +480,7 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
                 acc += u[1] * g0;
                 acc += u[2] * g1;
                 acc += u[3] * g1;
-                ex.fc[cidx] = ex.cwater[cidx] ? acc : 0.0f;
+                ex.fc[cidx] = cwv[(i - HR) / 2] ? acc : 0.0f;
             }
         }
     }

@@ -245,6 +245,7 @@ hipError_t tile_dispatch(cg_ctx *ctx, const LevelArgs &L, const float *f, const 
     switch (rows ? rows : ctx->tile_rows) {
         case 2: return launch_tile<2>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex, instances);
         case 4: return launch_tile<4>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex, instances);
+        case 6: return launch_tile<6>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex, instances);
         case 8: return launch_tile<8>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex, instances);
         case 24: return launch_tile<24>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex, instances);
         case 32: return launch_tile<32>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex, instances);
@@ -1605,6 +1606,13 @@ int cg_ocean_stencil_upload(cg_ctx *ctx, int across, int down, const uint8_t *wa
                     break;
                 }
         }
+    // The tiled smoother also takes the weights to be finite and non-negative (an M-matrix, as the
+    // circulation's are): its first half-sweep from zero then reduces to +0 - f exactly.
+    for (int r = 0; r < down && rc_w; ++r) {
+        const long b = (long)r * across;
+        for (const float w : {east[b], west[b], north[b], south[b]})
+            if (!(w >= 0.0f && w <= 3.4028235e38f)) rc_w = false;
+    }
     s->rowconst = rc_w && rc_c;
     s->h_water.assign(water, water + s->cells);
     int rc = 0;
@@ -2287,9 +2295,9 @@ int cg_ocean_plan_set_tile_rows(cg_plan *p, const int *rows) {
     cg_ctx *ctx = p->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     for (int l = 0; l < p->n; ++l)
-        if (rows[l] != 0 && rows[l] != 2 && rows[l] != 4 && rows[l] != 8 && rows[l] != 16 && rows[l] != 24 &&
-            rows[l] != 32) {
-            set_err(ctx, "tile rows of grid %d must be 0, 2, 4, 8, 16, 24 or 32", l);
+        if (rows[l] != 0 && rows[l] != 2 && rows[l] != 4 && rows[l] != 6 && rows[l] != 8 && rows[l] != 16 &&
+            rows[l] != 24 && rows[l] != 32) {
+            set_err(ctx, "tile rows of grid %d must be 0, 2, 4, 6, 8, 16, 24 or 32", l);
             return CG_ERR_ARG;
         }
     for (int l = 0; l < p->n; ++l) p->tuned_rows[l] = rows[l];
@@ -2310,7 +2318,7 @@ int cg_ocean_plan_autotune(cg_plan *p, int reps, int *rows_out) {
         set_err(ctx, "only the fused cycle's tiles are tuned");
         return CG_ERR_STATE;
     }
-    static const int kCandidates[] = {2, 4, 8, 16, 24, 32};
+    static const int kCandidates[] = {2, 4, 6, 8, 16, 24, 32};
     const bool graph = ctx->use_graph && !ctx->time_kernels;
     auto drop_graphs = [&]() {
         for (int i = 0; i < 2; ++i)

@@ -97,8 +97,8 @@ long cg_ocean_get_option(cg_ctx *ctx, int option);
 
 /* Makes an OceanStencil (OC:24-42) resident on the device: mask, the four neighbour weights and
  * the centre weight. Replaces GpuOcean.holding's identity-keyed LRU (GO:104-137) with an explicit
- * handle. Weights that are equal along every row (the circulation's, OC:207-213) are stored once
- * per row; others once per cell. centre may be NULL when only relaxation is asked for. */
+ * handle. Weights that are equal along every row, finite and non-negative (the circulation's,
+ * OC:207-213) are stored once per row and relaxed by the tiled smoother; others once per cell. centre may be NULL when only relaxation is asked for. */
 int cg_ocean_stencil_upload(cg_ctx *ctx, int across, int down, const uint8_t *water,
                             const float *east, const float *west, const float *north,
                             const float *south, const double *centre, cg_stencil *out);
@@ -208,13 +208,13 @@ int cg_ocean_plan_velocities(cg_plan *plan, double width_m, double height_m, flo
 
 /* The plan's tile autotuner: for every grid of the fused cycle above the bottom, finest first, times
  * `reps` whole refinement steps (graph replays, HIP events on the context's stream) with that grid's
- * tiles 2, 4, 8, 16, 24 and 32 rows high and the other grids as chosen so far, and keeps the fastest;
+ * tiles 2, 4, 6, 8, 16, 24 and 32 rows high and the other grids as chosen so far, and keeps the fastest;
  * the plan's later solves use the choice (results do not depend on the tiles). rows_out (NULL or
  * nlevels ints) receives the height per grid, 0 where a grid is not tiled. Call it between solves;
  * it leaves the plan without an answer to start a warm solve from. */
 int cg_ocean_plan_autotune(cg_plan *plan, int reps, int *rows_out);
 /* Sets the tile height of every grid of the plan (nlevels ints, each 0 for the context's rule, or 2,
- * 4, 8, 16, 24 or 32), as the autotuner would: to repeat a tuned configuration. */
+ * 4, 6, 8, 16, 24 or 32), as the autotuner would: to repeat a tuned configuration. */
 int cg_ocean_plan_set_tile_rows(cg_plan *plan, const int *rows);
 
 /* ---- host-side problem builders (no device needed) --------------------------------------------- */
