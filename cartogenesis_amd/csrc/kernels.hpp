@@ -49,6 +49,18 @@ __device__ __forceinline__ float keep_if(float v, uint32_t m, int bit) {
     return __builtin_bit_cast(float, __builtin_bit_cast(int, v) & sel);
 }
 
+// The object `byte` bytes past a uniform base, the offset a 32-bit value (every grid here is far
+// below 4 GiB), which the global_load/store "saddr" form takes directly, rather than a 64-bit
+// address formed per lane.
+template <typename T>
+__device__ __forceinline__ const T &at_byte(const void *base, unsigned byte) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte);
+}
+template <typename T>
+__device__ __forceinline__ T &at_byte(void *base, unsigned byte) {
+    return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + byte);
+}
+
 // Largest magnitudes are reduced as the uint64 patterns of |v|: for non-negative doubles the
 // order of the patterns is the order of the values, and a NaN's pattern exceeds all of them.
 __device__ __forceinline__ unsigned long long abs_bits(double v) {
@@ -229,37 +241,29 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
     constexpr int NX = MODE == TILE_UP_OUTER ? TH + 2 : 1;
     constexpr int NF = MODE == TILE_UP_OUTER ? TH : 1;
     constexpr int NW = MODE == TILE_DOWN ? TH / 2 : 1;
+    constexpr int kLateLoadStep = R / 2;
     float cv[NQ];
     // TILE_UP_OUTER's binary64 answer goes straight from memory to LDS (global_load_lds, 16 bytes a
     // lane: lane l's pair lands in slot l of its wave's row), so its rows hold no registers while
     // the sweeps run. Four waves a workgroup.
     constexpr bool kOuter = MODE == TILE_UP_OUTER;
     __shared__ double2 x64s[kOuter ? 4 : 1][kOuter ? NX : 1][kOuter ? 64 : 1];
+    // ... and the binary64 centres of its rows: 32-bit words, lane l loading word l of the tile's
+    // first row's centre onwards (lanes past the grid's last row load word 0 and go unused).
+    __shared__ __align__(16) uint32_t cenw[kOuter ? 4 : 1][kOuter ? 64 : 1];
     float2 f0v[NF];
     uint32_t cwv[NW];
-    // Branch-free loads (a row beyond a pole loads the edge row and is then zeroed), all issued
-    // before any is used, so that the tile's loads are in flight together.
+    // TILE_DOWN's restriction factor per row, loaded up front too (rows past the last clamp to it).
+    constexpr int NG = MODE == TILE_DOWN ? TH : 1;
+    float gv[NG];
+    if constexpr (MODE == TILE_DOWN) {
 #pragma unroll
-    for (int i = 0; i < R; ++i) {
-        // A row beyond a pole reads row `down`, the all-zero row every array the smoother reads
-        // carries past its last row: zero values, zero forcing, no water. The device mask holds
-        // 0 or 1 per cell.
-        const int r = row0 - HR + i;
-        const int rc = (r >= 0 && r < down) ? r : down;
-        const unsigned idx = (unsigned)rc * (unsigned)across + (unsigned)ca;  // cells < 2^32
-        const float2 fv = *reinterpret_cast<const float2 *>(f + idx);
-        mraw[i] = *reinterpret_cast<const uint16_t *>(L.water + idx);
-        fA[i] = fv.x;
-        fB[i] = fv.y;
-        if (ZERO_START) {
-            A[i] = 0.0f;
-            B[i] = 0.0f;
-        } else {
-            const float2 xv = *reinterpret_cast<const float2 *>(xin + idx);
-            A[i] = xv.x;
-            B[i] = xv.y;
-        }
+        for (int k = 0; k < NG; ++k) gv[k] = ex.g[row0 + k < down ? row0 + k : down - 1];
     }
+    // Branch-free loads (a row beyond a pole loads the edge row and is then zeroed), all issued
+    // before any is used, so that the tile's loads are in flight together: the coarse values first,
+    // then the rows in the order the sweeps reach them, then what only the end of the tile needs,
+    // so that the wave waits for each row only when the sweeps first touch it.
     if constexpr (kUp) {
         const int m = ca >> 1;
         const int kc0 = row0 >> 1;
@@ -267,25 +271,34 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
         for (int q = 0; q < NQ; ++q) {
             const int kr = kc0 + QMIN + q;
             const int kc = kr < 0 ? 0 : (kr >= ex.cd ? ex.cd - 1 : kr);
-            cv[q] = ex.xc[(unsigned)kc * (unsigned)ex.ca + (unsigned)m];
+            cv[q] = at_byte<float>(ex.xc, 4u * ((unsigned)kc * (unsigned)ex.ca + (unsigned)m));
         }
     }
-    if constexpr (MODE == TILE_UP_OUTER) {
 #pragma unroll
-        for (int j = 0; j < NX; ++j) {
-            const int r = row0 - 1 + j;
-            const int rc = r < 0 ? 0 : (r >= down ? down - 1 : r);
-            const double *src = ex.x64 + ((unsigned)rc * (unsigned)across + (unsigned)ca);
-            __builtin_amdgcn_global_load_lds(
-                (const void *)src,
-                (__attribute__((address_space(3))) void *)&x64s[threadIdx.x >> 6][j][0], 16, 0, 0);
+    for (int i = 0; i < R; ++i) {
+        // A row beyond a pole reads row `down`, the all-zero row every array the smoother reads
+        // carries past its last row: zero values, zero forcing, no water. The device mask holds
+        // 0 or 1 per cell.
+        const int r = row0 - HR + i;
+        const int rc = (r >= 0 && r < down) ? r : down;
+        // Byte offsets: a uniform row offset (scalar) plus the lane's column offset.
+        const unsigned idx = (unsigned)rc * (unsigned)across + (unsigned)ca;  // cells < 2^30
+        const unsigned off4 = (unsigned)rc * (4u * (unsigned)across) + 4u * (unsigned)ca;
+        const float2 fv = at_byte<float2>(f, off4);
+        mraw[i] = at_byte<uint16_t>(L.water, idx);
+        fA[i] = fv.x;
+        fB[i] = fv.y;
+        if (ZERO_START) {
+            A[i] = 0.0f;
+            B[i] = 0.0f;
+        } else {
+            const float2 xv = at_byte<float2>(xin, off4);
+            A[i] = xv.x;
+            B[i] = xv.y;
         }
-#pragma unroll
-        for (int k = 0; k < TH; ++k) {
-            const int r = row0 + k;
-            const int rc = r < down ? r : down;
-            f0v[k] = *reinterpret_cast<const float2 *>(ex.f0 + ((unsigned)rc * (unsigned)across + (unsigned)ca));
-        }
+        // Issue order is row order (the scheduler would otherwise sink the loads of the rows
+        // that need only one forcing component, the first the sweeps touch, to the end).
+        __builtin_amdgcn_sched_barrier(0);
     }
     if constexpr (MODE == TILE_DOWN) {
         const unsigned cacross = (unsigned)across >> 1;
@@ -293,46 +306,23 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
         for (int k = 0; k < NW; ++k) {
             const int r = row0 + 2 * k;
             const unsigned kc = r < down ? (unsigned)(r >> 1) : 0u;
-            cwv[k] = ex.cwater[kc * cacross + ((unsigned)ca >> 1)];
+            cwv[k] = at_byte<uint8_t>(ex.cwater, kc * cacross + ((unsigned)ca >> 1));
         }
     }
     __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int i = 0; i < R; ++i) {
-        wA |= (uint64_t)(mraw[i] & 1u) << i;
-        wB |= (uint64_t)((mraw[i] >> 8) & 1u) << i;
-    }
-
+    // TILE_UP's coarse values and their neighbours' (prolongAdd's `at`, OC:380-385: beyond a pole,
+    // the edge row times `beyond`; in range the factor is 1, and 1 * v is v exactly).
+    float cm[NQ], cl[NQ], cr[NQ];
     if constexpr (kUp) {
         const int kc0 = row0 >> 1;
-        float cm[NQ], cl[NQ], cr[NQ];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            // prolongAdd's `at` (OC:380-385): beyond a pole, the edge row times `beyond`. In range
-            // the factor is 1, and 1 * v is v exactly.
             const int kr = kc0 + QMIN + q;
             const float fac = (kr < 0 || kr >= ex.cd) ? ex.beyond : 1.0f;
             const float v = fac * cv[q];
             cm[q] = v;
             cl[q] = lane_from_below(v);
             cr[q] = lane_from_above(v);
-        }
-#pragma unroll
-        for (int i = 0; i < R; ++i) {
-            // q index of `above` for this row, relative to QMIN.
-            const int qa = ((i - HR - 1) >= 0 ? (i - HR - 1) / 2 : -((HR + 1 - i + 1) / 2)) - QMIN;
-            const bool r_even = ((i + HR) & 1) == 0;
-            const float db = r_even ? 0.75f : 0.25f;
-            // Column 2m: west m-1, east m, blend 0.75; column 2m+1: west m, east m+1, blend 0.25.
-            const float topA = cl[qa] * (1.0f - 0.75f) + cm[qa] * 0.75f;
-            const float botA = cl[qa + 1] * (1.0f - 0.75f) + cm[qa + 1] * 0.75f;
-            const float topB = cm[qa] * (1.0f - 0.25f) + cr[qa] * 0.25f;
-            const float botB = cm[qa + 1] * (1.0f - 0.25f) + cr[qa + 1] * 0.25f;
-            const uint32_t mA = i < 32 ? (uint32_t)wA : (uint32_t)(wA >> 32);
-            const uint32_t mB = i < 32 ? (uint32_t)wB : (uint32_t)(wB >> 32);
-            // Branch-free: on land the addend is +0, and a land cell holds +0, so it stays +0.
-            A[i] += keep_if(topA * (1.0f - db) + botA * db, mA, i & 31);
-            B[i] += keep_if(topB * (1.0f - db) + botB * db, mB, i & 31);
         }
     }
 
@@ -344,12 +334,63 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
     // Half-sweep h updates rows 1+h .. R-2-h only (a trapezoid): those read rows h .. R-1-h, which
     // are exactly the rows half-sweep h-1 updated, and the last half-sweep still covers every row
     // the tile writes or reads afterwards (the interior, plus its ring where HR = NH + 1).
+    // Row t + 1 is prepared at the start of step t, just before half-sweep 0 first reads it: its
+    // mask bits, and in TILE_UP its bilinear correction (column 2m: west m-1, east m, blend 0.75;
+    // column 2m+1: west m, east m+1, blend 0.25; branch-free: on land the addend is +0, and a land
+    // cell holds +0, so it stays +0). Steps -1 and 0 only prepare rows 0 and 1.
 #pragma unroll
-    for (int t = 1; t < R - 1; ++t) {
+    for (int t = -1; t < R - 1; ++t) {
+        {
+            const int i = t + 1;
+            wA |= (uint64_t)(mraw[i] & 1u) << i;
+            wB |= (uint64_t)((mraw[i] >> 8) & 1u) << i;
+            if constexpr (kUp) {
+                // q index of `above` for this row, relative to QMIN.
+                const int qa = ((i - HR - 1) >= 0 ? (i - HR - 1) / 2 : -((HR + 1 - i + 1) / 2)) - QMIN;
+                const bool r_even = ((i + HR) & 1) == 0;
+                const float db = r_even ? 0.75f : 0.25f;
+                const float topA = cl[qa] * (1.0f - 0.75f) + cm[qa] * 0.75f;
+                const float botA = cl[qa + 1] * (1.0f - 0.75f) + cm[qa + 1] * 0.75f;
+                const float topB = cm[qa] * (1.0f - 0.25f) + cr[qa] * 0.25f;
+                const float botB = cm[qa + 1] * (1.0f - 0.25f) + cr[qa + 1] * 0.25f;
+                const uint32_t mA = i < 32 ? (uint32_t)wA : (uint32_t)(wA >> 32);
+                const uint32_t mB = i < 32 ? (uint32_t)wB : (uint32_t)(wB >> 32);
+                A[i] += keep_if(topA * (1.0f - db) + botA * db, mA, i & 31);
+                B[i] += keep_if(topB * (1.0f - db) + botB * db, mB, i & 31);
+            }
+        }
+        // TILE_UP_OUTER's binary64 answer and forcing are only read after the sweeps: issued halfway
+        // through them, they arrive while the rest run, and the tile's first rows do not queue
+        // behind them.
+        if constexpr (MODE == TILE_UP_OUTER) {
+            if (t == kLateLoadStep) {
+#pragma unroll
+                for (int j = 0; j < NX; ++j) {
+                    const int r = row0 - 1 + j;
+                    const int rc = r < 0 ? 0 : (r >= down ? down - 1 : r);
+                    const void *src = &at_byte<double>(ex.x64, (unsigned)rc * (8u * (unsigned)across) + 8u * (unsigned)ca);
+                    __builtin_amdgcn_global_load_lds(
+                        (const void *)src,
+                        (__attribute__((address_space(3))) void *)&x64s[threadIdx.x >> 6][j][0], 16, 0, 0);
+                }
+#pragma unroll
+                for (int k = 0; k < TH; ++k) {
+                    const int r = row0 + k;
+                    const int rc = r < down ? r : down;
+                    f0v[k] = at_byte<float2>(ex.f0, (unsigned)rc * (4u * (unsigned)across) + 4u * (unsigned)ca);
+                }
+                {
+                    const unsigned word = 2u * (unsigned)row0 + (unsigned)lane;
+                    const void *src = &at_byte<uint32_t>(L.rowc, word < 2u * (unsigned)down ? 4u * word : 0u);
+                    __builtin_amdgcn_global_load_lds(
+                        src, (__attribute__((address_space(3))) void *)&cenw[threadIdx.x >> 6][0], 4, 0, 0);
+                }
+            }
+        }
 #pragma unroll
         for (int h = 0; h < NH; ++h) {
             const int i = t - h;
-            if (i < 1 + h || i > R - 2 - h) continue;
+            if (t < 1 || i < 1 + h || i > R - 2 - h) continue;
             const int colour = h & 1;
             const float4 wt = make_float4(lane_value(wrow.x, i), lane_value(wrow.y, i), lane_value(wrow.z, i),
                                           lane_value(wrow.w, i));
@@ -384,7 +425,7 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
 #pragma unroll
         for (int i = HR; i < HR + TH; ++i) {
             const int r = row0 - HR + i;
-            if (r < down) *reinterpret_cast<float2 *>(xout + ((unsigned)r * (unsigned)across + (unsigned)cabs)) = make_float2(A[i], B[i]);
+            if (r < down) at_byte<float2>(xout, (unsigned)r * (4u * (unsigned)across) + 4u * (unsigned)cabs) = make_float2(A[i], B[i]);
         }
     }
 
@@ -414,13 +455,14 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
             const float4 wt = make_float4(lane_value(wrow.x, i), lane_value(wrow.y, i), lane_value(wrow.z, i),
                                           lane_value(wrow.w, i));
             if (!writer || row >= down) continue;
-            const unsigned idx = (unsigned)row * (unsigned)across + (unsigned)cabs;
+            const unsigned off4 = (unsigned)row * (4u * (unsigned)across) + 4u * (unsigned)cabs;
             const float2 fv = f0v[k];
-            const double cen = L.rowc[row];
+            const double cen = *reinterpret_cast<const double *>(&cenw[threadIdx.x >> 6][2 * k]);
             const uint32_t mA = i < 32 ? (uint32_t)wA : (uint32_t)(wA >> 32);
             const uint32_t mB = i < 32 ? (uint32_t)wB : (uint32_t)(wB >> 32);
-            const double vn_x = row > 0 ? vx[k] : 0.0, vn_y = row > 0 ? vy[k] : 0.0;
-            const double vs_x = row + 1 < down ? vx[k + 2] : 0.0, vs_y = row + 1 < down ? vy[k + 2] : 0.0;
+            // The rows beyond a pole already hold zero (vx, vy above).
+            const double vn_x = vx[k], vn_y = vy[k];
+            const double vs_x = vx[k + 2], vs_y = vy[k + 2];
             double ux = 0.0, uy = 0.0, ox = 0.0, oy = 0.0;
             if ((mA >> (i & 31)) & 1) {
                 const double bal = (double)wt.x * vy[k + 1] + (double)wt.y * west + (double)wt.z * vn_x +
@@ -438,8 +480,8 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
                 const unsigned long long b = abs_bits(uy * cen);
                 mx = b > mx ? b : mx;
             }
-            *reinterpret_cast<double2 *>(ex.x64n + idx) = make_double2(ox, oy);
-            *reinterpret_cast<float2 *>(ex.fcn + idx) = make_float2((float)ux, (float)uy);
+            at_byte<double2>(ex.x64n, 2u * off4) = make_double2(ox, oy);
+            at_byte<float2>(ex.fcn, off4) = make_float2((float)ux, (float)uy);
         }
         // One relaxed atomicMax per wave into slot wave % kMaxSlots; the waves together zero the
         // slots the next step reduces into.
@@ -473,14 +515,14 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
                 u[2 * k + 1] = keep_if(ub, mB, ii & 31);
             }
             if (writer && in) {
-                const long cidx = (long)(r >> 1) * cacross + (cabs >> 1);
-                const float g0 = ex.g[r], g1 = ex.g[r + 1];
+                const unsigned cidx = (unsigned)(r >> 1) * (unsigned)cacross + ((unsigned)cabs >> 1);
+                const float g0 = gv[i - HR], g1 = gv[i - HR + 1];
                 float acc = 0.0f;
                 acc += u[0] * g0;
                 acc += u[1] * g0;
                 acc += u[2] * g1;
                 acc += u[3] * g1;
-                ex.fc[cidx] = cwv[(i - HR) / 2] ? acc : 0.0f;
+                at_byte<float>(ex.fc, 4u * cidx) = cwv[(i - HR) / 2] ? acc : 0.0f;
             }
         }
     }
@@ -966,25 +1008,44 @@ struct BottomView {
     const float *g;
 };
 
-__device__ __forceinline__ void lds_relax_pass(const BottomView &L) {
+// `passes` red-black passes of a bottom grid in LDS. Thread (tx, ty) owns the row-ty cells of
+// columns 2 tx and 2 tx + 1, one of each colour; the LDS indices of both cells and their neighbours
+// are formed once for all the passes, and every read of a half-sweep is issued before its one
+// write. Branch-free: rows beyond a pole and land cells are selected to +0, as the reference's
+// zero beyond the pole and its land cells, which never change, hold +0.
+__device__ __forceinline__ void lds_relax_passes(const BottomView &L, int passes) {
     const int tx = threadIdx.x & 31, row = threadIdx.x >> 5;
     const int across = L.across;
     const bool mine = row < L.down && tx < (across >> 1);
+    const bool has_n = row > 0, has_s = row + 1 < L.down;
+    int cell[2], ie[2], iw[2], in[2], is[2];
+    bool wet[2] = {false, false};
+#pragma unroll
     for (int colour = 0; colour <= 1; ++colour) {
-        if (mine) {
-            const int col = 2 * tx + ((colour + row) & 1);
-            const int cell = row * across + col;
-            float v = 0.0f;
-            if (L.w[cell]) {
-                const int ce = col + 1 == across ? 0 : col + 1;
-                const int cw = col == 0 ? across - 1 : col - 1;
-                const float xn = row > 0 ? L.x[cell - across] : 0.0f;
-                const float xs = row + 1 < L.down ? L.x[cell + across] : 0.0f;
-                v = rb_update(L.rw[row], L.x[row * across + ce], L.x[row * across + cw], xn, xs, L.f[cell]);
-            }
-            L.x[cell] = v;
+        const int col = 2 * tx + ((colour + row) & 1);
+        const int c = mine ? row * across + col : 0;
+        const int ce = col + 1 == across ? 0 : col + 1;
+        const int cw = col == 0 ? across - 1 : col - 1;
+        cell[colour] = c;
+        ie[colour] = mine ? row * across + ce : 0;
+        iw[colour] = mine ? row * across + cw : 0;
+        in[colour] = mine && has_n ? c - across : c;
+        is[colour] = mine && has_s ? c + across : c;
+        wet[colour] = mine && L.w[c] != 0;
+    }
+    const float4 wt = L.rw[mine ? row : 0];
+    for (int p = 0; p < passes; ++p) {
+#pragma unroll
+        for (int colour = 0; colour <= 1; ++colour) {
+            const float xe = L.x[ie[colour]], xw = L.x[iw[colour]];
+            float xn = L.x[in[colour]], xs = L.x[is[colour]];
+            const float fv = L.f[cell[colour]];
+            xn = has_n ? xn : 0.0f;
+            xs = has_s ? xs : 0.0f;
+            const float v = rb_update(wt, xe, xw, xn, xs, fv);
+            if (mine) L.x[cell[colour]] = wet[colour] ? v : 0.0f;
+            __syncthreads();
         }
-        __syncthreads();
     }
 }
 
@@ -1029,8 +1090,7 @@ __global__ __launch_bounds__(1024) void k_bottom(BottomArgs a) {
     // Down: relax twice, then the binary32 residual restricted with g (k_residual_restrict32's sums).
     for (int l = 0; l + 1 < a.nlev; ++l) {
         const BottomView L = view(l), C = view(l + 1);
-        lds_relax_pass(L);
-        lds_relax_pass(L);
+        lds_relax_passes(L, 2);
         // Each grid halves both axes into the next (the plan checks it): 2 x 2 blocks.
         if (ty < C.down && tx < C.across) {
             const int t = ty * C.across + tx;
@@ -1083,7 +1143,7 @@ __global__ __launch_bounds__(1024) void k_bottom(BottomArgs a) {
                 __syncthreads();
             }
         } else {
-            for (int pass = 0; pass < a.coarsest_passes; ++pass) lds_relax_pass(C);
+            lds_relax_passes(C, a.coarsest_passes);
         }
         __syncthreads();
     }
@@ -1128,8 +1188,7 @@ __global__ __launch_bounds__(1024) void k_bottom(BottomArgs a) {
                 if (L.w[cell]) L.x[cell] = pv[k];
             }
         __syncthreads();
-        lds_relax_pass(L);
-        lds_relax_pass(L);
+        lds_relax_passes(L, 2);
     }
     {
         const int cells = a.lv[0].across * a.lv[0].down;
