@@ -40,8 +40,10 @@ OUTER_BYTES = 29 * CELLS
 UP_BYTES = 14 * CELLS
 FUSED_BYTES = 34 * CELLS
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE and WRITE_SIZE (scaled by a 1 GiB copy),
-# for the default settings (tile rows 8, two passes per grid); see profiles/hbm_traffic_*.md.
-MEASURED_TRAFFIC = {"outer": 78691287, "up": 30635786, "fused": {4: 73281985, 8: 75150788}}
+# for the default settings (two passes per grid), by the finest grid's tile rows; see
+# profiles/hbm_traffic_*.md.
+MEASURED_TRAFFIC = {"outer": 78736610, "up": {4: 30604038, 6: 30726311, 8: 30631686},
+                    "fused": {4: 73863905, 6: 73924156, 8: 73819458}}
 
 
 def log(msg: str) -> None:
@@ -363,7 +365,7 @@ def main() -> int:
         },
         "roofline": ({
             "bound": "hbm",
-            "kernel": "k_relax_tile<4,8,false,3> (the finest up leg with the binary64 outer step fused in: "
+            "kernel": f"k_relax_tile<4,{rows0},false,3> (the finest up leg with the binary64 outer step fused in: "
                       "prolongation, 2 red-black passes, x64 += delta, binary64 residual, next forcing, max), the "
                       "largest HBM-bound kernel of a step",
             "achieved": round(fused_gbs, 1),
@@ -392,12 +394,12 @@ def main() -> int:
             "algorithmic_bytes_per_launch": OUTER_BYTES,
         }) | {
             "smoother": {
-                "kernel": "k_relax_tile<4,8,false,2> (finest up leg: prolongation + 2 fused red-black passes)",
+                "kernel": f"k_relax_tile<4,{rows0},false,2> (finest up leg: prolongation + 2 fused red-black passes)",
                 "achieved": None if up_gbs is None else round(up_gbs, 1),
                 "frac": None if up_gbs is None else round(up_gbs / HBM_PEAK_GBS, 4),
                 "avg_launch_us": round(us_up, 3),
                 "algorithmic_bytes_per_launch": UP_BYTES,
-                "traffic": MEASURED_TRAFFIC["up"] if defaults else None,
+                "traffic": MEASURED_TRAFFIC["up"].get(rows0) if defaults else None,
                 "down_leg_avg_launch_us": round(us_down, 3),
             },
             "bottom": {"kernel": "k_bottom (the grids of 32x64 and smaller in one workgroup, latency-bound)",
