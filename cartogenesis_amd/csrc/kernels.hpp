@@ -999,6 +999,10 @@ struct BottomArgs {
 // what keeps it short.
 constexpr int kBottomRows = 32;
 constexpr int kBottomPairs = 32;
+// The register-held coarsest inverse: with nw <= 128 water cells each of the 1024 threads holds at
+// most 16 entries (parts = 1024 / nw >= 8 groups of the j-sum).
+constexpr int kBottomDirectRegs = 16;
+constexpr int kBottomDirectRegsWater = 128;
 
 struct BottomView {
     int across, down;
@@ -1068,6 +1072,27 @@ __global__ __launch_bounds__(1024) void k_bottom(BottomArgs a) {
         v.g = reinterpret_cast<const float *>(smem + a.lv[l].off_g);
         return v;
     };
+    // The coarsest inverse's entries this thread multiplies, and their cells, loaded now so that
+    // their latency passes under the down leg (at most kBottomDirectRegs each, for nw of at most
+    // kBottomDirectRegsWater water cells; larger coarsest grids read them at the solve).
+    const int nw = a.nw;
+    const int parts = nw > 0 && nw <= (int)blockDim.x ? (int)blockDim.x / nw : 1;
+    const int chunk = (int)blockDim.x / parts;
+    const bool direct_regs = nw > 0 && nw <= kBottomDirectRegsWater;
+    const int di = (int)threadIdx.x % chunk, dp = (int)threadIdx.x / chunk;
+    float mreg[kBottomDirectRegs];
+    int jreg[kBottomDirectRegs];
+    int oreg = 0;
+    if (direct_regs) {
+#pragma unroll
+        for (int k = 0; k < kBottomDirectRegs; ++k) {
+            const int j = dp + k * parts;
+            const bool ok = di < nw && dp < parts && j < nw;
+            mreg[k] = ok ? a.Mt[(unsigned)j * (unsigned)nw + (unsigned)di] : 0.0f;
+            jreg[k] = ok ? a.idx[j] : 0;
+        }
+        oreg = di < nw ? a.idx[di] : 0;
+    }
     // The grids' masks, row weights and restriction factors never change: the plan keeps them as
     // one blob laid out as this LDS region, copied in one coalesced pass (one load per thread).
     {
@@ -1092,41 +1117,65 @@ __global__ __launch_bounds__(1024) void k_bottom(BottomArgs a) {
         const BottomView L = view(l), C = view(l + 1);
         lds_relax_passes(L, 2);
         // Each grid halves both axes into the next (the plan checks it): 2 x 2 blocks.
+        // Thread (tx, ty) forms coarse cell (ty, tx) from its 2 x 2 block: every LDS read issued
+        // first, then the sums in the fixed order (dr, dc), a land fine cell adding nothing and a
+        // land coarse cell taking +0, as k_residual_restrict32 does.
         if (ty < C.down && tx < C.across) {
             const int t = ty * C.across + tx;
-            float acc = 0.0f;
-            if (C.w[t]) {
+            const bool cwet = C.w[t] != 0;
+            float u[2][2];
+            bool wet[2][2];
+            float g[2];
 #pragma unroll
-                for (int dr = 0; dr < 2; ++dr) {
-                    const int row = 2 * ty + dr;
-                    const float4 wt = L.rw[row];
-                    const float g = L.g[row];
+            for (int dr = 0; dr < 2; ++dr) {
+                const int row = 2 * ty + dr;
+                const float4 wt = L.rw[row];
+                g[dr] = L.g[row];
+                const bool has_n = row > 0, has_s = row + 1 < L.down;
 #pragma unroll
-                    for (int dc = 0; dc < 2; ++dc) {
-                        const int col = 2 * tx + dc;
-                        const int cell = row * L.across + col;
-                        if (!L.w[cell]) continue;
-                        const int ce = col + 1 == L.across ? 0 : col + 1;
-                        const int cw = col == 0 ? L.across - 1 : col - 1;
-                        const float xn = row > 0 ? L.x[cell - L.across] : 0.0f;
-                        const float xs = row + 1 < L.down ? L.x[cell + L.across] : 0.0f;
-                        const float u = L.x[cell] - rb_update(wt, L.x[row * L.across + ce], L.x[row * L.across + cw],
-                                                              xn, xs, L.f[cell]);
-                        acc += u * g;
-                    }
+                for (int dc = 0; dc < 2; ++dc) {
+                    const int col = 2 * tx + dc;
+                    const int cell = row * L.across + col;
+                    const int ce = col + 1 == L.across ? 0 : col + 1;
+                    const int cw = col == 0 ? L.across - 1 : col - 1;
+                    const float xn0 = L.x[has_n ? cell - L.across : cell];
+                    const float xs0 = L.x[has_s ? cell + L.across : cell];
+                    const float xn = has_n ? xn0 : 0.0f, xs = has_s ? xs0 : 0.0f;
+                    wet[dr][dc] = L.w[cell] != 0;
+                    u[dr][dc] = L.x[cell] - rb_update(wt, L.x[row * L.across + ce], L.x[row * L.across + cw], xn,
+                                                      xs, L.f[cell]);
                 }
             }
-            C.f[t] = acc;
+            float acc = 0.0f;
+#pragma unroll
+            for (int dr = 0; dr < 2; ++dr)
+#pragma unroll
+                for (int dc = 0; dc < 2; ++dc) {
+                    const float sum = acc + u[dr][dc] * g[dr];
+                    acc = wet[dr][dc] ? sum : acc;
+                }
+            C.f[t] = cwet ? acc : 0.0f;
         }
         __syncthreads();
     }
     // Coarsest: x = M f with the j-sum split over `parts` thread groups, then summed in LDS.
     {
         const BottomView C = view(a.nlev - 1);
-        if (a.nw > 0) {
-            const int nw = a.nw;
-            const int parts = nw <= (int)blockDim.x ? (int)blockDim.x / nw : 1;
-            const int chunk = (int)blockDim.x / parts;
+        if (direct_regs) {
+            // The same sums in the same order as below, from registers.
+            float acc = 0.0f;
+#pragma unroll
+            for (int k = 0; k < kBottomDirectRegs; ++k)
+                if (dp + k * parts < nw) acc = fmaf(mreg[k], C.f[jreg[k]], acc);
+            partial[threadIdx.x] = acc;
+            __syncthreads();
+            if (dp == 0 && di < nw) {
+                float sum = 0.0f;
+                for (int q = 0; q < parts; ++q) sum += partial[q * chunk + di];
+                C.x[oreg] = sum;
+            }
+            __syncthreads();
+        } else if (nw > 0) {
             for (int base = 0; base < nw; base += chunk) {
                 const int i = base + (int)threadIdx.x % chunk;
                 const int p = (int)threadIdx.x / chunk;
@@ -1152,41 +1201,33 @@ __global__ __launch_bounds__(1024) void k_bottom(BottomArgs a) {
     for (int l = a.nlev - 2; l >= 0; --l) {
         const BottomView L = view(l), C = view(l + 1);
         const int ca = C.across, cd = C.down;
-        float pv[2] = {0.0f, 0.0f};
         const int row = ty;
         const bool mine = row < L.down && tx < (L.across >> 1);
         if (mine) {
-            // prolongAdd's positions with 2 fine cells per coarse one along each axis: dividing by
-            // 2.0f is exact, so the arithmetic is the reference's (OC:369-388).
-            const float crow = ((float)row + 0.5f) / 2.0f - 0.5f;
-            const int above = (int)floorf(crow);
-            const float db = crow - (float)above;
+            // prolongAdd's positions with 2 fine cells per coarse one along each axis (OC:369-388):
+            // fine row r lies between coarse rows above = floor((r + 0.5) / 2 - 0.5) = (r - 1) >> 1
+            // and above + 1 with weight db = 0.75 (r even) or 0.25 (r odd) on the lower one, and
+            // likewise for columns; these are the reference's values exactly (dividing by 2.0f is
+            // exact). Beyond a pole the edge row times `beyond`; in range the factor is 1, and
+            // 1 * v is v exactly. Each thread reads only coarse values and its own fine cells, so
+            // it writes at once.
+            const int above = (row - 1) >> 1;
+            const float db = (row & 1) ? 0.25f : 0.75f;
+            const int ra = above < 0 ? 0 : above, rb = above + 1 >= cd ? cd - 1 : above + 1;
+            const float fa = above < 0 ? a.beyond : 1.0f, fb = above + 1 >= cd ? a.beyond : 1.0f;
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
                 const int col = 2 * tx + k;
                 const int cell = row * L.across + col;
-                if (!L.w[cell]) continue;
-                const float ccol = ((float)col + 0.5f) / 2.0f - 0.5f;
-                const int left = (int)floorf(ccol);
-                const float ab = ccol - (float)left;
-                // left is in -1 .. ca - 1.
+                const int left = tx - 1 + k;  // (col - 1) >> 1
+                const float ab = k ? 0.25f : 0.75f;
                 const int wc = left < 0 ? left + ca : left, ec = left + 1 == ca ? 0 : left + 1;
-                auto at = [&](int rr, int cc) -> float {
-                    if (rr < 0) return a.beyond * C.x[cc];
-                    if (rr >= cd) return a.beyond * C.x[(cd - 1) * ca + cc];
-                    return C.x[rr * ca + cc];
-                };
-                const float top = at(above, wc) * (1.0f - ab) + at(above, ec) * ab;
-                const float bottom = at(above + 1, wc) * (1.0f - ab) + at(above + 1, ec) * ab;
-                pv[k] = L.x[cell] + (top * (1.0f - db) + bottom * db);
+                const float top = (fa * C.x[ra * ca + wc]) * (1.0f - ab) + (fa * C.x[ra * ca + ec]) * ab;
+                const float bottom = (fb * C.x[rb * ca + wc]) * (1.0f - ab) + (fb * C.x[rb * ca + ec]) * ab;
+                const float v = L.x[cell] + (top * (1.0f - db) + bottom * db);
+                if (L.w[cell]) L.x[cell] = v;
             }
         }
-        __syncthreads();
-        if (mine)
-            for (int k = 0; k < 2; ++k) {
-                const int cell = row * L.across + 2 * tx + k;
-                if (L.w[cell]) L.x[cell] = pv[k];
-            }
         __syncthreads();
         lds_relax_passes(L, 2);
     }
