@@ -507,8 +507,13 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
                                               lane_value(wrow.w, ii));
                 const float xw = lane_from_below(B[ii]);
                 const float xe = lane_from_above(A[ii]);
-                const float ua = A[ii] - rb_update(wt, B[ii], xw, A[ii - 1], A[ii + 1], fA[ii]);
-                const float ub = B[ii] - rb_update(wt, xe, A[ii], B[ii - 1], B[ii + 1], fB[ii]);
+                // The cells of the last half-sweep's colour were just set to exactly the update the
+                // residual would recompute from the same, unchanged neighbours, so their residual
+                // is x - x (+0, or NaN where x is not finite) without recomputing it.
+                constexpr int kLast = (NH - 1) & 1;
+                const bool a_last = ((ii + HR) & 1) == kLast;
+                const float ua = a_last ? A[ii] - A[ii] : A[ii] - rb_update(wt, B[ii], xw, A[ii - 1], A[ii + 1], fA[ii]);
+                const float ub = a_last ? B[ii] - rb_update(wt, xe, A[ii], B[ii - 1], B[ii + 1], fB[ii]) : B[ii] - B[ii];
                 const uint32_t mA = ii < 32 ? (uint32_t)wA : (uint32_t)(wA >> 32);
                 const uint32_t mB = ii < 32 ? (uint32_t)wB : (uint32_t)(wB >> 32);
                 u[2 * k] = keep_if(ua, mA, ii & 31);
