@@ -1147,8 +1147,12 @@ __global__ __launch_bounds__(1024) void k_bottom(BottomArgs a) {
                     const float xs0 = L.x[has_s ? cell + L.across : cell];
                     const float xn = has_n ? xn0 : 0.0f, xs = has_s ? xs0 : 0.0f;
                     wet[dr][dc] = L.w[cell] != 0;
-                    u[dr][dc] = L.x[cell] - rb_update(wt, L.x[row * L.across + ce], L.x[row * L.across + cw], xn,
-                                                      xs, L.f[cell]);
+                    // Colour 1 (row + col odd) was the last relaxed, from these same neighbours:
+                    // its residual is exactly x - x.
+                    u[dr][dc] = ((dr + dc) & 1)
+                                    ? L.x[cell] - L.x[cell]
+                                    : L.x[cell] - rb_update(wt, L.x[row * L.across + ce], L.x[row * L.across + cw],
+                                                            xn, xs, L.f[cell]);
                 }
             }
             float acc = 0.0f;
