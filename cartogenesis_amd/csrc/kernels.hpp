@@ -908,6 +908,56 @@ __global__ __launch_bounds__(256) void k_outer_residual(LevelArgs L, const float
     grid_max(m, red);
 }
 
+// k_outer_residual from x = 0 with no delta, as the first outer step of a solve from zero: every
+// value and neighbour is +0, so only the forcing, the mask and the weights are read; the
+// arithmetic is the same expression with the zeros written in (x64 new = 0, u = 0 - bal on water,
+// 0 on land), so the results are the same bits. Writes xnew = 0 (the binary64 answer of the next
+// parity), fc, and the largest |u c| into red.out (not cleared: the solve zeroes it first).
+__global__ __launch_bounds__(256) void k_first_outer(LevelArgs L, const float *__restrict__ f,
+                                                     double *__restrict__ xnew, float *__restrict__ fc,
+                                                     MaxReduce red, OuterBatch ob) {
+    const int inst = blockIdx.z;
+    if (inst) {
+        f += inst * ob.s32;
+        xnew += inst * ob.s64;
+        fc += inst * ob.s32;
+        red.out += inst * ob.red_stride;
+    }
+    const int across = L.across, down = L.down, half = across >> 1;
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    unsigned long long m = 0;
+    if (c < half) {
+        const int col = 2 * c;
+#pragma unroll
+        for (int k = 0; k < kRowsPerBlock; ++k) {
+            const int row = blockIdx.y * kRowsPerBlock + k;
+            if (row >= down) break;
+            const long idx = (long)row * across + col;
+            const float2 fv = *reinterpret_cast<const float2 *>(f + idx);
+            const uint16_t wm = *reinterpret_cast<const uint16_t *>(L.water + idx);
+            const float4 wt = weights_at(L, row, idx);
+            const double cen = centre_at(L, row, idx);
+            const double z = 0.0;
+            double ux = 0.0, uy = 0.0;
+            if (wm & 0xff) {
+                const double bal = (double)wt.x * z + (double)wt.y * z + (double)wt.z * z + (double)wt.w * z - (double)fv.x;
+                ux = z - bal;
+                const unsigned long long b = abs_bits(ux * cen);
+                m = b > m ? b : m;
+            }
+            if (wm >> 8) {
+                const double bal = (double)wt.x * z + (double)wt.y * z + (double)wt.z * z + (double)wt.w * z - (double)fv.y;
+                uy = z - bal;
+                const unsigned long long b = abs_bits(uy * cen);
+                m = b > m ? b : m;
+            }
+            *reinterpret_cast<double2 *>(xnew + idx) = make_double2(0.0, 0.0);
+            *reinterpret_cast<float2 *>(fc + idx) = make_float2((float)ux, (float)uy);
+        }
+    }
+    grid_max(m, red);
+}
+
 // Binary32 residual of the correction problem restricted to the next grid, fused: each thread owns
 // one coarse cell, sums u * centre_fine * share over the fine cells it covers and divides by the
 // coarse centre, giving the coarse forcing (zero on coarse land).

@@ -312,9 +312,12 @@ struct cg_plan {
     // Instances: K right-hand sides on one ladder. Every per-instance grid array holds K copies,
     // each a grid with its zero row (vstride(l) floats); the binary64 answers K copies of the finest
     // cells. The refined method's reduction slots of instance i and binary64 parity q are group
-    // red_base + 2 i + q of d_u64 (red_base 0 for one instance, 4 otherwise).
+    // red_base + 2 i + q of d_u64 (red_base 0 for one instance, 4 otherwise); the maximum of the
+    // first outer step of a solve from zero is group first_base + i (first_base = max(red_base +
+    // 2K, 4)), read with the first batch of steps.
     int K = 1;
     int red_base = 0;
+    int first_base = 4;
     unsigned long long *h_red = nullptr;  // pinned copy of d_u64
     int *d_active = nullptr, *h_active = nullptr;  // per instance: still solving (K > 1)
     std::vector<double> largest_k;        // per instance max |f c|, < 0 when unknown
@@ -1263,33 +1266,58 @@ int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *infos, const v
     int rc;
     for (int i = 0; i < K; ++i)
         if ((rc = plan_largest_at(p, i, &largest[i]))) return rc;
-    if (!warm || !p->x64_valid) {
-        HIPCHK(ctx, hipMemsetAsync(p->x64[p->x64cur], 0, sizeof(double) * st->cells * K, ctx->stream));
-    }
+    // From zero the first outer step needs no readback before the first cycle: every instance
+    // starts active, the first cycle runs, and the first step's maxima (their own groups) are read
+    // with it. An instance that was already under tol then keeps the first step's answer (zero) and
+    // no cycle, exactly as if it had been stopped before the cycle.
+    const bool cold = !warm || !p->x64_valid;
     p->x64_valid = true;
     if (K > 1) {
         for (int i = 0; i < K; ++i) p->h_active[i] = 1;
         HIPCHK(ctx, hipMemcpyAsync(p->d_active, p->h_active, sizeof(int) * K, hipMemcpyHostToDevice, ctx->stream));
     }
     unsigned long long *const groups = p->d_u64 + p->red_base * kMaxSlots;
-    hipLaunchKernelGGL(k_fill_u64, dim3(2 * K), dim3(kMaxSlots), 0, ctx->stream, groups, 2 * K * kMaxSlots, 0ull);
+    const int ngroups = p->first_base + K - p->red_base;  // parity groups, then first-step groups
+    hipLaunchKernelGGL(k_fill_u64, dim3(ngroups), dim3(kMaxSlots), 0, ctx->stream, groups, ngroups * kMaxSlots, 0ull);
     auto slot_max = [&](int i, int par) { return group_max(p->h_red + (p->red_base + 2 * i + par) * kMaxSlots); };
+    auto first_max = [&](int i) { return group_max(p->h_red + (p->first_base + i) * kMaxSlots); };
     auto read_all = [&]() -> int {
-        HIPCHK(ctx, hipMemcpyAsync(p->h_red + p->red_base * kMaxSlots, groups, 2 * K * kMaxSlots * sizeof(unsigned long long),
+        HIPCHK(ctx, hipMemcpyAsync(p->h_red + p->red_base * kMaxSlots, groups, ngroups * kMaxSlots * sizeof(unsigned long long),
                                    hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
         return CG_OK;
     };
     const int first_slot = p->x64cur;
-    if ((rc = outer_step(p, nullptr, outer_fused(p) ? level0_forcing(p, 1 - p->x64cur) : nullptr))) return rc;
-    if ((rc = read_all())) return rc;
-    p->ans_par.assign(K, p->x64cur);
+    float *const first_fc = outer_fused(p) ? level0_forcing(p, 1 - p->x64cur) : p->lv[0].f;
+    bool first_pending = false;
     int left = 0;
-    for (int i = 0; i < K; ++i) {
-        res[i] = largest[i] > 0.0 ? bits_to_double(slot_max(i, first_slot)) / largest[i] : 0.0;
-        on[i] = res[i] >= tol;
-        left += on[i];
+    if (cold) {
+        MaxReduce red;
+        red.out = p->d_u64 + p->first_base * kMaxSlots;
+        red.clear = nullptr;
+        OuterBatch ob{};
+        ob.s32 = vstride(st);
+        ob.s64 = st->cells;
+        ob.red_stride = kMaxSlots;
+        const dim3 grid(blocks_for(st->across / 2, 256), blocks_for(st->down, kRowsPerBlock), K);
+        hipLaunchKernelGGL(k_first_outer, grid, dim3(256), 0, ctx->stream, st->args(), p->f0, p->x64[1 - p->x64cur],
+                           first_fc, red, ob);
+        HIPCHK(ctx, hipGetLastError());
+        p->x64cur = 1 - p->x64cur;
+        p->ans_par.assign(K, p->x64cur);
+        first_pending = true;
+        left = K;
+    } else {
+        if ((rc = outer_step(p, nullptr, first_fc))) return rc;
+        if ((rc = read_all())) return rc;
+        p->ans_par.assign(K, p->x64cur);
+        for (int i = 0; i < K; ++i) {
+            res[i] = largest[i] > 0.0 ? bits_to_double(slot_max(i, first_slot)) / largest[i] : 0.0;
+            on[i] = res[i] >= tol;
+            left += on[i];
+        }
     }
+    const int first_par = p->x64cur;
     const bool graph = ctx->use_graph && !ctx->time_kernels;
     if (graph && left > 0) {
         if ((rc = build_step_graphs(p))) return rc;
@@ -1336,6 +1364,18 @@ int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *infos, const v
         if ((rc = read_all())) return rc;
         steps += batch;
         bool changed = false;
+        if (first_pending) {
+            first_pending = false;
+            for (int i = 0; i < K; ++i) {
+                res[i] = largest[i] > 0.0 ? bits_to_double(first_max(i)) / largest[i] : 0.0;
+                if (!(res[i] >= tol)) {
+                    on[i] = 0;
+                    --left;
+                    changed = true;
+                    p->ans_par[i] = first_par;
+                }
+            }
+        }
         for (int i = 0; i < K; ++i) {
             if (!on[i]) continue;
             cycles[i] += batch;
@@ -1352,6 +1392,11 @@ int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *infos, const v
             if (prev > 0.0 && res[i] > 0.0) rate[i] = std::pow(res[i] / prev, 1.0 / batch);
         }
         if (changed && left > 0 && (rc = publish_active())) return rc;
+    }
+    if (first_pending) {
+        // Cancelled before the first cycle: the first step's residual, and its answer (zero).
+        if ((rc = read_all())) return rc;
+        for (int i = 0; i < K; ++i) res[i] = largest[i] > 0.0 ? bits_to_double(first_max(i)) / largest[i] : 0.0;
     }
     for (int i = 0; i < K; ++i) {
         infos[i] = cg_solution{};
@@ -1788,8 +1833,8 @@ int cg_ocean_plan_create_batch(cg_ctx *ctx, const cg_stencil *levels, int nlevel
     if ((rc = dalloc(ctx, &p->r64, n0))) return bail(rc);
     if ((rc = dalloc(ctx, &p->x64[0], n0 * K))) return bail(rc);
     if ((rc = dalloc(ctx, &p->x64[1], n0 * K))) return bail(rc);
-    const int groups = p->red_base + 2 * K;
-    const int alloc_groups = groups > 4 ? groups : 4;
+    p->first_base = std::max(p->red_base + 2 * K, 4);
+    const int alloc_groups = p->first_base + K;
     if ((rc = dalloc(ctx, &p->d_u64, (size_t)alloc_groups * kMaxSlots))) return bail(rc);
     HIPCHK(ctx, hipMemset(p->d_u64, 0, (size_t)alloc_groups * kMaxSlots * sizeof(unsigned long long)));
     if (hipHostMalloc((void **)&p->h_red, (size_t)alloc_groups * kMaxSlots * sizeof(unsigned long long),
