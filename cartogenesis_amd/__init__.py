@@ -21,7 +21,7 @@ CG_ERR_ARG, CG_ERR_DEVICE, CG_ERR_NOMEM, CG_ERR_STATE = -1, -2, -3, -4
 CG_METHOD_REFERENCE, CG_METHOD_RESCUE, CG_METHOD_REFINED, CG_METHOD_FGMRES, CG_METHOD_KRYLOV = 0, 1, 2, 3, 4
 CG_OPT_MIN_CELLS, CG_OPT_TILE_ROWS, CG_OPT_USE_GRAPH = 1, 2, 3
 CG_OPT_FGMRES_RESTART, CG_OPT_TIME_KERNELS = 4, 6
-CG_OPT_FUSED, CG_OPT_LEVEL_PASSES, CG_OPT_FUSE_OUTER = 7, 8, 9
+CG_OPT_FUSED, CG_OPT_LEVEL_PASSES, CG_OPT_FUSE_OUTER, CG_OPT_DIRECT_BOTTOM = 7, 8, 9, 10
 
 
 class CgSolution(ctypes.Structure):

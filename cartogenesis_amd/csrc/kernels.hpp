@@ -1299,6 +1299,49 @@ __global__ __launch_bounds__(1024) void k_bottom(BottomArgs a) {
 }
 
 // ------------------------------------------------------------------------------------------------
+// The direct bottom: x = M f on a grid's water cells, M = -(I - W)^-1 dense (row-major, n x n,
+// binary32, n <= kDirectMaxN), as a chip-wide matrix-vector product: one wavefront per output.
+// Each workgroup gathers the water forcing into LDS; wave i sums row i of M against it, lane l
+// taking columns l, l + 64, ... in order with fmaf (each step a coalesced 256-byte read of the row),
+// then the lanes' sums meet in a fixed butterfly and lane 0 writes x at the cell. No atomics and
+// no order that depends on timing, so the result is the same every launch. Land cells of x are not
+// written: they hold +0.
+// ------------------------------------------------------------------------------------------------
+constexpr int kDirectMaxN = 2048;
+constexpr int kDirectWaves = 4;  // outputs (wavefronts) per workgroup
+
+struct DirectArgs {
+    const int *idx;      // water cells
+    const float *M;      // row-major
+    int n;
+    const float *f;      // forcing of the grid (instances f_stride apart)
+    float *x;            // values of the grid (instances x_stride apart)
+    long f_stride, x_stride;
+    const int *active;
+};
+
+__global__ __launch_bounds__(64 * kDirectWaves) void k_direct_bottom(DirectArgs a) {
+    const int inst = blockIdx.y;
+    if (a.active && !a.active[inst]) return;
+    const int n = a.n;
+    const float *f = a.f + inst * a.f_stride;
+    float *x = a.x + inst * a.x_stride;
+    __shared__ float fs[kDirectMaxN];
+    for (int j = threadIdx.x; j < n; j += blockDim.x) fs[j] = f[a.idx[j]];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * kDirectWaves + (threadIdx.x >> 6);
+    if (i >= n) return;
+    const float *row = a.M + (size_t)i * n;
+    float acc = 0.0f;
+#pragma unroll 8
+    for (int j = lane; j < n; j += 64) acc = fmaf(row[j], fs[j], acc);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) x[a.idx[i]] = acc;
+}
+
+// ------------------------------------------------------------------------------------------------
 // Flexible GMRES (Saad 1993, SIAM J. Sci. Comput. 14, 461-469) on the binary32 correction problem,
 // right-preconditioned by one binary32 V-cycle, inside the binary64 refinement. Unnormalised
 // Arnoldi: basis vectors q_j are kept with their norms nu_j, so that no pass only rescales. The

@@ -11,6 +11,7 @@
 #include <map>
 #include <mutex>
 #include <set>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -89,6 +90,7 @@ struct cg_ctx {
     int time_kernels = 0;
     int fused = 1;
     int fuse_outer = 1;  // the finest up leg and the binary64 outer step in one launch
+    int direct_bottom = 1;  // the binary32 cycle's bottom grid solved by its dense inverse
     // Passes per level of the fused binary32 cycle, 4 bits per level from level 0 up (2, 4 or 6;
     // levels past the eighth and the bottom's run 2).
     long level_passes = 0x22222222;
@@ -341,8 +343,15 @@ struct cg_plan {
     int nw = 0;
     int *d_idx = nullptr;
     float *d_Mt = nullptr;
-    // Graphs of one refinement step, for each parity of the binary64 ping-pong.
+    // Graphs of one refinement step, for each parity of the binary64 ping-pong, and the context
+    // settings they were captured under (a change of those rebuilds them).
     hipGraphExec_t step_exec[2] = {nullptr, nullptr};
+    long graph_key = -1;
+    // The direct bottom (CG_OPT_DIRECT_BOTTOM): level `bottom` solved by M = -(I - W)^-1 over its
+    // bnw water cells (row-major, binary32); bnw = 0 when the plan has none.
+    int bnw = 0;
+    int *d_bidx = nullptr;
+    float *d_bM = nullptr;
     // The fused cycle: tile levels 0..bottom-1 run fused down/up kernels, levels bottom..n-1 run
     // in one workgroup (k_bottom) with bottom_lds bytes of LDS; fusable is false when a level
     // has per-cell weights or does not halve both axes, and the unfused cycle runs instead.
@@ -414,6 +423,8 @@ void free_plan(cg_plan *p) {
     dfree(p->d_passes);
     dfree(p->d_idx);
     dfree(p->d_Mt);
+    dfree(p->d_bidx);
+    dfree(p->d_bM);
     dfree(p->tau_e);
     dfree(p->tau_n);
     dfree(p->depth);
@@ -495,6 +506,96 @@ int build_coarsest_inverse(cg_ctx *ctx, cg_plan *p) {
     if ((rc = dalloc(ctx, &p->d_Mt, (size_t)n * n))) return rc;
     HIPCHK(ctx, hipMemcpy(p->d_idx, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     HIPCHK(ctx, hipMemcpy(p->d_Mt, mt.data(), sizeof(float) * mt.size(), hipMemcpyHostToDevice));
+    return CG_OK;
+}
+
+// The direct bottom's inverse: M = -(I - W)^-1 over the water cells of level `bottom`, in
+// binary64, stored row-major in binary32. I - W over water cells (row-major order) is banded,
+// half-bandwidth at most the grid's width (the east-west wrap and the north and south neighbours),
+// and an M-matrix with weakly dominant diagonal (W >= 0, each row's weights summing to at most 1),
+// so it is factored by banded LU without pivoting, whose pivots stay positive (a pivot that is not
+// positive and finite leaves the plan without a direct bottom); column j of the inverse is then a
+// forward and a backward banded solve of e_j, the columns shared among host threads.
+int build_direct_bottom(cg_ctx *ctx, cg_plan *p) {
+    p->bnw = 0;
+    if (p->bottom <= 0) return CG_OK;
+    const Stencil *st = p->lv[p->bottom].st;
+    const int across = st->across, down = st->down;
+    std::vector<int> idx;
+    std::vector<int> pos(st->cells, -1);
+    for (long c = 0; c < st->cells; ++c)
+        if (st->h_water[c]) {
+            pos[c] = (int)idx.size();
+            idx.push_back((int)c);
+        }
+    const int n = (int)idx.size();
+    if (n == 0 || n > kDirectMaxN) return CG_OK;
+    auto neighbours = [&](int i, auto &&visit) {
+        const int cell = idx[i];
+        const int row = cell / across, col = cell % across;
+        const float4 w = st->rowconst ? st->h_w[row] : st->h_w[cell];
+        const int ce = col + 1 == across ? 0 : col + 1;
+        const int cw = col == 0 ? across - 1 : col - 1;
+        visit(row * across + ce, (double)w.x);
+        visit(row * across + cw, (double)w.y);
+        if (row > 0) visit(cell - across, (double)w.z);
+        if (row + 1 < down) visit(cell + across, (double)w.w);
+    };
+    int bw = 0;
+    for (int i = 0; i < n; ++i)
+        neighbours(i, [&](int nb, double) {
+            if (pos[nb] >= 0) bw = std::max(bw, std::abs(pos[nb] - i));
+        });
+    const int width = 2 * bw + 1;
+    std::vector<double> a((size_t)n * width, 0.0);
+    auto at = [&](int i, int j) -> double & { return a[(size_t)i * width + (j - i + bw)]; };
+    for (int i = 0; i < n; ++i) {
+        at(i, i) = 1.0;
+        neighbours(i, [&](int nb, double wt) {
+            if (pos[nb] >= 0) at(i, pos[nb]) -= wt;
+        });
+    }
+    for (int k = 0; k < n; ++k) {
+        const double piv = at(k, k);
+        if (!(piv > 0.0) || !std::isfinite(piv)) return CG_OK;
+        const int last = std::min(n - 1, k + bw);
+        for (int i = k + 1; i <= last; ++i) {
+            double &lik = at(i, k);
+            if (lik == 0.0) continue;
+            lik /= piv;
+            for (int j = k + 1; j <= last; ++j) at(i, j) -= lik * at(k, j);
+        }
+    }
+    std::vector<float> m((size_t)n * n);
+    const int nthreads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nthreads; ++t)
+        pool.emplace_back([&, t]() {
+            std::vector<double> y(n);
+            for (int j = t; j < n; j += nthreads) {
+                std::fill(y.begin(), y.end(), 0.0);
+                y[j] = 1.0;
+                for (int i = j + 1; i < n; ++i) {  // L y = e_j (unit lower, band)
+                    double v = y[i];
+                    for (int k = std::max(j, i - bw); k < i; ++k) v -= at(i, k) * y[k];
+                    y[i] = v;
+                }
+                for (int i = n - 1; i >= 0; --i) {  // U x = y
+                    double v = y[i];
+                    const int last = std::min(n - 1, i + bw);
+                    for (int k = i + 1; k <= last; ++k) v -= at(i, k) * y[k];
+                    y[i] = v / at(i, i);
+                }
+                for (int i = 0; i < n; ++i) m[(size_t)i * n + j] = (float)(-y[i]);
+            }
+        });
+    for (auto &th : pool) th.join();
+    int rc;
+    if ((rc = dalloc(ctx, &p->d_bidx, n))) return rc;
+    if ((rc = dalloc(ctx, &p->d_bM, (size_t)n * n))) return rc;
+    HIPCHK(ctx, hipMemcpy(p->d_bidx, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(p->d_bM, m.data(), sizeof(float) * m.size(), hipMemcpyHostToDevice));
+    p->bnw = n;
     return CG_OK;
 }
 
@@ -617,6 +718,7 @@ int plan_fusion_setup(cg_ctx *ctx, cg_plan *p) {
     p->bottom_lds = bytes;
     int rc;
     if ((rc = build_bottom_blob(ctx, p))) return rc;
+    if ((rc = build_direct_bottom(ctx, p))) return rc;
     p->fusable = true;
     return CG_OK;
 }
@@ -998,7 +1100,31 @@ int level_half_sweeps(const cg_ctx *ctx, int l) {
 }
 
 // The bottom grids (levels bottom .. n-1) in one workgroup per instance: k_bottom.
+// Whether the binary32 cycle solves level `bottom` by its dense inverse (k_direct_bottom) instead
+// of running k_bottom's levels.
+inline bool direct_bottom(const cg_plan *p) { return p->ctx->direct_bottom && p->bnw > 0 && p->bottom > 0; }
+
+// Level `bottom`'s values from its forcing by the dense inverse, every instance in one launch.
+void direct_bottom_launch(cg_plan *p) {
+    PlanLevel &B0 = p->lv[p->bottom];
+    DirectArgs da{};
+    da.idx = p->d_bidx;
+    da.M = p->d_bM;
+    da.n = p->bnw;
+    da.f = B0.f;
+    da.x = B0.x[B0.cur];
+    da.f_stride = vstride(B0.st);
+    da.x_stride = vstride(B0.st);
+    da.active = p->K > 1 ? p->d_active : nullptr;
+    hipLaunchKernelGGL(k_direct_bottom, dim3((p->bnw + kDirectWaves - 1) / kDirectWaves, p->K), dim3(64 * kDirectWaves),
+                       0, p->ctx->stream, da);
+}
+
 void bottom_launch(cg_plan *p, const float *in0, float *out0) {
+    if (direct_bottom(p)) {
+        direct_bottom_launch(p);
+        return;
+    }
     hipStream_t s = p->ctx->stream;
     const int b = p->bottom;
     const float beyond = p->pole_is_wall ? -1.0f : 1.0f;
@@ -1139,7 +1265,10 @@ int vcycle32(cg_plan *p, bool timed, const float *in0 = nullptr, float *out0 = n
 int vcycle32_unfused(cg_plan *p, bool timed) {
     cg_ctx *ctx = p->ctx;
     hipStream_t s = ctx->stream;
-    const int n = p->n;
+    // With the direct bottom the cycle ends at level `bottom`, solved by its dense inverse, as the
+    // fused cycle does; otherwise at the coarsest grid.
+    const bool direct = direct_bottom(p);
+    const int n = direct ? p->bottom + 1 : p->n;
     int rc;
     for (int l = 0; l < n - 1; ++l) {
         PlanLevel &L = p->lv[l];
@@ -1149,7 +1278,9 @@ int vcycle32_unfused(cg_plan *p, bool timed) {
                            L.x[L.cur], C.st->args(), C.f);
     }
     PlanLevel &B = p->lv[n - 1];
-    if (n == 1) {
+    if (direct) {
+        direct_bottom_launch(p);
+    } else if (n == 1) {
         if ((rc = relax_pp(ctx, B.st, B.f, B.x, &B.cur, 2, true, timed))) return rc;
     } else if (p->nw == 0) {
         hipLaunchKernelGGL(k_fill_f32, dim3(blocks_for(B.st->cells, 256)), dim3(256), 0, s, B.x[B.cur], B.st->cells, 0.0f);
@@ -1216,6 +1347,17 @@ int refine_step(cg_plan *p, bool timed) {
 // parity suffices.
 int build_step_graphs(cg_plan *p) {
     cg_ctx *ctx = p->ctx;
+    // The settings that shape a step; graphs captured under others are rebuilt.
+    const long key = ((long)ctx->level_passes << 4) | (ctx->fused << 3) | (ctx->fuse_outer << 2) |
+                     (ctx->direct_bottom << 1);
+    if (p->graph_key != key) {
+        for (int i = 0; i < 2; ++i)
+            if (p->step_exec[i]) {
+                (void)hipGraphExecDestroy(p->step_exec[i]);
+                p->step_exec[i] = nullptr;
+            }
+        p->graph_key = key;
+    }
     for (int par = 0; par < 2; ++par) {
         if (p->step_exec[par]) continue;
         int saved_x64 = p->x64cur;
@@ -1584,6 +1726,7 @@ int cg_ocean_set_option(cg_ctx *ctx, int option, long value) {
         case CG_OPT_TIME_KERNELS: ctx->time_kernels = value ? 1 : 0; return CG_OK;
         case CG_OPT_FUSED: ctx->fused = value ? 1 : 0; return CG_OK;
         case CG_OPT_FUSE_OUTER: ctx->fuse_outer = value ? 1 : 0; return CG_OK;
+        case CG_OPT_DIRECT_BOTTOM: ctx->direct_bottom = value ? 1 : 0; return CG_OK;
         case CG_OPT_LEVEL_PASSES:
             for (int l = 0; l < 8; ++l) {
                 const long v = (value >> (4 * l)) & 0xf;
@@ -1608,6 +1751,7 @@ long cg_ocean_get_option(cg_ctx *ctx, int option) {
         case CG_OPT_TIME_KERNELS: return ctx->time_kernels;
         case CG_OPT_FUSED: return ctx->fused;
         case CG_OPT_FUSE_OUTER: return ctx->fuse_outer;
+        case CG_OPT_DIRECT_BOTTOM: return ctx->direct_bottom;
         case CG_OPT_LEVEL_PASSES: return ctx->level_passes;
         default: return -1;
     }

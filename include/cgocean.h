@@ -67,6 +67,13 @@ extern "C" {
 #define CG_OPT_FUSE_OUTER 9      /* 1 (default): CG_METHOD_REFINED runs the binary64 outer step inside the
                                     finest grid's up-leg launch (fused cycle, two passes there); 0: as a
                                     launch of its own. Both give the same bits. */
+#define CG_OPT_DIRECT_BOTTOM 10  /* 1 (default): the binary32 cycle of CG_METHOD_REFINED and
+                                    CG_METHOD_FGMRES ends at the largest grid of at most 32 x 64 cells,
+                                    solved by its dense inverse (M = -(I - W)^-1 over its water cells,
+                                    built when the plan is created) in one chip-wide launch; 0: that
+                                    grid and the coarser ones are cycled in one workgroup down to the
+                                    coarsest. The reference method and the Krylov preconditioner
+                                    always cycle the whole ladder (OC:740-781). */
 
 typedef struct cg_ctx cg_ctx;
 typedef struct cg_plan cg_plan;
@@ -177,7 +184,9 @@ double cg_ocean_kernel_time_us(cg_ctx *ctx, int kind, long *launches);
 /* Times `reps` back-to-back launches of one of the plan's kernels between two HIP events on the
  * context's stream and returns the average in microseconds (-1 on failure): kind 0 the finest
  * grid's fused down leg, 1 its fused up leg, 2 the binary64 outer residual, 3 the finest up leg
- * with the outer step fused in, 4 the one-workgroup bottom of the cycle (its coarsest grids). Each launch reads the plan's current buffers and writes the same
+ * with the outer step fused in, 4 the bottom of the cycle (the dense-inverse launch with
+ * CG_OPT_DIRECT_BOTTOM, else the one-workgroup cycle of the coarsest grids). Each launch reads the
+ * plan's current buffers and writes the same
  * outputs (never the answer), so repeats leave the plan as one launch does; call it between
  * solves, not during one. */
 double cg_ocean_plan_time_kernel(cg_plan *plan, int kind, int reps);

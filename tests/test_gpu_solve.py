@@ -244,3 +244,33 @@ def test_krylov_rescue_matches_the_processor(acc, rows, tol):
     assert sol.relative_residual == pytest.approx(rr, rel=1e-3)
     assert np.allclose(sol.values, x, rtol=1e-5, atol=1e-6 * np.abs(x).max())
     OC.require_solved("the circulation", sol, tol)
+
+
+@pytest.mark.parametrize("rows", [128, 256])
+def test_direct_bottom_and_cycled_bottom_reach_the_oracle(acc, rows):
+    """CG_OPT_DIRECT_BOTTOM: the binary32 cycle ends at its 32 x 64 grid solved by the dense
+    inverse (one chip-wide launch), or cycles that grid and the coarser ones in one workgroup. Both
+    preconditioners leave the binary64 fixed point unchanged: each reaches the binary64 oracle, and
+    the exact bottom solve needs no more cycles than the cycled one (one more allowed for the
+    rounding of the last step), fused and unfused alike."""
+    from cartogenesis_amd import CG_OPT_DIRECT_BOTTOM, CG_OPT_FUSED
+    p = circulation_problem(rows, seed=6)
+    ref64 = oracle.solve64([to_oracle(s) for s in p.levels], 1e-13)[0]
+    cycles = {}
+    try:
+        for fused in (1, 0):
+            for direct in (1, 0):
+                acc.set_option(CG_OPT_FUSED, fused)
+                acc.set_option(CG_OPT_DIRECT_BOTTOM, direct)
+                print(f"OCEAN direct bottom {direct} fused {fused} {2 * rows}x{rows}", flush=True)
+                sol = acc.solve_levels(p.levels, None, 1e-11, True, CG_METHOD_REFINED)
+                assert sol.relative_residual < 1e-11
+                assert rel_l2(sol.values64, ref64) <= PARITY_L2
+                cycles[fused, direct] = sol.cycles
+    finally:
+        acc.set_option(CG_OPT_FUSED, 1)
+        acc.set_option(CG_OPT_DIRECT_BOTTOM, 1)
+    for fused in (1, 0):
+        assert cycles[fused, 1] <= cycles[fused, 0] + 1
+    # The fused and unfused cycles share the direct bottom launch.
+    assert abs(cycles[1, 1] - cycles[0, 1]) <= 1
