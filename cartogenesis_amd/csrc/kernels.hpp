@@ -1301,44 +1301,71 @@ __global__ __launch_bounds__(1024) void k_bottom(BottomArgs a) {
 // ------------------------------------------------------------------------------------------------
 // The direct bottom: x = M f on a grid's water cells, M = -(I - W)^-1 dense (row-major, n x n,
 // binary32, n <= kDirectMaxN), as a chip-wide matrix-vector product: one wavefront per output.
-// Each workgroup gathers the water forcing into LDS; wave i sums row i of M against it, lane l
-// taking columns l, l + 64, ... in order with fmaf (each step a coalesced 256-byte read of the row),
-// then the lanes' sums meet in a fixed butterfly and lane 0 writes x at the cell. No atomics and
-// no order that depends on timing, so the result is the same every launch. Land cells of x are not
-// written: they hold +0.
+// Each workgroup gathers the water forcing of its instances into LDS; wave i sums row i of M
+// against each, lane l taking columns l, l + 64, ... in order with fmaf (each step a coalesced
+// 256-byte read of the row), then the lanes' sums meet in a fixed butterfly and lane 0 writes x at
+// the cell. No atomics and no order that depends on timing, so the result is the same every launch
+// and for every instance count. Land cells of x are not written: they hold +0.
 // ------------------------------------------------------------------------------------------------
 constexpr int kDirectMaxN = 2048;
-constexpr int kDirectWaves = 4;  // outputs (wavefronts) per workgroup
+constexpr int kDirectWaves = 8;  // outputs (wavefronts) per workgroup
 
 struct DirectArgs {
     const int *idx;      // water cells
     const float *M;      // row-major
     int n;
+    int K;               // instances
     const float *f;      // forcing of the grid (instances f_stride apart)
     float *x;            // values of the grid (instances x_stride apart)
     long f_stride, x_stride;
     const int *active;
 };
 
+// Workgroup (bx, by): output rows bx*kDirectWaves .. +kDirectWaves for instances
+// by*NI .. +NI, so a row of M is read once per NI instances (1 for a plan of one instance, whose
+// forcing then takes a quarter of the LDS).
+template <int NI>
 __global__ __launch_bounds__(64 * kDirectWaves) void k_direct_bottom(DirectArgs a) {
-    const int inst = blockIdx.y;
-    if (a.active && !a.active[inst]) return;
+    constexpr int kDirectInst = NI;
     const int n = a.n;
-    const float *f = a.f + inst * a.f_stride;
-    float *x = a.x + inst * a.x_stride;
-    __shared__ float fs[kDirectMaxN];
-    for (int j = threadIdx.x; j < n; j += blockDim.x) fs[j] = f[a.idx[j]];
+    const int c0 = blockIdx.y * kDirectInst;
+    const int ni = a.K - c0 < kDirectInst ? a.K - c0 : kDirectInst;
+    bool on[kDirectInst];
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < kDirectInst; ++c) {
+        on[c] = c < ni && (!a.active || a.active[c0 + c]);
+        any = any || on[c];
+    }
+    if (!any) return;
+    __shared__ float fs[kDirectInst][kDirectMaxN];
+    for (int c = 0; c < ni; ++c) {
+        const float *f = a.f + (c0 + c) * a.f_stride;
+        for (int j = threadIdx.x; j < n; j += blockDim.x) fs[c][j] = f[a.idx[j]];
+    }
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * kDirectWaves + (threadIdx.x >> 6);
     if (i >= n) return;
     const float *row = a.M + (size_t)i * n;
-    float acc = 0.0f;
+    float acc[kDirectInst] = {};
+    if (NI == 1 || ni == 1) {
 #pragma unroll 8
-    for (int j = lane; j < n; j += 64) acc = fmaf(row[j], fs[j], acc);
+        for (int j = lane; j < n; j += 64) acc[0] = fmaf(row[j], fs[0][j], acc[0]);
+    } else {
+#pragma unroll 4
+        for (int j = lane; j < n; j += 64) {
+            const float m = row[j];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-    if (lane == 0) x[a.idx[i]] = acc;
+            for (int c = 0; c < kDirectInst; ++c) acc[c] = fmaf(m, fs[c][j], acc[c]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < kDirectInst; ++c) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off, 64);
+        if (lane == 0 && on[c]) a.x[(c0 + c) * a.x_stride + a.idx[i]] = acc[c];
+    }
 }
 
 // ------------------------------------------------------------------------------------------------

@@ -1111,13 +1111,18 @@ void direct_bottom_launch(cg_plan *p) {
     da.idx = p->d_bidx;
     da.M = p->d_bM;
     da.n = p->bnw;
+    da.K = p->K;
     da.f = B0.f;
     da.x = B0.x[B0.cur];
     da.f_stride = vstride(B0.st);
     da.x_stride = vstride(B0.st);
     da.active = p->K > 1 ? p->d_active : nullptr;
-    hipLaunchKernelGGL(k_direct_bottom, dim3((p->bnw + kDirectWaves - 1) / kDirectWaves, p->K), dim3(64 * kDirectWaves),
-                       0, p->ctx->stream, da);
+    const unsigned rows = (p->bnw + kDirectWaves - 1) / kDirectWaves;
+    if (p->K == 1)
+        hipLaunchKernelGGL(k_direct_bottom<1>, dim3(rows, 1), dim3(64 * kDirectWaves), 0, p->ctx->stream, da);
+    else
+        hipLaunchKernelGGL(k_direct_bottom<4>, dim3(rows, (p->K + 3) / 4), dim3(64 * kDirectWaves), 0, p->ctx->stream,
+                           da);
 }
 
 void bottom_launch(cg_plan *p, const float *in0, float *out0) {
