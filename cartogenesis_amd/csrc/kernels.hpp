@@ -1308,7 +1308,7 @@ __global__ __launch_bounds__(1024) void k_bottom(BottomArgs a) {
 // and for every instance count. Land cells of x are not written: they hold +0.
 // ------------------------------------------------------------------------------------------------
 constexpr int kDirectMaxN = 2048;
-constexpr int kDirectWaves = 8;  // outputs (wavefronts) per workgroup
+constexpr int kDirectWaves = 16;  // outputs (wavefronts) per workgroup
 
 struct DirectArgs {
     const int *idx;      // water cells
