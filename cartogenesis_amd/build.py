@@ -47,14 +47,14 @@ def build(verbose: bool = False) -> Path:
     objs = []
     for src in CXX_SOURCES:
         obj = obj_dir / (src + ".o")
-        run(["g++", *COMMON, "-c", str(CSRC / src), "-o", str(obj)])
+        run(["g++", *COMMON, "-pthread", "-c", str(CSRC / src), "-o", str(obj)])
         objs.append(obj)
     for src in HIP_SOURCES:
         obj = obj_dir / (src + ".o")
         run([hipcc(), *COMMON, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
         objs.append(obj)
     tmp = LIB.with_suffix(".so.tmp")
-    run([hipcc(), "-shared", "-fPIC", "--offload-arch=gfx950", *map(str, objs), "-o", str(tmp)])
+    run([hipcc(), "-shared", "-fPIC", "-pthread", "--offload-arch=gfx950", *map(str, objs), "-o", str(tmp)])
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "cgocean.h"
@@ -414,6 +415,92 @@ int cg_problem_stress(int across, int down, uint64_t seed, int month, double *ea
     if (eastward) zonal_stress(across, down, seed, month, eastward);
     if (northward) std::fill(northward, northward + cells, 0.0);
     return CG_OK;
+}
+
+// M = -(I - W)^-1 over a grid's water cells (row-major order), for a direct solve of that grid:
+// x = M f gives, on water, x = W x - f exactly. I - W over the water cells is banded (half-bandwidth
+// at most the grid's width: the east-west wrap and the north and south neighbours) and an M-matrix
+// with weakly dominant diagonal (weights non-negative, each row's summing to at most 1), so banded
+// LU without pivoting, in binary64, keeps its pivots positive; column j of the inverse is one
+// forward and one backward banded solve of e_j, the columns shared among `threads` host threads.
+// Writes the water cells to idx[n] and M row-major to m[n * n] in binary32; returns n, 0 when there
+// is no water, more than max_n water cells, or a pivot that is not positive and finite, and
+// CG_ERR_ARG on bad arguments.
+int cg_bottom_inverse(int across, int down, const uint8_t *water, const float *east, const float *west,
+                      const float *north, const float *south, int max_n, int threads, int32_t *idx,
+                      float *m) {
+    if (across <= 0 || down <= 0 || !water || !east || !west || !north || !south || max_n < 0 || !idx || !m)
+        return CG_ERR_ARG;
+    const long cells = (long)across * down;
+    std::vector<int> cell_of;
+    std::vector<int> pos(cells, -1);
+    for (long c = 0; c < cells; ++c)
+        if (water[c]) {
+            pos[c] = (int)cell_of.size();
+            cell_of.push_back((int)c);
+        }
+    const int n = (int)cell_of.size();
+    if (n == 0 || n > max_n) return 0;
+    auto neighbours = [&](int i, auto &&visit) {
+        const int cell = cell_of[i];
+        const int row = cell / across, col = cell % across;
+        const int ce = col + 1 == across ? 0 : col + 1;
+        const int cw = col == 0 ? across - 1 : col - 1;
+        visit(row * across + ce, (double)east[cell]);
+        visit(row * across + cw, (double)west[cell]);
+        if (row > 0) visit(cell - across, (double)north[cell]);
+        if (row + 1 < down) visit(cell + across, (double)south[cell]);
+    };
+    int bw = 0;
+    for (int i = 0; i < n; ++i)
+        neighbours(i, [&](int nb, double) {
+            if (pos[nb] >= 0) bw = std::max(bw, std::abs(pos[nb] - i));
+        });
+    const int width = 2 * bw + 1;
+    std::vector<double> a((size_t)n * width, 0.0);
+    auto at = [&](int i, int j) -> double & { return a[(size_t)i * width + (j - i + bw)]; };
+    for (int i = 0; i < n; ++i) {
+        at(i, i) = 1.0;
+        neighbours(i, [&](int nb, double wt) {
+            if (pos[nb] >= 0) at(i, pos[nb]) -= wt;
+        });
+    }
+    for (int k = 0; k < n; ++k) {
+        const double piv = at(k, k);
+        if (!(piv > 0.0) || !std::isfinite(piv)) return 0;
+        const int last = std::min(n - 1, k + bw);
+        for (int i = k + 1; i <= last; ++i) {
+            double &lik = at(i, k);
+            if (lik == 0.0) continue;
+            lik /= piv;
+            for (int j = k + 1; j <= last; ++j) at(i, j) -= lik * at(k, j);
+        }
+    }
+    const int nthreads = std::max(1, threads);
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nthreads; ++t)
+        pool.emplace_back([&, t]() {
+            std::vector<double> y(n);
+            for (int j = t; j < n; j += nthreads) {
+                std::fill(y.begin(), y.end(), 0.0);
+                y[j] = 1.0;
+                for (int i = j + 1; i < n; ++i) {  // L y = e_j (unit lower, banded)
+                    double v = y[i];
+                    for (int k = std::max(j, i - bw); k < i; ++k) v -= at(i, k) * y[k];
+                    y[i] = v;
+                }
+                for (int i = n - 1; i >= 0; --i) {  // U x = y
+                    double v = y[i];
+                    const int last = std::min(n - 1, i + bw);
+                    for (int k = i + 1; k <= last; ++k) v -= at(i, k) * y[k];
+                    y[i] = v / at(i, i);
+                }
+                for (int i = 0; i < n; ++i) m[(size_t)i * n + j] = (float)(-y[i]);
+            }
+        });
+    for (auto &th : pool) th.join();
+    for (int i = 0; i < n; ++i) idx[i] = cell_of[i];
+    return n;
 }
 
 }  // extern "C"

@@ -509,92 +509,32 @@ int build_coarsest_inverse(cg_ctx *ctx, cg_plan *p) {
     return CG_OK;
 }
 
-// The direct bottom's inverse: M = -(I - W)^-1 over the water cells of level `bottom`, in
-// binary64, stored row-major in binary32. I - W over water cells (row-major order) is banded,
-// half-bandwidth at most the grid's width (the east-west wrap and the north and south neighbours),
-// and an M-matrix with weakly dominant diagonal (W >= 0, each row's weights summing to at most 1),
-// so it is factored by banded LU without pivoting, whose pivots stay positive (a pivot that is not
-// positive and finite leaves the plan without a direct bottom); column j of the inverse is then a
-// forward and a backward banded solve of e_j, the columns shared among host threads.
+// The direct bottom's inverse (cg_bottom_inverse) of level `bottom`, uploaded for k_direct_bottom.
 int build_direct_bottom(cg_ctx *ctx, cg_plan *p) {
     p->bnw = 0;
     if (p->bottom <= 0) return CG_OK;
     const Stencil *st = p->lv[p->bottom].st;
-    const int across = st->across, down = st->down;
-    std::vector<int> idx;
-    std::vector<int> pos(st->cells, -1);
-    for (long c = 0; c < st->cells; ++c)
-        if (st->h_water[c]) {
-            pos[c] = (int)idx.size();
-            idx.push_back((int)c);
-        }
-    const int n = (int)idx.size();
-    if (n == 0 || n > kDirectMaxN) return CG_OK;
-    auto neighbours = [&](int i, auto &&visit) {
-        const int cell = idx[i];
-        const int row = cell / across, col = cell % across;
-        const float4 w = st->rowconst ? st->h_w[row] : st->h_w[cell];
-        const int ce = col + 1 == across ? 0 : col + 1;
-        const int cw = col == 0 ? across - 1 : col - 1;
-        visit(row * across + ce, (double)w.x);
-        visit(row * across + cw, (double)w.y);
-        if (row > 0) visit(cell - across, (double)w.z);
-        if (row + 1 < down) visit(cell + across, (double)w.w);
-    };
-    int bw = 0;
-    for (int i = 0; i < n; ++i)
-        neighbours(i, [&](int nb, double) {
-            if (pos[nb] >= 0) bw = std::max(bw, std::abs(pos[nb] - i));
-        });
-    const int width = 2 * bw + 1;
-    std::vector<double> a((size_t)n * width, 0.0);
-    auto at = [&](int i, int j) -> double & { return a[(size_t)i * width + (j - i + bw)]; };
-    for (int i = 0; i < n; ++i) {
-        at(i, i) = 1.0;
-        neighbours(i, [&](int nb, double wt) {
-            if (pos[nb] >= 0) at(i, pos[nb]) -= wt;
-        });
+    const long cells = st->cells;
+    std::vector<float> w[4];
+    for (int k = 0; k < 4; ++k) w[k].resize(cells);
+    for (long c = 0; c < cells; ++c) {
+        const float4 v = st->rowconst ? st->h_w[c / st->across] : st->h_w[c];
+        w[0][c] = v.x;
+        w[1][c] = v.y;
+        w[2][c] = v.z;
+        w[3][c] = v.w;
     }
-    for (int k = 0; k < n; ++k) {
-        const double piv = at(k, k);
-        if (!(piv > 0.0) || !std::isfinite(piv)) return CG_OK;
-        const int last = std::min(n - 1, k + bw);
-        for (int i = k + 1; i <= last; ++i) {
-            double &lik = at(i, k);
-            if (lik == 0.0) continue;
-            lik /= piv;
-            for (int j = k + 1; j <= last; ++j) at(i, j) -= lik * at(k, j);
-        }
-    }
-    std::vector<float> m((size_t)n * n);
-    const int nthreads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    std::vector<std::thread> pool;
-    for (int t = 0; t < nthreads; ++t)
-        pool.emplace_back([&, t]() {
-            std::vector<double> y(n);
-            for (int j = t; j < n; j += nthreads) {
-                std::fill(y.begin(), y.end(), 0.0);
-                y[j] = 1.0;
-                for (int i = j + 1; i < n; ++i) {  // L y = e_j (unit lower, band)
-                    double v = y[i];
-                    for (int k = std::max(j, i - bw); k < i; ++k) v -= at(i, k) * y[k];
-                    y[i] = v;
-                }
-                for (int i = n - 1; i >= 0; --i) {  // U x = y
-                    double v = y[i];
-                    const int last = std::min(n - 1, i + bw);
-                    for (int k = i + 1; k <= last; ++k) v -= at(i, k) * y[k];
-                    y[i] = v / at(i, i);
-                }
-                for (int i = 0; i < n; ++i) m[(size_t)i * n + j] = (float)(-y[i]);
-            }
-        });
-    for (auto &th : pool) th.join();
+    std::vector<int32_t> idx(kDirectMaxN);
+    std::vector<float> m((size_t)kDirectMaxN * kDirectMaxN);
+    const int threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const int n = cg_bottom_inverse(st->across, st->down, st->h_water.data(), w[0].data(), w[1].data(),
+                                    w[2].data(), w[3].data(), kDirectMaxN, threads, idx.data(), m.data());
+    if (n <= 0) return CG_OK;
     int rc;
     if ((rc = dalloc(ctx, &p->d_bidx, n))) return rc;
     if ((rc = dalloc(ctx, &p->d_bM, (size_t)n * n))) return rc;
     HIPCHK(ctx, hipMemcpy(p->d_bidx, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-    HIPCHK(ctx, hipMemcpy(p->d_bM, m.data(), sizeof(float) * m.size(), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(p->d_bM, m.data(), sizeof(float) * (size_t)n * n, hipMemcpyHostToDevice));
     p->bnw = n;
     return CG_OK;
 }

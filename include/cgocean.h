@@ -242,6 +242,14 @@ int cg_circulation_stencil(int across, int down, double dx, double dy, const uin
 /* OceanCirculation.coarseWater (OC:404-423). */
 int cg_coarse_water(const uint8_t *fine, int fine_across, int fine_down, int coarse_across,
                     int coarse_down, int every_cell_water, uint8_t *coarse);
+/* The direct bottom's inverse, M = -(I - W)^-1 over a grid's water cells (no counterpart in the
+ * reference, whose cycle relaxes its coarsest grid, OC:784-794): per-cell weights as an
+ * OceanStencil holds them; writes the water cells (row-major) to idx[n] and M row-major (binary32,
+ * computed in binary64 by banded LU of the M-matrix I - W) to m[n * n]. Returns n; 0 when there is
+ * no water, more than max_n water cells, or I - W has a pivot that is not positive. */
+int cg_bottom_inverse(int across, int down, const uint8_t *water, const float *east, const float *west,
+                      const float *north, const float *south, int max_n, int threads, int32_t *idx,
+                      float *m);
 /* The level shapes OceanCirculation.levels walks (OC:434-462): returns the count (<= max_levels)
  * and fills across/down/cell width/cell height per level, finest first. */
 int cg_level_shapes(int across, int down, double dx, double dy, int max_levels, int *level_across,

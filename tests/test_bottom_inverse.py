@@ -1,0 +1,68 @@
+"""The direct bottom's dense inverse (cg_bottom_inverse), on the CPU, against numpy.
+
+M = -(I - W)^-1 over a grid's water cells is checked two ways that a wrong sign, a transposed
+operand, a dropped neighbour or a wrong wrap would each break: entry by entry against
+numpy.linalg.inv of the dense I - W assembled here from the stencil's per-cell weights, and by
+the defining property that x = M f satisfies x = W x - f on water (the relaxation's fixed point,
+OceanCirculation.relax, OC:241-274) for random forcings. Declines: no water, too many water cells.
+"""
+import numpy as np
+import pytest
+
+from cartogenesis_amd import CG_ERR_ARG, lib
+from cartogenesis_amd.problem import circulation_problem
+
+
+def dense_operator(st):
+    a, d = st.cells_across, st.cells_down
+    wet = st.is_water.astype(bool)
+    cells = np.flatnonzero(wet)
+    pos = -np.ones(a * d, np.int64)
+    pos[cells] = np.arange(cells.size)
+    A = np.eye(cells.size)
+    for i, c in enumerate(cells):
+        r, col = divmod(int(c), a)
+        for nb, w in (((r * a + (col + 1) % a), st.east_weight[c]), ((r * a + (col - 1) % a), st.west_weight[c]),
+                      ((c - a) if r > 0 else -1, st.north_weight[c]), ((c + a) if r + 1 < d else -1, st.south_weight[c])):
+            if nb >= 0 and pos[nb] >= 0:
+                A[i, pos[nb]] -= float(w)
+    return cells, A
+
+
+def bottom_inverse(st, max_n=4096):
+    n_max = int(st.is_water.sum())
+    idx = np.zeros(max(1, n_max), np.int32)
+    m = np.zeros(max(1, n_max) ** 2, np.float32)
+    n = lib().cg_bottom_inverse(st.cells_across, st.cells_down, np.ascontiguousarray(st.is_water, np.uint8),
+                                st.east_weight, st.west_weight, st.north_weight, st.south_weight, max_n, 4, idx, m)
+    return n, idx[:max(n, 0)], m[:max(n, 0) ** 2].reshape(max(n, 0), max(n, 0))
+
+
+@pytest.mark.parametrize("rows,level", [(128, 2), (256, 3), (64, 1)])
+def test_inverse_matches_numpy_and_solves_the_grid(rows, level):
+    st = circulation_problem(rows, seed=3).levels[level]
+    n, idx, M = bottom_inverse(st)
+    cells, A = dense_operator(st)
+    assert n == cells.size > 0
+    assert np.array_equal(idx, cells)
+    ref = -np.linalg.inv(A)
+    assert np.abs(M - ref).max() <= 2e-6 * np.abs(ref).max()
+    rng = np.random.default_rng(rows)
+    W = np.eye(n) - A
+    for _ in range(3):
+        f = rng.standard_normal(n)
+        x = M.astype(np.float64) @ f
+        assert np.abs(x - (W @ x - f)).max() <= 1e-5 * np.abs(x).max()
+
+
+def test_declines_and_bad_arguments():
+    st = circulation_problem(64, seed=1).levels[1]
+    n_water = int(st.is_water.sum())
+    assert bottom_inverse(st, max_n=n_water - 1)[0] == 0
+    dry = np.zeros(st.cells_across * st.cells_down, np.uint8)
+    idx = np.zeros(1, np.int32)
+    m = np.zeros(1, np.float32)
+    assert lib().cg_bottom_inverse(st.cells_across, st.cells_down, dry, st.east_weight, st.west_weight,
+                                   st.north_weight, st.south_weight, 2048, 1, idx, m) == 0
+    assert lib().cg_bottom_inverse(0, st.cells_down, dry, st.east_weight, st.west_weight, st.north_weight,
+                                   st.south_weight, 2048, 1, idx, m) == CG_ERR_ARG

@@ -274,3 +274,30 @@ def test_direct_bottom_and_cycled_bottom_reach_the_oracle(acc, rows):
         assert cycles[fused, 1] <= cycles[fused, 0] + 1
     # The fused and unfused cycles share the direct bottom launch.
     assert abs(cycles[1, 1] - cycles[0, 1]) <= 1
+
+
+def test_option_changes_rebuild_the_step_graphs(acc):
+    """A plan's step graphs are captured under the context's settings; changing one
+    (CG_OPT_DIRECT_BOTTOM here) between solves of the same plan rebuilds them: the solve with the
+    setting restored repeats the first to the bit, and the other setting's answer, from a
+    different preconditioner, agrees with it to the tolerance."""
+    from cartogenesis_amd import CG_OPT_DIRECT_BOTTOM
+    p = circulation_problem(128, seed=7)
+    plan = Plan(acc, p.levels)
+    try:
+        print("OCEAN graph rebuild on option change 256x128", flush=True)
+        first = plan.solve(1e-11, CG_METHOD_REFINED)
+        x_first, _ = plan.read()
+        acc.set_option(CG_OPT_DIRECT_BOTTOM, 0)
+        other = plan.solve(1e-11, CG_METHOD_REFINED)
+        x_other, _ = plan.read()
+        acc.set_option(CG_OPT_DIRECT_BOTTOM, 1)
+        again = plan.solve(1e-11, CG_METHOD_REFINED)
+        x_again, _ = plan.read()
+    finally:
+        acc.set_option(CG_OPT_DIRECT_BOTTOM, 1)
+        plan.close()
+    assert again.cycles == first.cycles
+    assert np.array_equal(x_again, x_first)
+    assert other.relative_residual < 1e-11
+    assert rel_l2(x_other, x_first) <= 1e-9
