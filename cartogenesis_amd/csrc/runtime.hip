@@ -1416,10 +1416,12 @@ int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *infos, const v
         return CG_OK;
     };
     if ((rc = publish_active())) return rc;
-    // Steps run in batches without a host read in between: the batch is the number of steps the
-    // slowest instance's last observed convergence factor predicts are still needed, less one (at
+    // Steps run in batches without a host read in between: the batch is the fewest steps any
+    // active instance's last observed convergence factor predicts it still needs, less one (at
     // least one, at most 8), so the host waits on the device about twice per solve rather than once
-    // per step. The policy is the same with and without graphs, so both give the same steps and bits.
+    // per step, and no instance runs steps past its own convergence while slower ones go on (each
+    // check lets the instances under tol stop). The policy is the same with and without graphs, so
+    // both give the same steps and bits.
     int steps = 0;
     int status = CG_OK;
     while (left > 0 && steps < kRefinedMostSteps) {
@@ -1427,7 +1429,7 @@ int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *infos, const v
             status = CG_CANCELLED;
             break;
         }
-        int batch = 1;
+        int batch = 0;
         for (int i = 0; i < K; ++i) {
             if (!on[i]) continue;
             int want = 1;
@@ -1435,8 +1437,9 @@ int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *infos, const v
                 const double need = std::ceil(std::log(tol / res[i]) / std::log(rate[i]));
                 want = (int)std::max(1.0, std::min(8.0, need - 1.0));
             }
-            batch = i == 0 || want > batch ? want : batch;
+            batch = batch == 0 || want < batch ? want : batch;
         }
+        if (batch == 0) batch = 1;
         batch = std::min(batch, kRefinedMostSteps - steps);
         int slot = p->x64cur;
         for (int k = 0; k < batch; ++k) {
