@@ -132,6 +132,7 @@ __device__ __forceinline__ float rb_update(float4 wt, float xe, float xw, float 
 // ------------------------------------------------------------------------------------------------
 enum TileMode { TILE_PLAIN = 0, TILE_DOWN = 1, TILE_UP = 2, TILE_UP_OUTER = 3 };
 
+
 // Workgroups are dealt to the MI355X's 8 XCDs round-robin by workgroup id, and each XCD has its own
 // L2. Renumbering so that XCD k runs the k-th contiguous eighth of the logical blocks keeps
 // neighbouring tiles, which read each other's halo rows, on one L2. A bijection on [0, gridDim.x):
@@ -243,14 +244,10 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
     constexpr int NW = MODE == TILE_DOWN ? TH / 2 : 1;
     constexpr int kLateLoadStep = R / 2;
     float cv[NQ];
-    // TILE_UP_OUTER's binary64 answer goes straight from memory to LDS (global_load_lds, 16 bytes a
-    // lane: lane l's pair lands in slot l of its wave's row), so its rows hold no registers while
-    // the sweeps run. Four waves a workgroup.
+    // TILE_UP_OUTER's binary64 answer of rows HR-1 .. HR+TH (this lane's pair) and the rows'
+    // binary64 centres, loaded halfway through the sweeps into registers.
     constexpr bool kOuter = MODE == TILE_UP_OUTER;
-    __shared__ double2 x64s[kOuter ? 4 : 1][kOuter ? NX : 1][kOuter ? 64 : 1];
-    // ... and the binary64 centres of its rows: 32-bit words, lane l loading word l of the tile's
-    // first row's centre onwards (lanes past the grid's last row load word 0 and go unused).
-    __shared__ __align__(16) uint32_t cenw[kOuter ? 4 : 1][kOuter ? 64 : 1];
+    double px[kOuter ? NX : 1], py[kOuter ? NX : 1], cenr[kOuter ? NF : 1];
     float2 f0v[NF];
     uint32_t cwv[NW];
     // TILE_DOWN's restriction factor per row, loaded up front too (rows past the last clamp to it).
@@ -368,10 +365,10 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
                 for (int j = 0; j < NX; ++j) {
                     const int r = row0 - 1 + j;
                     const int rc = r < 0 ? 0 : (r >= down ? down - 1 : r);
-                    const void *src = &at_byte<double>(ex.x64, (unsigned)rc * (8u * (unsigned)across) + 8u * (unsigned)ca);
-                    __builtin_amdgcn_global_load_lds(
-                        (const void *)src,
-                        (__attribute__((address_space(3))) void *)&x64s[threadIdx.x >> 6][j][0], 16, 0, 0);
+                    const double2 xv =
+                        at_byte<double2>(ex.x64, (unsigned)rc * (8u * (unsigned)across) + 8u * (unsigned)ca);
+                    px[j] = xv.x;
+                    py[j] = xv.y;
                 }
 #pragma unroll
                 for (int k = 0; k < TH; ++k) {
@@ -379,12 +376,8 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
                     const int rc = r < down ? r : down;
                     f0v[k] = at_byte<float2>(ex.f0, (unsigned)rc * (4u * (unsigned)across) + 4u * (unsigned)ca);
                 }
-                {
-                    const unsigned word = 2u * (unsigned)row0 + (unsigned)lane;
-                    const void *src = &at_byte<uint32_t>(L.rowc, word < 2u * (unsigned)down ? 4u * word : 0u);
-                    __builtin_amdgcn_global_load_lds(
-                        src, (__attribute__((address_space(3))) void *)&cenw[threadIdx.x >> 6][0], 4, 0, 0);
-                }
+#pragma unroll
+                for (int k = 0; k < TH; ++k) cenr[k] = L.rowc[row0 + k < down ? row0 + k : down - 1];
             }
         }
 #pragma unroll
@@ -440,9 +433,8 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
             const int r = row0 - 1 + j;
             vx[j] = vy[j] = 0.0;
             if (r >= 0 && r < down) {
-                const double2 xv = x64s[threadIdx.x >> 6][j][lane];
-                vx[j] = xv.x + (double)A[HR - 1 + j];
-                vy[j] = xv.y + (double)B[HR - 1 + j];
+                vx[j] = px[j] + (double)A[HR - 1 + j];
+                vy[j] = py[j] + (double)B[HR - 1 + j];
             }
         }
         unsigned long long mx = 0;
@@ -457,7 +449,7 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
             if (!writer || row >= down) continue;
             const unsigned off4 = (unsigned)row * (4u * (unsigned)across) + 4u * (unsigned)cabs;
             const float2 fv = f0v[k];
-            const double cen = *reinterpret_cast<const double *>(&cenw[threadIdx.x >> 6][2 * k]);
+            const double cen = cenr[k];
             const uint32_t mA = i < 32 ? (uint32_t)wA : (uint32_t)(wA >> 32);
             const uint32_t mB = i < 32 ? (uint32_t)wB : (uint32_t)(wB >> 32);
             // The rows beyond a pole already hold zero (vx, vy above).

@@ -515,6 +515,11 @@ class BatchPlan:
         check(lib().cg_ocean_plan_set_forcing_at(self._p, int(instance), np.ascontiguousarray(forcing, dtype=np.float32)),
               self.acc.ctx, "cg_ocean_plan_set_forcing_at")
 
+    def set_tile_rows(self, rows) -> None:
+        """As Plan.set_tile_rows."""
+        arr = (ctypes.c_int * len(self.levels))(*[int(r) for r in rows])
+        check(lib().cg_ocean_plan_set_tile_rows(self._p, arr), self.acc.ctx, "cg_ocean_plan_set_tile_rows")
+
     def autotune(self, reps: int = 20) -> List[int]:
         """As Plan.autotune, timing the launches of all instances."""
         rows = (ctypes.c_int * len(self.levels))()

@@ -135,3 +135,37 @@ def test_autotuned_tiles_change_no_bits(acc):
             assert np.array_equal(batch.read(i)[0].view(np.uint64), x0[i].view(np.uint64))
     finally:
         batch.close()
+
+
+@pytest.mark.parametrize("rows0", [16, 6])
+def test_batched_months_repeat_their_single_solves_to_the_bit(acc, rows0):
+    """Twelve different months of one 1024 x 2048 world in one batched plan: every instance's answer
+    and cycle count equal its own single-instance solve's, on every one of three repeats. With many
+    more workgroups than the device holds at once, a kernel that depends on what an earlier
+    workgroup left in LDS, or on the order in which workgroups run, shows up here (and did, for the
+    finest up leg's staging of the binary64 rows through LDS, since replaced by registers)."""
+    p = circulation_problem(1024, seed=7, land_fraction=0.3)
+    months = monthly_forcings(p)
+    rows = [rows0] + [0] * (len(p.levels) - 1)
+    single = Plan(acc, p.levels)
+    ref = []
+    try:
+        single.set_tile_rows(rows)
+        for f in months:
+            single.set_forcing(f)
+            ref.append((single.solve(1e-8, CG_METHOD_REFINED).cycles, single.read()[0]))
+    finally:
+        single.close()
+    for rep in range(3):
+        batch = BatchPlan(acc, p.levels, len(months))
+        try:
+            batch.set_tile_rows(rows)
+            for i, f in enumerate(months):
+                batch.set_forcing(i, f)
+            print(f"OCEAN batched months repeat {rep} tile rows {rows0}", flush=True)
+            infos = batch.solve_all(1e-8)
+            for i in range(len(months)):
+                assert infos[i].cycles == ref[i][0], (rep, i)
+                assert np.array_equal(batch.read(i)[0], ref[i][1]), (rep, i)
+        finally:
+            batch.close()
