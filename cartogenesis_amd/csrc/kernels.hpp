@@ -226,14 +226,13 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
     float A[R], B[R], fA[R], fB[R];
     uint32_t mraw[R];
     uint64_t wA = 0, wB = 0;
-    // What the modes read besides the tile's rows, loaded together with them so that a wave waits
-    // on memory once: TILE_UP's coarse values, TILE_UP_OUTER's binary64 answer and forcing, and
-    // TILE_DOWN's coarse mask.
+    // What the modes read besides the tile's rows: TILE_UP's coarse values and TILE_DOWN's coarse
+    // mask, loaded with the rows; TILE_UP_OUTER's binary64 answer, forcing and centres, later.
     //  - Coarse values: fine row r = row0 - HR + i reads coarse rows above = floor((r - 1) / 2) and
     //    above + 1, i.e. row0 / 2 + q with q = floor((i - HR - 1) / 2), a compile-time offset since
     //    row0 is even; lane l's pair (2m, 2m+1) reads coarse columns m-1, m, m+1 (lanes l-1, l, l+1).
     //  - The binary64 answer of rows HR-1 .. HR+TH (this lane's pair), zero beyond a pole, and the
-    //    binary64 problem's forcing of rows HR .. HR+TH-1.
+    //    binary64 problem's forcing and centres of rows HR .. HR+TH-1.
     //  - The coarse mask of each row pair's coarse cell. Writer lanes' columns cabs equal ca.
     constexpr bool kUp = MODE == TILE_UP || MODE == TILE_UP_OUTER;
     constexpr int QMIN = -((HR + 2) / 2);  // floor((0 - HR - 1) / 2)
@@ -244,8 +243,8 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
     constexpr int NW = MODE == TILE_DOWN ? TH / 2 : 1;
     constexpr int kLateLoadStep = R / 2;
     float cv[NQ];
-    // TILE_UP_OUTER's binary64 answer of rows HR-1 .. HR+TH (this lane's pair) and the rows'
-    // binary64 centres, loaded halfway through the sweeps into registers.
+    // TILE_UP_OUTER's binary64 answer of rows HR-1 .. HR+TH (this lane's pair), loaded halfway
+    // through the sweeps, and the rows' binary64 centres and forcing, loaded after them.
     constexpr bool kOuter = MODE == TILE_UP_OUTER;
     double px[kOuter ? NX : 1], py[kOuter ? NX : 1], cenr[kOuter ? NF : 1];
     float2 f0v[NF];
@@ -356,9 +355,8 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
                 B[i] += keep_if(topB * (1.0f - db) + botB * db, mB, i & 31);
             }
         }
-        // TILE_UP_OUTER's binary64 answer and forcing are only read after the sweeps: issued halfway
-        // through them, they arrive while the rest run, and the tile's first rows do not queue
-        // behind them.
+        // TILE_UP_OUTER's binary64 answer is only read after the sweeps: issued halfway through
+        // them, it arrives while the rest run, and the tile's first rows do not queue behind it.
         if constexpr (MODE == TILE_UP_OUTER) {
             if (t == kLateLoadStep) {
 #pragma unroll
@@ -370,14 +368,6 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
                     px[j] = xv.x;
                     py[j] = xv.y;
                 }
-#pragma unroll
-                for (int k = 0; k < TH; ++k) {
-                    const int r = row0 + k;
-                    const int rc = r < down ? r : down;
-                    f0v[k] = at_byte<float2>(ex.f0, (unsigned)rc * (4u * (unsigned)across) + 4u * (unsigned)ca);
-                }
-#pragma unroll
-                for (int k = 0; k < TH; ++k) cenr[k] = L.rowc[row0 + k < down ? row0 + k : down - 1];
             }
         }
 #pragma unroll
@@ -427,6 +417,16 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
         // neighbours there). Every lane holds its own pair, so the writers' neighbour lanes hold
         // theirs for the shifts; the correction is exact on those lanes (the column halo keeps two
         // columns of slack after NH half-sweeps).
+        // The binary64 forcing and centres of the tile's rows, loaded here rather than with the
+        // answer: held through the sweeps they cost a wave per SIMD of occupancy, and their
+        // latency now overlaps the answer's update below.
+#pragma unroll
+        for (int k = 0; k < TH; ++k) {
+            const int r = row0 + k;
+            const int rc = r < down ? r : down;
+            f0v[k] = at_byte<float2>(ex.f0, (unsigned)rc * (4u * (unsigned)across) + 4u * (unsigned)ca);
+            cenr[k] = L.rowc[row0 + k < down ? row0 + k : down - 1];
+        }
         double vx[TH + 2], vy[TH + 2];
 #pragma unroll
         for (int j = 0; j < TH + 2; ++j) {
