@@ -1418,10 +1418,11 @@ int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *infos, const v
     if ((rc = publish_active())) return rc;
     // Steps run in batches without a host read in between: the batch is the fewest steps any
     // active instance's last observed convergence factor predicts it still needs, less one (at
-    // least one, at most 8), so the host waits on the device about twice per solve rather than once
-    // per step, and no instance runs steps past its own convergence while slower ones go on (each
-    // check lets the instances under tol stop). The policy is the same with and without graphs, so
-    // both give the same steps and bits.
+    // least one, at most 16 for one instance and 8 for several), so the host waits on the device a
+    // few times per solve rather than once per step, and no instance runs steps past its own
+    // convergence while slower ones go on (each check lets the instances under tol stop). The
+    // policy is the same with and without graphs, so both give the same steps and bits.
+    const double cap = K == 1 ? 16.0 : 8.0;
     int steps = 0;
     int status = CG_OK;
     while (left > 0 && steps < kRefinedMostSteps) {
@@ -1435,7 +1436,7 @@ int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *infos, const v
             int want = 1;
             if (rate[i] > 0.0 && rate[i] < 0.95 && res[i] > 0.0) {
                 const double need = std::ceil(std::log(tol / res[i]) / std::log(rate[i]));
-                want = (int)std::max(1.0, std::min(8.0, need - 1.0));
+                want = (int)std::max(1.0, std::min(cap, need - 1.0));
             }
             batch = batch == 0 || want < batch ? want : batch;
         }
