@@ -19,6 +19,7 @@ LIB_PATH = PKG / "libcgocean.so"
 CG_OK, CG_DECLINED, CG_CANCELLED = 0, 1, 2
 CG_ERR_ARG, CG_ERR_DEVICE, CG_ERR_NOMEM, CG_ERR_STATE = -1, -2, -3, -4
 CG_METHOD_REFERENCE, CG_METHOD_RESCUE, CG_METHOD_REFINED, CG_METHOD_FGMRES, CG_METHOD_KRYLOV = 0, 1, 2, 3, 4
+CG_METHOD_LEARNED = 5
 CG_OPT_MIN_CELLS, CG_OPT_TILE_ROWS, CG_OPT_USE_GRAPH = 1, 2, 3
 CG_OPT_FGMRES_RESTART, CG_OPT_TIME_KERNELS = 4, 6
 CG_OPT_FUSED, CG_OPT_LEVEL_PASSES, CG_OPT_FUSE_OUTER, CG_OPT_DIRECT_BOTTOM = 7, 8, 9, 10
@@ -35,6 +36,9 @@ _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _vp = ctypes.c_void_p
 _int, _dbl, _long, _u64 = ctypes.c_int, ctypes.c_double, ctypes.c_long, ctypes.c_uint64
+_flt, _size = ctypes.c_float, ctypes.c_size_t
+# int (*cg_precond_fn)(void *user, const float *v, float *z, void *stream)
+PRECOND_FN = ctypes.CFUNCTYPE(_int, _vp, _vp, _vp, _vp)
 
 # Every entry point of include/cgocean.h with its ctypes signature: (restype, argtypes).
 SIGNATURES = {
@@ -71,6 +75,17 @@ SIGNATURES = {
     "cg_ocean_curl_forcing": (_int, [_vp, _int, _int, _dbl, _dbl, _f64p, _f64p, _vp, _f64p]),
     "cg_ocean_plan_set_stress": (_int, [_vp, _dbl, _dbl, _f64p, _f64p, _vp]),
     "cg_ocean_plan_velocities": (_int, [_vp, _dbl, _dbl, _vp, _vp]),
+    "cg_ocean_plan_set_preconditioner": (_int, [_vp, PRECOND_FN, _vp]),
+    "cg_conv3x3": (_int, [_vp, _int, _vp, _int, _int, _int, _int, _vp, _int, _int, _vp, _int, _vp, _vp, _vp]),
+    "cg_conv3x3_adjoint": (_int, [_vp, _int, _int, _int, _int, _vp, _int, _vp, _int, _vp, _vp, _vp]),
+    "cg_conv3x3_wgrad_slices": (_int, [_int, _int, _int]),
+    "cg_conv3x3_wgrad": (_int, [_vp, _int, _vp, _int, _vp, _int, _int, _int, _int, _vp, _size, _vp, _vp]),
+    "cg_pool2": (_int, [_vp, _int, _int, _int, _int, _flt, _vp, _vp]),
+    "cg_upsample2": (_int, [_vp, _int, _int, _int, _int, _flt, _vp, _vp]),
+    "cg_relu_mask": (_int, [_vp, _vp, _long, _vp]),
+    "cg_add_bf16": (_int, [_vp, _vp, _long, _vp, _vp]),
+    "cg_unet_pack_input": (_int, [_vp, _vp, _flt, _long, _vp, _vp]),
+    "cg_unet_unpack_output": (_int, [_vp, _vp, _flt, _long, _vp, _vp]),
     "cg_bottom_drag": (_dbl, []),
     "cg_beta_rows": (_int, [_int, _dbl, _f64p]),
     "cg_circulation_stencil": (_int, [_int, _int, _dbl, _dbl, _u8p, _f64p, _dbl, _vp, _f32p, _f32p, _f32p,

@@ -366,6 +366,9 @@ struct cg_plan {
     float *kf32 = nullptr;
     std::vector<double> h_part;
     float *fq = nullptr, *fz = nullptr, *fa = nullptr, *fdelta = nullptr;
+    // The caller's preconditioner for CG_METHOD_LEARNED (cg_ocean_plan_set_preconditioner).
+    cg_precond_fn pre_fn = nullptr;
+    void *pre_user = nullptr;
     long vstride = 0;
     double *fpart = nullptr;
     FgmresState *fst = nullptr;
@@ -1505,7 +1508,12 @@ int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *infos, const v
 // forcing the outer kernel writes), delta from the least-squares combination, then the outer
 // binary64 residual. m is the context's restart length for the first step, then the number of
 // iterations the observed rate predicts are still needed (plus one), within 2..restart.
-int fgmres_solve(cg_plan *p, double tol, bool warm, cg_solution *info, const volatile int *cancel) {
+// learned: the first inner iteration of each outer step takes the caller's preconditioner instead
+// of the cycle (it sees the outer residual itself, the kind of input it was trained on; the cycles
+// of the later iterations and the least-squares step build on its direction), until an outer step
+// fails to halve the residual; from then on every iteration is a plain cycle.
+int fgmres_solve(cg_plan *p, double tol, bool warm, cg_solution *info, const volatile int *cancel,
+                 bool learned = false) {
     cg_ctx *ctx = p->ctx;
     hipStream_t s = ctx->stream;
     const Stencil *st = p->lv[0].st;
@@ -1538,6 +1546,7 @@ int fgmres_solve(cg_plan *p, double tol, bool warm, cg_solution *info, const vol
     const int nb = (int)(g2.x * g2.y);
     int cycles = 0, outer = 0;
     double rate = -1.0;
+    bool net_on = learned;
     while (res >= tol && cycles < kRefinedMostSteps) {
         if (cancel && *cancel) {
             info->status = CG_CANCELLED;
@@ -1551,7 +1560,15 @@ int fgmres_solve(cg_plan *p, double tol, bool warm, cg_solution *info, const vol
         for (int j = 0; j < m; ++j) {
             float *qj = p->fq + (long)j * p->vstride;
             float *zj = p->fz + (long)j * p->vstride;
-            if ((rc = vcycle32(p, false, qj, zj))) return rc;
+            if (net_on && j == 0) {
+                const int prc = p->pre_fn(p->pre_user, qj, zj, (void *)s);
+                if (prc != CG_OK) {
+                    set_err(ctx, "the learned preconditioner returned %d", prc);
+                    return prc < 0 ? prc : CG_ERR_STATE;
+                }
+            } else if ((rc = vcycle32(p, false, qj, zj))) {
+                return rc;
+            }
             hipLaunchKernelGGL(k_apply_dots, g2, dim3(256), 0, s, st->args(), zj, p->fa, p->fq, p->vstride, j, p->fpart);
             hipLaunchKernelGGL(k_fgmres_dots, dim3(1), dim3(256), 0, s, p->fpart, nb, j, p->fst);
             hipLaunchKernelGGL(k_orth, g2, dim3(256), 0, s, st->across, st->down, p->fa, p->fq, p->vstride, j, p->fst,
@@ -1572,6 +1589,7 @@ int fgmres_solve(cg_plan *p, double tol, bool warm, cg_solution *info, const vol
         res = largest > 0.0 ? r / largest : 0.0;
         if (std::isnan(r)) break;
         if (prev > 0.0 && res > 0.0) rate = std::pow(res / prev, 1.0 / m);
+        if (net_on && !(res <= 0.5 * prev)) net_on = false;  // plain cycles from here on
     }
     info->cycles = cycles;
     info->refinements = outer + 1;
@@ -2034,8 +2052,27 @@ static int plan_solve_locked(cg_plan *p, double tol, int method, int warm, cg_so
     }
     if (method == CG_METHOD_REFINED) return refined_solve(p, tol, warm != 0, info, cancel);
     if (method == CG_METHOD_FGMRES) return fgmres_solve(p, tol, warm != 0, info, cancel);
+    if (method == CG_METHOD_LEARNED) {
+        if (!p->pre_fn) {
+            set_err(ctx, "CG_METHOD_LEARNED needs cg_ocean_plan_set_preconditioner first");
+            return CG_ERR_STATE;
+        }
+        return fgmres_solve(p, tol, warm != 0, info, cancel, true);
+    }
     set_err(ctx, "unknown or unbuilt method %d", method);
     return CG_ERR_ARG;
+}
+
+int cg_ocean_plan_set_preconditioner(cg_plan *p, cg_precond_fn fn, void *user) {
+    if (!p) return CG_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(p->ctx->mu);
+    if (p->K > 1) {
+        set_err(p->ctx, "a learned preconditioner serves a plan of one instance");
+        return CG_ERR_STATE;
+    }
+    p->pre_fn = fn;
+    p->pre_user = fn ? user : nullptr;
+    return CG_OK;
 }
 
 int cg_ocean_plan_solve(cg_plan *p, double tol, int method, int warm, cg_solution *info) {

@@ -1,0 +1,336 @@
+// unet.hip: the learned preconditioner's kernels. A 3x3 convolution on bfloat16 activations with
+// binary32 accumulation on the matrix cores (v_mfma_f32_16x16x16_bf16), its adjoint (the data
+// gradient) and its weight gradient, plus the few elementwise steps a U-Net needs (average pool,
+// nearest upsampling, ReLU gradient mask, sum) and the packing between a solver vector and the
+// network's input and output. Not a reference component: the reference has no learned part.
+//
+// Layout: activations are NHWC bfloat16, act[((b * down + row) * across + col) * channels + c];
+// weights are [9][cout][cin] bfloat16 with tap = 3 * (dr + 1) + (dc + 1). The convolution pads rows
+// with zeros and wraps columns (the solve grid is periodic east-west). A convolution may read its
+// input from two tensors (channels [0, c1) from x1, [c1, c1 + c2) from x2): the U-Net's skip
+// concatenation without a copy. Channel counts are multiples of 16 and `across` a multiple of 16.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "cgocean.h"
+
+namespace {
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ inline uint16_t bf16_of(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;  // NaN stays NaN
+    u += 0x7fffu + ((u >> 16) & 1u);                    // round to nearest, ties to even
+    return (uint16_t)(u >> 16);
+}
+__device__ inline float float_of(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ inline int wrap(int c, int across) { return c < 0 ? c + across : (c >= across ? c - across : c); }
+
+// y[p][o] = act(sum over taps t and input channels i of x[p + off(t)][i] * W(t, o, i)).
+// Forward: W(t, o, i) = w[t][o][i]. ADJ (the data gradient of a forward convolution whose weights
+// are w[9][cin][cout]): W(t, o, i) = w[8 - t][i][o], so y is the gradient with respect to that
+// convolution's input when x is the gradient with respect to its output.
+// One wavefront computes 16 consecutive pixels of one row (the MFMA's M) times 16 * NB output
+// channels (N), summing over K = 9 taps x cin channels in steps of 16. Lane l feeds A[l % 16][4 (l / 16) + j]
+// (pixel l % 16, channels 4 (l / 16) .. + 3: 8 contiguous bytes of NHWC) and B[4 (l / 16) + j][l % 16],
+// and holds D[4 (l / 16) + j][l % 16] afterwards.
+// Output channels [0, o1) go to y1 (o1 channels per pixel), [o1, cout) to y2; with y32 set, all
+// cout channels go there as binary32 instead.
+template <int NB, bool ADJ>
+__global__ __launch_bounds__(256) void k_conv3x3(const uint16_t *__restrict__ x1, int c1, const uint16_t *__restrict__ x2,
+                                                 int c2, int n, int down, int across, const uint16_t *__restrict__ w,
+                                                 int cout, int relu, uint16_t *__restrict__ y1, int o1,
+                                                 uint16_t *__restrict__ y2, float *__restrict__ y32) {
+    const int lane = threadIdx.x & 63;
+    const long tile = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int per_row = across >> 4;
+    if (tile >= (long)n * down * per_row) return;  // the whole wavefront leaves together
+    const long row_id = tile / per_row;             // b * down + row
+    const int row = (int)(row_id % down);
+    const long img = row_id / down;
+    const int c0 = (int)(tile % per_row) * 16;
+    const int co0 = blockIdx.y * 16 * NB;
+    const int cin = c1 + c2;
+    const int m = lane & 15, kq = (lane >> 4) * 4;
+    f4 acc[NB];
+    for (int b = 0; b < NB; ++b) acc[b] = (f4){0.f, 0.f, 0.f, 0.f};
+    for (int tap = 0; tap < 9; ++tap) {
+        const int rr = row + tap / 3 - 1;
+        if (rr < 0 || rr >= down) continue;  // zero rows contribute nothing; uniform per wavefront
+        const long pix = (img * down + rr) * across + wrap(c0 + m + tap % 3 - 1, across);
+        for (int ci0 = 0; ci0 < cin; ci0 += 16) {
+            const s4 a = ci0 < c1 ? *(const s4 *)(x1 + pix * c1 + ci0 + kq)
+                                  : *(const s4 *)(x2 + pix * c2 + (ci0 - c1) + kq);
+            for (int b = 0; b < NB; ++b) {
+                const int co = co0 + b * 16 + m;
+                s4 bv;
+                if (!ADJ) {
+                    bv = *(const s4 *)(w + ((long)tap * cout + co) * cin + ci0 + kq);
+                } else {
+                    const uint16_t *src = w + ((long)(8 - tap) * cin + ci0 + kq) * cout + co;
+                    bv = (s4){(short)src[0], (short)src[cout], (short)src[2 * cout], (short)src[3 * cout]};
+                }
+                acc[b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc[b], 0, 0, 0);
+            }
+        }
+    }
+    const long p0 = row_id * across + c0 + kq;
+    for (int b = 0; b < NB; ++b) {
+        const int co = co0 + b * 16 + m;
+        for (int j = 0; j < 4; ++j) {
+            float v = acc[b][j];
+            if (relu) v = v > 0.f ? v : 0.f;
+            const long p = p0 + j;
+            if (y32)
+                y32[p * cout + co] = v;
+            else if (co < o1)
+                y1[p * o1 + co] = bf16_of(v);
+            else
+                y2[p * (cout - o1) + (co - o1)] = bf16_of(v);
+        }
+    }
+}
+
+// Weight gradient partials: part[s][t][o][i] = sum over the pixels p of slice s of
+// dy[p][o] * x[p + off(t)][i]. One wavefront per (16 output channels, 16 input channels, slice),
+// all nine taps at once (A = dy^T is loaded once per 16 pixels and used for every tap). M = output
+// channels, N = input channels, K = 16 pixels of one row (across is a multiple of 16).
+__global__ __launch_bounds__(256) void k_conv3x3_wgrad(const uint16_t *__restrict__ x1, int c1,
+                                                       const uint16_t *__restrict__ x2, int c2,
+                                                       const uint16_t *__restrict__ dy, int cout, int n, int down,
+                                                       int across, int slices, float *__restrict__ part) {
+    const int lane = threadIdx.x & 63;
+    const int slice = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (slice >= slices) return;
+    const int cin = c1 + c2;
+    const int cit = cin >> 4;
+    const int co0 = (blockIdx.x / cit) * 16, ci0 = (blockIdx.x % cit) * 16;
+    const bool first = ci0 < c1;
+    const uint16_t *xs = first ? x1 : x2;
+    const int cs = first ? c1 : c2, cofs = first ? ci0 : ci0 - c1;
+    const long groups = (long)n * down * across / 16;
+    const long g_lo = groups * slice / slices, g_hi = groups * (slice + 1) / slices;
+    const int m = lane & 15, kq = (lane >> 4) * 4;
+    f4 acc[9];
+    for (int t = 0; t < 9; ++t) acc[t] = (f4){0.f, 0.f, 0.f, 0.f};
+    for (long g = g_lo; g < g_hi; ++g) {
+        const long p0 = g * 16;
+        const long row_id = p0 / across;
+        const int c0 = (int)(p0 - row_id * across);
+        const int row = (int)(row_id % down);
+        const long img = row_id / down;
+        const uint16_t *d = dy + (p0 + kq) * cout + co0 + m;
+        const s4 a = (s4){(short)d[0], (short)d[cout], (short)d[2 * cout], (short)d[3 * cout]};
+        for (int t = 0; t < 9; ++t) {
+            const int rr = row + t / 3 - 1;
+            if (rr < 0 || rr >= down) continue;
+            const long base = (img * down + rr) * across;
+            short v[4];
+            for (int j = 0; j < 4; ++j)
+                v[j] = (short)xs[(base + wrap(c0 + kq + j + t % 3 - 1, across)) * cs + cofs + m];
+            const s4 bv = (s4){v[0], v[1], v[2], v[3]};
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc[t], 0, 0, 0);
+        }
+    }
+    float *dst = part + (long)slice * 9 * cout * cin;
+    for (int t = 0; t < 9; ++t)
+        for (int j = 0; j < 4; ++j) dst[((long)t * cout + co0 + kq + j) * cin + ci0 + m] = acc[t][j];
+}
+
+// dw[i] = sum over slices s of part[s][i], in slice order (deterministic).
+__global__ void k_wgrad_sum(const float *__restrict__ part, int slices, long count, float *__restrict__ dw) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    float s = 0.f;
+    for (int k = 0; k < slices; ++k) s += part[(long)k * count + i];
+    dw[i] = s;
+}
+
+// y (down/2 x across/2) = scale * mean of each 2x2 block of x.
+__global__ void k_pool2(const uint16_t *__restrict__ x, int ch, int n, int down, int across, float scale,
+                        uint16_t *__restrict__ y) {
+    const int hd = down / 2, ha = across / 2;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)n * hd * ha * ch) return;
+    const int c = (int)(i % ch);
+    const long q = i / ch;
+    const int col = (int)(q % ha);
+    const long rid = q / ha;
+    const int row = (int)(rid % hd);
+    const long img = rid / hd;
+    const long top = ((img * down + 2 * row) * across + 2 * col) * ch + c;
+    const long bot = top + (long)across * ch;
+    const float s = float_of(x[top]) + float_of(x[top + ch]) + float_of(x[bot]) + float_of(x[bot + ch]);
+    y[i] = bf16_of(0.25f * scale * s);
+}
+
+// y (2 down x 2 across) = scale * x at (row / 2, col / 2), where down, across are x's.
+__global__ void k_upsample2(const uint16_t *__restrict__ x, int ch, int n, int down, int across, float scale,
+                            uint16_t *__restrict__ y) {
+    const int fd = 2 * down, fa = 2 * across;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)n * fd * fa * ch) return;
+    const int c = (int)(i % ch);
+    const long q = i / ch;
+    const int col = (int)(q % fa);
+    const long rid = q / fa;
+    const int row = (int)(rid % fd);
+    const long img = rid / fd;
+    const uint16_t v = x[((img * down + row / 2) * across + col / 2) * ch + c];
+    y[i] = scale == 1.f ? v : bf16_of(scale * float_of(v));
+}
+
+// dy = y > 0 ? dy : 0 (the gradient through a ReLU whose output was y).
+__global__ void k_relu_mask(uint16_t *__restrict__ dy, const uint16_t *__restrict__ y, long count) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    if (!(float_of(y[i]) > 0.f)) dy[i] = 0;
+}
+
+// out = a + b, rounded once.
+__global__ void k_add_bf16(const uint16_t *__restrict__ a, const uint16_t *__restrict__ b, long count,
+                           uint16_t *__restrict__ out) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    out[i] = bf16_of(float_of(a[i]) + float_of(b[i]));
+}
+
+// The network's input: 16 channels per cell, channels 0..1 copied from `fixed` (mask, depth),
+// channel 2 = scale * v[cell] (the vector to precondition), the rest zero.
+__global__ void k_pack_input(const uint16_t *__restrict__ fixed, const float *__restrict__ v, float scale, long cells,
+                             uint16_t *__restrict__ out) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cells * 16) return;
+    const int c = (int)(i & 15);
+    const long cell = i >> 4;
+    out[i] = c < 2 ? fixed[cell * 2 + c] : (c == 2 ? bf16_of(scale * v[cell]) : (uint16_t)0);
+}
+
+// z[cell] = water ? scale * y32[cell][0] : 0 (channel 0 of the network's binary32 output).
+__global__ void k_unpack_output(const float *__restrict__ y32, const uint8_t *__restrict__ water, float scale,
+                                long cells, float *__restrict__ z) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cells) return;
+    z[i] = water[i] ? scale * y32[i * 16] : 0.f;
+}
+
+inline unsigned blocks(long count, int per) { return (unsigned)((count + per - 1) / per); }
+
+bool conv_shape_ok(int c1, int c2, int n, int down, int across, int cout) {
+    return c1 > 0 && c1 % 16 == 0 && c2 >= 0 && c2 % 16 == 0 && n > 0 && down > 0 && across > 0 &&
+           across % 16 == 0 && cout > 0 && cout % 16 == 0;
+}
+
+template <bool ADJ>
+int conv_launch(const uint16_t *x1, int c1, const uint16_t *x2, int c2, int n, int down, int across,
+                const uint16_t *w, int cout, int relu, uint16_t *y1, int o1, uint16_t *y2, float *y32, hipStream_t s) {
+    const long tiles = (long)n * down * (across / 16);
+    const int nb = cout % 64 == 0 ? 4 : (cout % 32 == 0 ? 2 : 1);
+    const dim3 grid(blocks(tiles, 4), cout / (16 * nb));
+    if (nb == 4)
+        hipLaunchKernelGGL((k_conv3x3<4, ADJ>), grid, dim3(256), 0, s, x1, c1, x2, c2, n, down, across, w, cout, relu,
+                           y1, o1, y2, y32);
+    else if (nb == 2)
+        hipLaunchKernelGGL((k_conv3x3<2, ADJ>), grid, dim3(256), 0, s, x1, c1, x2, c2, n, down, across, w, cout, relu,
+                           y1, o1, y2, y32);
+    else
+        hipLaunchKernelGGL((k_conv3x3<1, ADJ>), grid, dim3(256), 0, s, x1, c1, x2, c2, n, down, across, w, cout, relu,
+                           y1, o1, y2, y32);
+    return hipGetLastError() == hipSuccess ? CG_OK : CG_ERR_DEVICE;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cg_conv3x3(const uint16_t *x1, int c1, const uint16_t *x2, int c2, int n, int down, int across,
+               const uint16_t *w, int cout, int relu, uint16_t *y1, int o1, uint16_t *y2, float *y32,
+               void *stream) {
+    if (!conv_shape_ok(c1, c2, n, down, across, cout) || !x1 || (c2 && !x2) || !w) return CG_ERR_ARG;
+    if (!y32 && (!y1 || o1 < 0 || o1 > cout || o1 % 16 || (o1 < cout && !y2))) return CG_ERR_ARG;
+    return conv_launch<false>(x1, c1, c2 ? x2 : nullptr, c2, n, down, across, w, cout, relu, y1, y32 ? cout : o1, y2,
+                              y32, (hipStream_t)stream);
+}
+
+int cg_conv3x3_adjoint(const uint16_t *dy, int cout, int n, int down, int across, const uint16_t *w, int cin,
+                       uint16_t *dx1, int c1, uint16_t *dx2, float *dx32, void *stream) {
+    if (!conv_shape_ok(cout, 0, n, down, across, cin) || !dy || !w) return CG_ERR_ARG;
+    if (!dx32 && (!dx1 || c1 < 0 || c1 > cin || c1 % 16 || (c1 < cin && !dx2))) return CG_ERR_ARG;
+    return conv_launch<true>(dy, cout, nullptr, 0, n, down, across, w, cin, 0, dx1, dx32 ? cin : c1, dx2, dx32,
+                             (hipStream_t)stream);
+}
+
+int cg_conv3x3_wgrad_slices(int n, int down, int across) {
+    const long groups = (long)n * down * across / 16;
+    long s = groups / 32;
+    if (s > 256) s = 256;
+    s = (s / 4) * 4;
+    return s < 4 ? 4 : (int)s;
+}
+
+int cg_conv3x3_wgrad(const uint16_t *x1, int c1, const uint16_t *x2, int c2, const uint16_t *dy, int cout, int n,
+                     int down, int across, float *work, size_t work_floats, float *dw, void *stream) {
+    if (!conv_shape_ok(c1, c2, n, down, across, cout) || !x1 || (c2 && !x2) || !dy || !work || !dw)
+        return CG_ERR_ARG;
+    const int slices = cg_conv3x3_wgrad_slices(n, down, across);
+    const long count = 9L * cout * (c1 + c2);
+    if (work_floats < (size_t)slices * (size_t)count) return CG_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_conv3x3_wgrad, dim3((cout / 16) * ((c1 + c2) / 16), slices / 4), dim3(256), 0, s, x1, c1,
+                       c2 ? x2 : nullptr, c2, dy, cout, n, down, across, slices, work);
+    hipLaunchKernelGGL(k_wgrad_sum, dim3(blocks(count, 256)), dim3(256), 0, s, work, slices, count, dw);
+    return hipGetLastError() == hipSuccess ? CG_OK : CG_ERR_DEVICE;
+}
+
+int cg_pool2(const uint16_t *x, int channels, int n, int down, int across, float scale, uint16_t *y, void *stream) {
+    if (!x || !y || channels <= 0 || n <= 0 || down <= 0 || across <= 0 || down % 2 || across % 2) return CG_ERR_ARG;
+    const long count = (long)n * (down / 2) * (across / 2) * channels;
+    hipLaunchKernelGGL(k_pool2, dim3(blocks(count, 256)), dim3(256), 0, (hipStream_t)stream, x, channels, n, down,
+                       across, scale, y);
+    return hipGetLastError() == hipSuccess ? CG_OK : CG_ERR_DEVICE;
+}
+
+int cg_upsample2(const uint16_t *x, int channels, int n, int down, int across, float scale, uint16_t *y,
+                 void *stream) {
+    if (!x || !y || channels <= 0 || n <= 0 || down <= 0 || across <= 0) return CG_ERR_ARG;
+    const long count = (long)n * (2 * down) * (2 * across) * channels;
+    hipLaunchKernelGGL(k_upsample2, dim3(blocks(count, 256)), dim3(256), 0, (hipStream_t)stream, x, channels, n, down,
+                       across, scale, y);
+    return hipGetLastError() == hipSuccess ? CG_OK : CG_ERR_DEVICE;
+}
+
+int cg_relu_mask(uint16_t *dy, const uint16_t *y, long count, void *stream) {
+    if (!dy || !y || count < 0) return CG_ERR_ARG;
+    if (count == 0) return CG_OK;
+    hipLaunchKernelGGL(k_relu_mask, dim3(blocks(count, 256)), dim3(256), 0, (hipStream_t)stream, dy, y, count);
+    return hipGetLastError() == hipSuccess ? CG_OK : CG_ERR_DEVICE;
+}
+
+int cg_add_bf16(const uint16_t *a, const uint16_t *b, long count, uint16_t *out, void *stream) {
+    if (!a || !b || !out || count < 0) return CG_ERR_ARG;
+    if (count == 0) return CG_OK;
+    hipLaunchKernelGGL(k_add_bf16, dim3(blocks(count, 256)), dim3(256), 0, (hipStream_t)stream, a, b, count, out);
+    return hipGetLastError() == hipSuccess ? CG_OK : CG_ERR_DEVICE;
+}
+
+int cg_unet_pack_input(const uint16_t *fixed, const float *v, float scale, long cells, uint16_t *out,
+                       void *stream) {
+    if (!fixed || !v || !out || cells <= 0) return CG_ERR_ARG;
+    hipLaunchKernelGGL(k_pack_input, dim3(blocks(cells * 16, 256)), dim3(256), 0, (hipStream_t)stream, fixed, v,
+                       scale, cells, out);
+    return hipGetLastError() == hipSuccess ? CG_OK : CG_ERR_DEVICE;
+}
+
+int cg_unet_unpack_output(const float *y32, const uint8_t *water, float scale, long cells, float *z,
+                          void *stream) {
+    if (!y32 || !water || !z || cells <= 0) return CG_ERR_ARG;
+    hipLaunchKernelGGL(k_unpack_output, dim3(blocks(cells, 256)), dim3(256), 0, (hipStream_t)stream, y32, water,
+                       scale, cells, z);
+    return hipGetLastError() == hipSuccess ? CG_OK : CG_ERR_DEVICE;
+}
+
+}  // extern "C"

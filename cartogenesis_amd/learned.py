@@ -1,0 +1,370 @@
+"""The learned preconditioner: a 4-level U-Net of 3x3 convolutions (32 base channels) that maps
+(mask, depth, residual) to a correction, used inside flexible GMRES (CG_METHOD_LEARNED).
+
+Every convolution, forward and backward, and every pool, upsample and ReLU gradient runs in the
+library's kernels (cg_conv3x3, cg_conv3x3_adjoint, cg_conv3x3_wgrad, ...): bfloat16 activations,
+binary32 accumulation on the matrix cores. torch holds the device memory, the loss and the Adam
+update. Not a reference component: the reference's solve has no learned part; its training pairs
+come from this library's own multigrid solves of the synthetic worlds.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import CG_METHOD_REFINED, PRECOND_FN, check, lib
+
+BF16 = torch.bfloat16
+DEPTH_SCALE = 6000.0  # metres: depth enters the network as H / 6000
+
+
+def _p(t: Optional[torch.Tensor]):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def layer_table(base: int = 32) -> List[Tuple[str, int, int, int, int]]:
+    """(name, channels from the first input, channels from the skip input, output channels, relu)."""
+    b1, b2, b4, b8 = base, 2 * base, 4 * base, 8 * base
+    return [
+        ("enc0a", 16, 0, b1, 1), ("enc0b", b1, 0, b1, 1),
+        ("enc1a", b1, 0, b2, 1), ("enc1b", b2, 0, b2, 1),
+        ("enc2a", b2, 0, b4, 1), ("enc2b", b4, 0, b4, 1),
+        ("mid_a", b4, 0, b8, 1), ("mid_b", b8, 0, b8, 1),
+        ("dec2a", b8, b4, b4, 1), ("dec2b", b4, 0, b4, 1),
+        ("dec1a", b4, b2, b2, 1), ("dec1b", b2, 0, b2, 1),
+        ("dec0a", b2, b1, b1, 1), ("dec0b", b1, 0, b1, 1),
+        ("out", b1, 0, 16, 0),
+    ]
+
+
+class UNet:
+    """Weights in binary32 (the trained copy) and bfloat16 (what the kernels read), layout
+    [9][cout][cin]. Inputs are NHWC bfloat16 with 16 channels; the output is NHWC binary32 with 16
+    channels of which channel 0 is the correction."""
+
+    def __init__(self, base: int = 32, seed: int = 0, device: str = "cuda"):
+        self.base = base
+        self.device = device
+        self.layers = layer_table(base)
+        self.spec = {name: (c1, c2, co, relu) for name, c1, c2, co, relu in self.layers}
+        g = torch.Generator().manual_seed(seed)
+        self.w32: Dict[str, torch.Tensor] = {}
+        for name, c1, c2, co, _ in self.layers:
+            cin = c1 + c2
+            std = math.sqrt((1.0 if name == "out" else 2.0) / (9 * cin))
+            self.w32[name] = (torch.randn(9, co, cin, generator=g) * std).to(device).requires_grad_(False)
+        self.wb = {k: v.to(BF16).contiguous() for k, v in self.w32.items()}
+
+    def refresh(self) -> None:
+        """Rounds the binary32 weights into the bfloat16 copies the kernels read."""
+        for k, v in self.w32.items():
+            self.wb[k].copy_(v)
+
+    def state(self) -> Dict[str, np.ndarray]:
+        return {k: v.detach().cpu().numpy() for k, v in self.w32.items()}
+
+    def load(self, state: Dict[str, np.ndarray]) -> None:
+        for k, v in state.items():
+            self.w32[k].copy_(torch.from_numpy(np.asarray(v, np.float32)))
+        self.refresh()
+
+    # ---- forward -------------------------------------------------------------------------------
+    def _conv(self, name, a1, a2, n, d, a, stream, out32=False):
+        c1, c2, co, relu = self.spec[name]
+        if out32:
+            y = torch.empty((n, d, a, co), dtype=torch.float32, device=self.device)
+            check(lib().cg_conv3x3(_p(a1), c1, _p(a2), c2, n, d, a, _p(self.wb[name]), co, relu, None, 0, None,
+                                   _p(y), stream), what=f"cg_conv3x3 {name}")
+        else:
+            y = torch.empty((n, d, a, co), dtype=BF16, device=self.device)
+            check(lib().cg_conv3x3(_p(a1), c1, _p(a2), c2, n, d, a, _p(self.wb[name]), co, relu, _p(y), co, None,
+                                   None, stream), what=f"cg_conv3x3 {name}")
+        return y
+
+    def _pool(self, x, n, d, a, stream, scale=1.0):
+        ch = x.shape[-1]
+        y = torch.empty((n, d // 2, a // 2, ch), dtype=BF16, device=self.device)
+        check(lib().cg_pool2(_p(x), ch, n, d, a, float(scale), _p(y), stream), what="cg_pool2")
+        return y
+
+    def _up(self, x, n, d, a, stream, scale=1.0):
+        """x is d x a; the result 2d x 2a."""
+        ch = x.shape[-1]
+        y = torch.empty((n, 2 * d, 2 * a, ch), dtype=BF16, device=self.device)
+        check(lib().cg_upsample2(_p(x), ch, n, d, a, float(scale), _p(y), stream), what="cg_upsample2")
+        return y
+
+    def forward(self, x: torch.Tensor, stream=None, keep: bool = False):
+        """x: (n, down, across, 16) bfloat16; down a multiple of 8 and across of 128. Returns the
+        (n, down, across, 16) binary32 output, and with keep the activations for backward()."""
+        n, d, a, _ = x.shape
+        if d % 8 or a % 128:
+            raise ValueError("the U-Net needs down % 8 == 0 and across % 128 == 0")
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        t = {"x": x}
+        dims = [(d >> k, a >> k) for k in range(4)]
+        t["enc0a"] = self._conv("enc0a", x, None, n, *dims[0], s)
+        t["enc0b"] = self._conv("enc0b", t["enc0a"], None, n, *dims[0], s)
+        t["p1"] = self._pool(t["enc0b"], n, *dims[0], s)
+        t["enc1a"] = self._conv("enc1a", t["p1"], None, n, *dims[1], s)
+        t["enc1b"] = self._conv("enc1b", t["enc1a"], None, n, *dims[1], s)
+        t["p2"] = self._pool(t["enc1b"], n, *dims[1], s)
+        t["enc2a"] = self._conv("enc2a", t["p2"], None, n, *dims[2], s)
+        t["enc2b"] = self._conv("enc2b", t["enc2a"], None, n, *dims[2], s)
+        t["p3"] = self._pool(t["enc2b"], n, *dims[2], s)
+        t["mid_a"] = self._conv("mid_a", t["p3"], None, n, *dims[3], s)
+        t["mid_b"] = self._conv("mid_b", t["mid_a"], None, n, *dims[3], s)
+        t["u2"] = self._up(t["mid_b"], n, *dims[3], s)
+        t["dec2a"] = self._conv("dec2a", t["u2"], t["enc2b"], n, *dims[2], s)
+        t["dec2b"] = self._conv("dec2b", t["dec2a"], None, n, *dims[2], s)
+        t["u1"] = self._up(t["dec2b"], n, *dims[2], s)
+        t["dec1a"] = self._conv("dec1a", t["u1"], t["enc1b"], n, *dims[1], s)
+        t["dec1b"] = self._conv("dec1b", t["dec1a"], None, n, *dims[1], s)
+        t["u0"] = self._up(t["dec1b"], n, *dims[1], s)
+        t["dec0a"] = self._conv("dec0a", t["u0"], t["enc0b"], n, *dims[0], s)
+        t["dec0b"] = self._conv("dec0b", t["dec0a"], None, n, *dims[0], s)
+        out = self._conv("out", t["dec0b"], None, n, *dims[0], s, out32=True)
+        return (out, t) if keep else out
+
+    # ---- backward ------------------------------------------------------------------------------
+    def workspace_floats(self, n: int, down: int, across: int) -> int:
+        most = 0
+        for k, (name, c1, c2, co, _) in enumerate(self.layers):
+            lvl = {"enc0": 0, "enc1": 1, "enc2": 2, "mid_": 3, "dec2": 2, "dec1": 1, "dec0": 0, "out": 0}[name[:4]]
+            d, a = down >> lvl, across >> lvl
+            most = max(most, lib().cg_conv3x3_wgrad_slices(n, d, a) * 9 * co * (c1 + c2))
+        return most
+
+    def backward(self, dout: torch.Tensor, t: Dict[str, torch.Tensor], work: torch.Tensor,
+                 stream=None) -> Dict[str, torch.Tensor]:
+        """Weight gradients (binary32, [9][cout][cin]) of sum(dout * out) for the activations t of
+        forward(keep=True); dout is the (n, down, across, 16) binary32 gradient of the output."""
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        n, d, a, _ = dout.shape
+        dims = [(d >> k, a >> k) for k in range(4)]
+        grads: Dict[str, torch.Tensor] = {}
+        L = lib()
+
+        def wgrad(name, a1, a2, g, lvl):
+            c1, c2, co, _ = self.spec[name]
+            dw = torch.empty((9, co, c1 + c2), dtype=torch.float32, device=self.device)
+            check(L.cg_conv3x3_wgrad(_p(a1), c1, _p(a2), c2, _p(g), co, n, *dims[lvl], _p(work), work.numel(),
+                                     _p(dw), s), what=f"cg_conv3x3_wgrad {name}")
+            grads[name] = dw
+
+        def adjoint(name, g, lvl):
+            c1, c2, co, _ = self.spec[name]
+            g1 = torch.empty((n, *dims[lvl], c1), dtype=BF16, device=self.device)
+            g2 = torch.empty((n, *dims[lvl], c2), dtype=BF16, device=self.device) if c2 else None
+            check(L.cg_conv3x3_adjoint(_p(g), co, n, *dims[lvl], _p(self.wb[name]), c1 + c2, _p(g1), c1, _p(g2),
+                                       None, s), what=f"cg_conv3x3_adjoint {name}")
+            return g1, g2
+
+        def relu_back(g, y):
+            check(L.cg_relu_mask(_p(g), _p(y), g.numel(), s), what="cg_relu_mask")
+            return g
+
+        def add(x, y):
+            check(L.cg_add_bf16(_p(x), _p(y), x.numel(), _p(x), s), what="cg_add_bf16")
+            return x
+
+        g = dout.to(BF16).contiguous()
+        wgrad("out", t["dec0b"], None, g, 0)
+        g, _ = adjoint("out", g, 0)
+        # decoder, level 0
+        g = relu_back(g, t["dec0b"])
+        wgrad("dec0b", t["dec0a"], None, g, 0)
+        g, _ = adjoint("dec0b", g, 0)
+        g = relu_back(g, t["dec0a"])
+        wgrad("dec0a", t["u0"], t["enc0b"], g, 0)
+        g_u0, g_e0 = adjoint("dec0a", g, 0)
+        g = self._pool(g_u0, n, *dims[0], s, scale=4.0)  # the adjoint of nearest upsampling: 2x2 sums
+        # level 1
+        g = relu_back(g, t["dec1b"])
+        wgrad("dec1b", t["dec1a"], None, g, 1)
+        g, _ = adjoint("dec1b", g, 1)
+        g = relu_back(g, t["dec1a"])
+        wgrad("dec1a", t["u1"], t["enc1b"], g, 1)
+        g_u1, g_e1 = adjoint("dec1a", g, 1)
+        g = self._pool(g_u1, n, *dims[1], s, scale=4.0)
+        # level 2
+        g = relu_back(g, t["dec2b"])
+        wgrad("dec2b", t["dec2a"], None, g, 2)
+        g, _ = adjoint("dec2b", g, 2)
+        g = relu_back(g, t["dec2a"])
+        wgrad("dec2a", t["u2"], t["enc2b"], g, 2)
+        g_u2, g_e2 = adjoint("dec2a", g, 2)
+        g = self._pool(g_u2, n, *dims[2], s, scale=4.0)
+        # bottleneck
+        g = relu_back(g, t["mid_b"])
+        wgrad("mid_b", t["mid_a"], None, g, 3)
+        g, _ = adjoint("mid_b", g, 3)
+        g = relu_back(g, t["mid_a"])
+        wgrad("mid_a", t["p3"], None, g, 3)
+        g, _ = adjoint("mid_a", g, 3)
+        # encoder: the pool's adjoint spreads a quarter to each cell of the block
+        g = add(self._up(g, n, *dims[3], s, scale=0.25), g_e2)
+        g = relu_back(g, t["enc2b"])
+        wgrad("enc2b", t["enc2a"], None, g, 2)
+        g, _ = adjoint("enc2b", g, 2)
+        g = relu_back(g, t["enc2a"])
+        wgrad("enc2a", t["p2"], None, g, 2)
+        g, _ = adjoint("enc2a", g, 2)
+        g = add(self._up(g, n, *dims[2], s, scale=0.25), g_e1)
+        g = relu_back(g, t["enc1b"])
+        wgrad("enc1b", t["enc1a"], None, g, 1)
+        g, _ = adjoint("enc1b", g, 1)
+        g = relu_back(g, t["enc1a"])
+        wgrad("enc1a", t["p1"], None, g, 1)
+        g, _ = adjoint("enc1a", g, 1)
+        g = add(self._up(g, n, *dims[1], s, scale=0.25), g_e0)
+        g = relu_back(g, t["enc0b"])
+        wgrad("enc0b", t["enc0a"], None, g, 0)
+        g, _ = adjoint("enc0b", g, 0)
+        g = relu_back(g, t["enc0a"])
+        wgrad("enc0a", t["x"], None, g, 0)
+        return grads
+
+
+# ---- training pairs and training -----------------------------------------------------------------
+
+def fixed_channels(water: np.ndarray, depth: np.ndarray) -> np.ndarray:
+    """(cells, 2) binary32: the mask and the scaled depth, the network's two constant inputs."""
+    return np.stack([(np.asarray(water) != 0).astype(np.float32),
+                     np.asarray(depth, np.float32) / DEPTH_SCALE], axis=1)
+
+
+def centre_scale(stencil) -> float:
+    """The mean diagonal weight over water: the factor that makes the network's target O(1)."""
+    w = np.asarray(stencil.is_water) != 0
+    c = (np.asarray(stencil.east_weight, np.float64) + stencil.west_weight + stencil.north_weight
+         + stencil.south_weight)
+    return float(c[w].mean()) if w.any() else 1.0
+
+
+def training_pairs(acc, rows: int, seeds: Sequence[int], months: Sequence[int] = (-1,), tol: float = 1e-8):
+    """(input (P, rows, 2 rows, 16) bfloat16, target (P, rows, 2 rows) binary32, water (P, rows, 2 rows)
+    binary32) on the device, from this library's refined solves of the synthetic worlds: the input
+    holds the mask, the depth and the forcing scaled to unit root mean square; the target is the
+    solution scaled to unit root mean square (flexible GMRES weighs each direction it is given, so
+    only the shape of the correction matters)."""
+    from .ocean import Plan
+    from .problem import circulation_problem, monthly_forcings
+
+    cols = 2 * rows
+    xs, ts, ws = [], [], []
+    for seed in seeds:
+        prob = circulation_problem(rows, seed=int(seed), land_fraction=0.3)
+        forcings = monthly_forcings(prob, months) if tuple(months) != (-1,) else [prob.levels[0].forcing]
+        plan = Plan(acc, prob.levels)
+        fixed = fixed_channels(prob.water, prob.depth)
+        try:
+            for f in forcings:
+                f = np.asarray(f, np.float32)
+                plan.set_forcing(f)
+                info = plan.solve(tol, CG_METHOD_REFINED)
+                if info.relative_residual > tol:
+                    raise RuntimeError(f"training solve of seed {seed} stopped at {info.relative_residual:.2e}")
+                psi, _ = plan.read()
+                scale = math.sqrt(f.size) / max(float(np.linalg.norm(f.astype(np.float64))), 1e-300)
+                x = np.zeros((rows * cols, 16), np.float32)
+                x[:, :2] = fixed
+                x[:, 2] = f * scale
+                xs.append(x.reshape(rows, cols, 16))
+                rms = max(float(np.linalg.norm(psi)) / math.sqrt(psi.size), 1e-300)
+                ts.append((psi / rms).astype(np.float32).reshape(rows, cols))
+                ws.append((prob.water != 0).astype(np.float32).reshape(rows, cols))
+        finally:
+            plan.close()
+    dev = "cuda"
+    return (torch.from_numpy(np.stack(xs)).to(dev).to(BF16), torch.from_numpy(np.stack(ts)).to(dev),
+            torch.from_numpy(np.stack(ws)).to(dev))
+
+
+def loss_and_grad(out: torch.Tensor, target: torch.Tensor, water: torch.Tensor):
+    """Mean squared error of channel 0 over water cells and its gradient with respect to out."""
+    count = water.sum().clamp_min(1.0)
+    diff = (out[..., 0] - target) * water
+    loss = (diff * diff).sum() / count
+    dout = torch.zeros_like(out)
+    dout[..., 0] = 2.0 * diff / count
+    return loss, dout
+
+
+def train(net: UNet, data, steps: int, batch: int = 4, lr: float = 1e-3, seed: int = 0) -> List[float]:
+    """Adam on the mean squared error; returns the loss of every step."""
+    x_all, t_all, w_all = data
+    P = x_all.shape[0]
+    params = [net.w32[k].requires_grad_(True) for k, *_ in net.layers]
+    opt = torch.optim.Adam(params, lr=lr)
+    g = torch.Generator().manual_seed(seed)
+    n = min(batch, P)
+    work = torch.empty(net.workspace_floats(n, x_all.shape[1], x_all.shape[2]), dtype=torch.float32,
+                       device=net.device)
+    losses = []
+    for _ in range(steps):
+        idx = torch.randperm(P, generator=g)[:n].to(x_all.device)
+        x, t, w = x_all[idx].contiguous(), t_all[idx].contiguous(), w_all[idx].contiguous()
+        out, acts = net.forward(x, keep=True)
+        loss, dout = loss_and_grad(out, t, w)
+        grads = net.backward(dout, acts, work)
+        for k, *_ in net.layers:
+            net.w32[k].grad = grads[k]
+        opt.step()
+        with torch.no_grad():
+            net.refresh()
+        losses.append(float(loss))
+    for p in params:
+        p.requires_grad_(False)
+        p.grad = None
+    return losses
+
+
+class LearnedPreconditioner:
+    """The trained network as a plan's preconditioner (cg_ocean_plan_set_preconditioner): z = the
+    network's correction for v scaled to unit root mean square, scaled back, zero on land. Its
+    buffers are allocated here once; each application only enqueues kernels on the plan's stream."""
+
+    def __init__(self, net: UNet, stencil, depth):
+        self.net = net
+        self.down, self.across = stencil.cells_down, stencil.cells_across
+        cells = self.down * self.across
+        self.cells = cells
+        dev = net.device
+        self.fixed = torch.from_numpy(fixed_channels(stencil.is_water, depth)).to(dev).to(BF16).contiguous()
+        self.water = torch.from_numpy((np.asarray(stencil.is_water) != 0).astype(np.uint8)).to(dev)
+        self.x = torch.zeros((1, self.down, self.across, 16), dtype=BF16, device=dev)
+        self.scale_in = math.sqrt(cells)  # FGMRES hands unit-norm vectors: this makes them unit RMS
+        self.scale_out = 1.0 / (centre_scale(stencil) * self.scale_in)  # any scale serves FGMRES
+        self.applications = 0
+        self.error: Optional[BaseException] = None
+        torch.cuda.synchronize()
+        self._cb = PRECOND_FN(self._apply)
+
+    def _apply(self, user, v, z, stream) -> int:
+        try:
+            L = lib()
+            s = ctypes.c_void_p(stream)
+            check(L.cg_unet_pack_input(_p(self.fixed), ctypes.c_void_p(v), self.scale_in, self.cells, _p(self.x), s),
+                  what="cg_unet_pack_input")
+            out = self.net.forward(self.x, stream=s)
+            check(L.cg_unet_unpack_output(_p(out), _p(self.water), self.scale_out, self.cells, ctypes.c_void_p(z), s),
+                  what="cg_unet_unpack_output")
+            self.applications += 1
+            return 0
+        except BaseException as e:  # an exception must not cross the C boundary
+            self.error = e
+            return -4
+
+    def attach(self, plan) -> None:
+        check(lib().cg_ocean_plan_set_preconditioner(plan.handle, self._cb, None), plan.acc.ctx,
+              "cg_ocean_plan_set_preconditioner")
+
+    @staticmethod
+    def detach(plan) -> None:
+        check(lib().cg_ocean_plan_set_preconditioner(plan.handle, PRECOND_FN(), None), plan.acc.ctx,
+              "cg_ocean_plan_set_preconditioner")

@@ -54,6 +54,12 @@ extern "C" {
  * to 3 times while it stops short of tol: the heat's solve (OS:486-497). cg_solution.cycles counts
  * the iterations of every start, refinements the starts. */
 #define CG_METHOD_KRYLOV 4
+/* CG_METHOD_FGMRES with the plan's learned preconditioner (cg_ocean_plan_set_preconditioner) on
+ * the first inner iteration of every outer step (the outer residual itself) and the binary32
+ * V(2,2) cycle on the others; once an outer step fails
+ * to halve the residual, the rest of the solve uses plain cycles only (the multigrid fallback).
+ * cg_solution.cycles counts inner iterations of both kinds. Not a reference method. */
+#define CG_METHOD_LEARNED 5
 
 /* ---- options (cg_ocean_set_option) ------------------------------------------------------------ */
 #define CG_OPT_MIN_CELLS 1       /* relax declines grids with fewer cells (GO:167-172); default 65536 */
@@ -277,6 +283,52 @@ int cg_resample(const float *field, int from_across, int from_down, int to_acros
  * entrainment). */
 int cg_coarse_mean(const float *fine, int fine_across, int fine_down, int coarse_across,
                    int coarse_down, float *coarse);
+
+/* ---- learned preconditioner ------------------------------------------------------------------ */
+/* No counterpart in the reference (its solve has no learned part). A U-Net of 3x3 convolutions on
+ * bfloat16 NHWC activations with binary32 accumulation on the matrix cores. Activations:
+ * act[((b * down + row) * across + col) * channels + c]; weights [9][cout][cin] bfloat16, tap =
+ * 3 * (dr + 1) + (dc + 1); rows are zero-padded and columns wrap. Channel counts are multiples of
+ * 16, across a multiple of 16. Pointers are device pointers; `stream` is a hipStream_t (NULL: the
+ * default stream); every call only enqueues work. */
+
+/* The preconditioner callback of CG_METHOD_LEARNED: z = M(v) for one binary32 vector of the
+ * plan's finest grid (cells values), enqueued on `stream`; returns CG_OK or an error. */
+typedef int (*cg_precond_fn)(void *user, const float *v, float *z, void *stream);
+/* Sets (fn non-NULL) or clears the plan's learned preconditioner. Plans of one instance only. */
+int cg_ocean_plan_set_preconditioner(cg_plan *plan, cg_precond_fn fn, void *user);
+
+/* y = relu?(conv(concat(x1[c1], x2[c2]), w)) with cout output channels: channels [0, o1) into y1
+ * (o1 per pixel) and [o1, cout) into y2; or, with y32 non-NULL, all of them into y32 as binary32. */
+int cg_conv3x3(const uint16_t *x1, int c1, const uint16_t *x2, int c2, int n, int down, int across,
+               const uint16_t *w, int cout, int relu, uint16_t *y1, int o1, uint16_t *y2, float *y32,
+               void *stream);
+/* The data gradient of cg_conv3x3 with weights w[9][cout][cin]: dx = conv^T(dy), split as the
+ * forward's inputs were (channels [0, c1) into dx1, the rest into dx2), or binary32 into dx32. */
+int cg_conv3x3_adjoint(const uint16_t *dy, int cout, int n, int down, int across, const uint16_t *w, int cin,
+                       uint16_t *dx1, int c1, uint16_t *dx2, float *dx32, void *stream);
+/* The weight gradient dw[9][cout][c1 + c2] (binary32) of cg_conv3x3 for input x and output gradient
+ * dy, summed over all pixels in a fixed order. work holds slices x 9 x cout x (c1 + c2) floats,
+ * slices = cg_conv3x3_wgrad_slices(n, down, across). */
+int cg_conv3x3_wgrad_slices(int n, int down, int across);
+int cg_conv3x3_wgrad(const uint16_t *x1, int c1, const uint16_t *x2, int c2, const uint16_t *dy, int cout, int n,
+                     int down, int across, float *work, size_t work_floats, float *dw, void *stream);
+/* y = scale x the mean of each 2x2 block (y is down/2 x across/2). */
+int cg_pool2(const uint16_t *x, int channels, int n, int down, int across, float scale, uint16_t *y, void *stream);
+/* y (2 down x 2 across) = scale x the nearest coarse value. */
+int cg_upsample2(const uint16_t *x, int channels, int n, int down, int across, float scale, uint16_t *y,
+                 void *stream);
+/* dy = 0 wherever y <= 0 (the gradient through a ReLU with output y), in place. */
+int cg_relu_mask(uint16_t *dy, const uint16_t *y, long count, void *stream);
+/* out = a + b elementwise (out may alias a or b). */
+int cg_add_bf16(const uint16_t *a, const uint16_t *b, long count, uint16_t *out, void *stream);
+/* The network input: 16 channels per cell; 0..1 from fixed[cell][2] (mask, scaled depth), channel
+ * 2 = scale x v[cell], the rest zero. */
+int cg_unet_pack_input(const uint16_t *fixed, const float *v, float scale, long cells, uint16_t *out,
+                       void *stream);
+/* z[cell] = water[cell] ? scale x y32[cell * 16] : 0 (channel 0 of the network's output). */
+int cg_unet_unpack_output(const float *y32, const uint8_t *water, float scale, long cells, float *z,
+                          void *stream);
 
 /* Synthetic circulation inputs: a seeded 4-octave value-noise land mask thresholded so that
  * `land_fraction` of cells are land, depth H ~ U(200, 6000) m on water, and the balance
