@@ -12,6 +12,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 from pathlib import Path
@@ -204,6 +205,105 @@ def batched_combined(acc, tol: float):
             "ms_per_size_one_at_a_time": per, "cycles_min_max": cycles, "max_relative_residual": worst}
 
 
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bfloat16 matrix-core peak
+
+
+def unet_flops(rows: int, cols: int, base: int = 32) -> dict:
+    """Multiply-add flops (2 per product) of each U-Net convolution on a rows x cols grid."""
+    from cartogenesis_amd.learned import layer_table
+
+    level = {"enc0": 0, "enc1": 1, "enc2": 2, "mid_": 3, "dec2": 2, "dec1": 1, "dec0": 0, "out": 0}
+    return {name: 2 * 9 * (c1 + c2) * co * (rows >> level[name[:4]]) * (cols >> level[name[:4]])
+            for name, c1, c2, co, _ in layer_table(base)}
+
+
+def learned_leg(acc, prob, tol: float, plain_value: float, rows: int, seeds: int, steps: int, reps: int):
+    """The learned preconditioner: a U-Net trained on the GPU (the library's convolution kernels) on
+    refined solves of `seeds` synthetic worlds of rows x 2 rows, then FGMRES with and without it on
+    the benchmark's 1024x2048 problem, and the network's matrix-core rate at 1024x2048."""
+    import torch
+    import cartogenesis_amd as cg
+    from cartogenesis_amd.learned import LearnedPreconditioner, UNet, train, training_pairs
+    from cartogenesis_amd.ocean import Plan
+
+    log(f"bench: learned: training on {seeds} worlds of {rows}x{2 * rows}, {steps} steps")
+    t0 = time.perf_counter()
+    data = training_pairs(acc, rows, seeds=range(100, 100 + seeds))
+    t_data = time.perf_counter() - t0
+    net = UNet(32, seed=0)
+    t0 = time.perf_counter()
+    losses = train(net, data, steps=steps, batch=4, lr=1e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    del data
+    plan = Plan(acc, prob.levels)
+    out = {"training": {"worlds": seeds, "grid": f"{rows}x{2 * rows}", "steps": steps, "batch": 4,
+                        "data_s": round(t_data, 2), "train_s": round(t_train, 2),
+                        "loss_first": round(losses[0], 4), "loss_last20": round(float(np.mean(losses[-20:])), 4)}}
+    try:
+        def timed(method):
+            info = plan.solve(tol, method)
+            t = time.perf_counter()
+            for _ in range(reps):
+                info = plan.solve(tol, method)
+            return reps / (time.perf_counter() - t), info
+
+        fg_value, fg = timed(cg.CG_METHOD_FGMRES)
+        pre = LearnedPreconditioner(net, prob.levels[0], prob.depth)
+        pre.attach(plan)
+        ln_value, ln = timed(cg.CG_METHOD_LEARNED)
+        LearnedPreconditioner.detach(plan)
+        if pre.error is not None:
+            raise pre.error
+        out |= {"solves_per_s": round(ln_value, 3), "iterations": ln.cycles,
+                "relative_residual": ln.relative_residual,
+                "fgmres_without_network_solves_per_s": round(fg_value, 3), "fgmres_iterations": fg.cycles,
+                "plain_mg_solves_per_s": round(plain_value, 3),
+                "learned_vs_plain_mg": round(ln_value / plain_value, 4) if plain_value > 0 else None}
+        # The network alone at 1024x2048: every convolution, timed with events on torch's stream
+        # (the stream the kernels are launched on when none is given).
+        x = torch.zeros((1, ROWS, COLS, 16), dtype=torch.bfloat16, device="cuda")
+        x[..., 0] = 1.0
+        x[..., 2] = torch.randn(ROWS, COLS, device="cuda").to(torch.bfloat16)
+        net.forward(x)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(5):
+            net.forward(x)
+        e1.record()
+        torch.cuda.synchronize()
+        fwd_ms = e0.elapsed_time(e1) / 5
+        flops = unet_flops(ROWS, COLS)
+        total = sum(flops.values())
+        # the largest single convolution: dec0a (96 -> 32 channels on the full grid)
+        a1 = torch.zeros((1, ROWS, COLS, 64), dtype=torch.bfloat16, device="cuda")
+        a2 = torch.zeros((1, ROWS, COLS, 32), dtype=torch.bfloat16, device="cuda")
+        a1.normal_()
+        a2.normal_()
+        net._conv("dec0a", a1, a2, 1, ROWS, COLS, torch.cuda.current_stream().cuda_stream)
+        e0.record()
+        for _ in range(20):
+            net._conv("dec0a", a1, a2, 1, ROWS, COLS, torch.cuda.current_stream().cuda_stream)
+        e1.record()
+        torch.cuda.synchronize()
+        conv_us = e0.elapsed_time(e1) / 20 * 1000.0
+        conv_tf = flops["dec0a"] / (conv_us * 1e-6) / 1e12
+        out["mfma_roofline"] = {
+            "bound": "mfma", "kernel": "k_conv3x3<2,false> (dec0a: 96 -> 32 channels, 1024x2048, bf16 in, fp32 "
+                                      "accumulate)",
+            "achieved": round(conv_tf, 2), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(conv_tf / BF16_PEAK_TFLOPS, 4), "avg_launch_us": round(conv_us, 2),
+            "flops_per_launch": flops["dec0a"],
+            "network_forward_ms": round(fwd_ms, 3), "network_flops": total,
+            "network_tflops": round(total / (fwd_ms * 1e-3) / 1e12, 2)}
+        log(f"bench: learned {ln_value:.1f} solves/s ({ln.cycles} iterations) against FGMRES {fg_value:.1f} "
+            f"({fg.cycles}) and plain MG {plain_value:.1f}; U-Net forward {fwd_ms:.2f} ms "
+            f"({total / (fwd_ms * 1e-3) / 1e12:.1f} TFLOP/s), dec0a {conv_us:.1f} us ({conv_tf:.1f} TFLOP/s)")
+    finally:
+        plan.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -230,6 +330,12 @@ def main() -> int:
     ap.add_argument("--combined", action="store_true",
                     help="also time 64 instances: 16 monthly forcings each at 256x512, 512x1024, 1024x2048 and "
                          "2048x4096, one plan of 16 per size")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned preconditioner's leg")
+    ap.add_argument("--learned-only", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--plain-value", type=float, default=0.0, help=argparse.SUPPRESS)
+    ap.add_argument("--learned-rows", type=int, default=256, help="training grid rows (columns = 2 rows)")
+    ap.add_argument("--learned-seeds", type=int, default=16, help="training worlds (seeds 100, 101, ...)")
+    ap.add_argument("--learned-steps", type=int, default=300, help="training steps (batch 4)")
     ap.add_argument("--worlds", action="store_true",
                     help="also time the four-world year: 12 months on each of 512x1024, 768x1536, 1024x2048 and "
                          "1024x2048 (seeds 1-4, land 20/27/33/40%%)")
@@ -254,6 +360,23 @@ def main() -> int:
             c.copy_(b)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes_per_copy": n * 4, "copies": 6}))
+        return 0
+
+    if args.learned_only:
+        import torch  # before the library: its HIP runtime is the process's (see the learned leg in main)
+
+        torch.cuda.init()
+        from cartogenesis_amd.ocean import HipOcean
+        from cartogenesis_amd.problem import circulation_problem
+
+        prob = circulation_problem(ROWS, COLS, seed=0, land_fraction=0.3)
+        acc = HipOcean.create_or_null(0).accelerator
+        try:
+            out = learned_leg(acc, prob, args.tol, args.plain_value, args.learned_rows, args.learned_seeds,
+                              args.learned_steps, reps=5)
+        finally:
+            acc.close()
+        print(json.dumps(out), flush=True)
         return 0
 
     import cartogenesis_amd as cg
@@ -320,6 +443,18 @@ def main() -> int:
     batched = None
     if args.batch_reps > 0:
         batched = batched_year(acc, prob, args.tol, args.batch_reps, args.warmup, not args.no_autotune)
+    learned = None
+    if not args.no_learned:
+        # In a child process: torch's bundled HIP runtime has the soname the library links, and a
+        # process holds one runtime (the first loaded), so the child imports torch before the library.
+        log("bench: learned preconditioner leg (child process)")
+        cmd = [sys.executable, os.path.abspath(__file__), "--learned-only", "--tol", str(args.tol),
+               "--plain-value", repr(value), "--learned-rows", str(args.learned_rows),
+               "--learned-seeds", str(args.learned_seeds), "--learned-steps", str(args.learned_steps)]
+        proc = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=600)
+        if proc.returncode != 0:
+            raise RuntimeError(f"the learned leg failed with status {proc.returncode}")
+        learned = json.loads(proc.stdout.strip().splitlines()[-1])
     worlds = batched_worlds(acc, args.tol) if args.worlds else None
     combined = batched_combined(acc, args.tol) if args.combined else None
 
@@ -355,7 +490,7 @@ def main() -> int:
         "config": {
             "workload": "circulation 1024x2048 (seed 0, 30% land, annual forcing) from zero to 1e-8: plain GPU "
                         "multigrid, binary32 V(2,2) red-black cycles of the reference ladder inside binary64 "
-                        "iterative refinement; learned preconditioner not built",
+                        "iterative refinement (the learned preconditioner's leg is reported under \"learned\")",
             "rows": ROWS, "cols": COLS, "tol": args.tol, "method": args.method,
             "graph": not args.no_graph, "tile_rows": args.tile_rows, "level_passes": args.level_passes,
             "autotuned_tile_rows": tuned,
@@ -407,6 +542,8 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
     }
+    if learned:
+        out["learned"] = learned
     if batched:
         out["batched"] = batched
     if worlds:

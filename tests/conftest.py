@@ -9,6 +9,14 @@ ROOT = Path(__file__).resolve().parent.parent
 if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
 
+# torch bundles its own HIP runtime under the soname the library links (libamdhip64.so.7); one
+# runtime serves a process, the first loaded. The learned preconditioner's tests use torch's, so
+# torch is loaded before the library is: every test of the session then runs on that one runtime.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) and the built HIP library")
