@@ -95,6 +95,71 @@ __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t *__restrict__ x1
     }
 }
 
+// The forward convolution when every input channel count is a multiple of 32: gfx950's
+// v_mfma_f32_16x16x32_bf16 (K = 32 per instruction, 16-byte operand loads) and MT tiles of 16
+// pixels per wavefront (16 MT consecutive pixels of one row), so each B operand (weights) serves MT
+// instructions and each A operand (pixels) NB. Lane l feeds A[l % 16][8 (l / 16) + j] and
+// B[8 (l / 16) + j][l % 16], j = 0..7, and holds D[4 (l / 16) + j][l % 16] as in k_conv3x3.
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+
+template <int MT, int NB>
+__global__ __launch_bounds__(256) void k_conv3x3_k32(const uint16_t *__restrict__ x1, int c1,
+                                                     const uint16_t *__restrict__ x2, int c2, int n, int down,
+                                                     int across, const uint16_t *__restrict__ w, int cout, int relu,
+                                                     uint16_t *__restrict__ y1, int o1, uint16_t *__restrict__ y2,
+                                                     float *__restrict__ y32) {
+    const int lane = threadIdx.x & 63;
+    const long tile = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int per_row = across / (16 * MT);
+    if (tile >= (long)n * down * per_row) return;  // the whole wavefront leaves together
+    const long row_id = tile / per_row;
+    const int row = (int)(row_id % down);
+    const long img = row_id / down;
+    const int c0 = (int)(tile % per_row) * 16 * MT;
+    const int co0 = blockIdx.y * 16 * NB;
+    const int cin = c1 + c2;
+    const int m = lane & 15, k8 = (lane >> 4) * 8, kq = (lane >> 4) * 4;
+    f4 acc[MT][NB];
+    for (int t = 0; t < MT; ++t)
+        for (int b = 0; b < NB; ++b) acc[t][b] = (f4){0.f, 0.f, 0.f, 0.f};
+    for (int tap = 0; tap < 9; ++tap) {
+        const int rr = row + tap / 3 - 1;
+        if (rr < 0 || rr >= down) continue;
+        const long rbase = (img * down + rr) * across;
+        long pix[MT];
+        for (int t = 0; t < MT; ++t) pix[t] = rbase + wrap(c0 + 16 * t + m + tap % 3 - 1, across);
+        const uint16_t *wt = w + ((long)tap * cout + co0 + m) * cin + k8;
+        for (int ci0 = 0; ci0 < cin; ci0 += 32) {
+            const bool first = ci0 < c1;
+            const uint16_t *xs = first ? x1 + ci0 + k8 : x2 + (ci0 - c1) + k8;
+            const int cs = first ? c1 : c2;
+            b8 a[MT], bv[NB];
+            for (int t = 0; t < MT; ++t) a[t] = *(const b8 *)(xs + pix[t] * cs);
+            for (int b = 0; b < NB; ++b) bv[b] = *(const b8 *)(wt + (long)b * 16 * cin + ci0);
+            for (int t = 0; t < MT; ++t)
+                for (int b = 0; b < NB; ++b)
+                    acc[t][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], bv[b], acc[t][b], 0, 0, 0);
+        }
+    }
+    for (int t = 0; t < MT; ++t) {
+        const long p0 = row_id * across + c0 + 16 * t + kq;
+        for (int b = 0; b < NB; ++b) {
+            const int co = co0 + b * 16 + m;
+            for (int j = 0; j < 4; ++j) {
+                float v = acc[t][b][j];
+                if (relu) v = v > 0.f ? v : 0.f;
+                const long p = p0 + j;
+                if (y32)
+                    y32[p * cout + co] = v;
+                else if (co < o1)
+                    y1[p * o1 + co] = bf16_of(v);
+                else
+                    y2[p * (cout - o1) + (co - o1)] = bf16_of(v);
+            }
+        }
+    }
+}
+
 // Weight gradient partials: part[s][t][o][i] = sum over the pixels p of slice s of
 // dy[p][o] * x[p + off(t)][i]. One wavefront per (16 output channels, 16 input channels, slice),
 // all nine taps at once (A = dy^T is loaded once per 16 pixels and used for every tap). M = output
@@ -225,9 +290,34 @@ bool conv_shape_ok(int c1, int c2, int n, int down, int across, int cout) {
            across % 16 == 0 && cout > 0 && cout % 16 == 0;
 }
 
+template <int MT>
+void conv_k32_launch(const uint16_t *x1, int c1, const uint16_t *x2, int c2, int n, int down, int across,
+                     const uint16_t *w, int cout, int relu, uint16_t *y1, int o1, uint16_t *y2, float *y32,
+                     hipStream_t s) {
+    const long tiles = (long)n * down * (across / (16 * MT));
+    const int nb = cout % 64 == 0 ? 4 : (cout % 32 == 0 ? 2 : 1);
+    const dim3 grid(blocks(tiles, 4), cout / (16 * nb));
+    if (nb == 4)
+        hipLaunchKernelGGL((k_conv3x3_k32<MT, 4>), grid, dim3(256), 0, s, x1, c1, x2, c2, n, down, across, w, cout,
+                           relu, y1, o1, y2, y32);
+    else if (nb == 2)
+        hipLaunchKernelGGL((k_conv3x3_k32<MT, 2>), grid, dim3(256), 0, s, x1, c1, x2, c2, n, down, across, w, cout,
+                           relu, y1, o1, y2, y32);
+    else
+        hipLaunchKernelGGL((k_conv3x3_k32<MT, 1>), grid, dim3(256), 0, s, x1, c1, x2, c2, n, down, across, w, cout,
+                           relu, y1, o1, y2, y32);
+}
+
 template <bool ADJ>
 int conv_launch(const uint16_t *x1, int c1, const uint16_t *x2, int c2, int n, int down, int across,
                 const uint16_t *w, int cout, int relu, uint16_t *y1, int o1, uint16_t *y2, float *y32, hipStream_t s) {
+    if (!ADJ && c1 % 32 == 0 && c2 % 32 == 0) {
+        if (across % 64 == 0)
+            conv_k32_launch<4>(x1, c1, x2, c2, n, down, across, w, cout, relu, y1, o1, y2, y32, s);
+        else
+            conv_k32_launch<1>(x1, c1, x2, c2, n, down, across, w, cout, relu, y1, o1, y2, y32, s);
+        return hipGetLastError() == hipSuccess ? CG_OK : CG_ERR_DEVICE;
+    }
     const long tiles = (long)n * down * (across / 16);
     const int nb = cout % 64 == 0 ? 4 : (cout % 32 == 0 ? 2 : 1);
     const dim3 grid(blocks(tiles, 4), cout / (16 * nb));

@@ -38,9 +38,12 @@ def rand_bf16(*shape, seed=0, scale=1.0):
     return (torch.randn(*shape, generator=g) * scale).to(torch.bfloat16)
 
 
-@pytest.mark.parametrize("c1,c2,cout,relu", [(32, 16, 64, 1), (16, 0, 32, 0), (64, 32, 16, 1), (32, 64, 96, 0)])
-def test_conv_matches_cpu(c1, c2, cout, relu):
-    n, d, a = 2, 8, 48
+@pytest.mark.parametrize("c1,c2,cout,relu,a", [(32, 16, 64, 1, 48), (16, 0, 32, 0, 48), (64, 32, 16, 1, 48),
+                                               (32, 64, 96, 0, 48), (32, 32, 64, 1, 128), (64, 0, 32, 0, 192)])
+def test_conv_matches_cpu(c1, c2, cout, relu, a):
+    """Channel counts of 16 take the K = 16 kernel, multiples of 32 the K = 32 one (one 16-pixel tile
+    per wavefront below 64 columns, four from 64)."""
+    n, d = 2, 8
     x1, x2 = rand_bf16(n, d, a, c1, seed=1), rand_bf16(n, d, a, c2, seed=2) if c2 else None
     w = rand_bf16(9, cout, c1 + c2, seed=3, scale=0.1)
     # device copies held in names: a temporary's memory could be handed to the next one
