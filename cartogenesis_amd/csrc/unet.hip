@@ -102,16 +102,25 @@ __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t *__restrict__ x1
 // B[8 (l / 16) + j][l % 16], j = 0..7, and holds D[4 (l / 16) + j][l % 16] as in k_conv3x3.
 typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 
+// The weights of one 32-channel chunk (9 taps x 16 NB output channels) are staged in LDS by the
+// whole workgroup and read by its four wavefronts; rows padded to 40 elements against bank conflicts.
+constexpr int kSbStride = 40;
+
 template <int MT, int NB>
 __global__ __launch_bounds__(256) void k_conv3x3_k32(const uint16_t *__restrict__ x1, int c1,
                                                      const uint16_t *__restrict__ x2, int c2, int n, int down,
                                                      int across, const uint16_t *__restrict__ w, int cout, int relu,
                                                      uint16_t *__restrict__ y1, int o1, uint16_t *__restrict__ y2,
                                                      float *__restrict__ y32) {
+    __shared__ __attribute__((aligned(16))) uint16_t sb[9 * 16 * NB * kSbStride];
     const int lane = threadIdx.x & 63;
-    const long tile = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const long tile0 = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int per_row = across / (16 * MT);
-    if (tile >= (long)n * down * per_row) return;  // the whole wavefront leaves together
+    const long tiles = (long)n * down * per_row;
+    // A wavefront past the last tile still stages weights and meets every barrier; it computes the
+    // last tile again and stores nothing.
+    const bool valid = tile0 < tiles;
+    const long tile = valid ? tile0 : tiles - 1;
     const long row_id = tile / per_row;
     const int row = (int)(row_id % down);
     const long img = row_id / down;
@@ -122,25 +131,32 @@ __global__ __launch_bounds__(256) void k_conv3x3_k32(const uint16_t *__restrict_
     f4 acc[MT][NB];
     for (int t = 0; t < MT; ++t)
         for (int b = 0; b < NB; ++b) acc[t][b] = (f4){0.f, 0.f, 0.f, 0.f};
-    for (int tap = 0; tap < 9; ++tap) {
-        const int rr = row + tap / 3 - 1;
-        if (rr < 0 || rr >= down) continue;
-        const long rbase = (img * down + rr) * across;
-        long pix[MT];
-        for (int t = 0; t < MT; ++t) pix[t] = rbase + wrap(c0 + 16 * t + m + tap % 3 - 1, across);
-        const uint16_t *wt = w + ((long)tap * cout + co0 + m) * cin + k8;
-        for (int ci0 = 0; ci0 < cin; ci0 += 32) {
-            const bool first = ci0 < c1;
-            const uint16_t *xs = first ? x1 + ci0 + k8 : x2 + (ci0 - c1) + k8;
-            const int cs = first ? c1 : c2;
+    for (int ci0 = 0; ci0 < cin; ci0 += 32) {
+        __syncthreads();  // every wavefront is done with the previous chunk's weights
+        for (int e = threadIdx.x; e < 9 * 16 * NB * 4; e += 256) {
+            const int tap = e / (16 * NB * 4), r = e % (16 * NB * 4), co_l = r >> 2, q = r & 3;
+            *(uint4 *)&sb[(tap * 16 * NB + co_l) * kSbStride + q * 8] =
+                *(const uint4 *)(w + ((long)tap * cout + co0 + co_l) * cin + ci0 + q * 8);
+        }
+        __syncthreads();
+        const bool first = ci0 < c1;
+        const uint16_t *xs = first ? x1 + ci0 + k8 : x2 + (ci0 - c1) + k8;
+        const int cs = first ? c1 : c2;
+        for (int tap = 0; tap < 9; ++tap) {
+            const int rr = row + tap / 3 - 1;
+            if (rr < 0 || rr >= down) continue;  // no barrier inside this loop
+            const long rbase = (img * down + rr) * across;
             b8 a[MT], bv[NB];
-            for (int t = 0; t < MT; ++t) a[t] = *(const b8 *)(xs + pix[t] * cs);
-            for (int b = 0; b < NB; ++b) bv[b] = *(const b8 *)(wt + (long)b * 16 * cin + ci0);
+            for (int t = 0; t < MT; ++t)
+                a[t] = *(const b8 *)(xs + (rbase + wrap(c0 + 16 * t + m + tap % 3 - 1, across)) * cs);
+            for (int b = 0; b < NB; ++b)
+                bv[b] = *(const b8 *)&sb[(tap * 16 * NB + b * 16 + m) * kSbStride + k8];
             for (int t = 0; t < MT; ++t)
                 for (int b = 0; b < NB; ++b)
                     acc[t][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], bv[b], acc[t][b], 0, 0, 0);
         }
     }
+    if (!valid) return;
     for (int t = 0; t < MT; ++t) {
         const long p0 = row_id * across + c0 + 16 * t + kq;
         for (int b = 0; b < NB; ++b) {
