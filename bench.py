@@ -289,7 +289,7 @@ def learned_leg(acc, prob, tol: float, plain_value: float, rows: int, seeds: int
         conv_us = e0.elapsed_time(e1) / 20 * 1000.0
         conv_tf = flops["dec0a"] / (conv_us * 1e-6) / 1e12
         out["mfma_roofline"] = {
-            "bound": "mfma", "kernel": "k_conv3x3_k32<4,2> (dec0a: 96 -> 32 channels, 1024x2048, bf16 in, fp32 "
+            "bound": "mfma", "kernel": "k_conv3x3_tile<2> (dec0a: 96 -> 32 channels, 1024x2048, bf16 in, fp32 "
                                       "accumulate)",
             "achieved": round(conv_tf, 2), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(conv_tf / BF16_PEAK_TFLOPS, 4), "avg_launch_us": round(conv_us, 2),

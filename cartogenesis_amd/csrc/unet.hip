@@ -176,6 +176,86 @@ __global__ __launch_bounds__(256) void k_conv3x3_k32(const uint16_t *__restrict_
     }
 }
 
+// The forward convolution with both operands staged in LDS: a workgroup computes 4 rows x 64
+// pixels (one row per wavefront, four 16-pixel MFMA tiles each) times 16 NB output channels. Per
+// 32-channel chunk it stages the 6 input rows those 4 rows read (66 pixels each: the tile and one
+// halo column either side, wrapped; rows beyond the grid as zeros) and the chunk's weights for all
+// nine taps, so each input value is fetched from L2 once per chunk rather than once per tap and
+// output row, and the nine taps read shifted windows of LDS. Pixel and weight rows are padded to
+// 40 elements against bank conflicts. Needs across % 64 == 0 and down % 4 == 0.
+constexpr int kApx = 66;
+
+template <int NB>
+__global__ __launch_bounds__(256) void k_conv3x3_tile(const uint16_t *__restrict__ x1, int c1,
+                                                      const uint16_t *__restrict__ x2, int c2, int n, int down,
+                                                      int across, const uint16_t *__restrict__ w, int cout, int relu,
+                                                      uint16_t *__restrict__ y1, int o1, uint16_t *__restrict__ y2,
+                                                      float *__restrict__ y32) {
+    __shared__ __attribute__((aligned(16))) uint16_t sa[6 * kApx * kSbStride];
+    __shared__ __attribute__((aligned(16))) uint16_t sb[9 * 16 * NB * kSbStride];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col_tiles = across / 64, row_blocks = down / 4;
+    const int ct = (int)(blockIdx.x % col_tiles);
+    const long rb = blockIdx.x / col_tiles;
+    const int r0 = (int)(rb % row_blocks) * 4;
+    const long img = rb / row_blocks;
+    const int c0 = ct * 64;
+    const int co0 = blockIdx.y * 16 * NB;
+    const int cin = c1 + c2;
+    const int m = lane & 15, k8 = (lane >> 4) * 8, kq = (lane >> 4) * 4;
+    f4 acc[4][NB];
+    for (int t = 0; t < 4; ++t)
+        for (int b = 0; b < NB; ++b) acc[t][b] = (f4){0.f, 0.f, 0.f, 0.f};
+    for (int ci0 = 0; ci0 < cin; ci0 += 32) {
+        const bool first = ci0 < c1;
+        const uint16_t *xs = first ? x1 + ci0 : x2 + (ci0 - c1);
+        const int cs = first ? c1 : c2;
+        __syncthreads();  // every wavefront is done with the previous chunk
+        for (int u = threadIdx.x; u < 6 * kApx * 4; u += 256) {
+            const int i = u / (kApx * 4), px = (u >> 2) % kApx, q = u & 3;
+            const int rr = r0 - 1 + i;
+            uint4 v = {0u, 0u, 0u, 0u};
+            if (rr >= 0 && rr < down)
+                v = *(const uint4 *)(xs + ((img * down + rr) * across + wrap(c0 - 1 + px, across)) * cs + q * 8);
+            *(uint4 *)&sa[(i * kApx + px) * kSbStride + q * 8] = v;
+        }
+        for (int e = threadIdx.x; e < 9 * 16 * NB * 4; e += 256) {
+            const int tap = e / (16 * NB * 4), r = e % (16 * NB * 4), co_l = r >> 2, q = r & 3;
+            *(uint4 *)&sb[(tap * 16 * NB + co_l) * kSbStride + q * 8] =
+                *(const uint4 *)(w + ((long)tap * cout + co0 + co_l) * cin + ci0 + q * 8);
+        }
+        __syncthreads();
+        for (int tap = 0; tap < 9; ++tap) {
+            const int i = wave + tap / 3;  // staged row of input row r0 + wave + dr, dr = tap / 3 - 1
+            const int dc = tap % 3;        // staged pixel offset: px = 16 t + m + 1 + (tap % 3 - 1)
+            b8 a[4], bv[NB];
+            for (int t = 0; t < 4; ++t) a[t] = *(const b8 *)&sa[(i * kApx + 16 * t + m + dc) * kSbStride + k8];
+            for (int b = 0; b < NB; ++b)
+                bv[b] = *(const b8 *)&sb[(tap * 16 * NB + b * 16 + m) * kSbStride + k8];
+            for (int t = 0; t < 4; ++t)
+                for (int b = 0; b < NB; ++b)
+                    acc[t][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], bv[b], acc[t][b], 0, 0, 0);
+        }
+    }
+    const long prow = (img * down + r0 + wave) * across + c0;
+    for (int t = 0; t < 4; ++t) {
+        for (int b = 0; b < NB; ++b) {
+            const int co = co0 + b * 16 + m;
+            for (int j = 0; j < 4; ++j) {
+                float v = acc[t][b][j];
+                if (relu) v = v > 0.f ? v : 0.f;
+                const long p = prow + 16 * t + kq + j;
+                if (y32)
+                    y32[p * cout + co] = v;
+                else if (co < o1)
+                    y1[p * o1 + co] = bf16_of(v);
+                else
+                    y2[p * (cout - o1) + (co - o1)] = bf16_of(v);
+            }
+        }
+    }
+}
+
 // Weight gradient partials: part[s][t][o][i] = sum over the pixels p of slice s of
 // dy[p][o] * x[p + off(t)][i]. One wavefront per (16 output channels, 16 input channels, slice),
 // all nine taps at once (A = dy^T is loaded once per 16 pixels and used for every tap). M = output
@@ -327,6 +407,20 @@ void conv_k32_launch(const uint16_t *x1, int c1, const uint16_t *x2, int c2, int
 template <bool ADJ>
 int conv_launch(const uint16_t *x1, int c1, const uint16_t *x2, int c2, int n, int down, int across,
                 const uint16_t *w, int cout, int relu, uint16_t *y1, int o1, uint16_t *y2, float *y32, hipStream_t s) {
+    if (!ADJ && c1 % 32 == 0 && c2 % 32 == 0 && across % 64 == 0 && down % 4 == 0) {
+        const int nb = cout % 64 == 0 ? 4 : (cout % 32 == 0 ? 2 : 1);
+        const dim3 grid((unsigned)((long)n * (down / 4) * (across / 64)), cout / (16 * nb));
+        if (nb == 4)
+            hipLaunchKernelGGL((k_conv3x3_tile<4>), grid, dim3(256), 0, s, x1, c1, x2, c2, n, down, across, w, cout,
+                               relu, y1, o1, y2, y32);
+        else if (nb == 2)
+            hipLaunchKernelGGL((k_conv3x3_tile<2>), grid, dim3(256), 0, s, x1, c1, x2, c2, n, down, across, w, cout,
+                               relu, y1, o1, y2, y32);
+        else
+            hipLaunchKernelGGL((k_conv3x3_tile<1>), grid, dim3(256), 0, s, x1, c1, x2, c2, n, down, across, w, cout,
+                               relu, y1, o1, y2, y32);
+        return hipGetLastError() == hipSuccess ? CG_OK : CG_ERR_DEVICE;
+    }
     if (!ADJ && c1 % 32 == 0 && c2 % 32 == 0) {
         if (across % 64 == 0)
             conv_k32_launch<4>(x1, c1, x2, c2, n, down, across, w, cout, relu, y1, o1, y2, y32, s);
