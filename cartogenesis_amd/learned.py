@@ -342,18 +342,25 @@ class LearnedPreconditioner:
         self.scale_out = 1.0 / (centre_scale(stencil) * self.scale_in)  # any scale serves FGMRES
         self.applications = 0
         self.error: Optional[BaseException] = None
+        self._streams: Dict[int, torch.cuda.ExternalStream] = {}
         torch.cuda.synchronize()
         self._cb = PRECOND_FN(self._apply)
 
     def _apply(self, user, v, z, stream) -> int:
         try:
             L = lib()
-            s = ctypes.c_void_p(stream)
-            check(L.cg_unet_pack_input(_p(self.fixed), ctypes.c_void_p(v), self.scale_in, self.cells, _p(self.x), s),
-                  what="cg_unet_pack_input")
-            out = self.net.forward(self.x, stream=s)
-            check(L.cg_unet_unpack_output(_p(out), _p(self.water), self.scale_out, self.cells, ctypes.c_void_p(z), s),
-                  what="cg_unet_unpack_output")
+            key = int(stream or 0)
+            if key not in self._streams:
+                self._streams[key] = torch.cuda.ExternalStream(key) if key else torch.cuda.default_stream()
+            # The network's activations are allocated with the plan's stream current, so torch's
+            # caching allocator hands their memory on only in that stream's order.
+            with torch.cuda.stream(self._streams[key]):
+                s = torch.cuda.current_stream().cuda_stream
+                check(L.cg_unet_pack_input(_p(self.fixed), ctypes.c_void_p(v), self.scale_in, self.cells, _p(self.x),
+                                           s), what="cg_unet_pack_input")
+                out = self.net.forward(self.x)
+                check(L.cg_unet_unpack_output(_p(out), _p(self.water), self.scale_out, self.cells, ctypes.c_void_p(z),
+                                              s), what="cg_unet_unpack_output")
             self.applications += 1
             return 0
         except BaseException as e:  # an exception must not cross the C boundary
