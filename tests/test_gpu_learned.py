@@ -195,3 +195,38 @@ def test_learned_preconditioner_inside_fgmres():
         assert rel < 1e-6
     finally:
         acc.close()
+
+
+def test_learned_method_errors_are_reported():
+    """CG_METHOD_LEARNED without a preconditioner is refused; a preconditioner that returns an error
+    fails the solve with it (both forced from the host); the plan then solves as before."""
+    from cartogenesis_amd import CG_METHOD_REFINED, PRECOND_FN, CgError
+    from cartogenesis_amd.ocean import HipOcean, Plan
+    from cartogenesis_amd.problem import circulation_problem
+
+    acc = HipOcean.create_or_null(0).accelerator
+    try:
+        prob = circulation_problem(256, seed=151)
+        plan = Plan(acc, prob.levels)
+        try:
+            with pytest.raises(CgError, match="needs cg_ocean_plan_set_preconditioner"):
+                plan.solve(1e-8, CG_METHOD_LEARNED)
+            calls = []
+
+            def failing(user, v, z, stream):
+                calls.append(1)
+                return -1
+
+            cb = PRECOND_FN(failing)
+            check(lib().cg_ocean_plan_set_preconditioner(plan.handle, cb, None), acc.ctx, "set_preconditioner")
+            with pytest.raises(CgError, match="learned preconditioner returned -1"):
+                plan.solve(1e-8, CG_METHOD_LEARNED)
+            assert calls == [1]
+            check(lib().cg_ocean_plan_set_preconditioner(plan.handle, PRECOND_FN(), None), acc.ctx,
+                  "set_preconditioner")
+            info = plan.solve(1e-8, CG_METHOD_REFINED)
+            assert info.relative_residual < 1e-8
+        finally:
+            plan.close()
+    finally:
+        acc.close()
