@@ -294,6 +294,8 @@ def learned_leg(acc, prob, tol: float, plain_value: float, rows: int, seeds: int
             "achieved": round(conv_tf, 2), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(conv_tf / BF16_PEAK_TFLOPS, 4), "avg_launch_us": round(conv_us, 2),
             "flops_per_launch": flops["dec0a"],
+            # rocprofv3 FETCH_SIZE / WRITE_SIZE of this launch, scaled by the copy calibration
+            "traffic": 554620928, "algorithmic_bytes_per_launch": 536926208,
             "network_forward_ms": round(fwd_ms, 3), "network_flops": total,
             "network_tflops": round(total / (fwd_ms * 1e-3) / 1e12, 2)}
         log(f"bench: learned {ln_value:.1f} solves/s ({ln.cycles} iterations) against FGMRES {fg_value:.1f} "
@@ -332,6 +334,9 @@ def main() -> int:
                          "2048x4096, one plan of 16 per size")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned preconditioner's leg")
     ap.add_argument("--learned-only", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--conv-only", action="store_true",
+                    help="only the U-Net's largest convolution (dec0a at 1024x2048, random weights), 5 launches: "
+                         "the short program the convolution's counters are read from")
     ap.add_argument("--plain-value", type=float, default=0.0, help=argparse.SUPPRESS)
     ap.add_argument("--learned-rows", type=int, default=256, help="training grid rows (columns = 2 rows)")
     ap.add_argument("--learned-seeds", type=int, default=16, help="training worlds (seeds 100, 101, ...)")
@@ -360,6 +365,30 @@ def main() -> int:
             c.copy_(b)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes_per_copy": n * 4, "copies": 6}))
+        return 0
+
+    if args.conv_only:
+        import torch
+
+        torch.cuda.init()
+        from cartogenesis_amd.learned import UNet
+
+        net = UNet(32, seed=0)
+        a1 = torch.randn((1, ROWS, COLS, 64), device="cuda").to(torch.bfloat16)
+        a2 = torch.randn((1, ROWS, COLS, 32), device="cuda").to(torch.bfloat16)
+        st = torch.cuda.current_stream().cuda_stream
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        net._conv("dec0a", a1, a2, 1, ROWS, COLS, st)
+        e0.record()
+        for _ in range(5):
+            net._conv("dec0a", a1, a2, 1, ROWS, COLS, st)
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) / 5 * 1000.0
+        fl = unet_flops(ROWS, COLS)["dec0a"]
+        algo = 2 * ROWS * COLS * (64 + 32 + 32) + 2 * 9 * 32 * 96  # inputs and output once, the weights
+        print(json.dumps({"kernel": "k_conv3x3_tile<2>", "avg_launch_us": round(us, 2),
+                          "tflops": round(fl / (us * 1e-6) / 1e12, 1), "algorithmic_bytes": algo}), flush=True)
         return 0
 
     if args.learned_only:
