@@ -4,8 +4,10 @@ One step = one solve of the synthetic 1024 x 2048 world (seed 0, 30% land, annua
 psi = 0 to max|F - L psi| / max|F| < 1e-8, with the ladder, forcing and answer resident in HBM
 (a device-resident plan). Prints one JSON line.
 
-Plain GPU multigrid (binary32 V(2,2) cycles of the reference's ladder inside binary64 iterative
-refinement) is the method measured; the learned preconditioner is not built yet.
+`value` is plain GPU multigrid (binary32 V(2,2) cycles of the reference's ladder inside binary64
+iterative refinement). The learned preconditioner's leg runs in a child process and is reported
+under "learned": the U-Net trained on the GPU, FGMRES with and without it on the same problem, its
+solves/s over the plain multigrid's, and its largest convolution against the matrix-core roofline.
 """
 from __future__ import annotations
 
