@@ -446,6 +446,16 @@ __global__ void k_upsample2_v8(const uint16_t *__restrict__ x, int ch, int n, in
     *(uint4 *)(y + i * 8) = v;
 }
 
+// wt[8 - t][i][o] = w[t][o][i]: the weights with which the forward convolution computes the data
+// gradient of the convolution by w (taps mirrored, input and output channels exchanged).
+__global__ void k_flip_weights(const uint16_t *__restrict__ w, int cout, int cin, uint16_t *__restrict__ wt) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long per_tap = (long)cout * cin;
+    if (i >= 9 * per_tap) return;
+    const int t = (int)(i / per_tap), o = (int)((i / cin) % cout), c = (int)(i % cin);
+    wt[((long)(8 - t) * cin + c) * cout + o] = w[i];
+}
+
 // dy = y > 0 ? dy : 0 (the gradient through a ReLU whose output was y).
 __global__ void k_relu_mask(uint16_t *__restrict__ dy, const uint16_t *__restrict__ y, long count) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -563,6 +573,13 @@ int cg_conv3x3_adjoint(const uint16_t *dy, int cout, int n, int down, int across
     if (!dx32 && (!dx1 || c1 < 0 || c1 > cin || c1 % 16 || (c1 < cin && !dx2))) return CG_ERR_ARG;
     return conv_launch<true>(dy, cout, nullptr, 0, n, down, across, w, cin, 0, dx1, dx32 ? cin : c1, dx2, dx32,
                              (hipStream_t)stream);
+}
+
+int cg_conv3x3_flip_weights(const uint16_t *w, int cout, int cin, uint16_t *wt, void *stream) {
+    if (!w || !wt || cout <= 0 || cin <= 0) return CG_ERR_ARG;
+    hipLaunchKernelGGL(k_flip_weights, dim3(blocks(9L * cout * cin, 256)), dim3(256), 0, (hipStream_t)stream, w, cout,
+                       cin, wt);
+    return hipGetLastError() == hipSuccess ? CG_OK : CG_ERR_DEVICE;
 }
 
 int cg_conv3x3_wgrad_slices(int n, int down, int across) {

@@ -147,6 +147,7 @@ class UNet:
         n, d, a, _ = dout.shape
         dims = [(d >> k, a >> k) for k in range(4)]
         grads: Dict[str, torch.Tensor] = {}
+        keep: List[torch.Tensor] = []  # flipped weights stay allocated until the enqueued work is done
         L = lib()
 
         def wgrad(name, a1, a2, g, lvl):
@@ -160,8 +161,17 @@ class UNet:
             c1, c2, co, _ = self.spec[name]
             g1 = torch.empty((n, *dims[lvl], c1), dtype=BF16, device=self.device)
             g2 = torch.empty((n, *dims[lvl], c2), dtype=BF16, device=self.device) if c2 else None
-            check(L.cg_conv3x3_adjoint(_p(g), co, n, *dims[lvl], _p(self.wb[name]), c1 + c2, _p(g1), c1, _p(g2),
-                                       None, s), what=f"cg_conv3x3_adjoint {name}")
+            if co % 32 == 0:
+                # the forward kernels (K = 32, staged in LDS) with the mirrored, transposed weights
+                wt = torch.empty((9, c1 + c2, co), dtype=BF16, device=self.device)
+                check(L.cg_conv3x3_flip_weights(_p(self.wb[name]), co, c1 + c2, _p(wt), s),
+                      what=f"cg_conv3x3_flip_weights {name}")
+                check(L.cg_conv3x3(_p(g), co, None, 0, n, *dims[lvl], _p(wt), c1 + c2, 0, _p(g1), c1, _p(g2), None, s),
+                      what=f"cg_conv3x3 (data gradient) {name}")
+                keep.append(wt)
+            else:
+                check(L.cg_conv3x3_adjoint(_p(g), co, n, *dims[lvl], _p(self.wb[name]), c1 + c2, _p(g1), c1,
+                                           _p(g2), None, s), what=f"cg_conv3x3_adjoint {name}")
             return g1, g2
 
         def relu_back(g, y):

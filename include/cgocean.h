@@ -307,6 +307,9 @@ int cg_conv3x3(const uint16_t *x1, int c1, const uint16_t *x2, int c2, int n, in
  * forward's inputs were (channels [0, c1) into dx1, the rest into dx2), or binary32 into dx32. */
 int cg_conv3x3_adjoint(const uint16_t *dy, int cout, int n, int down, int across, const uint16_t *w, int cin,
                        uint16_t *dx1, int c1, uint16_t *dx2, float *dx32, void *stream);
+/* wt[9][cin][cout] = w[9][cout][cin] with the taps mirrored: cg_conv3x3 of dy with wt (cin output
+ * channels) is cg_conv3x3_adjoint of dy with w, on the forward kernels. */
+int cg_conv3x3_flip_weights(const uint16_t *w, int cout, int cin, uint16_t *wt, void *stream);
 /* The weight gradient dw[9][cout][c1 + c2] (binary32) of cg_conv3x3 for input x and output gradient
  * dy, summed over all pixels in a fixed order. work holds slices x 9 x cout x (c1 + c2) floats,
  * slices = cg_conv3x3_wgrad_slices(n, down, across). */

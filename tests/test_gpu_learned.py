@@ -84,6 +84,18 @@ def test_conv_adjoint_and_weight_gradient_match_cpu_autograd():
     check(lib().cg_conv3x3_adjoint(_p(dyd), cout, n, d, a, _p(wd), c1 + c2, _p(g1), c1, _p(g2), None, None),
           what="cg_conv3x3_adjoint split")
     assert torch.equal(torch.cat([g1, g2], -1).cpu(), dx.cpu().to(torch.bfloat16))
+    # the same gradient from the forward kernels with the mirrored, transposed weights
+    wt = torch.empty(9, c1 + c2, cout, dtype=torch.bfloat16, device="cuda")
+    check(lib().cg_conv3x3_flip_weights(_p(wd), cout, c1 + c2, _p(wt), None), what="cg_conv3x3_flip_weights")
+    assert torch.equal(wt.cpu(), w.reshape(3, 3, cout, c1 + c2).flip(0, 1).reshape(9, cout, c1 + c2).transpose(1, 2))
+    for across in (a, 128):  # the K = 32 kernel with one 16-pixel tile a wavefront, then the staged one
+        dyw = rand_bf16(n, d, across, cout, seed=9).cuda()
+        xw = rand_bf16(n, d, across, c1 + c2, seed=10).float().requires_grad_(True)
+        (cpu_conv(xw, w.float()) * dyw.cpu().float()).sum().backward()
+        dxf = torch.empty(n, d, across, c1 + c2, device="cuda")
+        check(lib().cg_conv3x3(_p(dyw), cout, None, 0, n, d, across, _p(wt), c1 + c2, 0, None, 0, None, _p(dxf), None),
+              what="cg_conv3x3 flipped")
+        assert (dxf.cpu() - xw.grad).abs().max() <= 1e-5 * xw.grad.abs().max()
     # weight gradient, with the input split over two tensors
     slices = lib().cg_conv3x3_wgrad_slices(n, d, a)
     work = torch.empty(slices * 9 * cout * (c1 + c2), device="cuda")
