@@ -206,12 +206,15 @@ __global__ __launch_bounds__(256) void k_conv3x3_tile(const uint16_t *__restrict
     f4 acc[4][NB];
     for (int t = 0; t < 4; ++t)
         for (int b = 0; b < NB; ++b) acc[t][b] = (f4){0.f, 0.f, 0.f, 0.f};
-    // The next chunk's 16-byte units are loaded into registers while the current chunk's MFMAs run
-    // (kAU pixel units and kBU weight units per thread), then written to LDS between two barriers.
+    // The next chunk's pixel units are loaded into registers while the current chunk's MFMAs run
+    // (kAU a thread), and with at most 32 output channels a wavefront its weight units too (kBU);
+    // with 64 the weights' registers cost more occupancy than the overlap gains, so they are loaded
+    // as they are stored. Registers go to LDS between two barriers.
     constexpr int kAUnits = 6 * kApx * 4, kBUnits = 9 * 16 * NB * 4;
     constexpr int kAU = (kAUnits + 255) / 256, kBU = (kBUnits + 255) / 256;
-    uint4 ra[kAU], rw[kBU];
-    auto load_chunk = [&](int ci0) {
+    constexpr bool kPrefetchB = NB <= 2;
+    uint4 ra[kAU], rw[kPrefetchB ? kBU : 1];
+    auto load_a = [&](int ci0) {
         const bool first = ci0 < c1;
         const uint16_t *xs = first ? x1 + ci0 : x2 + (ci0 - c1);
         const int cs = first ? c1 : c2;
@@ -226,56 +229,46 @@ __global__ __launch_bounds__(256) void k_conv3x3_tile(const uint16_t *__restrict
                                              q * 8);
             }
         }
-        for (int k = 0; k < kBU; ++k) {
-            const int e = threadIdx.x + 256 * k;
-            if (e < kBUnits) {
-                const int tap = e / (16 * NB * 4), r = e % (16 * NB * 4), co_l = r >> 2, q = r & 3;
-                rw[k] = *(const uint4 *)(w + ((long)tap * cout + co0 + co_l) * cin + ci0 + q * 8);
-            }
-        }
     };
-    // Prefetching holds the units across the MFMAs: with 64 output channels a wavefront (NB = 4) the
-    // registers cost more occupancy than the overlap gains, so those load just before storing.
-    constexpr bool kPrefetch = NB <= 2;
-    if (kPrefetch) load_chunk(0);
-    for (int ci0 = 0; ci0 < cin; ci0 += 32) {
-        __syncthreads();  // every wavefront is done with the previous chunk
-        if (!kPrefetch) {
-            // one unit at a time: load and store
-            const bool first = ci0 < c1;
-            const uint16_t *xs = first ? x1 + ci0 : x2 + (ci0 - c1);
-            const int cs = first ? c1 : c2;
-            for (int u = threadIdx.x; u < kAUnits; u += 256) {
-                const int i = u / (kApx * 4), px = (u >> 2) % kApx, q = u & 3;
-                const int rr = r0 - 1 + i;
-                uint4 v = {0u, 0u, 0u, 0u};
-                if (rr >= 0 && rr < down)
-                    v = *(const uint4 *)(xs + ((img * down + rr) * across + wrap(c0 - 1 + px, across)) * cs + q * 8);
-                *(uint4 *)&sa[(i * kApx + px) * kSbStride + q * 8] = v;
-            }
-            for (int e = threadIdx.x; e < kBUnits; e += 256) {
-                const int tap = e / (16 * NB * 4), r = e % (16 * NB * 4), co_l = r >> 2, q = r & 3;
-                *(uint4 *)&sb[(tap * 16 * NB + co_l) * kSbStride + q * 8] =
-                    *(const uint4 *)(w + ((long)tap * cout + co0 + co_l) * cin + ci0 + q * 8);
-            }
-        } else {  // from the registers the previous chunk filled
-            for (int k = 0; k < kAU; ++k) {
-                const int u = threadIdx.x + 256 * k;
-                if (u < kAUnits) {
-                    const int i = u / (kApx * 4), px = (u >> 2) % kApx, q = u & 3;
-                    *(uint4 *)&sa[(i * kApx + px) * kSbStride + q * 8] = ra[k];
-                }
-            }
+    auto weight_unit = [&](int ci0, int e) -> const uint4 * {
+        const int tap = e / (16 * NB * 4), r = e % (16 * NB * 4), co_l = r >> 2, q = r & 3;
+        return (const uint4 *)(w + ((long)tap * cout + co0 + co_l) * cin + ci0 + q * 8);
+    };
+    auto weight_slot = [&](int e) -> uint4 * {
+        const int tap = e / (16 * NB * 4), r = e % (16 * NB * 4), co_l = r >> 2, q = r & 3;
+        return (uint4 *)&sb[(tap * 16 * NB + co_l) * kSbStride + q * 8];
+    };
+    auto load_b = [&](int ci0) {
+        if constexpr (kPrefetchB)
             for (int k = 0; k < kBU; ++k) {
                 const int e = threadIdx.x + 256 * k;
-                if (e < kBUnits) {
-                    const int tap = e / (16 * NB * 4), r = e % (16 * NB * 4), co_l = r >> 2, q = r & 3;
-                    *(uint4 *)&sb[(tap * 16 * NB + co_l) * kSbStride + q * 8] = rw[k];
-                }
+                if (e < kBUnits) rw[k] = *weight_unit(ci0, e);
+            }
+    };
+    load_a(0);
+    load_b(0);
+    for (int ci0 = 0; ci0 < cin; ci0 += 32) {
+        __syncthreads();  // every wavefront is done with the previous chunk
+        for (int k = 0; k < kAU; ++k) {
+            const int u = threadIdx.x + 256 * k;
+            if (u < kAUnits) {
+                const int i = u / (kApx * 4), px = (u >> 2) % kApx, q = u & 3;
+                *(uint4 *)&sa[(i * kApx + px) * kSbStride + q * 8] = ra[k];
             }
         }
+        if constexpr (kPrefetchB) {
+            for (int k = 0; k < kBU; ++k) {
+                const int e = threadIdx.x + 256 * k;
+                if (e < kBUnits) *weight_slot(e) = rw[k];
+            }
+        } else {
+            for (int e = threadIdx.x; e < kBUnits; e += 256) *weight_slot(e) = *weight_unit(ci0, e);
+        }
         __syncthreads();
-        if (kPrefetch && ci0 + 32 < cin) load_chunk(ci0 + 32);
+        if (ci0 + 32 < cin) {
+            load_a(ci0 + 32);
+            load_b(ci0 + 32);
+        }
         for (int tap = 0; tap < 9; ++tap) {
             const int i = wave + tap / 3;  // staged row of input row r0 + wave + dr, dr = tap / 3 - 1
             const int dc = tap % 3;        // staged pixel offset: px = 16 t + m + 1 + (tap % 3 - 1)
