@@ -346,6 +346,71 @@ __global__ __launch_bounds__(256) void k_conv3x3_wgrad(const uint16_t *__restric
         for (int j = 0; j < 4; ++j) dst[((long)t * cout + co0 + kq + j) * cin + ci0 + m] = acc[t][j];
 }
 
+// The weight gradient partials as k_conv3x3_wgrad computes them, one wavefront (one workgroup) per
+// (16 output channels, 16 input channels, slice), on v_mfma_f32_16x16x32_bf16 over groups of 32
+// pixels of one row (across % 32 == 0). Per group the wavefront stages, with 8-byte loads, the
+// output gradient's 32 x 16 block and the input's 3 rows x 34 pixels (a halo column either side,
+// wrapped; rows beyond the grid as zeros) x 16 channels in LDS, and forms the pixel-major operands
+// there (lane l: channel l % 16, pixels 8 (l / 16) .. + 7). Pixel rows are 20 elements apart so the
+// four lane groups fall in different banks.
+constexpr int kWgStride = 20;
+typedef unsigned short us8 __attribute__((ext_vector_type(8)));
+
+__global__ __launch_bounds__(64) void k_conv3x3_wgrad32(const uint16_t *__restrict__ x1, int c1,
+                                                        const uint16_t *__restrict__ x2, int c2,
+                                                        const uint16_t *__restrict__ dy, int cout, int n, int down,
+                                                        int across, int slices, float *__restrict__ part) {
+    __shared__ __attribute__((aligned(16))) uint16_t sdy[32 * kWgStride];
+    __shared__ __attribute__((aligned(16))) uint16_t sx[3 * 34 * kWgStride];
+    const int lane = threadIdx.x;
+    const int slice = blockIdx.y;
+    const int cin = c1 + c2;
+    const int cit = cin >> 4;
+    const int co0 = (blockIdx.x / cit) * 16, ci0 = (blockIdx.x % cit) * 16;
+    const bool first = ci0 < c1;
+    const uint16_t *xs = first ? x1 : x2;
+    const int cs = first ? c1 : c2, cofs = first ? ci0 : ci0 - c1;
+    const long groups = (long)n * down * across / 32;
+    const long g_lo = groups * slice / slices, g_hi = groups * (slice + 1) / slices;
+    const int m = lane & 15, k8 = (lane >> 4) * 8;
+    f4 acc[9];
+    for (int t = 0; t < 9; ++t) acc[t] = (f4){0.f, 0.f, 0.f, 0.f};
+    for (long g = g_lo; g < g_hi; ++g) {
+        const long p0 = g * 32;
+        const long row_id = p0 / across;
+        const int c0 = (int)(p0 - row_id * across);
+        const int row = (int)(row_id % down);
+        const long img = row_id / down;
+        __syncthreads();  // the previous group's operands have been read
+        for (int u = lane; u < 32 * 4; u += 64) {  // 32 pixels x 4 pieces of 4 channels
+            const int px = u >> 2, q = u & 3;
+            *(uint2 *)&sdy[px * kWgStride + q * 4] = *(const uint2 *)(dy + (p0 + px) * cout + co0 + q * 4);
+        }
+        for (int u = lane; u < 3 * 34 * 4; u += 64) {
+            const int r = u / (34 * 4), px = (u >> 2) % 34, q = u & 3;
+            const int rr = row + r - 1;
+            uint2 v = {0u, 0u};
+            if (rr >= 0 && rr < down)
+                v = *(const uint2 *)(xs + ((img * down + rr) * across + wrap(c0 - 1 + px, across)) * cs + cofs + q * 4);
+            *(uint2 *)&sx[(r * 34 + px) * kWgStride + q * 4] = v;
+        }
+        __syncthreads();
+        us8 av;
+        for (int j = 0; j < 8; ++j) av[j] = sdy[(k8 + j) * kWgStride + m];
+        const b8 a = __builtin_bit_cast(b8, av);
+        for (int t = 0; t < 9; ++t) {
+            const int r = t / 3, dc = t % 3;  // staged row of row + dr; staged pixel of pixel + dc - 1
+            us8 bvv;
+            for (int j = 0; j < 8; ++j) bvv[j] = sx[(r * 34 + k8 + j + dc) * kWgStride + m];
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(b8, bvv), acc[t], 0, 0, 0);
+        }
+    }
+    const int kq = (lane >> 4) * 4;
+    float *dst = part + (long)slice * 9 * cout * cin;
+    for (int t = 0; t < 9; ++t)
+        for (int j = 0; j < 4; ++j) dst[((long)t * cout + co0 + kq + j) * cin + ci0 + m] = acc[t][j];
+}
+
 // dw[i] = sum over slices s of part[s][i], in slice order (deterministic).
 __global__ void k_wgrad_sum(const float *__restrict__ part, int slices, long count, float *__restrict__ dw) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -591,8 +656,12 @@ int cg_conv3x3_wgrad(const uint16_t *x1, int c1, const uint16_t *x2, int c2, con
     const long count = 9L * cout * (c1 + c2);
     if (work_floats < (size_t)slices * (size_t)count) return CG_ERR_ARG;
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_conv3x3_wgrad, dim3((cout / 16) * ((c1 + c2) / 16), slices / 4), dim3(256), 0, s, x1, c1,
-                       c2 ? x2 : nullptr, c2, dy, cout, n, down, across, slices, work);
+    if (across % 32 == 0)
+        hipLaunchKernelGGL(k_conv3x3_wgrad32, dim3((cout / 16) * ((c1 + c2) / 16), slices), dim3(64), 0, s, x1, c1,
+                           c2 ? x2 : nullptr, c2, dy, cout, n, down, across, slices, work);
+    else
+        hipLaunchKernelGGL(k_conv3x3_wgrad, dim3((cout / 16) * ((c1 + c2) / 16), slices / 4), dim3(256), 0, s, x1, c1,
+                           c2 ? x2 : nullptr, c2, dy, cout, n, down, across, slices, work);
     hipLaunchKernelGGL(k_wgrad_sum, dim3(blocks(count, 256)), dim3(256), 0, s, work, slices, count, dw);
     return hipGetLastError() == hipSuccess ? CG_OK : CG_ERR_DEVICE;
 }
