@@ -182,7 +182,9 @@ __global__ __launch_bounds__(256) void k_conv3x3_k32(const uint16_t *__restrict_
 // halo column either side, wrapped; rows beyond the grid as zeros) and the chunk's weights for all
 // nine taps, so each input value is fetched from L2 once per chunk rather than once per tap and
 // output row, and the nine taps read shifted windows of LDS. Pixel and weight rows are padded to
-// 40 elements against bank conflicts. Needs across % 64 == 0 and down % 4 == 0.
+// 40 elements against bank conflicts. Needs across % 64 == 0 and down % 4 == 0. With up1 the first
+// input is read through nearest upsampling from a tensor of half the height and width (the U-Net's
+// decoder input), so the upsampled tensor is never written.
 constexpr int kApx = 66;
 
 template <int NB>
@@ -190,7 +192,7 @@ __global__ __launch_bounds__(256) void k_conv3x3_tile(const uint16_t *__restrict
                                                       const uint16_t *__restrict__ x2, int c2, int n, int down,
                                                       int across, const uint16_t *__restrict__ w, int cout, int relu,
                                                       uint16_t *__restrict__ y1, int o1, uint16_t *__restrict__ y2,
-                                                      float *__restrict__ y32) {
+                                                      float *__restrict__ y32, int up1) {
     __shared__ __attribute__((aligned(16))) uint16_t sa[6 * kApx * kSbStride];
     __shared__ __attribute__((aligned(16))) uint16_t sb[9 * 16 * NB * kSbStride];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -224,9 +226,13 @@ __global__ __launch_bounds__(256) void k_conv3x3_tile(const uint16_t *__restrict
             if (u < kAUnits) {
                 const int i = u / (kApx * 4), px = (u >> 2) % kApx, q = u & 3;
                 const int rr = r0 - 1 + i;
-                if (rr >= 0 && rr < down)
-                    ra[k] = *(const uint4 *)(xs + ((img * down + rr) * across + wrap(c0 - 1 + px, across)) * cs +
-                                             q * 8);
+                if (rr >= 0 && rr < down) {
+                    const int cc = wrap(c0 - 1 + px, across);
+                    // up1: the first input is half as tall and wide, read through nearest upsampling
+                    const long pix = first && up1 ? (img * (down >> 1) + (rr >> 1)) * (across >> 1) + (cc >> 1)
+                                                  : (img * down + rr) * across + cc;
+                    ra[k] = *(const uint4 *)(xs + pix * cs + q * 8);
+                }
             }
         }
     };
@@ -573,21 +579,26 @@ void conv_k32_launch(const uint16_t *x1, int c1, const uint16_t *x2, int c2, int
                            relu, y1, o1, y2, y32);
 }
 
+bool tile_ok(int c1, int c2, int down, int across) {
+    return c1 % 32 == 0 && c2 % 32 == 0 && across % 64 == 0 && down % 4 == 0;
+}
+
 template <bool ADJ>
 int conv_launch(const uint16_t *x1, int c1, const uint16_t *x2, int c2, int n, int down, int across,
-                const uint16_t *w, int cout, int relu, uint16_t *y1, int o1, uint16_t *y2, float *y32, hipStream_t s) {
-    if (!ADJ && c1 % 32 == 0 && c2 % 32 == 0 && across % 64 == 0 && down % 4 == 0) {
+                const uint16_t *w, int cout, int relu, uint16_t *y1, int o1, uint16_t *y2, float *y32, hipStream_t s,
+                int up1 = 0) {
+    if (!ADJ && tile_ok(c1, c2, down, across)) {
         const int nb = cout % 64 == 0 ? 4 : (cout % 32 == 0 ? 2 : 1);
         const dim3 grid((unsigned)((long)n * (down / 4) * (across / 64)), cout / (16 * nb));
         if (nb == 4)
             hipLaunchKernelGGL((k_conv3x3_tile<4>), grid, dim3(256), 0, s, x1, c1, x2, c2, n, down, across, w, cout,
-                               relu, y1, o1, y2, y32);
+                               relu, y1, o1, y2, y32, up1);
         else if (nb == 2)
             hipLaunchKernelGGL((k_conv3x3_tile<2>), grid, dim3(256), 0, s, x1, c1, x2, c2, n, down, across, w, cout,
-                               relu, y1, o1, y2, y32);
+                               relu, y1, o1, y2, y32, up1);
         else
             hipLaunchKernelGGL((k_conv3x3_tile<1>), grid, dim3(256), 0, s, x1, c1, x2, c2, n, down, across, w, cout,
-                               relu, y1, o1, y2, y32);
+                               relu, y1, o1, y2, y32, up1);
         return hipGetLastError() == hipSuccess ? CG_OK : CG_ERR_DEVICE;
     }
     if (!ADJ && c1 % 32 == 0 && c2 % 32 == 0) {
@@ -623,6 +634,16 @@ int cg_conv3x3(const uint16_t *x1, int c1, const uint16_t *x2, int c2, int n, in
     if (!y32 && (!y1 || o1 < 0 || o1 > cout || o1 % 16 || (o1 < cout && !y2))) return CG_ERR_ARG;
     return conv_launch<false>(x1, c1, c2 ? x2 : nullptr, c2, n, down, across, w, cout, relu, y1, y32 ? cout : o1, y2,
                               y32, (hipStream_t)stream);
+}
+
+int cg_conv3x3_up(const uint16_t *x1, int c1, const uint16_t *x2, int c2, int n, int down, int across,
+                  const uint16_t *w, int cout, int relu, uint16_t *y1, int o1, uint16_t *y2, float *y32,
+                  void *stream) {
+    if (!conv_shape_ok(c1, c2, n, down, across, cout) || !tile_ok(c1, c2, down, across) || !x1 || (c2 && !x2) || !w)
+        return CG_ERR_ARG;
+    if (!y32 && (!y1 || o1 < 0 || o1 > cout || o1 % 16 || (o1 < cout && !y2))) return CG_ERR_ARG;
+    return conv_launch<false>(x1, c1, c2 ? x2 : nullptr, c2, n, down, across, w, cout, relu, y1, y32 ? cout : o1, y2,
+                              y32, (hipStream_t)stream, 1);
 }
 
 int cg_conv3x3_adjoint(const uint16_t *dy, int cout, int n, int down, int across, const uint16_t *w, int cin,

@@ -118,14 +118,25 @@ class UNet:
         t["p3"] = self._pool(t["enc2b"], n, *dims[2], s)
         t["mid_a"] = self._conv("mid_a", t["p3"], None, n, *dims[3], s)
         t["mid_b"] = self._conv("mid_b", t["mid_a"], None, n, *dims[3], s)
-        t["u2"] = self._up(t["mid_b"], n, *dims[3], s)
-        t["dec2a"] = self._conv("dec2a", t["u2"], t["enc2b"], n, *dims[2], s)
+        def decoder(name, coarse, skip, up, lvl):
+            """The decoder convolution of level lvl on (upsampled coarse, skip). Without keep it reads
+            the coarse tensor through the upsampling itself where the staged kernel serves the shape;
+            backward needs the upsampled tensor, so with keep it is formed."""
+            c1, c2, co, relu = self.spec[name]
+            d_, a_ = dims[lvl]
+            if not keep and c1 % 32 == 0 and c2 % 32 == 0 and a_ % 64 == 0 and d_ % 4 == 0:
+                y = torch.empty((n, d_, a_, co), dtype=BF16, device=self.device)
+                check(lib().cg_conv3x3_up(_p(coarse), c1, _p(skip), c2, n, d_, a_, _p(self.wb[name]), co, relu, _p(y),
+                                          co, None, None, s), what=f"cg_conv3x3_up {name}")
+                return y
+            t[up] = self._up(coarse, n, *dims[lvl + 1], s)
+            return self._conv(name, t[up], skip, n, d_, a_, s)
+
+        t["dec2a"] = decoder("dec2a", t["mid_b"], t["enc2b"], "u2", 2)
         t["dec2b"] = self._conv("dec2b", t["dec2a"], None, n, *dims[2], s)
-        t["u1"] = self._up(t["dec2b"], n, *dims[2], s)
-        t["dec1a"] = self._conv("dec1a", t["u1"], t["enc1b"], n, *dims[1], s)
+        t["dec1a"] = decoder("dec1a", t["dec2b"], t["enc1b"], "u1", 1)
         t["dec1b"] = self._conv("dec1b", t["dec1a"], None, n, *dims[1], s)
-        t["u0"] = self._up(t["dec1b"], n, *dims[1], s)
-        t["dec0a"] = self._conv("dec0a", t["u0"], t["enc0b"], n, *dims[0], s)
+        t["dec0a"] = decoder("dec0a", t["dec1b"], t["enc0b"], "u0", 0)
         t["dec0b"] = self._conv("dec0b", t["dec0a"], None, n, *dims[0], s)
         out = self._conv("out", t["dec0b"], None, n, *dims[0], s, out32=True)
         return (out, t) if keep else out

@@ -303,6 +303,12 @@ int cg_ocean_plan_set_preconditioner(cg_plan *plan, cg_precond_fn fn, void *user
 int cg_conv3x3(const uint16_t *x1, int c1, const uint16_t *x2, int c2, int n, int down, int across,
                const uint16_t *w, int cout, int relu, uint16_t *y1, int o1, uint16_t *y2, float *y32,
                void *stream);
+/* cg_conv3x3 whose first input x1 is down/2 x across/2 and read through nearest upsampling (the
+ * U-Net decoder's input without the upsampled copy). Needs c1 and c2 multiples of 32, across % 64
+ * == 0 and down % 4 == 0. */
+int cg_conv3x3_up(const uint16_t *x1, int c1, const uint16_t *x2, int c2, int n, int down, int across,
+                  const uint16_t *w, int cout, int relu, uint16_t *y1, int o1, uint16_t *y2, float *y32,
+                  void *stream);
 /* The data gradient of cg_conv3x3 with weights w[9][cout][cin]: dx = conv^T(dy), split as the
  * forward's inputs were (channels [0, c1) into dx1, the rest into dx2), or binary32 into dx32. */
 int cg_conv3x3_adjoint(const uint16_t *dy, int cout, int n, int down, int across, const uint16_t *w, int cin,

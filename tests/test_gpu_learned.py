@@ -173,6 +173,23 @@ def test_unet_gradients_match_cpu_autograd():
         assert abs(float(got.norm() / ref.norm()) - 1.0) < 0.05, name
 
 
+def test_forward_without_keep_reads_the_decoder_input_through_the_upsampling():
+    """The inference forward (the decoder convolutions read their coarse input through nearest
+    upsampling) gives the training forward's output to the bit; cg_conv3x3_up refuses shapes its
+    staged kernel does not serve."""
+    from cartogenesis_amd.learned import UNet
+
+    net = UNet(base=32, seed=5)
+    x = rand_bf16(2, 16, 512, 16, seed=12).cuda()
+    fused = net.forward(x)
+    kept, acts = net.forward(x, keep=True)
+    assert "u0" in acts  # training forms the upsampled tensor for the weight gradient
+    assert torch.equal(fused.cpu(), kept.cpu())
+    p = ctypes.c_void_p(16)
+    assert lib().cg_conv3x3_up(p, 32, p, 16, 1, 16, 512, p, 32, 1, p, 32, None, None, None) < 0  # c2 % 32
+    assert lib().cg_conv3x3_up(p, 32, p, 32, 1, 16, 96, p, 32, 1, p, 32, None, None, None) < 0  # across % 64
+
+
 def test_learned_preconditioner_inside_fgmres():
     """Trains the network briefly on this library's solves of sixteen worlds (256 x 512) and solves
     another with FGMRES, with and without it: both reach 1e-8, and the same answer; prints the gain."""
