@@ -38,12 +38,14 @@ def rand_bf16(*shape, seed=0, scale=1.0):
     return (torch.randn(*shape, generator=g) * scale).to(torch.bfloat16)
 
 
-@pytest.mark.parametrize("c1,c2,cout,relu,a", [(32, 16, 64, 1, 48), (16, 0, 32, 0, 48), (64, 32, 16, 1, 48),
-                                               (32, 64, 96, 0, 48), (32, 32, 64, 1, 128), (64, 0, 32, 0, 192)])
-def test_conv_matches_cpu(c1, c2, cout, relu, a):
-    """Channel counts of 16 take the K = 16 kernel, multiples of 32 the K = 32 one (one 16-pixel tile
-    per wavefront below 64 columns, four from 64)."""
-    n, d = 2, 8
+@pytest.mark.parametrize("c1,c2,cout,relu,a,d", [(32, 16, 64, 1, 48, 8), (16, 0, 32, 0, 48, 8), (64, 32, 16, 1, 48, 8),
+                                                 (32, 64, 96, 0, 48, 8), (32, 32, 64, 1, 128, 8),
+                                                 (64, 0, 32, 0, 192, 8), (32, 32, 64, 1, 128, 6)])
+def test_conv_matches_cpu(c1, c2, cout, relu, a, d):
+    """Channel counts of 16 take the K = 16 kernel; multiples of 32 the LDS-staged one where across
+    % 64 == 0 and down % 4 == 0 (4 rows a workgroup), else the K = 32 one reading L1/L2 (one 16-pixel
+    tile per wavefront below 64 columns, four from 64)."""
+    n = 2
     x1, x2 = rand_bf16(n, d, a, c1, seed=1), rand_bf16(n, d, a, c2, seed=2) if c2 else None
     w = rand_bf16(9, cout, c1 + c2, seed=3, scale=0.1)
     # device copies held in names: a temporary's memory could be handed to the next one
@@ -104,6 +106,20 @@ def test_conv_adjoint_and_weight_gradient_match_cpu_autograd():
     check(lib().cg_conv3x3_wgrad(_p(x1d), c1, _p(x2d), c2, _p(dyd), cout, n, d, a, _p(work), work.numel(), _p(dw),
                                  None), what="cg_conv3x3_wgrad")
     assert (dw.cpu() - wr.grad).abs().max() <= 1e-5 * wr.grad.abs().max()
+    # rows that are not a multiple of 32 wide take the two-byte-gather kernel: the same sums
+    a48 = 48
+    x48 = rand_bf16(n, d, a48, c1 + c2, seed=13)
+    dy48 = rand_bf16(n, d, a48, cout, seed=14)
+    w48 = w.float().requires_grad_(True)
+    (cpu_conv(x48.float(), w48) * dy48.float()).sum().backward()
+    x48d, dy48d = x48.cuda(), dy48.cuda()
+    x481, x482 = x48d[..., :c1].contiguous(), x48d[..., c1:].contiguous()
+    s48 = lib().cg_conv3x3_wgrad_slices(n, d, a48)
+    work48 = torch.empty(s48 * 9 * cout * (c1 + c2), device="cuda")
+    dw48 = torch.empty(9, cout, c1 + c2, device="cuda")
+    check(lib().cg_conv3x3_wgrad(_p(x481), c1, _p(x482), c2, _p(dy48d), cout, n, d, a48, _p(work48), work48.numel(),
+                                 _p(dw48), None), what="cg_conv3x3_wgrad 48")
+    assert (dw48.cpu() - w48.grad).abs().max() <= 1e-5 * w48.grad.abs().max()
     # too small a workspace is refused before anything runs
     assert lib().cg_conv3x3_wgrad(_p(x1d), c1, _p(x2d), c2, _p(dyd), cout, n, d, a, _p(work), 16, _p(dw), None) < 0
 
