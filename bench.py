@@ -211,11 +211,12 @@ BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bfloat16 matrix-core peak
 
 
 def unet_flops(rows: int, cols: int, base: int = 32) -> dict:
-    """Multiply-add flops (2 per product) of each U-Net convolution on a rows x cols grid."""
+    """Multiply-add flops (2 per product) of each U-Net convolution on a rows x cols grid; the first
+    layer's input counts its 3 data channels (mask, depth, vector), which are all it multiplies."""
     from cartogenesis_amd.learned import layer_table
 
     level = {"enc0": 0, "enc1": 1, "enc2": 2, "mid_": 3, "dec2": 2, "dec1": 1, "dec0": 0, "out": 0}
-    return {name: 2 * 9 * (c1 + c2) * co * (rows >> level[name[:4]]) * (cols >> level[name[:4]])
+    return {name: 2 * 9 * (3 if name == "enc0a" else c1 + c2) * co * (rows >> level[name[:4]]) * (cols >> level[name[:4]])
             for name, c1, c2, co, _ in layer_table(base)}
 
 

@@ -77,6 +77,7 @@ SIGNATURES = {
     "cg_ocean_plan_velocities": (_int, [_vp, _dbl, _dbl, _vp, _vp]),
     "cg_ocean_plan_set_preconditioner": (_int, [_vp, PRECOND_FN, _vp]),
     "cg_conv3x3": (_int, [_vp, _int, _vp, _int, _int, _int, _int, _vp, _int, _int, _vp, _int, _vp, _vp, _vp]),
+    "cg_conv3x3_in3": (_int, [_vp, _int, _int, _int, _vp, _int, _int, _vp, _vp]),
     "cg_conv3x3_up": (_int, [_vp, _int, _vp, _int, _int, _int, _int, _vp, _int, _int, _vp, _int, _vp, _vp, _vp]),
     "cg_conv3x3_adjoint": (_int, [_vp, _int, _int, _int, _int, _vp, _int, _vp, _int, _vp, _vp, _vp]),
     "cg_conv3x3_flip_weights": (_int, [_vp, _int, _int, _vp, _vp]),

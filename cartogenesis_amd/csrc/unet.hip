@@ -27,6 +27,10 @@ __device__ inline uint16_t bf16_of(float f) {
     return (uint16_t)(u >> 16);
 }
 __device__ inline float float_of(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+// Two bfloat16 values in one 32-bit word (the lower channel in the low half).
+__device__ inline uint32_t pack2(float lo, float hi) { return (uint32_t)bf16_of(lo) | ((uint32_t)bf16_of(hi) << 16); }
+__device__ inline float lo_of(uint32_t u) { return __uint_as_float(u << 16); }
+__device__ inline float hi_of(uint32_t u) { return __uint_as_float(u & 0xffff0000u); }
 
 __device__ inline int wrap(int c, int across) { return c < 0 ? c + across : (c >= across ? c - across : c); }
 
@@ -417,6 +421,48 @@ __global__ __launch_bounds__(64) void k_conv3x3_wgrad32(const uint16_t *__restri
         for (int j = 0; j < 4; ++j) dst[((long)t * cout + co0 + kq + j) * cin + ci0 + m] = acc[t][j];
 }
 
+// The network's first convolution, whose 16-channel input holds data in channels 0..2 only (mask,
+// depth, vector; the rest zero by construction): one thread per pixel, the 27 products per output
+// channel in binary32 on the vector units (the matrix cores would multiply 13 zero channels), the
+// weights w[9][cout][16] of channels 0..2 staged in LDS as binary32. cout <= 32.
+constexpr int kIn3MaxOut = 32;
+__global__ __launch_bounds__(256) void k_conv3x3_in3(const uint16_t *__restrict__ x, int n, int down, int across,
+                                                     const uint16_t *__restrict__ w, int cout, int relu,
+                                                     uint16_t *__restrict__ y) {
+    __shared__ float sw[9 * 3 * kIn3MaxOut];  // [tap][channel][out]
+    for (int i = threadIdx.x; i < 9 * 3 * cout; i += blockDim.x) {
+        const int t = i / (3 * cout), c = (i / cout) % 3, o = i % cout;
+        sw[(t * 3 + c) * kIn3MaxOut + o] = float_of(w[((long)t * cout + o) * 16 + c]);
+    }
+    __syncthreads();
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long)n * down * across) return;
+    const long row_id = p / across;
+    const int col = (int)(p - row_id * across), row = (int)(row_id % down);
+    const long img = row_id / down;
+    float in[9][3];
+    for (int t = 0; t < 9; ++t) {
+        const int rr = row + t / 3 - 1;
+        uint2 v = {0u, 0u};
+        if (rr >= 0 && rr < down)
+            v = *(const uint2 *)(x + ((img * down + rr) * across + wrap(col + t % 3 - 1, across)) * 16);
+        in[t][0] = lo_of(v.x);
+        in[t][1] = hi_of(v.x);
+        in[t][2] = lo_of(v.y);
+    }
+    for (int o0 = 0; o0 < cout; o0 += 8) {
+        float acc[8];
+        for (int k = 0; k < 8; ++k) acc[k] = 0.f;
+        for (int t = 0; t < 9; ++t)
+            for (int c = 0; c < 3; ++c)
+                for (int k = 0; k < 8; ++k) acc[k] = fmaf(in[t][c], sw[(t * 3 + c) * kIn3MaxOut + o0 + k], acc[k]);
+        if (relu)
+            for (int k = 0; k < 8; ++k) acc[k] = acc[k] > 0.f ? acc[k] : 0.f;
+        *(uint4 *)(y + p * cout + o0) =
+            (uint4){pack2(acc[0], acc[1]), pack2(acc[2], acc[3]), pack2(acc[4], acc[5]), pack2(acc[6], acc[7])};
+    }
+}
+
 // dw[i] = sum over slices s of part[s][i], in slice order (deterministic).
 __global__ void k_wgrad_sum(const float *__restrict__ part, int slices, long count, float *__restrict__ dw) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -461,9 +507,6 @@ __global__ void k_upsample2(const uint16_t *__restrict__ x, int ch, int n, int d
 }
 
 // The same two steps eight channels (16 bytes) a thread, for channel counts divisible by 8.
-__device__ inline uint32_t pack2(float lo, float hi) { return (uint32_t)bf16_of(lo) | ((uint32_t)bf16_of(hi) << 16); }
-__device__ inline float lo_of(uint32_t u) { return __uint_as_float(u << 16); }
-__device__ inline float hi_of(uint32_t u) { return __uint_as_float(u & 0xffff0000u); }
 
 __global__ void k_pool2_v8(const uint16_t *__restrict__ x, int ch, int n, int down, int across, float scale,
                            uint16_t *__restrict__ y) {
@@ -634,6 +677,16 @@ int cg_conv3x3(const uint16_t *x1, int c1, const uint16_t *x2, int c2, int n, in
     if (!y32 && (!y1 || o1 < 0 || o1 > cout || o1 % 16 || (o1 < cout && !y2))) return CG_ERR_ARG;
     return conv_launch<false>(x1, c1, c2 ? x2 : nullptr, c2, n, down, across, w, cout, relu, y1, y32 ? cout : o1, y2,
                               y32, (hipStream_t)stream);
+}
+
+int cg_conv3x3_in3(const uint16_t *x, int n, int down, int across, const uint16_t *w, int cout, int relu, uint16_t *y,
+                   void *stream) {
+    if (!x || !w || !y || n <= 0 || down <= 0 || across <= 0 || cout <= 0 || cout > kIn3MaxOut || cout % 8)
+        return CG_ERR_ARG;
+    const long pixels = (long)n * down * across;
+    hipLaunchKernelGGL(k_conv3x3_in3, dim3(blocks(pixels, 256)), dim3(256), 0, (hipStream_t)stream, x, n, down, across,
+                       w, cout, relu, y);
+    return hipGetLastError() == hipSuccess ? CG_OK : CG_ERR_DEVICE;
 }
 
 int cg_conv3x3_up(const uint16_t *x1, int c1, const uint16_t *x2, int c2, int n, int down, int across,

@@ -107,7 +107,13 @@ class UNet:
         s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         t = {"x": x}
         dims = [(d >> k, a >> k) for k in range(4)]
-        t["enc0a"] = self._conv("enc0a", x, None, n, *dims[0], s)
+        c1, _, co0, relu0 = self.spec["enc0a"]
+        if c1 == 16 and co0 <= 32 and co0 % 8 == 0:  # the input's channels 3..15 are zero by construction
+            t["enc0a"] = torch.empty((n, d, a, co0), dtype=BF16, device=self.device)
+            check(lib().cg_conv3x3_in3(_p(x), n, d, a, _p(self.wb["enc0a"]), co0, relu0, _p(t["enc0a"]), s),
+                  what="cg_conv3x3_in3 enc0a")
+        else:
+            t["enc0a"] = self._conv("enc0a", x, None, n, *dims[0], s)
         t["enc0b"] = self._conv("enc0b", t["enc0a"], None, n, *dims[0], s)
         t["p1"] = self._pool(t["enc0b"], n, *dims[0], s)
         t["enc1a"] = self._conv("enc1a", t["p1"], None, n, *dims[1], s)

@@ -303,6 +303,11 @@ int cg_ocean_plan_set_preconditioner(cg_plan *plan, cg_precond_fn fn, void *user
 int cg_conv3x3(const uint16_t *x1, int c1, const uint16_t *x2, int c2, int n, int down, int across,
                const uint16_t *w, int cout, int relu, uint16_t *y1, int o1, uint16_t *y2, float *y32,
                void *stream);
+/* cg_conv3x3 of a 16-channel input whose channels 3..15 are zero (the network's input: mask, depth,
+ * vector), reading channels 0..2 only, in binary32 on the vector units; w is [9][cout][16], cout a
+ * multiple of 8 up to 32; bfloat16 output. */
+int cg_conv3x3_in3(const uint16_t *x, int n, int down, int across, const uint16_t *w, int cout, int relu, uint16_t *y,
+                   void *stream);
 /* cg_conv3x3 whose first input x1 is down/2 x across/2 and read through nearest upsampling (the
  * U-Net decoder's input without the upsampled copy). Needs c1 and c2 multiples of 32, across % 64
  * == 0 and down % 4 == 0. */

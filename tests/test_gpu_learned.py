@@ -67,6 +67,26 @@ def test_conv_matches_cpu(c1, c2, cout, relu, a, d):
     assert torch.equal(both, got.to(torch.bfloat16))
 
 
+@pytest.mark.parametrize("cout,relu", [(32, 1), (16, 0)])
+def test_first_layer_kernel_matches_cpu(cout, relu):
+    """cg_conv3x3_in3 (channels 0..2 of a 16-channel input; the others zero) against the CPU
+    convolution: binary32 sums of 27 products, rounded once to bfloat16."""
+    n, d, a = 2, 8, 40
+    x = torch.zeros(n, d, a, 16)
+    x[..., :3] = torch.randn(n, d, a, 3, generator=torch.Generator().manual_seed(15))
+    x = x.to(torch.bfloat16)
+    w = rand_bf16(9, cout, 16, seed=16, scale=0.2)
+    xd, wd = x.cuda(), w.cuda()
+    y = torch.empty(n, d, a, cout, dtype=torch.bfloat16, device="cuda")
+    check(lib().cg_conv3x3_in3(_p(xd), n, d, a, _p(wd), cout, relu, _p(y), None), what="cg_conv3x3_in3")
+    ref = cpu_conv(x.float(), w.float(), bool(relu))
+    got = y.cpu().float()
+    assert ((got - ref).abs() <= 2.0 ** -8 * ref.abs() + 1e-6 * ref.abs().max()).all()
+    p = ctypes.c_void_p(16)
+    assert lib().cg_conv3x3_in3(p, 1, 8, 32, p, 48, 1, p, None) < 0 and lib().cg_conv3x3_in3(p, 1, 8, 32, p, 12, 1, p,
+                                                                                               None) < 0
+
+
 def test_conv_adjoint_and_weight_gradient_match_cpu_autograd():
     n, d, a, c1, c2, cout = 2, 8, 32, 32, 16, 64
     x = rand_bf16(n, d, a, c1 + c2, seed=4)
