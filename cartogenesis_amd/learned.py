@@ -150,7 +150,7 @@ class UNet:
     # ---- backward ------------------------------------------------------------------------------
     def workspace_floats(self, n: int, down: int, across: int) -> int:
         most = 0
-        for k, (name, c1, c2, co, _) in enumerate(self.layers):
+        for name, c1, c2, co, _ in self.layers:
             lvl = {"enc0": 0, "enc1": 1, "enc2": 2, "mid_": 3, "dec2": 2, "dec1": 1, "dec0": 0, "out": 0}[name[:4]]
             d, a = down >> lvl, across >> lvl
             most = max(most, lib().cg_conv3x3_wgrad_slices(n, d, a) * 9 * co * (c1 + c2))
