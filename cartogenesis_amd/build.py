@@ -41,18 +41,20 @@ def run(cmd: list[str]) -> None:
 
 
 def build(verbose: bool = False) -> Path:
-    """Compiles every source and links ``libcgocean.so``; always rebuilds."""
+    """Compiles every source (the translation units side by side) and links ``libcgocean.so``;
+    always rebuilds."""
+    from concurrent.futures import ThreadPoolExecutor
+
     obj_dir = PKG / "_build"
     obj_dir.mkdir(exist_ok=True)
-    objs = []
+    jobs = []
     for src in CXX_SOURCES:
-        obj = obj_dir / (src + ".o")
-        run(["g++", *COMMON, "-pthread", "-c", str(CSRC / src), "-o", str(obj)])
-        objs.append(obj)
+        jobs.append(["g++", *COMMON, "-pthread", "-c", str(CSRC / src), "-o", str(obj_dir / (src + ".o"))])
     for src in HIP_SOURCES:
-        obj = obj_dir / (src + ".o")
-        run([hipcc(), *COMMON, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
-        objs.append(obj)
+        jobs.append([hipcc(), *COMMON, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(obj_dir / (src + ".o"))])
+    objs = [obj_dir / (src + ".o") for src in CXX_SOURCES + HIP_SOURCES]
+    with ThreadPoolExecutor(max_workers=len(jobs)) as pool:
+        list(pool.map(run, jobs))
     tmp = LIB.with_suffix(".so.tmp")
     run([hipcc(), "-shared", "-fPIC", "-pthread", "--offload-arch=gfx950", *map(str, objs), "-o", str(tmp)])
     os.replace(tmp, LIB)

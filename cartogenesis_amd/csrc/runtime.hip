@@ -91,6 +91,11 @@ struct cg_ctx {
     int fused = 1;
     int fuse_outer = 1;  // the finest up leg and the binary64 outer step in one launch
     int direct_bottom = 1;  // the binary32 cycle's bottom grid solved by its dense inverse
+    // The binary64-refined methods stop once the residual is under tol x stop_permille / 1000: the
+    // answer's error against the exact solution of the reference's operator keeps falling with every
+    // step after the residual first passes tol, and stopping a third of the way below it keeps that
+    // error two orders of magnitude under tol.
+    int stop_permille = 333;
     // Passes per level of the fused binary32 cycle, 4 bits per level from level 0 up (2, 4 or 6;
     // levels past the eighth and the bottom's run 2).
     long level_passes = 0x22222222;
@@ -1295,9 +1300,10 @@ int refine_step(cg_plan *p, bool timed) {
 // parity suffices.
 int build_step_graphs(cg_plan *p) {
     cg_ctx *ctx = p->ctx;
-    // The settings that shape a step; graphs captured under others are rebuilt.
-    const long key = ((long)ctx->level_passes << 4) | (ctx->fused << 3) | (ctx->fuse_outer << 2) |
-                     (ctx->direct_bottom << 1);
+    // The settings that shape a step, the context's tile rows among them (grids the plan has no
+    // tuned height for take theirs); graphs captured under others are rebuilt.
+    const long key = ((long)ctx->level_passes << 12) | ((long)ctx->tile_rows << 4) | (ctx->fused << 3) |
+                     (ctx->fuse_outer << 2) | (ctx->direct_bottom << 1);
     if (p->graph_key != key) {
         for (int i = 0; i < 2; ++i)
             if (p->step_exec[i]) {
@@ -1346,9 +1352,10 @@ int build_step_graphs(cg_plan *p) {
 // The refined method over every instance of the plan; infos[i] receives instance i's result.
 // Instance i stops at the first check where its residual is under tol (or not a number); with
 // several instances its kernels then skip it (the active flags) while the others go on.
-int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *infos, const volatile int *cancel) {
+int refined_solve(cg_plan *p, double tol_asked, bool warm, cg_solution *infos, const volatile int *cancel) {
     cg_ctx *ctx = p->ctx;
     const int K = p->K;
+    const double tol = tol_asked * (ctx->stop_permille / 1000.0);
     const Stencil *st = p->lv[0].st;
     std::vector<double> largest(K), res(K), rate(K, -1.0);
     std::vector<int> cycles(K, 0);
@@ -1512,9 +1519,10 @@ int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *infos, const v
 // of the cycle (it sees the outer residual itself, the kind of input it was trained on; the cycles
 // of the later iterations and the least-squares step build on its direction), until an outer step
 // fails to halve the residual; from then on every iteration is a plain cycle.
-int fgmres_solve(cg_plan *p, double tol, bool warm, cg_solution *info, const volatile int *cancel,
+int fgmres_solve(cg_plan *p, double tol_asked, bool warm, cg_solution *info, const volatile int *cancel,
                  bool learned = false) {
     cg_ctx *ctx = p->ctx;
+    const double tol = tol_asked * (ctx->stop_permille / 1000.0);
     hipStream_t s = ctx->stream;
     const Stencil *st = p->lv[0].st;
     const int M = std::min(ctx->fgmres_m, kFgmresMax);
@@ -1694,6 +1702,13 @@ int cg_ocean_set_option(cg_ctx *ctx, int option, long value) {
         case CG_OPT_FUSED: ctx->fused = value ? 1 : 0; return CG_OK;
         case CG_OPT_FUSE_OUTER: ctx->fuse_outer = value ? 1 : 0; return CG_OK;
         case CG_OPT_DIRECT_BOTTOM: ctx->direct_bottom = value ? 1 : 0; return CG_OK;
+        case CG_OPT_STOP_PERMILLE:
+            if (value < 1 || value > 1000) {
+                set_err(ctx, "the refined methods' stopping share must be 1..1000 per mille of tol");
+                return CG_ERR_ARG;
+            }
+            ctx->stop_permille = (int)value;
+            return CG_OK;
         case CG_OPT_LEVEL_PASSES:
             for (int l = 0; l < 8; ++l) {
                 const long v = (value >> (4 * l)) & 0xf;
@@ -1719,6 +1734,7 @@ long cg_ocean_get_option(cg_ctx *ctx, int option) {
         case CG_OPT_FUSED: return ctx->fused;
         case CG_OPT_FUSE_OUTER: return ctx->fuse_outer;
         case CG_OPT_DIRECT_BOTTOM: return ctx->direct_bottom;
+        case CG_OPT_STOP_PERMILLE: return ctx->stop_permille;
         case CG_OPT_LEVEL_PASSES: return ctx->level_passes;
         default: return -1;
     }

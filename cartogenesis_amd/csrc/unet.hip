@@ -709,6 +709,9 @@ int cg_conv3x3_in3(const uint16_t *x, int n, int down, int across, const uint16_
                    void *stream) {
     if (!x || !w || !y || n <= 0 || down <= 0 || across <= 0 || cout <= 0 || cout > kIn3MaxOut || cout % 8)
         return CG_ERR_ARG;
+    // The kernel reads each input pixel's channels 0..3 as one 8-byte load and writes outputs 16 bytes
+    // at a time.
+    if (reinterpret_cast<uintptr_t>(x) % 8 || reinterpret_cast<uintptr_t>(y) % 16) return CG_ERR_ARG;
     const long pixels = (long)n * down * across;
     hipLaunchKernelGGL(k_conv3x3_in3, dim3(blocks(pixels, 256)), dim3(256), 0, (hipStream_t)stream, x, n, down, across,
                        w, cout, relu, y);

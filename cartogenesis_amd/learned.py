@@ -99,8 +99,10 @@ class UNet:
         return y
 
     def forward(self, x: torch.Tensor, stream=None, keep: bool = False):
-        """x: (n, down, across, 16) bfloat16; down a multiple of 8 and across of 128. Returns the
-        (n, down, across, 16) binary32 output, and with keep the activations for backward()."""
+        """x: (n, down, across, 16) bfloat16; down a multiple of 8 and across of 128; channels 3..15
+        must be zero (the first layer reads channels 0..2 only: mask, depth, vector, as pack_input
+        writes them). Returns the (n, down, across, 16) binary32 output, and with keep the
+        activations for backward()."""
         n, d, a, _ = x.shape
         if d % 8 or a % 128:
             raise ValueError("the U-Net needs down % 8 == 0 and across % 128 == 0")

@@ -80,6 +80,10 @@ extern "C" {
                                     grid and the coarser ones are cycled in one workgroup down to the
                                     coarsest. The reference method and the Krylov preconditioner
                                     always cycle the whole ladder (OC:740-781). */
+#define CG_OPT_STOP_PERMILLE 11  /* CG_METHOD_REFINED, FGMRES and LEARNED stop once the relative residual is
+                                    under tol x value / 1000 (1..1000, default 333): past tol each binary64
+                                    step still lowers the answer's error against the exact solution, and a
+                                    third of tol keeps it near 1e-2 x tol (1e-10 at tol 1e-8, 1024 x 2048) */
 
 typedef struct cg_ctx cg_ctx;
 typedef struct cg_plan cg_plan;
@@ -304,8 +308,10 @@ int cg_conv3x3(const uint16_t *x1, int c1, const uint16_t *x2, int c2, int n, in
                const uint16_t *w, int cout, int relu, uint16_t *y1, int o1, uint16_t *y2, float *y32,
                void *stream);
 /* cg_conv3x3 of a 16-channel input whose channels 3..15 are zero (the network's input: mask, depth,
- * vector), reading channels 0..2 only, in binary32 on the vector units; w is [9][cout][16], cout a
- * multiple of 8 up to 32; bfloat16 output. */
+ * vector), reading channels 0..2 only, in binary32 on the vector units: values in channels 3..15 are
+ * ignored, so the result equals cg_conv3x3's only when they are zero. w is [9][cout][16], cout a
+ * multiple of 8 up to 32; bfloat16 output. x must be 8-byte and y 16-byte aligned (CG_ERR_ARG
+ * otherwise). */
 int cg_conv3x3_in3(const uint16_t *x, int n, int down, int across, const uint16_t *w, int cout, int relu, uint16_t *y,
                    void *stream);
 /* cg_conv3x3 whose first input x1 is down/2 x across/2 and read through nearest upsampling (the
