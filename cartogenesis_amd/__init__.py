@@ -53,6 +53,7 @@ SIGNATURES = {
                                        ctypes.POINTER(_u64)]),
     "cg_ocean_stencil_release": (_int, [_vp, _u64]),
     "cg_ocean_relax": (_int, [_vp, _u64, _f32p, _f32p, _f32p, _int, _vp]),
+    "cg_ocean_relax_noalloc": (_int, [_vp, _u64, _vp, _vp, _vp, _int, _vp]),
     "cg_ocean_solve": (_int, [_vp, ctypes.POINTER(_u64), _int, _f32p, _vp, _dbl, _int, _int, _vp, _vp,
                               ctypes.POINTER(CgSolution), _vp]),
     "cg_ocean_plan_create": (_int, [_vp, ctypes.POINTER(_u64), _int, _int, ctypes.POINTER(_vp)]),

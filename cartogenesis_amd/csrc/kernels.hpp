@@ -683,6 +683,16 @@ __global__ void k_fill_f32(float *p, long n, float v) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
+// dst[i] = src[i] for i < n and 0 for n <= i < total: a grid copied into a buffer that carries the
+// smoother's all-zero row after the last.
+__global__ void k_stage_grid(const float *__restrict__ src, float *__restrict__ dst, long n, long total) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < total) dst[i] = i < n ? src[i] : 0.0f;
+}
+__global__ void k_copy_f32(const float *__restrict__ src, float *__restrict__ dst, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
 __global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;

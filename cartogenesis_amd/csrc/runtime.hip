@@ -77,8 +77,19 @@ constexpr int kKindSmooth = 1;
 
 }  // namespace
 
+// The allocation-free relaxation's scratch (cg_ocean_relax_noalloc): the forcing and the two value
+// buffers of the ping-pong, each a grid of up to kScratchCells cells plus one all-zero row of up to
+// kScratchAcross columns, in one device array that every GPU the library runs on holds until the
+// process ends (25.2 MB). Host code never passes `scratch` itself to a kernel: its address on each
+// device comes from hipGetSymbolAddress when a context opens on that device.
+constexpr long kScratchCells = 1L << 21;   // 1024 x 2048
+constexpr long kScratchAcross = 4096;
+constexpr long kScratchGrid = kScratchCells + kScratchAcross;
+static __device__ __align__(256) float scratch[3 * kScratchGrid];
+
 struct cg_ctx {
     int device = 0;
+    float *scratch = nullptr;  // this device's address of `scratch`
     hipStream_t stream = nullptr;
     std::recursive_mutex mu;
     std::string err;
@@ -248,24 +259,27 @@ hipError_t launch_tile(hipStream_t s, const LevelArgs &L, const float *f, const 
 
 hipError_t tile_dispatch(cg_ctx *ctx, const LevelArgs &L, const float *f, const float *xin, float *xout, int nh,
                          bool zero, int mode = TILE_PLAIN, const TileExtra &ex = TileExtra{}, int rows = 0,
-                         int instances = 1) {
+                         int instances = 1, hipStream_t stream = nullptr) {
+    hipStream_t s = stream ? stream : ctx->stream;
     switch (rows ? rows : ctx->tile_rows) {
-        case 2: return launch_tile<2>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex, instances);
-        case 4: return launch_tile<4>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex, instances);
-        case 6: return launch_tile<6>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex, instances);
-        case 8: return launch_tile<8>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex, instances);
-        case 24: return launch_tile<24>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex, instances);
-        case 32: return launch_tile<32>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex, instances);
-        default: return launch_tile<16>(ctx->stream, L, f, xin, xout, nh, zero, mode, ex, instances);
+        case 2: return launch_tile<2>(s, L, f, xin, xout, nh, zero, mode, ex, instances);
+        case 4: return launch_tile<4>(s, L, f, xin, xout, nh, zero, mode, ex, instances);
+        case 6: return launch_tile<6>(s, L, f, xin, xout, nh, zero, mode, ex, instances);
+        case 8: return launch_tile<8>(s, L, f, xin, xout, nh, zero, mode, ex, instances);
+        case 24: return launch_tile<24>(s, L, f, xin, xout, nh, zero, mode, ex, instances);
+        case 32: return launch_tile<32>(s, L, f, xin, xout, nh, zero, mode, ex, instances);
+        default: return launch_tile<16>(s, L, f, xin, xout, nh, zero, mode, ex, instances);
     }
 }
 
 // Runs `passes` passes on level values held in a ping-pong pair x[0], x[1] with x[*cur] current;
 // on return *cur names the buffer holding the result. zero_start: the current values are all zero
-// and need not be read. timed: record the launches under kernel kind 0.
+// and need not be read. timed: record the launches under kernel kind 0. stream: where to enqueue
+// (nullptr: the context's stream).
 int relax_pp(cg_ctx *ctx, const Stencil *st, const float *f, float *x[2], int *cur, int passes, bool zero_start,
-             bool timed) {
+             bool timed, hipStream_t stream = nullptr) {
     if (passes <= 0) return CG_OK;
+    hipStream_t s = stream ? stream : ctx->stream;
     const LevelArgs L = st->args();
     if (st->rowconst && st->across >= 2 && (st->across & 1) == 0) {
         int left = passes;
@@ -273,7 +287,7 @@ int relax_pp(cg_ctx *ctx, const Stencil *st, const float *f, float *x[2], int *c
         while (left > 0) {
             const int nh = left >= 2 ? 4 : 2;
             TimeScope ts(ctx, timed && nh == 4, zero ? kKindSmoothFromZero : kKindSmooth);
-            hipError_t e = tile_dispatch(ctx, L, f, x[*cur], x[1 - *cur], nh, zero);
+            hipError_t e = tile_dispatch(ctx, L, f, x[*cur], x[1 - *cur], nh, zero, TILE_PLAIN, TileExtra{}, 0, 1, s);
             if (e != hipSuccess) {
                 set_err(ctx, "smoother launch failed: %s", hipGetErrorString(e));
                 return CG_ERR_DEVICE;
@@ -285,14 +299,12 @@ int relax_pp(cg_ctx *ctx, const Stencil *st, const float *f, float *x[2], int *c
         return CG_OK;
     }
     if (zero_start) {
-        hipLaunchKernelGGL(k_fill_f32, dim3(blocks_for(st->cells, 256)), dim3(256), 0, ctx->stream, x[*cur], st->cells,
-                           0.0f);
+        hipLaunchKernelGGL(k_fill_f32, dim3(blocks_for(st->cells, 256)), dim3(256), 0, s, x[*cur], st->cells, 0.0f);
     }
     const long half = (long)(st->across / 2) * st->down;
     for (int p = 0; p < passes; ++p)
         for (int colour = 0; colour <= 1; ++colour)
-            hipLaunchKernelGGL(k_relax_colour, dim3(blocks_for(half, 256)), dim3(256), 0, ctx->stream, L, f, x[*cur],
-                               colour);
+            hipLaunchKernelGGL(k_relax_colour, dim3(blocks_for(half, 256)), dim3(256), 0, s, L, f, x[*cur], colour);
     HIPCHK(ctx, hipGetLastError());
     return CG_OK;
 }
@@ -1632,8 +1644,27 @@ int cg_ocean_open(int device, cg_ctx **out, char *reason, size_t reason_len) {
         return fail(msg);
     }
     if (hipSetDevice(device) != hipSuccess) return fail("hipSetDevice failed");
+    // The scratch array's address on this device, looked up once per device (each device holds its
+    // own copy of the array).
+    static std::mutex scratch_mu;
+    static std::map<int, float *> scratch_at;
+    float *scratch_ptr = nullptr;
+    {
+        std::lock_guard<std::mutex> g(scratch_mu);
+        auto it = scratch_at.find(device);
+        if (it != scratch_at.end()) {
+            scratch_ptr = it->second;
+        } else {
+            if (hipGetSymbolAddress((void **)&scratch_ptr, HIP_SYMBOL(scratch)) != hipSuccess || !scratch_ptr) {
+                (void)hipGetLastError();
+                return fail("the device address of the relaxation scratch could not be read");
+            }
+            scratch_at[device] = scratch_ptr;
+        }
+    }
     cg_ctx *ctx = new cg_ctx();
     ctx->device = device;
+    ctx->scratch = scratch_ptr;
     ctx->name = prop.name;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
@@ -1890,6 +1921,56 @@ int cg_ocean_relax(cg_ctx *ctx, cg_stencil h, const float *forcing, const float 
     }
     HIPCHK(ctx, hipMemcpyAsync(out, s->xbuf[cur], sizeof(float) * s->cells, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return CG_OK;
+}
+
+int cg_ocean_relax_noalloc(cg_ctx *ctx, cg_stencil h, const float *forcing, const float *start, float *out,
+                           int passes, void *stream) {
+    if (!ctx) return CG_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    Stencil *st = find_stencil(ctx, h);
+    if (!st) {
+        set_err(ctx, "no stencil %llu", (unsigned long long)h);
+        return CG_ERR_ARG;
+    }
+    if (st->across <= 0 || st->across % 2 != 0 || st->down <= 0 || passes < 0) return CG_DECLINED;
+    if (st->cells < ctx->min_cells) return CG_DECLINED;
+    if (!forcing || !start || !out) {
+        set_err(ctx, "relax_noalloc: null array");
+        return CG_ERR_ARG;
+    }
+    if (st->cells > kScratchCells || st->across > kScratchAcross) {
+        set_err(ctx, "relax_noalloc: %d x %d exceeds the scratch's %ld cells of at most %ld columns", st->across,
+                st->down, kScratchCells, kScratchAcross);
+        return CG_ERR_ARG;
+    }
+    if (!ctx->scratch) {
+        set_err(ctx, "relax_noalloc: the context has no scratch address");
+        return CG_ERR_STATE;
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    const long n = st->cells, total = n + st->across;
+    if (passes == 0) {
+        hipLaunchKernelGGL(k_copy_f32, dim3(blocks_for(n, 256)), dim3(256), 0, s, start, out, n);
+        HIPCHK(ctx, hipGetLastError());
+        return CG_OK;
+    }
+    // The scratch as three grids with their zero rows: the forcing, then the ping-pong pair. Every
+    // call writes all the rows the kernels read (the zero rows included), since the array keeps
+    // whatever the previous call, of any size, left in it.
+    float *f = ctx->scratch;
+    float *x[2] = {ctx->scratch + kScratchGrid, ctx->scratch + 2 * kScratchGrid};
+    hipLaunchKernelGGL(k_stage_grid, dim3(blocks_for(total, 256)), dim3(256), 0, s, forcing, f, n, total);
+    hipLaunchKernelGGL(k_stage_grid, dim3(blocks_for(total, 256)), dim3(256), 0, s, start, x[0], n, total);
+    hipLaunchKernelGGL(k_fill_f32, dim3(blocks_for(st->across, 256)), dim3(256), 0, s, x[1] + n, (long)st->across,
+                       0.0f);
+    HIPCHK(ctx, hipGetLastError());
+    int cur = 0;
+    int rc = relax_pp(ctx, st, f, x, &cur, passes, false, false, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_copy_f32, dim3(blocks_for(n, 256)), dim3(256), 0, s, x[cur], out, n);
+    HIPCHK(ctx, hipGetLastError());
     return CG_OK;
 }
 

@@ -134,6 +134,18 @@ int cg_ocean_stencil_release(cg_ctx *ctx, cg_stencil stencil);
 int cg_ocean_relax(cg_ctx *ctx, cg_stencil stencil, const float *forcing, const float *start,
                    float *out, int passes, const volatile int *cancel);
 
+/* No counterpart in the reference: cg_ocean_relax for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). forcing,
+ * start and out are DEVICE pointers of the stencil's cells (out may alias start); the work is only
+ * enqueued on `stream` (a hipStream_t; NULL: the context's stream), and the call allocates nothing:
+ * the forcing and the two value buffers of the ping-pong live in a static device array of the
+ * library (25.2 MB per GPU, held until the process ends), so grids of at most 2^21 cells and 4096
+ * columns are accepted (CG_ERR_ARG beyond). Declines as cg_ocean_relax does; no cancellation. Every
+ * call on one GPU shares that array: calls that overlap on one GPU (on two streams or two contexts)
+ * race, and the caller must order them. */
+int cg_ocean_relax_noalloc(cg_ctx *ctx, cg_stencil stencil, const float *forcing, const float *start,
+                           float *out, int passes, void *stream);
+
 /* ---- whole solves --------------------------------------------------------------------------- */
 
 /* OceanCirculation.solve / solveWithKrylovRescue (OC:473-528) on the device, over a ladder of
