@@ -53,26 +53,70 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(levels, budget_s: float, threads: int):
-    """The oracle's binary64 twin of the reference multigrid (C, OpenMP row bands) on the host,
-    solving the same 1024 x 2048 problem to 1e-8, repeated until `budget_s` of CPU time."""
-    import oracle
+def host_cores():
+    """(nproc, lscpu model name, threads used): the threads are the OpenMP share the host allots to
+    one GPU (OMP_NUM_THREADS) when it sets one, else every core."""
+    nproc = os.cpu_count() or 1
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return nproc, model, (min(share, nproc) if share > 0 else nproc)
 
-    oracle.set_threads(threads)
-    ladder = [oracle.Stencil(s.cells_across, s.cells_down, s.is_water, s.east_weight, s.west_weight, s.north_weight,
-                             s.south_weight, s.forcing, s.centre_weight) for s in levels]
-    times = []
+
+def _timed_solves(fn, budget_s: float, most: int):
+    times, last = [], None
     t_all = time.perf_counter()
     while True:
         t0 = time.perf_counter()
-        _, cycles, rr = oracle.solve64(ladder, 1e-8)
+        last = fn()
         times.append(time.perf_counter() - t0)
-        if time.perf_counter() - t_all > budget_s or len(times) >= 20:
+        if time.perf_counter() - t_all > budget_s or len(times) >= most:
             break
-    value = len(times) / sum(times)
+    return len(times) / sum(times), len(times), last
+
+
+def cpu_baseline(levels, levels256, budget_s: float, threads: int, nproc: int, model):
+    """The oracle (the reference algorithm restated in C: OceanCirculation's levels, V(2,2), row-band
+    threads like parallelChunks) on the host. The headline figure: its binary64 twin solving the
+    benchmark's 1024 x 2048 problem to 1e-8 with `threads` OpenMP threads, repeated until budget_s.
+    Beside it, bounded samples of: the same on one thread; the reference method itself (binary32
+    values, binary64 residual) to the reference's own tolerance 1e-3 at 1024 x 2048, on all threads
+    and one; the binary64 twin to 1e-10 at 256 x 512 (seed 0), on all threads and one."""
+    import oracle
+
+    def ladder_of(lv):
+        return [oracle.Stencil(s.cells_across, s.cells_down, s.is_water, s.east_weight, s.west_weight, s.north_weight,
+                               s.south_weight, s.forcing, s.centre_weight) for s in lv]
+
+    big, small = ladder_of(levels), ladder_of(levels256)
+    zero = np.zeros(big[0].cells, np.float32)
+    legs = {}
+    oracle.set_threads(threads)
+    value, count, (_, cycles, _) = _timed_solves(lambda: oracle.solve64(big, 1e-8), budget_s, 20)
+    for nt in (threads, 1):
+        oracle.set_threads(nt)
+        tag = "all" if nt == threads else "one_thread"
+        if nt == 1:
+            v1, c1, _ = _timed_solves(lambda: oracle.solve64(big, 1e-8), budget_s / 2, 3)
+            legs["binary64_1e-8_1024x2048_one_thread"] = {"solves_per_s": round(v1, 4), "solves": c1}
+        vr, cr, (_, cyc_r, rr) = _timed_solves(lambda: oracle.solve(big, zero, 1e-3), budget_s / 4, 5)
+        legs[f"reference_method_1e-3_1024x2048_{tag}"] = {"solves_per_s": round(vr, 4), "solves": cr,
+                                                          "cycles": cyc_r, "relative_residual": rr}
+        vs, cs_, (_, cyc_s, _) = _timed_solves(lambda: oracle.solve64(small, 1e-10), budget_s / 4, 20)
+        legs[f"binary64_1e-10_256x512_{tag}"] = {"solves_per_s": round(vs, 4), "solves": cs_, "cycles": cyc_s}
+    oracle.set_threads(threads)
     return {"value": round(value, 4), "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": f"{len(times)} full 1024x2048 solves to 1e-8 ({cycles} V(2,2) cycles each) by the oracle's "
-                      f"binary64 C restatement of OceanCirculation.solve, {threads} OpenMP threads"}
+            "nproc": nproc, "cpu_model": model,
+            "sample": f"{count} full 1024x2048 solves to 1e-8 ({cycles} V(2,2) cycles each) by the oracle's "
+                      f"binary64 C restatement of OceanCirculation.solve, {threads} OpenMP threads (the host's "
+                      f"per-GPU share of its {nproc} cores)",
+            "legs": legs}
 
 
 def batched_year(acc, prob, tol: float, reps: int, warmup: int, autotune: bool = True):
@@ -332,9 +376,9 @@ def main() -> int:
                     help="passes per level of the binary32 cycle, one hex digit per level, finest last")
     ap.add_argument("--batch-reps", type=int, default=5,
                     help="timed repetitions of the batched year (12 monthly forcings solved together); 0 skips it")
-    ap.add_argument("--combined", action="store_true",
-                    help="also time 64 instances: 16 monthly forcings each at 256x512, 512x1024, 1024x2048 and "
-                         "2048x4096, one plan of 16 per size")
+    ap.add_argument("--combined", action=argparse.BooleanOptionalAction, default=True,
+                    help="time 64 instances: 16 monthly forcings each at 256x512, 512x1024, 1024x2048 and "
+                         "2048x4096, one plan of 16 per size (default on)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned preconditioner's leg")
     ap.add_argument("--learned-only", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--conv-only", action="store_true",
@@ -344,9 +388,9 @@ def main() -> int:
     ap.add_argument("--learned-rows", type=int, default=256, help="training grid rows (columns = 2 rows)")
     ap.add_argument("--learned-seeds", type=int, default=16, help="training worlds (seeds 100, 101, ...)")
     ap.add_argument("--learned-steps", type=int, default=300, help="training steps (batch 4)")
-    ap.add_argument("--worlds", action="store_true",
-                    help="also time the four-world year: 12 months on each of 512x1024, 768x1536, 1024x2048 and "
-                         "1024x2048 (seeds 1-4, land 20/27/33/40%%)")
+    ap.add_argument("--worlds", action=argparse.BooleanOptionalAction, default=True,
+                    help="time the four-world year: 12 months on each of 512x1024, 768x1536, 1024x2048 and "
+                         "1024x2048 (seeds 1-4, land 20/27/33/40%%) (default on)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -472,6 +516,25 @@ def main() -> int:
         (f", up leg with the outer step fused {us_fused:.2f} us ({fused_gbs:.0f} GB/s)" if fused else "") +
         f", bottom {us_bottom:.2f} us")
 
+    # Beside the headline: the reference method itself (binary32 values, binary64 residual, the
+    # coarsest relaxed) on the device to the reference's own tolerance 1e-3, and the 256 x 512
+    # problem (seed 0) to 1e-10 by the refined method, for the host legs of the same names.
+    prob256 = circulation_problem(256, 512, seed=0, land_fraction=0.3)
+    side = {}
+    for name, pr, tol, m in (("reference_method_1e-3_1024x2048", prob, 1e-3, cg.CG_METHOD_REFERENCE),
+                             ("refined_1e-10_256x512", prob256, 1e-10, cg.CG_METHOD_REFINED)):
+        sp = Plan(acc, pr.levels)
+        try:
+            sp.solve(tol, m)
+            t1 = time.perf_counter()
+            for _ in range(20):
+                si = sp.solve(tol, m)
+            side[name] = {"solves_per_s": round(20 / (time.perf_counter() - t1), 2), "cycles": si.cycles,
+                          "relative_residual": si.relative_residual}
+        finally:
+            sp.close()
+    log(f"bench: device side legs {side}")
+
     batched = None
     if args.batch_reps > 0:
         batched = batched_year(acc, prob, args.tol, args.batch_reps, args.warmup, not args.no_autotune)
@@ -494,7 +557,7 @@ def main() -> int:
     cpu = None
     if not args.no_oracle_check or not args.no_cpu:
         import oracle
-        threads = min(16, os.cpu_count() or 1)
+        nproc, model, threads = host_cores()
         oracle.set_threads(threads)
         ladder = [oracle.Stencil(s.cells_across, s.cells_down, s.is_water, s.east_weight, s.west_weight,
                                  s.north_weight, s.south_weight, s.forcing, s.centre_weight) for s in prob.levels]
@@ -503,8 +566,8 @@ def main() -> int:
             ref, _, _ = oracle.solve64(ladder, 1e-12)
             err = float(np.linalg.norm(x64 - ref) / np.linalg.norm(ref))
         if not args.no_cpu:
-            log("bench: CPU baseline")
-            cpu = cpu_baseline(prob.levels, args.cpu_budget, threads)
+            log(f"bench: CPU baseline ({threads} of {nproc} cores, {model})")
+            cpu = cpu_baseline(prob.levels, prob256.levels, args.cpu_budget, threads, nproc, model)
 
     out = {
         "metric": METRIC,
@@ -520,9 +583,11 @@ def main() -> int:
         "dtype": "f32+f64",
         "data": "synthetic",
         "config": {
-            "workload": "circulation 1024x2048 (seed 0, 30% land, annual forcing) from zero to 1e-8: plain GPU "
-                        "multigrid, binary32 V(2,2) red-black cycles of the reference ladder inside binary64 "
-                        "iterative refinement (the learned preconditioner's leg is reported under \"learned\")",
+            "workload": "circulation 1024x2048 (seed 0, 30% land, annual forcing) from zero to residual 1e-8 and "
+                        "estimated error 1e-10: `value` is the plain GPU multigrid, one instance (binary32 V(2,2) "
+                        "red-black cycles of the reference ladder inside binary64 iterative refinement); the learned "
+                        "preconditioner's solves/s and its ratio to `value` are learned_solves_per_s and "
+                        "learned_vs_plain_mg (details under \"learned\")",
             "rows": ROWS, "cols": COLS, "tol": args.tol, "method": args.method,
             "graph": not args.no_graph, "tile_rows": args.tile_rows, "level_passes": args.level_passes,
             "autotuned_tile_rows": tuned,
@@ -573,9 +638,12 @@ def main() -> int:
                        "avg_launch_us": round(us_bottom, 3)},
         },
         "cpu_baseline": cpu,
+        "device_side_legs": side,
     }
     if learned:
         out["learned"] = learned
+        out["learned_solves_per_s"] = learned.get("solves_per_s")
+        out["learned_vs_plain_mg"] = learned.get("learned_vs_plain_mg")
     if batched:
         out["batched"] = batched
     if worlds:

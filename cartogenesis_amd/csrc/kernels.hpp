@@ -95,6 +95,121 @@ struct MaxReduce {
     unsigned long long *clear;
 };
 
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// The refinement's error estimate: the binary64 outer step's sums of the correction's squares
+// (slots 0 .. kMaxSlots-1) and of the new answer's squares (the next kMaxSlots), each wave adding its
+// sums into slot wave % kMaxSlots. As with the maxima, the slots start at zero and the step zeroes the
+// group the next step accumulates into.
+struct SumReduce {
+    double *out;
+    double *clear;
+};
+
+// Per instance: whether the refinement goes on, how many steps it has run, whether the step the
+// gate let run computes the error estimate's sums, and the sum of squared corrections of the step
+// before (the contraction of the next one is measured against it; 0 when it had no sums).
+struct StepGate {
+    int active;
+    int steps;
+    int want;
+    int pad;
+    double dprev2;
+};
+
+// Steps compute the estimate's sums only once the residual is within this factor of its threshold
+// (the last few steps of a solve): before, the residual alone keeps the refinement going.
+constexpr double kSumsAhead = 100.0;
+
+// The stopping test of the binary64 refinement, run on the device before every step so that each
+// instance stops at exactly the first step where it holds, however the host batches the steps: the
+// previous step's residual max |u c| under thr[inst] (tol x max |f c|), and, once a correction has
+// been applied, its estimated relative L2 error under etol: for a contraction rho per step,
+// e_k = rho e_(k-1) and delta_k = e_(k-1) - e_k, so |e_k| = rho / (1 - rho) |delta_k|, with rho the
+// ratio of successive correction norms. A residual that is not a number stops the instance too.
+// Reads the previous step's groups and state `in`; the wave with `writer` set writes state `out`
+// and the active flag the rest of the step reads.
+struct GateArgs {
+    const unsigned long long *res;  // instance 0's residual slots of the previous step
+    const double *sums;             // instance 0's sums of the previous step (2 kMaxSlots)
+    const StepGate *in;
+    StepGate *out;
+    int *active;
+    int *want_sums;  // per instance: the step's outer kernel computes the sums
+    const double *thr;
+    double etol;  // <= 0: the residual alone decides
+    int res_stride, sums_stride;
+};
+
+__device__ bool gate_decide(const GateArgs &g, int inst, bool writer) {
+    const int lane = threadIdx.x & 63;
+    const StepGate s = g.in[inst];
+    bool go = false, want = false;
+    double d2 = 0.0;
+    if (s.active) {
+        const unsigned long long r = wave_max_u64(g.res[inst * g.res_stride + lane]);
+        double x2 = 0.0;
+        if (s.want) {
+            d2 = wave_sum_f64(g.sums[inst * g.sums_stride + lane]);
+            x2 = wave_sum_f64(g.sums[inst * g.sums_stride + kMaxSlots + lane]);
+        }
+        const double res = __longlong_as_double((long long)r);
+        const double thr = g.thr[inst];
+        bool err_ok = true;
+        if (g.etol > 0.0 && s.steps > 0) {
+            if (!s.want) {
+                err_ok = false;  // the step ran without the sums: no estimate yet
+            } else if (x2 > 0.0) {
+                double rho = s.dprev2 > 0.0 ? sqrt(d2 / s.dprev2) : 0.5;
+                rho = rho < 0.95 ? rho : 0.95;
+                err_ok = sqrt(d2 / x2) * (rho / (1.0 - rho)) <= g.etol;
+            } else {
+                err_ok = d2 <= 0.0;
+            }
+        }
+        go = !(res != res || (res < thr && err_ok));
+        want = go && g.etol > 0.0 && res < kSumsAhead * thr;
+    }
+    if (writer && lane == 0) {
+        StepGate o;
+        o.active = go ? 1 : 0;
+        o.steps = s.steps + (go ? 1 : 0);
+        o.want = want ? 1 : 0;
+        o.pad = 0;
+        o.dprev2 = go ? (s.want ? d2 : 0.0) : s.dprev2;
+        g.out[inst] = o;
+        g.active[inst] = o.active;
+        g.want_sums[inst] = o.want;
+    }
+    return go;
+}
+
+// A refinement's start: `clear` (the residual slots and sums of every instance) zeroed, and the
+// gate state the first step reads set to active with no steps, every active flag 1 and no sums.
+__global__ void k_check_init(StepGate *state, int *active, int *want, unsigned long long *clear, long nclear, int K) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nclear) clear[i] = 0ull;
+    if (i < K) {
+        StepGate g;
+        g.active = 1;
+        g.steps = 0;
+        g.want = 0;
+        g.pad = 0;
+        g.dprev2 = 0.0;
+        state[i] = g;
+        active[i] = 1;
+        want[i] = 0;
+    }
+}
+
+// The stopping test as a launch of its own (one wavefront per instance, blockIdx.y), for steps whose
+// first kernel is not the finest grid's fused down leg.
+__global__ __launch_bounds__(64) void k_step_gate(GateArgs g) { (void)gate_decide(g, blockIdx.y, true); }
+
 // Binary32 update of one cell in the reference's order: ((e xE + w xW) + n xN) + s xS - f.
 __device__ __forceinline__ float rb_update(float4 wt, float xe, float xw, float xn, float xs, float f) {
     return wt.x * xe + wt.y * xw + wt.z * xn + wt.w * xs - f;
@@ -160,7 +275,7 @@ struct TileExtra {
     const int *active;
     // TILE_UP_OUTER: the binary64 answer in and out (instances s64 doubles apart), the finest forcing
     // of the binary64 problem and the next cycle's forcing out (fs apart), the maxima's slots
-    // (red_stride apart).
+    // (red_stride apart) and, when sums.out is set, the error estimate's sums (sums_stride apart).
     const double *x64;
     double *x64n;
     const float *f0;
@@ -168,6 +283,16 @@ struct TileExtra {
     MaxReduce red;
     long s64;
     int red_stride;
+    SumReduce sums;
+    int sums_stride;
+    const int *want_sums;  // when set: per instance, whether this step computes the sums
+    // TILE_DOWN of the finest grid in a refinement step: the stopping test (gate.in set) instead of
+    // the active flag; the kernel writes that flag for the rest of the step. With late set (one
+    // instance, whose other kernels never read the flag: what they write is the next step's
+    // scratch) only tile 0's wave runs the test, after its own work, and TILE_UP_OUTER reads the
+    // flag with its loads and holds back only its outputs.
+    GateArgs gate;
+    int late;
 };
 
 template <int NH, int MODE>
@@ -190,7 +315,10 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
     const int across = L.across, down = L.down;
     const int lane = threadIdx.x & 63;
     const int inst = blockIdx.y;
-    if (ex.active && !ex.active[inst]) return;
+    const bool gated = MODE == TILE_DOWN && ex.gate.in;
+    if (!gated && ex.active && !ex.late && !ex.active[inst]) return;
+    // One instance: the outer step's outputs wait on the flag, read here with the loads.
+    const int go_late = (MODE == TILE_UP_OUTER && ex.active && ex.late) ? ex.active[inst] : 1;
     if (inst) {
         f += inst * ex.fs;
         if (xin) xin += inst * ex.fs;
@@ -204,12 +332,19 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
             ex.fcn += inst * ex.fs;
             ex.red.out += inst * ex.red_stride;
             ex.red.clear += inst * ex.red_stride;
+            if (ex.sums.out) {
+                ex.sums.out += inst * ex.sums_stride;
+                ex.sums.clear += inst * ex.sums_stride;
+            }
         }
     }
     const int wave = xcd_block() * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int tiles_x = (across + IW - 1) / IW;
     const int tiles_y = (down + TH - 1) / TH;
     if (wave >= tiles_x * tiles_y) return;
+    // Every wave of the instance evaluates the test on the same inputs and so reaches the same
+    // decision; tile 0's wave records it.
+    if (gated && !ex.late && !gate_decide(ex.gate, inst, wave == 0)) return;
     const int tx = wave % tiles_x, ty = wave / tiles_x;
     const int col0 = tx * IW, row0 = ty * TH;
     int ca = (col0 - HC + 2 * lane) % across;
@@ -438,6 +573,12 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
             }
         }
         unsigned long long mx = 0;
+        if (!go_late) return;
+        // Squares of the correction and of the new answer over the water cells, summed in binary32
+        // per lane (an estimate needs a few digits) and in binary64 across the wave, on the steps
+        // that the gate asks for them.
+        const bool sums_on = ex.sums.out && (!ex.want_sums || ex.want_sums[inst]);
+        float d2 = 0.0f, x2 = 0.0f;
 #pragma unroll
         for (int k = 0; k < TH; ++k) {
             const int i = HR + k;
@@ -463,6 +604,11 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
                 ux = ox - bal;
                 const unsigned long long b = abs_bits(ux * cen);
                 mx = b > mx ? b : mx;
+                if (sums_on) {
+                    const float xs = (float)ox;
+                    d2 = __fmaf_rn(A[i], A[i], d2);
+                    x2 = __fmaf_rn(xs, xs, x2);
+                }
             }
             if ((mB >> (i & 31)) & 1) {
                 const double bal = (double)wt.x * east + (double)wt.y * vx[k + 1] + (double)wt.z * vn_y +
@@ -471,17 +617,32 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
                 uy = oy - bal;
                 const unsigned long long b = abs_bits(uy * cen);
                 mx = b > mx ? b : mx;
+                if (sums_on) {
+                    const float ys = (float)oy;
+                    d2 = __fmaf_rn(B[i], B[i], d2);
+                    x2 = __fmaf_rn(ys, ys, x2);
+                }
             }
             at_byte<double2>(ex.x64n, 2u * off4) = make_double2(ox, oy);
             at_byte<float2>(ex.fcn, off4) = make_float2((float)ux, (float)uy);
         }
-        // One relaxed atomicMax per wave into slot wave % kMaxSlots; the waves together zero the
-        // slots the next step reduces into.
+        // One relaxed atomicMax per wave into slot wave % kMaxSlots (and, with sums, one add of each
+        // sum); the waves together zero the slots the next step reduces into.
         mx = wave_max_u64(mx);
+        double d2w = 0.0, x2w = 0.0;
+        if (sums_on) {
+            d2w = wave_sum_f64((double)d2);
+            x2w = wave_sum_f64((double)x2);
+        }
         if (lane == 0) {
             const int nw = tiles_x * tiles_y;
             for (int j = wave; j < kMaxSlots; j += nw) ex.red.clear[j] = 0ull;
             if (mx) atomicMax(ex.red.out + (wave & (kMaxSlots - 1)), mx);
+            if (ex.sums.out) {
+                for (int j = wave; j < 2 * kMaxSlots; j += nw) ex.sums.clear[j] = 0.0;
+                if (d2w != 0.0) atomicAdd(ex.sums.out + (wave & (kMaxSlots - 1)), d2w);
+                if (x2w != 0.0) atomicAdd(ex.sums.out + kMaxSlots + (wave & (kMaxSlots - 1)), x2w);
+            }
         }
     }
 
@@ -522,6 +683,7 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
                 at_byte<float>(ex.fc, 4u * cidx) = cwv[(i - HR) / 2] ? acc : 0.0f;
             }
         }
+        if (gated && ex.late && wave == 0) (void)gate_decide(ex.gate, inst, true);
     }
 }
 
@@ -816,7 +978,37 @@ struct OuterBatch {
     long s32, s64;
     int red_stride;
     const int *active;
+    SumReduce sums;  // with out set: the error estimate's sums, as TILE_UP_OUTER adds them
+    int sums_stride;
+    const int *want_sums;  // when set: per instance, whether this step computes the sums
 };
+
+// A grid-wide sum of each of two values into the slots of `sums` (block b adds into slot b %
+// kMaxSlots and kMaxSlots + b % kMaxSlots); the blocks together zero sums.clear.
+__device__ void grid_sums(double a, double b2, const SumReduce &sums) {
+    __shared__ double sh[2][16];
+    const int b = blockIdx.x + blockIdx.y * gridDim.x;
+    const int nb = gridDim.x * gridDim.y;
+    if (threadIdx.x == 0 && sums.clear)
+        for (int i = b; i < 2 * kMaxSlots; i += nb) sums.clear[i] = 0.0;
+    a = wave_sum_f64(a);
+    b2 = wave_sum_f64(b2);
+    const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][w] = a;
+        sh[1][w] = b2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sa = 0.0, sb = 0.0;
+        for (int i = 0; i < nw; ++i) {
+            sa += sh[0][i];
+            sb += sh[1][i];
+        }
+        if (sa != 0.0) atomicAdd(sums.out + (b & (kMaxSlots - 1)), sa);
+        if (sb != 0.0) atomicAdd(sums.out + kMaxSlots + (b & (kMaxSlots - 1)), sb);
+    }
+}
 
 __global__ __launch_bounds__(256) void k_outer_residual(LevelArgs L, const float *__restrict__ f,
                                                         const double *__restrict__ x, const float *__restrict__ delta,
@@ -832,6 +1024,10 @@ __global__ __launch_bounds__(256) void k_outer_residual(LevelArgs L, const float
         fc += inst * ob.s32;
         red.out += inst * ob.red_stride;
         if (red.clear) red.clear += inst * ob.red_stride;
+        if (ob.sums.out) {
+            ob.sums.out += inst * ob.sums_stride;
+            if (ob.sums.clear) ob.sums.clear += inst * ob.sums_stride;
+        }
     }
     const int across = L.across, down = L.down, half = across >> 1;
     const int lane = threadIdx.x & 63;
@@ -864,6 +1060,7 @@ __global__ __launch_bounds__(256) void k_outer_residual(LevelArgs L, const float
     const bool lane_w = lane > 0 && col > 0;
     const bool lane_e = lane < 63 && c + 1 < half;
     unsigned long long m = 0;
+    double d2 = 0.0, x2 = 0.0;
 #pragma unroll
     for (int k = 0; k < kRowsPerBlock; ++k) {
         const int i = k + 1;
@@ -895,6 +1092,7 @@ __global__ __launch_bounds__(256) void k_outer_residual(LevelArgs L, const float
             ux = ox - bal;
             const unsigned long long b = abs_bits(ux * cen);
             m = b > m ? b : m;
+            x2 += ox * ox;
         }
         if (wm >> 8) {
             const double bal = (double)wt.x * east + (double)wt.y * vx[i] + (double)wt.z * vn_y + (double)wt.w * vs_y -
@@ -903,11 +1101,21 @@ __global__ __launch_bounds__(256) void k_outer_residual(LevelArgs L, const float
             uy = oy - bal;
             const unsigned long long b = abs_bits(uy * cen);
             m = b > m ? b : m;
+            x2 += oy * oy;
+        }
+        if (delta && (wm & 0xffff)) {
+            const float2 dv = *reinterpret_cast<const float2 *>(delta + idx);
+            if (wm & 0xff) d2 += (double)dv.x * (double)dv.x;
+            if (wm >> 8) d2 += (double)dv.y * (double)dv.y;
         }
         *reinterpret_cast<double2 *>(xnew + idx) = make_double2(ox, oy);
         *reinterpret_cast<float2 *>(fc + idx) = make_float2((float)ux, (float)uy);
     }
     grid_max(m, red);
+    if (ob.sums.out) {
+        const bool on = !ob.want_sums || ob.want_sums[inst];
+        grid_sums(on ? d2 : 0.0, on ? x2 : 0.0, ob.sums);
+    }
 }
 
 // k_outer_residual from x = 0 with no delta, as the first outer step of a solve from zero: every

@@ -102,11 +102,11 @@ struct cg_ctx {
     int fused = 1;
     int fuse_outer = 1;  // the finest up leg and the binary64 outer step in one launch
     int direct_bottom = 1;  // the binary32 cycle's bottom grid solved by its dense inverse
-    // The binary64-refined methods stop once the residual is under tol x stop_permille / 1000: the
-    // answer's error against the exact solution of the reference's operator keeps falling with every
-    // step after the residual first passes tol, and stopping a third of the way below it keeps that
-    // error two orders of magnitude under tol.
-    int stop_permille = 333;
+    // The binary64-refined methods stop once the residual is under tol x stop_permille / 1000
+    int stop_permille = 1000;
+    // and, once a correction has been applied, their estimated relative L2 error (the step gate's)
+    // is at most error_tol (0: the residual alone decides).
+    double error_tol = 1e-10;
     // Passes per level of the fused binary32 cycle, 4 bits per level from level 0 up (2, 4 or 6;
     // levels past the eighth and the bottom's run 2).
     long level_passes = 0x22222222;
@@ -330,17 +330,32 @@ struct cg_plan {
     int pole_is_wall = 1;
     // Instances: K right-hand sides on one ladder. Every per-instance grid array holds K copies,
     // each a grid with its zero row (vstride(l) floats); the binary64 answers K copies of the finest
-    // cells. The refined method's reduction slots of instance i and binary64 parity q are group
-    // red_base + 2 i + q of d_u64 (red_base 0 for one instance, 4 otherwise); the maximum of the
-    // first outer step of a solve from zero is group first_base + i (first_base = max(red_base +
-    // 2K, 4)), read with the first batch of steps.
+    // cells. The refined method's residual slots of instance i and binary64 parity q are group
+    // 2 i + q of d_res: the step writing binary64 parity 1 - q (reading q) reduces into group q. Its
+    // error-estimate sums are group 2 i + q of d_sums (2 kMaxSlots doubles each; groups 2K and 2K + 1
+    // serve kernel timing). The gate states, d_res and d_sums are consecutive pieces of one block
+    // (d_check), so one copy brings a check's data to the host (h_check).
     int K = 1;
     int red_base = 0;
     int first_base = 4;
     unsigned long long *h_red = nullptr;  // pinned copy of d_u64
-    int *d_active = nullptr, *h_active = nullptr;  // per instance: still solving (K > 1)
+    int *d_active = nullptr, *h_active = nullptr;  // per instance: still solving
+    // The refined method's step gate (gate_decide): state per binary64 parity and instance, the
+    // residual thresholds per instance, the sums; `gated` while a refined solve (or the autotuner)
+    // runs steps, so that every kernel of a step honours the active flags.
+    unsigned char *d_check = nullptr, *h_check = nullptr;
+    size_t check_bytes = 0;  // the piece a check copies (gate states, residual groups, sums)
+    StepGate *d_gate = nullptr, *h_gate = nullptr;
+    unsigned long long *d_res = nullptr, *h_res = nullptr;
+    double *d_sums = nullptr, *h_sums = nullptr;
+    double *d_thr = nullptr, *h_thr = nullptr;
+    std::vector<double> thr_up;  // the thresholds last uploaded
+    int *d_want = nullptr;
+    bool gated = false;
+    bool gate_in_down = false;  // the next finest down leg runs the gate (set by refine_step)
     std::vector<double> largest_k;        // per instance max |f c|, < 0 when unknown
     std::vector<int> ans_par;             // per instance: the binary64 parity holding its answer
+    std::vector<int> last_steps;          // per instance: steps of its last solve from zero (0: none)
     PlanLevel lv[kMaxLevels];
     float *f0 = nullptr;          // the finest forcing
     float *f_alt = nullptr;       // with the fused outer step: level 0's other correction forcing
@@ -364,6 +379,7 @@ struct cg_plan {
     // settings they were captured under (a change of those rebuilds them).
     hipGraphExec_t step_exec[2] = {nullptr, nullptr};
     long graph_key = -1;
+    double graph_etol = -1.0;
     // The direct bottom (CG_OPT_DIRECT_BOTTOM): level `bottom` solved by M = -(I - W)^-1 over its
     // bnw water cells (row-major, binary32); bnw = 0 when the plan has none.
     int bnw = 0;
@@ -438,8 +454,13 @@ void free_plan(cg_plan *p) {
     dfree(p->x64[1]);
     dfree(p->d_u64);
     dfree(p->d_active);
+    dfree(p->d_check);
+    dfree(p->d_thr);
+    dfree(p->d_want);
     if (p->h_red) (void)hipHostFree(p->h_red);
     if (p->h_active) (void)hipHostFree(p->h_active);
+    for (void *h : {(void *)p->h_check, (void *)p->h_thr})
+        if (h) (void)hipHostFree(h);
     dfree(p->d_passes);
     dfree(p->d_idx);
     dfree(p->d_Mt);
@@ -690,6 +711,16 @@ unsigned long long group_max(const unsigned long long *slots) {
     return m;
 }
 
+// Reads the maximum held in instance 0's residual group of binary64 parity q (d_res).
+int read_res(cg_plan *p, int q, unsigned long long *v) {
+    cg_ctx *ctx = p->ctx;
+    HIPCHK(ctx, hipMemcpyAsync(p->h_res + q * kMaxSlots, p->d_res + q * kMaxSlots, kMaxSlots * sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    *v = group_max(p->h_res + q * kMaxSlots);
+    return CG_OK;
+}
+
 // Reads the maximum held in slot group `group` (kMaxSlots slots at d_u64 + group * kMaxSlots).
 int read_u64(cg_plan *p, int group, unsigned long long *v) {
     cg_ctx *ctx = p->ctx;
@@ -713,6 +744,86 @@ inline dim3 grid2d(const Stencil *st) {
 
 // Floats between two instances' copies of a grid: the grid and its zero row.
 inline long vstride(const Stencil *st) { return (long)st->across * (st->down + 1); }
+
+// The per-instance flags the kernels of a step honour: several instances' (a stopped instance's
+// kernels return at once). One instance's step kernels read none, except the binary64 outer step
+// while the refined method's gate decides (outer_flags): what the others write is the next step's
+// scratch, so an extra step after the gate stopped the instance changes nothing but time.
+inline const int *active_flags(const cg_plan *p) { return p->K > 1 ? p->d_active : nullptr; }
+inline const int *outer_flags(const cg_plan *p) { return (p->K > 1 || p->gated) ? p->d_active : nullptr; }
+
+// Instance 0's error-estimate sums for binary64 parity q (instances 4 kMaxSlots apart).
+inline double *sums_group(const cg_plan *p, int q) { return p->d_sums + (size_t)q * 2 * kMaxSlots; }
+constexpr int kSumsStride = 4 * kMaxSlots;
+
+// The gate of the step that reads binary64 parity q = x64cur: the previous step's groups (parity
+// 1 - q), state in from parity 1 - q, out to parity q.
+GateArgs gate_args(const cg_plan *p) {
+    const int q = p->x64cur;
+    GateArgs g{};
+    g.res = p->d_res + (size_t)(1 - q) * kMaxSlots;
+    g.res_stride = 2 * kMaxSlots;
+    g.sums = sums_group(p, 1 - q);
+    g.sums_stride = kSumsStride;
+    g.in = p->d_gate + (size_t)(1 - q) * p->K;
+    g.out = p->d_gate + (size_t)q * p->K;
+    g.active = p->d_active;
+    g.want_sums = p->d_want;
+    g.thr = p->d_thr;
+    g.etol = p->ctx->error_tol;
+    return g;
+}
+
+// The host's replay of gate_decide on the values read back, in the same arithmetic (the wave sums'
+// butterfly, correctly rounded division and square root), so the host knows when the next gate
+// would stop an instance without running that step.
+double butterfly_sum(const double *v) {
+    double a[kMaxSlots];
+    for (int l = 0; l < kMaxSlots; ++l) a[l] = v[l];
+    for (int off = 32; off > 0; off >>= 1) {
+        double b[kMaxSlots];
+        for (int l = 0; l < kMaxSlots; ++l) b[l] = a[l] + a[l ^ off];
+        for (int l = 0; l < kMaxSlots; ++l) a[l] = b[l];
+    }
+    return a[0];
+}
+
+struct GateView {
+    double res;   // max |u c| of the last step
+    double d2, x2;
+    double est;   // estimated relative L2 error (< 0 when the step applied no correction)
+    double rho;   // contraction of the correction's norm
+    bool stop;    // what the next gate decides
+};
+
+GateView gate_view(const unsigned long long *res_slots, const double *sums, const StepGate &st, double thr,
+                   double etol) {
+    GateView v{};
+    unsigned long long m = 0;
+    for (int l = 0; l < kMaxSlots; ++l) m = res_slots[l] > m ? res_slots[l] : m;
+    std::memcpy(&v.res, &m, sizeof v.res);
+    v.d2 = st.want ? butterfly_sum(sums) : 0.0;
+    v.x2 = st.want ? butterfly_sum(sums + kMaxSlots) : 0.0;
+    v.est = -1.0;
+    v.rho = 0.5;
+    bool err_ok = true;
+    if (etol > 0.0 && st.steps > 0) {
+        if (!st.want) {
+            v.d2 = v.x2 = 0.0;
+            err_ok = false;
+        } else if (v.x2 > 0.0) {
+            double rho = st.dprev2 > 0.0 ? std::sqrt(v.d2 / st.dprev2) : 0.5;
+            rho = rho < 0.95 ? rho : 0.95;
+            v.rho = rho;
+            v.est = std::sqrt(v.d2 / v.x2) * (rho / (1.0 - rho));
+            err_ok = v.est <= etol;
+        } else {
+            err_ok = v.d2 <= 0.0;
+        }
+    }
+    v.stop = !st.active || v.res != v.res || (v.res < thr && err_ok);
+    return v;
+}
 
 // A reduction into slot group `group`, zeroed first by a fill kernel.
 MaxReduce max_into(cg_plan *p, int group) {
@@ -1076,7 +1187,7 @@ void direct_bottom_launch(cg_plan *p) {
     da.x = B0.x[B0.cur];
     da.f_stride = vstride(B0.st);
     da.x_stride = vstride(B0.st);
-    da.active = p->K > 1 ? p->d_active : nullptr;
+    da.active = active_flags(p);
     const unsigned rows = (p->bnw + kDirectWaves - 1) / kDirectWaves;
     if (p->K == 1)
         hipLaunchKernelGGL(k_direct_bottom<1>, dim3(rows, 1), dim3(64 * kDirectWaves), 0, p->ctx->stream, da);
@@ -1108,7 +1219,7 @@ void bottom_launch(cg_plan *p, const float *in0, float *out0) {
     ba.beyond = beyond;
     ba.f_stride = vstride(B0.st);
     ba.x_stride = vstride(B0.st);
-    ba.active = p->K > 1 ? p->d_active : nullptr;
+    ba.active = active_flags(p);
     hipLaunchKernelGGL(k_bottom, dim3(p->K), dim3(1024), p->bottom_lds, s, ba);
 }
 
@@ -1144,7 +1255,11 @@ hipError_t down_leg(cg_plan *p, int l, const float *in0, int rows) {
     ex.fc = C.f;
     ex.fs = vstride(L.st);
     ex.cs = vstride(C.st);
-    ex.active = p->K > 1 ? p->d_active : nullptr;
+    ex.active = active_flags(p);
+    if (l == 0 && p->gate_in_down) {
+        ex.gate = gate_args(p);
+        ex.late = p->K == 1;
+    }
     const float *fl = (l == 0 && in0) ? in0 : L.f;
     return tile_dispatch(p->ctx, L.st->args(), fl, L.x[L.cur], L.x[1 - L.cur], level_half_sweeps(p->ctx, l), true,
                          TILE_DOWN, ex, rows, p->K);
@@ -1162,7 +1277,7 @@ hipError_t up_leg(cg_plan *p, int l, const float *in0, float *out0, bool fuse_ou
     ex.beyond = p->pole_is_wall ? -1.0f : 1.0f;
     ex.fs = vstride(L.st);
     ex.cs = vstride(C.st);
-    ex.active = p->K > 1 ? p->d_active : nullptr;
+    ex.active = active_flags(p);
     const float *fl = (l == 0 && in0) ? in0 : L.f;
     float *xo = (l == 0 && out0) ? out0 : L.x[1 - L.cur];
     int mode = TILE_UP;
@@ -1173,10 +1288,16 @@ hipError_t up_leg(cg_plan *p, int l, const float *in0, float *out0, bool fuse_ou
         ex.x64n = p->x64[1 - q];
         ex.f0 = p->f0;
         ex.fcn = level0_forcing(p, 1 - q);
-        ex.red.out = p->d_u64 + (p->red_base + q) * kMaxSlots;
-        ex.red.clear = p->d_u64 + (p->red_base + 1 - q) * kMaxSlots;
+        ex.red.out = p->d_res + q * kMaxSlots;
+        ex.red.clear = p->d_res + (1 - q) * kMaxSlots;
         ex.s64 = L.st->cells;
         ex.red_stride = 2 * kMaxSlots;
+        ex.sums.out = sums_group(p, q);
+        ex.sums.clear = sums_group(p, 1 - q);
+        ex.sums_stride = kSumsStride;
+        ex.want_sums = p->gated ? p->d_want : nullptr;
+        ex.active = outer_flags(p);
+        ex.late = p->K == 1;
     }
     return tile_dispatch(p->ctx, L.st->args(), fl, L.x[L.cur], xo, level_half_sweeps(p->ctx, l), false, mode, ex, rows,
                          p->K);
@@ -1276,13 +1397,17 @@ int outer_step(cg_plan *p, const float *delta, float *fc = nullptr) {
     // Slot x64cur receives the maximum (zeroed by the previous step); the other slot is zeroed for
     // the next step.
     MaxReduce red;
-    red.out = p->d_u64 + (p->red_base + p->x64cur) * kMaxSlots;
-    red.clear = p->d_u64 + (p->red_base + 1 - p->x64cur) * kMaxSlots;
+    red.out = p->d_res + p->x64cur * kMaxSlots;
+    red.clear = p->d_res + (1 - p->x64cur) * kMaxSlots;
     OuterBatch ob{};
     ob.s32 = vstride(st);
     ob.s64 = st->cells;
     ob.red_stride = 2 * kMaxSlots;
-    ob.active = p->K > 1 ? p->d_active : nullptr;
+    ob.active = outer_flags(p);
+    ob.sums.out = sums_group(p, p->x64cur);
+    ob.sums.clear = sums_group(p, 1 - p->x64cur);
+    ob.sums_stride = kSumsStride;
+    ob.want_sums = p->gated ? p->d_want : nullptr;
     const dim3 grid(blocks_for(st->across / 2, 256), blocks_for(st->down, kRowsPerBlock), p->K);
     hipLaunchKernelGGL(k_outer_residual, grid, dim3(256), 0, ctx->stream, st->args(), p->f0,
                        p->x64[p->x64cur], delta, p->x64[1 - p->x64cur], fc ? fc : p->lv[0].f, red, ob);
@@ -1294,13 +1419,24 @@ int outer_step(cg_plan *p, const float *delta, float *fc = nullptr) {
 // One refinement step: a binary32 cycle on the current correction forcing and the binary64 outer
 // step, fused into the finest up leg where outer_fused, else as its own launch.
 int refine_step(cg_plan *p, bool timed) {
+    // The step gate: inside the finest down leg when the step starts with one, else a launch of its
+    // own first.
+    const bool down_first = p->ctx->fused && p->fusable && p->bottom > 0;
+    if (p->gated && !down_first) {
+        hipLaunchKernelGGL(k_step_gate, dim3(1, p->K), dim3(64), 0, p->ctx->stream, gate_args(p));
+        HIPCHK(p->ctx, hipGetLastError());
+    }
+    p->gate_in_down = p->gated && down_first;
+    int rc;
     if (outer_fused(p)) {
-        const int rc = vcycle32_fused(p, timed, level0_forcing(p, p->x64cur), nullptr, true);
+        rc = vcycle32_fused(p, timed, level0_forcing(p, p->x64cur), nullptr, true);
+        p->gate_in_down = false;
         if (rc) return rc;
         p->x64cur = 1 - p->x64cur;
         return CG_OK;
     }
-    const int rc = vcycle32(p, timed);
+    rc = vcycle32(p, timed);
+    p->gate_in_down = false;
     if (rc) return rc;
     return outer_step(p, p->lv[0].x[p->lv[0].cur]);
 }
@@ -1316,7 +1452,8 @@ int build_step_graphs(cg_plan *p) {
     // tuned height for take theirs); graphs captured under others are rebuilt.
     const long key = ((long)ctx->level_passes << 12) | ((long)ctx->tile_rows << 4) | (ctx->fused << 3) |
                      (ctx->fuse_outer << 2) | (ctx->direct_bottom << 1);
-    if (p->graph_key != key) {
+    if (p->graph_key != key || p->graph_etol != ctx->error_tol) {
+        p->graph_etol = ctx->error_tol;
         for (int i = 0; i < 2; ++i)
             if (p->step_exec[i]) {
                 (void)hipGraphExecDestroy(p->step_exec[i]);
@@ -1361,163 +1498,186 @@ int build_step_graphs(cg_plan *p) {
     return CG_OK;
 }
 
+// Sets up the step gate for a refinement whose first graph step reads binary64 parity first_par:
+// every instance active with no steps run, residual thresholds thr[i] (absolute: tol x max |f c|),
+// and every instance's residual and sums groups zeroed (the first outer step reduces into parity
+// 1 - first_par's, which the first gate reads).
+int gate_reset(cg_plan *p, int first_par, const std::vector<double> &thr) {
+    cg_ctx *ctx = p->ctx;
+    const int K = p->K;
+    // The thresholds cross the bus only when they change (a new forcing or tol).
+    bool same = (int)p->thr_up.size() == K;
+    for (int i = 0; same && i < K; ++i) same = p->thr_up[i] == thr[i];
+    if (!same) {
+        for (int i = 0; i < K; ++i) p->h_thr[i] = thr[i];
+        HIPCHK(ctx, hipMemcpyAsync(p->d_thr, p->h_thr, sizeof(double) * K, hipMemcpyHostToDevice, ctx->stream));
+        p->thr_up = thr;
+    }
+    const long nclear = (long)2 * K * kMaxSlots + (long)4 * K * kMaxSlots;  // residual groups, then sums
+    hipLaunchKernelGGL(k_check_init, dim3(blocks_for(nclear, 256)), dim3(256), 0, ctx->stream,
+                       p->d_gate + (size_t)(1 - first_par) * K, p->d_active, p->d_want, p->d_res, nclear, K);
+    HIPCHK(ctx, hipGetLastError());
+    return CG_OK;
+}
+
 // The refined method over every instance of the plan; infos[i] receives instance i's result.
-// Instance i stops at the first check where its residual is under tol (or not a number); with
-// several instances its kernels then skip it (the active flags) while the others go on.
+// Every step starts with the gate (gate_decide) that stops an instance at the first step where its
+// residual is under tol and its estimated error under the context's error bound, so each instance
+// runs exactly the steps it would alone, however the host groups the steps into batches between
+// its reads; a stopped instance's kernels return at once and its answer stays where it stopped.
 int refined_solve(cg_plan *p, double tol_asked, bool warm, cg_solution *infos, const volatile int *cancel) {
     cg_ctx *ctx = p->ctx;
     const int K = p->K;
-    const double tol = tol_asked * (ctx->stop_permille / 1000.0);
     const Stencil *st = p->lv[0].st;
-    std::vector<double> largest(K), res(K), rate(K, -1.0);
-    std::vector<int> cycles(K, 0);
-    std::vector<char> on(K, 1);
+    const double tol = tol_asked * (ctx->stop_permille / 1000.0);
+    const double etol = ctx->error_tol;
+    std::vector<double> largest(K), thr(K);
     int rc;
-    for (int i = 0; i < K; ++i)
+    for (int i = 0; i < K; ++i) {
         if ((rc = plan_largest_at(p, i, &largest[i]))) return rc;
-    // From zero the first outer step needs no readback before the first cycle: every instance
-    // starts active, the first cycle runs, and the first step's maxima (their own groups) are read
-    // with it. An instance that was already under tol then keeps the first step's answer (zero) and
-    // no cycle, exactly as if it had been stopped before the cycle.
+        thr[i] = largest[i] > 0.0 ? tol * largest[i] : INFINITY;
+    }
     const bool cold = !warm || !p->x64_valid;
     p->x64_valid = true;
-    if (K > 1) {
-        for (int i = 0; i < K; ++i) p->h_active[i] = 1;
-        HIPCHK(ctx, hipMemcpyAsync(p->d_active, p->h_active, sizeof(int) * K, hipMemcpyHostToDevice, ctx->stream));
-    }
-    unsigned long long *const groups = p->d_u64 + p->red_base * kMaxSlots;
-    const int ngroups = p->first_base + K - p->red_base;  // parity groups, then first-step groups
-    hipLaunchKernelGGL(k_fill_u64, dim3(ngroups), dim3(kMaxSlots), 0, ctx->stream, groups, ngroups * kMaxSlots, 0ull);
-    auto slot_max = [&](int i, int par) { return group_max(p->h_red + (p->red_base + 2 * i + par) * kMaxSlots); };
-    auto first_max = [&](int i) { return group_max(p->h_red + (p->first_base + i) * kMaxSlots); };
-    auto read_all = [&]() -> int {
-        HIPCHK(ctx, hipMemcpyAsync(p->h_red + p->red_base * kMaxSlots, groups, ngroups * kMaxSlots * sizeof(unsigned long long),
-                                   hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-        return CG_OK;
-    };
-    const int first_slot = p->x64cur;
-    float *const first_fc = outer_fused(p) ? level0_forcing(p, 1 - p->x64cur) : p->lv[0].f;
-    bool first_pending = false;
-    int left = 0;
+    // The first outer step reads parity q0 (zero, from cold) and writes 1 - q0, where the steps start.
+    const int q0 = p->x64cur;
+    const int first_par = 1 - q0;
+    if ((rc = gate_reset(p, first_par, thr))) return rc;
+    float *const first_fc = outer_fused(p) ? level0_forcing(p, first_par) : p->lv[0].f;
     if (cold) {
+        // From zero: x = 0, u = 0 - (0 - f) on water; its maxima in parity q0's groups.
         MaxReduce red;
-        red.out = p->d_u64 + p->first_base * kMaxSlots;
+        red.out = p->d_res + (size_t)q0 * kMaxSlots;
         red.clear = nullptr;
         OuterBatch ob{};
         ob.s32 = vstride(st);
         ob.s64 = st->cells;
-        ob.red_stride = kMaxSlots;
+        ob.red_stride = 2 * kMaxSlots;
         const dim3 grid(blocks_for(st->across / 2, 256), blocks_for(st->down, kRowsPerBlock), K);
-        hipLaunchKernelGGL(k_first_outer, grid, dim3(256), 0, ctx->stream, st->args(), p->f0, p->x64[1 - p->x64cur],
+        hipLaunchKernelGGL(k_first_outer, grid, dim3(256), 0, ctx->stream, st->args(), p->f0, p->x64[first_par],
                            first_fc, red, ob);
         HIPCHK(ctx, hipGetLastError());
-        p->x64cur = 1 - p->x64cur;
-        p->ans_par.assign(K, p->x64cur);
-        first_pending = true;
-        left = K;
-    } else {
-        if ((rc = outer_step(p, nullptr, first_fc))) return rc;
-        if ((rc = read_all())) return rc;
-        p->ans_par.assign(K, p->x64cur);
-        for (int i = 0; i < K; ++i) {
-            res[i] = largest[i] > 0.0 ? bits_to_double(slot_max(i, first_slot)) / largest[i] : 0.0;
-            on[i] = res[i] >= tol;
-            left += on[i];
-        }
+        p->x64cur = first_par;
+    } else if ((rc = outer_step(p, nullptr, first_fc))) {
+        return rc;
     }
-    const int first_par = p->x64cur;
     const bool graph = ctx->use_graph && !ctx->time_kernels;
-    if (graph && left > 0) {
-        if ((rc = build_step_graphs(p))) return rc;
-    }
-    auto publish_active = [&]() -> int {
-        if (K == 1) return CG_OK;
-        for (int i = 0; i < K; ++i) p->h_active[i] = on[i];
-        HIPCHK(ctx, hipMemcpyAsync(p->d_active, p->h_active, sizeof(int) * K, hipMemcpyHostToDevice, ctx->stream));
+    p->gated = true;
+    struct Ungate {
+        cg_plan *p;
+        ~Ungate() { p->gated = false; }
+    } ungate{p};
+    if (graph && (rc = build_step_graphs(p))) return rc;
+    const unsigned long long *const res_groups = p->h_res;
+    // Reads the gate states of both parities, the residual groups and the sums of every instance.
+    auto read_back = [&]() -> int {
+        HIPCHK(ctx, hipMemcpyAsync(p->h_check, p->d_check, p->check_bytes, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
         return CG_OK;
     };
-    if ((rc = publish_active())) return rc;
-    // Steps run in batches without a host read in between: the batch is the fewest steps any
-    // active instance's last observed convergence factor predicts it still needs, less one (at
-    // least one, at most 16 for one instance and 8 for several), so the host waits on the device a
-    // few times per solve rather than once per step, and no instance runs steps past its own
-    // convergence while slower ones go on (each check lets the instances under tol stop). The
-    // policy is the same with and without graphs, so both give the same steps and bits.
-    const double cap = K == 1 ? 16.0 : 8.0;
-    int steps = 0;
-    int status = CG_OK;
-    while (left > 0 && steps < kRefinedMostSteps) {
+    // What the host knows of instance i after `launched` steps: the state the last gate wrote (or
+    // the initial one), and the gate's view of the last step the instance ran.
+    auto state_of = [&](int i, int launched) -> StepGate {
+        const int par = launched == 0 ? 1 - first_par : (first_par + launched - 1) & 1;
+        return p->h_gate[(size_t)par * K + i];
+    };
+    auto view_of = [&](int i, const StepGate &sg) -> GateView {
+        const int g = (first_par + sg.steps + 1) & 1;  // the group its last step (or the first outer) reduced into
+        return gate_view(res_groups + (size_t)(2 * i + g) * kMaxSlots, p->h_sums + (size_t)(2 * i + g) * 2 * kMaxSlots,
+                         sg, thr[i], etol);
+    };
+    // Steps in batches between host reads: one fewer than the most steps any instance still running
+    // is predicted to need by its residual's rate and its correction's contraction (the device stops
+    // each one at its own step, so a batch sized for the slowest costs the others only skipped
+    // launches), at most 16; 8 before any rate is known. The last read of each instance replays its next gate, so a solve
+    // ends without launching the step that would only stop it.
+    // From zero the relative residual before the first step is 1 (the zero answer's), which dates the
+    // first batch's rate.
+    std::vector<double> res_prev(K, cold ? 1.0 : -1.0), rate(K, -1.0);
+    std::vector<char> done(K, 0);
+    int launched = 0, status = CG_OK, last_batch = 0;
+    // A warm start reads its first outer step's residual before any step: it may already be under tol.
+    if (!cold && (rc = read_back())) return rc;
+    while (launched < kRefinedMostSteps) {
+        int batch = 0;
+        if (launched == 0 && cold) {
+            // From zero the residual is max |f c| itself: every instance with a forcing needs steps,
+            // about as many as its last solve from zero took (one fewer, so the batch ends before
+            // the stop), else 8.
+            for (int i = 0; i < K; ++i) {
+                if (largest[i] > 0.0) batch = std::max(batch, p->last_steps[i] > 1 ? p->last_steps[i] - 1 : 8);
+                else done[i] = 1;
+            }
+            batch = std::min(batch, 32);
+        } else {
+            for (int i = 0; i < K; ++i) {
+                if (done[i]) continue;
+                const StepGate sg = state_of(i, launched);
+                const GateView v = view_of(i, sg);
+                if (v.stop) {
+                    done[i] = 1;
+                    continue;
+                }
+                const double r = largest[i] > 0.0 ? v.res / largest[i] : 0.0;
+                if (last_batch > 0 && res_prev[i] > 0.0 && r > 0.0)
+                    rate[i] = std::pow(r / res_prev[i], 1.0 / last_batch);
+                res_prev[i] = r;
+                int need_res = 1, need_err = 0;
+                if (v.res < thr[i]) need_res = 0;
+                else if (rate[i] > 0.0 && rate[i] < 0.95)
+                    need_res = (int)std::ceil(std::log(tol / r) / std::log(rate[i]));
+                if (etol > 0.0) {
+                    if (v.est < 0.0) need_err = need_res + 1;
+                    else if (v.est > etol && v.rho < 0.95)
+                        need_err = (int)std::ceil(std::log(etol / v.est) / std::log(v.rho));
+                    else if (v.est > etol) need_err = 1;
+                }
+                // One step short of the prediction: the early steps' rate is slower than the last
+                // ones', so predictions run long, and a step past the stop costs more than a read.
+                const int want = std::max(1, std::max(need_res, need_err) - 1);
+                batch = std::max(batch, want);
+            }
+            batch = std::min(batch, 16);
+        }
+        if (batch <= 0) break;  // every instance's next gate would stop it
         if (cancel && *cancel) {
             status = CG_CANCELLED;
             break;
         }
-        int batch = 0;
-        for (int i = 0; i < K; ++i) {
-            if (!on[i]) continue;
-            int want = 1;
-            if (rate[i] > 0.0 && rate[i] < 0.95 && res[i] > 0.0) {
-                const double need = std::ceil(std::log(tol / res[i]) / std::log(rate[i]));
-                want = (int)std::max(1.0, std::min(cap, need - 1.0));
-            }
-            batch = batch == 0 || want < batch ? want : batch;
-        }
-        if (batch == 0) batch = 1;
-        batch = std::min(batch, kRefinedMostSteps - steps);
-        int slot = p->x64cur;
+        batch = std::min(batch, kRefinedMostSteps - launched);
         for (int k = 0; k < batch; ++k) {
-            slot = p->x64cur;
             if (graph) {
                 HIPCHK(ctx, hipGraphLaunch(p->step_exec[p->x64cur], ctx->stream));
                 p->x64cur = 1 - p->x64cur;
-            } else {
-                if ((rc = refine_step(p, true))) return rc;
+            } else if ((rc = refine_step(p, true))) {
+                return rc;
             }
         }
-        if ((rc = read_all())) return rc;
-        steps += batch;
-        bool changed = false;
-        if (first_pending) {
-            first_pending = false;
-            for (int i = 0; i < K; ++i) {
-                res[i] = largest[i] > 0.0 ? bits_to_double(first_max(i)) / largest[i] : 0.0;
-                if (!(res[i] >= tol)) {
-                    on[i] = 0;
-                    --left;
-                    changed = true;
-                    p->ans_par[i] = first_par;
-                }
-            }
-        }
+        launched += batch;
+        last_batch = batch;
+        if ((rc = read_back())) return rc;
+        int left = 0;
         for (int i = 0; i < K; ++i) {
-            if (!on[i]) continue;
-            cycles[i] += batch;
-            const double r = bits_to_double(slot_max(i, slot));
-            const double prev = res[i];
-            res[i] = largest[i] > 0.0 ? r / largest[i] : 0.0;
-            p->ans_par[i] = p->x64cur;
-            if (std::isnan(r) || res[i] < tol) {
-                on[i] = 0;
-                --left;
-                changed = true;
-                continue;
-            }
-            if (prev > 0.0 && res[i] > 0.0) rate[i] = std::pow(res[i] / prev, 1.0 / batch);
+            if (done[i]) continue;
+            const StepGate sg = state_of(i, launched);
+            if (!sg.active || view_of(i, sg).stop) done[i] = 1;
+            else ++left;
         }
-        if (changed && left > 0 && (rc = publish_active())) return rc;
+        if (left == 0) break;
     }
-    if (first_pending) {
-        // Cancelled before the first cycle: the first step's residual, and its answer (zero).
-        if ((rc = read_all())) return rc;
-        for (int i = 0; i < K; ++i) res[i] = largest[i] > 0.0 ? bits_to_double(first_max(i)) / largest[i] : 0.0;
-    }
+    if (launched == 0 && (rc = read_back())) return rc;
     for (int i = 0; i < K; ++i) {
+        const StepGate sg = state_of(i, launched);
+        const GateView v = view_of(i, sg);
         infos[i] = cg_solution{};
-        infos[i].cycles = cycles[i];
-        infos[i].refinements = cycles[i] + 1;
-        infos[i].relative_residual = res[i];
+        infos[i].cycles = sg.steps;
+        infos[i].refinements = sg.steps + 1;
+        infos[i].relative_residual = largest[i] > 0.0 ? v.res / largest[i] : 0.0;
         infos[i].status = status;
+        p->ans_par[i] = (first_par + sg.steps) & 1;
+        if (cold && status == CG_OK) p->last_steps[i] = sg.steps;
     }
+    if (K == 1) p->x64cur = p->ans_par[0];
     p->last_answer = 1;
     return CG_OK;
 }
@@ -1556,18 +1716,25 @@ int fgmres_solve(cg_plan *p, double tol_asked, bool warm, cg_solution *info, con
     if ((rc = plan_largest(p, &largest))) return rc;
     if (!warm || !p->x64_valid) HIPCHK(ctx, hipMemsetAsync(p->x64[p->x64cur], 0, sizeof(double) * st->cells, s));
     p->x64_valid = true;
-    hipLaunchKernelGGL(k_fill_u64, dim3(2), dim3(kMaxSlots), 0, s, p->d_u64, 2 * kMaxSlots, 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(2), dim3(kMaxSlots), 0, s, p->d_res, 2 * kMaxSlots, 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(4), dim3(kMaxSlots), 0, s, reinterpret_cast<unsigned long long *>(p->d_sums),
+                       4 * kMaxSlots, 0ull);
     int slot = p->x64cur;
     if ((rc = outer_step(p, nullptr, p->fq))) return rc;
     unsigned long long bits;
-    if ((rc = read_u64(p, slot, &bits))) return rc;
+    if ((rc = read_res(p, slot, &bits))) return rc;
     double res = largest > 0.0 ? bits_to_double(bits) / largest : 0.0;
     const dim3 g2 = grid2d(st);
     const int nb = (int)(g2.x * g2.y);
     int cycles = 0, outer = 0;
     double rate = -1.0;
     bool net_on = learned;
-    while (res >= tol && cycles < kRefinedMostSteps) {
+    // The error estimate of the refined method's gate, per outer step: its correction's norm against
+    // the answer's, times rho / (1 - rho) for the contraction of successive corrections.
+    const double etol = ctx->error_tol;
+    double d2prev = 0.0;
+    bool err_ok = true;  // no correction applied yet
+    while ((res >= tol || !err_ok) && cycles < kRefinedMostSteps) {
         if (cancel && *cancel) {
             info->status = CG_CANCELLED;
             break;
@@ -1601,7 +1768,22 @@ int fgmres_solve(cg_plan *p, double tol_asked, bool warm, cg_solution *info, con
         HIPCHK(ctx, hipGetLastError());
         slot = p->x64cur;
         if ((rc = outer_step(p, p->fdelta, p->fq))) return rc;
-        if ((rc = read_u64(p, slot, &bits))) return rc;
+        if ((rc = read_res(p, slot, &bits))) return rc;
+        {
+            double sums[2 * kMaxSlots];
+            HIPCHK(ctx, hipMemcpy(sums, sums_group(p, slot), sizeof sums, hipMemcpyDeviceToHost));
+            const double d2 = butterfly_sum(sums), x2 = butterfly_sum(sums + kMaxSlots);
+            if (etol <= 0.0) {
+                err_ok = true;
+            } else if (x2 > 0.0) {
+                double rho = d2prev > 0.0 ? std::sqrt(d2 / d2prev) : 0.5;
+                rho = rho < 0.95 ? rho : 0.95;
+                err_ok = std::sqrt(d2 / x2) * (rho / (1.0 - rho)) <= etol;
+            } else {
+                err_ok = d2 <= 0.0;
+            }
+            d2prev = d2;
+        }
         cycles += m;
         outer++;
         const double prev = res;
@@ -1733,6 +1915,13 @@ int cg_ocean_set_option(cg_ctx *ctx, int option, long value) {
         case CG_OPT_FUSED: ctx->fused = value ? 1 : 0; return CG_OK;
         case CG_OPT_FUSE_OUTER: ctx->fuse_outer = value ? 1 : 0; return CG_OK;
         case CG_OPT_DIRECT_BOTTOM: ctx->direct_bottom = value ? 1 : 0; return CG_OK;
+        case CG_OPT_ERROR_PICO:
+            if (value < 0) {
+                set_err(ctx, "the error bound must be >= 0 (in units of 1e-12; 0: the residual alone)");
+                return CG_ERR_ARG;
+            }
+            ctx->error_tol = (double)value * 1e-12;
+            return CG_OK;
         case CG_OPT_STOP_PERMILLE:
             if (value < 1 || value > 1000) {
                 set_err(ctx, "the refined methods' stopping share must be 1..1000 per mille of tol");
@@ -1766,6 +1955,7 @@ long cg_ocean_get_option(cg_ctx *ctx, int option) {
         case CG_OPT_FUSE_OUTER: return ctx->fuse_outer;
         case CG_OPT_DIRECT_BOTTOM: return ctx->direct_bottom;
         case CG_OPT_STOP_PERMILLE: return ctx->stop_permille;
+        case CG_OPT_ERROR_PICO: return (long)std::llround(ctx->error_tol * 1e12);
         case CG_OPT_LEVEL_PASSES: return ctx->level_passes;
         default: return -1;
     }
@@ -2001,6 +2191,7 @@ int cg_ocean_plan_create_batch(cg_ctx *ctx, const cg_stencil *levels, int nlevel
     p->red_base = K == 1 ? 0 : 4;
     p->largest_k.assign(K, -1.0);
     p->ans_par.assign(K, 0);
+    p->last_steps.assign(K, 0);
     int rc = 0;
     auto bail = [&](int code) {
         free_plan(p);
@@ -2051,13 +2242,34 @@ int cg_ocean_plan_create_batch(cg_ctx *ctx, const cg_stencil *levels, int nlevel
         set_err(ctx, "pinned host memory for the plan's maxima was refused");
         return bail(CG_ERR_NOMEM);
     }
-    if (K > 1) {
-        if ((rc = dalloc(ctx, &p->d_active, K))) return bail(rc);
-        if (hipHostMalloc((void **)&p->h_active, sizeof(int) * K, hipHostMallocDefault) != hipSuccess) {
-            p->h_active = nullptr;
-            set_err(ctx, "pinned host memory for the plan's flags was refused");
+    if ((rc = dalloc(ctx, &p->d_active, K))) return bail(rc);
+    {
+        const size_t gate_bytes = (sizeof(StepGate) * 2 * K + 511) & ~(size_t)511;
+        const size_t res_bytes = sizeof(unsigned long long) * 2 * K * kMaxSlots;
+        const size_t sums_bytes = sizeof(double) * (2 * K + 2) * 2 * kMaxSlots;
+        p->check_bytes = gate_bytes + res_bytes + sizeof(double) * 4 * K * kMaxSlots;
+        if ((rc = dalloc(ctx, &p->d_check, gate_bytes + res_bytes + sums_bytes))) return bail(rc);
+        HIPCHK(ctx, hipMemset(p->d_check, 0, gate_bytes + res_bytes + sums_bytes));
+        if (hipHostMalloc((void **)&p->h_check, p->check_bytes, hipHostMallocDefault) != hipSuccess) {
+            p->h_check = nullptr;
+            set_err(ctx, "pinned host memory for the plan's checks was refused");
             return bail(CG_ERR_NOMEM);
         }
+        p->d_gate = reinterpret_cast<StepGate *>(p->d_check);
+        p->d_res = reinterpret_cast<unsigned long long *>(p->d_check + gate_bytes);
+        p->d_sums = reinterpret_cast<double *>(p->d_check + gate_bytes + res_bytes);
+        p->h_gate = reinterpret_cast<StepGate *>(p->h_check);
+        p->h_res = reinterpret_cast<unsigned long long *>(p->h_check + gate_bytes);
+        p->h_sums = reinterpret_cast<double *>(p->h_check + gate_bytes + res_bytes);
+    }
+    if ((rc = dalloc(ctx, &p->d_thr, K))) return bail(rc);
+    if ((rc = dalloc(ctx, &p->d_want, K))) return bail(rc);
+    HIPCHK(ctx, hipMemset(p->d_want, 0, sizeof(int) * K));
+    HIPCHK(ctx, hipMemset(p->d_active, 0, sizeof(int) * K));
+    if (hipHostMalloc((void **)&p->h_active, sizeof(int) * K, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&p->h_thr, sizeof(double) * K, hipHostMallocDefault) != hipSuccess) {
+        set_err(ctx, "pinned host memory for the plan's flags was refused");
+        return bail(CG_ERR_NOMEM);
     }
     if ((rc = dalloc(ctx, &p->d_passes, 4))) return bail(rc);
     HIPCHK(ctx, hipMemset(p->x64[0], 0, sizeof(double) * n0 * K));
@@ -2543,6 +2755,8 @@ double cg_ocean_plan_time_kernel(cg_plan *p, int kind, int reps) {
             ex.red.clear = p->d_u64 + 2 * kMaxSlots;
             ex.s64 = L.st->cells;
             ex.red_stride = 2 * kMaxSlots;
+            ex.sums.out = sums_group(p, 2 * p->K);
+            ex.sums.clear = sums_group(p, 2 * p->K + 1);
             return tile_dispatch(ctx, L.st->args(), level0_forcing(p, q), L.x[L.cur], L.x[1 - L.cur], nh, false,
                                  TILE_UP_OUTER, ex, th);
         }
@@ -2603,14 +2817,16 @@ int cg_ocean_plan_autotune(cg_plan *p, int reps, int *rows_out) {
     const int K = p->K;
     const Stencil *st = p->lv[0].st;
     HIPCHK(ctx, hipMemsetAsync(p->x64[p->x64cur], 0, sizeof(double) * st->cells * K, ctx->stream));
-    if (K > 1) {
-        for (int i = 0; i < K; ++i) p->h_active[i] = 1;
-        HIPCHK(ctx, hipMemcpyAsync(p->d_active, p->h_active, sizeof(int) * K, hipMemcpyHostToDevice, ctx->stream));
-    }
-    hipLaunchKernelGGL(k_fill_u64, dim3(2 * K), dim3(kMaxSlots), 0, ctx->stream, p->d_u64 + p->red_base * kMaxSlots,
-                       2 * K * kMaxSlots, 0ull);
-    int rc = outer_step(p, nullptr, outer_fused(p) ? level0_forcing(p, 1 - p->x64cur) : nullptr);
+    // The step gate with thresholds of zero never stops an instance, so every timed step runs.
+    int rc = gate_reset(p, 1 - p->x64cur, std::vector<double>(K, 0.0));
     if (rc) return rc;
+    rc = outer_step(p, nullptr, outer_fused(p) ? level0_forcing(p, 1 - p->x64cur) : nullptr);
+    if (rc) return rc;
+    p->gated = true;
+    struct Ungate {
+        cg_plan *p;
+        ~Ungate() { p->gated = false; }
+    } ungate{p};
     hipEvent_t a, b;
     HIPCHK(ctx, hipEventCreate(&a));
     HIPCHK(ctx, hipEventCreate(&b));

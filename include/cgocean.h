@@ -80,10 +80,16 @@ extern "C" {
                                     grid and the coarser ones are cycled in one workgroup down to the
                                     coarsest. The reference method and the Krylov preconditioner
                                     always cycle the whole ladder (OC:740-781). */
-#define CG_OPT_STOP_PERMILLE 11  /* CG_METHOD_REFINED, FGMRES and LEARNED stop once the relative residual is
-                                    under tol x value / 1000 (1..1000, default 333): past tol each binary64
-                                    step still lowers the answer's error against the exact solution, and a
-                                    third of tol keeps it near 1e-2 x tol (1e-10 at tol 1e-8, 1024 x 2048) */
+#define CG_OPT_STOP_PERMILLE 11  /* CG_METHOD_REFINED, FGMRES and LEARNED take tol x value / 1000 as their
+                                    residual target (1..1000, default 1000) */
+#define CG_OPT_ERROR_PICO 12     /* ... and stop only once, besides, the estimated relative L2 error of the
+                                    binary64 answer against the exact solution of the operator is at most
+                                    value x 1e-12 (default 100: 1e-10; 0: the residual alone decides). The
+                                    estimate is rho / (1 - rho) |delta| / |x| for the last correction delta
+                                    and its contraction rho against the one before; a residual under tol
+                                    alone does not bound the error (a forcing of fine-scale noise leaves a
+                                    smooth error whose residual is small). Each instance stops at the first
+                                    step where both hold, tested on the device before every step. */
 
 typedef struct cg_ctx cg_ctx;
 typedef struct cg_plan cg_plan;

@@ -6,7 +6,7 @@
 - The twelve months of that world in one batched plan: three months against the oracle the same way.
 - Four worlds of uneven size (512 x 1024, 768 x 1536, 1024 x 2048 twice; seeds 1-4, land 20, 27,
   33 and 40%), twelve months each in one batched plan per world: every instance's residual
-  recomputed on the host under 1e-8, one month per world within 1e-10 of the oracle.
+  recomputed on the host under 1e-8, its own month and month 3 (noise alone) within 1e-10 of the oracle.
 - 2048 x 4096: one instance against the oracle, exact linearity and bit-identical repeats; sixteen
   monthly forcings in one batched plan, each residual recomputed, two instances bit-identical to
   their single-instance solves.
@@ -108,14 +108,16 @@ def test_four_world_year(acc, rows, seed, land):
             assert infos[i].relative_residual < TOL and res < 1.01 * TOL, f"month {i}: {res}"
     finally:
         batch.close()
-    month = seed % 12
+    # The world's own month and month 3, whose band amplitude cos(pi / 2) vanishes: a forcing of
+    # noise alone, whose residual under tol leaves the largest error (the error bound must hold).
     ladder = [to_oracle(s) for s in p.levels]
-    ladder[0] = st.with_forcing(forcings[month])
-    ref, _, _ = oracle.solve64(ladder, 1e-12)
-    err = rel_l2(answers[month], ref)
-    print(f"OCEAN four-world {2 * rows}x{rows}: cycles {min(i.cycles for i in infos)}-{max(i.cycles for i in infos)}, "
-          f"month {month} rel L2 {err:.3e}")
-    assert err <= PARITY_L2
+    for month in sorted({seed % 12, 3}):
+        ladder[0] = st.with_forcing(forcings[month])
+        ref, _, _ = oracle.solve64(ladder, 1e-12)
+        err = rel_l2(answers[month], ref)
+        print(f"OCEAN four-world {2 * rows}x{rows}: cycles {min(i.cycles for i in infos)}-"
+              f"{max(i.cycles for i in infos)}, month {month}: {infos[month].cycles} cycles, rel L2 {err:.3e}")
+        assert err <= PARITY_L2, f"month {month}"
 
 
 def test_largest_grid_single_instance(acc):

@@ -301,3 +301,51 @@ def test_option_changes_rebuild_the_step_graphs(acc):
     assert np.array_equal(x_again, x_first)
     assert other.relative_residual < 1e-11
     assert rel_l2(x_other, x_first) <= 1e-9
+
+
+def test_error_bound_decides_where_the_residual_alone_stops_short(acc):
+    """A month whose stress band vanishes (cos(2 pi 3 / 12) = 0: the forcing is the noise alone) has a
+    residual under tol long before its answer is within 1e-10 of the exact one. With the error bound
+    (CG_OPT_ERROR_PICO, default 1e-10) the solve runs on until its estimated error is under it, and
+    the true error (against the binary64 oracle) is; with the residual alone (0) it stops earlier."""
+    from cartogenesis_amd import CG_OPT_ERROR_PICO
+    from cartogenesis_amd.problem import monthly_forcings
+    p = circulation_problem(512, seed=1, land_fraction=0.2)
+    f3 = monthly_forcings(p, months=(3,))[0]
+    ladder = [to_oracle(s) for s in p.levels]
+    ladder[0] = ladder[0].with_forcing(f3)
+    ref = oracle.solve64(ladder, 1e-13)[0]
+    plan = Plan(acc, p.levels)
+    try:
+        plan.set_forcing(f3)
+        bounded = plan.solve(1e-8, CG_METHOD_REFINED)
+        xb = plan.read()[0]
+        acc.set_option(CG_OPT_ERROR_PICO, 0)
+        try:
+            plain = plan.solve(1e-8, CG_METHOD_REFINED)
+            xp = plan.read()[0]
+        finally:
+            acc.set_option(CG_OPT_ERROR_PICO, 100)
+    finally:
+        plan.close()
+    eb, ep = rel_l2(xb, ref), rel_l2(xp, ref)
+    print(f"OCEAN error bound 1024x512 month 3: {bounded.cycles} cycles, rel L2 {eb:.3e}; residual alone "
+          f"{plain.cycles} cycles, rel L2 {ep:.3e}")
+    assert bounded.relative_residual < 1e-8 and plain.relative_residual < 1e-8
+    assert eb <= PARITY_L2
+    assert plain.cycles < bounded.cycles and ep > eb
+
+
+def test_warm_start_at_its_answer_runs_no_step(acc):
+    p = circulation_problem(256, seed=2)
+    plan = Plan(acc, p.levels)
+    try:
+        first = plan.solve(1e-9, CG_METHOD_REFINED)
+        x1 = plan.read()[0]
+        again = plan.solve(1e-9, CG_METHOD_REFINED, warm=True)
+        x2 = plan.read()[0]
+    finally:
+        plan.close()
+    assert first.cycles > 0 and again.cycles == 0
+    assert again.relative_residual < 1e-9
+    assert np.array_equal(x1.view(np.uint64), x2.view(np.uint64))
