@@ -264,93 +264,148 @@ def unet_flops(rows: int, cols: int, base: int = 32) -> dict:
             for name, c1, c2, co, _ in layer_table(base)}
 
 
-def learned_leg(acc, prob, tol: float, plain_value: float, rows: int, seeds: int, steps: int, reps: int):
-    """The learned preconditioner: a U-Net trained on the GPU (the library's convolution kernels) on
-    refined solves of `seeds` synthetic worlds of rows x 2 rows, then FGMRES with and without it on
-    the benchmark's 1024x2048 problem, and the network's matrix-core rate at 1024x2048."""
+def coarse_net_flops(down: int, across: int, base: int = 32) -> int:
+    """Multiply-add flops (2 per product) of one coarse-network application on a down x across grid;
+    the first layer counts its 3 data channels."""
+    from cartogenesis_amd.learned import layer_table
+
+    level = {"enc0": 0, "enc1": 1, "enc2": 2, "mid_": 3, "dec2": 2, "dec1": 1, "dec0": 0, "out": 0}
+    return sum(2 * 9 * (3 if name == "enc0a" else c1 + c2) * co * (down >> level[name[:4]]) * (across >> level[name[:4]])
+               for name, c1, c2, co, _ in layer_table(base))
+
+
+def learned_leg(acc, prob, tol: float, worlds: int, steps: int, reps: int, batch_reps: int):
+    """The learned coarse-grid correction (CG_METHOD_LEARNED): a U-Net trained on the GPU (the library's
+    kernels) on the coarse grid's forcings of plain solves of `worlds` synthetic 1024x2048 worlds (seeds
+    100, 101, ..., config[1]'s distributions) and their exact solutions, then set on the benchmark's
+    plan: solves/s with and without it, one instance and the batched year, the steps, the error against
+    the binary64 oracle, and the network's matrix-core rate."""
     import torch
     import cartogenesis_amd as cg
-    from cartogenesis_amd.learned import LearnedPreconditioner, UNet, train, training_pairs
-    from cartogenesis_amd.ocean import Plan
+    import oracle
+    from cartogenesis_amd.learned import coarse_level, coarse_pairs, train_coarse
+    from cartogenesis_amd.ocean import BatchPlan, Plan
+    from cartogenesis_amd.problem import monthly_forcings
 
-    log(f"bench: learned: training on {seeds} worlds of {rows}x{2 * rows}, {steps} steps")
+    log(f"bench: learned: coarse pairs from {worlds} worlds of 1024x2048, {steps} training steps")
     t0 = time.perf_counter()
-    data = training_pairs(acc, rows, seeds=range(100, 100 + seeds))
+    data = coarse_pairs(acc, range(100, 100 + worlds), rows=ROWS, threads=int(os.environ.get("OMP_NUM_THREADS", "8")))
     t_data = time.perf_counter() - t0
-    net = UNet(32, seed=0)
     t0 = time.perf_counter()
-    losses = train(net, data, steps=steps, batch=4, lr=1e-3)
+    cn, losses = train_coarse(data, base=32, steps=steps, batch=16)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
+    level, shape = data["level"], data["shape"]
+    out = {"design": "coarse-grid correction: the U-Net on the cycle's %dx%d grid (level %d) in place of the grids "
+                     "below it, inside the graph-captured refined step" % (shape[1], shape[0], level),
+           "training": {"worlds": worlds, "solves": data["solves"], "pairs": data["pairs"], "grid": f"{shape[1]}x{shape[0]}",
+                        "seeds": f"100-{99 + worlds}", "steps": steps, "batch": 16, "data_s": round(t_data, 2),
+                        "train_s": round(t_train, 2), "loss_first": round(losses[0], 4),
+                        "loss_last20": round(float(np.mean(losses[-20:])), 4), "out_scale": round(data["out_scale"], 6)}}
     del data
     plan = Plan(acc, prob.levels)
-    out = {"training": {"worlds": seeds, "grid": f"{rows}x{2 * rows}", "steps": steps, "batch": 4,
-                        "data_s": round(t_data, 2), "train_s": round(t_train, 2),
-                        "loss_first": round(losses[0], 4), "loss_last20": round(float(np.mean(losses[-20:])), 4)}}
     try:
-        def timed(method):
+        plan.autotune(20)
+
+        def timed(method, n):
             info = plan.solve(tol, method)
             t = time.perf_counter()
-            for _ in range(reps):
+            for _ in range(n):
                 info = plan.solve(tol, method)
-            return reps / (time.perf_counter() - t), info
+            return n / (time.perf_counter() - t), info
 
-        fg_value, fg = timed(cg.CG_METHOD_FGMRES)
-        pre = LearnedPreconditioner(net, prob.levels[0], prob.depth)
-        pre.attach(plan)
-        ln_value, ln = timed(cg.CG_METHOD_LEARNED)
-        LearnedPreconditioner.detach(plan)
-        if pre.error is not None:
-            raise pre.error
-        out |= {"solves_per_s": round(ln_value, 3), "iterations": ln.cycles,
-                "relative_residual": ln.relative_residual,
-                "fgmres_without_network_solves_per_s": round(fg_value, 3), "fgmres_iterations": fg.cycles,
-                "plain_mg_solves_per_s": round(plain_value, 3),
-                "learned_vs_plain_mg": round(ln_value / plain_value, 4) if plain_value > 0 else None}
-        # The network alone at 1024x2048: every convolution, timed with events on torch's stream
-        # (the stream the kernels are launched on when none is given).
-        x = torch.zeros((1, ROWS, COLS, 16), dtype=torch.bfloat16, device="cuda")
-        x[..., 0] = 1.0
-        x[..., 2] = torch.randn(ROWS, COLS, device="cuda").to(torch.bfloat16)
-        net.forward(x)
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(5):
-            net.forward(x)
-        e1.record()
-        torch.cuda.synchronize()
-        fwd_ms = e0.elapsed_time(e1) / 5
-        flops = unet_flops(ROWS, COLS)
-        total = sum(flops.values())
-        # the largest single convolution: dec0a (96 -> 32 channels on the full grid)
-        a1 = torch.zeros((1, ROWS, COLS, 64), dtype=torch.bfloat16, device="cuda")
-        a2 = torch.zeros((1, ROWS, COLS, 32), dtype=torch.bfloat16, device="cuda")
-        a1.normal_()
-        a2.normal_()
-        net._conv("dec0a", a1, a2, 1, ROWS, COLS, torch.cuda.current_stream().cuda_stream)
-        e0.record()
+        plain_value, plain = timed(cg.CG_METHOD_REFINED, reps)
+        assert coarse_level(prob.levels) == level
+        cn.attach(plan, prob)
+        ln_value, ln = timed(cg.CG_METHOD_LEARNED, reps)
+        fell = cg.lib().cg_ocean_plan_net_fallbacks(plan.handle)
+        x64, _ = plan.read()
+        ladder = [oracle.Stencil(s.cells_across, s.cells_down, s.is_water, s.east_weight, s.west_weight,
+                                 s.north_weight, s.south_weight, s.forcing, s.centre_weight) for s in prob.levels]
+        oracle.set_threads(int(os.environ.get("OMP_NUM_THREADS", "8")))
+        ref, _, _ = oracle.solve64(ladder, 1e-12)
+        err = float(np.linalg.norm(x64 - ref) / np.linalg.norm(ref))
+        out |= {"solves_per_s": round(ln_value, 3), "steps": ln.cycles, "relative_residual": ln.relative_residual,
+                "rel_l2_vs_fp64_oracle": err, "fallbacks": fell,
+                "plain_mg_solves_per_s": round(plain_value, 3), "plain_mg_steps": plain.cycles,
+                "learned_vs_plain_mg": round(ln_value / plain_value, 4) if plain_value > 0 else None,
+                "ms_per_step_learned": round(1000.0 / ln_value / max(ln.cycles, 1), 4),
+                "ms_per_step_plain": round(1000.0 / plain_value / max(plain.cycles, 1), 4)}
+        log(f"bench: learned {ln_value:.1f} solves/s ({ln.cycles} steps, {fell} fallbacks, rel L2 {err:.2e}) against "
+            f"plain MG {plain_value:.1f} ({plain.cycles} steps) in the same process")
+        # The network alone on its grid: one application's launches, timed with HIP events on the
+        # stream they are enqueued on (apply_coarse_net synchronises its own copies).
+        L = prob.levels[level]
+        f = np.where(L.is_water != 0, 1.0, 0.0).astype(np.float32)
+        z = np.empty(L.cells, np.float32)
+        cg.lib().cg_ocean_plan_apply_coarse_net(plan.handle, f, z)
+        t = time.perf_counter()
         for _ in range(20):
-            net._conv("dec0a", a1, a2, 1, ROWS, COLS, torch.cuda.current_stream().cuda_stream)
-        e1.record()
-        torch.cuda.synchronize()
-        conv_us = e0.elapsed_time(e1) / 20 * 1000.0
-        conv_tf = flops["dec0a"] / (conv_us * 1e-6) / 1e12
-        out["mfma_roofline"] = {
-            "bound": "mfma", "kernel": "k_conv3x3_tile<2> (dec0a: 96 -> 32 channels, 1024x2048, bf16 in, fp32 "
-                                      "accumulate)",
-            "achieved": round(conv_tf, 2), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(conv_tf / BF16_PEAK_TFLOPS, 4), "avg_launch_us": round(conv_us, 2),
-            "flops_per_launch": flops["dec0a"],
-            # rocprofv3 FETCH_SIZE / WRITE_SIZE of this launch, scaled by the copy calibration
-            "traffic": 554620928, "algorithmic_bytes_per_launch": 536926208,
-            "network_forward_ms": round(fwd_ms, 3), "network_flops": total,
-            "network_tflops": round(total / (fwd_ms * 1e-3) / 1e12, 2)}
-        log(f"bench: learned {ln_value:.1f} solves/s ({ln.cycles} iterations) against FGMRES {fg_value:.1f} "
-            f"({fg.cycles}) and plain MG {plain_value:.1f}; U-Net forward {fwd_ms:.2f} ms "
-            f"({total / (fwd_ms * 1e-3) / 1e12:.1f} TFLOP/s), dec0a {conv_us:.1f} us ({conv_tf:.1f} TFLOP/s)")
+            cg.lib().cg_ocean_plan_apply_coarse_net(plan.handle, f, z)
+        apply_us = (time.perf_counter() - t) / 20 * 1e6
+        fl = coarse_net_flops(L.cells_down, L.cells_across)
+        out["mfma_roofline_network"] = {
+            "bound": "mfma (latency-bound at this size)", "flops_per_application": fl,
+            "application_us_with_copies": round(apply_us, 2),
+            "achieved": round(fl / (apply_us * 1e-6) / 1e12, 3), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(fl / (apply_us * 1e-6) / 1e12 / BF16_PEAK_TFLOPS, 6)}
     finally:
         plan.close()
+    if batch_reps > 0:
+        forcings = monthly_forcings(prob)
+        bp = BatchPlan(acc, prob.levels, len(forcings))
+        try:
+            for i, f in enumerate(forcings):
+                bp.set_forcing(i, f)
+            bp.autotune(5)
+            cn.attach(bp, prob)
+            res = {}
+            for name, m in (("plain", cg.CG_METHOD_REFINED), ("learned", cg.CG_METHOD_LEARNED)):
+                infos = (cg.CgSolution * len(forcings))()
+                cg.lib().cg_ocean_plan_solve_all(bp.handle, tol, m, infos)
+                t = time.perf_counter()
+                for _ in range(batch_reps):
+                    cg.check(cg.lib().cg_ocean_plan_solve_all(bp.handle, tol, m, infos), acc.ctx, "solve_all")
+                res[name] = (len(forcings) * batch_reps / (time.perf_counter() - t), [int(i.cycles) for i in infos])
+            out["batched_year"] = {"plain_solves_per_s": round(res["plain"][0], 3), "plain_steps": res["plain"][1],
+                                   "learned_solves_per_s": round(res["learned"][0], 3),
+                                   "learned_steps": res["learned"][1],
+                                   "fallbacks": cg.lib().cg_ocean_plan_net_fallbacks(bp.handle),
+                                   "learned_vs_plain_mg": round(res["learned"][0] / res["plain"][0], 4)}
+            log(f"bench: learned batched year {res['learned'][0]:.1f} solves/s against plain {res['plain'][0]:.1f}")
+        finally:
+            bp.close()
+    # The largest U-Net convolution at the full 1024x2048 grid (dec0a, 96 -> 32 channels), the matrix-core
+    # kernel's rate where its size is not latency-bound.
+    out["mfma_roofline"] = conv_roofline(cn.net)
     return out
+
+
+def conv_roofline(net) -> dict:
+    import torch
+
+    flops = unet_flops(ROWS, COLS)["dec0a"]
+    a1 = torch.randn((1, ROWS, COLS, 64), device="cuda").to(torch.bfloat16)
+    a2 = torch.randn((1, ROWS, COLS, 32), device="cuda").to(torch.bfloat16)
+    st = torch.cuda.current_stream().cuda_stream
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    net._conv("dec0a", a1, a2, 1, ROWS, COLS, st)
+    e0.record()
+    for _ in range(20):
+        net._conv("dec0a", a1, a2, 1, ROWS, COLS, st)
+    e1.record()
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) / 20 * 1000.0
+    tf = flops / (us * 1e-6) / 1e12
+    algo = 2 * ROWS * COLS * (64 + 32 + 32) + 2 * 9 * 32 * 96
+    # the memory roof at this intensity: flops per algorithmic byte x 8 TB/s
+    mem_roof = flops / algo * HBM_PEAK_GBS * 1e9 / 1e12
+    return {"bound": "hbm at this intensity (%.0f flop/B, under the %.0f flop/B ridge of 2.5 PFLOP/s over 8 TB/s)"
+                     % (flops / algo, BF16_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9)),
+            "kernel": "k_conv3x3_tile<2> (dec0a: 96 -> 32 channels, 1024x2048, bf16 in, fp32 accumulate)",
+            "achieved": round(tf, 2), "peak": round(min(BF16_PEAK_TFLOPS, mem_roof), 1), "unit": "TFLOP/s",
+            "frac": round(tf / min(BF16_PEAK_TFLOPS, mem_roof), 4), "frac_of_mfma_peak": round(tf / BF16_PEAK_TFLOPS, 4),
+            "avg_launch_us": round(us, 2), "flops_per_launch": flops, "algorithmic_bytes_per_launch": algo}
 
 
 def main() -> int:
@@ -385,9 +440,9 @@ def main() -> int:
                     help="only the U-Net's largest convolution (dec0a at 1024x2048, random weights), 5 launches: "
                          "the short program the convolution's counters are read from")
     ap.add_argument("--plain-value", type=float, default=0.0, help=argparse.SUPPRESS)
-    ap.add_argument("--learned-rows", type=int, default=256, help="training grid rows (columns = 2 rows)")
-    ap.add_argument("--learned-seeds", type=int, default=16, help="training worlds (seeds 100, 101, ...)")
-    ap.add_argument("--learned-steps", type=int, default=300, help="training steps (batch 4)")
+    ap.add_argument("--learned-worlds", type=int, default=64,
+                    help="1024x2048 training worlds of the coarse network (seeds 100, 101, ...)")
+    ap.add_argument("--learned-steps", type=int, default=1000, help="training steps (batch 16)")
     ap.add_argument("--worlds", action=argparse.BooleanOptionalAction, default=True,
                     help="time the four-world year: 12 months on each of 512x1024, 768x1536, 1024x2048 and "
                          "1024x2048 (seeds 1-4, land 20/27/33/40%%) (default on)")
@@ -448,8 +503,8 @@ def main() -> int:
         prob = circulation_problem(ROWS, COLS, seed=0, land_fraction=0.3)
         acc = HipOcean.create_or_null(0).accelerator
         try:
-            out = learned_leg(acc, prob, args.tol, args.plain_value, args.learned_rows, args.learned_seeds,
-                              args.learned_steps, reps=5)
+            out = learned_leg(acc, prob, args.tol, args.learned_worlds, args.learned_steps, reps=args.steps,
+                              batch_reps=args.batch_reps)
         finally:
             acc.close()
         print(json.dumps(out), flush=True)
@@ -544,8 +599,8 @@ def main() -> int:
         # process holds one runtime (the first loaded), so the child imports torch before the library.
         log("bench: learned preconditioner leg (child process)")
         cmd = [sys.executable, os.path.abspath(__file__), "--learned-only", "--tol", str(args.tol),
-               "--plain-value", repr(value), "--learned-rows", str(args.learned_rows),
-               "--learned-seeds", str(args.learned_seeds), "--learned-steps", str(args.learned_steps)]
+               "--steps", str(args.steps), "--batch-reps", str(args.batch_reps),
+               "--learned-worlds", str(args.learned_worlds), "--learned-steps", str(args.learned_steps)]
         proc = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=600)
         if proc.returncode != 0:
             raise RuntimeError(f"the learned leg failed with status {proc.returncode}")

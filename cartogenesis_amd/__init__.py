@@ -19,11 +19,11 @@ LIB_PATH = PKG / "libcgocean.so"
 CG_OK, CG_DECLINED, CG_CANCELLED = 0, 1, 2
 CG_ERR_ARG, CG_ERR_DEVICE, CG_ERR_NOMEM, CG_ERR_STATE = -1, -2, -3, -4
 CG_METHOD_REFERENCE, CG_METHOD_RESCUE, CG_METHOD_REFINED, CG_METHOD_FGMRES, CG_METHOD_KRYLOV = 0, 1, 2, 3, 4
-CG_METHOD_LEARNED = 5
+CG_METHOD_LEARNED, CG_METHOD_LEARNED_FGMRES = 5, 6
 CG_OPT_MIN_CELLS, CG_OPT_TILE_ROWS, CG_OPT_USE_GRAPH = 1, 2, 3
 CG_OPT_FGMRES_RESTART, CG_OPT_TIME_KERNELS = 4, 6
 CG_OPT_FUSED, CG_OPT_LEVEL_PASSES, CG_OPT_FUSE_OUTER, CG_OPT_DIRECT_BOTTOM = 7, 8, 9, 10
-CG_OPT_STOP_PERMILLE, CG_OPT_ERROR_PICO = 11, 12
+CG_OPT_STOP_PERMILLE, CG_OPT_ERROR_PICO, CG_OPT_NET_FALLBACK = 11, 12, 13
 
 
 class CgSolution(ctypes.Structure):
@@ -78,6 +78,11 @@ SIGNATURES = {
     "cg_ocean_plan_set_stress": (_int, [_vp, _dbl, _dbl, _f64p, _f64p, _vp]),
     "cg_ocean_plan_velocities": (_int, [_vp, _dbl, _dbl, _vp, _vp]),
     "cg_ocean_plan_set_preconditioner": (_int, [_vp, PRECOND_FN, _vp]),
+    "cg_ocean_plan_set_coarse_net": (_int, [_vp, _int, _int, _vp, _size, _vp, _dbl]),
+    "cg_unet_weight_count": (_int, [_int]),
+    "cg_ocean_plan_apply_coarse_net": (_int, [_vp, _f32p, _f32p]),
+    "cg_ocean_plan_capture_coarse": (_int, [_vp, _int, _dbl, _int, _vp]),
+    "cg_ocean_plan_net_fallbacks": (_int, [_vp]),
     "cg_conv3x3": (_int, [_vp, _int, _vp, _int, _int, _int, _int, _vp, _int, _int, _vp, _int, _vp, _vp, _vp]),
     "cg_conv3x3_in3": (_int, [_vp, _int, _int, _int, _vp, _int, _int, _vp, _vp]),
     "cg_conv3x3_up": (_int, [_vp, _int, _vp, _int, _int, _int, _int, _vp, _int, _int, _vp, _int, _vp, _vp, _vp]),

@@ -18,7 +18,7 @@ CSRC = PKG / "csrc"
 INCLUDE = ROOT / "include"
 LIB = PKG / "libcgocean.so"
 
-HIP_SOURCES = ["runtime.hip", "unet.hip"]
+HIP_SOURCES = ["runtime.hip", "unet.hip", "coarse_net.hip"]
 CXX_SOURCES = ["host_builders.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", f"-I{INCLUDE}", f"-I{CSRC}"]

@@ -111,14 +111,19 @@ struct SumReduce {
 };
 
 // Per instance: whether the refinement goes on, how many steps it has run, whether the step the
-// gate let run computes the error estimate's sums, and the sum of squared corrections of the step
-// before (the contraction of the next one is measured against it; 0 when it had no sums).
+// gate let run computes the error estimate's sums, whether a learned solve stopped it for its
+// stalling residual (fell), the sums of squared corrections of the two steps before (the
+// contraction of the next one is measured against them; 0 where a step had no sums) and the
+// residuals of the two steps before.
 struct StepGate {
     int active;
     int steps;
     int want;
-    int pad;
+    int fell;
     double dprev2;
+    double dprev2b;
+    double res_prev;
+    double res_prev2;
 };
 
 // Steps compute the estimate's sums only once the residual is within this factor of its threshold
@@ -130,7 +135,8 @@ constexpr double kSumsAhead = 100.0;
 // previous step's residual max |u c| under thr[inst] (tol x max |f c|), and, once a correction has
 // been applied, its estimated relative L2 error under etol: for a contraction rho per step,
 // e_k = rho e_(k-1) and delta_k = e_(k-1) - e_k, so |e_k| = rho / (1 - rho) |delta_k|, with rho the
-// ratio of successive correction norms. A residual that is not a number stops the instance too.
+// larger of the last two ratios of successive correction norms (a contraction that varies from step
+// to step, as a learned correction's does, is not underestimated). A residual that is not a number stops the instance too.
 // Reads the previous step's groups and state `in`; the wave with `writer` set writes state `out`
 // and the active flag the rest of the step reads.
 struct GateArgs {
@@ -142,14 +148,18 @@ struct GateArgs {
     int *want_sums;  // per instance: the step's outer kernel computes the sums
     const double *thr;
     double etol;  // <= 0: the residual alone decides
+    // A learned step's fallback: from the fourth step on, a residual above fall_rate squared times
+    // the one two steps before (a rate of fall_rate a step over two steps) stops the instance with
+    // `fell` set (the host goes on from its answer with the plain cycle). fall_rate <= 0: never.
+    double fall_rate;
     int res_stride, sums_stride;
 };
 
 __device__ bool gate_decide(const GateArgs &g, int inst, bool writer) {
     const int lane = threadIdx.x & 63;
     const StepGate s = g.in[inst];
-    bool go = false, want = false;
-    double d2 = 0.0;
+    bool go = false, want = false, fell = s.fell != 0;
+    double d2 = 0.0, res = 0.0;
     if (s.active) {
         const unsigned long long r = wave_max_u64(g.res[inst * g.res_stride + lane]);
         double x2 = 0.0;
@@ -157,7 +167,7 @@ __device__ bool gate_decide(const GateArgs &g, int inst, bool writer) {
             d2 = wave_sum_f64(g.sums[inst * g.sums_stride + lane]);
             x2 = wave_sum_f64(g.sums[inst * g.sums_stride + kMaxSlots + lane]);
         }
-        const double res = __longlong_as_double((long long)r);
+        res = __longlong_as_double((long long)r);
         const double thr = g.thr[inst];
         bool err_ok = true;
         if (g.etol > 0.0 && s.steps > 0) {
@@ -165,6 +175,10 @@ __device__ bool gate_decide(const GateArgs &g, int inst, bool writer) {
                 err_ok = false;  // the step ran without the sums: no estimate yet
             } else if (x2 > 0.0) {
                 double rho = s.dprev2 > 0.0 ? sqrt(d2 / s.dprev2) : 0.5;
+                if (s.dprev2 > 0.0 && s.dprev2b > 0.0) {
+                    const double before = sqrt(s.dprev2 / s.dprev2b);
+                    rho = before > rho ? before : rho;
+                }
                 rho = rho < 0.95 ? rho : 0.95;
                 err_ok = sqrt(d2 / x2) * (rho / (1.0 - rho)) <= g.etol;
             } else {
@@ -172,6 +186,10 @@ __device__ bool gate_decide(const GateArgs &g, int inst, bool writer) {
             }
         }
         go = !(res != res || (res < thr && err_ok));
+        if (go && g.fall_rate > 0.0 && s.steps >= 3 && !(res <= g.fall_rate * g.fall_rate * s.res_prev2)) {
+            go = false;
+            fell = true;
+        }
         want = go && g.etol > 0.0 && res < kSumsAhead * thr;
     }
     if (writer && lane == 0) {
@@ -179,8 +197,11 @@ __device__ bool gate_decide(const GateArgs &g, int inst, bool writer) {
         o.active = go ? 1 : 0;
         o.steps = s.steps + (go ? 1 : 0);
         o.want = want ? 1 : 0;
-        o.pad = 0;
+        o.fell = fell ? 1 : 0;
         o.dprev2 = go ? (s.want ? d2 : 0.0) : s.dprev2;
+        o.dprev2b = go ? s.dprev2 : s.dprev2b;
+        o.res_prev = s.active ? res : s.res_prev;
+        o.res_prev2 = s.active ? s.res_prev : s.res_prev2;
         g.out[inst] = o;
         g.active[inst] = o.active;
         g.want_sums[inst] = o.want;
@@ -190,18 +211,24 @@ __device__ bool gate_decide(const GateArgs &g, int inst, bool writer) {
 
 // A refinement's start: `clear` (the residual slots and sums of every instance) zeroed, and the
 // gate state the first step reads set to active with no steps, every active flag 1 and no sums.
-__global__ void k_check_init(StepGate *state, int *active, int *want, unsigned long long *clear, long nclear, int K) {
+// With `only` set, an instance whose only[i] is 0 starts stopped (a solve of some instances).
+__global__ void k_check_init(StepGate *state, int *active, int *want, unsigned long long *clear, long nclear, int K,
+                             const int *only) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nclear) clear[i] = 0ull;
     if (i < K) {
+        const int on = only ? (only[i] != 0) : 1;
         StepGate g;
-        g.active = 1;
+        g.active = on;
         g.steps = 0;
         g.want = 0;
-        g.pad = 0;
+        g.fell = 0;
         g.dprev2 = 0.0;
+        g.dprev2b = 0.0;
+        g.res_prev = 0.0;
+        g.res_prev2 = 0.0;
         state[i] = g;
-        active[i] = 1;
+        active[i] = on;
         want[i] = 0;
     }
 }

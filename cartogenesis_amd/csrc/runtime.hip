@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "cgocean.h"
+#include "coarse_net.hpp"
 #include "kernels.hpp"
 
 using namespace cg;
@@ -30,6 +31,8 @@ constexpr int kKrylovRestarts = 3;        // OC:535
 constexpr int kMaxInstances = 64;          // right-hand sides one plan solves together
 constexpr int kDirectMaxWater = 2048;     // largest coarsest grid solved by a dense inverse
 constexpr int kRefinedMostSteps = 1000;
+// The learned correction's fallback: a residual falling by less than this factor a step.
+constexpr double kNetFallbackRate = 0.8;
 
 struct Stencil {
     int across = 0, down = 0;
@@ -101,6 +104,7 @@ struct cg_ctx {
     int time_kernels = 0;
     int fused = 1;
     int fuse_outer = 1;  // the finest up leg and the binary64 outer step in one launch
+    int net_fallback = 1;  // a learned solve switches to the plain cycle when its residual stalls
     int direct_bottom = 1;  // the binary32 cycle's bottom grid solved by its dense inverse
     // The binary64-refined methods stop once the residual is under tol x stop_permille / 1000
     int stop_permille = 1000;
@@ -380,6 +384,15 @@ struct cg_plan {
     hipGraphExec_t step_exec[2] = {nullptr, nullptr};
     long graph_key = -1;
     double graph_etol = -1.0;
+    // The learned coarse-grid correction (cg_ocean_plan_set_coarse_net): the network, whether the
+    // steps being built or run use it, their graphs (same keys), and the last learned solve's
+    // fallbacks to the plain cycle.
+    CoarseNet net;
+    bool net_on = false;
+    hipGraphExec_t net_exec[2] = {nullptr, nullptr};
+    long net_graph_key = -1;
+    double net_graph_etol = -1.0;
+    int net_fallbacks = 0;
     // The direct bottom (CG_OPT_DIRECT_BOTTOM): level `bottom` solved by M = -(I - W)^-1 over its
     // bnw water cells (row-major, binary32); bnw = 0 when the plan has none.
     int bnw = 0;
@@ -476,6 +489,9 @@ void free_plan(cg_plan *p) {
         dfree(v);
     for (int i = 0; i < 2; ++i)
         if (p->step_exec[i]) (void)hipGraphExecDestroy(p->step_exec[i]);
+    for (int i = 0; i < 2; ++i)
+        if (p->net_exec[i]) (void)hipGraphExecDestroy(p->net_exec[i]);
+    coarse_net_free(p->net);
     delete p;
 }
 
@@ -771,6 +787,7 @@ GateArgs gate_args(const cg_plan *p) {
     g.want_sums = p->d_want;
     g.thr = p->d_thr;
     g.etol = p->ctx->error_tol;
+    g.fall_rate = (p->net_on && p->ctx->net_fallback) ? kNetFallbackRate : 0.0;
     return g;
 }
 
@@ -813,6 +830,10 @@ GateView gate_view(const unsigned long long *res_slots, const double *sums, cons
             err_ok = false;
         } else if (v.x2 > 0.0) {
             double rho = st.dprev2 > 0.0 ? std::sqrt(v.d2 / st.dprev2) : 0.5;
+            if (st.dprev2 > 0.0 && st.dprev2b > 0.0) {
+                const double before = std::sqrt(st.dprev2 / st.dprev2b);
+                rho = before > rho ? before : rho;
+            }
             rho = rho < 0.95 ? rho : 0.95;
             v.rho = rho;
             v.est = std::sqrt(v.d2 / v.x2) * (rho / (1.0 - rho));
@@ -1305,7 +1326,10 @@ hipError_t up_leg(cg_plan *p, int l, const float *in0, float *out0, bool fuse_ou
 
 int vcycle32_fused(cg_plan *p, bool timed, const float *in0, float *out0, bool fuse_outer) {
     cg_ctx *ctx = p->ctx;
-    const int b = p->bottom;
+    // With the learned correction the cycle goes down to the network's grid, whose correction the
+    // network proposes from its forcing, in place of the grids below it.
+    const bool net = p->net_on && p->net.level > 0;
+    const int b = net ? p->net.level : p->bottom;
     for (int l = 0; l < b; ++l) {
         TimeScope ts(ctx, timed && l == 0, kKindSmoothFromZero);
         hipError_t e = down_leg(p, l, in0, plan_tile_rows(p, l));
@@ -1315,7 +1339,16 @@ int vcycle32_fused(cg_plan *p, bool timed, const float *in0, float *out0, bool f
         }
         p->lv[l].cur = 1 - p->lv[l].cur;
     }
-    bottom_launch(p, in0, out0);
+    if (net) {
+        PlanLevel &N = p->lv[b];
+        const int rc = coarse_net_forward(p->net, N.f, N.x[N.cur], ctx->stream);
+        if (rc) {
+            set_err(ctx, "the coarse network's launches failed (%d)", rc);
+            return rc;
+        }
+    } else {
+        bottom_launch(p, in0, out0);
+    }
     for (int l = b - 1; l >= 0; --l) {
         TimeScope ts(ctx, timed && l == 0, kKindSmooth);
         hipError_t e = up_leg(p, l, in0, out0, fuse_outer, plan_tile_rows(p, l));
@@ -1446,23 +1479,35 @@ int refine_step(cg_plan *p, bool timed) {
 // outside the graph. The binary32 level buffers' ping-pong parities are the same at the start of
 // every step (each level runs an even number of launches per cycle), so one graph per binary64
 // parity suffices.
+void drop_graphs(cg_plan *p) {
+    for (int i = 0; i < 2; ++i) {
+        if (p->step_exec[i]) (void)hipGraphExecDestroy(p->step_exec[i]);
+        if (p->net_exec[i]) (void)hipGraphExecDestroy(p->net_exec[i]);
+        p->step_exec[i] = p->net_exec[i] = nullptr;
+    }
+}
+
+// The step graphs of the plain cycle, or with net_on those of the learned one.
 int build_step_graphs(cg_plan *p) {
     cg_ctx *ctx = p->ctx;
     // The settings that shape a step, the context's tile rows among them (grids the plan has no
     // tuned height for take theirs); graphs captured under others are rebuilt.
     const long key = ((long)ctx->level_passes << 12) | ((long)ctx->tile_rows << 4) | (ctx->fused << 3) |
-                     (ctx->fuse_outer << 2) | (ctx->direct_bottom << 1);
-    if (p->graph_key != key || p->graph_etol != ctx->error_tol) {
-        p->graph_etol = ctx->error_tol;
+                     (ctx->fuse_outer << 2) | (ctx->direct_bottom << 1) | (p->net_on ? ctx->net_fallback : 0);
+    hipGraphExec_t *exec = p->net_on ? p->net_exec : p->step_exec;
+    long &gkey = p->net_on ? p->net_graph_key : p->graph_key;
+    double &getol = p->net_on ? p->net_graph_etol : p->graph_etol;
+    if (gkey != key || getol != ctx->error_tol) {
+        getol = ctx->error_tol;
         for (int i = 0; i < 2; ++i)
-            if (p->step_exec[i]) {
-                (void)hipGraphExecDestroy(p->step_exec[i]);
-                p->step_exec[i] = nullptr;
+            if (exec[i]) {
+                (void)hipGraphExecDestroy(exec[i]);
+                exec[i] = nullptr;
             }
-        p->graph_key = key;
+        gkey = key;
     }
     for (int par = 0; par < 2; ++par) {
-        if (p->step_exec[par]) continue;
+        if (exec[par]) continue;
         int saved_x64 = p->x64cur;
         int saved_cur[kMaxLevels];
         for (int l = 0; l < p->n; ++l) saved_cur[l] = p->lv[l].cur;
@@ -1487,11 +1532,11 @@ int build_step_graphs(cg_plan *p) {
             set_err(ctx, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
             return CG_ERR_DEVICE;
         }
-        e = hipGraphInstantiate(&p->step_exec[par], g, nullptr, nullptr, 0);
+        e = hipGraphInstantiate(&exec[par], g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (e != hipSuccess) {
             set_err(ctx, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
-            p->step_exec[par] = nullptr;
+            exec[par] = nullptr;
             return CG_ERR_DEVICE;
         }
     }
@@ -1502,7 +1547,7 @@ int build_step_graphs(cg_plan *p) {
 // every instance active with no steps run, residual thresholds thr[i] (absolute: tol x max |f c|),
 // and every instance's residual and sums groups zeroed (the first outer step reduces into parity
 // 1 - first_par's, which the first gate reads).
-int gate_reset(cg_plan *p, int first_par, const std::vector<double> &thr) {
+int gate_reset(cg_plan *p, int first_par, const std::vector<double> &thr, const std::vector<char> *only = nullptr) {
     cg_ctx *ctx = p->ctx;
     const int K = p->K;
     // The thresholds cross the bus only when they change (a new forcing or tol).
@@ -1514,8 +1559,13 @@ int gate_reset(cg_plan *p, int first_par, const std::vector<double> &thr) {
         p->thr_up = thr;
     }
     const long nclear = (long)2 * K * kMaxSlots + (long)4 * K * kMaxSlots;  // residual groups, then sums
+    if (only) {
+        for (int i = 0; i < K; ++i) p->h_active[i] = (*only)[i] ? 1 : 0;
+        HIPCHK(ctx, hipMemcpyAsync(p->d_active, p->h_active, sizeof(int) * K, hipMemcpyHostToDevice, ctx->stream));
+    }
     hipLaunchKernelGGL(k_check_init, dim3(blocks_for(nclear, 256)), dim3(256), 0, ctx->stream,
-                       p->d_gate + (size_t)(1 - first_par) * K, p->d_active, p->d_want, p->d_res, nclear, K);
+                       p->d_gate + (size_t)(1 - first_par) * K, p->d_active, p->d_want, p->d_res, nclear, K,
+                       only ? p->d_active : nullptr);
     HIPCHK(ctx, hipGetLastError());
     return CG_OK;
 }
@@ -1525,7 +1575,8 @@ int gate_reset(cg_plan *p, int first_par, const std::vector<double> &thr) {
 // residual is under tol and its estimated error under the context's error bound, so each instance
 // runs exactly the steps it would alone, however the host groups the steps into batches between
 // its reads; a stopped instance's kernels return at once and its answer stays where it stopped.
-int refined_solve(cg_plan *p, double tol_asked, bool warm, cg_solution *infos, const volatile int *cancel) {
+int refined_solve_core(cg_plan *p, double tol_asked, bool warm, cg_solution *infos, const volatile int *cancel,
+                       bool learned, const std::vector<char> *only) {
     cg_ctx *ctx = p->ctx;
     const int K = p->K;
     const Stencil *st = p->lv[0].st;
@@ -1542,7 +1593,7 @@ int refined_solve(cg_plan *p, double tol_asked, bool warm, cg_solution *infos, c
     // The first outer step reads parity q0 (zero, from cold) and writes 1 - q0, where the steps start.
     const int q0 = p->x64cur;
     const int first_par = 1 - q0;
-    if ((rc = gate_reset(p, first_par, thr))) return rc;
+    if ((rc = gate_reset(p, first_par, thr, only))) return rc;
     float *const first_fc = outer_fused(p) ? level0_forcing(p, first_par) : p->lv[0].f;
     if (cold) {
         // From zero: x = 0, u = 0 - (0 - f) on water; its maxima in parity q0's groups.
@@ -1563,10 +1614,15 @@ int refined_solve(cg_plan *p, double tol_asked, bool warm, cg_solution *infos, c
     }
     const bool graph = ctx->use_graph && !ctx->time_kernels;
     p->gated = true;
+    p->net_on = learned;
     struct Ungate {
         cg_plan *p;
-        ~Ungate() { p->gated = false; }
+        ~Ungate() {
+            p->gated = false;
+            p->net_on = false;
+        }
     } ungate{p};
+    if (learned) p->net_fallbacks = 0;
     if (graph && (rc = build_step_graphs(p))) return rc;
     const unsigned long long *const res_groups = p->h_res;
     // Reads the gate states of both parities, the residual groups and the sums of every instance.
@@ -1595,6 +1651,8 @@ int refined_solve(cg_plan *p, double tol_asked, bool warm, cg_solution *infos, c
     // first batch's rate.
     std::vector<double> res_prev(K, cold ? 1.0 : -1.0), rate(K, -1.0);
     std::vector<char> done(K, 0);
+    if (only)
+        for (int i = 0; i < K; ++i) done[i] = !(*only)[i];
     int launched = 0, status = CG_OK, last_batch = 0;
     // A warm start reads its first outer step's residual before any step: it may already be under tol.
     if (!cold && (rc = read_back())) return rc;
@@ -1605,6 +1663,7 @@ int refined_solve(cg_plan *p, double tol_asked, bool warm, cg_solution *infos, c
             // about as many as its last solve from zero took (one fewer, so the batch ends before
             // the stop), else 8.
             for (int i = 0; i < K; ++i) {
+                if (done[i]) continue;
                 if (largest[i] > 0.0) batch = std::max(batch, p->last_steps[i] > 1 ? p->last_steps[i] - 1 : 8);
                 else done[i] = 1;
             }
@@ -1645,9 +1704,10 @@ int refined_solve(cg_plan *p, double tol_asked, bool warm, cg_solution *infos, c
             break;
         }
         batch = std::min(batch, kRefinedMostSteps - launched);
+        hipGraphExec_t *const exec = p->net_on ? p->net_exec : p->step_exec;
         for (int k = 0; k < batch; ++k) {
             if (graph) {
-                HIPCHK(ctx, hipGraphLaunch(p->step_exec[p->x64cur], ctx->stream));
+                HIPCHK(ctx, hipGraphLaunch(exec[p->x64cur], ctx->stream));
                 p->x64cur = 1 - p->x64cur;
             } else if ((rc = refine_step(p, true))) {
                 return rc;
@@ -1667,6 +1727,7 @@ int refined_solve(cg_plan *p, double tol_asked, bool warm, cg_solution *infos, c
     }
     if (launched == 0 && (rc = read_back())) return rc;
     for (int i = 0; i < K; ++i) {
+        if (only && !(*only)[i]) continue;
         const StepGate sg = state_of(i, launched);
         const GateView v = view_of(i, sg);
         infos[i] = cg_solution{};
@@ -1674,11 +1735,66 @@ int refined_solve(cg_plan *p, double tol_asked, bool warm, cg_solution *infos, c
         infos[i].refinements = sg.steps + 1;
         infos[i].relative_residual = largest[i] > 0.0 ? v.res / largest[i] : 0.0;
         infos[i].status = status;
+        infos[i].refinements = sg.fell;  // the caller (refined_solve) reads and resets it
         p->ans_par[i] = (first_par + sg.steps) & 1;
-        if (cold && status == CG_OK) p->last_steps[i] = sg.steps;
+        if (cold && status == CG_OK && !learned) p->last_steps[i] = sg.steps;
     }
     if (K == 1) p->x64cur = p->ans_par[0];
     p->last_answer = 1;
+    return CG_OK;
+}
+
+// The refined method, or with `learned` its learned coarse-grid correction, over every instance.
+// The multigrid fallback of the learned correction: an instance the gate stopped for its stalling
+// residual goes on from its answer with the plain cycle (the steps of both count), and if a learned
+// answer is not a number the whole solve starts again, plainly.
+int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *infos, const volatile int *cancel,
+                  bool learned = false) {
+    cg_ctx *ctx = p->ctx;
+    const int K = p->K;
+    if (learned) p->net_fallbacks = 0;
+    int rc = refined_solve_core(p, tol, warm, infos, cancel, learned, nullptr);
+    if (rc || !learned) {
+        for (int i = 0; i < K; ++i) infos[i].refinements = infos[i].cycles + 1;
+        return rc;
+    }
+    std::vector<char> fell(K, 0);
+    bool any = false, bad = false;
+    for (int i = 0; i < K; ++i) {
+        fell[i] = infos[i].refinements != 0 && infos[i].status == CG_OK;
+        any = any || fell[i];
+        bad = bad || !(infos[i].relative_residual == infos[i].relative_residual);
+        infos[i].refinements = infos[i].cycles + 1;
+    }
+    if (bad && ctx->net_fallback) {
+        p->net_fallbacks = K;
+        rc = refined_solve_core(p, tol, false, infos, cancel, false, nullptr);
+        for (int i = 0; i < K; ++i) infos[i].refinements = infos[i].cycles + 1;
+        return rc;
+    }
+    if (!any) return CG_OK;
+    // Each stopped instance's answer into the parity the plain solve reads, then that solve from
+    // those answers for them alone.
+    const long n = p->lv[0].st->cells;
+    const int q = p->x64cur;
+    int count = 0;
+    for (int i = 0; i < K; ++i) {
+        if (!fell[i]) continue;
+        ++count;
+        if (p->ans_par[i] != q)
+            HIPCHK(ctx, hipMemcpyAsync(p->x64[q] + i * n, p->x64[p->ans_par[i]] + i * n, sizeof(double) * n,
+                                       hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    p->net_fallbacks = count;
+    std::vector<cg_solution> more(infos, infos + K);
+    if ((rc = refined_solve_core(p, tol, true, more.data(), cancel, false, &fell))) return rc;
+    for (int i = 0; i < K; ++i) {
+        if (!fell[i]) continue;
+        infos[i].cycles += more[i].cycles;
+        infos[i].relative_residual = more[i].relative_residual;
+        infos[i].status = more[i].status;
+        infos[i].refinements = infos[i].cycles + 1;
+    }
     return CG_OK;
 }
 
@@ -1732,7 +1848,7 @@ int fgmres_solve(cg_plan *p, double tol_asked, bool warm, cg_solution *info, con
     // The error estimate of the refined method's gate, per outer step: its correction's norm against
     // the answer's, times rho / (1 - rho) for the contraction of successive corrections.
     const double etol = ctx->error_tol;
-    double d2prev = 0.0;
+    double d2prev = 0.0, d2prev2 = 0.0;
     bool err_ok = true;  // no correction applied yet
     while ((res >= tol || !err_ok) && cycles < kRefinedMostSteps) {
         if (cancel && *cancel) {
@@ -1777,11 +1893,13 @@ int fgmres_solve(cg_plan *p, double tol_asked, bool warm, cg_solution *info, con
                 err_ok = true;
             } else if (x2 > 0.0) {
                 double rho = d2prev > 0.0 ? std::sqrt(d2 / d2prev) : 0.5;
+                if (d2prev > 0.0 && d2prev2 > 0.0) rho = std::max(rho, std::sqrt(d2prev / d2prev2));
                 rho = rho < 0.95 ? rho : 0.95;
                 err_ok = std::sqrt(d2 / x2) * (rho / (1.0 - rho)) <= etol;
             } else {
                 err_ok = d2 <= 0.0;
             }
+            d2prev2 = d2prev;
             d2prev = d2;
         }
         cycles += m;
@@ -1915,6 +2033,7 @@ int cg_ocean_set_option(cg_ctx *ctx, int option, long value) {
         case CG_OPT_FUSED: ctx->fused = value ? 1 : 0; return CG_OK;
         case CG_OPT_FUSE_OUTER: ctx->fuse_outer = value ? 1 : 0; return CG_OK;
         case CG_OPT_DIRECT_BOTTOM: ctx->direct_bottom = value ? 1 : 0; return CG_OK;
+        case CG_OPT_NET_FALLBACK: ctx->net_fallback = value ? 1 : 0; return CG_OK;
         case CG_OPT_ERROR_PICO:
             if (value < 0) {
                 set_err(ctx, "the error bound must be >= 0 (in units of 1e-12; 0: the residual alone)");
@@ -1955,6 +2074,7 @@ long cg_ocean_get_option(cg_ctx *ctx, int option) {
         case CG_OPT_FUSE_OUTER: return ctx->fuse_outer;
         case CG_OPT_DIRECT_BOTTOM: return ctx->direct_bottom;
         case CG_OPT_STOP_PERMILLE: return ctx->stop_permille;
+        case CG_OPT_NET_FALLBACK: return ctx->net_fallback;
         case CG_OPT_ERROR_PICO: return (long)std::llround(ctx->error_tol * 1e12);
         case CG_OPT_LEVEL_PASSES: return ctx->level_passes;
         default: return -1;
@@ -2307,20 +2427,34 @@ int cg_ocean_plan_set_forcing_at(cg_plan *p, int instance, const float *forcing)
     return CG_OK;
 }
 
+// Whether the plan can run CG_METHOD_LEARNED: a coarse network set, and the fused cycle.
+static bool learned_ready(cg_plan *p) {
+    if (p->net.level <= 0) {
+        set_err(p->ctx, "CG_METHOD_LEARNED needs cg_ocean_plan_set_coarse_net first");
+        return false;
+    }
+    if (!(p->ctx->fused && p->fusable)) {
+        set_err(p->ctx, "CG_METHOD_LEARNED needs the fused cycle (row-constant weights, halving grids)");
+        return false;
+    }
+    return true;
+}
+
 int cg_ocean_plan_solve_all(cg_plan *p, double tol, int method, cg_solution *infos) {
     if (!p || !infos) return CG_ERR_ARG;
     cg_ctx *ctx = p->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    if (method != CG_METHOD_REFINED) {
-        set_err(ctx, "a plan's instances solve together by CG_METHOD_REFINED only");
+    if (method != CG_METHOD_REFINED && method != CG_METHOD_LEARNED) {
+        set_err(ctx, "a plan's instances solve together by CG_METHOD_REFINED or CG_METHOD_LEARNED only");
         return CG_ERR_ARG;
     }
     if (p->K > 1 && !(ctx->fused && p->fusable)) {
         set_err(ctx, "several instances need the fused cycle (row-constant weights, halving grids)");
         return CG_ERR_STATE;
     }
-    return refined_solve(p, tol, false, infos, nullptr);
+    if (method == CG_METHOD_LEARNED && !learned_ready(p)) return CG_ERR_STATE;
+    return refined_solve(p, tol, false, infos, nullptr, method == CG_METHOD_LEARNED);
 }
 
 int cg_ocean_plan_read_at(cg_plan *p, int instance, double *out64, float *out32) {
@@ -2362,8 +2496,12 @@ static int plan_solve_locked(cg_plan *p, double tol, int method, int warm, cg_so
     if (method == CG_METHOD_REFINED) return refined_solve(p, tol, warm != 0, info, cancel);
     if (method == CG_METHOD_FGMRES) return fgmres_solve(p, tol, warm != 0, info, cancel);
     if (method == CG_METHOD_LEARNED) {
+        if (!learned_ready(p)) return CG_ERR_STATE;
+        return refined_solve(p, tol, warm != 0, info, cancel, true);
+    }
+    if (method == CG_METHOD_LEARNED_FGMRES) {
         if (!p->pre_fn) {
-            set_err(ctx, "CG_METHOD_LEARNED needs cg_ocean_plan_set_preconditioner first");
+            set_err(ctx, "CG_METHOD_LEARNED_FGMRES needs cg_ocean_plan_set_preconditioner first");
             return CG_ERR_STATE;
         }
         return fgmres_solve(p, tol, warm != 0, info, cancel, true);
@@ -2371,6 +2509,104 @@ static int plan_solve_locked(cg_plan *p, double tol, int method, int warm, cg_so
     set_err(ctx, "unknown or unbuilt method %d", method);
     return CG_ERR_ARG;
 }
+
+int cg_unet_weight_count(int base) { return base >= 16 && base % 16 == 0 ? (int)net_weight_count(base) : -1; }
+
+int cg_ocean_plan_set_coarse_net(cg_plan *p, int level, int base, const uint16_t *weights, size_t count,
+                                 const float *depth, double out_scale) {
+    if (!p) return CG_ERR_ARG;
+    cg_ctx *ctx = p->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    drop_graphs(p);
+    coarse_net_free(p->net);
+    if (!weights) return CG_OK;  // removed
+    if (!p->fusable || level < 1 || level > p->bottom || level >= p->n) {
+        set_err(ctx, "the coarse network serves a grid 1..%d of the fused cycle (asked %d)", p->bottom, level);
+        return CG_ERR_ARG;
+    }
+    const Stencil *st = p->lv[level].st;
+    if (!coarse_net_shape_ok(st->down, st->across)) {
+        set_err(ctx, "grid %d is %d x %d: the network needs rows %% 8 == 0 and columns %% 128 == 0", level, st->down,
+                st->across);
+        return CG_ERR_ARG;
+    }
+    if (!(out_scale > 0.0) || !std::isfinite(out_scale)) {
+        set_err(ctx, "the network's output scale must be positive and finite");
+        return CG_ERR_ARG;
+    }
+    const int rc = coarse_net_setup(p->net, level, base, p->K, st->down, st->across, vstride(st), weights, count, depth,
+                                    st->water, (float)out_scale);
+    if (rc) {
+        set_err(ctx, "coarse network setup failed (%d): base %d needs %zu weights, %zu given", rc, base,
+                net_weight_count(base), count);
+        return rc;
+    }
+    return CG_OK;
+}
+
+int cg_ocean_plan_apply_coarse_net(cg_plan *p, const float *forcing, float *out) {
+    if (!p || !forcing || !out) return CG_ERR_ARG;
+    cg_ctx *ctx = p->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (p->net.level <= 0) {
+        set_err(ctx, "the plan has no coarse network");
+        return CG_ERR_STATE;
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    PlanLevel &N = p->lv[p->net.level];
+    const long cells = N.st->cells;
+    // Every instance gets the given forcing (the network runs them all); instance 0's answer returns.
+    for (int i = 0; i < p->K; ++i)
+        HIPCHK(ctx, hipMemcpyAsync(N.f + i * vstride(N.st), forcing, sizeof(float) * cells, hipMemcpyHostToDevice,
+                                   ctx->stream));
+    const int rc = coarse_net_forward(p->net, N.f, N.x[N.cur], ctx->stream);
+    if (rc) {
+        set_err(ctx, "the coarse network's launches failed (%d)", rc);
+        return rc;
+    }
+    HIPCHK(ctx, hipMemcpyAsync(out, N.x[N.cur], sizeof(float) * cells, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return CG_OK;
+}
+
+int cg_ocean_plan_capture_coarse(cg_plan *p, int level, double tol, int most_steps, float *out) {
+    if (!p || !out || most_steps < 1) return CG_ERR_ARG;
+    cg_ctx *ctx = p->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (p->K != 1 || !(ctx->fused && p->fusable) || level < 1 || level > p->bottom) {
+        set_err(ctx, "capture needs a plan of one instance on the fused cycle and a grid 1..%d", p->bottom);
+        return CG_ERR_ARG;
+    }
+    double largest;
+    int rc = plan_largest(p, &largest);
+    if (rc) return rc;
+    const Stencil *st0 = p->lv[0].st;
+    const long cells = p->lv[level].st->cells;
+    // The plain refinement from zero, step by step, the grid's forcing copied out after each step.
+    HIPCHK(ctx, hipMemsetAsync(p->x64[p->x64cur], 0, sizeof(double) * st0->cells, ctx->stream));
+    hipLaunchKernelGGL(k_fill_u64, dim3(2), dim3(kMaxSlots), 0, ctx->stream, p->d_res, 2 * kMaxSlots, 0ull);
+    if ((rc = outer_step(p, nullptr, outer_fused(p) ? level0_forcing(p, 1 - p->x64cur) : nullptr))) return rc;
+    int steps = 0;
+    while (steps < most_steps) {
+        const int slot = p->x64cur;
+        if ((rc = refine_step(p, false))) return rc;
+        const PlanLevel &N = p->lv[level];
+        HIPCHK(ctx, hipMemcpyAsync(out + (size_t)steps * cells, N.f, sizeof(float) * cells, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        ++steps;
+        unsigned long long bits;
+        if ((rc = read_res(p, slot, &bits))) return rc;
+        const double r = largest > 0.0 ? bits_to_double(bits) / largest : 0.0;
+        if (!(r >= tol)) break;
+    }
+    p->x64_valid = false;
+    return steps;
+}
+
+int cg_ocean_plan_net_fallbacks(cg_plan *p) { return p ? p->net_fallbacks : CG_ERR_ARG; }
 
 int cg_ocean_plan_set_preconditioner(cg_plan *p, cg_precond_fn fn, void *user) {
     if (!p) return CG_ERR_ARG;
@@ -2787,11 +3023,7 @@ int cg_ocean_plan_set_tile_rows(cg_plan *p, const int *rows) {
             return CG_ERR_ARG;
         }
     for (int l = 0; l < p->n; ++l) p->tuned_rows[l] = rows[l];
-    for (int i = 0; i < 2; ++i)
-        if (p->step_exec[i]) {
-            (void)hipGraphExecDestroy(p->step_exec[i]);
-            p->step_exec[i] = nullptr;
-        }
+    drop_graphs(p);
     return CG_OK;
 }
 
@@ -2806,13 +3038,7 @@ int cg_ocean_plan_autotune(cg_plan *p, int reps, int *rows_out) {
     }
     static const int kCandidates[] = {2, 4, 6, 8, 16, 24, 32};
     const bool graph = ctx->use_graph && !ctx->time_kernels;
-    auto drop_graphs = [&]() {
-        for (int i = 0; i < 2; ++i)
-            if (p->step_exec[i]) {
-                (void)hipGraphExecDestroy(p->step_exec[i]);
-                p->step_exec[i] = nullptr;
-            }
-    };
+
     // A refinement in progress from zero, as a solve starts one, so the steps timed are the solve's.
     const int K = p->K;
     const Stencil *st = p->lv[0].st;
@@ -2833,7 +3059,7 @@ int cg_ocean_plan_autotune(cg_plan *p, int reps, int *rows_out) {
     // Coordinate descent over the grids, finest first: each candidate height timed over `reps`
     // whole refinement steps (graph replays where graphs are on), the other grids as chosen so far.
     auto time_steps = [&](double *ms_out) -> int {
-        drop_graphs();
+        drop_graphs(p);
         int r;
         if (graph && (r = build_step_graphs(p))) return r;
         auto step = [&]() -> int {
@@ -2888,7 +3114,7 @@ int cg_ocean_plan_autotune(cg_plan *p, int reps, int *rows_out) {
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    drop_graphs();
+    drop_graphs(p);
     p->x64_valid = false;
     if (rc) return rc;
     if (rows_out)

@@ -1,5 +1,9 @@
-"""The learned preconditioner: a 4-level U-Net of 3x3 convolutions (32 base channels) that maps
-(mask, depth, residual) to a correction, used inside flexible GMRES (CG_METHOD_LEARNED).
+"""The learned part of the solve: a 4-level U-Net of 3x3 convolutions (32 base channels) that maps
+(mask, depth, residual) to a correction. Two uses: the learned coarse-grid correction of
+CG_METHOD_LEARNED (the network on a coarse grid of the refined method's cycle, in place of the
+grids below it; trained here on the forcings that grid sees in plain solves and their exact
+solutions; run natively inside the plan's step graphs), and a preconditioner inside flexible GMRES
+(CG_METHOD_LEARNED_FGMRES, a Python callback on the finest grid).
 
 Every convolution, forward and backward, and every pool, upsample and ReLU gradient runs in the
 library's kernels (cg_conv3x3, cg_conv3x3_adjoint, cg_conv3x3_wgrad, ...): bfloat16 activations,
@@ -404,3 +408,142 @@ class LearnedPreconditioner:
     def detach(plan) -> None:
         check(lib().cg_ocean_plan_set_preconditioner(plan.handle, PRECOND_FN(), None), plan.acc.ctx,
               "cg_ocean_plan_set_preconditioner")
+
+
+# ---- the learned coarse-grid correction (CG_METHOD_LEARNED) ---------------------------------------
+
+COARSE_MOST_CELLS = 64 * 128  # the network serves the finest grid of at most this many cells
+
+
+def coarse_level(levels, most_cells: int = COARSE_MOST_CELLS) -> int:
+    """The grid of a ladder the coarse network serves: the finest of at most most_cells cells whose
+    shape the network takes (rows a multiple of 8, columns of 128), not the finest grid; -1 if none."""
+    for k, st in enumerate(levels):
+        if k == 0 or st.cells > most_cells:
+            continue
+        if st.cells_down % 8 == 0 and st.cells_across % 128 == 0:
+            return k
+        return -1
+    return -1
+
+
+def coarse_depth(prob, level: int) -> np.ndarray:
+    """The depth on a coarse grid of the problem's ladder: block means of the finest grid's (0 on land)."""
+    from .ocean import coarse_mean
+
+    st = prob.levels[level]
+    return coarse_mean(np.asarray(prob.depth, np.float32), prob.cols, prob.rows, st.cells_across, st.cells_down)
+
+
+def _bf16_bits(t: torch.Tensor) -> np.ndarray:
+    return t.detach().to(BF16).contiguous().view(torch.int16).cpu().numpy().view(np.uint16)
+
+
+def coarse_pairs(acc, seeds: Sequence[int], rows: int = 1024, land_fraction: float = 0.3, tol: float = 1e-8,
+                 most_steps: int = 40, threads: int = 16):
+    """Training pairs of the coarse network from plain multigrid solves of the synthetic worlds of
+    rows x 2 rows (seeds given, annual forcing): every step's forcing of the grid the network serves
+    (cg_ocean_plan_capture_coarse) with the exact solution of that grid's problem (the refined method
+    on the ladder below it, to 1e-11), the forcing scaled to unit rms and the solution by the same rms
+    and the output scale (the median ratio of the solutions' rms to the forcings'). Returns a dict:
+    inputs (P, down, across, 16) bfloat16, targets and water (P, down, across) binary32 on the device,
+    out_scale, level, the grid's shape, and the counts of worlds and solves behind them."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from .ocean import Plan
+    from .problem import circulation_problem
+
+    seeds = [int(v) for v in seeds]
+    xs, fs, ws, ratios = [], [], [], []
+    level, shape = -1, None
+    solves = 0
+
+    def make(seed):
+        return circulation_problem(rows, seed=seed, land_fraction=land_fraction)
+
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as pool:
+        for prob in pool.map(make, seeds):
+            L = coarse_level(prob.levels)
+            if L < 0:
+                raise ValueError(f"no grid of the {rows}-row ladder suits the coarse network")
+            st = prob.levels[L]
+            if shape is None:
+                level, shape = L, (st.cells_down, st.cells_across)
+            elif (st.cells_down, st.cells_across) != shape:
+                raise ValueError("the worlds' coarse grids differ in shape")
+            cells = st.cells
+            plan = Plan(acc, prob.levels)
+            sub = Plan(acc, prob.levels[L:])
+            try:
+                out = np.empty((most_steps, cells), np.float32)
+                n = check(lib().cg_ocean_plan_capture_coarse(plan.handle, L, float(tol), most_steps,
+                                                             out.ctypes.data_as(ctypes.c_void_p)),
+                          acc.ctx, "cg_ocean_plan_capture_coarse")
+                solves += 1
+                depth = coarse_depth(prob, L)
+                water = (np.asarray(st.is_water) != 0)
+                fixed = np.stack([water.astype(np.float32), depth / DEPTH_SCALE], axis=1)
+                for f in out[:n]:
+                    rms_f = float(np.sqrt(np.mean(f.astype(np.float64) ** 2)))
+                    if rms_f == 0.0:
+                        continue
+                    sub.set_forcing(f)
+                    info = sub.solve(1e-11, CG_METHOD_REFINED)
+                    if not info.relative_residual < 1e-11:
+                        raise RuntimeError(f"coarse target solve stopped at {info.relative_residual:.2e}")
+                    x64, _ = sub.read()
+                    rms_f32 = np.float32(rms_f)
+                    xin = np.zeros((cells, 16), np.float32)
+                    xin[:, :2] = fixed
+                    xin[:, 2] = f / rms_f32
+                    xs.append(xin.reshape(*shape, 16))
+                    fs.append((x64 / rms_f).astype(np.float32).reshape(shape))
+                    ws.append(water.astype(np.float32).reshape(shape))
+                    ratios.append(float(np.sqrt(np.mean(x64 ** 2))) / rms_f)
+            finally:
+                sub.close()
+                plan.close()
+    if not xs:
+        raise RuntimeError("no training pairs")
+    out_scale = float(np.median(ratios))
+    dev = "cuda"
+    targets = torch.from_numpy(np.stack(fs) / np.float32(out_scale)).to(dev)
+    return {"inputs": torch.from_numpy(np.stack(xs)).to(dev).to(BF16), "targets": targets,
+            "water": torch.from_numpy(np.stack(ws)).to(dev), "out_scale": out_scale, "level": level,
+            "shape": shape, "worlds": len(seeds), "solves": solves, "pairs": len(xs)}
+
+
+class CoarseNetwork:
+    """A trained U-Net as a plan's learned coarse-grid correction (cg_ocean_plan_set_coarse_net):
+    the library holds a copy of its bfloat16 weights and runs it inside the plan's step graphs."""
+
+    def __init__(self, net: UNet, out_scale: float):
+        self.net = net
+        self.out_scale = float(out_scale)
+
+    def weights(self) -> np.ndarray:
+        return np.concatenate([_bf16_bits(self.net.wb[name]).reshape(-1) for name, *_ in self.net.layers])
+
+    def attach(self, plan, prob, level: Optional[int] = None) -> int:
+        """Sets the network on the plan's grid `level` (default coarse_level of the problem's ladder),
+        with that grid's depth; returns the level."""
+        L = coarse_level(prob.levels) if level is None else int(level)
+        w = self.weights()
+        depth = np.ascontiguousarray(coarse_depth(prob, L), np.float32)
+        check(lib().cg_ocean_plan_set_coarse_net(plan.handle, L, self.net.base, w.ctypes.data_as(ctypes.c_void_p),
+                                                 w.size, depth.ctypes.data_as(ctypes.c_void_p), self.out_scale),
+              plan.acc.ctx, "cg_ocean_plan_set_coarse_net")
+        return L
+
+    @staticmethod
+    def detach(plan) -> None:
+        check(lib().cg_ocean_plan_set_coarse_net(plan.handle, 0, 32, None, 0, None, 1.0), plan.acc.ctx,
+              "cg_ocean_plan_set_coarse_net")
+
+
+def train_coarse(data, base: int = 32, steps: int = 1000, batch: int = 16, lr: float = 1e-3, seed: int = 0):
+    """A U-Net trained on coarse_pairs' data (Adam on the mean squared error over water, the library's
+    forward and backward kernels); returns (CoarseNetwork, losses)."""
+    net = UNet(base, seed=seed)
+    losses = train(net, (data["inputs"], data["targets"], data["water"]), steps=steps, batch=batch, lr=lr, seed=seed)
+    return CoarseNetwork(net, data["out_scale"]), losses

@@ -54,12 +54,19 @@ extern "C" {
  * to 3 times while it stops short of tol: the heat's solve (OS:486-497). cg_solution.cycles counts
  * the iterations of every start, refinements the starts. */
 #define CG_METHOD_KRYLOV 4
-/* CG_METHOD_FGMRES with the plan's learned preconditioner (cg_ocean_plan_set_preconditioner) on
- * the first inner iteration of every outer step (the outer residual itself) and the binary32
- * V(2,2) cycle on the others; once an outer step fails
- * to halve the residual, the rest of the solve uses plain cycles only (the multigrid fallback).
- * cg_solution.cycles counts inner iterations of both kinds. Not a reference method. */
+/* CG_METHOD_REFINED whose binary32 cycle takes the plan's learned coarse-grid correction
+ * (cg_ocean_plan_set_coarse_net): down to the network's grid as usual, then the network's correction
+ * for that grid's forcing in place of the grids below it, then up as usual; every step one captured
+ * hipGraph, batched plans included. The multigrid fallback: once the residual falls by less than a
+ * factor 0.8 a step (CG_OPT_NET_FALLBACK), or an answer is not a number, the solve goes on (or
+ * starts again) with the plain cycle; cg_ocean_plan_net_fallbacks reports it. The stopping test and
+ * the binary64 fixed point are CG_METHOD_REFINED's. Not a reference method. */
 #define CG_METHOD_LEARNED 5
+/* CG_METHOD_FGMRES with a caller's preconditioner (cg_ocean_plan_set_preconditioner) on the first
+ * inner iteration of every outer step (the outer residual itself) and the binary32 V(2,2) cycle on
+ * the others; once an outer step fails to halve the residual, the rest of the solve uses plain
+ * cycles only. cg_solution.cycles counts inner iterations of both kinds. Not a reference method. */
+#define CG_METHOD_LEARNED_FGMRES 6
 
 /* ---- options (cg_ocean_set_option) ------------------------------------------------------------ */
 #define CG_OPT_MIN_CELLS 1       /* relax declines grids with fewer cells (GO:167-172); default 65536 */
@@ -82,6 +89,8 @@ extern "C" {
                                     always cycle the whole ladder (OC:740-781). */
 #define CG_OPT_STOP_PERMILLE 11  /* CG_METHOD_REFINED, FGMRES and LEARNED take tol x value / 1000 as their
                                     residual target (1..1000, default 1000) */
+#define CG_OPT_NET_FALLBACK 13   /* 1 (default): CG_METHOD_LEARNED falls back to the plain cycle as described
+                                    there; 0: it never does (to measure the network without the fallback) */
 #define CG_OPT_ERROR_PICO 12     /* ... and stop only once, besides, the estimated relative L2 error of the
                                     binary64 answer against the exact solution of the operator is at most
                                     value x 1e-12 (default 100: 1e-10; 0: the residual alone decides). The
@@ -314,7 +323,34 @@ int cg_coarse_mean(const float *fine, int fine_across, int fine_down, int coarse
  * 16, across a multiple of 16. Pointers are device pointers; `stream` is a hipStream_t (NULL: the
  * default stream); every call only enqueues work. */
 
-/* The preconditioner callback of CG_METHOD_LEARNED: z = M(v) for one binary32 vector of the
+/* The learned coarse-grid correction of CG_METHOD_LEARNED: a 4-level U-Net of 3x3 convolutions with
+ * b = base channels (a multiple of 16), in this order: enc0a 16->b, enc0b b->b, (2x2 mean pool)
+ * enc1a b->2b, enc1b, (pool) enc2a 2b->4b, enc2b, (pool) mid_a 4b->8b, mid_b, dec2a (mid_b upsampled,
+ * 8b, and enc2b, 4b) -> 4b, dec2b, dec1a (dec2b upsampled, 4b, and enc1b, 2b) -> 2b, dec1b,
+ * dec0a (dec1b upsampled, 2b, and enc0b, b) -> b, dec0b, out b->16 (every layer ReLU but out; nearest
+ * upsampling; the layout of cg_conv3x3), on grid `level` of the plan
+ * (1 .. the fused cycle's bottom grid; its rows a multiple of 8 and columns of 128). Input per cell:
+ * the grid's mask, depth[cell] / 6000 m (depth: host, the grid's cells in metres, NULL for zeros), and
+ * the grid's binary32 forcing over its root mean square; the correction is out_scale x that rms x the
+ * output's channel 0 on water, zero on land. weights: host bfloat16, every layer's [9][cout][cin] in
+ * table order, count = cg_unet_weight_count(base). Copied into the plan, which owns the network's
+ * buffers for all its instances; weights NULL removes the network. */
+int cg_ocean_plan_set_coarse_net(cg_plan *plan, int level, int base, const uint16_t *weights, size_t count,
+                                 const float *depth, double out_scale);
+/* The bfloat16 weight count of the U-Net of `base` channels (-1 for a base it does not take). */
+int cg_unet_weight_count(int base);
+/* The plan's coarse network on one forcing of its grid (host, the grid's cells), its correction into
+ * out (host): what the network contributes to a learned step. Between solves only. */
+int cg_ocean_plan_apply_coarse_net(cg_plan *plan, const float *forcing, float *out);
+/* Training data for the coarse network: the plain refined method from zero on a plan of one instance,
+ * step by step, with grid `level`'s forcing after every step copied into out (most_steps x the grid's
+ * cells, host) until the residual is under tol; returns the steps captured. Leaves the plan without an
+ * answer to start a warm solve from. */
+int cg_ocean_plan_capture_coarse(cg_plan *plan, int level, double tol, int most_steps, float *out);
+/* How many times the last CG_METHOD_LEARNED solve of the plan fell back to the plain cycle. */
+int cg_ocean_plan_net_fallbacks(cg_plan *plan);
+
+/* The preconditioner callback of CG_METHOD_LEARNED_FGMRES: z = M(v) for one binary32 vector of the
  * plan's finest grid (cells values), enqueued on `stream`; returns CG_OK or an error. */
 typedef int (*cg_precond_fn)(void *user, const float *v, float *z, void *stream);
 /* Sets (fn non-NULL) or clears the plan's learned preconditioner. Plans of one instance only. */

@@ -16,7 +16,7 @@ F = torch.nn.functional
 
 pytestmark = pytest.mark.gpu
 
-from cartogenesis_amd import CG_METHOD_FGMRES, CG_METHOD_LEARNED, check, lib  # noqa: E402
+from cartogenesis_amd import CG_METHOD_FGMRES, CG_METHOD_LEARNED_FGMRES, check, lib  # noqa: E402
 
 
 def _p(t):
@@ -246,7 +246,7 @@ def test_learned_preconditioner_inside_fgmres():
             x_plain, _ = plan.read()
             pre = LearnedPreconditioner(net, prob.levels[0], prob.depth)
             pre.attach(plan)
-            learned = plan.solve(1e-8, CG_METHOD_LEARNED)
+            learned = plan.solve(1e-8, CG_METHOD_LEARNED_FGMRES)
             assert pre.error is None, pre.error
             x_learned, _ = plan.read()
             LearnedPreconditioner.detach(plan)
@@ -263,7 +263,7 @@ def test_learned_preconditioner_inside_fgmres():
 
 
 def test_learned_method_errors_are_reported():
-    """CG_METHOD_LEARNED without a preconditioner is refused; a preconditioner that returns an error
+    """CG_METHOD_LEARNED_FGMRES without a preconditioner is refused; a preconditioner that returns an error
     fails the solve with it (both forced from the host); the plan then solves as before."""
     from cartogenesis_amd import CG_METHOD_REFINED, PRECOND_FN, CgError
     from cartogenesis_amd.ocean import HipOcean, Plan
@@ -275,7 +275,7 @@ def test_learned_method_errors_are_reported():
         plan = Plan(acc, prob.levels)
         try:
             with pytest.raises(CgError, match="needs cg_ocean_plan_set_preconditioner"):
-                plan.solve(1e-8, CG_METHOD_LEARNED)
+                plan.solve(1e-8, CG_METHOD_LEARNED_FGMRES)
             calls = []
 
             def failing(user, v, z, stream):
@@ -285,7 +285,7 @@ def test_learned_method_errors_are_reported():
             cb = PRECOND_FN(failing)
             check(lib().cg_ocean_plan_set_preconditioner(plan.handle, cb, None), acc.ctx, "set_preconditioner")
             with pytest.raises(CgError, match="learned preconditioner returned -1"):
-                plan.solve(1e-8, CG_METHOD_LEARNED)
+                plan.solve(1e-8, CG_METHOD_LEARNED_FGMRES)
             assert calls == [1]
             check(lib().cg_ocean_plan_set_preconditioner(plan.handle, PRECOND_FN(), None), acc.ctx,
                   "set_preconditioner")
