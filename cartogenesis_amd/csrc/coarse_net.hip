@@ -28,19 +28,22 @@ uint16_t host_bf16(float f) {
 }
 
 // Per instance (blockIdx.x): the root mean square of its forcing over the grid's cells, summed in
-// binary64 in a fixed order (thread-strided partial sums, then a tree), 1 where the forcing is zero.
-__global__ __launch_bounds__(256) void k_grid_rms(const float *__restrict__ f, long vstride, long cells,
-                                                  float *__restrict__ scale) {
-    __shared__ double sh[256];
+// binary64 in a fixed order (each of 1024 threads four cells apart a step, then a tree), 1 where the
+// forcing is zero.
+__global__ __launch_bounds__(1024) void k_grid_rms(const float *__restrict__ f, long vstride, long cells,
+                                                   float *__restrict__ scale) {
+    __shared__ double sh[1024];
     const float *g = f + blockIdx.x * vstride;
     double s = 0.0;
-    for (long i = threadIdx.x; i < cells; i += 256) {
-        const double v = (double)g[i];
-        s += v * v;
+    const long quads = cells / 4;
+    for (long i = threadIdx.x; i < quads; i += 1024) {
+        const float4 v = reinterpret_cast<const float4 *>(g)[i];
+        s += (double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z + (double)v.w * v.w;
     }
+    for (long i = 4 * quads + threadIdx.x; i < cells; i += 1024) s += (double)g[i] * g[i];
     sh[threadIdx.x] = s;
     __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
+    for (int h = 512; h > 0; h >>= 1) {
         if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
         __syncthreads();
     }
@@ -73,6 +76,10 @@ __global__ void k_unpack_grid(const float *__restrict__ y32, const uint8_t *__re
 }
 
 inline unsigned nblocks(long count, int per) { return (unsigned)((count + per - 1) / per); }
+
+// Decoders of fewer pixels than this read an upsampled copy: the staged kernel's 4 x 64-pixel
+// workgroups leave a small image's launch a handful of them.
+constexpr long kUpStagedPixels = 65536;
 
 template <typename T>
 int alloc(CoarseNet &net, T **p, size_t count) {
@@ -183,7 +190,7 @@ int coarse_net_setup(CoarseNet &net, int level, int base, int n, int down, int a
 int coarse_net_forward(const CoarseNet &net, const float *f, float *x, hipStream_t s) {
     const long cells = (long)net.down * net.across, px = cells * net.n;
     const int n = net.n;
-    hipLaunchKernelGGL(k_grid_rms, dim3(n), dim3(256), 0, s, f, net.vstride, cells, net.scale);
+    hipLaunchKernelGGL(k_grid_rms, dim3(n), dim3(1024), 0, s, f, net.vstride, cells, net.scale);
     hipLaunchKernelGGL(k_pack_grid, dim3(nblocks(px, 256)), dim3(256), 0, s, net.fixed, f, net.vstride, cells, n,
                        net.scale, net.x);
     if (hipGetLastError() != hipSuccess) return CG_ERR_DEVICE;
@@ -206,13 +213,14 @@ int coarse_net_forward(const CoarseNet &net, const float *f, float *x, hipStream
         return cg_pool2(in, ch, n, d, a, 1.0f, out, s);
     };
     // A decoder of level lvl on (its coarse input upsampled, the skip): read through the upsampling by
-    // the staged kernel where it serves the shape, else upsampled into a tensor first.
+    // the staged kernel where it serves the shape and the image is large (UNet.forward's rule), else
+    // upsampled into a tensor first.
     auto decoder = [&](int i, const uint16_t *coarse, const uint16_t *skip, int lvl) -> int {
         const NetLayer &L = net.layers[i];
         int d, a;
         dims(lvl, &d, &a);
         const uint16_t *w = net.w + net.w_off[i];
-        if (L.c1 % 32 == 0 && L.c2 % 32 == 0 && a % 64 == 0 && d % 4 == 0)
+        if (L.c1 % 32 == 0 && L.c2 % 32 == 0 && a % 64 == 0 && d % 4 == 0 && (long)n * d * a >= kUpStagedPixels)
             return cg_conv3x3_up(coarse, L.c1, skip, L.c2, n, d, a, w, L.cout, L.relu, net.act[i], L.cout, nullptr,
                                  nullptr, s);
         int rc = cg_upsample2(coarse, L.c1, n, d / 2, a / 2, 1.0f, net.upsampled[lvl], s);

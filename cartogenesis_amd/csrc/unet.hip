@@ -630,12 +630,22 @@ bool conv_shape_ok(int c1, int c2, int n, int down, int across, int cout) {
            across % 16 == 0 && cout > 0 && cout % 16 == 0;
 }
 
+// Output-channel tiles of 64, 32 or 16 a wavefront: the widest the channel count takes, narrowed while
+// the launch would hold fewer than 512 workgroups (small images: their waves, each summing all
+// 9 x cin products of its tile, are the whole launch). Each output's sum is the same whatever the
+// tile, so the choice changes no bits.
+int pick_nb(long row_blocks, int cout) {
+    int nb = cout % 64 == 0 ? 4 : (cout % 32 == 0 ? 2 : 1);
+    while (nb > 1 && row_blocks * (cout / (16 * nb)) < 512) nb >>= 1;
+    return nb;
+}
+
 template <int MT>
 void conv_k32_launch(const uint16_t *x1, int c1, const uint16_t *x2, int c2, int n, int down, int across,
                      const uint16_t *w, int cout, int relu, uint16_t *y1, int o1, uint16_t *y2, float *y32,
                      hipStream_t s) {
     const long tiles = (long)n * down * (across / (16 * MT));
-    const int nb = cout % 64 == 0 ? 4 : (cout % 32 == 0 ? 2 : 1);
+    const int nb = pick_nb((long)blocks(tiles, 4), cout);
     const dim3 grid(blocks(tiles, 4), cout / (16 * nb));
     if (nb == 4)
         hipLaunchKernelGGL((k_conv3x3_k32<MT, 4>), grid, dim3(256), 0, s, x1, c1, x2, c2, n, down, across, w, cout,
@@ -652,11 +662,16 @@ bool tile_ok(int c1, int c2, int down, int across) {
     return c1 % 32 == 0 && c2 % 32 == 0 && across % 64 == 0 && down % 4 == 0;
 }
 
+// Images of fewer pixels than this take the one-row kernel with 16-pixel tiles (more wavefronts)
+// rather than the LDS-staged 4 x 64-pixel tiles, except where the input is read through upsampling.
+constexpr long kSmallImage = 65536;
+
 template <bool ADJ>
 int conv_launch(const uint16_t *x1, int c1, const uint16_t *x2, int c2, int n, int down, int across,
                 const uint16_t *w, int cout, int relu, uint16_t *y1, int o1, uint16_t *y2, float *y32, hipStream_t s,
                 int up1 = 0) {
-    if (!ADJ && tile_ok(c1, c2, down, across)) {
+    const bool small = (long)n * down * across < kSmallImage && !up1;
+    if (!ADJ && tile_ok(c1, c2, down, across) && !small) {
         const int nb = cout % 64 == 0 ? 4 : (cout % 32 == 0 ? 2 : 1);
         const dim3 grid((unsigned)((long)n * (down / 4) * (across / 64)), cout / (16 * nb));
         if (nb == 4)
@@ -671,7 +686,7 @@ int conv_launch(const uint16_t *x1, int c1, const uint16_t *x2, int c2, int n, i
         return hipGetLastError() == hipSuccess ? CG_OK : CG_ERR_DEVICE;
     }
     if (!ADJ && c1 % 32 == 0 && c2 % 32 == 0) {
-        if (across % 64 == 0)
+        if (across % 64 == 0 && !small)
             conv_k32_launch<4>(x1, c1, x2, c2, n, down, across, w, cout, relu, y1, o1, y2, y32, s);
         else
             conv_k32_launch<1>(x1, c1, x2, c2, n, down, across, w, cout, relu, y1, o1, y2, y32, s);

@@ -132,11 +132,12 @@ class UNet:
         t["mid_b"] = self._conv("mid_b", t["mid_a"], None, n, *dims[3], s)
         def decoder(name, coarse, skip, up, lvl):
             """The decoder convolution of level lvl on (upsampled coarse, skip). Without keep it reads
-            the coarse tensor through the upsampling itself where the staged kernel serves the shape;
-            backward needs the upsampled tensor, so with keep it is formed."""
+            the coarse tensor through the upsampling itself where the staged kernel serves the shape
+            and the image has at least 65536 pixels (below, that kernel's few workgroups are slower
+            than upsampling first); backward needs the upsampled tensor, so with keep it is formed."""
             c1, c2, co, relu = self.spec[name]
             d_, a_ = dims[lvl]
-            if not keep and c1 % 32 == 0 and c2 % 32 == 0 and a_ % 64 == 0 and d_ % 4 == 0:
+            if not keep and c1 % 32 == 0 and c2 % 32 == 0 and a_ % 64 == 0 and d_ % 4 == 0 and n * d_ * a_ >= 65536:
                 y = torch.empty((n, d_, a_, co), dtype=BF16, device=self.device)
                 check(lib().cg_conv3x3_up(_p(coarse), c1, _p(skip), c2, n, d_, a_, _p(self.wb[name]), co, relu, _p(y),
                                           co, None, None, s), what=f"cg_conv3x3_up {name}")
