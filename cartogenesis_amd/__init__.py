@@ -101,7 +101,7 @@ SIGNATURES = {
     "cg_circulation_stencil": (_int, [_int, _int, _dbl, _dbl, _u8p, _f64p, _dbl, _vp, _f32p, _f32p, _f32p,
                                       _f32p, _f64p, _f32p]),
     "cg_coarse_water": (_int, [_u8p, _int, _int, _int, _int, _int, _u8p]),
-    "cg_bottom_inverse": (_int, [_int, _int, _u8p, _f32p, _f32p, _f32p, _f32p, _int, _int, _i32p, _f32p]),
+    "cg_bottom_inverse": (_int, [_int, _int, _u8p, _f32p, _f32p, _f32p, _f32p, _int, _int, _i32p, _f64p]),
     "cg_level_shapes": (_int, [_int, _int, _dbl, _dbl, _int, _i32p, _i32p, _f64p, _f64p]),
     "cg_heat_diffusivity": (_dbl, [_dbl, _dbl]),
     "cg_heat_stencil": (_int, [_int, _int, _dbl, _dbl, _u8p, _f32p, _vp, _dbl, _int, _dbl, _dbl, _vp, _vp,

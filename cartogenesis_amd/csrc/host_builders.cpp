@@ -428,7 +428,7 @@ int cg_problem_stress(int across, int down, uint64_t seed, int month, double *ea
 // CG_ERR_ARG on bad arguments.
 int cg_bottom_inverse(int across, int down, const uint8_t *water, const float *east, const float *west,
                       const float *north, const float *south, int max_n, int threads, int32_t *idx,
-                      float *m) {
+                      double *m) {
     if (across <= 0 || down <= 0 || !water || !east || !west || !north || !south || max_n < 0 || !idx || !m)
         return CG_ERR_ARG;
     const long cells = (long)across * down;
@@ -495,7 +495,7 @@ int cg_bottom_inverse(int across, int down, const uint8_t *water, const float *e
                     for (int k = i + 1; k <= last; ++k) v -= at(i, k) * y[k];
                     y[i] = v / at(i, i);
                 }
-                for (int i = 0; i < n; ++i) m[(size_t)i * n + j] = (float)(-y[i]);
+                for (int i = 0; i < n; ++i) m[(size_t)i * n + j] = -y[i];
             }
         });
     for (auto &th : pool) th.join();

@@ -1549,7 +1549,7 @@ constexpr int kDirectWaves = 16;  // outputs (wavefronts) per workgroup
 
 struct DirectArgs {
     const int *idx;      // water cells
-    const float *M;      // row-major
+    const double *M;     // row-major, binary64
     int n;
     int K;               // instances
     const float *f;      // forcing of the grid (instances f_stride apart)
@@ -1584,24 +1584,24 @@ __global__ __launch_bounds__(64 * kDirectWaves) void k_direct_bottom(DirectArgs 
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * kDirectWaves + (threadIdx.x >> 6);
     if (i >= n) return;
-    const float *row = a.M + (size_t)i * n;
-    float acc[kDirectInst] = {};
+    const double *row = a.M + (size_t)i * n;
+    double acc[kDirectInst] = {};
     if (NI == 1 || ni == 1) {
 #pragma unroll 8
-        for (int j = lane; j < n; j += 64) acc[0] = fmaf(row[j], fs[0][j], acc[0]);
+        for (int j = lane; j < n; j += 64) acc[0] = fma(row[j], (double)fs[0][j], acc[0]);
     } else {
 #pragma unroll 4
         for (int j = lane; j < n; j += 64) {
-            const float m = row[j];
+            const double m = row[j];
 #pragma unroll
-            for (int c = 0; c < kDirectInst; ++c) acc[c] = fmaf(m, fs[c][j], acc[c]);
+            for (int c = 0; c < kDirectInst; ++c) acc[c] = fma(m, (double)fs[c][j], acc[c]);
         }
     }
 #pragma unroll
     for (int c = 0; c < kDirectInst; ++c) {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off, 64);
-        if (lane == 0 && on[c]) a.x[(c0 + c) * a.x_stride + a.idx[i]] = acc[c];
+        if (lane == 0 && on[c]) a.x[(c0 + c) * a.x_stride + a.idx[i]] = (float)acc[c];
     }
 }
 

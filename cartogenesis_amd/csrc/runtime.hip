@@ -394,10 +394,10 @@ struct cg_plan {
     double net_graph_etol = -1.0;
     int net_fallbacks = 0;
     // The direct bottom (CG_OPT_DIRECT_BOTTOM): level `bottom` solved by M = -(I - W)^-1 over its
-    // bnw water cells (row-major, binary32); bnw = 0 when the plan has none.
+    // bnw water cells (row-major, binary64); bnw = 0 when the plan has none.
     int bnw = 0;
     int *d_bidx = nullptr;
-    float *d_bM = nullptr;
+    double *d_bM = nullptr;  // binary64
     // The fused cycle: tile levels 0..bottom-1 run fused down/up kernels, levels bottom..n-1 run
     // in one workgroup (k_bottom) with bottom_lds bytes of LDS; fusable is false when a level
     // has per-cell weights or does not halve both axes, and the unfused cycle runs instead.
@@ -582,7 +582,7 @@ int build_direct_bottom(cg_ctx *ctx, cg_plan *p) {
         w[3][c] = v.w;
     }
     std::vector<int32_t> idx(kDirectMaxN);
-    std::vector<float> m((size_t)kDirectMaxN * kDirectMaxN);
+    std::vector<double> m((size_t)kDirectMaxN * kDirectMaxN);
     const int threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     const int n = cg_bottom_inverse(st->across, st->down, st->h_water.data(), w[0].data(), w[1].data(),
                                     w[2].data(), w[3].data(), kDirectMaxN, threads, idx.data(), m.data());
@@ -591,7 +591,7 @@ int build_direct_bottom(cg_ctx *ctx, cg_plan *p) {
     if ((rc = dalloc(ctx, &p->d_bidx, n))) return rc;
     if ((rc = dalloc(ctx, &p->d_bM, (size_t)n * n))) return rc;
     HIPCHK(ctx, hipMemcpy(p->d_bidx, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-    HIPCHK(ctx, hipMemcpy(p->d_bM, m.data(), sizeof(float) * (size_t)n * n, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(p->d_bM, m.data(), sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice));
     p->bnw = n;
     return CG_OK;
 }

@@ -83,7 +83,8 @@ extern "C" {
 #define CG_OPT_DIRECT_BOTTOM 10  /* 1 (default): the binary32 cycle of CG_METHOD_REFINED and
                                     CG_METHOD_FGMRES ends at the largest grid of at most 32 x 64 cells,
                                     solved by its dense inverse (M = -(I - W)^-1 over its water cells,
-                                    built when the plan is created) in one chip-wide launch; 0: that
+                                    built when the plan is created, held and applied in binary64) in one
+                                    chip-wide launch; 0: that
                                     grid and the coarser ones are cycled in one workgroup down to the
                                     coarsest. The reference method and the Krylov preconditioner
                                     always cycle the whole ladder (OC:740-781). */
@@ -281,12 +282,12 @@ int cg_coarse_water(const uint8_t *fine, int fine_across, int fine_down, int coa
                     int coarse_down, int every_cell_water, uint8_t *coarse);
 /* The direct bottom's inverse, M = -(I - W)^-1 over a grid's water cells (no counterpart in the
  * reference, whose cycle relaxes its coarsest grid, OC:784-794): per-cell weights as an
- * OceanStencil holds them; writes the water cells (row-major) to idx[n] and M row-major (binary32,
- * computed in binary64 by banded LU of the M-matrix I - W) to m[n * n]. Returns n; 0 when there is
- * no water, more than max_n water cells, or I - W has a pivot that is not positive. */
+ * OceanStencil holds them; writes the water cells (row-major) to idx[n] and M row-major (binary64,
+ * by banded LU of the M-matrix I - W) to m[n * n]. Returns n; 0 when there is no water, more than
+ * max_n water cells, or I - W has a pivot that is not positive. */
 int cg_bottom_inverse(int across, int down, const uint8_t *water, const float *east, const float *west,
                       const float *north, const float *south, int max_n, int threads, int32_t *idx,
-                      float *m);
+                      double *m);
 /* The level shapes OceanCirculation.levels walks (OC:434-462): returns the count (<= max_levels)
  * and fills across/down/cell width/cell height per level, finest first. */
 int cg_level_shapes(int across, int down, double dx, double dy, int max_levels, int *level_across,
