@@ -111,6 +111,17 @@ SIGNATURES = {
     "cg_coarse_mean": (_int, [_f32p, _int, _int, _int, _int, _f32p]),
     "cg_problem_generate": (_int, [_int, _int, _u64, _dbl, _int, _vp, _vp, _vp]),
     "cg_problem_stress": (_int, [_int, _int, _u64, _int, _vp, _vp]),
+    # include/cgsave.h: the ocean's fields in the save format, host only.
+    "cg_save_crc32": (ctypes.c_uint32, [_vp, _size, ctypes.c_uint32]),
+    "cg_save_chunk_checksum": (ctypes.c_uint32, [ctypes.c_uint32, _int, _vp, _size]),
+    "cg_save_directory": (_int, [_int, _int, _vp, _int, ctypes.POINTER(ctypes.c_int64)]),
+    "cg_ocean_records_write": (_int, [_int, _f32p, _f32p, _f32p, _f32p, _vp, _size, ctypes.POINTER(_size)]),
+    "cg_ocean_records_read": (_int, [_vp, _size, _int, _f32p, _f32p, _f32p, _f32p, ctypes.c_char_p, _size]),
+    "cg_save_inspect": (_int, [_vp, _size, ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.POINTER(ctypes.c_int64),
+                               ctypes.POINTER(ctypes.c_int64), ctypes.c_char_p, _size]),
+    "cg_save_read_ocean": (_int, [_vp, _size, _f32p, _f32p, _f32p, _f32p, ctypes.c_char_p, _size]),
+    "cg_save_replace_ocean": (_int, [_vp, _size, _f32p, _f32p, _f32p, _f32p, _int, _vp, _size, ctypes.POINTER(_size),
+                                     ctypes.c_char_p, _size]),
 }
 
 _lib = None

@@ -19,7 +19,7 @@ INCLUDE = ROOT / "include"
 LIB = PKG / "libcgocean.so"
 
 HIP_SOURCES = ["runtime.hip", "unet.hip", "coarse_net.hip"]
-CXX_SOURCES = ["host_builders.cpp"]
+CXX_SOURCES = ["host_builders.cpp", "save_sections.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", f"-I{INCLUDE}", f"-I{CSRC}"]
 HIP_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -56,7 +56,7 @@ def build(verbose: bool = False) -> Path:
     with ThreadPoolExecutor(max_workers=len(jobs)) as pool:
         list(pool.map(run, jobs))
     tmp = LIB.with_suffix(".so.tmp")
-    run([hipcc(), "-shared", "-fPIC", "-pthread", "--offload-arch=gfx950", *map(str, objs), "-o", str(tmp)])
+    run([hipcc(), "-shared", "-fPIC", "-pthread", "--offload-arch=gfx950", *map(str, objs), "-lz", "-o", str(tmp)])
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
