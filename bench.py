@@ -5,9 +5,11 @@ psi = 0 to max|F - L psi| / max|F| < 1e-8, with the ladder, forcing and answer r
 (a device-resident plan). Prints one JSON line.
 
 `value` is plain GPU multigrid (binary32 V(2,2) cycles of the reference's ladder inside binary64
-iterative refinement). The learned preconditioner's leg runs in a child process and is reported
-under "learned": the U-Net trained on the GPU, FGMRES with and without it on the same problem, its
-solves/s over the plain multigrid's, and its largest convolution against the matrix-core roofline.
+iterative refinement). The learned leg runs in a child process and is reported under "learned" and
+at the top level (learned_solves_per_s, learned_vs_plain_mg): the U-Net trained on the GPU as the
+cycle's coarse-grid correction, the refined solve with it (one instance and the batched year) over
+the plain multigrid's, the network's rate, and its largest convolution at the full grid against the
+roofline that bounds it.
 """
 from __future__ import annotations
 
@@ -45,8 +47,8 @@ FUSED_BYTES = 34 * CELLS
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE and WRITE_SIZE (scaled by a 1 GiB copy),
 # for the default settings (two passes per grid), by the finest grid's tile rows; see
 # profiles/hbm_traffic_*.md.
-MEASURED_TRAFFIC = {"outer": 78736610, "up": {4: 30604038, 6: 30726311, 8: 30631686},
-                    "fused": {4: 73863905, 6: 73722880, 8: 73819458}}
+MEASURED_TRAFFIC = {"outer": 82696380, "up": {4: 30604038, 6: 30726349, 8: 30631686},
+                    "fused": {4: 73863905, 6: 73841418, 8: 73819458}}
 
 
 def log(msg: str) -> None:
@@ -274,7 +276,8 @@ def coarse_net_flops(down: int, across: int, base: int = 32) -> int:
                for name, c1, c2, co, _ in layer_table(base))
 
 
-def learned_leg(acc, prob, tol: float, worlds: int, steps: int, reps: int, batch_reps: int):
+def learned_leg(acc, prob, tol: float, worlds: int, steps: int, reps: int, batch_reps: int, rows=None,
+                autotune: bool = True):
     """The learned coarse-grid correction (CG_METHOD_LEARNED): a U-Net trained on the GPU (the library's
     kernels) on the coarse grid's forcings of plain solves of `worlds` synthetic 1024x2048 worlds (seeds
     100, 101, ..., config[1]'s distributions) and their exact solutions, then set on the benchmark's
@@ -305,7 +308,10 @@ def learned_leg(acc, prob, tol: float, worlds: int, steps: int, reps: int, batch
     del data
     plan = Plan(acc, prob.levels)
     try:
-        plan.autotune(20)
+        if rows:
+            plan.set_tile_rows(rows + [0] * (len(prob.levels) - len(rows)))
+        elif autotune:
+            plan.autotune(20)
 
         def timed(method, n):
             info = plan.solve(tol, method)
@@ -503,8 +509,9 @@ def main() -> int:
         prob = circulation_problem(ROWS, COLS, seed=0, land_fraction=0.3)
         acc = HipOcean.create_or_null(0).accelerator
         try:
+            rows = [int(v) for v in args.rows.split(",")] if args.rows else None
             out = learned_leg(acc, prob, args.tol, args.learned_worlds, args.learned_steps, reps=args.steps,
-                              batch_reps=args.batch_reps)
+                              batch_reps=args.batch_reps, rows=rows, autotune=not args.no_autotune)
         finally:
             acc.close()
         print(json.dumps(out), flush=True)
@@ -560,6 +567,7 @@ def main() -> int:
     us_down = lib.cg_ocean_plan_time_kernel(plan.handle, 0, reps)
     us_fused = lib.cg_ocean_plan_time_kernel(plan.handle, 3, reps)
     us_bottom = lib.cg_ocean_plan_time_kernel(plan.handle, 4, reps)
+    us_fused_sums = lib.cg_ocean_plan_time_kernel(plan.handle, 5, reps)
     fused = us_fused > 0
     defaults = args.tile_rows == 8 and args.level_passes == "22222222"
     rows0 = tuned[0] if tuned and tuned[0] else 8  # the finest grid's tile rows
@@ -601,6 +609,10 @@ def main() -> int:
         cmd = [sys.executable, os.path.abspath(__file__), "--learned-only", "--tol", str(args.tol),
                "--steps", str(args.steps), "--batch-reps", str(args.batch_reps),
                "--learned-worlds", str(args.learned_worlds), "--learned-steps", str(args.learned_steps)]
+        if args.rows:
+            cmd += ["--rows", args.rows]
+        if args.no_autotune:
+            cmd += ["--no-autotune"]
         proc = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=600)
         if proc.returncode != 0:
             raise RuntimeError(f"the learned leg failed with status {proc.returncode}")
@@ -662,6 +674,7 @@ def main() -> int:
             "traffic": MEASURED_TRAFFIC["fused"].get(rows0) if defaults else None,
             "tile_rows": rows0,
             "avg_launch_us": round(us_fused, 3),
+            "avg_launch_us_with_error_sums": round(us_fused_sums, 3) if us_fused_sums > 0 else None,
             "algorithmic_bytes_per_launch": FUSED_BYTES,
             "separate_outer_residual": {"kernel": "k_outer_residual", "avg_launch_us": round(us_outer, 3),
                                         "achieved": None if outer_gbs is None else round(outer_gbs, 1),
@@ -689,7 +702,7 @@ def main() -> int:
                 "traffic": MEASURED_TRAFFIC["up"].get(rows0) if defaults else None,
                 "down_leg_avg_launch_us": round(us_down, 3),
             },
-            "bottom": {"kernel": "k_bottom (the grids of 32x64 and smaller in one workgroup, latency-bound)",
+            "bottom": {"kernel": "k_direct_bottom (the cycle's 32x64 grid by its binary64 dense inverse)",
                        "avg_launch_us": round(us_bottom, 3)},
         },
         "cpu_baseline": cpu,

@@ -2938,10 +2938,12 @@ int cg_ocean_solve(cg_ctx *ctx, const cg_stencil *levels, int nlevels, const flo
 }
 
 double cg_ocean_plan_time_kernel(cg_plan *p, int kind, int reps) {
-    if (!p || reps < 1 || kind < 0 || kind > 4) return -1.0;
+    if (!p || reps < 1 || kind < 0 || kind > 5) return -1.0;
     cg_ctx *ctx = p->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     if (hipSetDevice(ctx->device) != hipSuccess) return -1.0;
+    const bool with_sums = kind == 5;
+    if (with_sums) kind = 3;
     if (kind == 3 && !outer_fused(p)) {
         set_err(ctx, "the plan's steps do not fuse the outer step");
         return -1.0;
@@ -2991,8 +2993,10 @@ double cg_ocean_plan_time_kernel(cg_plan *p, int kind, int reps) {
             ex.red.clear = p->d_u64 + 2 * kMaxSlots;
             ex.s64 = L.st->cells;
             ex.red_stride = 2 * kMaxSlots;
-            ex.sums.out = sums_group(p, 2 * p->K);
-            ex.sums.clear = sums_group(p, 2 * p->K + 1);
+            if (with_sums) {  // the error estimate's sums, which only a solve's last steps take
+                ex.sums.out = sums_group(p, 2 * p->K);
+                ex.sums.clear = sums_group(p, 2 * p->K + 1);
+            }
             return tile_dispatch(ctx, L.st->args(), level0_forcing(p, q), L.x[L.cur], L.x[1 - L.cur], nh, false,
                                  TILE_UP_OUTER, ex, th);
         }

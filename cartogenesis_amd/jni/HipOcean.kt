@@ -93,6 +93,15 @@ class HipOcean private constructor(private val ctx: Long, override val name: Str
             OceanCirculation.Solution(stream, info[0].toInt(), info[1])
         }
 
+        /**
+         * The learned coarse-grid correction for this round's circulation: a trained U-Net's bf16
+         * weights ([weights], as cg_unet_weight_count(base) shorts, layer by layer) on the ladder's grid
+         * [level], with that grid's depth in metres; afterwards `circulation(..., method =
+         * CG_METHOD_LEARNED)` runs it inside each refinement step. Null weights remove it.
+         */
+        fun setCoarseNet(level: Int, base: Int, weights: ShortArray?, depth: FloatArray?, outScale: Double): Boolean =
+            synchronized(this@HipOcean) { nPlanSetCoarseNet(flow, level, base, weights, depth, outScale) == CG_OK }
+
         /** OceanCirculation.velocities (OceanCirculation.kt:801-828) of the last stream. */
         fun velocities(widthMeters: Double, heightMeters: Double, eastward: FloatArray, northward: FloatArray): Boolean =
             synchronized(this@HipOcean) { nPlanVelocities(flow, widthMeters, heightMeters, eastward, northward) == CG_OK }
@@ -158,6 +167,7 @@ class HipOcean private constructor(private val ctx: Long, override val name: Str
         const val CG_METHOD_RESCUE = 1
         const val CG_METHOD_REFINED = 2
         const val CG_METHOD_KRYLOV = 4
+        const val CG_METHOD_LEARNED = 5
         private const val MOST_RESIDENT_STENCILS = 24
 
         /** GpuOcean.createOrNull's contract (GpuOcean.kt:189-202): an accelerator, or why there is none. */
@@ -198,6 +208,9 @@ class HipOcean private constructor(private val ctx: Long, override val name: Str
         @JvmStatic private external fun nPlanRead(plan: Long, out: FloatArray): Int
         @JvmStatic private external fun nPlanVelocities(
             plan: Long, widthM: Double, heightM: Double, eastward: FloatArray, northward: FloatArray
+        ): Int
+        @JvmStatic private external fun nPlanSetCoarseNet(
+            plan: Long, level: Int, base: Int, weights: ShortArray?, depth: FloatArray?, outScale: Double
         ): Int
     }
 }

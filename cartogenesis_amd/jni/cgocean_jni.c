@@ -233,3 +233,18 @@ JNIEXPORT jint JNICALL Java_com_cartogenesis_desktop_HipOcean_nPlanVelocities(JN
     (*env)->ReleaseFloatArrayElements(env, eastward, e, rc == CG_OK ? 0 : JNI_ABORT);
     return rc;
 }
+
+JNIEXPORT jint JNICALL Java_com_cartogenesis_desktop_HipOcean_nPlanSetCoarseNet(JNIEnv *env, jclass k, jlong plan,
+                                                                                jint level, jint base,
+                                                                                jshortArray weights,
+                                                                                jfloatArray depth, jdouble out_scale) {
+    (void)k;
+    if (!weights) return cg_ocean_plan_set_coarse_net(PLAN(plan), level, base, NULL, 0, NULL, out_scale);
+    jsize n = (*env)->GetArrayLength(env, weights);
+    jshort *w = (*env)->GetShortArrayElements(env, weights, NULL);
+    jfloat *d = depth ? (*env)->GetFloatArrayElements(env, depth, NULL) : NULL;
+    int rc = cg_ocean_plan_set_coarse_net(PLAN(plan), level, base, (const uint16_t *)w, (size_t)n, d, out_scale);
+    if (d) (*env)->ReleaseFloatArrayElements(env, depth, d, JNI_ABORT);
+    (*env)->ReleaseShortArrayElements(env, weights, w, JNI_ABORT);
+    return rc;
+}
