@@ -609,9 +609,11 @@ def main() -> int:
         cmd = [sys.executable, os.path.abspath(__file__), "--learned-only", "--tol", str(args.tol),
                "--steps", str(args.steps), "--batch-reps", str(args.batch_reps),
                "--learned-worlds", str(args.learned_worlds), "--learned-steps", str(args.learned_steps)]
-        if args.rows:
-            cmd += ["--rows", args.rows]
-        if args.no_autotune:
+        # The main plan's tile heights (autotuned or given): tuning again in the child, after the
+        # network's training, has picked tiles several times slower.
+        if tuned:
+            cmd += ["--rows", ",".join(str(v) for v in tuned)]
+        elif args.no_autotune:
             cmd += ["--no-autotune"]
         proc = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=600)
         if proc.returncode != 0:

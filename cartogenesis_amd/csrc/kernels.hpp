@@ -242,6 +242,16 @@ __device__ __forceinline__ float rb_update(float4 wt, float xe, float xw, float 
     return wt.x * xe + wt.y * xw + wt.z * xn + wt.w * xs - f;
 }
 
+// The same update by three fused multiply-adds, for the refined method's binary32 correction
+// cycle: that cycle only has to reduce its residual by a factor (its rounding never reaches the
+// binary64 fixed point), so it need not round as the reference does, and it saves three of eight
+// operations per update. Every kernel of that cycle takes it, so its bits do not depend on tiling.
+template <bool FUSED>
+__device__ __forceinline__ float rb_step(float4 wt, float xe, float xw, float xn, float xs, float f) {
+    if constexpr (FUSED) return __builtin_fmaf(wt.w, xs, __builtin_fmaf(wt.z, xn, __builtin_fmaf(wt.y, xw, wt.x * xe))) - f;
+    else return rb_update(wt, xe, xw, xn, xs, f);
+}
+
 // ------------------------------------------------------------------------------------------------
 // Fused red-black smoother, row-constant weights.
 //
@@ -337,6 +347,7 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
     constexpr int HC = TileGeom<NH, MODE>::HC;
     constexpr int IW = TileGeom<NH, MODE>::IW;
     constexpr int R = TH + 2 * HR;
+    constexpr bool kFusedUpdate = MODE != TILE_PLAIN;  // the refined cycle's legs (rb_step)
     static_assert(R <= 64, "tile rows must fit a 64-bit row mask");
     static_assert(TH % 2 == 0, "tiles start on even rows");
     const int across = L.across, down = L.down;
@@ -552,10 +563,10 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
                     B[i] = keep_if(0.0f - fB[i], mB, i & 31);
             } else if (((i + HR) & 1) == colour) {
                 const float xw = lane_from_below(B[i]);
-                A[i] = keep_if(rb_update(wt, B[i], xw, A[i - 1], A[i + 1], fA[i]), mA, i & 31);
+                A[i] = keep_if(rb_step<kFusedUpdate>(wt, B[i], xw, A[i - 1], A[i + 1], fA[i]), mA, i & 31);
             } else {
                 const float xe = lane_from_above(A[i]);
-                B[i] = keep_if(rb_update(wt, xe, A[i], B[i - 1], B[i + 1], fB[i]), mB, i & 31);
+                B[i] = keep_if(rb_step<kFusedUpdate>(wt, xe, A[i], B[i - 1], B[i + 1], fB[i]), mB, i & 31);
             }
         }
         // Keep each step's weights local to the step: without this the scheduler hoists every
@@ -692,8 +703,8 @@ __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__
                 // is x - x (+0, or NaN where x is not finite) without recomputing it.
                 constexpr int kLast = (NH - 1) & 1;
                 const bool a_last = ((ii + HR) & 1) == kLast;
-                const float ua = a_last ? A[ii] - A[ii] : A[ii] - rb_update(wt, B[ii], xw, A[ii - 1], A[ii + 1], fA[ii]);
-                const float ub = a_last ? B[ii] - rb_update(wt, xe, A[ii], B[ii - 1], B[ii + 1], fB[ii]) : B[ii] - B[ii];
+                const float ua = a_last ? A[ii] - A[ii] : A[ii] - rb_step<kFusedUpdate>(wt, B[ii], xw, A[ii - 1], A[ii + 1], fA[ii]);
+                const float ub = a_last ? B[ii] - rb_step<kFusedUpdate>(wt, xe, A[ii], B[ii - 1], B[ii + 1], fB[ii]) : B[ii] - B[ii];
                 const uint32_t mA = ii < 32 ? (uint32_t)wA : (uint32_t)(wA >> 32);
                 const uint32_t mB = ii < 32 ? (uint32_t)wB : (uint32_t)(wB >> 32);
                 u[2 * k] = keep_if(ua, mA, ii & 31);
