@@ -445,6 +445,9 @@ def main() -> int:
     ap.add_argument("--conv-only", action="store_true",
                     help="only the U-Net's largest convolution (dec0a at 1024x2048, random weights), 5 launches: "
                          "the short program the convolution's counters are read from")
+    ap.add_argument("--unet-forward", action="store_true",
+                    help="only the U-Net's forward at 1024x2048 (random weights, batch 1), one warm-up and five "
+                         "timed: the short program the per-layer kernel trace is read from")
     ap.add_argument("--plain-value", type=float, default=0.0, help=argparse.SUPPRESS)
     ap.add_argument("--learned-worlds", type=int, default=64,
                     help="1024x2048 training worlds of the coarse network (seeds 100, 101, ...)")
@@ -497,6 +500,32 @@ def main() -> int:
         algo = 2 * ROWS * COLS * (64 + 32 + 32) + 2 * 9 * 32 * 96  # inputs and output once, the weights
         print(json.dumps({"kernel": "k_conv3x3_tile<2>", "avg_launch_us": round(us, 2),
                           "tflops": round(fl / (us * 1e-6) / 1e12, 1), "algorithmic_bytes": algo}), flush=True)
+        return 0
+
+    if args.unet_forward:
+        import torch
+
+        torch.cuda.init()
+        from cartogenesis_amd.learned import UNet
+
+        net = UNet(32, seed=0)
+        x = torch.zeros((1, ROWS, COLS, 16), device="cuda")
+        x[..., 0] = 1.0
+        x[..., 1] = torch.rand((1, ROWS, COLS), device="cuda")
+        x[..., 2] = torch.randn((1, ROWS, COLS), device="cuda")
+        x = x.to(torch.bfloat16)
+        st = torch.cuda.current_stream().cuda_stream
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        net.forward(x, st)
+        e0.record()
+        for _ in range(5):
+            net.forward(x, st)
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) / 5 * 1000.0
+        fl = sum(unet_flops(ROWS, COLS).values())
+        print(json.dumps({"unet_forward_us": round(us, 1), "flops": fl, "tflops": round(fl / (us * 1e-6) / 1e12, 1),
+                          "layer_flops": unet_flops(ROWS, COLS)}), flush=True)
         return 0
 
     if args.learned_only:
