@@ -2938,10 +2938,35 @@ int cg_ocean_solve(cg_ctx *ctx, const cg_stencil *levels, int nlevels, const flo
 }
 
 double cg_ocean_plan_time_kernel(cg_plan *p, int kind, int reps) {
-    if (!p || reps < 1 || kind < 0 || kind > 5) return -1.0;
+    if (!p || reps < 1 || kind < 0 || (kind > 5 && kind < 16) || kind >= 16 + 2 * kMaxLevels) return -1.0;
     cg_ctx *ctx = p->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     if (hipSetDevice(ctx->device) != hipSuccess) return -1.0;
+    if (kind >= 16) {
+        // Grid l's down leg (even kinds) or up leg (odd) as the refined cycle launches it.
+        const int l = (kind - 16) / 2;
+        const bool up = (kind - 16) % 2 == 1;
+        if (!(ctx->fused && p->fusable) || l >= p->bottom) {
+            set_err(ctx, "the plan has no fused leg on grid %d", l);
+            return -1.0;
+        }
+        auto leg = [&]() -> hipError_t {
+            return up ? up_leg(p, l, nullptr, nullptr, false, plan_tile_rows(p, l))
+                      : down_leg(p, l, nullptr, plan_tile_rows(p, l));
+        };
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return -1.0;
+        if (leg() != hipSuccess) return -1.0;  // warm
+        (void)hipEventRecord(a, ctx->stream);
+        for (int i = 0; i < reps; ++i)
+            if (leg() != hipSuccess) return -1.0;
+        (void)hipEventRecord(b, ctx->stream);
+        float ms = 0.0f;
+        const bool ok = hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess;
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        return ok ? 1000.0 * ms / reps : -1.0;
+    }
     const bool with_sums = kind == 5;
     if (with_sums) kind = 3;
     if (kind == 3 && !outer_fused(p)) {

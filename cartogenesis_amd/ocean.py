@@ -315,6 +315,9 @@ class HipOcean(OceanAccelerator):
     def set_option(self, option: int, value: int) -> None:
         check(lib().cg_ocean_set_option(self._ctx, option, int(value)), self._ctx, "cg_ocean_set_option")
 
+    def get_option(self, option: int) -> int:
+        return int(lib().cg_ocean_get_option(self._ctx, option))
+
     def holding(self, stencil: OceanStencil) -> int:
         """The device handle of a stencil, uploading it the first time it is seen."""
         key = id(stencil.east_weight)
