@@ -163,7 +163,7 @@ def uneven_batches(acc, specs, instances: int, tol: float):
     latency-bound launches overlap on the GPU. Returns (sequential, concurrent) solves/s, per-plan
     milliseconds of the sequential run, and the largest residual."""
     import threading
-    from cartogenesis_amd.ocean import BatchPlan, HipOcean
+    from cartogenesis_amd.ocean import BatchPlan, HipOcean, PlanGroup
     from cartogenesis_amd.problem import circulation_problem, monthly_forcings
 
     worlds = []
@@ -182,6 +182,7 @@ def uneven_batches(acc, specs, instances: int, tol: float):
 
     total = instances * len(worlds)
     plans = [make(acc, prob, forcings) for _, prob, forcings in worlds]
+    group = None
     try:
         t0 = time.perf_counter()
         per, cycles, worst = {}, {}, 0.0
@@ -192,7 +193,18 @@ def uneven_batches(acc, specs, instances: int, tol: float):
             worst = worst_of(worst, infos)
             cycles[f"{k}:{rows}x{2 * rows}"] = [min(i.cycles for i in infos), max(i.cycles for i in infos)]
         sequential = total / (time.perf_counter() - t0)
+        # The same plans as one group: one host loop, each plan's step graph on a stream of its own.
+        group = PlanGroup(acc, plans)
+        group.solve(tol)  # warm: the composed graphs
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for infos in group.solve(tol):
+                worst = worst_of(worst, infos)
+        grouped = reps * total / (time.perf_counter() - t0)
     finally:
+        if group:
+            group.close()
         for plan in plans:
             plan.close()
     accs = [HipOcean.create_or_null(0).accelerator for _ in worlds]
@@ -217,7 +229,7 @@ def uneven_batches(acc, specs, instances: int, tol: float):
             plan.close()
         for a in accs:
             a.close()
-    return sequential, concurrent, per, cycles, worst
+    return sequential, concurrent, grouped, per, cycles, worst
 
 
 def worst_of(worst: float, infos) -> float:
@@ -234,10 +246,13 @@ def batched_worlds(acc, tol: float):
     """Four worlds of uneven size, twelve months each (one batched plan of twelve per world)."""
     log("bench: four-world year")
     specs = [(512, 1, 0.20), (768, 2, 0.27), (1024, 3, 0.33), (1024, 4, 0.40)]
-    seq, conc, per, cycles, worst = uneven_batches(acc, specs, 12, tol)
-    log(f"bench: four-world year {seq:.1f} solves/s one world after another, {conc:.1f} with the worlds concurrent")
+    seq, conc, grouped, per, cycles, worst = uneven_batches(acc, specs, 12, tol)
+    log(f"bench: four-world year {seq:.1f} solves/s one world after another, {conc:.1f} with the worlds concurrent, "
+        f"{grouped:.1f} as one group")
     return {"workload": "12 months x worlds 512x1024, 768x1536, 1024x2048, 1024x2048 (seeds 1-4, land 20/27/33/40%)",
-            "solves_per_s": round(conc, 3), "solves_per_s_one_world_at_a_time": round(seq, 3),
+            "solves_per_s": round(grouped, 3), "method": "one group (cg_ocean_group_solve): one host loop, each "
+            "world's refinement step one hipGraph launch for its 12 months, the worlds on streams of their own",
+            "solves_per_s_concurrent_streams": round(conc, 3), "solves_per_s_one_world_at_a_time": round(seq, 3),
             "ms_per_world_one_at_a_time": per, "cycles_min_max": cycles, "max_relative_residual": worst}
 
 
@@ -246,10 +261,13 @@ def batched_combined(acc, tol: float):
     0..11 then 0..3), one batched plan of 16 per size."""
     log("bench: combined 64 instances")
     specs = [(256, 5, 0.3), (512, 6, 0.3), (1024, 7, 0.3), (2048, 8, 0.3)]
-    seq, conc, per, cycles, worst = uneven_batches(acc, specs, 16, tol)
-    log(f"bench: combined 64 instances {seq:.1f} solves/s one size after another, {conc:.1f} concurrent")
+    seq, conc, grouped, per, cycles, worst = uneven_batches(acc, specs, 16, tol)
+    log(f"bench: combined 64 instances {seq:.1f} solves/s one size after another, {conc:.1f} concurrent, "
+        f"{grouped:.1f} as one group")
     return {"workload": "64 instances: 16 monthly forcings each at 256x512, 512x1024, 1024x2048, 2048x4096",
-            "solves_per_s": round(conc, 3), "solves_per_s_one_size_at_a_time": round(seq, 3),
+            "solves_per_s": round(grouped, 3), "method": "one group (cg_ocean_group_solve): one host loop, each "
+            "size's refinement step one hipGraph launch for its 16 instances, the sizes on streams of their own",
+            "solves_per_s_concurrent_streams": round(conc, 3), "solves_per_s_one_size_at_a_time": round(seq, 3),
             "ms_per_size_one_at_a_time": per, "cycles_min_max": cycles, "max_relative_residual": worst}
 
 

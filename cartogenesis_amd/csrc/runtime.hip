@@ -29,6 +29,7 @@ constexpr double kCoarsestShare = 0.1;    // OC:110
 constexpr int kPrecondCoarsestPasses = 200;  // OC:629
 constexpr int kKrylovRestarts = 3;        // OC:535
 constexpr int kMaxInstances = 64;          // right-hand sides one plan solves together
+constexpr int kMaxGroupPlans = 16;         // plans one group solves together
 constexpr int kDirectMaxWater = 2048;     // largest coarsest grid solved by a dense inverse
 constexpr int kRefinedMostSteps = 1000;
 // The learned correction's fallback: a residual falling by less than this factor a step.
@@ -382,6 +383,8 @@ struct cg_plan {
     // Graphs of one refinement step, for each parity of the binary64 ping-pong, and the context
     // settings they were captured under (a change of those rebuilds them).
     hipGraphExec_t step_exec[2] = {nullptr, nullptr};
+    hipGraph_t step_graph[2] = {nullptr, nullptr};  // their graphs, which a group of plans composes
+    int group_refs = 0;                              // groups (cg_ocean_group_create) holding the plan
     long graph_key = -1;
     double graph_etol = -1.0;
     // The learned coarse-grid correction (cg_ocean_plan_set_coarse_net): the network, whether the
@@ -432,6 +435,18 @@ struct cg_plan {
     int *d_body_seg0 = nullptr, *d_body_nseg = nullptr;
     double *d_margin = nullptr, *d_seg_sum = nullptr;
     float *d_shift = nullptr;
+};
+
+// Several plans of one context solved together (cg_ocean_group_*): worlds of uneven size, each a
+// batched plan of its months, stepped by one host loop. Each plan's step graph is launched on a
+// stream of the group's own (one per plan, default priority), so the plans' steps run concurrently
+// on the device; events join those streams to the context's before each read of the gates.
+struct cg_group {
+    cg_ctx *ctx = nullptr;
+    std::vector<cg_plan *> plans;
+    std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> done;  // per plan: its stream's last launch
+    hipEvent_t start = nullptr;    // the context's stream after the solves' first outer steps
 };
 
 namespace {
@@ -1483,7 +1498,9 @@ void drop_graphs(cg_plan *p) {
     for (int i = 0; i < 2; ++i) {
         if (p->step_exec[i]) (void)hipGraphExecDestroy(p->step_exec[i]);
         if (p->net_exec[i]) (void)hipGraphExecDestroy(p->net_exec[i]);
+        if (p->step_graph[i]) (void)hipGraphDestroy(p->step_graph[i]);
         p->step_exec[i] = p->net_exec[i] = nullptr;
+        p->step_graph[i] = nullptr;
     }
 }
 
@@ -1499,11 +1516,16 @@ int build_step_graphs(cg_plan *p) {
     double &getol = p->net_on ? p->net_graph_etol : p->graph_etol;
     if (gkey != key || getol != ctx->error_tol) {
         getol = ctx->error_tol;
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < 2; ++i) {
             if (exec[i]) {
                 (void)hipGraphExecDestroy(exec[i]);
                 exec[i] = nullptr;
             }
+            if (!p->net_on && p->step_graph[i]) {
+                (void)hipGraphDestroy(p->step_graph[i]);
+                p->step_graph[i] = nullptr;
+            }
+        }
         gkey = key;
     }
     for (int par = 0; par < 2; ++par) {
@@ -1533,7 +1555,13 @@ int build_step_graphs(cg_plan *p) {
             return CG_ERR_DEVICE;
         }
         e = hipGraphInstantiate(&exec[par], g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
+        // The plain step's graph is kept: a group of plans composes its steps from them.
+        if (!p->net_on && e == hipSuccess) {
+            if (p->step_graph[par]) (void)hipGraphDestroy(p->step_graph[par]);
+            p->step_graph[par] = g;
+        } else {
+            (void)hipGraphDestroy(g);
+        }
         if (e != hipSuccess) {
             set_err(ctx, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
             exec[par] = nullptr;
@@ -1570,32 +1598,69 @@ int gate_reset(cg_plan *p, int first_par, const std::vector<double> &thr, const 
     return CG_OK;
 }
 
-// The refined method over every instance of the plan; infos[i] receives instance i's result.
-// Every step starts with the gate (gate_decide) that stops an instance at the first step where its
-// residual is under tol and its estimated error under the context's error bound, so each instance
-// runs exactly the steps it would alone, however the host groups the steps into batches between
-// its reads; a stopped instance's kernels return at once and its answer stays where it stopped.
-int refined_solve_core(cg_plan *p, double tol_asked, bool warm, cg_solution *infos, const volatile int *cancel,
-                       bool learned, const std::vector<char> *only) {
+// One refined solve's host side on one plan: thresholds, the gate's view of each instance, the
+// batch prediction, the results. refined_solve_core runs one; a group of plans runs several, one
+// step of every plan per launch.
+struct RefineRun {
+    cg_plan *p = nullptr;
+    int K = 0;
+    double tol = 0.0, etol = 0.0;
+    bool cold = true, learned = false;
+    int first_par = 0;
+    const std::vector<char> *only = nullptr;
+    std::vector<double> largest, thr, res_prev, rate;
+    std::vector<char> done;
+    int launched = 0;    // steps launched on this plan
+    int last_batch = 0;  // steps of its last batch
+
+    // What the host knows of instance i after `launched` steps: the state the last gate wrote (or
+    // the initial one), and the gate's view of the last step the instance ran.
+    StepGate state_of(int i, int launched) const {
+        const int par = launched == 0 ? 1 - first_par : (first_par + launched - 1) & 1;
+        return p->h_gate[(size_t)par * K + i];
+    }
+    GateView view_of(int i, const StepGate &sg) const {
+        const int g = (first_par + sg.steps + 1) & 1;  // the group its last step (or the first outer) reduced into
+        return gate_view(p->h_res + (size_t)(2 * i + g) * kMaxSlots, p->h_sums + (size_t)(2 * i + g) * 2 * kMaxSlots,
+                         sg, thr[i], etol);
+    }
+    // Enqueues the copy of the gate states of both parities, the residual groups and the sums of
+    // every instance (the caller synchronizes the stream).
+    int read_async() const {
+        HIPCHK(p->ctx, hipMemcpyAsync(p->h_check, p->d_check, p->check_bytes, hipMemcpyDeviceToHost, p->ctx->stream));
+        return CG_OK;
+    }
+};
+
+// Starts a refined solve on plan p: thresholds, the gate, the first outer step (from zero, or from
+// the plan's answer when warm); the plan's steps gated (and learned) from here on, its step graphs
+// built when `graph`.
+int run_begin(RefineRun &r, cg_plan *p, double tol_asked, bool warm, bool learned, const std::vector<char> *only,
+              bool graph) {
     cg_ctx *ctx = p->ctx;
+    r.p = p;
+    r.K = p->K;
+    r.tol = tol_asked * (ctx->stop_permille / 1000.0);
+    r.etol = ctx->error_tol;
+    r.learned = learned;
+    r.only = only;
     const int K = p->K;
     const Stencil *st = p->lv[0].st;
-    const double tol = tol_asked * (ctx->stop_permille / 1000.0);
-    const double etol = ctx->error_tol;
-    std::vector<double> largest(K), thr(K);
+    r.largest.assign(K, 0.0);
+    r.thr.assign(K, 0.0);
     int rc;
     for (int i = 0; i < K; ++i) {
-        if ((rc = plan_largest_at(p, i, &largest[i]))) return rc;
-        thr[i] = largest[i] > 0.0 ? tol * largest[i] : INFINITY;
+        if ((rc = plan_largest_at(p, i, &r.largest[i]))) return rc;
+        r.thr[i] = r.largest[i] > 0.0 ? r.tol * r.largest[i] : INFINITY;
     }
-    const bool cold = !warm || !p->x64_valid;
+    r.cold = !warm || !p->x64_valid;
     p->x64_valid = true;
     // The first outer step reads parity q0 (zero, from cold) and writes 1 - q0, where the steps start.
     const int q0 = p->x64cur;
-    const int first_par = 1 - q0;
-    if ((rc = gate_reset(p, first_par, thr, only))) return rc;
-    float *const first_fc = outer_fused(p) ? level0_forcing(p, first_par) : p->lv[0].f;
-    if (cold) {
+    r.first_par = 1 - q0;
+    if ((rc = gate_reset(p, r.first_par, r.thr, only))) return rc;
+    float *const first_fc = outer_fused(p) ? level0_forcing(p, r.first_par) : p->lv[0].f;
+    if (r.cold) {
         // From zero: x = 0, u = 0 - (0 - f) on water; its maxima in parity q0's groups.
         MaxReduce red;
         red.out = p->d_res + (size_t)q0 * kMaxSlots;
@@ -1605,16 +1670,170 @@ int refined_solve_core(cg_plan *p, double tol_asked, bool warm, cg_solution *inf
         ob.s64 = st->cells;
         ob.red_stride = 2 * kMaxSlots;
         const dim3 grid(blocks_for(st->across / 2, 256), blocks_for(st->down, kRowsPerBlock), K);
-        hipLaunchKernelGGL(k_first_outer, grid, dim3(256), 0, ctx->stream, st->args(), p->f0, p->x64[first_par],
+        hipLaunchKernelGGL(k_first_outer, grid, dim3(256), 0, ctx->stream, st->args(), p->f0, p->x64[r.first_par],
                            first_fc, red, ob);
         HIPCHK(ctx, hipGetLastError());
-        p->x64cur = first_par;
+        p->x64cur = r.first_par;
     } else if ((rc = outer_step(p, nullptr, first_fc))) {
         return rc;
     }
-    const bool graph = ctx->use_graph && !ctx->time_kernels;
     p->gated = true;
     p->net_on = learned;
+    if (learned) p->net_fallbacks = 0;
+    if (graph && (rc = build_step_graphs(p))) return rc;
+    // From zero the relative residual before the first step is 1 (the zero answer's), which dates the
+    // first batch's rate.
+    r.res_prev.assign(K, r.cold ? 1.0 : -1.0);
+    r.rate.assign(K, -1.0);
+    r.done.assign(K, 0);
+    if (only)
+        for (int i = 0; i < K; ++i) r.done[i] = !(*only)[i];
+    return CG_OK;
+}
+
+// The steps run `r` wants next: one fewer than the most steps any of its instances still running
+// is predicted to need by its residual's rate and its correction's contraction (the device stops
+// each one at its own step, so a batch sized for the slowest costs the others only skipped
+// launches), at most 16; 8 before any rate is known. The last read of each instance replays its
+// next gate, so a solve ends without launching the step that would only stop it.
+int run_want(RefineRun &r, int launched, int last_batch) {
+    cg_plan *p = r.p;
+    int batch = 0;
+    if (launched == 0 && r.cold) {
+        // From zero the residual is max |f c| itself: every instance with a forcing needs steps,
+        // about as many as its last solve from zero took (one fewer, so the batch ends before the
+        // stop), else 8.
+        for (int i = 0; i < r.K; ++i) {
+            if (r.done[i]) continue;
+            if (r.largest[i] > 0.0) batch = std::max(batch, p->last_steps[i] > 1 ? p->last_steps[i] - 1 : 8);
+            else r.done[i] = 1;
+        }
+        return std::min(batch, 32);
+    }
+    for (int i = 0; i < r.K; ++i) {
+        if (r.done[i]) continue;
+        const StepGate sg = r.state_of(i, launched);
+        const GateView v = r.view_of(i, sg);
+        if (v.stop) {
+            r.done[i] = 1;
+            continue;
+        }
+        const double rr = r.largest[i] > 0.0 ? v.res / r.largest[i] : 0.0;
+        if (last_batch > 0 && r.res_prev[i] > 0.0 && rr > 0.0) r.rate[i] = std::pow(rr / r.res_prev[i], 1.0 / last_batch);
+        r.res_prev[i] = rr;
+        int need_res = 1, need_err = 0;
+        if (v.res < r.thr[i]) need_res = 0;
+        else if (r.rate[i] > 0.0 && r.rate[i] < 0.95)
+            need_res = (int)std::ceil(std::log(r.tol / rr) / std::log(r.rate[i]));
+        if (r.etol > 0.0) {
+            if (v.est < 0.0) need_err = need_res + 1;
+            else if (v.est > r.etol && v.rho < 0.95)
+                need_err = (int)std::ceil(std::log(r.etol / v.est) / std::log(v.rho));
+            else if (v.est > r.etol) need_err = 1;
+        }
+        // One step short of the prediction: the early steps' rate is slower than the last ones',
+        // so predictions run long, and a step past the stop costs more than a read.
+        const int want = std::max(1, std::max(need_res, need_err) - 1);
+        batch = std::max(batch, want);
+    }
+    return std::min(batch, 16);
+}
+
+// After a read: marks the instances that have stopped; returns how many still run.
+int run_left(RefineRun &r, int launched) {
+    int left = 0;
+    for (int i = 0; i < r.K; ++i) {
+        if (r.done[i]) continue;
+        const StepGate sg = r.state_of(i, launched);
+        if (!sg.active || r.view_of(i, sg).stop) r.done[i] = 1;
+        else ++left;
+    }
+    return left;
+}
+
+void run_finish(RefineRun &r, int launched, int status, cg_solution *infos) {
+    cg_plan *p = r.p;
+    for (int i = 0; i < r.K; ++i) {
+        if (r.only && !(*r.only)[i]) continue;
+        const StepGate sg = r.state_of(i, launched);
+        const GateView v = r.view_of(i, sg);
+        infos[i] = cg_solution{};
+        infos[i].cycles = sg.steps;
+        infos[i].refinements = sg.steps + 1;
+        infos[i].relative_residual = r.largest[i] > 0.0 ? v.res / r.largest[i] : 0.0;
+        infos[i].status = status;
+        infos[i].refinements = sg.fell;  // the caller (refined_solve) reads and resets it
+        p->ans_par[i] = (r.first_par + sg.steps) & 1;
+        if (r.cold && status == CG_OK && !r.learned) p->last_steps[i] = sg.steps;
+    }
+    if (r.K == 1) p->x64cur = p->ans_par[0];
+    p->last_answer = 1;
+}
+
+// Steps the runs (plans of one context) together until every instance of every run has stopped:
+// each batch is as long as the longest any run wants, and `step(on)` enqueues one refinement step
+// of every run k with on[k] (the runs with an instance still running). `join` orders the
+// context's stream after every launch before the gates are read. Returns the status.
+template <typename Step, typename Join>
+int run_steps(std::vector<RefineRun> &runs, const volatile int *cancel, Step step, Join join, int *status_out) {
+    cg_ctx *ctx = runs[0].p->ctx;
+    auto read_back = [&]() -> int {
+        int rc = join();
+        if (rc) return rc;
+        for (const RefineRun &r : runs) {
+            rc = r.read_async();
+            if (rc) return rc;
+        }
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        return CG_OK;
+    };
+    int status = CG_OK, rc;
+    // A warm start reads its first outer step's residual before any step: it may already be under tol.
+    bool warm = false;
+    for (const RefineRun &r : runs) warm = warm || !r.cold;
+    if (warm && (rc = read_back())) return rc;
+    std::vector<char> on(runs.size(), 0);
+    int total = 0;
+    while (total < kRefinedMostSteps) {
+        int batch = 0;
+        for (size_t k = 0; k < runs.size(); ++k) {
+            const int want = run_want(runs[k], runs[k].launched, runs[k].last_batch);
+            on[k] = want > 0;
+            batch = std::max(batch, want);
+        }
+        if (batch <= 0) break;  // every instance's next gate would stop it
+        if (cancel && *cancel) {
+            status = CG_CANCELLED;
+            break;
+        }
+        batch = std::min(batch, kRefinedMostSteps - total);
+        for (int k = 0; k < batch; ++k)
+            if ((rc = step(on))) return rc;
+        total += batch;
+        int left = 0;
+        for (size_t k = 0; k < runs.size(); ++k) {
+            runs[k].last_batch = on[k] ? batch : 0;
+            if (on[k]) runs[k].launched += batch;
+        }
+        if ((rc = read_back())) return rc;
+        for (size_t k = 0; k < runs.size(); ++k)
+            if (on[k]) left += run_left(runs[k], runs[k].launched);
+        if (left == 0) break;
+    }
+    if (total == 0 && (rc = read_back())) return rc;
+    *status_out = status;
+    return CG_OK;
+}
+
+// The refined method over every instance of the plan; infos[i] receives instance i's result.
+// Every step starts with the gate (gate_decide) that stops an instance at the first step where its
+// residual is under tol and its estimated error under the context's error bound, so each instance
+// runs exactly the steps it would alone, however the host groups the steps into batches between
+// its reads; a stopped instance's kernels return at once and its answer stays where it stopped.
+int refined_solve_core(cg_plan *p, double tol_asked, bool warm, cg_solution *infos, const volatile int *cancel,
+                       bool learned, const std::vector<char> *only) {
+    cg_ctx *ctx = p->ctx;
+    const bool graph = ctx->use_graph && !ctx->time_kernels;
     struct Ungate {
         cg_plan *p;
         ~Ungate() {
@@ -1622,125 +1841,21 @@ int refined_solve_core(cg_plan *p, double tol_asked, bool warm, cg_solution *inf
             p->net_on = false;
         }
     } ungate{p};
-    if (learned) p->net_fallbacks = 0;
-    if (graph && (rc = build_step_graphs(p))) return rc;
-    const unsigned long long *const res_groups = p->h_res;
-    // Reads the gate states of both parities, the residual groups and the sums of every instance.
-    auto read_back = [&]() -> int {
-        HIPCHK(ctx, hipMemcpyAsync(p->h_check, p->d_check, p->check_bytes, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-        return CG_OK;
+    std::vector<RefineRun> runs(1);
+    int rc = run_begin(runs[0], p, tol_asked, warm, learned, only, graph);
+    if (rc) return rc;
+    auto step = [&](const std::vector<char> &) -> int {
+        if (graph) {
+            hipGraphExec_t *const exec = p->net_on ? p->net_exec : p->step_exec;
+            HIPCHK(ctx, hipGraphLaunch(exec[p->x64cur], ctx->stream));
+            p->x64cur = 1 - p->x64cur;
+            return CG_OK;
+        }
+        return refine_step(p, true);
     };
-    // What the host knows of instance i after `launched` steps: the state the last gate wrote (or
-    // the initial one), and the gate's view of the last step the instance ran.
-    auto state_of = [&](int i, int launched) -> StepGate {
-        const int par = launched == 0 ? 1 - first_par : (first_par + launched - 1) & 1;
-        return p->h_gate[(size_t)par * K + i];
-    };
-    auto view_of = [&](int i, const StepGate &sg) -> GateView {
-        const int g = (first_par + sg.steps + 1) & 1;  // the group its last step (or the first outer) reduced into
-        return gate_view(res_groups + (size_t)(2 * i + g) * kMaxSlots, p->h_sums + (size_t)(2 * i + g) * 2 * kMaxSlots,
-                         sg, thr[i], etol);
-    };
-    // Steps in batches between host reads: one fewer than the most steps any instance still running
-    // is predicted to need by its residual's rate and its correction's contraction (the device stops
-    // each one at its own step, so a batch sized for the slowest costs the others only skipped
-    // launches), at most 16; 8 before any rate is known. The last read of each instance replays its next gate, so a solve
-    // ends without launching the step that would only stop it.
-    // From zero the relative residual before the first step is 1 (the zero answer's), which dates the
-    // first batch's rate.
-    std::vector<double> res_prev(K, cold ? 1.0 : -1.0), rate(K, -1.0);
-    std::vector<char> done(K, 0);
-    if (only)
-        for (int i = 0; i < K; ++i) done[i] = !(*only)[i];
-    int launched = 0, status = CG_OK, last_batch = 0;
-    // A warm start reads its first outer step's residual before any step: it may already be under tol.
-    if (!cold && (rc = read_back())) return rc;
-    while (launched < kRefinedMostSteps) {
-        int batch = 0;
-        if (launched == 0 && cold) {
-            // From zero the residual is max |f c| itself: every instance with a forcing needs steps,
-            // about as many as its last solve from zero took (one fewer, so the batch ends before
-            // the stop), else 8.
-            for (int i = 0; i < K; ++i) {
-                if (done[i]) continue;
-                if (largest[i] > 0.0) batch = std::max(batch, p->last_steps[i] > 1 ? p->last_steps[i] - 1 : 8);
-                else done[i] = 1;
-            }
-            batch = std::min(batch, 32);
-        } else {
-            for (int i = 0; i < K; ++i) {
-                if (done[i]) continue;
-                const StepGate sg = state_of(i, launched);
-                const GateView v = view_of(i, sg);
-                if (v.stop) {
-                    done[i] = 1;
-                    continue;
-                }
-                const double r = largest[i] > 0.0 ? v.res / largest[i] : 0.0;
-                if (last_batch > 0 && res_prev[i] > 0.0 && r > 0.0)
-                    rate[i] = std::pow(r / res_prev[i], 1.0 / last_batch);
-                res_prev[i] = r;
-                int need_res = 1, need_err = 0;
-                if (v.res < thr[i]) need_res = 0;
-                else if (rate[i] > 0.0 && rate[i] < 0.95)
-                    need_res = (int)std::ceil(std::log(tol / r) / std::log(rate[i]));
-                if (etol > 0.0) {
-                    if (v.est < 0.0) need_err = need_res + 1;
-                    else if (v.est > etol && v.rho < 0.95)
-                        need_err = (int)std::ceil(std::log(etol / v.est) / std::log(v.rho));
-                    else if (v.est > etol) need_err = 1;
-                }
-                // One step short of the prediction: the early steps' rate is slower than the last
-                // ones', so predictions run long, and a step past the stop costs more than a read.
-                const int want = std::max(1, std::max(need_res, need_err) - 1);
-                batch = std::max(batch, want);
-            }
-            batch = std::min(batch, 16);
-        }
-        if (batch <= 0) break;  // every instance's next gate would stop it
-        if (cancel && *cancel) {
-            status = CG_CANCELLED;
-            break;
-        }
-        batch = std::min(batch, kRefinedMostSteps - launched);
-        hipGraphExec_t *const exec = p->net_on ? p->net_exec : p->step_exec;
-        for (int k = 0; k < batch; ++k) {
-            if (graph) {
-                HIPCHK(ctx, hipGraphLaunch(exec[p->x64cur], ctx->stream));
-                p->x64cur = 1 - p->x64cur;
-            } else if ((rc = refine_step(p, true))) {
-                return rc;
-            }
-        }
-        launched += batch;
-        last_batch = batch;
-        if ((rc = read_back())) return rc;
-        int left = 0;
-        for (int i = 0; i < K; ++i) {
-            if (done[i]) continue;
-            const StepGate sg = state_of(i, launched);
-            if (!sg.active || view_of(i, sg).stop) done[i] = 1;
-            else ++left;
-        }
-        if (left == 0) break;
-    }
-    if (launched == 0 && (rc = read_back())) return rc;
-    for (int i = 0; i < K; ++i) {
-        if (only && !(*only)[i]) continue;
-        const StepGate sg = state_of(i, launched);
-        const GateView v = view_of(i, sg);
-        infos[i] = cg_solution{};
-        infos[i].cycles = sg.steps;
-        infos[i].refinements = sg.steps + 1;
-        infos[i].relative_residual = largest[i] > 0.0 ? v.res / largest[i] : 0.0;
-        infos[i].status = status;
-        infos[i].refinements = sg.fell;  // the caller (refined_solve) reads and resets it
-        p->ans_par[i] = (first_par + sg.steps) & 1;
-        if (cold && status == CG_OK && !learned) p->last_steps[i] = sg.steps;
-    }
-    if (K == 1) p->x64cur = p->ans_par[0];
-    p->last_answer = 1;
+    int status = CG_OK;
+    if ((rc = run_steps(runs, cancel, step, []() { return CG_OK; }, &status))) return rc;
+    run_finish(runs[0], runs[0].launched, status, infos);
     return CG_OK;
 }
 
@@ -2404,6 +2519,10 @@ int cg_ocean_plan_destroy(cg_plan *p) {
     if (!p) return CG_ERR_ARG;
     cg_ctx *ctx = p->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (p->group_refs > 0) {
+        set_err(ctx, "the plan belongs to %d group(s): destroy them first", p->group_refs);
+        return CG_ERR_STATE;
+    }
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     free_plan(p);
@@ -2635,6 +2754,134 @@ int cg_ocean_plan_solve_batch(cg_plan *const *plans, int count, double tol, int 
     for (int i = 0; i < count; ++i) {
         int rc = cg_ocean_plan_solve(plans[i], tol, method, 0, infos ? &infos[i] : nullptr);
         if (rc) return rc;
+    }
+    return CG_OK;
+}
+
+int cg_ocean_group_create(cg_plan *const *plans, int count, cg_group **out) {
+    if (!out) return CG_ERR_ARG;
+    *out = nullptr;
+    if (!plans || count < 1 || count > kMaxGroupPlans) return CG_ERR_ARG;
+    cg_ctx *ctx = plans[0] ? plans[0]->ctx : nullptr;
+    if (!ctx) return CG_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    for (int i = 0; i < count; ++i) {
+        if (!plans[i] || plans[i]->ctx != ctx) {
+            set_err(ctx, "every plan of a group must belong to one context");
+            return CG_ERR_ARG;
+        }
+        for (int j = 0; j < i; ++j)
+            if (plans[j] == plans[i]) {
+                set_err(ctx, "a plan appears twice in the group");
+                return CG_ERR_ARG;
+            }
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    cg_group *gr = new cg_group();
+    gr->ctx = ctx;
+    gr->plans.assign(plans, plans + count);
+    bool ok = hipEventCreateWithFlags(&gr->start, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; ok && i < count; ++i) {
+        hipStream_t s = nullptr;
+        hipEvent_t e = nullptr;
+        ok = hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess;
+        if (ok) gr->streams.push_back(s);
+        ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+        if (ok) gr->done.push_back(e);
+    }
+    if (!ok) {
+        set_err(ctx, "the group's streams or events could not be created");
+        for (hipStream_t s : gr->streams) (void)hipStreamDestroy(s);
+        for (hipEvent_t e : gr->done) (void)hipEventDestroy(e);
+        if (gr->start) (void)hipEventDestroy(gr->start);
+        delete gr;
+        return CG_ERR_DEVICE;
+    }
+    for (cg_plan *p : gr->plans) p->group_refs++;
+    *out = gr;
+    return CG_OK;
+}
+
+int cg_ocean_group_destroy(cg_group *gr) {
+    if (!gr) return CG_ERR_ARG;
+    cg_ctx *ctx = gr->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (hipStream_t s : gr->streams) {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+    }
+    for (hipEvent_t e : gr->done) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(gr->start);
+    for (cg_plan *p : gr->plans) p->group_refs--;
+    delete gr;
+    return CG_OK;
+}
+
+int cg_ocean_group_solve(cg_group *gr, double tol, cg_solution *infos) {
+    if (!gr || !infos || !(tol > 0.0)) return CG_ERR_ARG;
+    cg_ctx *ctx = gr->ctx;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    for (cg_plan *p : gr->plans) {
+        if (!(ctx->fused && p->fusable)) {
+            set_err(ctx, "a group's plans need the fused cycle (row-constant weights, halving grids)");
+            return CG_ERR_STATE;
+        }
+    }
+    const bool graph = ctx->use_graph && !ctx->time_kernels;
+    struct Ungate {
+        std::vector<cg_plan *> &ps;
+        ~Ungate() {
+            for (cg_plan *p : ps) {
+                p->gated = false;
+                p->net_on = false;
+            }
+        }
+    } ungate{gr->plans};
+    // Every plan starts from zero.
+    std::vector<RefineRun> runs(gr->plans.size());
+    int rc;
+    for (size_t k = 0; k < gr->plans.size(); ++k) {
+        cg_plan *p = gr->plans[k];
+        p->x64cur = 0;
+        if ((rc = run_begin(runs[k], p, tol, false, false, nullptr, graph))) return rc;
+    }
+    // The plans' streams start after the first outer steps on the context's stream.
+    if (graph) {
+        HIPCHK(ctx, hipEventRecord(gr->start, ctx->stream));
+        for (hipStream_t s : gr->streams) HIPCHK(ctx, hipStreamWaitEvent(s, gr->start, 0));
+    }
+    auto step = [&](const std::vector<char> &on) -> int {
+        for (size_t k = 0; k < gr->plans.size(); ++k) {
+            if (!on[k]) continue;
+            cg_plan *p = gr->plans[k];
+            if (graph) {
+                HIPCHK(ctx, hipGraphLaunch(p->step_exec[p->x64cur], gr->streams[k]));
+                p->x64cur = 1 - p->x64cur;
+            } else {
+                int r = refine_step(p, true);
+                if (r) return r;
+            }
+        }
+        return CG_OK;
+    };
+    auto join = [&]() -> int {
+        if (!graph) return CG_OK;
+        for (size_t k = 0; k < gr->plans.size(); ++k) {
+            HIPCHK(ctx, hipEventRecord(gr->done[k], gr->streams[k]));
+            HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, gr->done[k], 0));
+        }
+        return CG_OK;
+    };
+    int status = CG_OK;
+    if ((rc = run_steps(runs, nullptr, step, join, &status))) return rc;
+    size_t at = 0;
+    for (RefineRun &r : runs) {
+        run_finish(r, r.launched, status, infos + at);
+        for (int i = 0; i < r.K; ++i) infos[at + i].refinements = infos[at + i].cycles + 1;
+        at += (size_t)r.K;
     }
     return CG_OK;
 }

@@ -547,3 +547,34 @@ class BatchPlan:
         if self._p:
             lib().cg_ocean_plan_destroy(self._p)
             self._p = None
+
+
+class PlanGroup:
+    """Plans of one accelerator solved together by the refined method from zero (cg_ocean_group_*):
+    the worlds of uneven size of one batch, each a Plan or a BatchPlan of its own ladder, stepped by
+    one host loop, each plan's step graph on a stream of the group's own. Each instance stops at the
+    step, and with the answer, it would reach alone. Close the group before its plans."""
+
+    def __init__(self, acc: HipOcean, plans):
+        self.acc = acc
+        self.plans = list(plans)
+        arr = (ctypes.c_void_p * len(self.plans))(*[p.handle for p in self.plans])
+        self._g = ctypes.c_void_p()
+        check(lib().cg_ocean_group_create(arr, len(self.plans), ctypes.byref(self._g)), acc.ctx,
+              "cg_ocean_group_create")
+
+    def solve(self, tol: float) -> List[List[CgSolution]]:
+        """One list of results per plan (one per instance)."""
+        counts = [getattr(p, "instances", 1) for p in self.plans]
+        infos = (CgSolution * sum(counts))()
+        check(lib().cg_ocean_group_solve(self._g, float(tol), infos), self.acc.ctx, "cg_ocean_group_solve")
+        out, at = [], 0
+        for c in counts:
+            out.append(list(infos[at:at + c]))
+            at += c
+        return out
+
+    def close(self) -> None:
+        if self._g:
+            lib().cg_ocean_group_destroy(self._g)
+            self._g = None

@@ -200,6 +200,20 @@ int cg_ocean_plan_solve(cg_plan *plan, double tol, int method, int warm, cg_solu
  * instance; right-hand sides on one ladder go faster as one batched plan, below). */
 int cg_ocean_plan_solve_batch(cg_plan *const *plans, int count, double tol, int method,
                               cg_solution *infos);
+/* A group of plans of one context solved together by CG_METHOD_REFINED, from zero: the worlds of
+ * uneven size of one batch, each a plan of its own ladder (batched: its months). No reference
+ * counterpart (the reference solves one grid per call, OceanStage.kt:436-443). One host loop batches
+ * the steps of every instance of every plan between reads of their step gates; each plan's
+ * refinement step is its captured hipGraph (one launch for all its instances), launched on a stream
+ * of the group's own so that the plans' steps run concurrently, and a plan whose instances have all
+ * stopped gets no more launches. Each instance stops at the step it would stop at alone, with the
+ * same answer to the bit. A plan may belong to several groups (at most 16 plans a group) and cannot
+ * be destroyed while it does. */
+typedef struct cg_group cg_group;
+int cg_ocean_group_create(cg_plan *const *plans, int count, cg_group **out);
+/* infos: one per instance, the plans' instances in the order of the plans given to create. */
+int cg_ocean_group_solve(cg_group *group, double tol, cg_solution *infos);
+int cg_ocean_group_destroy(cg_group *group);
 /* The finest grid's water bodies (OceanCirculation.WaterBodies, OC:670-697: body per cell, -1 on
  * land, `count` bodies) for correctBodyMeans (OC:709-731), which then follows every cycle of
  * CG_METHOD_REFERENCE/RESCUE and every preconditioning cycle of RESCUE/KRYLOV. count 0 removes them.
