@@ -399,10 +399,68 @@ def learned_leg(acc, prob, tol: float, worlds: int, steps: int, reps: int, batch
             log(f"bench: learned batched year {res['learned'][0]:.1f} solves/s against plain {res['plain'][0]:.1f}")
         finally:
             bp.close()
+        out["combined"] = learned_combined(acc, cn, tol, batch_reps)
     # The largest U-Net convolution at the full 1024x2048 grid (dec0a, 96 -> 32 channels), the matrix-core
     # kernel's rate where its size is not latency-bound.
     out["mfma_roofline"] = conv_roofline(cn.net)
     return out
+
+
+def learned_combined(acc, cn, tol: float, reps: int) -> dict:
+    """The 64 uneven instances (16 monthly forcings each at 256x512, 512x1024, 1024x2048 and 2048x4096)
+    as one group, every plan with the network on its 64x128 grid (the same globe grid in every
+    ladder), by the learned method and plainly; each size's first instance against the binary64
+    oracle."""
+    import cartogenesis_amd as cg
+    import oracle
+    from cartogenesis_amd.ocean import BatchPlan, PlanGroup
+    from cartogenesis_amd.problem import circulation_problem, monthly_forcings
+
+    log("bench: learned combined 64 instances")
+    specs = [(256, 5, 0.3), (512, 6, 0.3), (1024, 7, 0.3), (2048, 8, 0.3)]
+    probs, plans, group = [], [], None
+    try:
+        for rows, seed, land in specs:
+            prob = circulation_problem(rows, seed=seed, land_fraction=land)
+            forcings = monthly_forcings(prob)
+            plan = BatchPlan(acc, prob.levels, 16)
+            for i in range(16):
+                plan.set_forcing(i, forcings[i % 12])
+            plan.autotune(3)
+            cn.attach(plan, prob)
+            probs.append((prob, forcings[0]))
+            plans.append(plan)
+        group = PlanGroup(acc, plans)
+        res = {}
+        for name, m in (("plain", cg.CG_METHOD_REFINED), ("learned", cg.CG_METHOD_LEARNED)):
+            group.solve(tol, m)
+            t = time.perf_counter()
+            for _ in range(max(reps, 1)):
+                infos = group.solve(tol, m)
+            res[name] = (64 * max(reps, 1) / (time.perf_counter() - t), infos)
+        oracle.set_threads(int(os.environ.get("OMP_NUM_THREADS", "8")))
+        errs, steps = {}, {}
+        for (rows, _, _), (prob, f0), plan, infos in zip(specs, probs, plans, res["learned"][1]):
+            ladder = [oracle.Stencil(s.cells_across, s.cells_down, s.is_water, s.east_weight, s.west_weight,
+                                     s.north_weight, s.south_weight, s.forcing, s.centre_weight) for s in prob.levels]
+            ladder[0] = ladder[0].with_forcing(f0)
+            ref = oracle.solve64(ladder, 1e-12)[0]
+            errs[f"{rows}x{2 * rows}"] = float(np.linalg.norm(plan.read(0)[0] - ref) / np.linalg.norm(ref))
+            steps[f"{rows}x{2 * rows}"] = [min(i.cycles for i in infos), max(i.cycles for i in infos)]
+        fallbacks = sum(cg.lib().cg_ocean_plan_net_fallbacks(p.handle) for p in plans)
+        worst = max(max(i.relative_residual for i in infos) for infos in res["learned"][1])
+    finally:
+        if group:
+            group.close()
+        for plan in plans:
+            plan.close()
+    log(f"bench: learned combined {res['learned'][0]:.1f} solves/s against plain {res['plain'][0]:.1f}")
+    return {"workload": "64 instances: 16 monthly forcings each at 256x512, 512x1024, 1024x2048, 2048x4096, one "
+                        "group (cg_ocean_group_solve), the network on every ladder's 64x128 grid",
+            "learned_solves_per_s": round(res["learned"][0], 3), "plain_solves_per_s": round(res["plain"][0], 3),
+            "learned_vs_plain_mg": round(res["learned"][0] / res["plain"][0], 4), "learned_steps_min_max": steps,
+            "fallbacks": fallbacks, "max_relative_residual": worst,
+            "rel_l2_vs_fp64_oracle_month0": errs}
 
 
 def conv_roofline(net) -> dict:

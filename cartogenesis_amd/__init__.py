@@ -66,7 +66,7 @@ SIGNATURES = {
     "cg_ocean_plan_solve": (_int, [_vp, _dbl, _int, _int, ctypes.POINTER(CgSolution)]),
     "cg_ocean_plan_solve_batch": (_int, [ctypes.POINTER(_vp), _int, _dbl, _int, ctypes.POINTER(CgSolution)]),
     "cg_ocean_group_create": (_int, [ctypes.POINTER(_vp), _int, ctypes.POINTER(_vp)]),
-    "cg_ocean_group_solve": (_int, [_vp, _dbl, ctypes.POINTER(CgSolution)]),
+    "cg_ocean_group_solve": (_int, [_vp, _dbl, _int, ctypes.POINTER(CgSolution)]),
     "cg_ocean_group_destroy": (_int, [_vp]),
     "cg_ocean_plan_set_bodies": (_int, [_vp, _vp, _int]),
     "cg_ocean_plan_correct_bodies": (_int, [_vp, _f32p, _f32p]),

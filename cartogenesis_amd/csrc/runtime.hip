@@ -1859,20 +1859,14 @@ int refined_solve_core(cg_plan *p, double tol_asked, bool warm, cg_solution *inf
     return CG_OK;
 }
 
-// The refined method, or with `learned` its learned coarse-grid correction, over every instance.
-// The multigrid fallback of the learned correction: an instance the gate stopped for its stalling
-// residual goes on from its answer with the plain cycle (the steps of both count), and if a learned
-// answer is not a number the whole solve starts again, plainly.
-int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *infos, const volatile int *cancel,
-                  bool learned = false) {
+// The multigrid fallback of the learned correction, after a learned solve's steps (infos[i].refinements
+// holding whether instance i's gate stopped it for its stalling residual): such an instance goes on
+// from its answer with the plain cycle (the steps of both count), and if a learned answer is not a
+// number the whole solve starts again, plainly.
+int learned_fallback(cg_plan *p, double tol, cg_solution *infos, const volatile int *cancel) {
     cg_ctx *ctx = p->ctx;
     const int K = p->K;
-    if (learned) p->net_fallbacks = 0;
-    int rc = refined_solve_core(p, tol, warm, infos, cancel, learned, nullptr);
-    if (rc || !learned) {
-        for (int i = 0; i < K; ++i) infos[i].refinements = infos[i].cycles + 1;
-        return rc;
-    }
+    int rc;
     std::vector<char> fell(K, 0);
     bool any = false, bad = false;
     for (int i = 0; i < K; ++i) {
@@ -1911,6 +1905,20 @@ int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *infos, const v
         infos[i].refinements = infos[i].cycles + 1;
     }
     return CG_OK;
+}
+
+// The refined method, or with `learned` its learned coarse-grid correction and its fallback, over
+// every instance.
+int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *infos, const volatile int *cancel,
+                  bool learned = false) {
+    const int K = p->K;
+    if (learned) p->net_fallbacks = 0;
+    int rc = refined_solve_core(p, tol, warm, infos, cancel, learned, nullptr);
+    if (rc || !learned) {
+        for (int i = 0; i < K; ++i) infos[i].refinements = infos[i].cycles + 1;
+        return rc;
+    }
+    return learned_fallback(p, tol, infos, cancel);
 }
 
 // The binary64 refinement whose inner solve is FGMRES(m) right-preconditioned by the binary32
@@ -2819,16 +2827,22 @@ int cg_ocean_group_destroy(cg_group *gr) {
     return CG_OK;
 }
 
-int cg_ocean_group_solve(cg_group *gr, double tol, cg_solution *infos) {
+int cg_ocean_group_solve(cg_group *gr, double tol, int method, cg_solution *infos) {
     if (!gr || !infos || !(tol > 0.0)) return CG_ERR_ARG;
     cg_ctx *ctx = gr->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (method != CG_METHOD_REFINED && method != CG_METHOD_LEARNED) {
+        set_err(ctx, "a group solves by CG_METHOD_REFINED or CG_METHOD_LEARNED");
+        return CG_ERR_ARG;
+    }
+    const bool learned = method == CG_METHOD_LEARNED;
     for (cg_plan *p : gr->plans) {
         if (!(ctx->fused && p->fusable)) {
             set_err(ctx, "a group's plans need the fused cycle (row-constant weights, halving grids)");
             return CG_ERR_STATE;
         }
+        if (learned && !learned_ready(p)) return CG_ERR_STATE;
     }
     const bool graph = ctx->use_graph && !ctx->time_kernels;
     struct Ungate {
@@ -2846,7 +2860,7 @@ int cg_ocean_group_solve(cg_group *gr, double tol, cg_solution *infos) {
     for (size_t k = 0; k < gr->plans.size(); ++k) {
         cg_plan *p = gr->plans[k];
         p->x64cur = 0;
-        if ((rc = run_begin(runs[k], p, tol, false, false, nullptr, graph))) return rc;
+        if ((rc = run_begin(runs[k], p, tol, false, learned, nullptr, graph))) return rc;
     }
     // The plans' streams start after the first outer steps on the context's stream.
     if (graph) {
@@ -2858,7 +2872,8 @@ int cg_ocean_group_solve(cg_group *gr, double tol, cg_solution *infos) {
             if (!on[k]) continue;
             cg_plan *p = gr->plans[k];
             if (graph) {
-                HIPCHK(ctx, hipGraphLaunch(p->step_exec[p->x64cur], gr->streams[k]));
+                hipGraphExec_t *const exec = p->net_on ? p->net_exec : p->step_exec;
+                HIPCHK(ctx, hipGraphLaunch(exec[p->x64cur], gr->streams[k]));
                 p->x64cur = 1 - p->x64cur;
             } else {
                 int r = refine_step(p, true);
@@ -2880,7 +2895,13 @@ int cg_ocean_group_solve(cg_group *gr, double tol, cg_solution *infos) {
     size_t at = 0;
     for (RefineRun &r : runs) {
         run_finish(r, r.launched, status, infos + at);
-        for (int i = 0; i < r.K; ++i) infos[at + i].refinements = infos[at + i].cycles + 1;
+        r.p->gated = false;
+        r.p->net_on = false;
+        if (learned) {
+            if ((rc = learned_fallback(r.p, tol, infos + at, nullptr))) return rc;
+        } else {
+            for (int i = 0; i < r.K; ++i) infos[at + i].refinements = infos[at + i].cycles + 1;
+        }
         at += (size_t)r.K;
     }
     return CG_OK;

@@ -563,11 +563,13 @@ class PlanGroup:
         check(lib().cg_ocean_group_create(arr, len(self.plans), ctypes.byref(self._g)), acc.ctx,
               "cg_ocean_group_create")
 
-    def solve(self, tol: float) -> List[List[CgSolution]]:
-        """One list of results per plan (one per instance)."""
+    def solve(self, tol: float, method: int = CG_METHOD_REFINED) -> List[List[CgSolution]]:
+        """One list of results per plan (one per instance); method CG_METHOD_REFINED or, with every
+        plan's coarse network set, CG_METHOD_LEARNED."""
         counts = [getattr(p, "instances", 1) for p in self.plans]
         infos = (CgSolution * sum(counts))()
-        check(lib().cg_ocean_group_solve(self._g, float(tol), infos), self.acc.ctx, "cg_ocean_group_solve")
+        check(lib().cg_ocean_group_solve(self._g, float(tol), int(method), infos), self.acc.ctx,
+              "cg_ocean_group_solve")
         out, at = [], 0
         for c in counts:
             out.append(list(infos[at:at + c]))

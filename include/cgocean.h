@@ -211,8 +211,10 @@ int cg_ocean_plan_solve_batch(cg_plan *const *plans, int count, double tol, int 
  * be destroyed while it does. */
 typedef struct cg_group cg_group;
 int cg_ocean_group_create(cg_plan *const *plans, int count, cg_group **out);
-/* infos: one per instance, the plans' instances in the order of the plans given to create. */
-int cg_ocean_group_solve(cg_group *group, double tol, cg_solution *infos);
+/* method: CG_METHOD_REFINED, or CG_METHOD_LEARNED (every plan with its coarse network set; the
+ * fallback to the plain cycle per plan as for cg_ocean_plan_solve_all). infos: one per instance, the
+ * plans' instances in the order of the plans given to create. */
+int cg_ocean_group_solve(cg_group *group, double tol, int method, cg_solution *infos);
 int cg_ocean_group_destroy(cg_group *group);
 /* The finest grid's water bodies (OceanCirculation.WaterBodies, OC:670-697: body per cell, -1 on
  * land, `count` bodies) for correctBodyMeans (OC:709-731), which then follows every cycle of

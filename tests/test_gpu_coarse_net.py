@@ -10,6 +10,9 @@ method's step graphs on a coarse grid of the cycle, in place of the grids below 
 - A network of zero weights stalls the residual: the solve falls back to the plain cycle and still
   reaches the oracle.
 - Instances of a batched plan take the steps and bits of their single-instance learned solves.
+- Plans of uneven size (256x512, 512x1024 batched, 2048x4096) with one network on their common 64x128
+  grid, solved as one group by the learned method: each instance takes the steps and bits of its
+  plan's own learned solve, and reaches the binary64 oracle to 1e-10.
 - The entry points refuse bad shapes, counts and levels.
 """
 import ctypes
@@ -24,7 +27,7 @@ from conftest import to_oracle
 from cartogenesis_amd import CG_METHOD_LEARNED, CG_METHOD_REFINED, CG_OPT_NET_FALLBACK
 from cartogenesis_amd.learned import (CoarseNetwork, UNet, coarse_depth, coarse_level, coarse_pairs, layer_table,
                                       train_coarse)
-from cartogenesis_amd.ocean import BatchPlan, Plan
+from cartogenesis_amd.ocean import BatchPlan, Plan, PlanGroup
 from cartogenesis_amd.problem import circulation_problem, monthly_forcings
 
 pytestmark = pytest.mark.gpu
@@ -151,6 +154,47 @@ def test_batched_learned_instances_repeat_their_single_solves(acc, trained):
     finally:
         batch.close()
     print(f"OCEAN batched learned months: {[r[0] for r in ref]} steps")
+
+
+def test_learned_group_of_uneven_sizes(acc, trained):
+    worlds = [(256, 11, 2), (512, 12, 2), (2048, 13, 1)]
+    probs, plans, alone = [], [], []
+    group = None
+    try:
+        for rows, seed, months in worlds:
+            p = circulation_problem(rows, seed=seed)
+            forcings = monthly_forcings(p, months=range(0, 12, 12 // months))[:months]
+            plan = BatchPlan(acc, p.levels, months)
+            for i, f in enumerate(forcings):
+                plan.set_forcing(i, f)
+            trained.attach(plan, p)
+            probs.append((p, forcings))
+            plans.append(plan)
+            infos = (cg.CgSolution * months)()
+            print(f"OCEAN learned group: {2 * rows}x{rows} alone", flush=True)
+            cg.check(cg.lib().cg_ocean_plan_solve_all(plan.handle, 1e-8, CG_METHOD_LEARNED, infos), acc.ctx, "solve_all")
+            alone.append(([i.cycles for i in infos], [plan.read(k)[0].copy() for k in range(months)]))
+        group = PlanGroup(acc, plans)
+        print("OCEAN learned group: together", flush=True)
+        together = group.solve(1e-8, CG_METHOD_LEARNED)
+        for (rows, _, months), (p, forcings), plan, infos, (cycles, answers) in zip(worlds, probs, plans, together,
+                                                                                  alone):
+            assert [i.cycles for i in infos] == cycles, rows
+            for k in range(months):
+                x = plan.read(k)[0]
+                assert np.array_equal(x.view(np.uint64), answers[k].view(np.uint64)), (rows, k)
+                assert infos[k].relative_residual < 1e-8
+            ladder = [to_oracle(s) for s in p.levels]
+            ladder[0] = ladder[0].with_forcing(forcings[0])
+            ref = oracle.solve64(ladder, 1e-12)[0]
+            err = rel_l2(plan.read(0)[0], ref)
+            print(f"OCEAN learned group {2 * rows}x{rows}: steps {[i.cycles for i in infos]}, month 0 rel L2 {err:.2e}")
+            assert err <= 1e-10
+    finally:
+        if group:
+            group.close()
+        for plan in plans:
+            plan.close()
 
 
 def test_coarse_net_entry_points_refuse_bad_arguments(acc):
