@@ -47,8 +47,8 @@ FUSED_BYTES = 34 * CELLS
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE and WRITE_SIZE (scaled by a 1 GiB copy),
 # for the default settings (two passes per grid), by the finest grid's tile rows; see
 # profiles/hbm_traffic_*.md.
-MEASURED_TRAFFIC = {"outer": 82696380, "up": {4: 30604038, 6: 30726349, 8: 30631686},
-                    "fused": {4: 73863905, 6: 73841418, 8: 73819458}}
+MEASURED_TRAFFIC = {"outer": 82793991, "up": {4: 30604038, 6: 30719584, 8: 30631686},
+                    "fused": {4: 73863905, 6: 73804771, 8: 73819458}}
 
 
 def log(msg: str) -> None:
