@@ -147,7 +147,7 @@ struct GateArgs {
     int *active;
     int *want_sums;  // per instance: the step's outer kernel computes the sums
     const double *thr;
-    double etol;  // <= 0: the residual alone decides
+    const double *etol;  // per instance; <= 0: the residual alone decides
     // A learned step's fallback: from the fourth step on, a residual above fall_rate squared times
     // the one two steps before (a rate of fall_rate a step over two steps) stops the instance with
     // `fell` set (the host goes on from its answer with the plain cycle). fall_rate <= 0: never.
@@ -159,7 +159,7 @@ __device__ bool gate_decide(const GateArgs &g, int inst, bool writer) {
     const int lane = threadIdx.x & 63;
     const StepGate s = g.in[inst];
     bool go = false, want = false, fell = s.fell != 0;
-    double d2 = 0.0, res = 0.0;
+    double d2 = 0.0, res = 0.0, etol = 0.0;
     if (s.active) {
         const unsigned long long r = wave_max_u64(g.res[inst * g.res_stride + lane]);
         double x2 = 0.0;
@@ -169,8 +169,9 @@ __device__ bool gate_decide(const GateArgs &g, int inst, bool writer) {
         }
         res = __longlong_as_double((long long)r);
         const double thr = g.thr[inst];
+        etol = g.etol[inst];
         bool err_ok = true;
-        if (g.etol > 0.0 && s.steps > 0) {
+        if (etol > 0.0 && s.steps > 0) {
             if (!s.want) {
                 err_ok = false;  // the step ran without the sums: no estimate yet
             } else if (x2 > 0.0) {
@@ -180,7 +181,7 @@ __device__ bool gate_decide(const GateArgs &g, int inst, bool writer) {
                     rho = before > rho ? before : rho;
                 }
                 rho = rho < 0.95 ? rho : 0.95;
-                err_ok = sqrt(d2 / x2) * (rho / (1.0 - rho)) <= g.etol;
+                err_ok = sqrt(d2 / x2) * (rho / (1.0 - rho)) <= etol;
             } else {
                 err_ok = d2 <= 0.0;
             }
@@ -190,7 +191,7 @@ __device__ bool gate_decide(const GateArgs &g, int inst, bool writer) {
             go = false;
             fell = true;
         }
-        want = go && g.etol > 0.0 && res < kSumsAhead * thr;
+        want = go && etol > 0.0 && res < kSumsAhead * thr;
     }
     if (writer && lane == 0) {
         StepGate o;

@@ -34,6 +34,11 @@ constexpr int kDirectMaxWater = 2048;     // largest coarsest grid solved by a d
 constexpr int kRefinedMostSteps = 1000;
 // The learned correction's fallback: a residual falling by less than this factor a step.
 constexpr double kNetFallbackRate = 0.8;
+// The learned correction's share of the error bound: its contraction varies from step to step more
+// than the plain cycle's, so the estimate rho / (1 - rho) |delta| taken from the last two steps can
+// fall short of the error by a few percent; its steps (and the plain steps that go on from a learned
+// answer after a fallback) stop once the estimate is under half the bound.
+constexpr double kNetErrorShare = 0.5;
 
 struct Stencil {
     int across = 0, down = 0;
@@ -353,8 +358,8 @@ struct cg_plan {
     StepGate *d_gate = nullptr, *h_gate = nullptr;
     unsigned long long *d_res = nullptr, *h_res = nullptr;
     double *d_sums = nullptr, *h_sums = nullptr;
-    double *d_thr = nullptr, *h_thr = nullptr;
-    std::vector<double> thr_up;  // the thresholds last uploaded
+    double *d_thr = nullptr, *h_thr = nullptr;  // per instance the residual threshold, then the error bound
+    std::vector<double> thr_up;  // the thresholds and bounds last uploaded
     int *d_want = nullptr;
     bool gated = false;
     bool gate_in_down = false;  // the next finest down leg runs the gate (set by refine_step)
@@ -801,7 +806,7 @@ GateArgs gate_args(const cg_plan *p) {
     g.active = p->d_active;
     g.want_sums = p->d_want;
     g.thr = p->d_thr;
-    g.etol = p->ctx->error_tol;
+    g.etol = p->d_thr + p->K;  // set with the thresholds by gate_reset
     g.fall_rate = (p->net_on && p->ctx->net_fallback) ? kNetFallbackRate : 0.0;
     return g;
 }
@@ -1572,19 +1577,21 @@ int build_step_graphs(cg_plan *p) {
 }
 
 // Sets up the step gate for a refinement whose first graph step reads binary64 parity first_par:
-// every instance active with no steps run, residual thresholds thr[i] (absolute: tol x max |f c|),
-// and every instance's residual and sums groups zeroed (the first outer step reduces into parity
-// 1 - first_par's, which the first gate reads).
-int gate_reset(cg_plan *p, int first_par, const std::vector<double> &thr, const std::vector<char> *only = nullptr) {
+// every instance active with no steps run, residual thresholds thr[i] (absolute: tol x max |f c|)
+// and the error bound etol of every instance, and every instance's residual and sums groups zeroed
+// (the first outer step reduces into parity 1 - first_par's, which the first gate reads). The
+// bounds live in device memory beside the thresholds, so one step graph serves every bound.
+int gate_reset(cg_plan *p, int first_par, const std::vector<double> &thr, double etol,
+               const std::vector<char> *only = nullptr) {
     cg_ctx *ctx = p->ctx;
     const int K = p->K;
-    // The thresholds cross the bus only when they change (a new forcing or tol).
-    bool same = (int)p->thr_up.size() == K;
-    for (int i = 0; same && i < K; ++i) same = p->thr_up[i] == thr[i];
-    if (!same) {
-        for (int i = 0; i < K; ++i) p->h_thr[i] = thr[i];
-        HIPCHK(ctx, hipMemcpyAsync(p->d_thr, p->h_thr, sizeof(double) * K, hipMemcpyHostToDevice, ctx->stream));
-        p->thr_up = thr;
+    // The thresholds and bounds cross the bus only when they change (a new forcing, tol or method).
+    std::vector<double> up(thr);
+    up.resize(2 * K, etol);
+    if (p->thr_up != up) {
+        for (int i = 0; i < 2 * K; ++i) p->h_thr[i] = up[i];
+        HIPCHK(ctx, hipMemcpyAsync(p->d_thr, p->h_thr, sizeof(double) * 2 * K, hipMemcpyHostToDevice, ctx->stream));
+        p->thr_up = up;
     }
     const long nclear = (long)2 * K * kMaxSlots + (long)4 * K * kMaxSlots;  // residual groups, then sums
     if (only) {
@@ -1636,12 +1643,12 @@ struct RefineRun {
 // the plan's answer when warm); the plan's steps gated (and learned) from here on, its step graphs
 // built when `graph`.
 int run_begin(RefineRun &r, cg_plan *p, double tol_asked, bool warm, bool learned, const std::vector<char> *only,
-              bool graph) {
+              bool graph, double error_share = 1.0) {
     cg_ctx *ctx = p->ctx;
     r.p = p;
     r.K = p->K;
     r.tol = tol_asked * (ctx->stop_permille / 1000.0);
-    r.etol = ctx->error_tol;
+    r.etol = ctx->error_tol * error_share;
     r.learned = learned;
     r.only = only;
     const int K = p->K;
@@ -1658,7 +1665,7 @@ int run_begin(RefineRun &r, cg_plan *p, double tol_asked, bool warm, bool learne
     // The first outer step reads parity q0 (zero, from cold) and writes 1 - q0, where the steps start.
     const int q0 = p->x64cur;
     r.first_par = 1 - q0;
-    if ((rc = gate_reset(p, r.first_par, r.thr, only))) return rc;
+    if ((rc = gate_reset(p, r.first_par, r.thr, r.etol, only))) return rc;
     float *const first_fc = outer_fused(p) ? level0_forcing(p, r.first_par) : p->lv[0].f;
     if (r.cold) {
         // From zero: x = 0, u = 0 - (0 - f) on water; its maxima in parity q0's groups.
@@ -1831,7 +1838,7 @@ int run_steps(std::vector<RefineRun> &runs, const volatile int *cancel, Step ste
 // runs exactly the steps it would alone, however the host groups the steps into batches between
 // its reads; a stopped instance's kernels return at once and its answer stays where it stopped.
 int refined_solve_core(cg_plan *p, double tol_asked, bool warm, cg_solution *infos, const volatile int *cancel,
-                       bool learned, const std::vector<char> *only) {
+                       bool learned, const std::vector<char> *only, double error_share = 1.0) {
     cg_ctx *ctx = p->ctx;
     const bool graph = ctx->use_graph && !ctx->time_kernels;
     struct Ungate {
@@ -1842,7 +1849,7 @@ int refined_solve_core(cg_plan *p, double tol_asked, bool warm, cg_solution *inf
         }
     } ungate{p};
     std::vector<RefineRun> runs(1);
-    int rc = run_begin(runs[0], p, tol_asked, warm, learned, only, graph);
+    int rc = run_begin(runs[0], p, tol_asked, warm, learned, only, graph, error_share);
     if (rc) return rc;
     auto step = [&](const std::vector<char> &) -> int {
         if (graph) {
@@ -1896,7 +1903,9 @@ int learned_fallback(cg_plan *p, double tol, cg_solution *infos, const volatile 
     }
     p->net_fallbacks = count;
     std::vector<cg_solution> more(infos, infos + K);
-    if ((rc = refined_solve_core(p, tol, true, more.data(), cancel, false, &fell))) return rc;
+    // The plain steps that go on from a learned answer stop under the learned method's bound too:
+    // their gate starts afresh, so its first contraction estimates rest on one or two ratios.
+    if ((rc = refined_solve_core(p, tol, true, more.data(), cancel, false, &fell, kNetErrorShare))) return rc;
     for (int i = 0; i < K; ++i) {
         if (!fell[i]) continue;
         infos[i].cycles += more[i].cycles;
@@ -1913,7 +1922,7 @@ int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *infos, const v
                   bool learned = false) {
     const int K = p->K;
     if (learned) p->net_fallbacks = 0;
-    int rc = refined_solve_core(p, tol, warm, infos, cancel, learned, nullptr);
+    int rc = refined_solve_core(p, tol, warm, infos, cancel, learned, nullptr, learned ? kNetErrorShare : 1.0);
     if (rc || !learned) {
         for (int i = 0; i < K; ++i) infos[i].refinements = infos[i].cycles + 1;
         return rc;
@@ -2505,12 +2514,12 @@ int cg_ocean_plan_create_batch(cg_ctx *ctx, const cg_stencil *levels, int nlevel
         p->h_res = reinterpret_cast<unsigned long long *>(p->h_check + gate_bytes);
         p->h_sums = reinterpret_cast<double *>(p->h_check + gate_bytes + res_bytes);
     }
-    if ((rc = dalloc(ctx, &p->d_thr, K))) return bail(rc);
+    if ((rc = dalloc(ctx, &p->d_thr, 2 * K))) return bail(rc);
     if ((rc = dalloc(ctx, &p->d_want, K))) return bail(rc);
     HIPCHK(ctx, hipMemset(p->d_want, 0, sizeof(int) * K));
     HIPCHK(ctx, hipMemset(p->d_active, 0, sizeof(int) * K));
     if (hipHostMalloc((void **)&p->h_active, sizeof(int) * K, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&p->h_thr, sizeof(double) * K, hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void **)&p->h_thr, sizeof(double) * 2 * K, hipHostMallocDefault) != hipSuccess) {
         set_err(ctx, "pinned host memory for the plan's flags was refused");
         return bail(CG_ERR_NOMEM);
     }
@@ -2860,7 +2869,7 @@ int cg_ocean_group_solve(cg_group *gr, double tol, int method, cg_solution *info
     for (size_t k = 0; k < gr->plans.size(); ++k) {
         cg_plan *p = gr->plans[k];
         p->x64cur = 0;
-        if ((rc = run_begin(runs[k], p, tol, false, learned, nullptr, graph))) return rc;
+        if ((rc = run_begin(runs[k], p, tol, false, learned, nullptr, graph, learned ? kNetErrorShare : 1.0))) return rc;
     }
     // The plans' streams start after the first outer steps on the context's stream.
     if (graph) {
@@ -3341,7 +3350,7 @@ int cg_ocean_plan_autotune(cg_plan *p, int reps, int *rows_out) {
     const Stencil *st = p->lv[0].st;
     HIPCHK(ctx, hipMemsetAsync(p->x64[p->x64cur], 0, sizeof(double) * st->cells * K, ctx->stream));
     // The step gate with thresholds of zero never stops an instance, so every timed step runs.
-    int rc = gate_reset(p, 1 - p->x64cur, std::vector<double>(K, 0.0));
+    int rc = gate_reset(p, 1 - p->x64cur, std::vector<double>(K, 0.0), ctx->error_tol);
     if (rc) return rc;
     rc = outer_step(p, nullptr, outer_fused(p) ? level0_forcing(p, 1 - p->x64cur) : nullptr);
     if (rc) return rc;

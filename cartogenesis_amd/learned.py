@@ -455,7 +455,10 @@ def coarse_pairs(acc, seeds: Sequence[int], rows: int = 1024, land_fraction: flo
     from .problem import circulation_problem
 
     seeds = [int(v) for v in seeds]
-    xs, fs, ws, ratios = [], [], [], []
+    # Per pair only the scaled forcing, the target and the pair's world; per world its mask and depth
+    # (the network's fixed input channels). The 16-channel inputs are assembled on the device.
+    fins, fs, world_of, ratios = [], [], [], []
+    fixed_w, water_w = [], []
     level, shape = -1, None
     solves = 0
 
@@ -483,7 +486,9 @@ def coarse_pairs(acc, seeds: Sequence[int], rows: int = 1024, land_fraction: flo
                 solves += 1
                 depth = coarse_depth(prob, L)
                 water = (np.asarray(st.is_water) != 0)
-                fixed = np.stack([water.astype(np.float32), depth / DEPTH_SCALE], axis=1)
+                w = len(fixed_w)
+                fixed_w.append(np.stack([water.astype(np.float32), depth / DEPTH_SCALE], axis=1))
+                water_w.append(water.astype(np.float32))
                 for f in out[:n]:
                     rms_f = float(np.sqrt(np.mean(f.astype(np.float64) ** 2)))
                     if rms_f == 0.0:
@@ -493,25 +498,27 @@ def coarse_pairs(acc, seeds: Sequence[int], rows: int = 1024, land_fraction: flo
                     if not info.relative_residual < 1e-11:
                         raise RuntimeError(f"coarse target solve stopped at {info.relative_residual:.2e}")
                     x64, _ = sub.read()
-                    rms_f32 = np.float32(rms_f)
-                    xin = np.zeros((cells, 16), np.float32)
-                    xin[:, :2] = fixed
-                    xin[:, 2] = f / rms_f32
-                    xs.append(xin.reshape(*shape, 16))
-                    fs.append((x64 / rms_f).astype(np.float32).reshape(shape))
-                    ws.append(water.astype(np.float32).reshape(shape))
+                    fins.append(f / np.float32(rms_f))
+                    fs.append((x64 / rms_f).astype(np.float32))
+                    world_of.append(w)
                     ratios.append(float(np.sqrt(np.mean(x64 ** 2))) / rms_f)
             finally:
                 sub.close()
                 plan.close()
-    if not xs:
+    if not fins:
         raise RuntimeError("no training pairs")
     out_scale = float(np.median(ratios))
     dev = "cuda"
-    targets = torch.from_numpy(np.stack(fs) / np.float32(out_scale)).to(dev)
-    return {"inputs": torch.from_numpy(np.stack(xs)).to(dev).to(BF16), "targets": targets,
-            "water": torch.from_numpy(np.stack(ws)).to(dev), "out_scale": out_scale, "level": level,
-            "shape": shape, "worlds": len(seeds), "solves": solves, "pairs": len(xs)}
+    P = len(fins)
+    idx = torch.as_tensor(np.asarray(world_of, np.int64), device=dev)
+    inputs = torch.zeros((P, shape[0] * shape[1], 16), dtype=BF16, device=dev)
+    inputs[:, :, :2] = torch.from_numpy(np.stack(fixed_w)).to(dev)[idx].to(BF16)
+    inputs[:, :, 2] = torch.from_numpy(np.stack(fins)).to(dev).to(BF16)
+    del fins
+    targets = torch.from_numpy(np.stack(fs)).to(dev).reshape(P, *shape) / np.float32(out_scale)
+    water = torch.from_numpy(np.stack(water_w)).to(dev)[idx].reshape(P, *shape)
+    return {"inputs": inputs.reshape(P, *shape, 16), "targets": targets, "water": water, "out_scale": out_scale,
+            "level": level, "shape": shape, "worlds": len(seeds), "solves": solves, "pairs": P}
 
 
 class CoarseNetwork:

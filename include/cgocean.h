@@ -99,7 +99,9 @@ extern "C" {
                                     and its contraction rho against the one before; a residual under tol
                                     alone does not bound the error (a forcing of fine-scale noise leaves a
                                     smooth error whose residual is small). Each instance stops at the first
-                                    step where both hold, tested on the device before every step. */
+                                    step where both hold, tested on the device before every step.
+                                    CG_METHOD_LEARNED takes half the bound (its contraction varies from
+                                    step to step, so its estimate needs the margin). */
 
 typedef struct cg_ctx cg_ctx;
 typedef struct cg_plan cg_plan;
