@@ -147,7 +147,8 @@ struct GateArgs {
     int *active;
     int *want_sums;  // per instance: the step's outer kernel computes the sums
     const double *thr;
-    const double *etol;  // per instance; <= 0: the residual alone decides
+    const double *etol;       // per instance; <= 0: the residual alone decides
+    const double *min_steps;  // per instance: no stop before this many steps of this refinement
     // A learned step's fallback: from the fourth step on, a residual above fall_rate squared times
     // the one two steps before (a rate of fall_rate a step over two steps) stops the instance with
     // `fell` set (the host goes on from its answer with the plain cycle). fall_rate <= 0: never.
@@ -186,7 +187,7 @@ __device__ bool gate_decide(const GateArgs &g, int inst, bool writer) {
                 err_ok = d2 <= 0.0;
             }
         }
-        go = !(res != res || (res < thr && err_ok));
+        go = !(res != res || (res < thr && err_ok && s.steps >= (int)g.min_steps[inst]));
         if (go && g.fall_rate > 0.0 && s.steps >= 3 && !(res <= g.fall_rate * g.fall_rate * s.res_prev2)) {
             go = false;
             fell = true;

@@ -34,11 +34,19 @@ constexpr int kDirectMaxWater = 2048;     // largest coarsest grid solved by a d
 constexpr int kRefinedMostSteps = 1000;
 // The learned correction's fallback: a residual falling by less than this factor a step.
 constexpr double kNetFallbackRate = 0.8;
-// The learned correction's share of the error bound: its contraction varies from step to step more
-// than the plain cycle's, so the estimate rho / (1 - rho) |delta| taken from the last two steps can
-// fall short of the error by a few percent; its steps (and the plain steps that go on from a learned
-// answer after a fallback) stop once the estimate is under half the bound.
-constexpr double kNetErrorShare = 0.5;
+// Plain multigrid judges a learned answer's error: the estimate rho / (1 - rho) |delta| assumes one
+// contraction for every component of the error, which the plain cycle's nearly uniform smoothing
+// and exact coarse solve give, but a learned correction does not (a smooth component it gets
+// slightly wrong decays slowly while |delta| shrinks with the fast ones). So the learned steps stop
+// on the residual alone, and every instance then goes on from its learned answer with plain steps,
+// at least kJudgeSteps of them (so that the plain estimate's contraction is the larger of two ratios
+// of plain corrections: after two steps its one ratio still carries the learned steps' fast
+// transient), until the plain method's own stopping rule holds.
+constexpr int kJudgeSteps = 3;
+// ... and under this share of the error bound: warm from a learned answer, the plain estimate can
+// still fall a percent or so short of the error at the step where it first holds (measured at
+// 512 x 1024: 1.01e-10 under the bound 1e-10; one more step gives 3e-11).
+constexpr double kJudgeErrorShare = 0.5;
 
 struct Stencil {
     int across = 0, down = 0;
@@ -358,7 +366,8 @@ struct cg_plan {
     StepGate *d_gate = nullptr, *h_gate = nullptr;
     unsigned long long *d_res = nullptr, *h_res = nullptr;
     double *d_sums = nullptr, *h_sums = nullptr;
-    double *d_thr = nullptr, *h_thr = nullptr;  // per instance the residual threshold, then the error bound
+    // Per instance the residual threshold, then the error bound, then the fewest steps before a stop.
+    double *d_thr = nullptr, *h_thr = nullptr;
     std::vector<double> thr_up;  // the thresholds and bounds last uploaded
     int *d_want = nullptr;
     bool gated = false;
@@ -806,7 +815,8 @@ GateArgs gate_args(const cg_plan *p) {
     g.active = p->d_active;
     g.want_sums = p->d_want;
     g.thr = p->d_thr;
-    g.etol = p->d_thr + p->K;  // set with the thresholds by gate_reset
+    g.etol = p->d_thr + p->K;           // set with the thresholds by gate_reset
+    g.min_steps = p->d_thr + 2 * p->K;  // likewise
     g.fall_rate = (p->net_on && p->ctx->net_fallback) ? kNetFallbackRate : 0.0;
     return g;
 }
@@ -834,7 +844,7 @@ struct GateView {
 };
 
 GateView gate_view(const unsigned long long *res_slots, const double *sums, const StepGate &st, double thr,
-                   double etol) {
+                   double etol, int min_steps) {
     GateView v{};
     unsigned long long m = 0;
     for (int l = 0; l < kMaxSlots; ++l) m = res_slots[l] > m ? res_slots[l] : m;
@@ -862,7 +872,7 @@ GateView gate_view(const unsigned long long *res_slots, const double *sums, cons
             err_ok = v.d2 <= 0.0;
         }
     }
-    v.stop = !st.active || v.res != v.res || (v.res < thr && err_ok);
+    v.stop = !st.active || v.res != v.res || (v.res < thr && err_ok && st.steps >= min_steps);
     return v;
 }
 
@@ -1577,20 +1587,22 @@ int build_step_graphs(cg_plan *p) {
 }
 
 // Sets up the step gate for a refinement whose first graph step reads binary64 parity first_par:
-// every instance active with no steps run, residual thresholds thr[i] (absolute: tol x max |f c|)
-// and the error bound etol of every instance, and every instance's residual and sums groups zeroed
-// (the first outer step reduces into parity 1 - first_par's, which the first gate reads). The
-// bounds live in device memory beside the thresholds, so one step graph serves every bound.
-int gate_reset(cg_plan *p, int first_par, const std::vector<double> &thr, double etol,
+// every instance active with no steps run, residual thresholds thr[i] (absolute: tol x max |f c|),
+// the error bound etol and the fewest steps min_steps of every instance, and every instance's
+// residual and sums groups zeroed (the first outer step reduces into parity 1 - first_par's, which
+// the first gate reads). Bounds and counts live in device memory beside the thresholds, so one
+// step graph serves every setting.
+int gate_reset(cg_plan *p, int first_par, const std::vector<double> &thr, double etol, int min_steps,
                const std::vector<char> *only = nullptr) {
     cg_ctx *ctx = p->ctx;
     const int K = p->K;
-    // The thresholds and bounds cross the bus only when they change (a new forcing, tol or method).
+    // They cross the bus only when they change (a new forcing, tol or method).
     std::vector<double> up(thr);
     up.resize(2 * K, etol);
+    up.resize(3 * K, (double)min_steps);
     if (p->thr_up != up) {
-        for (int i = 0; i < 2 * K; ++i) p->h_thr[i] = up[i];
-        HIPCHK(ctx, hipMemcpyAsync(p->d_thr, p->h_thr, sizeof(double) * 2 * K, hipMemcpyHostToDevice, ctx->stream));
+        for (int i = 0; i < 3 * K; ++i) p->h_thr[i] = up[i];
+        HIPCHK(ctx, hipMemcpyAsync(p->d_thr, p->h_thr, sizeof(double) * 3 * K, hipMemcpyHostToDevice, ctx->stream));
         p->thr_up = up;
     }
     const long nclear = (long)2 * K * kMaxSlots + (long)4 * K * kMaxSlots;  // residual groups, then sums
@@ -1612,6 +1624,7 @@ struct RefineRun {
     cg_plan *p = nullptr;
     int K = 0;
     double tol = 0.0, etol = 0.0;
+    int min_steps = 0;
     bool cold = true, learned = false;
     int first_par = 0;
     const std::vector<char> *only = nullptr;
@@ -1629,7 +1642,7 @@ struct RefineRun {
     GateView view_of(int i, const StepGate &sg) const {
         const int g = (first_par + sg.steps + 1) & 1;  // the group its last step (or the first outer) reduced into
         return gate_view(p->h_res + (size_t)(2 * i + g) * kMaxSlots, p->h_sums + (size_t)(2 * i + g) * 2 * kMaxSlots,
-                         sg, thr[i], etol);
+                         sg, thr[i], etol, min_steps);
     }
     // Enqueues the copy of the gate states of both parities, the residual groups and the sums of
     // every instance (the caller synchronizes the stream).
@@ -1643,12 +1656,13 @@ struct RefineRun {
 // the plan's answer when warm); the plan's steps gated (and learned) from here on, its step graphs
 // built when `graph`.
 int run_begin(RefineRun &r, cg_plan *p, double tol_asked, bool warm, bool learned, const std::vector<char> *only,
-              bool graph, double error_share = 1.0) {
+              bool graph, double error_share = 1.0, int min_steps = 0) {
     cg_ctx *ctx = p->ctx;
     r.p = p;
     r.K = p->K;
     r.tol = tol_asked * (ctx->stop_permille / 1000.0);
     r.etol = ctx->error_tol * error_share;
+    r.min_steps = min_steps;
     r.learned = learned;
     r.only = only;
     const int K = p->K;
@@ -1665,7 +1679,7 @@ int run_begin(RefineRun &r, cg_plan *p, double tol_asked, bool warm, bool learne
     // The first outer step reads parity q0 (zero, from cold) and writes 1 - q0, where the steps start.
     const int q0 = p->x64cur;
     r.first_par = 1 - q0;
-    if ((rc = gate_reset(p, r.first_par, r.thr, r.etol, only))) return rc;
+    if ((rc = gate_reset(p, r.first_par, r.thr, r.etol, r.min_steps, only))) return rc;
     float *const first_fc = outer_fused(p) ? level0_forcing(p, r.first_par) : p->lv[0].f;
     if (r.cold) {
         // From zero: x = 0, u = 0 - (0 - f) on water; its maxima in parity q0's groups.
@@ -1838,7 +1852,7 @@ int run_steps(std::vector<RefineRun> &runs, const volatile int *cancel, Step ste
 // runs exactly the steps it would alone, however the host groups the steps into batches between
 // its reads; a stopped instance's kernels return at once and its answer stays where it stopped.
 int refined_solve_core(cg_plan *p, double tol_asked, bool warm, cg_solution *infos, const volatile int *cancel,
-                       bool learned, const std::vector<char> *only, double error_share = 1.0) {
+                       bool learned, const std::vector<char> *only, double error_share = 1.0, int min_steps = 0) {
     cg_ctx *ctx = p->ctx;
     const bool graph = ctx->use_graph && !ctx->time_kernels;
     struct Ungate {
@@ -1849,7 +1863,7 @@ int refined_solve_core(cg_plan *p, double tol_asked, bool warm, cg_solution *inf
         }
     } ungate{p};
     std::vector<RefineRun> runs(1);
-    int rc = run_begin(runs[0], p, tol_asked, warm, learned, only, graph, error_share);
+    int rc = run_begin(runs[0], p, tol_asked, warm, learned, only, graph, error_share, min_steps);
     if (rc) return rc;
     auto step = [&](const std::vector<char> &) -> int {
         if (graph) {
@@ -1866,19 +1880,22 @@ int refined_solve_core(cg_plan *p, double tol_asked, bool warm, cg_solution *inf
     return CG_OK;
 }
 
-// The multigrid fallback of the learned correction, after a learned solve's steps (infos[i].refinements
-// holding whether instance i's gate stopped it for its stalling residual): such an instance goes on
-// from its answer with the plain cycle (the steps of both count), and if a learned answer is not a
-// number the whole solve starts again, plainly.
+// After a learned solve's steps (infos[i].refinements holding whether instance i's gate stopped it
+// for its stalling residual: a fallback): plain multigrid judges every answer. Each instance goes on
+// from its learned answer (or, if one is not a number, the whole solve starts again plainly) with the
+// plain cycle, at least kJudgeSteps steps and then until the plain method's residual and error rule
+// holds under kJudgeErrorShare of the bound; the steps of both count. net_fallbacks counts the
+// instances whose residual stalled.
 int learned_fallback(cg_plan *p, double tol, cg_solution *infos, const volatile int *cancel) {
     cg_ctx *ctx = p->ctx;
     const int K = p->K;
     int rc;
-    std::vector<char> fell(K, 0);
-    bool any = false, bad = false;
+    std::vector<char> judged(K, 0);
+    bool bad = false;
+    int stalled = 0;
     for (int i = 0; i < K; ++i) {
-        fell[i] = infos[i].refinements != 0 && infos[i].status == CG_OK;
-        any = any || fell[i];
+        stalled += infos[i].refinements != 0 && infos[i].status == CG_OK;
+        judged[i] = infos[i].status == CG_OK;
         bad = bad || !(infos[i].relative_residual == infos[i].relative_residual);
         infos[i].refinements = infos[i].cycles + 1;
     }
@@ -1888,26 +1905,25 @@ int learned_fallback(cg_plan *p, double tol, cg_solution *infos, const volatile 
         for (int i = 0; i < K; ++i) infos[i].refinements = infos[i].cycles + 1;
         return rc;
     }
-    if (!any) return CG_OK;
-    // Each stopped instance's answer into the parity the plain solve reads, then that solve from
-    // those answers for them alone.
+    p->net_fallbacks = stalled;
+    if (!ctx->net_fallback) return CG_OK;  // the network measured alone: no plain steps at all
+    // Each answer into the parity the plain steps read, then those steps from the answers.
     const long n = p->lv[0].st->cells;
     const int q = p->x64cur;
-    int count = 0;
+    bool any = false;
     for (int i = 0; i < K; ++i) {
-        if (!fell[i]) continue;
-        ++count;
+        if (!judged[i]) continue;
+        any = true;
         if (p->ans_par[i] != q)
             HIPCHK(ctx, hipMemcpyAsync(p->x64[q] + i * n, p->x64[p->ans_par[i]] + i * n, sizeof(double) * n,
                                        hipMemcpyDeviceToDevice, ctx->stream));
     }
-    p->net_fallbacks = count;
+    if (!any) return CG_OK;
     std::vector<cg_solution> more(infos, infos + K);
-    // The plain steps that go on from a learned answer stop under the learned method's bound too:
-    // their gate starts afresh, so its first contraction estimates rest on one or two ratios.
-    if ((rc = refined_solve_core(p, tol, true, more.data(), cancel, false, &fell, kNetErrorShare))) return rc;
+    if ((rc = refined_solve_core(p, tol, true, more.data(), cancel, false, &judged, kJudgeErrorShare, kJudgeSteps)))
+        return rc;
     for (int i = 0; i < K; ++i) {
-        if (!fell[i]) continue;
+        if (!judged[i]) continue;
         infos[i].cycles += more[i].cycles;
         infos[i].relative_residual = more[i].relative_residual;
         infos[i].status = more[i].status;
@@ -1916,13 +1932,13 @@ int learned_fallback(cg_plan *p, double tol, cg_solution *infos, const volatile 
     return CG_OK;
 }
 
-// The refined method, or with `learned` its learned coarse-grid correction and its fallback, over
-// every instance.
+// The refined method, or with `learned` its learned coarse-grid correction and the plain judge of its
+// answers, over every instance.
 int refined_solve(cg_plan *p, double tol, bool warm, cg_solution *infos, const volatile int *cancel,
                   bool learned = false) {
     const int K = p->K;
     if (learned) p->net_fallbacks = 0;
-    int rc = refined_solve_core(p, tol, warm, infos, cancel, learned, nullptr, learned ? kNetErrorShare : 1.0);
+    int rc = refined_solve_core(p, tol, warm, infos, cancel, learned, nullptr, learned ? 0.0 : 1.0);
     if (rc || !learned) {
         for (int i = 0; i < K; ++i) infos[i].refinements = infos[i].cycles + 1;
         return rc;
@@ -2514,12 +2530,12 @@ int cg_ocean_plan_create_batch(cg_ctx *ctx, const cg_stencil *levels, int nlevel
         p->h_res = reinterpret_cast<unsigned long long *>(p->h_check + gate_bytes);
         p->h_sums = reinterpret_cast<double *>(p->h_check + gate_bytes + res_bytes);
     }
-    if ((rc = dalloc(ctx, &p->d_thr, 2 * K))) return bail(rc);
+    if ((rc = dalloc(ctx, &p->d_thr, 3 * K))) return bail(rc);
     if ((rc = dalloc(ctx, &p->d_want, K))) return bail(rc);
     HIPCHK(ctx, hipMemset(p->d_want, 0, sizeof(int) * K));
     HIPCHK(ctx, hipMemset(p->d_active, 0, sizeof(int) * K));
     if (hipHostMalloc((void **)&p->h_active, sizeof(int) * K, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&p->h_thr, sizeof(double) * 2 * K, hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void **)&p->h_thr, sizeof(double) * 3 * K, hipHostMallocDefault) != hipSuccess) {
         set_err(ctx, "pinned host memory for the plan's flags was refused");
         return bail(CG_ERR_NOMEM);
     }
@@ -2869,7 +2885,7 @@ int cg_ocean_group_solve(cg_group *gr, double tol, int method, cg_solution *info
     for (size_t k = 0; k < gr->plans.size(); ++k) {
         cg_plan *p = gr->plans[k];
         p->x64cur = 0;
-        if ((rc = run_begin(runs[k], p, tol, false, learned, nullptr, graph, learned ? kNetErrorShare : 1.0))) return rc;
+        if ((rc = run_begin(runs[k], p, tol, false, learned, nullptr, graph, learned ? 0.0 : 1.0))) return rc;
     }
     // The plans' streams start after the first outer steps on the context's stream.
     if (graph) {
@@ -3350,7 +3366,7 @@ int cg_ocean_plan_autotune(cg_plan *p, int reps, int *rows_out) {
     const Stencil *st = p->lv[0].st;
     HIPCHK(ctx, hipMemsetAsync(p->x64[p->x64cur], 0, sizeof(double) * st->cells * K, ctx->stream));
     // The step gate with thresholds of zero never stops an instance, so every timed step runs.
-    int rc = gate_reset(p, 1 - p->x64cur, std::vector<double>(K, 0.0), ctx->error_tol);
+    int rc = gate_reset(p, 1 - p->x64cur, std::vector<double>(K, 0.0), ctx->error_tol, 0);
     if (rc) return rc;
     rc = outer_step(p, nullptr, outer_fused(p) ? level0_forcing(p, 1 - p->x64cur) : nullptr);
     if (rc) return rc;

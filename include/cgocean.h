@@ -57,10 +57,13 @@ extern "C" {
 /* CG_METHOD_REFINED whose binary32 cycle takes the plan's learned coarse-grid correction
  * (cg_ocean_plan_set_coarse_net): down to the network's grid as usual, then the network's correction
  * for that grid's forcing in place of the grids below it, then up as usual; every step one captured
- * hipGraph, batched plans included. The multigrid fallback: once the residual falls by less than a
- * factor 0.8 a step (CG_OPT_NET_FALLBACK), or an answer is not a number, the solve goes on (or
- * starts again) with the plain cycle; cg_ocean_plan_net_fallbacks reports it. The stopping test and
- * the binary64 fixed point are CG_METHOD_REFINED's. Not a reference method. */
+ * hipGraph, batched plans included. The learned steps stop once the residual is under tol, or as a
+ * fallback once it falls by less than a factor 0.8 a step (cg_ocean_plan_net_fallbacks counts those).
+ * Plain multigrid then judges every answer: each instance goes on from it with the plain cycle, at
+ * least three steps and until CG_METHOD_REFINED's residual and error rule holds with half the error
+ * bound (an answer that is not
+ * a number starts the solve again, plainly). With CG_OPT_NET_FALLBACK 0 neither happens (the network
+ * measured alone). The binary64 fixed point is CG_METHOD_REFINED's. Not a reference method. */
 #define CG_METHOD_LEARNED 5
 /* CG_METHOD_FGMRES with a caller's preconditioner (cg_ocean_plan_set_preconditioner) on the first
  * inner iteration of every outer step (the outer residual itself) and the binary32 V(2,2) cycle on
@@ -90,8 +93,9 @@ extern "C" {
                                     always cycle the whole ladder (OC:740-781). */
 #define CG_OPT_STOP_PERMILLE 11  /* CG_METHOD_REFINED, FGMRES and LEARNED take tol x value / 1000 as their
                                     residual target (1..1000, default 1000) */
-#define CG_OPT_NET_FALLBACK 13   /* 1 (default): CG_METHOD_LEARNED falls back to the plain cycle as described
-                                    there; 0: it never does (to measure the network without the fallback) */
+#define CG_OPT_NET_FALLBACK 13   /* 1 (default): CG_METHOD_LEARNED falls back to the plain cycle and has its
+                                    answers judged by plain steps as described there; 0: neither (to
+                                    measure the network alone) */
 #define CG_OPT_ERROR_PICO 12     /* ... and stop only once, besides, the estimated relative L2 error of the
                                     binary64 answer against the exact solution of the operator is at most
                                     value x 1e-12 (default 100: 1e-10; 0: the residual alone decides). The
@@ -100,8 +104,8 @@ extern "C" {
                                     alone does not bound the error (a forcing of fine-scale noise leaves a
                                     smooth error whose residual is small). Each instance stops at the first
                                     step where both hold, tested on the device before every step.
-                                    CG_METHOD_LEARNED takes half the bound (its contraction varies from
-                                    step to step, so its estimate needs the margin). */
+                                    CG_METHOD_LEARNED's steps stop on the residual alone; plain steps
+                                    then judge its answers (see CG_METHOD_LEARNED). */
 
 typedef struct cg_ctx cg_ctx;
 typedef struct cg_plan cg_plan;
@@ -214,7 +218,7 @@ int cg_ocean_plan_solve_batch(cg_plan *const *plans, int count, double tol, int 
 typedef struct cg_group cg_group;
 int cg_ocean_group_create(cg_plan *const *plans, int count, cg_group **out);
 /* method: CG_METHOD_REFINED, or CG_METHOD_LEARNED (every plan with its coarse network set; the
- * fallback to the plain cycle per plan as for cg_ocean_plan_solve_all). infos: one per instance, the
+ * fallback and the plain judge of the answers per plan, as for cg_ocean_plan_solve_all). infos: one per instance, the
  * plans' instances in the order of the plans given to create. */
 int cg_ocean_group_solve(cg_group *group, double tol, int method, cg_solution *infos);
 int cg_ocean_group_destroy(cg_group *group);
