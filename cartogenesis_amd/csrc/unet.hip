@@ -11,7 +11,10 @@
 // concatenation without a copy. Channel counts are multiples of 16 and `across` a multiple of 16.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cstdint>
+#include <cstdlib>
 
 #include "cgocean.h"
 
@@ -180,133 +183,181 @@ __global__ __launch_bounds__(256) void k_conv3x3_k32(const uint16_t *__restrict_
     }
 }
 
-// The forward convolution with both operands staged in LDS: a workgroup computes 4 rows x 64
-// pixels (one row per wavefront, four 16-pixel MFMA tiles each) times 16 NB output channels. Per
-// 32-channel chunk it stages the 6 input rows those 4 rows read (66 pixels each: the tile and one
-// halo column either side, wrapped; rows beyond the grid as zeros) and the chunk's weights for all
-// nine taps, so each input value is fetched from L2 once per chunk rather than once per tap and
-// output row, and the nine taps read shifted windows of LDS. Pixel and weight rows are padded to
-// 40 elements against bank conflicts. Needs across % 64 == 0 and down % 4 == 0. With up1 the first
-// input is read through nearest upsampling from a tensor of half the height and width (the U-Net's
-// decoder input), so the upsampled tensor is never written.
+// The forward convolution with both operands staged in LDS: a workgroup computes tiles of 4 rows x
+// 64 pixels (one row per wavefront, four 16-pixel MFMA tiles each) times 16 NB output channels. Per
+// 32-channel chunk it stages the 6 input rows a tile reads (66 pixels each: the tile and one halo
+// column either side, wrapped; rows beyond the grid as zeros) and the chunk's weights for all nine
+// taps, so each input value is fetched from L2 once per chunk rather than once per tap and output
+// row, and the nine taps read shifted windows of LDS. Needs across % 64 == 0 and down % 4 == 0.
+// With up1 the first input is read through nearest upsampling from a tensor of half the height and
+// width (the U-Net's decoder input), so the upsampled tensor is never written.
+//
+// The grid holds as many workgroups as stay resident; workgroup k takes tiles k, k + grid, ... and
+// walks their (tile, chunk) items in order, loading the next item's operands into registers while
+// the current item's MFMAs and output stores run, so every workgroup keeps one item's loads in flight
+// from its first item to its last. With one chunk (32 input channels) the weights are staged once.
+//
+// LDS holds 16-byte slots of 8 channels, four a pixel (row-major over the 6 x 66 staged pixels)
+// or output channel (tap-major), the channel group q of item p in slot 4 p + (q ^ ((p >> 1) & 3)).
+// Both a fragment read (lane m + 16 g takes item base + m's group g) and a staging store (lane l
+// writes item l / 4's group l % 4) then touch every bank once for each pair of 8-lane blocks that
+// gfx950's LDS serves together (measured: with 80-byte padded pixel rows both conflicted two-way).
 constexpr int kApx = 66;
 
+__device__ inline int swz(int p, int q) { return p * 4 + (q ^ ((p >> 1) & 3)); }
+
+// 16 bytes as a plain vector (HIP's uint4 is a struct, whose copies can keep an array in scratch).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 template <int NB>
-__global__ __launch_bounds__(256) void k_conv3x3_tile(const uint16_t *__restrict__ x1, int c1,
+__global__ __launch_bounds__(256, 2) void k_conv3x3_tile(const uint16_t *__restrict__ x1, int c1,
                                                       const uint16_t *__restrict__ x2, int c2, int n, int down,
                                                       int across, const uint16_t *__restrict__ w, int cout, int relu,
                                                       uint16_t *__restrict__ y1, int o1, uint16_t *__restrict__ y2,
                                                       float *__restrict__ y32, int up1) {
-    __shared__ __attribute__((aligned(16))) uint16_t sa[6 * kApx * kSbStride];
-    __shared__ __attribute__((aligned(16))) uint16_t sb[9 * 16 * NB * kSbStride];
+    __shared__ __attribute__((aligned(16))) uint16_t sa[6 * kApx * 4 * 8];
+    __shared__ __attribute__((aligned(16))) uint16_t sb[9 * 16 * NB * 4 * 8];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col_tiles = across / 64, row_blocks = down / 4;
-    const int ct = (int)(blockIdx.x % col_tiles);
-    const long rb = blockIdx.x / col_tiles;
-    const int r0 = (int)(rb % row_blocks) * 4;
-    const long img = rb / row_blocks;
-    const int c0 = ct * 64;
+    const long tiles = (long)n * row_blocks * col_tiles;
     const int co0 = blockIdx.y * 16 * NB;
     const int cin = c1 + c2;
-    const int m = lane & 15, k8 = (lane >> 4) * 8, kq = (lane >> 4) * 4;
+    const int m = lane & 15, g = lane >> 4, kq = (lane >> 4) * 4;
+    long tile = blockIdx.x;
+    if (tile >= tiles) return;  // the whole workgroup leaves together
     f4 acc[4][NB];
+    #pragma unroll
     for (int t = 0; t < 4; ++t)
+        #pragma unroll
         for (int b = 0; b < NB; ++b) acc[t][b] = (f4){0.f, 0.f, 0.f, 0.f};
-    // The next chunk's pixel units are loaded into registers while the current chunk's MFMAs run
-    // (kAU a thread), and with at most 32 output channels a wavefront its weight units too (kBU);
-    // with 64 the weights' registers cost more occupancy than the overlap gains, so they are loaded
-    // as they are stored. Registers go to LDS between two barriers.
+    // Staging units of 16 bytes: pixel unit u is channel group u & 3 of staged pixel u >> 2 (row-major
+    // over the 6 x 66 staged pixels), weight unit e channel group e & 3 of output channel (e >> 2) %
+    // (16 NB) of tap e / (64 NB). Units are loaded into registers (kAU and kBU a thread; the weights
+    // only with at most 32 output channels a workgroup: with 64 their registers cost more occupancy
+    // than the overlap gains, so they are loaded as they are stored) and go to LDS between two barriers.
     constexpr int kAUnits = 6 * kApx * 4, kBUnits = 9 * 16 * NB * 4;
     constexpr int kAU = (kAUnits + 255) / 256, kBU = (kBUnits + 255) / 256;
     constexpr bool kPrefetchB = NB <= 2;
-    uint4 ra[kAU], rw[kPrefetchB ? kBU : 1];
-    auto load_a = [&](int ci0) {
+    u32x4 ra[kAU], rw[kPrefetchB ? kBU : 1];
+    auto load_a = [&](long tl, int ci0) {
+        const int c0 = (int)(tl % col_tiles) * 64;
+        const long rb = tl / col_tiles;
+        const int r0 = (int)(rb % row_blocks) * 4;
+        const long img = rb / row_blocks;
         const bool first = ci0 < c1;
         const uint16_t *xs = first ? x1 + ci0 : x2 + (ci0 - c1);
         const int cs = first ? c1 : c2;
+        #pragma unroll
         for (int k = 0; k < kAU; ++k) {
             const int u = threadIdx.x + 256 * k;
-            ra[k] = (uint4){0u, 0u, 0u, 0u};
+            ra[k] = (u32x4){0u, 0u, 0u, 0u};
             if (u < kAUnits) {
                 const int i = u / (kApx * 4), px = (u >> 2) % kApx, q = u & 3;
                 const int rr = r0 - 1 + i;
                 if (rr >= 0 && rr < down) {
                     const int cc = wrap(c0 - 1 + px, across);
-                    // up1: the first input is half as tall and wide, read through nearest upsampling
                     const long pix = first && up1 ? (img * (down >> 1) + (rr >> 1)) * (across >> 1) + (cc >> 1)
                                                   : (img * down + rr) * across + cc;
-                    ra[k] = *(const uint4 *)(xs + pix * cs + q * 8);
+                    ra[k] = *(const u32x4 *)(xs + pix * cs + q * 8);
                 }
             }
         }
     };
-    auto weight_unit = [&](int ci0, int e) -> const uint4 * {
+    auto weight_unit = [&](int ci0, int e) -> const u32x4 * {
         const int tap = e / (16 * NB * 4), r = e % (16 * NB * 4), co_l = r >> 2, q = r & 3;
-        return (const uint4 *)(w + ((long)tap * cout + co0 + co_l) * cin + ci0 + q * 8);
+        return (const u32x4 *)(w + ((long)tap * cout + co0 + co_l) * cin + ci0 + q * 8);
     };
-    auto weight_slot = [&](int e) -> uint4 * {
+    auto weight_slot = [&](int e) -> u32x4 * {
         const int tap = e / (16 * NB * 4), r = e % (16 * NB * 4), co_l = r >> 2, q = r & 3;
-        return (uint4 *)&sb[(tap * 16 * NB + co_l) * kSbStride + q * 8];
+        return (u32x4 *)&sb[swz(tap * 16 * NB + co_l, q) * 8];
     };
     auto load_b = [&](int ci0) {
         if constexpr (kPrefetchB)
+            #pragma unroll
             for (int k = 0; k < kBU; ++k) {
                 const int e = threadIdx.x + 256 * k;
                 if (e < kBUnits) rw[k] = *weight_unit(ci0, e);
             }
     };
-    load_a(0);
+    const bool one_chunk = cin == 32;
+    bool weights_staged = false;
+    int ci0 = 0;
+    load_a(tile, 0);
     load_b(0);
-    for (int ci0 = 0; ci0 < cin; ci0 += 32) {
-        __syncthreads();  // every wavefront is done with the previous chunk
+    for (;;) {
+        __syncthreads();  // every wavefront is done with the previous item
+        #pragma unroll
         for (int k = 0; k < kAU; ++k) {
             const int u = threadIdx.x + 256 * k;
             if (u < kAUnits) {
-                const int i = u / (kApx * 4), px = (u >> 2) % kApx, q = u & 3;
-                *(uint4 *)&sa[(i * kApx + px) * kSbStride + q * 8] = ra[k];
+                *(u32x4 *)&sa[swz(u >> 2, u & 3) * 8] = ra[k];
             }
         }
-        if constexpr (kPrefetchB) {
-            for (int k = 0; k < kBU; ++k) {
-                const int e = threadIdx.x + 256 * k;
-                if (e < kBUnits) *weight_slot(e) = rw[k];
+        if (!weights_staged) {
+            if constexpr (kPrefetchB) {
+                #pragma unroll
+                for (int k = 0; k < kBU; ++k) {
+                    const int e = threadIdx.x + 256 * k;
+                    if (e < kBUnits) *weight_slot(e) = rw[k];
+                }
+            } else {
+                for (int e = threadIdx.x; e < kBUnits; e += 256) *weight_slot(e) = *weight_unit(ci0, e);
             }
-        } else {
-            for (int e = threadIdx.x; e < kBUnits; e += 256) *weight_slot(e) = *weight_unit(ci0, e);
+            weights_staged = one_chunk;
         }
         __syncthreads();
-        if (ci0 + 32 < cin) {
-            load_a(ci0 + 32);
-            load_b(ci0 + 32);
+        // The next item: the next chunk of this tile, else this workgroup's next tile from chunk 0.
+        const bool last_chunk = ci0 + 32 == cin;
+        const long next_tile = last_chunk ? tile + gridDim.x : tile;
+        const int next_ci = last_chunk ? 0 : ci0 + 32;
+        const bool more = next_tile < tiles;
+        if (more) {
+            load_a(next_tile, next_ci);
+            if (!one_chunk) load_b(next_ci);
         }
+        #pragma unroll
         for (int tap = 0; tap < 9; ++tap) {
             const int i = wave + tap / 3;  // staged row of input row r0 + wave + dr, dr = tap / 3 - 1
             const int dc = tap % 3;        // staged pixel offset: px = 16 t + m + 1 + (tap % 3 - 1)
             b8 a[4], bv[NB];
-            for (int t = 0; t < 4; ++t) a[t] = *(const b8 *)&sa[(i * kApx + 16 * t + m + dc) * kSbStride + k8];
-            for (int b = 0; b < NB; ++b)
-                bv[b] = *(const b8 *)&sb[(tap * 16 * NB + b * 16 + m) * kSbStride + k8];
+            #pragma unroll
+            for (int t = 0; t < 4; ++t) a[t] = *(const b8 *)&sa[swz(i * kApx + 16 * t + m + dc, g) * 8];
+            #pragma unroll
+            for (int b = 0; b < NB; ++b) bv[b] = *(const b8 *)&sb[swz(tap * 16 * NB + b * 16 + m, g) * 8];
+            #pragma unroll
             for (int t = 0; t < 4; ++t)
+                #pragma unroll
                 for (int b = 0; b < NB; ++b)
                     acc[t][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], bv[b], acc[t][b], 0, 0, 0);
         }
-    }
-    const long prow = (img * down + r0 + wave) * across + c0;
-    for (int t = 0; t < 4; ++t) {
-        for (int b = 0; b < NB; ++b) {
-            const int co = co0 + b * 16 + m;
-            for (int j = 0; j < 4; ++j) {
-                float v = acc[t][b][j];
-                if (relu) v = v > 0.f ? v : 0.f;
-                const long p = prow + 16 * t + kq + j;
-                if (y32)
-                    y32[p * cout + co] = v;
-                else if (co < o1)
-                    y1[p * o1 + co] = bf16_of(v);
-                else
-                    y2[p * (cout - o1) + (co - o1)] = bf16_of(v);
+        if (last_chunk) {
+            const int c0 = (int)(tile % col_tiles) * 64;
+            const long rb = tile / col_tiles;
+            const long prow = ((rb / row_blocks) * down + (rb % row_blocks) * 4 + wave) * across + c0;
+            #pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                #pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    const int co = co0 + b * 16 + m;
+                    #pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        float v = acc[t][b][j];
+                        if (relu) v = v > 0.f ? v : 0.f;
+                        const long p = prow + 16 * t + kq + j;
+                        if (y32)
+                            y32[p * cout + co] = v;
+                        else if (co < o1)
+                            y1[p * o1 + co] = bf16_of(v);
+                        else
+                            y2[p * (cout - o1) + (co - o1)] = bf16_of(v);
+                    }
+                    acc[t][b] = (f4){0.f, 0.f, 0.f, 0.f};
+                }
             }
         }
+        if (!more) break;
+        tile = next_tile;
+        ci0 = next_ci;
     }
 }
 
@@ -658,33 +709,67 @@ void conv_k32_launch(const uint16_t *x1, int c1, const uint16_t *x2, int c2, int
                            relu, y1, o1, y2, y32);
 }
 
+// What the current device holds of k_conv3x3_tile<nb>: its CU count and the workgroups it keeps
+// resident at once (the staged kernel's grid, each workgroup walking tiles until none is left).
+struct TileFit {
+    int cus;
+    long resident;
+};
+TileFit tile_fit(int nb) {
+    static std::atomic<long> cached[64][3];
+    static std::atomic<int> cached_cus[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    const int v = nb == 4 ? 0 : (nb == 2 ? 1 : 2);
+    long have = cached[dev][v].load(std::memory_order_relaxed);
+    if (have > 0) return {cached_cus[dev].load(std::memory_order_relaxed), have};
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    hipError_t e = nb == 4   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_conv3x3_tile<4>, 256, 0)
+                   : nb == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_conv3x3_tile<2>, 256, 0)
+                             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_conv3x3_tile<1>, 256, 0);
+    if (e != hipSuccess || per_cu <= 0) per_cu = 1;
+    have = (long)cus * per_cu;
+    cached_cus[dev].store(cus, std::memory_order_relaxed);
+    cached[dev][v].store(have, std::memory_order_relaxed);
+    return {cus, have};
+}
+
 bool tile_ok(int c1, int c2, int down, int across) {
     return c1 % 32 == 0 && c2 % 32 == 0 && across % 64 == 0 && down % 4 == 0;
 }
 
-// Images of fewer pixels than this take the one-row kernel with 16-pixel tiles (more wavefronts)
-// rather than the LDS-staged 4 x 64-pixel tiles, except where the input is read through upsampling.
+// Images of fewer pixels than this take the one-row kernel with 16-pixel tiles rather than 64.
 constexpr long kSmallImage = 65536;
 
 template <bool ADJ>
 int conv_launch(const uint16_t *x1, int c1, const uint16_t *x2, int c2, int n, int down, int across,
                 const uint16_t *w, int cout, int relu, uint16_t *y1, int o1, uint16_t *y2, float *y32, hipStream_t s,
                 int up1 = 0) {
-    const bool small = (long)n * down * across < kSmallImage && !up1;
-    if (!ADJ && tile_ok(c1, c2, down, across) && !small) {
+    // The staged kernel unless it would have fewer tiles (times output-channel blocks) than the device
+    // has CUs: such layers take the one-row kernels below (more, shorter wavefronts), except where
+    // the input is read through upsampling, which only the staged kernel does.
+    if (!ADJ && tile_ok(c1, c2, down, across)) {
         const int nb = cout % 64 == 0 ? 4 : (cout % 32 == 0 ? 2 : 1);
-        const dim3 grid((unsigned)((long)n * (down / 4) * (across / 64)), cout / (16 * nb));
-        if (nb == 4)
-            hipLaunchKernelGGL((k_conv3x3_tile<4>), grid, dim3(256), 0, s, x1, c1, x2, c2, n, down, across, w, cout,
-                               relu, y1, o1, y2, y32, up1);
-        else if (nb == 2)
-            hipLaunchKernelGGL((k_conv3x3_tile<2>), grid, dim3(256), 0, s, x1, c1, x2, c2, n, down, across, w, cout,
-                               relu, y1, o1, y2, y32, up1);
-        else
-            hipLaunchKernelGGL((k_conv3x3_tile<1>), grid, dim3(256), 0, s, x1, c1, x2, c2, n, down, across, w, cout,
-                               relu, y1, o1, y2, y32, up1);
-        return hipGetLastError() == hipSuccess ? CG_OK : CG_ERR_DEVICE;
+        const long tiles = (long)n * (down / 4) * (across / 64);
+        const int ys = cout / (16 * nb);
+        const TileFit fit = tile_fit(nb);
+        if (up1 || tiles * ys >= fit.cus) {
+            const long per_y = std::max<long>(1, (fit.resident + ys - 1) / ys);
+            const dim3 grid((unsigned)std::min<long>(tiles, per_y), ys);
+            if (nb == 4)
+                hipLaunchKernelGGL((k_conv3x3_tile<4>), grid, dim3(256), 0, s, x1, c1, x2, c2, n, down, across, w,
+                                   cout, relu, y1, o1, y2, y32, up1);
+            else if (nb == 2)
+                hipLaunchKernelGGL((k_conv3x3_tile<2>), grid, dim3(256), 0, s, x1, c1, x2, c2, n, down, across, w,
+                                   cout, relu, y1, o1, y2, y32, up1);
+            else
+                hipLaunchKernelGGL((k_conv3x3_tile<1>), grid, dim3(256), 0, s, x1, c1, x2, c2, n, down, across, w,
+                                   cout, relu, y1, o1, y2, y32, up1);
+            return hipGetLastError() == hipSuccess ? CG_OK : CG_ERR_DEVICE;
+        }
     }
+    const bool small = (long)n * down * across < kSmallImage;
     if (!ADJ && c1 % 32 == 0 && c2 % 32 == 0) {
         if (across % 64 == 0 && !small)
             conv_k32_launch<4>(x1, c1, x2, c2, n, down, across, w, cout, relu, y1, o1, y2, y32, s);

@@ -42,9 +42,9 @@ def rand_bf16(*shape, seed=0, scale=1.0):
                                                  (32, 64, 96, 0, 48, 8), (32, 32, 64, 1, 128, 8),
                                                  (64, 0, 32, 0, 192, 8), (32, 32, 64, 1, 128, 6)])
 def test_conv_matches_cpu(c1, c2, cout, relu, a, d):
-    """Channel counts of 16 take the K = 16 kernel; multiples of 32 the LDS-staged one where across
-    % 64 == 0 and down % 4 == 0 (4 rows a workgroup), else the K = 32 one reading L1/L2 (one 16-pixel
-    tile per wavefront below 64 columns, four from 64)."""
+    """Channel counts of 16 take the K = 16 kernel; multiples of 32 the K = 32 one reading L1/L2 (one
+    16-pixel tile per wavefront below 64 columns, four from 64): these images have too few 4 x 64
+    tiles for the LDS-staged kernel, which the next test covers."""
     n = 2
     x1, x2 = rand_bf16(n, d, a, c1, seed=1), rand_bf16(n, d, a, c2, seed=2) if c2 else None
     w = rand_bf16(9, cout, c1 + c2, seed=3, scale=0.1)
@@ -63,6 +63,39 @@ def test_conv_matches_cpu(c1, c2, cout, relu, a, d):
     yb = torch.empty(n, d, a, cout - o1, dtype=torch.bfloat16, device="cuda") if cout > o1 else None
     check(lib().cg_conv3x3(_p(x1d), c1, _p(x2d), c2, n, d, a, _p(wd), cout, relu, _p(ya), o1, _p(yb), None, None),
           what="cg_conv3x3 bf16")
+    both = torch.cat([ya.cpu(), yb.cpu()], -1) if yb is not None else ya.cpu()
+    assert torch.equal(both, got.to(torch.bfloat16))
+
+
+@pytest.mark.parametrize("c1,c2,cout,up1,n", [(64, 32, 32, 0, 4), (32, 0, 16, 0, 4), (64, 32, 128, 1, 2),
+                                               (32, 0, 64, 0, 2)])
+def test_staged_conv_walks_many_tiles_per_workgroup(c1, c2, cout, up1, n):
+    """Layers with at least as many 4 x 64 tiles (times output-channel blocks) as the device has CUs
+    take the LDS-staged kernel, whose grid holds only the workgroups that stay resident: at 64 x 1024
+    with n images each workgroup walks several tiles, several 32-channel chunks per tile (96 input
+    channels: the first input's two chunks, then the skip input's), or one chunk whose weights it
+    stages once (32 input channels), at 16, 32, 64 and 128 output channels (one or two channel
+    blocks), and with up1 reads the first input through nearest upsampling. Against the CPU
+    convolution; the bfloat16 outputs split over two tensors round the binary32 ones."""
+    d, a = 64, 1024
+    x1 = rand_bf16(n, d >> up1, a >> up1, c1, seed=21)
+    x2 = rand_bf16(n, d, a, c2, seed=22) if c2 else None
+    w = rand_bf16(9, cout, c1 + c2, seed=23, scale=0.1)
+    x1d, x2d, wd = x1.cuda(), x2.cuda() if c2 else None, w.cuda()
+    conv = lib().cg_conv3x3_up if up1 else lib().cg_conv3x3
+    y = torch.empty(n, d, a, cout, device="cuda")
+    print(f"OCEAN staged conv {c1}+{c2}->{cout} up1={up1} n={n}", flush=True)
+    check(conv(_p(x1d), c1, _p(x2d), c2, n, d, a, _p(wd), cout, 1, None, 0, None, _p(y), None), what="staged conv")
+    x1f = x1.float().repeat_interleave(2, 1).repeat_interleave(2, 2) if up1 else x1.float()
+    xin = torch.cat([x1f, x2.float()], -1) if c2 else x1f
+    ref = cpu_conv(xin, w.float(), True)
+    got = y.cpu()
+    assert (got - ref).abs().max() <= 1e-5 * ref.abs().max()
+    o1 = 16
+    ya = torch.empty(n, d, a, o1, dtype=torch.bfloat16, device="cuda")
+    yb = torch.empty(n, d, a, cout - o1, dtype=torch.bfloat16, device="cuda") if cout > o1 else None
+    check(conv(_p(x1d), c1, _p(x2d), c2, n, d, a, _p(wd), cout, 1, _p(ya), o1, _p(yb), None, None),
+          what="staged conv bf16")
     both = torch.cat([ya.cpu(), yb.cpu()], -1) if yb is not None else ya.cpu()
     assert torch.equal(both, got.to(torch.bfloat16))
 
