@@ -124,6 +124,7 @@ def cpu_baseline(levels, levels256, budget_s: float, threads: int, nproc: int, m
 def batched_year(acc, prob, tol: float, reps: int, warmup: int, autotune: bool = True):
     """The 12 monthly forcings of the benchmark world solved together in one plan (one launch per
     kernel for all twelve), from zero to tol each: solves/s = 12 x reps / wall time."""
+    import cartogenesis_amd as cg
     from cartogenesis_amd.ocean import BatchPlan
     from cartogenesis_amd.problem import monthly_forcings
 
@@ -140,10 +141,25 @@ def batched_year(acc, prob, tol: float, reps: int, warmup: int, autotune: bool =
         for _ in range(reps):
             infos = plan.solve_all(tol)
         elapsed = time.perf_counter() - t0
+        # The roofline kernel as the batched steps launch it: the finest up leg with the outer step
+        # fused in, all twelve instances in one launch, 200 launches between HIP events on the
+        # library's stream.
+        us_batched = cg.lib().cg_ocean_plan_time_kernel(plan.handle, 6, 200)
     finally:
         plan.close()
     value = len(forcings) * reps / elapsed
-    log(f"bench: batched year {value:.1f} solves/s, {1000 * elapsed / reps:.2f} ms per year")
+    log(f"bench: batched year {value:.1f} solves/s, {1000 * elapsed / reps:.2f} ms per year, fused kernel "
+        f"{us_batched:.2f} us for {len(forcings)} instances")
+    rows0 = tuned[0] if tuned and tuned[0] else 8
+    roof = None
+    if us_batched > 0:
+        gbs = len(forcings) * FUSED_BYTES / (us_batched * 1e-6) / 1e9
+        roof = {"bound": "hbm",
+                "kernel": f"k_relax_tile<4,{rows0},false,3> over {len(forcings)} instances (grid y), as the batched "
+                          "steps launch it",
+                "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                "avg_launch_us": round(us_batched, 3),
+                "algorithmic_bytes_per_launch": len(forcings) * FUSED_BYTES}
     return {
         "workload": "12 monthly forcings (band amplitude 0.1 cos(2 pi m / 12), shifted 10 deg sin(2 pi m / 12)) of "
                     "the 1024x2048 world, solved together in one plan from zero to tol each",
@@ -153,6 +169,7 @@ def batched_year(acc, prob, tol: float, reps: int, warmup: int, autotune: bool =
         "cycles": [int(i.cycles) for i in infos],
         "max_relative_residual": max(float(i.relative_residual) for i in infos),
         "autotuned_tile_rows": tuned,
+        "roofline": roof,
     }
 
 
@@ -728,6 +745,7 @@ def main() -> int:
     combined = batched_combined(acc, args.tol) if args.combined else None
 
     err = None
+    vs_ref = None
     cpu = None
     if not args.no_oracle_check or not args.no_cpu:
         import oracle
@@ -739,6 +757,17 @@ def main() -> int:
             log("bench: binary64 oracle to 1e-12 for the error report")
             ref, _, _ = oracle.solve64(ladder, 1e-12)
             err = float(np.linalg.norm(x64 - ref) / np.linalg.norm(ref))
+            # The reference method itself (OceanCirculation.solve: binary32 values, binary64 residual)
+            # on the CPU from zero to the same tolerance, capped at its 200 cycles: the answer a
+            # caller of the reference gets on these inputs, and how far this solve is from it.
+            log("bench: the reference method on the CPU to the same tolerance")
+            t1 = time.perf_counter()
+            refm, rcyc, rres = oracle.solve(ladder, np.zeros(ladder[0].cells, np.float32), args.tol)
+            vs_ref = {"rel_l2": float(np.linalg.norm(x64 - refm) / np.linalg.norm(refm)),
+                      "reference_cycles": rcyc, "reference_relative_residual": rres,
+                      "reference_rel_l2_vs_fp64_oracle": float(np.linalg.norm(refm - ref) / np.linalg.norm(ref)),
+                      "seconds": round(time.perf_counter() - t1, 2)}
+            log(f"bench: vs the reference method {vs_ref}")
         if not args.no_cpu:
             log(f"bench: CPU baseline ({threads} of {nproc} cores, {model})")
             cpu = cpu_baseline(prob.levels, prob256.levels, args.cpu_budget, threads, nproc, model)
@@ -768,6 +797,7 @@ def main() -> int:
             "cycles_per_solve": float(np.mean(cycles)),
             "relative_residual": rel_res,
             "rel_l2_vs_fp64_oracle": err,
+            "vs_cpu_reference_method": vs_ref,
         },
         "roofline": ({
             "bound": "hbm",
@@ -821,6 +851,10 @@ def main() -> int:
         out["learned_vs_plain_mg"] = learned.get("learned_vs_plain_mg")
     if batched:
         out["batched"] = batched
+        # The same roofline kernel at the batched year's shape (twelve instances a launch), beside the
+        # single-instance figure above.
+        if batched.get("roofline"):
+            out["roofline_batched"] = batched["roofline"]
     if worlds:
         out["batched_worlds"] = worlds
     if combined:

@@ -3231,7 +3231,7 @@ int cg_ocean_solve(cg_ctx *ctx, const cg_stencil *levels, int nlevels, const flo
 }
 
 double cg_ocean_plan_time_kernel(cg_plan *p, int kind, int reps) {
-    if (!p || reps < 1 || kind < 0 || (kind > 5 && kind < 16) || kind >= 16 + 2 * kMaxLevels) return -1.0;
+    if (!p || reps < 1 || kind < 0 || (kind > 6 && kind < 16) || kind >= 16 + 2 * kMaxLevels) return -1.0;
     cg_ctx *ctx = p->ctx;
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     if (hipSetDevice(ctx->device) != hipSuccess) return -1.0;
@@ -3261,7 +3261,9 @@ double cg_ocean_plan_time_kernel(cg_plan *p, int kind, int reps) {
         return ok ? 1000.0 * ms / reps : -1.0;
     }
     const bool with_sums = kind == 5;
-    if (with_sums) kind = 3;
+    // Kind 6: kind 3 over every instance of the plan, as a batched step launches it.
+    const int instances = kind == 6 ? p->K : 1;
+    if (with_sums || kind == 6) kind = 3;
     if (kind == 3 && !outer_fused(p)) {
         set_err(ctx, "the plan's steps do not fuse the outer step");
         return -1.0;
@@ -3315,8 +3317,17 @@ double cg_ocean_plan_time_kernel(cg_plan *p, int kind, int reps) {
                 ex.sums.out = sums_group(p, 2 * p->K);
                 ex.sums.clear = sums_group(p, 2 * p->K + 1);
             }
+            if (instances > 1) {
+                // Every instance's maxima into its own pair of the plan's residual slot groups
+                // (parity q out, 1 - q cleared: 2 K kMaxSlots in all); no active flags, so every
+                // instance runs.
+                ex.fs = vstride(L.st);
+                ex.cs = vstride(C.st);
+                ex.red.out = p->d_res + q * kMaxSlots;
+                ex.red.clear = p->d_res + (1 - q) * kMaxSlots;
+            }
             return tile_dispatch(ctx, L.st->args(), level0_forcing(p, q), L.x[L.cur], L.x[1 - L.cur], nh, false,
-                                 TILE_UP_OUTER, ex, th);
+                                 TILE_UP_OUTER, ex, th, instances);
         }
         return tile_dispatch(ctx, L.st->args(), L.f, L.x[L.cur], L.x[1 - L.cur], nh, false, TILE_UP, ex, th);
     };

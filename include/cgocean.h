@@ -246,12 +246,11 @@ double cg_ocean_kernel_time_us(cg_ctx *ctx, int kind, long *launches);
  * grid's fused down leg, 1 its fused up leg, 2 the binary64 outer residual, 3 the finest up leg
  * with the outer step fused in (as most steps run it: without the error estimate's sums), 4 the
  * bottom of the cycle (the dense-inverse launch with CG_OPT_DIRECT_BOTTOM, else the one-workgroup
- * cycle of the coarsest grids), 5 kind 3 with the sums (as a solve's last steps run it), 16 + 2 l
- * grid l's down leg and 17 + 2 l its up leg (without the outer step), for grids above the bottom.
- * Each launch reads the
- * plan's current buffers and writes the same
- * outputs (never the answer), so repeats leave the plan as one launch does; call it between
- * solves, not during one. */
+ * cycle of the coarsest grids), 5 kind 3 with the sums (as a solve's last steps run it), 6 kind 3
+ * over every instance of a batched plan in one launch (each instance active), 16 + 2 l grid l's
+ * down leg and 17 + 2 l its up leg (without the outer step), for grids above the bottom. Each
+ * launch reads the plan's current buffers and writes the same outputs (never the answer), so
+ * repeats leave the plan as one launch does; call it between solves, not during one. */
 double cg_ocean_plan_time_kernel(cg_plan *plan, int kind, int reps);
 
 /* ---- the round's input and output steps, on the device ------------------------------------- */

@@ -169,3 +169,32 @@ def test_batched_months_repeat_their_single_solves_to_the_bit(acc, rows0):
                 assert np.array_equal(batch.read(i)[0], ref[i][1]), (rep, i)
         finally:
             batch.close()
+
+
+def test_kernel_timer_leaves_the_answers_and_the_next_solve_unchanged(acc):
+    """cg_ocean_plan_time_kernel on a batched plan, the batched kind (6: the fused finest up leg and
+    outer step over every instance in one launch) included: positive times, the answers read after
+    timing are those read before to the bit, and a solve after timing repeats the one before."""
+    import cartogenesis_amd as cg
+    p = circulation_problem(256, seed=3)
+    forcings = monthly_forcings(p, months=range(3))
+    batch = BatchPlan(acc, p.levels, 3)
+    try:
+        for i, f in enumerate(forcings):
+            batch.set_forcing(i, f)
+        print("OCEAN batch timer 512x256 x3", flush=True)
+        infos = batch.solve_all(1e-8)
+        before = [batch.read(i)[0].copy() for i in range(3)]
+        lib = cg.lib()
+        for kind in (0, 3, 5, 6):
+            us = lib.cg_ocean_plan_time_kernel(batch.handle, kind, 5)
+            assert us > 0, (kind, us)
+        assert lib.cg_ocean_plan_time_kernel(batch.handle, 7, 5) < 0  # no such kind
+        for i in range(3):
+            assert np.array_equal(batch.read(i)[0].view(np.uint64), before[i].view(np.uint64))
+        again = batch.solve_all(1e-8)
+        for i in range(3):
+            assert again[i].cycles == infos[i].cycles
+            assert np.array_equal(batch.read(i)[0].view(np.uint64), before[i].view(np.uint64))
+    finally:
+        batch.close()
