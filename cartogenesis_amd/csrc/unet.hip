@@ -514,6 +514,58 @@ __global__ __launch_bounds__(256) void k_conv3x3_in3(const uint16_t *__restrict_
     }
 }
 
+// The first layer on the matrix cores, for output channel counts that are multiples of 16: one
+// wavefront computes 16 consecutive pixels (of the flattened image, row, column order) times every
+// output channel as D = W^T X on v_mfma_f32_16x16x32_bf16, with K = 4 tap + channel over taps 0..8
+// and channels 0..3 (channel 3 zeroed in both operands, so what the input holds there never enters),
+// in two K steps of 32: taps 0..7, then tap 8 and zeros. Lane m + 16 g feeds pixel m's taps 2g and
+// 2g + 1 (step 0) and tap 8 + 2g (step 1, beyond tap 8 zero), one 8-byte load each, and the weights of
+// output channel m of each 16-channel block at the same K; it holds output channels 4g .. 4g + 3 of
+// pixel m, stored as one 8-byte write. Sums are binary32, rounded once to bfloat16.
+__global__ __launch_bounds__(256) void k_conv3x3_in3_mfma(const uint16_t *__restrict__ x, long npix, int down,
+                                                          int across, const uint16_t *__restrict__ w, int cout,
+                                                          int relu, uint16_t *__restrict__ y) {
+    const int lane = threadIdx.x & 63;
+    const int m = lane & 15, g = lane >> 4;
+    const long p0 = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+    if (p0 >= npix) return;
+    const long p = p0 + m;
+    const bool pin = p < npix;
+    const long row_id = pin ? p / across : 0;
+    const int col = pin ? (int)(p - row_id * across) : 0;
+    const int row = (int)(row_id % down);
+    const long img = row_id / down;
+    auto tap_in = [&](int t) -> uint2 {
+        uint2 v = {0u, 0u};
+        const int rr = row + t / 3 - 1;
+        if (pin && t < 9 && rr >= 0 && rr < down)
+            v = *(const uint2 *)(x + ((img * down + rr) * across + wrap(col + t % 3 - 1, across)) * 16);
+        v.y &= 0xFFFFu;
+        return v;
+    };
+    const uint2 i0 = tap_in(2 * g), i1 = tap_in(2 * g + 1), i2 = tap_in(8 + 2 * g);
+    const b8 bx0 = __builtin_bit_cast(b8, (u32x4){i0.x, i0.y, i1.x, i1.y});
+    const b8 bx1 = __builtin_bit_cast(b8, (u32x4){i2.x, i2.y, 0u, 0u});
+    for (int ob = 0; ob < cout; ob += 16) {
+        const int o = ob + m;
+        auto tap_w = [&](int t) -> uint2 {
+            uint2 v = {0u, 0u};
+            if (t < 9) v = *(const uint2 *)(w + ((long)t * cout + o) * 16);
+            v.y &= 0xFFFFu;
+            return v;
+        };
+        const uint2 w0 = tap_w(2 * g), w1 = tap_w(2 * g + 1), w2 = tap_w(8 + 2 * g);
+        const b8 aw0 = __builtin_bit_cast(b8, (u32x4){w0.x, w0.y, w1.x, w1.y});
+        const b8 aw1 = __builtin_bit_cast(b8, (u32x4){w2.x, w2.y, 0u, 0u});
+        f4 acc = (f4){0.f, 0.f, 0.f, 0.f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aw0, bx0, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aw1, bx1, acc, 0, 0, 0);
+        float v[4];
+        for (int j = 0; j < 4; ++j) v[j] = relu && acc[j] < 0.f ? 0.f : acc[j];
+        if (pin) *(uint2 *)(y + p * cout + ob + 4 * g) = (uint2){pack2(v[0], v[1]), pack2(v[2], v[3])};
+    }
+}
+
 // dw[i] = sum over slices s of part[s][i], in slice order (deterministic).
 __global__ void k_wgrad_sum(const float *__restrict__ part, int slices, long count, float *__restrict__ dw) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -813,8 +865,12 @@ int cg_conv3x3_in3(const uint16_t *x, int n, int down, int across, const uint16_
     // at a time.
     if (reinterpret_cast<uintptr_t>(x) % 8 || reinterpret_cast<uintptr_t>(y) % 16) return CG_ERR_ARG;
     const long pixels = (long)n * down * across;
-    hipLaunchKernelGGL(k_conv3x3_in3, dim3(blocks(pixels, 256)), dim3(256), 0, (hipStream_t)stream, x, n, down, across,
-                       w, cout, relu, y);
+    if (cout % 16 == 0 && reinterpret_cast<uintptr_t>(w) % 8 == 0)  // matrix cores: 16 pixels a wavefront
+        hipLaunchKernelGGL(k_conv3x3_in3_mfma, dim3(blocks(pixels, 64)), dim3(256), 0, (hipStream_t)stream, x, pixels,
+                           down, across, w, cout, relu, y);
+    else
+        hipLaunchKernelGGL(k_conv3x3_in3, dim3(blocks(pixels, 256)), dim3(256), 0, (hipStream_t)stream, x, n, down,
+                           across, w, cout, relu, y);
     return hipGetLastError() == hipSuccess ? CG_OK : CG_ERR_DEVICE;
 }
 

@@ -100,19 +100,26 @@ def test_staged_conv_walks_many_tiles_per_workgroup(c1, c2, cout, up1, n):
     assert torch.equal(both, got.to(torch.bfloat16))
 
 
-@pytest.mark.parametrize("cout,relu", [(32, 1), (16, 0)])
-def test_first_layer_kernel_matches_cpu(cout, relu):
-    """cg_conv3x3_in3 (channels 0..2 of a 16-channel input; the others zero) against the CPU
-    convolution: binary32 sums of 27 products, rounded once to bfloat16."""
+@pytest.mark.parametrize("cout,relu,junk", [(32, 1, False), (16, 0, False), (24, 1, False), (32, 1, True)])
+def test_first_layer_kernel_matches_cpu(cout, relu, junk):
+    """cg_conv3x3_in3 (channels 0..2 of a 16-channel input) against the CPU convolution: binary32
+    sums of 27 products, rounded once to bfloat16. Output channel counts that are multiples of 16
+    take the matrix-core kernel (16 pixels a wavefront; 40 columns, so tiles cross rows), 24 the
+    vector one. With junk, channels 3..15 hold large values and NaN, which must not reach the output."""
     n, d, a = 2, 8, 40
     x = torch.zeros(n, d, a, 16)
+    if junk:
+        x[..., 3:] = 1e30
+        x[..., 3] = float("nan")
     x[..., :3] = torch.randn(n, d, a, 3, generator=torch.Generator().manual_seed(15))
     x = x.to(torch.bfloat16)
     w = rand_bf16(9, cout, 16, seed=16, scale=0.2)
     xd, wd = x.cuda(), w.cuda()
     y = torch.empty(n, d, a, cout, dtype=torch.bfloat16, device="cuda")
     check(lib().cg_conv3x3_in3(_p(xd), n, d, a, _p(wd), cout, relu, _p(y), None), what="cg_conv3x3_in3")
-    ref = cpu_conv(x.float(), w.float(), bool(relu))
+    xr = x.float()
+    xr[..., 3:] = 0.0
+    ref = cpu_conv(xr, w.float(), bool(relu))
     got = y.cpu().float()
     assert ((got - ref).abs() <= 2.0 ** -8 * ref.abs() + 1e-6 * ref.abs().max()).all()
     p = ctypes.c_void_p(16)
