@@ -47,6 +47,7 @@ FUSED_BYTES = 34 * CELLS
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE and WRITE_SIZE (scaled by a 1 GiB copy),
 # for the default settings (two passes per grid), by the finest grid's tile rows; see
 # profiles/hbm_traffic_*.md.
+COPY_BYTES = 1 << 30
 MEASURED_TRAFFIC = {"outer": 82793991, "up": {4: 30604038, 6: 30719584, 8: 30631686},
                     "fused": {4: 73863905, 6: 73804771, 8: 73819458}}
 
@@ -507,6 +508,99 @@ def conv_roofline(net) -> dict:
             "avg_launch_us": round(us, 2), "flops_per_launch": flops, "algorithmic_bytes_per_launch": algo}
 
 
+def traffic_probe(rows: str) -> int:
+    """The short program the live traffic reading profiles: three 1 GiB device-to-device copies (the
+    reference of known size) and, on the 1024x2048 plan with the given tile heights after one solve,
+    six launches of the fused finest up leg with the outer step (kind 3 of the kernel timer)."""
+    import ctypes
+
+    import cartogenesis_amd as cg
+    from cartogenesis_amd.ocean import HipOcean, Plan
+    from cartogenesis_amd.problem import circulation_problem
+
+    acc = HipOcean.create_or_null(0).accelerator
+    if acc is None:
+        return 1
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    n = COPY_BYTES
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)):
+        return 1
+    hip.hipMemset(a, 0, ctypes.c_size_t(n))
+    for _ in range(3):
+        hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    prob = circulation_problem(ROWS, COLS, seed=0, land_fraction=0.3)
+    plan = Plan(acc, prob.levels)
+    tuned = [int(v) for v in rows.split(",")]
+    plan.set_tile_rows(tuned + [0] * (len(prob.levels) - len(tuned)))
+    plan.solve(1e-8)
+    us = cg.lib().cg_ocean_plan_time_kernel(plan.handle, 3, 5)
+    plan.close()
+    acc.close()
+    return 0 if us > 0 else 1
+
+
+def pmc_dispatches(csv_dir: str, counter: str):
+    """{kernel name: [counter value of each dispatch]} from rocprofv3's counter CSV under csv_dir."""
+    import csv
+    import glob
+
+    per = {}
+    for path in glob.glob(os.path.join(csv_dir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as fh:
+            for r in csv.DictReader(fh):
+                if r.get("Counter_Name") != counter:
+                    continue
+                per.setdefault(r["Kernel_Name"], {}).setdefault(r["Dispatch_Id"], 0.0)
+                per[r["Kernel_Name"]][r["Dispatch_Id"]] += float(r["Counter_Value"])
+    return {k: list(v.values()) for k, v in per.items()}
+
+
+def live_traffic(rows0: int, rows: str):
+    """HBM bytes per launch of the roofline kernel, read in this run: two rocprofv3 passes over
+    `--traffic-probe` (FETCH_SIZE, then WRITE_SIZE: one counter a pass), each counter scaled by what it
+    reads for a 1 GiB copy in the same pass. Returns (read, written, note); None where it could not be
+    read (no profiler, a pass that failed or timed out)."""
+    import shutil
+    import signal
+    import statistics
+    import tempfile
+
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None, None, "rocprofv3 not found"
+    name = f"k_relax_tile<4, {rows0}, false, 3>"
+    out = {}
+    env = dict(os.environ, TMPDIR="/tmp")
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="cg_pmc_", dir="/tmp")
+        cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable,
+               os.path.abspath(__file__), "--traffic-probe", "--rows", rows]
+        log(f"bench: traffic pass {counter}")
+        proc = subprocess.Popen(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, cwd="/tmp", env=env,
+                                start_new_session=True)
+        try:
+            rc = proc.wait(timeout=240)
+        except subprocess.TimeoutExpired:
+            os.killpg(proc.pid, signal.SIGKILL)
+            proc.wait()
+            return None, None, f"the {counter} pass timed out"
+        disp = pmc_dispatches(d, counter)
+        shutil.rmtree(d, ignore_errors=True)
+        # The three 1 GiB copies are the largest copy dispatches (the plan's uploads copy small
+        # buffers through the same kernels); the fused kernel's dispatches all move the same bytes.
+        copies = sorted((v for k, vals in disp.items() if "copyBuffer" in k for v in vals), reverse=True)[:3]
+        fused = [v for k, vals in disp.items() if name in k for v in vals]
+        if rc != 0 or len(copies) < 3 or not fused or copies[-1] <= 0:
+            return None, None, f"the {counter} pass gave no reading (status {rc})"
+        out[counter] = statistics.median(fused) / statistics.median(copies) * COPY_BYTES
+    return out["FETCH_SIZE"], out["WRITE_SIZE"], ("rocprofv3 FETCH_SIZE and WRITE_SIZE of this run's probe, each "
+                                                  "scaled by its reading for a 1 GiB device-to-device copy")
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -524,6 +618,11 @@ def main() -> int:
     ap.add_argument("--no-oracle-check", action="store_true")
     ap.add_argument("--method", choices=["refined", "fgmres"], default="refined")
     ap.add_argument("--restart", type=int, default=8, help="FGMRES restart length")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only the short program the live traffic reading profiles (three 1 GiB copies, six "
+                         "launches of the fused finest kernel at the --rows tile heights) and exit")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="report the committed traffic reading instead of profiling this run's kernel")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy six times (for scaling HBM counters) and exit")
     ap.add_argument("--level-passes", default="22222222",
@@ -554,6 +653,9 @@ def main() -> int:
         print("bench.py measures one GPU only: the circulation solve is a single-GPU path here "
               "(instances would shard one per GPU with no exchange); run with --gpus 1", file=sys.stderr)
         return 2
+
+    if args.traffic_probe:
+        return traffic_probe(args.rows or "6,6,4,4,4")
 
     if args.copy_calibration:
         # A plain copy of known size, far beyond L2 and the infinity cache: the reference point
@@ -701,6 +803,14 @@ def main() -> int:
         (f", up leg with the outer step fused {us_fused:.2f} us ({fused_gbs:.0f} GB/s)" if fused else "") +
         f", bottom {us_bottom:.2f} us")
 
+    # The roofline kernel's HBM traffic, read live by two short profiled runs of the same kernel at the
+    # same tile heights (the committed reading where the profiler cannot run).
+    t_read = t_written = None
+    t_note = "--no-traffic"
+    if fused and not args.no_traffic:
+        t_read, t_written, t_note = live_traffic(rows0, ",".join(str(v) for v in tuned) if tuned else "0")
+        log(f"bench: traffic read {t_read} written {t_written} ({t_note})")
+
     # Beside the headline: the reference method itself (binary32 values, binary64 residual, the
     # coarsest relaxed) on the device to the reference's own tolerance 1e-3, and the 256 x 512
     # problem (seed 0) to 1e-10 by the refined method, for the host legs of the same names.
@@ -808,7 +918,12 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(fused_gbs / HBM_PEAK_GBS, 4),
-            "traffic": MEASURED_TRAFFIC["fused"].get(rows0) if defaults else None,
+            "traffic": (round(t_read + t_written) if t_read is not None else
+                        MEASURED_TRAFFIC["fused"].get(rows0) if defaults else None),
+            "traffic_read": None if t_read is None else round(t_read),
+            "traffic_written": None if t_written is None else round(t_written),
+            "traffic_source": (t_note if t_read is not None else
+                               f"committed reading (profiles/hbm_traffic_refined_1024x2048.md): {t_note}"),
             "tile_rows": rows0,
             "avg_launch_us": round(us_fused, 3),
             "avg_launch_us_with_error_sums": round(us_fused_sums, 3) if us_fused_sums > 0 else None,

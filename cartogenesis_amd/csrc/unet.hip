@@ -515,37 +515,58 @@ __global__ __launch_bounds__(256) void k_conv3x3_in3(const uint16_t *__restrict_
 }
 
 // The first layer on the matrix cores, for output channel counts that are multiples of 16: one
-// wavefront computes 16 consecutive pixels (of the flattened image, row, column order) times every
-// output channel as D = W^T X on v_mfma_f32_16x16x32_bf16, with K = 4 tap + channel over taps 0..8
-// and channels 0..3 (channel 3 zeroed in both operands, so what the input holds there never enters),
-// in two K steps of 32: taps 0..7, then tap 8 and zeros. Lane m + 16 g feeds pixel m's taps 2g and
-// 2g + 1 (step 0) and tap 8 + 2g (step 1, beyond tap 8 zero), one 8-byte load each, and the weights of
-// output channel m of each 16-channel block at the same K; it holds output channels 4g .. 4g + 3 of
-// pixel m, stored as one 8-byte write. Sums are binary32, rounded once to bfloat16.
+// wavefront computes kIn3Groups groups of 16 consecutive pixels (of the flattened image, row, column
+// order) times every output channel as D = W^T X on v_mfma_f32_16x16x32_bf16, with K = 4 tap + channel
+// over taps 0..8 and channels 0..3 (channel 3 zeroed in both operands, so what the input holds there
+// never enters), in two K steps of 32: taps 0..7, then tap 8 and zeros. Lane m + 16 g feeds each
+// group's pixel m with taps 2g and 2g + 1 (step 0) and tap 8 + 2g (step 1, beyond tap 8 zero), one
+// 8-byte load each, all groups' loads issued before the first product, and the weights of output
+// channel m of each 16-channel block at the same K; it holds output channels 4g .. 4g + 3 of pixel m,
+// stored as one 8-byte write. Sums are binary32, rounded once to bfloat16.
+constexpr int kIn3Groups = 4;
+
 __global__ __launch_bounds__(256) void k_conv3x3_in3_mfma(const uint16_t *__restrict__ x, long npix, int down,
                                                           int across, const uint16_t *__restrict__ w, int cout,
                                                           int relu, uint16_t *__restrict__ y) {
+    // The wavefront's output tile (64 pixels x cout channels, bfloat16) is assembled in LDS and
+    // written back in contiguous 16-byte pieces.
+    __shared__ __attribute__((aligned(16))) uint16_t so[4][16 * kIn3Groups * kIn3MaxOut];
     const int lane = threadIdx.x & 63;
     const int m = lane & 15, g = lane >> 4;
-    const long p0 = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
-    if (p0 >= npix) return;
-    const long p = p0 + m;
-    const bool pin = p < npix;
-    const long row_id = pin ? p / across : 0;
-    const int col = pin ? (int)(p - row_id * across) : 0;
-    const int row = (int)(row_id % down);
-    const long img = row_id / down;
-    auto tap_in = [&](int t) -> uint2 {
-        uint2 v = {0u, 0u};
-        const int rr = row + t / 3 - 1;
-        if (pin && t < 9 && rr >= 0 && rr < down)
-            v = *(const uint2 *)(x + ((img * down + rr) * across + wrap(col + t % 3 - 1, across)) * 16);
-        v.y &= 0xFFFFu;
-        return v;
-    };
-    const uint2 i0 = tap_in(2 * g), i1 = tap_in(2 * g + 1), i2 = tap_in(8 + 2 * g);
-    const b8 bx0 = __builtin_bit_cast(b8, (u32x4){i0.x, i0.y, i1.x, i1.y});
-    const b8 bx1 = __builtin_bit_cast(b8, (u32x4){i2.x, i2.y, 0u, 0u});
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const long pw = ((long)blockIdx.x * 4 + wv) * 16 * kIn3Groups;
+    if (pw >= npix) return;  // no barrier below: each wavefront reads back only its own tile
+    // The wavefront's first pixel's image, row and column, once (uniform); each lane steps from there
+    // with carries rather than dividing per pixel.
+    const long rid0 = pw / across;
+    const int c0 = (int)(pw - rid0 * across), r0 = (int)(rid0 % down);
+    const long im0 = rid0 / down;
+    b8 bx0[kIn3Groups], bx1[kIn3Groups];
+#pragma unroll
+    for (int q = 0; q < kIn3Groups; ++q) {
+        const long p = pw + 16 * q + m;
+        const bool pin = p < npix;
+        int col = c0 + 16 * q + m, row = r0;
+        long img = im0;
+        while (col >= across) {  // at most 64 / across + 1 times
+            col -= across;
+            if (++row == down) {
+                row = 0;
+                ++img;
+            }
+        }
+        auto tap_in = [&](int t) -> uint2 {
+            uint2 v = {0u, 0u};
+            const int rr = row + t / 3 - 1;
+            if (pin && t < 9 && rr >= 0 && rr < down)
+                v = *(const uint2 *)(x + ((img * down + rr) * across + wrap(col + t % 3 - 1, across)) * 16);
+            v.y &= 0xFFFFu;
+            return v;
+        };
+        const uint2 i0 = tap_in(2 * g), i1 = tap_in(2 * g + 1), i2 = tap_in(8 + 2 * g);
+        bx0[q] = __builtin_bit_cast(b8, (u32x4){i0.x, i0.y, i1.x, i1.y});
+        bx1[q] = __builtin_bit_cast(b8, (u32x4){i2.x, i2.y, 0u, 0u});
+    }
     for (int ob = 0; ob < cout; ob += 16) {
         const int o = ob + m;
         auto tap_w = [&](int t) -> uint2 {
@@ -557,13 +578,25 @@ __global__ __launch_bounds__(256) void k_conv3x3_in3_mfma(const uint16_t *__rest
         const uint2 w0 = tap_w(2 * g), w1 = tap_w(2 * g + 1), w2 = tap_w(8 + 2 * g);
         const b8 aw0 = __builtin_bit_cast(b8, (u32x4){w0.x, w0.y, w1.x, w1.y});
         const b8 aw1 = __builtin_bit_cast(b8, (u32x4){w2.x, w2.y, 0u, 0u});
-        f4 acc = (f4){0.f, 0.f, 0.f, 0.f};
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aw0, bx0, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aw1, bx1, acc, 0, 0, 0);
-        float v[4];
-        for (int j = 0; j < 4; ++j) v[j] = relu && acc[j] < 0.f ? 0.f : acc[j];
-        if (pin) *(uint2 *)(y + p * cout + ob + 4 * g) = (uint2){pack2(v[0], v[1]), pack2(v[2], v[3])};
+#pragma unroll
+        for (int q = 0; q < kIn3Groups; ++q) {
+            f4 acc = (f4){0.f, 0.f, 0.f, 0.f};
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aw0, bx0[q], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aw1, bx1[q], acc, 0, 0, 0);
+            float v[4];
+            for (int j = 0; j < 4; ++j) v[j] = relu && acc[j] < 0.f ? 0.f : acc[j];
+            *(uint2 *)&so[wv][(16 * q + m) * cout + ob + 4 * g] = (uint2){pack2(v[0], v[1]), pack2(v[2], v[3])};
+        }
     }
+    // The LDS writes above and reads below are one wavefront's, issued in order; the fence keeps the
+    // compiler from moving the reads above the writes.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const long left = npix - pw;
+    const int pieces = (int)((left < 16 * kIn3Groups ? left : 16 * kIn3Groups) * cout / 8);
+    for (int c = lane; c < pieces; c += 64)
+        *(uint4 *)(y + pw * cout + 8 * c) = *(const uint4 *)&so[wv][8 * c];
 }
 
 // dw[i] = sum over slices s of part[s][i], in slice order (deterministic).
@@ -865,9 +898,9 @@ int cg_conv3x3_in3(const uint16_t *x, int n, int down, int across, const uint16_
     // at a time.
     if (reinterpret_cast<uintptr_t>(x) % 8 || reinterpret_cast<uintptr_t>(y) % 16) return CG_ERR_ARG;
     const long pixels = (long)n * down * across;
-    if (cout % 16 == 0 && reinterpret_cast<uintptr_t>(w) % 8 == 0)  // matrix cores: 16 pixels a wavefront
-        hipLaunchKernelGGL(k_conv3x3_in3_mfma, dim3(blocks(pixels, 64)), dim3(256), 0, (hipStream_t)stream, x, pixels,
-                           down, across, w, cout, relu, y);
+    if (cout % 16 == 0 && reinterpret_cast<uintptr_t>(w) % 8 == 0)  // matrix cores: 64 pixels a wavefront
+        hipLaunchKernelGGL(k_conv3x3_in3_mfma, dim3(blocks(pixels, 64 * kIn3Groups)), dim3(256), 0,
+                           (hipStream_t)stream, x, pixels, down, across, w, cout, relu, y);
     else
         hipLaunchKernelGGL(k_conv3x3_in3, dim3(blocks(pixels, 256)), dim3(256), 0, (hipStream_t)stream, x, n, down,
                            across, w, cout, relu, y);
