@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import sys
 from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
@@ -465,46 +466,63 @@ def coarse_pairs(acc, seeds: Sequence[int], rows: int = 1024, land_fraction: flo
     def make(seed):
         return circulation_problem(rows, seed=seed, land_fraction=land_fraction)
 
-    with ThreadPoolExecutor(max_workers=max(1, threads)) as pool:
-        for prob in pool.map(make, seeds):
-            L = coarse_level(prob.levels)
-            if L < 0:
-                raise ValueError(f"no grid of the {rows}-row ladder suits the coarse network")
-            st = prob.levels[L]
-            if shape is None:
-                level, shape = L, (st.cells_down, st.cells_across)
-            elif (st.cells_down, st.cells_across) != shape:
-                raise ValueError("the worlds' coarse grids differ in shape")
-            cells = st.cells
-            plan = Plan(acc, prob.levels)
-            sub = Plan(acc, prob.levels[L:])
-            try:
-                out = np.empty((most_steps, cells), np.float32)
-                n = check(lib().cg_ocean_plan_capture_coarse(plan.handle, L, float(tol), most_steps,
-                                                             out.ctypes.data_as(ctypes.c_void_p)),
-                          acc.ctx, "cg_ocean_plan_capture_coarse")
-                solves += 1
-                depth = coarse_depth(prob, L)
-                water = (np.asarray(st.is_water) != 0)
-                w = len(fixed_w)
-                fixed_w.append(np.stack([water.astype(np.float32), depth / DEPTH_SCALE], axis=1))
-                water_w.append(water.astype(np.float32))
-                for f in out[:n]:
-                    rms_f = float(np.sqrt(np.mean(f.astype(np.float64) ** 2)))
-                    if rms_f == 0.0:
-                        continue
-                    sub.set_forcing(f)
-                    info = sub.solve(1e-11, CG_METHOD_REFINED)
-                    if not info.relative_residual < 1e-11:
-                        raise RuntimeError(f"coarse target solve stopped at {info.relative_residual:.2e}")
-                    x64, _ = sub.read()
-                    fins.append(f / np.float32(rms_f))
-                    fs.append((x64 / rms_f).astype(np.float32))
-                    world_of.append(w)
-                    ratios.append(float(np.sqrt(np.mean(x64 ** 2))) / rms_f)
-            finally:
-                sub.close()
-                plan.close()
+    def generated():
+        # At most 2 x threads worlds (a few tens of MB each at 1024 x 2048) are generated ahead of the
+        # device, however many seeds there are; they come back in seed order.
+        ahead = 2 * max(1, threads)
+        with ThreadPoolExecutor(max_workers=max(1, threads)) as pool:
+            pending = [pool.submit(make, sd) for sd in seeds[:ahead]]
+            nxt = len(pending)
+            done = 0
+            while pending:
+                fut = pending.pop(0)
+                if nxt < len(seeds):
+                    pending.append(pool.submit(make, seeds[nxt]))
+                    nxt += 1
+                yield fut.result()
+                done += 1
+                if len(seeds) > 256 and done % 250 == 0:
+                    print(f"coarse pairs: {done} of {len(seeds)} worlds", file=sys.stderr, flush=True)
+
+    for prob in generated():
+        L = coarse_level(prob.levels)
+        if L < 0:
+            raise ValueError(f"no grid of the {rows}-row ladder suits the coarse network")
+        st = prob.levels[L]
+        if shape is None:
+            level, shape = L, (st.cells_down, st.cells_across)
+        elif (st.cells_down, st.cells_across) != shape:
+            raise ValueError("the worlds' coarse grids differ in shape")
+        cells = st.cells
+        plan = Plan(acc, prob.levels)
+        sub = Plan(acc, prob.levels[L:])
+        try:
+            out = np.empty((most_steps, cells), np.float32)
+            n = check(lib().cg_ocean_plan_capture_coarse(plan.handle, L, float(tol), most_steps,
+                                                         out.ctypes.data_as(ctypes.c_void_p)),
+                      acc.ctx, "cg_ocean_plan_capture_coarse")
+            solves += 1
+            depth = coarse_depth(prob, L)
+            water = (np.asarray(st.is_water) != 0)
+            w = len(fixed_w)
+            fixed_w.append(np.stack([water.astype(np.float32), depth / DEPTH_SCALE], axis=1))
+            water_w.append(water.astype(np.float32))
+            for f in out[:n]:
+                rms_f = float(np.sqrt(np.mean(f.astype(np.float64) ** 2)))
+                if rms_f == 0.0:
+                    continue
+                sub.set_forcing(f)
+                info = sub.solve(1e-11, CG_METHOD_REFINED)
+                if not info.relative_residual < 1e-11:
+                    raise RuntimeError(f"coarse target solve stopped at {info.relative_residual:.2e}")
+                x64, _ = sub.read()
+                fins.append(f / np.float32(rms_f))
+                fs.append((x64 / rms_f).astype(np.float32))
+                world_of.append(w)
+                ratios.append(float(np.sqrt(np.mean(x64 ** 2))) / rms_f)
+        finally:
+            sub.close()
+            plan.close()
     if not fins:
         raise RuntimeError("no training pairs")
     out_scale = float(np.median(ratios))
