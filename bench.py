@@ -211,15 +211,24 @@ def uneven_batches(acc, specs, instances: int, tol: float):
             worst = worst_of(worst, infos)
             cycles[f"{k}:{rows}x{2 * rows}"] = [min(i.cycles for i in infos), max(i.cycles for i in infos)]
         sequential = total / (time.perf_counter() - t0)
-        # The same plans as one group: one host loop, each plan's step graph on a stream of its own.
+        # The same plans as one group: one host loop; the merged step (one graph a step, each tile
+        # launch covering one depth of every plan's ladder), then each plan's own step graph on a
+        # stream of the group's.
+        import cartogenesis_amd as cg
+
         group = PlanGroup(acc, plans)
-        group.solve(tol)  # warm: the composed graphs
         reps = 3
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            for infos in group.solve(tol):
-                worst = worst_of(worst, infos)
-        grouped = reps * total / (time.perf_counter() - t0)
+        timed = {}
+        for merged in (1, 0):
+            acc.set_option(cg.CG_OPT_GROUP_MERGED, merged)
+            group.solve(tol)  # warm: the graphs
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                for infos in group.solve(tol):
+                    worst = worst_of(worst, infos)
+            timed[merged] = reps * total / (time.perf_counter() - t0)
+        acc.set_option(cg.CG_OPT_GROUP_MERGED, 1)
+        grouped = (timed[1], timed[0])
     finally:
         if group:
             group.close()
@@ -266,10 +275,12 @@ def batched_worlds(acc, tol: float):
     specs = [(512, 1, 0.20), (768, 2, 0.27), (1024, 3, 0.33), (1024, 4, 0.40)]
     seq, conc, grouped, per, cycles, worst = uneven_batches(acc, specs, 12, tol)
     log(f"bench: four-world year {seq:.1f} solves/s one world after another, {conc:.1f} with the worlds concurrent, "
-        f"{grouped:.1f} as one group")
+        f"{grouped[0]:.1f} as one group (merged step), {grouped[1]:.1f} as one group (a stream per world)")
     return {"workload": "12 months x worlds 512x1024, 768x1536, 1024x2048, 1024x2048 (seeds 1-4, land 20/27/33/40%)",
-            "solves_per_s": round(grouped, 3), "method": "one group (cg_ocean_group_solve): one host loop, each "
-            "world's refinement step one hipGraph launch for its 12 months, the worlds on streams of their own",
+            "solves_per_s": round(grouped[0], 3), "method": "one group (cg_ocean_group_solve), merged step: one host "
+            "loop, one hipGraph launch a refinement step for the 48 instances, each tile launch covering one depth of "
+            "every world's ladder and all its months",
+            "solves_per_s_group_streams": round(grouped[1], 3),
             "solves_per_s_concurrent_streams": round(conc, 3), "solves_per_s_one_world_at_a_time": round(seq, 3),
             "ms_per_world_one_at_a_time": per, "cycles_min_max": cycles, "max_relative_residual": worst}
 
@@ -281,10 +292,12 @@ def batched_combined(acc, tol: float):
     specs = [(256, 5, 0.3), (512, 6, 0.3), (1024, 7, 0.3), (2048, 8, 0.3)]
     seq, conc, grouped, per, cycles, worst = uneven_batches(acc, specs, 16, tol)
     log(f"bench: combined 64 instances {seq:.1f} solves/s one size after another, {conc:.1f} concurrent, "
-        f"{grouped:.1f} as one group")
+        f"{grouped[0]:.1f} as one group (merged step), {grouped[1]:.1f} as one group (a stream per size)")
     return {"workload": "64 instances: 16 monthly forcings each at 256x512, 512x1024, 1024x2048, 2048x4096",
-            "solves_per_s": round(grouped, 3), "method": "one group (cg_ocean_group_solve): one host loop, each "
-            "size's refinement step one hipGraph launch for its 16 instances, the sizes on streams of their own",
+            "solves_per_s": round(grouped[0], 3), "method": "one group (cg_ocean_group_solve), merged step: one host "
+            "loop, one hipGraph launch a refinement step for the 64 instances, each tile launch covering one depth of "
+            "every size's ladder and all its instances",
+            "solves_per_s_group_streams": round(grouped[1], 3),
             "solves_per_s_concurrent_streams": round(conc, 3), "solves_per_s_one_size_at_a_time": round(seq, 3),
             "ms_per_size_one_at_a_time": per, "cycles_min_max": cycles, "max_relative_residual": worst}
 

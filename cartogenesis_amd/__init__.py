@@ -23,7 +23,7 @@ CG_METHOD_LEARNED, CG_METHOD_LEARNED_FGMRES = 5, 6
 CG_OPT_MIN_CELLS, CG_OPT_TILE_ROWS, CG_OPT_USE_GRAPH = 1, 2, 3
 CG_OPT_FGMRES_RESTART, CG_OPT_TIME_KERNELS = 4, 6
 CG_OPT_FUSED, CG_OPT_LEVEL_PASSES, CG_OPT_FUSE_OUTER, CG_OPT_DIRECT_BOTTOM = 7, 8, 9, 10
-CG_OPT_STOP_PERMILLE, CG_OPT_ERROR_PICO, CG_OPT_NET_FALLBACK = 11, 12, 13
+CG_OPT_STOP_PERMILLE, CG_OPT_ERROR_PICO, CG_OPT_NET_FALLBACK, CG_OPT_GROUP_MERGED = 11, 12, 13, 14
 
 
 class CgSolution(ctypes.Structure):

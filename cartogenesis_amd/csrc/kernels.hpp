@@ -345,386 +345,47 @@ template <int NH, int TH, bool ZERO_START, int MODE>
 __global__ __launch_bounds__(256) void k_relax_tile(LevelArgs L, const float *__restrict__ f,
                                                     const float *__restrict__ xin, float *__restrict__ xout,
                                                     TileExtra ex) {
-    constexpr int HR = TileGeom<NH, MODE>::HR;
-    constexpr int HC = TileGeom<NH, MODE>::HC;
-    constexpr int IW = TileGeom<NH, MODE>::IW;
-    constexpr int R = TH + 2 * HR;
-    constexpr bool kFusedUpdate = MODE != TILE_PLAIN;  // the refined cycle's legs (rb_step)
-    static_assert(R <= 64, "tile rows must fit a 64-bit row mask");
-    static_assert(TH % 2 == 0, "tiles start on even rows");
-    const int across = L.across, down = L.down;
-    const int lane = threadIdx.x & 63;
-    const int inst = blockIdx.y;
-    const bool gated = MODE == TILE_DOWN && ex.gate.in;
-    if (!gated && ex.active && !ex.late && !ex.active[inst]) return;
-    // One instance: the outer step's outputs wait on the flag, read here with the loads.
-    const int go_late = (MODE == TILE_UP_OUTER && ex.active && ex.late) ? ex.active[inst] : 1;
-    if (inst) {
-        f += inst * ex.fs;
-        if (xin) xin += inst * ex.fs;
-        xout += inst * ex.fs;
-        if (MODE == TILE_DOWN) ex.fc += inst * ex.cs;
-        if (MODE == TILE_UP || MODE == TILE_UP_OUTER) ex.xc += inst * ex.cs;
-        if (MODE == TILE_UP_OUTER) {
-            ex.x64 += inst * ex.s64;
-            ex.x64n += inst * ex.s64;
-            ex.f0 += inst * ex.fs;
-            ex.fcn += inst * ex.fs;
-            ex.red.out += inst * ex.red_stride;
-            ex.red.clear += inst * ex.red_stride;
-            if (ex.sums.out) {
-                ex.sums.out += inst * ex.sums_stride;
-                ex.sums.clear += inst * ex.sums_stride;
-            }
-        }
-    }
-    const int wave = xcd_block() * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int tiles_x = (across + IW - 1) / IW;
-    const int tiles_y = (down + TH - 1) / TH;
-    if (wave >= tiles_x * tiles_y) return;
-    // Every wave of the instance evaluates the test on the same inputs and so reaches the same
-    // decision; tile 0's wave records it.
-    if (gated && !ex.late && !gate_decide(ex.gate, inst, wave == 0)) return;
-    const int tx = wave % tiles_x, ty = wave / tiles_x;
-    const int col0 = tx * IW, row0 = ty * TH;
-    int ca = (col0 - HC + 2 * lane) % across;
-    if (ca < 0) ca += across;
+#define TILE_INST blockIdx.y
+#define TILE_BLOCK() xcd_block()
+#include "tile_body.inc"
+#undef TILE_INST
+#undef TILE_BLOCK
+}
 
-    // Lane j holds the weights of tile row j (rows beyond a pole take the edge row's; their cells
-    // are never water, so the weights only multiply zeros); readlane hands them out per row.
-    float4 wrow = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (lane < R) {
-        int rr = row0 - HR + lane;
-        rr = rr < 0 ? 0 : (rr >= down ? down - 1 : rr);
-        wrow = L.roww[rr];
-    }
-    float A[R], B[R], fA[R], fB[R];
-    uint32_t mraw[R];
-    uint64_t wA = 0, wB = 0;
-    // What the modes read besides the tile's rows: TILE_UP's coarse values and TILE_DOWN's coarse
-    // mask, loaded with the rows; TILE_UP_OUTER's binary64 answer, forcing and centres, later.
-    //  - Coarse values: fine row r = row0 - HR + i reads coarse rows above = floor((r - 1) / 2) and
-    //    above + 1, i.e. row0 / 2 + q with q = floor((i - HR - 1) / 2), a compile-time offset since
-    //    row0 is even; lane l's pair (2m, 2m+1) reads coarse columns m-1, m, m+1 (lanes l-1, l, l+1).
-    //  - The binary64 answer of rows HR-1 .. HR+TH (this lane's pair), zero beyond a pole, and the
-    //    binary64 problem's forcing and centres of rows HR .. HR+TH-1.
-    //  - The coarse mask of each row pair's coarse cell. Writer lanes' columns cabs equal ca.
-    constexpr bool kUp = MODE == TILE_UP || MODE == TILE_UP_OUTER;
-    constexpr int QMIN = -((HR + 2) / 2);  // floor((0 - HR - 1) / 2)
-    constexpr int QMAX = (R - HR - 2) >= 0 ? (R - HR - 2) / 2 + 1 : 0;
-    constexpr int NQ = kUp ? QMAX - QMIN + 1 : 1;
-    constexpr int NX = MODE == TILE_UP_OUTER ? TH + 2 : 1;
-    constexpr int NF = MODE == TILE_UP_OUTER ? TH : 1;
-    constexpr int NW = MODE == TILE_DOWN ? TH / 2 : 1;
-    constexpr int kLateLoadStep = R / 2;
-    float cv[NQ];
-    // TILE_UP_OUTER's binary64 answer of rows HR-1 .. HR+TH (this lane's pair), loaded halfway
-    // through the sweeps, and the rows' binary64 centres and forcing, loaded after them.
-    constexpr bool kOuter = MODE == TILE_UP_OUTER;
-    double px[kOuter ? NX : 1], py[kOuter ? NX : 1], cenr[kOuter ? NF : 1];
-    float2 f0v[NF];
-    uint32_t cwv[NW];
-    // TILE_DOWN's restriction factor per row, loaded up front too (rows past the last clamp to it).
-    constexpr int NG = MODE == TILE_DOWN ? TH : 1;
-    float gv[NG];
-    if constexpr (MODE == TILE_DOWN) {
-#pragma unroll
-        for (int k = 0; k < NG; ++k) gv[k] = ex.g[row0 + k < down ? row0 + k : down - 1];
-    }
-    // Branch-free loads (a row beyond a pole loads the edge row and is then zeroed), all issued
-    // before any is used, so that the tile's loads are in flight together: the coarse values first,
-    // then the rows in the order the sweeps reach them, then what only the end of the tile needs,
-    // so that the wave waits for each row only when the sweeps first touch it.
-    if constexpr (kUp) {
-        const int m = ca >> 1;
-        const int kc0 = row0 >> 1;
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const int kr = kc0 + QMIN + q;
-            const int kc = kr < 0 ? 0 : (kr >= ex.cd ? ex.cd - 1 : kr);
-            cv[q] = at_byte<float>(ex.xc, 4u * ((unsigned)kc * (unsigned)ex.ca + (unsigned)m));
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < R; ++i) {
-        // A row beyond a pole reads row `down`, the all-zero row every array the smoother reads
-        // carries past its last row: zero values, zero forcing, no water. The device mask holds
-        // 0 or 1 per cell.
-        const int r = row0 - HR + i;
-        const int rc = (r >= 0 && r < down) ? r : down;
-        // Byte offsets: a uniform row offset (scalar) plus the lane's column offset.
-        const unsigned idx = (unsigned)rc * (unsigned)across + (unsigned)ca;  // cells < 2^30
-        const unsigned off4 = (unsigned)rc * (4u * (unsigned)across) + 4u * (unsigned)ca;
-        const float2 fv = at_byte<float2>(f, off4);
-        mraw[i] = at_byte<uint16_t>(L.water, idx);
-        fA[i] = fv.x;
-        fB[i] = fv.y;
-        if (ZERO_START) {
-            A[i] = 0.0f;
-            B[i] = 0.0f;
-        } else {
-            const float2 xv = at_byte<float2>(xin, off4);
-            A[i] = xv.x;
-            B[i] = xv.y;
-        }
-        // Issue order is row order (the scheduler would otherwise sink the loads of the rows
-        // that need only one forcing component, the first the sweeps touch, to the end).
-        __builtin_amdgcn_sched_barrier(0);
-    }
-    if constexpr (MODE == TILE_DOWN) {
-        const unsigned cacross = (unsigned)across >> 1;
-#pragma unroll
-        for (int k = 0; k < NW; ++k) {
-            const int r = row0 + 2 * k;
-            const unsigned kc = r < down ? (unsigned)(r >> 1) : 0u;
-            cwv[k] = at_byte<uint8_t>(ex.cwater, kc * cacross + ((unsigned)ca >> 1));
-        }
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    // TILE_UP's coarse values and their neighbours' (prolongAdd's `at`, OC:380-385: beyond a pole,
-    // the edge row times `beyond`; in range the factor is 1, and 1 * v is v exactly).
-    float cm[NQ], cl[NQ], cr[NQ];
-    if constexpr (kUp) {
-        const int kc0 = row0 >> 1;
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const int kr = kc0 + QMIN + q;
-            const float fac = (kr < 0 || kr >= ex.cd) ? ex.beyond : 1.0f;
-            const float v = fac * cv[q];
-            cm[q] = v;
-            cl[q] = lane_from_below(v);
-            cr[q] = lane_from_above(v);
-        }
-    }
+// One leg of several plans' cycles in one launch (the grids of uneven worlds at the same depth of
+// their ladders, each with its own instances): job j owns the `count` workgroups from `first` on,
+// `per_inst` of them per instance. first and per_inst are multiples of the XCD count, so workgroup
+// b's XCD (b mod 8) is its local index's, and each XCD works on one contiguous band of every
+// instance's tiles as in k_relax_tile.
+struct TileJob {
+    LevelArgs L;
+    const float *f;
+    const float *xin;
+    float *xout;
+    TileExtra ex;
+    int first;
+    int per_inst;
+    int count;
+};
 
-    // Skewed schedule: half-sweep h reaches row i at step t = i + h. A colour-k update reads only
-    // other-colour cells (its row's pair partner and the same column in the rows above and below),
-    // which half-sweep h leaves untouched, so row i-1 being one half-sweep ahead is harmless; and
-    // row i+1 has finished half-sweep h-1 earlier in the same step. The result equals NH full
-    // half-sweeps in order, and each row's weights are live for only NH consecutive steps.
-    // Half-sweep h updates rows 1+h .. R-2-h only (a trapezoid): those read rows h .. R-1-h, which
-    // are exactly the rows half-sweep h-1 updated, and the last half-sweep still covers every row
-    // the tile writes or reads afterwards (the interior, plus its ring where HR = NH + 1).
-    // Row t + 1 is prepared at the start of step t, just before half-sweep 0 first reads it: its
-    // mask bits, and in TILE_UP its bilinear correction (column 2m: west m-1, east m, blend 0.75;
-    // column 2m+1: west m, east m+1, blend 0.25; branch-free: on land the addend is +0, and a land
-    // cell holds +0, so it stays +0). Steps -1 and 0 only prepare rows 0 and 1.
-#pragma unroll
-    for (int t = -1; t < R - 1; ++t) {
-        {
-            const int i = t + 1;
-            wA |= (uint64_t)(mraw[i] & 1u) << i;
-            wB |= (uint64_t)((mraw[i] >> 8) & 1u) << i;
-            if constexpr (kUp) {
-                // q index of `above` for this row, relative to QMIN.
-                const int qa = ((i - HR - 1) >= 0 ? (i - HR - 1) / 2 : -((HR + 1 - i + 1) / 2)) - QMIN;
-                const bool r_even = ((i + HR) & 1) == 0;
-                const float db = r_even ? 0.75f : 0.25f;
-                const float topA = cl[qa] * (1.0f - 0.75f) + cm[qa] * 0.75f;
-                const float botA = cl[qa + 1] * (1.0f - 0.75f) + cm[qa + 1] * 0.75f;
-                const float topB = cm[qa] * (1.0f - 0.25f) + cr[qa] * 0.25f;
-                const float botB = cm[qa + 1] * (1.0f - 0.25f) + cr[qa + 1] * 0.25f;
-                const uint32_t mA = i < 32 ? (uint32_t)wA : (uint32_t)(wA >> 32);
-                const uint32_t mB = i < 32 ? (uint32_t)wB : (uint32_t)(wB >> 32);
-                A[i] += keep_if(topA * (1.0f - db) + botA * db, mA, i & 31);
-                B[i] += keep_if(topB * (1.0f - db) + botB * db, mB, i & 31);
-            }
-        }
-        // TILE_UP_OUTER's binary64 answer is only read after the sweeps: issued halfway through
-        // them, it arrives while the rest run, and the tile's first rows do not queue behind it.
-        if constexpr (MODE == TILE_UP_OUTER) {
-            if (t == kLateLoadStep) {
-#pragma unroll
-                for (int j = 0; j < NX; ++j) {
-                    const int r = row0 - 1 + j;
-                    const int rc = r < 0 ? 0 : (r >= down ? down - 1 : r);
-                    const double2 xv =
-                        at_byte<double2>(ex.x64, (unsigned)rc * (8u * (unsigned)across) + 8u * (unsigned)ca);
-                    px[j] = xv.x;
-                    py[j] = xv.y;
-                }
-            }
-        }
-#pragma unroll
-        for (int h = 0; h < NH; ++h) {
-            const int i = t - h;
-            if (t < 1 || i < 1 + h || i > R - 2 - h) continue;
-            const int colour = h & 1;
-            const float4 wt = make_float4(lane_value(wrow.x, i), lane_value(wrow.y, i), lane_value(wrow.z, i),
-                                          lane_value(wrow.w, i));
-            const uint32_t mA = i < 32 ? (uint32_t)wA : (uint32_t)(wA >> 32);
-            const uint32_t mB = i < 32 ? (uint32_t)wB : (uint32_t)(wB >> 32);
-            // Row (row0 - HR + i) has the parity of (i + HR) since row0 is even; the even column
-            // of the pair has colour (row & 1).
-            if (ZERO_START && h == 0) {
-                // From zero every neighbour is +0 and the weights are non-negative, so each product
-                // and partial sum is +0 and the update is +0 - f exactly.
-                if (((i + HR) & 1) == colour)
-                    A[i] = keep_if(0.0f - fA[i], mA, i & 31);
-                else
-                    B[i] = keep_if(0.0f - fB[i], mB, i & 31);
-            } else if (((i + HR) & 1) == colour) {
-                const float xw = lane_from_below(B[i]);
-                A[i] = keep_if(rb_step<kFusedUpdate>(wt, B[i], xw, A[i - 1], A[i + 1], fA[i]), mA, i & 31);
-            } else {
-                const float xe = lane_from_above(A[i]);
-                B[i] = keep_if(rb_step<kFusedUpdate>(wt, xe, A[i], B[i - 1], B[i + 1], fB[i]), mB, i & 31);
-            }
-        }
-        // Keep each step's weights local to the step: without this the scheduler hoists every
-        // row's readlanes to the top and spills the scalars.
-        __builtin_amdgcn_sched_barrier(0);
-    }
-
-    const int lo = HC / 2;
-    const int cabs = col0 + 2 * (lane - lo);
-    const bool writer = lane >= lo && lane < lo + IW / 2 && cabs < across;
-    if (writer && MODE != TILE_UP_OUTER) {
-#pragma unroll
-        for (int i = HR; i < HR + TH; ++i) {
-            const int r = row0 - HR + i;
-            if (r < down) at_byte<float2>(xout, (unsigned)r * (4u * (unsigned)across) + 4u * (unsigned)cabs) = make_float2(A[i], B[i]);
-        }
-    }
-
-    if constexpr (MODE == TILE_UP_OUTER) {
-        // Rows HR-1 .. HR+TH of the tile: x64n = x64 + delta, zero beyond a pole (k_outer_residual's
-        // neighbours there). Every lane holds its own pair, so the writers' neighbour lanes hold
-        // theirs for the shifts; the correction is exact on those lanes (the column halo keeps two
-        // columns of slack after NH half-sweeps).
-        // The binary64 forcing and centres of the tile's rows, loaded here rather than with the
-        // answer: held through the sweeps they cost a wave per SIMD of occupancy, and their
-        // latency now overlaps the answer's update below.
-#pragma unroll
-        for (int k = 0; k < TH; ++k) {
-            const int r = row0 + k;
-            const int rc = r < down ? r : down;
-            f0v[k] = at_byte<float2>(ex.f0, (unsigned)rc * (4u * (unsigned)across) + 4u * (unsigned)ca);
-            cenr[k] = L.rowc[row0 + k < down ? row0 + k : down - 1];
-        }
-        double vx[TH + 2], vy[TH + 2];
-#pragma unroll
-        for (int j = 0; j < TH + 2; ++j) {
-            const int r = row0 - 1 + j;
-            vx[j] = vy[j] = 0.0;
-            if (r >= 0 && r < down) {
-                vx[j] = px[j] + (double)A[HR - 1 + j];
-                vy[j] = py[j] + (double)B[HR - 1 + j];
-            }
-        }
-        unsigned long long mx = 0;
-        if (!go_late) return;
-        // Squares of the correction and of the new answer over the water cells, summed in binary32
-        // per lane (an estimate needs a few digits) and in binary64 across the wave, on the steps
-        // that the gate asks for them.
-        const bool sums_on = ex.sums.out && (!ex.want_sums || ex.want_sums[inst]);
-        float d2 = 0.0f, x2 = 0.0f;
-#pragma unroll
-        for (int k = 0; k < TH; ++k) {
-            const int i = HR + k;
-            const int row = row0 + k;
-            const double west = dpp_double<0x138>(vy[k + 1]);  // lane - 1's odd column
-            const double east = dpp_double<0x130>(vx[k + 1]);  // lane + 1's even column
-            const float4 wt = make_float4(lane_value(wrow.x, i), lane_value(wrow.y, i), lane_value(wrow.z, i),
-                                          lane_value(wrow.w, i));
-            if (!writer || row >= down) continue;
-            const unsigned off4 = (unsigned)row * (4u * (unsigned)across) + 4u * (unsigned)cabs;
-            const float2 fv = f0v[k];
-            const double cen = cenr[k];
-            const uint32_t mA = i < 32 ? (uint32_t)wA : (uint32_t)(wA >> 32);
-            const uint32_t mB = i < 32 ? (uint32_t)wB : (uint32_t)(wB >> 32);
-            // The rows beyond a pole already hold zero (vx, vy above).
-            const double vn_x = vx[k], vn_y = vy[k];
-            const double vs_x = vx[k + 2], vs_y = vy[k + 2];
-            double ux = 0.0, uy = 0.0, ox = 0.0, oy = 0.0;
-            if ((mA >> (i & 31)) & 1) {
-                const double bal = (double)wt.x * vy[k + 1] + (double)wt.y * west + (double)wt.z * vn_x +
-                                   (double)wt.w * vs_x - (double)fv.x;
-                ox = vx[k + 1];
-                ux = ox - bal;
-                const unsigned long long b = abs_bits(ux * cen);
-                mx = b > mx ? b : mx;
-                if (sums_on) {
-                    const float xs = (float)ox;
-                    d2 = __fmaf_rn(A[i], A[i], d2);
-                    x2 = __fmaf_rn(xs, xs, x2);
-                }
-            }
-            if ((mB >> (i & 31)) & 1) {
-                const double bal = (double)wt.x * east + (double)wt.y * vx[k + 1] + (double)wt.z * vn_y +
-                                   (double)wt.w * vs_y - (double)fv.y;
-                oy = vy[k + 1];
-                uy = oy - bal;
-                const unsigned long long b = abs_bits(uy * cen);
-                mx = b > mx ? b : mx;
-                if (sums_on) {
-                    const float ys = (float)oy;
-                    d2 = __fmaf_rn(B[i], B[i], d2);
-                    x2 = __fmaf_rn(ys, ys, x2);
-                }
-            }
-            at_byte<double2>(ex.x64n, 2u * off4) = make_double2(ox, oy);
-            at_byte<float2>(ex.fcn, off4) = make_float2((float)ux, (float)uy);
-        }
-        // One relaxed atomicMax per wave into slot wave % kMaxSlots (and, with sums, one add of each
-        // sum); the waves together zero the slots the next step reduces into.
-        mx = wave_max_u64(mx);
-        double d2w = 0.0, x2w = 0.0;
-        if (sums_on) {
-            d2w = wave_sum_f64((double)d2);
-            x2w = wave_sum_f64((double)x2);
-        }
-        if (lane == 0) {
-            const int nw = tiles_x * tiles_y;
-            for (int j = wave; j < kMaxSlots; j += nw) ex.red.clear[j] = 0ull;
-            if (mx) atomicMax(ex.red.out + (wave & (kMaxSlots - 1)), mx);
-            if (ex.sums.out) {
-                for (int j = wave; j < 2 * kMaxSlots; j += nw) ex.sums.clear[j] = 0.0;
-                if (d2w != 0.0) atomicAdd(ex.sums.out + (wave & (kMaxSlots - 1)), d2w);
-                if (x2w != 0.0) atomicAdd(ex.sums.out + kMaxSlots + (wave & (kMaxSlots - 1)), x2w);
-            }
-        }
-    }
-
-    if constexpr (MODE == TILE_DOWN) {
-        const int cacross = across >> 1;
-#pragma unroll
-        for (int i = HR; i < HR + TH; i += 2) {
-            const int r = row0 - HR + i;  // even
-            const bool in = r < down;
-            float u[4];
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const int ii = i + k;
-                const float4 wt = make_float4(lane_value(wrow.x, ii), lane_value(wrow.y, ii), lane_value(wrow.z, ii),
-                                              lane_value(wrow.w, ii));
-                const float xw = lane_from_below(B[ii]);
-                const float xe = lane_from_above(A[ii]);
-                // The cells of the last half-sweep's colour were just set to exactly the update the
-                // residual would recompute from the same, unchanged neighbours, so their residual
-                // is x - x (+0, or NaN where x is not finite) without recomputing it.
-                constexpr int kLast = (NH - 1) & 1;
-                const bool a_last = ((ii + HR) & 1) == kLast;
-                const float ua = a_last ? A[ii] - A[ii] : A[ii] - rb_step<kFusedUpdate>(wt, B[ii], xw, A[ii - 1], A[ii + 1], fA[ii]);
-                const float ub = a_last ? B[ii] - rb_step<kFusedUpdate>(wt, xe, A[ii], B[ii - 1], B[ii + 1], fB[ii]) : B[ii] - B[ii];
-                const uint32_t mA = ii < 32 ? (uint32_t)wA : (uint32_t)(wA >> 32);
-                const uint32_t mB = ii < 32 ? (uint32_t)wB : (uint32_t)(wB >> 32);
-                u[2 * k] = keep_if(ua, mA, ii & 31);
-                u[2 * k + 1] = keep_if(ub, mB, ii & 31);
-            }
-            if (writer && in) {
-                const unsigned cidx = (unsigned)(r >> 1) * (unsigned)cacross + ((unsigned)cabs >> 1);
-                const float g0 = gv[i - HR], g1 = gv[i - HR + 1];
-                float acc = 0.0f;
-                acc += u[0] * g0;
-                acc += u[1] * g0;
-                acc += u[2] * g1;
-                acc += u[3] * g1;
-                at_byte<float>(ex.fc, 4u * cidx) = cwv[(i - HR) / 2] ? acc : 0.0f;
-            }
-        }
-        if (gated && ex.late && wave == 0) (void)gate_decide(ex.gate, inst, true);
-    }
+template <int NH, int TH, bool ZERO_START, int MODE>
+__global__ __launch_bounds__(256) void k_relax_tile_jobs(const TileJob *__restrict__ jobs, int njobs) {
+    int j = 0;
+    while (j + 1 < njobs && (int)blockIdx.x >= jobs[j + 1].first) ++j;
+    const int local = (int)blockIdx.x - jobs[j].first;
+    if (local < 0 || local >= jobs[j].count) return;
+    const int job_inst = local / jobs[j].per_inst, lb = local % jobs[j].per_inst;
+    const int job_block = (lb % kXcds) * (jobs[j].per_inst / kXcds) + lb / kXcds;
+    LevelArgs L = jobs[j].L;
+    const float *__restrict__ f = jobs[j].f;
+    const float *__restrict__ xin = jobs[j].xin;
+    float *__restrict__ xout = jobs[j].xout;
+    TileExtra ex = jobs[j].ex;
+#define TILE_INST job_inst
+#define TILE_BLOCK() job_block
+#include "tile_body.inc"
+#undef TILE_INST
+#undef TILE_BLOCK
 }
 
 // One colour of one red-black pass, in place, any weights (per row or per cell): the general and
@@ -1575,10 +1236,10 @@ struct DirectArgs {
 // by*NI .. +NI, so a row of M is read once per NI instances (1 for a plan of one instance, whose
 // forcing then takes a quarter of the LDS).
 template <int NI>
-__global__ __launch_bounds__(64 * kDirectWaves) void k_direct_bottom(DirectArgs a) {
+__device__ __forceinline__ void direct_bottom_body(const DirectArgs &a, const int bx, const int by) {
     constexpr int kDirectInst = NI;
     const int n = a.n;
-    const int c0 = blockIdx.y * kDirectInst;
+    const int c0 = by * kDirectInst;
     const int ni = a.K - c0 < kDirectInst ? a.K - c0 : kDirectInst;
     bool on[kDirectInst];
     bool any = false;
@@ -1595,7 +1256,7 @@ __global__ __launch_bounds__(64 * kDirectWaves) void k_direct_bottom(DirectArgs 
     }
     __syncthreads();
     const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * kDirectWaves + (threadIdx.x >> 6);
+    const int i = bx * kDirectWaves + (threadIdx.x >> 6);
     if (i >= n) return;
     const double *row = a.M + (size_t)i * n;
     double acc[kDirectInst] = {};
@@ -1616,6 +1277,31 @@ __global__ __launch_bounds__(64 * kDirectWaves) void k_direct_bottom(DirectArgs 
         for (int off = 32; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off, 64);
         if (lane == 0 && on[c]) a.x[(c0 + c) * a.x_stride + a.idx[i]] = (float)acc[c];
     }
+}
+
+template <int NI>
+__global__ __launch_bounds__(64 * kDirectWaves) void k_direct_bottom(DirectArgs a) {
+    direct_bottom_body<NI>(a, blockIdx.x, blockIdx.y);
+}
+
+// Several plans' direct bottoms in one launch: job j owns `count` = rows x groups workgroups from
+// `first` on, workgroup first + g rows + r taking output rows r kDirectWaves .. and instance group g.
+struct DirectJob {
+    DirectArgs a;
+    int first;
+    int rows;
+    int count;
+};
+
+template <int NI>
+__global__ __launch_bounds__(64 * kDirectWaves) void k_direct_bottom_jobs(const DirectJob *__restrict__ jobs,
+                                                                          int njobs) {
+    int j = 0;
+    while (j + 1 < njobs && (int)blockIdx.x >= jobs[j + 1].first) ++j;
+    const int local = (int)blockIdx.x - jobs[j].first;
+    if (local < 0 || local >= jobs[j].count) return;
+    const DirectArgs a = jobs[j].a;
+    direct_bottom_body<NI>(a, local % jobs[j].rows, local / jobs[j].rows);
 }
 
 // ------------------------------------------------------------------------------------------------

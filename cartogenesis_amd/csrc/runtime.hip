@@ -120,6 +120,7 @@ struct cg_ctx {
     int fuse_outer = 1;  // the finest up leg and the binary64 outer step in one launch
     int net_fallback = 1;  // a learned solve switches to the plain cycle when its residual stalls
     int direct_bottom = 1;  // the binary32 cycle's bottom grid solved by its dense inverse
+    int group_merged = 1;   // a group's refined steps as one graph whose launches cover every plan
     // The binary64-refined methods stop once the residual is under tol x stop_permille / 1000
     int stop_permille = 1000;
     // and, once a correction has been applied, their estimated relative L2 error (the step gate's)
@@ -288,6 +289,33 @@ hipError_t tile_dispatch(cg_ctx *ctx, const LevelArgs &L, const float *f, const 
         case 32: return launch_tile<32>(s, L, f, xin, xout, nh, zero, mode, ex, instances);
         default: return launch_tile<16>(s, L, f, xin, xout, nh, zero, mode, ex, instances);
     }
+}
+
+// Launches of k_relax_tile_jobs: the legs of several plans (one mode, tile height and two passes for
+// all) in one launch, their job table in device memory. Tile heights 2, 4, 6 and 8.
+constexpr int kJobRows[4] = {2, 4, 6, 8};
+
+template <int TH, int MODE>
+hipError_t launch_jobs_th(hipStream_t s, const TileJob *jobs, int njobs, unsigned blocks) {
+    constexpr bool zero = MODE == TILE_DOWN;
+    hipLaunchKernelGGL((k_relax_tile_jobs<4, TH, zero, MODE>), dim3(blocks), dim3(256), 0, s, jobs, njobs);
+    return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t launch_jobs_mode(hipStream_t s, int rows, const TileJob *jobs, int njobs, unsigned blocks) {
+    switch (rows) {
+        case 2: return launch_jobs_th<2, MODE>(s, jobs, njobs, blocks);
+        case 4: return launch_jobs_th<4, MODE>(s, jobs, njobs, blocks);
+        case 6: return launch_jobs_th<6, MODE>(s, jobs, njobs, blocks);
+        default: return launch_jobs_th<8, MODE>(s, jobs, njobs, blocks);
+    }
+}
+
+hipError_t launch_jobs(hipStream_t s, int mode, int rows, const TileJob *jobs, int njobs, unsigned blocks) {
+    if (mode == TILE_DOWN) return launch_jobs_mode<TILE_DOWN>(s, rows, jobs, njobs, blocks);
+    if (mode == TILE_UP_OUTER) return launch_jobs_mode<TILE_UP_OUTER>(s, rows, jobs, njobs, blocks);
+    return launch_jobs_mode<TILE_UP>(s, rows, jobs, njobs, blocks);
 }
 
 // Runs `passes` passes on level values held in a ping-pong pair x[0], x[1] with x[*cur] current;
@@ -461,6 +489,14 @@ struct cg_group {
     std::vector<hipStream_t> streams;
     std::vector<hipEvent_t> done;  // per plan: its stream's last launch
     hipEvent_t start = nullptr;    // the context's stream after the solves' first outer steps
+    // The merged step (CG_OPT_GROUP_MERGED): one graph per binary64 parity whose tile launches each
+    // cover one depth of every plan's ladder, the job tables those launches read, and the settings
+    // they were built under.
+    hipGraphExec_t merged_exec[2] = {nullptr, nullptr};
+    TileJob *d_jobs = nullptr;
+    size_t jobs_cap = 0;
+    DirectJob *d_djobs = nullptr;  // the direct bottoms' table, per parity
+    std::vector<long> merged_key;
 };
 
 namespace {
@@ -1227,7 +1263,7 @@ int level_half_sweeps(const cg_ctx *ctx, int l) {
 inline bool direct_bottom(const cg_plan *p) { return p->ctx->direct_bottom && p->bnw > 0 && p->bottom > 0; }
 
 // Level `bottom`'s values from its forcing by the dense inverse, every instance in one launch.
-void direct_bottom_launch(cg_plan *p) {
+DirectArgs direct_args(cg_plan *p) {
     PlanLevel &B0 = p->lv[p->bottom];
     DirectArgs da{};
     da.idx = p->d_bidx;
@@ -1239,6 +1275,11 @@ void direct_bottom_launch(cg_plan *p) {
     da.f_stride = vstride(B0.st);
     da.x_stride = vstride(B0.st);
     da.active = active_flags(p);
+    return da;
+}
+
+void direct_bottom_launch(cg_plan *p) {
+    const DirectArgs da = direct_args(p);
     const unsigned rows = (p->bnw + kDirectWaves - 1) / kDirectWaves;
     if (p->K == 1)
         hipLaunchKernelGGL(k_direct_bottom<1>, dim3(rows, 1), dim3(64 * kDirectWaves), 0, p->ctx->stream, da);
@@ -1295,63 +1336,102 @@ int plan_tile_rows(const cg_plan *p, int l) {
     return p->tuned_rows[l] ? p->tuned_rows[l] : level_tile_rows(p->ctx, p->lv[l].st);
 }
 
+// One tile launch of a plan's cycle as data: what tile_dispatch takes, so that the legs of several
+// plans can also share one launch (a group's merged step, launch_jobs).
+struct LegSpec {
+    LevelArgs L;
+    const float *f = nullptr;
+    const float *xin = nullptr;
+    float *xout = nullptr;
+    TileExtra ex{};
+    int nh = 4;
+    bool zero = false;
+    int mode = TILE_PLAIN;
+    int rows = 0;
+    int K = 1;
+};
+
+hipError_t launch_spec(cg_ctx *ctx, const LegSpec &s) {
+    return tile_dispatch(ctx, s.L, s.f, s.xin, s.xout, s.nh, s.zero, s.mode, s.ex, s.rows, s.K);
+}
+
 // Level l's down leg (smooth from zero, residual, restriction into level l + 1's forcing) with
 // `rows` tile rows; level 0 reads in0 when given. Leaves the result in L.x[1 - L.cur].
-hipError_t down_leg(cg_plan *p, int l, const float *in0, int rows) {
+LegSpec down_leg_spec(cg_plan *p, int l, const float *in0, int rows) {
     PlanLevel &L = p->lv[l];
     PlanLevel &C = p->lv[l + 1];
-    TileExtra ex{};
-    ex.cwater = C.st->water;
-    ex.g = L.g;
-    ex.fc = C.f;
-    ex.fs = vstride(L.st);
-    ex.cs = vstride(C.st);
-    ex.active = active_flags(p);
+    LegSpec s;
+    s.ex.cwater = C.st->water;
+    s.ex.g = L.g;
+    s.ex.fc = C.f;
+    s.ex.fs = vstride(L.st);
+    s.ex.cs = vstride(C.st);
+    s.ex.active = active_flags(p);
     if (l == 0 && p->gate_in_down) {
-        ex.gate = gate_args(p);
-        ex.late = p->K == 1;
+        s.ex.gate = gate_args(p);
+        s.ex.late = p->K == 1;
     }
-    const float *fl = (l == 0 && in0) ? in0 : L.f;
-    return tile_dispatch(p->ctx, L.st->args(), fl, L.x[L.cur], L.x[1 - L.cur], level_half_sweeps(p->ctx, l), true,
-                         TILE_DOWN, ex, rows, p->K);
+    s.L = L.st->args();
+    s.f = (l == 0 && in0) ? in0 : L.f;
+    s.xin = L.x[L.cur];
+    s.xout = L.x[1 - L.cur];
+    s.nh = level_half_sweeps(p->ctx, l);
+    s.zero = true;
+    s.mode = TILE_DOWN;
+    s.rows = rows;
+    s.K = p->K;
+    return s;
+}
+
+hipError_t down_leg(cg_plan *p, int l, const float *in0, int rows) {
+    return launch_spec(p->ctx, down_leg_spec(p, l, in0, rows));
 }
 
 // Level l's up leg (prolongation of level l + 1's values, smoothing) with `rows` tile rows; at
 // level 0 with fuse_outer, the outer step too (TILE_UP_OUTER, see vcycle32_fused).
-hipError_t up_leg(cg_plan *p, int l, const float *in0, float *out0, bool fuse_outer, int rows) {
+LegSpec up_leg_spec(cg_plan *p, int l, const float *in0, float *out0, bool fuse_outer, int rows) {
     PlanLevel &L = p->lv[l];
     PlanLevel &C = p->lv[l + 1];
-    TileExtra ex{};
-    ex.xc = C.x[C.cur];
-    ex.ca = C.st->across;
-    ex.cd = C.st->down;
-    ex.beyond = p->pole_is_wall ? -1.0f : 1.0f;
-    ex.fs = vstride(L.st);
-    ex.cs = vstride(C.st);
-    ex.active = active_flags(p);
-    const float *fl = (l == 0 && in0) ? in0 : L.f;
-    float *xo = (l == 0 && out0) ? out0 : L.x[1 - L.cur];
-    int mode = TILE_UP;
+    LegSpec s;
+    s.ex.xc = C.x[C.cur];
+    s.ex.ca = C.st->across;
+    s.ex.cd = C.st->down;
+    s.ex.beyond = p->pole_is_wall ? -1.0f : 1.0f;
+    s.ex.fs = vstride(L.st);
+    s.ex.cs = vstride(C.st);
+    s.ex.active = active_flags(p);
+    s.mode = TILE_UP;
     if (l == 0 && fuse_outer) {
         const int q = p->x64cur;
-        mode = TILE_UP_OUTER;
-        ex.x64 = p->x64[q];
-        ex.x64n = p->x64[1 - q];
-        ex.f0 = p->f0;
-        ex.fcn = level0_forcing(p, 1 - q);
-        ex.red.out = p->d_res + q * kMaxSlots;
-        ex.red.clear = p->d_res + (1 - q) * kMaxSlots;
-        ex.s64 = L.st->cells;
-        ex.red_stride = 2 * kMaxSlots;
-        ex.sums.out = sums_group(p, q);
-        ex.sums.clear = sums_group(p, 1 - q);
-        ex.sums_stride = kSumsStride;
-        ex.want_sums = p->gated ? p->d_want : nullptr;
-        ex.active = outer_flags(p);
-        ex.late = p->K == 1;
+        s.mode = TILE_UP_OUTER;
+        s.ex.x64 = p->x64[q];
+        s.ex.x64n = p->x64[1 - q];
+        s.ex.f0 = p->f0;
+        s.ex.fcn = level0_forcing(p, 1 - q);
+        s.ex.red.out = p->d_res + q * kMaxSlots;
+        s.ex.red.clear = p->d_res + (1 - q) * kMaxSlots;
+        s.ex.s64 = L.st->cells;
+        s.ex.red_stride = 2 * kMaxSlots;
+        s.ex.sums.out = sums_group(p, q);
+        s.ex.sums.clear = sums_group(p, 1 - q);
+        s.ex.sums_stride = kSumsStride;
+        s.ex.want_sums = p->gated ? p->d_want : nullptr;
+        s.ex.active = outer_flags(p);
+        s.ex.late = p->K == 1;
     }
-    return tile_dispatch(p->ctx, L.st->args(), fl, L.x[L.cur], xo, level_half_sweeps(p->ctx, l), false, mode, ex, rows,
-                         p->K);
+    s.L = L.st->args();
+    s.f = (l == 0 && in0) ? in0 : L.f;
+    s.xin = L.x[L.cur];
+    s.xout = (l == 0 && out0) ? out0 : L.x[1 - L.cur];
+    s.nh = level_half_sweeps(p->ctx, l);
+    s.zero = false;
+    s.rows = rows;
+    s.K = p->K;
+    return s;
+}
+
+hipError_t up_leg(cg_plan *p, int l, const float *in0, float *out0, bool fuse_outer, int rows) {
+    return launch_spec(p->ctx, up_leg_spec(p, l, in0, out0, fuse_outer, rows));
 }
 
 int vcycle32_fused(cg_plan *p, bool timed, const float *in0, float *out0, bool fuse_outer) {
@@ -1795,8 +1875,11 @@ void run_finish(RefineRun &r, int launched, int status, cg_solution *infos) {
 // each batch is as long as the longest any run wants, and `step(on)` enqueues one refinement step
 // of every run k with on[k] (the runs with an instance still running). `join` orders the
 // context's stream after every launch before the gates are read. Returns the status.
+// With lockstep every run takes every step while any wants one (the merged group step launches all
+// the plans' kernels together; a run whose instances have all stopped has them skip).
 template <typename Step, typename Join>
-int run_steps(std::vector<RefineRun> &runs, const volatile int *cancel, Step step, Join join, int *status_out) {
+int run_steps(std::vector<RefineRun> &runs, const volatile int *cancel, Step step, Join join, int *status_out,
+              bool lockstep = false) {
     cg_ctx *ctx = runs[0].p->ctx;
     auto read_back = [&]() -> int {
         int rc = join();
@@ -1823,6 +1906,7 @@ int run_steps(std::vector<RefineRun> &runs, const volatile int *cancel, Step ste
             batch = std::max(batch, want);
         }
         if (batch <= 0) break;  // every instance's next gate would stop it
+        if (lockstep) std::fill(on.begin(), on.end(), 1);
         if (cancel && *cancel) {
             status = CG_CANCELLED;
             break;
@@ -2066,6 +2150,187 @@ int fgmres_solve(cg_plan *p, double tol_asked, bool warm, cg_solution *info, con
     return CG_OK;
 }
 
+// ------------------------------------------------------------------------------------------------
+// A group's merged step: every plan's refinement step in one graph, each tile launch covering the
+// grids of one depth of every plan's ladder (k_relax_tile_jobs), then every plan's direct bottom,
+// then the up legs the same way, the finest one with the binary64 outer step fused in. The plans
+// step together (run_steps' lockstep); each instance still stops at its own step by its gate.
+// ------------------------------------------------------------------------------------------------
+
+// Every plan fuses the outer step into its finest up leg, ends its cycle at the direct bottom and
+// runs two passes a leg, and the learned correction is off.
+bool group_mergeable(const cg_group *gr) {
+    const cg_ctx *ctx = gr->ctx;
+    if (!ctx->group_merged || !ctx->use_graph || ctx->time_kernels) return false;
+    for (const cg_plan *p : gr->plans) {
+        if (!outer_fused(p) || !direct_bottom(p) || p->net_on) return false;
+        for (int l = 0; l < p->bottom; ++l)
+            if (level_half_sweeps(ctx, l) != 4) return false;
+    }
+    return true;
+}
+
+// The merged launches' tile height at depth l: that of the plan with the largest grid there (its
+// tuned height, else the context's rule), within the job kernels' 2..8. Tiles only partition the
+// work, so the answers do not depend on it.
+int merged_rows(const cg_group *gr, int l) {
+    const cg_plan *best = nullptr;
+    for (const cg_plan *p : gr->plans)
+        if (p->bottom > l && (!best || p->lv[l].st->cells > best->lv[l].st->cells)) best = p;
+    const int r = best ? plan_tile_rows(best, l) : 8;
+    for (int k : kJobRows)
+        if (r <= k) return k;
+    return 8;
+}
+
+std::vector<long> merged_key(const cg_group *gr) {
+    const cg_ctx *ctx = gr->ctx;
+    std::vector<long> k{ctx->level_passes, ctx->tile_rows, ctx->fused, ctx->fuse_outer, ctx->direct_bottom,
+                        (long)std::llround(ctx->error_tol * 1e15)};
+    for (int l = 0; l < kMaxLevels; ++l) k.push_back(merged_rows(gr, l));
+    return k;
+}
+
+void drop_merged(cg_group *gr) {
+    for (hipGraphExec_t &e : gr->merged_exec) {
+        if (e) (void)hipGraphExecDestroy(e);
+        e = nullptr;
+    }
+}
+
+// Captures the merged step for both binary64 parities (after run_begin has gated every plan). The
+// job tables are filled on the host while the launches that read them are captured, and copied to
+// the device before the graphs first run.
+int group_build_merged(cg_group *gr) {
+    cg_ctx *ctx = gr->ctx;
+    const std::vector<long> key = merged_key(gr);
+    if (gr->merged_exec[0] && gr->merged_exec[1] && key == gr->merged_key) return CG_OK;
+    drop_merged(gr);
+    const size_t P = gr->plans.size();
+    int maxB = 0;
+    for (const cg_plan *p : gr->plans) maxB = std::max(maxB, p->bottom);
+    const size_t tables = 2 * 2 * (size_t)maxB;  // per parity: maxB down launches, maxB up launches
+    if (gr->jobs_cap < tables * P) {
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));  // no earlier step still reads the old tables
+        dfree(gr->d_jobs);
+        gr->d_jobs = nullptr;
+        gr->jobs_cap = 0;
+        int rc = dalloc(ctx, &gr->d_jobs, tables * P);
+        if (rc) return rc;
+        gr->jobs_cap = tables * P;
+    }
+    if (!gr->d_djobs) {
+        int rc = dalloc(ctx, &gr->d_djobs, 2 * (size_t)kMaxGroupPlans);
+        if (rc) return rc;
+    }
+    std::vector<TileJob> host(tables * P);
+    std::vector<DirectJob> dhost(2 * P);
+    for (int par = 0; par < 2; ++par) {
+        std::vector<int> saved_x64(P);
+        std::vector<std::vector<int>> saved_cur(P);
+        for (size_t k = 0; k < P; ++k) {
+            cg_plan *p = gr->plans[k];
+            saved_x64[k] = p->x64cur;
+            for (int l = 0; l < p->n; ++l) saved_cur[k].push_back(p->lv[l].cur);
+            p->x64cur = par;
+            p->gate_in_down = p->gated;
+        }
+        hipGraph_t g = nullptr;
+        HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipSuccess;
+        size_t t = (size_t)par * 2 * maxB;
+        // One launch of the given legs (each plan's instances, its own grid and ladder).
+        auto emit = [&](const std::vector<LegSpec> &specs, int rows, int mode) {
+            TileJob *tab = host.data() + t * P;
+            const TileJob *dtab = gr->d_jobs + t * P;
+            ++t;
+            const int iw = mode == TILE_DOWN ? TileGeom<4, TILE_DOWN>::IW
+                                             : (mode == TILE_UP ? TileGeom<4, TILE_UP>::IW : TileGeom<4, TILE_UP_OUTER>::IW);
+            int first = 0, n = 0;
+            for (const LegSpec &sp : specs) {
+                const long waves = (long)((sp.L.across + iw - 1) / iw) * ((sp.L.down + rows - 1) / rows);
+                const int per_inst = (int)((waves + 4 * kXcds - 1) / (4 * kXcds)) * kXcds;
+                TileJob &jb = tab[n++];
+                jb.L = sp.L;
+                jb.f = sp.f;
+                jb.xin = sp.xin;
+                jb.xout = sp.xout;
+                jb.ex = sp.ex;
+                jb.first = first;
+                jb.per_inst = per_inst;
+                jb.count = per_inst * sp.K;
+                first += jb.count;
+            }
+            if (n > 0 && e == hipSuccess) e = launch_jobs(ctx->stream, mode, rows, dtab, n, (unsigned)first);
+        };
+        for (int l = 0; l < maxB; ++l) {
+            std::vector<LegSpec> specs;
+            const int rows = merged_rows(gr, l);
+            for (cg_plan *p : gr->plans)
+                if (p->bottom > l) {
+                    specs.push_back(down_leg_spec(p, l, l == 0 ? level0_forcing(p, p->x64cur) : nullptr, rows));
+                    p->lv[l].cur = 1 - p->lv[l].cur;
+                }
+            emit(specs, rows, TILE_DOWN);
+        }
+        {
+            // Every plan's direct bottom in one launch, four instances a workgroup row.
+            DirectJob *tab = dhost.data() + (size_t)par * P;
+            int first = 0;
+            for (size_t k = 0; k < P; ++k) {
+                cg_plan *p = gr->plans[k];
+                tab[k].a = direct_args(p);
+                tab[k].first = first;
+                tab[k].rows = (p->bnw + kDirectWaves - 1) / kDirectWaves;
+                tab[k].count = tab[k].rows * ((p->K + 3) / 4);
+                first += tab[k].count;
+            }
+            if (e == hipSuccess) {
+                hipLaunchKernelGGL(k_direct_bottom_jobs<4>, dim3(first), dim3(64 * kDirectWaves), 0, ctx->stream,
+                                   gr->d_djobs + (size_t)par * P, (int)P);
+                e = hipGetLastError();
+            }
+        }
+        for (int l = maxB - 1; l >= 0; --l) {
+            std::vector<LegSpec> specs;
+            const int rows = merged_rows(gr, l);
+            for (cg_plan *p : gr->plans)
+                if (p->bottom > l) {
+                    specs.push_back(up_leg_spec(p, l, l == 0 ? level0_forcing(p, p->x64cur) : nullptr, nullptr, true,
+                                                rows));
+                    p->lv[l].cur = 1 - p->lv[l].cur;
+                }
+            emit(specs, rows, l == 0 ? TILE_UP_OUTER : TILE_UP);
+        }
+        const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+        for (size_t k = 0; k < P; ++k) {
+            cg_plan *p = gr->plans[k];
+            p->x64cur = saved_x64[k];
+            for (int l = 0; l < p->n; ++l) p->lv[l].cur = saved_cur[k][l];
+            p->gate_in_down = false;
+        }
+        if (e != hipSuccess || ec != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            set_err(ctx, "capturing the group's merged step failed: %s",
+                    hipGetErrorString(e != hipSuccess ? e : ec));
+            return CG_ERR_DEVICE;
+        }
+        e = hipGraphInstantiate(&gr->merged_exec[par], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            gr->merged_exec[par] = nullptr;
+            drop_merged(gr);
+            set_err(ctx, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+            return CG_ERR_DEVICE;
+        }
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipMemcpy(gr->d_jobs, host.data(), sizeof(TileJob) * host.size(), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(gr->d_djobs, dhost.data(), sizeof(DirectJob) * dhost.size(), hipMemcpyHostToDevice));
+    gr->merged_key = key;
+    return CG_OK;
+}
+
 }  // namespace
 
 // ================================================================================================
@@ -2182,6 +2447,7 @@ int cg_ocean_set_option(cg_ctx *ctx, int option, long value) {
         case CG_OPT_FUSE_OUTER: ctx->fuse_outer = value ? 1 : 0; return CG_OK;
         case CG_OPT_DIRECT_BOTTOM: ctx->direct_bottom = value ? 1 : 0; return CG_OK;
         case CG_OPT_NET_FALLBACK: ctx->net_fallback = value ? 1 : 0; return CG_OK;
+        case CG_OPT_GROUP_MERGED: ctx->group_merged = value ? 1 : 0; return CG_OK;
         case CG_OPT_ERROR_PICO:
             if (value < 0) {
                 set_err(ctx, "the error bound must be >= 0 (in units of 1e-12; 0: the residual alone)");
@@ -2223,6 +2489,7 @@ long cg_ocean_get_option(cg_ctx *ctx, int option) {
         case CG_OPT_DIRECT_BOTTOM: return ctx->direct_bottom;
         case CG_OPT_STOP_PERMILLE: return ctx->stop_permille;
         case CG_OPT_NET_FALLBACK: return ctx->net_fallback;
+        case CG_OPT_GROUP_MERGED: return ctx->group_merged;
         case CG_OPT_ERROR_PICO: return (long)std::llround(ctx->error_tol * 1e12);
         case CG_OPT_LEVEL_PASSES: return ctx->level_passes;
         default: return -1;
@@ -2847,6 +3114,9 @@ int cg_ocean_group_destroy(cg_group *gr) {
     }
     for (hipEvent_t e : gr->done) (void)hipEventDestroy(e);
     (void)hipEventDestroy(gr->start);
+    drop_merged(gr);
+    dfree(gr->d_jobs);
+    dfree(gr->d_djobs);
     for (cg_plan *p : gr->plans) p->group_refs--;
     delete gr;
     return CG_OK;
@@ -2887,12 +3157,27 @@ int cg_ocean_group_solve(cg_group *gr, double tol, int method, cg_solution *info
         p->x64cur = 0;
         if ((rc = run_begin(runs[k], p, tol, false, learned, nullptr, graph, learned ? 0.0 : 1.0))) return rc;
     }
-    // The plans' streams start after the first outer steps on the context's stream.
-    if (graph) {
+    // The merged step: one graph a step for every plan, on the context's stream.
+    const bool merged = graph && !learned && group_mergeable(gr);
+    if (merged && (rc = group_build_merged(gr))) return rc;
+    // Otherwise the plans' streams start after the first outer steps on the context's stream.
+    if (graph && !merged) {
         HIPCHK(ctx, hipEventRecord(gr->start, ctx->stream));
         for (hipStream_t s : gr->streams) HIPCHK(ctx, hipStreamWaitEvent(s, gr->start, 0));
     }
     auto step = [&](const std::vector<char> &on) -> int {
+        if (merged) {
+            const int q = gr->plans[0]->x64cur;
+            HIPCHK(ctx, hipGraphLaunch(gr->merged_exec[q], ctx->stream));
+            for (cg_plan *p : gr->plans) {
+                if (p->x64cur != q) {
+                    set_err(ctx, "the group's plans left their common binary64 parity");
+                    return CG_ERR_STATE;
+                }
+                p->x64cur = 1 - q;
+            }
+            return CG_OK;
+        }
         for (size_t k = 0; k < gr->plans.size(); ++k) {
             if (!on[k]) continue;
             cg_plan *p = gr->plans[k];
@@ -2908,7 +3193,7 @@ int cg_ocean_group_solve(cg_group *gr, double tol, int method, cg_solution *info
         return CG_OK;
     };
     auto join = [&]() -> int {
-        if (!graph) return CG_OK;
+        if (!graph || merged) return CG_OK;
         for (size_t k = 0; k < gr->plans.size(); ++k) {
             HIPCHK(ctx, hipEventRecord(gr->done[k], gr->streams[k]));
             HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, gr->done[k], 0));
@@ -2916,7 +3201,7 @@ int cg_ocean_group_solve(cg_group *gr, double tol, int method, cg_solution *info
         return CG_OK;
     };
     int status = CG_OK;
-    if ((rc = run_steps(runs, nullptr, step, join, &status))) return rc;
+    if ((rc = run_steps(runs, nullptr, step, join, &status, merged))) return rc;
     size_t at = 0;
     for (RefineRun &r : runs) {
         run_finish(r, r.launched, status, infos + at);

@@ -96,6 +96,11 @@ extern "C" {
 #define CG_OPT_NET_FALLBACK 13   /* 1 (default): CG_METHOD_LEARNED falls back to the plain cycle and has its
                                     answers judged by plain steps as described there; 0: neither (to
                                     measure the network alone) */
+#define CG_OPT_GROUP_MERGED 14   /* 1 (default): cg_ocean_group_solve by CG_METHOD_REFINED steps every plan of
+                                    the group together, one captured graph a step whose tile launches each
+                                    cover the grids of one depth of every plan's ladder (the leg of every
+                                    world and every month in one launch); 0: each plan's own step graph on
+                                    a stream of the group's. Both give the same bits. */
 #define CG_OPT_ERROR_PICO 12     /* ... and stop only once, besides, the estimated relative L2 error of the
                                     binary64 answer against the exact solution of the operator is at most
                                     value x 1e-12 (default 100: 1e-10; 0: the residual alone decides). The
@@ -209,12 +214,14 @@ int cg_ocean_plan_solve_batch(cg_plan *const *plans, int count, double tol, int 
 /* A group of plans of one context solved together by CG_METHOD_REFINED, from zero: the worlds of
  * uneven size of one batch, each a plan of its own ladder (batched: its months). No reference
  * counterpart (the reference solves one grid per call, OceanStage.kt:436-443). One host loop batches
- * the steps of every instance of every plan between reads of their step gates; each plan's
- * refinement step is its captured hipGraph (one launch for all its instances), launched on a stream
- * of the group's own so that the plans' steps run concurrently, and a plan whose instances have all
- * stopped gets no more launches. Each instance stops at the step it would stop at alone, with the
- * same answer to the bit. A plan may belong to several groups (at most 16 plans a group) and cannot
- * be destroyed while it does. */
+ * the steps of every instance of every plan between reads of their step gates. With
+ * CG_OPT_GROUP_MERGED (the default) the plans step together: a refinement step of the whole group
+ * is one captured hipGraph launch, each of whose tile launches covers the grids of one depth of
+ * every plan's ladder (every world, every month), then one launch for every plan's direct bottom,
+ * then the up legs the same way; otherwise each plan's own step graph is launched on a stream of
+ * the group's, and a plan whose instances have all stopped gets no more launches. Each instance
+ * stops at the step it would stop at alone, with the same answer to the bit. A plan may belong to
+ * several groups (at most 16 plans a group) and cannot be destroyed while it does. */
 typedef struct cg_group cg_group;
 int cg_ocean_group_create(cg_plan *const *plans, int count, cg_group **out);
 /* method: CG_METHOD_REFINED, or CG_METHOD_LEARNED (every plan with its coarse network set; the

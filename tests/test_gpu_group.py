@@ -1,6 +1,8 @@
-"""Plans of uneven size solved together (cg_ocean_group_*): one host loop, each plan's step graph on a
-stream of the group's own. Each instance must stop at the step it stops at alone, with the same
-answer to the bit, since the device's step gate decides per instance whatever the host batches.
+"""Plans of uneven size solved together (cg_ocean_group_*): one host loop, and either the merged step
+(CG_OPT_GROUP_MERGED, the default: one graph a step whose tile launches cover one depth of every
+plan's ladder) or each plan's step graph on a stream of the group's own. Each instance must stop at
+the step it stops at alone, with the same answer to the bit, since the device's step gate decides per
+instance whatever the host batches.
 
 - The four-world year (twelve months on each of 512x1024, 768x1536, 1024x2048 and 1024x2048, seeds
   1-4, land 20/27/33/40%): every instance of the group's solve against its plan's own solve_all.
@@ -31,7 +33,16 @@ def batched_world(acc, rows, seed, land, months=12):
     return plan
 
 
-def test_four_world_year_in_one_group_matches_each_world_alone(acc):
+@pytest.fixture(params=[1, 0], ids=["merged", "streams"])
+def merged(acc, request):
+    from cartogenesis_amd import CG_OPT_GROUP_MERGED, lib
+
+    assert lib().cg_ocean_set_option(acc.ctx, CG_OPT_GROUP_MERGED, request.param) == 0
+    yield request.param
+    assert lib().cg_ocean_set_option(acc.ctx, CG_OPT_GROUP_MERGED, 1) == 0
+
+
+def test_four_world_year_in_one_group_matches_each_world_alone(acc, merged):
     plans = [batched_world(acc, *w) for w in WORLDS]
     group = None
     try:
@@ -42,7 +53,7 @@ def test_four_world_year_in_one_group_matches_each_world_alone(acc):
             alone.append(([i.cycles for i in infos], [i.relative_residual for i in infos],
                           [plan.read(k)[0].copy() for k in range(plan.instances)]))
         group = PlanGroup(acc, plans)
-        print("OCEAN group: the four worlds together", flush=True)
+        print(f"OCEAN group: the four worlds together (merged {merged})", flush=True)
         together = group.solve(1e-8)
         for (rows, _, _), plan, infos, (cycles, res, answers) in zip(WORLDS, plans, together, alone):
             assert [i.cycles for i in infos] == cycles, rows
@@ -57,7 +68,7 @@ def test_four_world_year_in_one_group_matches_each_world_alone(acc):
             plan.close()
 
 
-def test_group_of_a_single_plan_and_batched_plans(acc):
+def test_group_of_a_single_plan_and_batched_plans(acc, merged):
     single_p = circulation_problem(256, seed=7)
     single = Plan(acc, single_p.levels)
     batched = [batched_world(acc, 128, 5, 0.3, months=3), batched_world(acc, 384, 6, 0.35, months=4)]
@@ -71,7 +82,7 @@ def test_group_of_a_single_plan_and_batched_plans(acc):
             refs.append(([i.cycles for i in infos], [plan.read(k)[0].copy() for k in range(plan.instances)]))
         group = PlanGroup(acc, [single] + batched)
         for rep in range(2):
-            print(f"OCEAN group: mixed, solve {rep}", flush=True)
+            print(f"OCEAN group: mixed, solve {rep} (merged {merged})", flush=True)
             out = group.solve(1e-10)
             assert out[0][0].cycles == ref.cycles
             assert out[0][0].relative_residual == ref.relative_residual
