@@ -656,7 +656,7 @@ def main() -> int:
     ap.add_argument("--plain-value", type=float, default=0.0, help=argparse.SUPPRESS)
     ap.add_argument("--learned-worlds", type=int, default=64,
                     help="1024x2048 training worlds of the coarse network (seeds 100, 101, ...)")
-    ap.add_argument("--learned-steps", type=int, default=1000, help="training steps (batch 16)")
+    ap.add_argument("--learned-steps", type=int, default=3000, help="training steps (batch 16)")
     ap.add_argument("--worlds", action=argparse.BooleanOptionalAction, default=True,
                     help="time the four-world year: 12 months on each of 512x1024, 768x1536, 1024x2048 and "
                          "1024x2048 (seeds 1-4, land 20/27/33/40%%) (default on)")

@@ -330,12 +330,15 @@ def loss_and_grad(out: torch.Tensor, target: torch.Tensor, water: torch.Tensor):
     return loss, dout
 
 
-def train(net: UNet, data, steps: int, batch: int = 4, lr: float = 1e-3, seed: int = 0) -> List[float]:
-    """Adam on the mean squared error; returns the loss of every step."""
+def train(net: UNet, data, steps: int, batch: int = 4, lr: float = 1e-3, seed: int = 0,
+          decay: bool = True) -> List[float]:
+    """Adam on the mean squared error, the step size decaying from lr to 0 along a cosine over the
+    steps (decay) or held at lr; returns the loss of every step."""
     x_all, t_all, w_all = data
     P = x_all.shape[0]
     params = [net.w32[k].requires_grad_(True) for k, *_ in net.layers]
     opt = torch.optim.Adam(params, lr=lr)
+    sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, T_max=max(1, steps)) if decay else None
     g = torch.Generator().manual_seed(seed)
     n = min(batch, P)
     work = torch.empty(net.workspace_floats(n, x_all.shape[1], x_all.shape[2]), dtype=torch.float32,
@@ -350,6 +353,8 @@ def train(net: UNet, data, steps: int, batch: int = 4, lr: float = 1e-3, seed: i
         for k, *_ in net.layers:
             net.w32[k].grad = grads[k]
         opt.step()
+        if sched:
+            sched.step()
         with torch.no_grad():
             net.refresh()
         losses.append(float(loss))
