@@ -394,10 +394,11 @@ int cg_conv3x3(const uint16_t *x1, int c1, const uint16_t *x2, int c2, int n, in
                const uint16_t *w, int cout, int relu, uint16_t *y1, int o1, uint16_t *y2, float *y32,
                void *stream);
 /* cg_conv3x3 of a 16-channel input whose channels 3..15 are zero (the network's input: mask, depth,
- * vector), reading channels 0..2 only, in binary32 on the vector units: values in channels 3..15 are
- * ignored, so the result equals cg_conv3x3's only when they are zero. w is [9][cout][16], cout a
- * multiple of 8 up to 32; bfloat16 output. x must be 8-byte and y 16-byte aligned (CG_ERR_ARG
- * otherwise). */
+ * vector), reading channels 0..2 only: values in channels 3..15 are ignored (never multiplied, so not
+ * even a NaN there reaches the output), and the result equals cg_conv3x3's only when they are zero.
+ * Sums in binary32: on the matrix cores (K = 9 taps x channels 0..3, channel 3 zeroed) when cout is a
+ * multiple of 16 and w 8-byte aligned, else on the vector units. w is [9][cout][16], cout a multiple
+ * of 8 up to 32; bfloat16 output. x must be 8-byte and y 16-byte aligned (CG_ERR_ARG otherwise). */
 int cg_conv3x3_in3(const uint16_t *x, int n, int down, int across, const uint16_t *w, int cout, int relu, uint16_t *y,
                    void *stream);
 /* cg_conv3x3 whose first input x1 is down/2 x across/2 and read through nearest upsampling (the
