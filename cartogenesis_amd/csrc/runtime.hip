@@ -489,13 +489,14 @@ struct cg_group {
     std::vector<hipStream_t> streams;
     std::vector<hipEvent_t> done;  // per plan: its stream's last launch
     hipEvent_t start = nullptr;    // the context's stream after the solves' first outer steps
-    // The merged step (CG_OPT_GROUP_MERGED): one graph per binary64 parity whose tile launches each
-    // cover one depth of every plan's ladder, the job tables those launches read, and the settings
-    // they were built under.
-    hipGraphExec_t merged_exec[2] = {nullptr, nullptr};
-    TileJob *d_jobs = nullptr;
-    size_t jobs_cap = 0;
-    DirectJob *d_djobs = nullptr;  // the direct bottoms' table, per parity
+    // The merged steps (CG_OPT_GROUP_MERGED), one per set of plans still running (bit k: plan k),
+    // built when a solve first needs it, and the settings they were built under.
+    struct Merged {
+        hipGraphExec_t exec[2] = {nullptr, nullptr};  // by binary64 parity
+        TileJob *d_jobs = nullptr;                    // the tile launches' job tables
+        DirectJob *d_djobs = nullptr;                 // the direct bottoms' (per parity)
+    };
+    std::map<unsigned, Merged> merged;
     std::vector<long> merged_key;
 };
 
@@ -1875,11 +1876,8 @@ void run_finish(RefineRun &r, int launched, int status, cg_solution *infos) {
 // each batch is as long as the longest any run wants, and `step(on)` enqueues one refinement step
 // of every run k with on[k] (the runs with an instance still running). `join` orders the
 // context's stream after every launch before the gates are read. Returns the status.
-// With lockstep every run takes every step while any wants one (the merged group step launches all
-// the plans' kernels together; a run whose instances have all stopped has them skip).
 template <typename Step, typename Join>
-int run_steps(std::vector<RefineRun> &runs, const volatile int *cancel, Step step, Join join, int *status_out,
-              bool lockstep = false) {
+int run_steps(std::vector<RefineRun> &runs, const volatile int *cancel, Step step, Join join, int *status_out) {
     cg_ctx *ctx = runs[0].p->ctx;
     auto read_back = [&]() -> int {
         int rc = join();
@@ -1906,7 +1904,6 @@ int run_steps(std::vector<RefineRun> &runs, const volatile int *cancel, Step ste
             batch = std::max(batch, want);
         }
         if (batch <= 0) break;  // every instance's next gate would stop it
-        if (lockstep) std::fill(on.begin(), on.end(), 1);
         if (cancel && *cancel) {
             status = CG_CANCELLED;
             break;
@@ -2151,10 +2148,12 @@ int fgmres_solve(cg_plan *p, double tol_asked, bool warm, cg_solution *info, con
 }
 
 // ------------------------------------------------------------------------------------------------
-// A group's merged step: every plan's refinement step in one graph, each tile launch covering the
-// grids of one depth of every plan's ladder (k_relax_tile_jobs), then every plan's direct bottom,
-// then the up legs the same way, the finest one with the binary64 outer step fused in. The plans
-// step together (run_steps' lockstep); each instance still stops at its own step by its gate.
+// A group's merged step: the refinement step of every plan still running in one graph, each tile
+// launch covering the grids of one depth of those plans' ladders (k_relax_tile_jobs), then every
+// plan's direct bottom in one launch, then the up legs the same way, the finest one with the binary64
+// outer step fused in. One graph per set of running plans (a plan whose instances have all stopped
+// drops out, as it gets no more launches on the streams path); each instance stops at its own step
+// by its gate.
 // ------------------------------------------------------------------------------------------------
 
 // Every plan fuses the outer step into its finest up leg, ends its cycle at the direct bottom and
@@ -2170,8 +2169,8 @@ bool group_mergeable(const cg_group *gr) {
     return true;
 }
 
-// The merged launches' tile height at depth l: that of the plan with the largest grid there (its
-// tuned height, else the context's rule), within the job kernels' 2..8. Tiles only partition the
+// The merged launches' tile height at depth l: that of the group's plan with the largest grid there
+// (its tuned height, else the context's rule), within the job kernels' 2..8. Tiles only partition the
 // work, so the answers do not depend on it.
 int merged_rows(const cg_group *gr, int l) {
     const cg_plan *best = nullptr;
@@ -2192,57 +2191,69 @@ std::vector<long> merged_key(const cg_group *gr) {
 }
 
 void drop_merged(cg_group *gr) {
-    for (hipGraphExec_t &e : gr->merged_exec) {
-        if (e) (void)hipGraphExecDestroy(e);
-        e = nullptr;
+    for (auto &kv : gr->merged) {
+        for (hipGraphExec_t &e : kv.second.exec)
+            if (e) (void)hipGraphExecDestroy(e);
+        dfree(kv.second.d_jobs);
+        dfree(kv.second.d_djobs);
     }
+    gr->merged.clear();
 }
 
-// Captures the merged step for both binary64 parities (after run_begin has gated every plan). The
-// job tables are filled on the host while the launches that read them are captured, and copied to
-// the device before the graphs first run.
-int group_build_merged(cg_group *gr) {
+// The merged step of the plans in `mask` (after run_begin has gated every plan, with the stream idle):
+// built for both binary64 parities on first use, the job tables filled on the host while the launches
+// that read them are captured and copied to the device before the graphs first run.
+int merged_for(cg_group *gr, unsigned mask, cg_group::Merged **out) {
     cg_ctx *ctx = gr->ctx;
-    const std::vector<long> key = merged_key(gr);
-    if (gr->merged_exec[0] && gr->merged_exec[1] && key == gr->merged_key) return CG_OK;
-    drop_merged(gr);
-    const size_t P = gr->plans.size();
+    auto it = gr->merged.find(mask);
+    if (it != gr->merged.end()) {
+        *out = &it->second;
+        return CG_OK;
+    }
+    std::vector<cg_plan *> plans;
+    for (size_t k = 0; k < gr->plans.size(); ++k)
+        if (mask >> k & 1u) plans.push_back(gr->plans[k]);
+    const size_t P = plans.size();
     int maxB = 0;
-    for (const cg_plan *p : gr->plans) maxB = std::max(maxB, p->bottom);
+    for (const cg_plan *p : plans) maxB = std::max(maxB, p->bottom);
     const size_t tables = 2 * 2 * (size_t)maxB;  // per parity: maxB down launches, maxB up launches
-    if (gr->jobs_cap < tables * P) {
-        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));  // no earlier step still reads the old tables
-        dfree(gr->d_jobs);
-        gr->d_jobs = nullptr;
-        gr->jobs_cap = 0;
-        int rc = dalloc(ctx, &gr->d_jobs, tables * P);
-        if (rc) return rc;
-        gr->jobs_cap = tables * P;
+    cg_group::Merged m;
+    int rc = dalloc(ctx, &m.d_jobs, tables * P);
+    if (!rc) rc = dalloc(ctx, &m.d_djobs, 2 * P);
+    if (rc) {
+        dfree(m.d_jobs);
+        return rc;
     }
-    if (!gr->d_djobs) {
-        int rc = dalloc(ctx, &gr->d_djobs, 2 * (size_t)kMaxGroupPlans);
-        if (rc) return rc;
-    }
+    auto fail = [&](int code) {
+        for (hipGraphExec_t &e : m.exec)
+            if (e) (void)hipGraphExecDestroy(e);
+        dfree(m.d_jobs);
+        dfree(m.d_djobs);
+        return code;
+    };
     std::vector<TileJob> host(tables * P);
     std::vector<DirectJob> dhost(2 * P);
     for (int par = 0; par < 2; ++par) {
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            set_err(ctx, "hipStreamBeginCapture failed");
+            return fail(CG_ERR_DEVICE);
+        }
         std::vector<int> saved_x64(P);
         std::vector<std::vector<int>> saved_cur(P);
         for (size_t k = 0; k < P; ++k) {
-            cg_plan *p = gr->plans[k];
+            cg_plan *p = plans[k];
             saved_x64[k] = p->x64cur;
             for (int l = 0; l < p->n; ++l) saved_cur[k].push_back(p->lv[l].cur);
             p->x64cur = par;
             p->gate_in_down = p->gated;
         }
-        hipGraph_t g = nullptr;
-        HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         hipError_t e = hipSuccess;
         size_t t = (size_t)par * 2 * maxB;
         // One launch of the given legs (each plan's instances, its own grid and ladder).
         auto emit = [&](const std::vector<LegSpec> &specs, int rows, int mode) {
             TileJob *tab = host.data() + t * P;
-            const TileJob *dtab = gr->d_jobs + t * P;
+            const TileJob *dtab = m.d_jobs + t * P;
             ++t;
             const int iw = mode == TILE_DOWN ? TileGeom<4, TILE_DOWN>::IW
                                              : (mode == TILE_UP ? TileGeom<4, TILE_UP>::IW : TileGeom<4, TILE_UP_OUTER>::IW);
@@ -2266,7 +2277,7 @@ int group_build_merged(cg_group *gr) {
         for (int l = 0; l < maxB; ++l) {
             std::vector<LegSpec> specs;
             const int rows = merged_rows(gr, l);
-            for (cg_plan *p : gr->plans)
+            for (cg_plan *p : plans)
                 if (p->bottom > l) {
                     specs.push_back(down_leg_spec(p, l, l == 0 ? level0_forcing(p, p->x64cur) : nullptr, rows));
                     p->lv[l].cur = 1 - p->lv[l].cur;
@@ -2278,7 +2289,7 @@ int group_build_merged(cg_group *gr) {
             DirectJob *tab = dhost.data() + (size_t)par * P;
             int first = 0;
             for (size_t k = 0; k < P; ++k) {
-                cg_plan *p = gr->plans[k];
+                cg_plan *p = plans[k];
                 tab[k].a = direct_args(p);
                 tab[k].first = first;
                 tab[k].rows = (p->bnw + kDirectWaves - 1) / kDirectWaves;
@@ -2287,14 +2298,14 @@ int group_build_merged(cg_group *gr) {
             }
             if (e == hipSuccess) {
                 hipLaunchKernelGGL(k_direct_bottom_jobs<4>, dim3(first), dim3(64 * kDirectWaves), 0, ctx->stream,
-                                   gr->d_djobs + (size_t)par * P, (int)P);
+                                   m.d_djobs + (size_t)par * P, (int)P);
                 e = hipGetLastError();
             }
         }
         for (int l = maxB - 1; l >= 0; --l) {
             std::vector<LegSpec> specs;
             const int rows = merged_rows(gr, l);
-            for (cg_plan *p : gr->plans)
+            for (cg_plan *p : plans)
                 if (p->bottom > l) {
                     specs.push_back(up_leg_spec(p, l, l == 0 ? level0_forcing(p, p->x64cur) : nullptr, nullptr, true,
                                                 rows));
@@ -2304,7 +2315,7 @@ int group_build_merged(cg_group *gr) {
         }
         const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
         for (size_t k = 0; k < P; ++k) {
-            cg_plan *p = gr->plans[k];
+            cg_plan *p = plans[k];
             p->x64cur = saved_x64[k];
             for (int l = 0; l < p->n; ++l) p->lv[l].cur = saved_cur[k][l];
             p->gate_in_down = false;
@@ -2313,21 +2324,22 @@ int group_build_merged(cg_group *gr) {
             if (g) (void)hipGraphDestroy(g);
             set_err(ctx, "capturing the group's merged step failed: %s",
                     hipGetErrorString(e != hipSuccess ? e : ec));
-            return CG_ERR_DEVICE;
+            return fail(CG_ERR_DEVICE);
         }
-        e = hipGraphInstantiate(&gr->merged_exec[par], g, nullptr, nullptr, 0);
+        e = hipGraphInstantiate(&m.exec[par], g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (e != hipSuccess) {
-            gr->merged_exec[par] = nullptr;
-            drop_merged(gr);
+            m.exec[par] = nullptr;
             set_err(ctx, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
-            return CG_ERR_DEVICE;
+            return fail(CG_ERR_DEVICE);
         }
     }
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    HIPCHK(ctx, hipMemcpy(gr->d_jobs, host.data(), sizeof(TileJob) * host.size(), hipMemcpyHostToDevice));
-    HIPCHK(ctx, hipMemcpy(gr->d_djobs, dhost.data(), sizeof(DirectJob) * dhost.size(), hipMemcpyHostToDevice));
-    gr->merged_key = key;
+    if (hipMemcpy(m.d_jobs, host.data(), sizeof(TileJob) * host.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(m.d_djobs, dhost.data(), sizeof(DirectJob) * dhost.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        set_err(ctx, "copying the merged step's job tables failed");
+        return fail(CG_ERR_DEVICE);
+    }
+    *out = &(gr->merged[mask] = m);
     return CG_OK;
 }
 
@@ -3115,8 +3127,6 @@ int cg_ocean_group_destroy(cg_group *gr) {
     for (hipEvent_t e : gr->done) (void)hipEventDestroy(e);
     (void)hipEventDestroy(gr->start);
     drop_merged(gr);
-    dfree(gr->d_jobs);
-    dfree(gr->d_djobs);
     for (cg_plan *p : gr->plans) p->group_refs--;
     delete gr;
     return CG_OK;
@@ -3159,7 +3169,13 @@ int cg_ocean_group_solve(cg_group *gr, double tol, int method, cg_solution *info
     }
     // The merged step: one graph a step for every plan, on the context's stream.
     const bool merged = graph && !learned && group_mergeable(gr);
-    if (merged && (rc = group_build_merged(gr))) return rc;
+    if (merged) {
+        const std::vector<long> key = merged_key(gr);
+        if (key != gr->merged_key) {
+            drop_merged(gr);
+            gr->merged_key = key;
+        }
+    }
     // Otherwise the plans' streams start after the first outer steps on the context's stream.
     if (graph && !merged) {
         HIPCHK(ctx, hipEventRecord(gr->start, ctx->stream));
@@ -3167,15 +3183,26 @@ int cg_ocean_group_solve(cg_group *gr, double tol, int method, cg_solution *info
     }
     auto step = [&](const std::vector<char> &on) -> int {
         if (merged) {
-            const int q = gr->plans[0]->x64cur;
-            HIPCHK(ctx, hipGraphLaunch(gr->merged_exec[q], ctx->stream));
-            for (cg_plan *p : gr->plans) {
-                if (p->x64cur != q) {
-                    set_err(ctx, "the group's plans left their common binary64 parity");
+            // The plans still running step together; they share a binary64 parity (every plan starts
+            // at the same, and one that drops out never returns within the solve).
+            unsigned mask = 0;
+            int q = -1;
+            for (size_t k = 0; k < gr->plans.size(); ++k) {
+                if (!on[k]) continue;
+                mask |= 1u << k;
+                if (q >= 0 && gr->plans[k]->x64cur != q) {
+                    set_err(ctx, "the group's running plans left their common binary64 parity");
                     return CG_ERR_STATE;
                 }
-                p->x64cur = 1 - q;
+                q = gr->plans[k]->x64cur;
             }
+            if (!mask) return CG_OK;
+            cg_group::Merged *ms = nullptr;
+            int r = merged_for(gr, mask, &ms);
+            if (r) return r;
+            HIPCHK(ctx, hipGraphLaunch(ms->exec[q], ctx->stream));
+            for (size_t k = 0; k < gr->plans.size(); ++k)
+                if (on[k]) gr->plans[k]->x64cur = 1 - q;
             return CG_OK;
         }
         for (size_t k = 0; k < gr->plans.size(); ++k) {
@@ -3201,7 +3228,7 @@ int cg_ocean_group_solve(cg_group *gr, double tol, int method, cg_solution *info
         return CG_OK;
     };
     int status = CG_OK;
-    if ((rc = run_steps(runs, nullptr, step, join, &status, merged))) return rc;
+    if ((rc = run_steps(runs, nullptr, step, join, &status))) return rc;
     size_t at = 0;
     for (RefineRun &r : runs) {
         run_finish(r, r.launched, status, infos + at);

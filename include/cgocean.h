@@ -215,11 +215,11 @@ int cg_ocean_plan_solve_batch(cg_plan *const *plans, int count, double tol, int 
  * uneven size of one batch, each a plan of its own ladder (batched: its months). No reference
  * counterpart (the reference solves one grid per call, OceanStage.kt:436-443). One host loop batches
  * the steps of every instance of every plan between reads of their step gates. With
- * CG_OPT_GROUP_MERGED (the default) the plans step together: a refinement step of the whole group
- * is one captured hipGraph launch, each of whose tile launches covers the grids of one depth of
- * every plan's ladder (every world, every month), then one launch for every plan's direct bottom,
- * then the up legs the same way; otherwise each plan's own step graph is launched on a stream of
- * the group's, and a plan whose instances have all stopped gets no more launches. Each instance
+ * CG_OPT_GROUP_MERGED (the default) the plans still running step together: a refinement step of
+ * them is one captured hipGraph launch, each of whose tile launches covers the grids of one depth of
+ * every such plan's ladder (every world, every month), then one launch for every plan's direct
+ * bottom, then the up legs the same way; otherwise each plan's own step graph is launched on a
+ * stream of the group's. Either way a plan whose instances have all stopped gets no more launches. Each instance
  * stops at the step it would stop at alone, with the same answer to the bit. A plan may belong to
  * several groups (at most 16 plans a group) and cannot be destroyed while it does. */
 typedef struct cg_group cg_group;

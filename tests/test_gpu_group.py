@@ -1,6 +1,6 @@
 """Plans of uneven size solved together (cg_ocean_group_*): one host loop, and either the merged step
 (CG_OPT_GROUP_MERGED, the default: one graph a step whose tile launches cover one depth of every
-plan's ladder) or each plan's step graph on a stream of the group's own. Each instance must stop at
+running plan's ladder) or each plan's step graph on a stream of the group's own. Each instance must stop at
 the step it stops at alone, with the same answer to the bit, since the device's step gate decides per
 instance whatever the host batches.
 
