@@ -552,8 +552,10 @@ class BatchPlan:
 class PlanGroup:
     """Plans of one accelerator solved together by the refined method from zero (cg_ocean_group_*):
     the worlds of uneven size of one batch, each a Plan or a BatchPlan of its own ladder, stepped by
-    one host loop, each plan's step graph on a stream of the group's own. Each instance stops at the
-    step, and with the answer, it would reach alone. Close the group before its plans."""
+    one host loop: by default (CG_OPT_GROUP_MERGED) one graph launch a step for the plans still
+    running, each kernel launch covering one depth of every such plan's ladder; else each plan's step
+    graph on a stream of the group's own. Each instance stops at the step, and with the answer, it
+    would reach alone. Close the group before its plans."""
 
     def __init__(self, acc: HipOcean, plans):
         self.acc = acc
