@@ -30,6 +30,7 @@ constexpr int kPrecondCoarsestPasses = 200;  // OC:629
 constexpr int kKrylovRestarts = 3;        // OC:535
 constexpr int kMaxInstances = 64;          // right-hand sides one plan solves together
 constexpr int kMaxGroupPlans = 16;         // plans one group solves together
+constexpr size_t kMaxMergedSets = 32;       // merged step graphs a group keeps (sets of running plans)
 constexpr int kDirectMaxWater = 2048;     // largest coarsest grid solved by a dense inverse
 constexpr int kRefinedMostSteps = 1000;
 // The learned correction's fallback: a residual falling by less than this factor a step.
@@ -2209,6 +2210,12 @@ int merged_for(cg_group *gr, unsigned mask, cg_group::Merged **out) {
     if (it != gr->merged.end()) {
         *out = &it->second;
         return CG_OK;
+    }
+    // The sets of running plans that solves meet are few (each solve's sets shrink one plan at a
+    // time); a cache that has grown past kMaxMergedSets is emptied, with no step of it in flight.
+    if (gr->merged.size() >= kMaxMergedSets) {
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        drop_merged(gr);
     }
     std::vector<cg_plan *> plans;
     for (size_t k = 0; k < gr->plans.size(); ++k)
