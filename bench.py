@@ -174,7 +174,7 @@ def batched_year(acc, prob, tol: float, reps: int, warmup: int, autotune: bool =
     }
 
 
-def uneven_batches(acc, specs, instances: int, tol: float):
+def uneven_batches(acc, specs, instances: int, tol: float, autotune: bool = True):
     """Batched plans of several worlds of uneven size, each plan holding `instances` monthly
     forcings of its world, timed two ways: the plans one after another on one context, and each
     plan on a context (a HIP stream) of its own, solved by its own host thread, so that the worlds'
@@ -194,7 +194,8 @@ def uneven_batches(acc, specs, instances: int, tol: float):
         plan = BatchPlan(ctx_acc, prob.levels, instances)
         for i, f in enumerate(forcings):
             plan.set_forcing(i, f)
-        plan.autotune(3)
+        if autotune:
+            plan.autotune(3)
         plan.solve_all(tol)  # warm
         return plan
 
@@ -269,11 +270,11 @@ def worst_of(worst: float, infos) -> float:
     return worst
 
 
-def batched_worlds(acc, tol: float):
+def batched_worlds(acc, tol: float, autotune: bool = True):
     """Four worlds of uneven size, twelve months each (one batched plan of twelve per world)."""
     log("bench: four-world year")
     specs = [(512, 1, 0.20), (768, 2, 0.27), (1024, 3, 0.33), (1024, 4, 0.40)]
-    seq, conc, grouped, per, cycles, worst = uneven_batches(acc, specs, 12, tol)
+    seq, conc, grouped, per, cycles, worst = uneven_batches(acc, specs, 12, tol, autotune)
     log(f"bench: four-world year {seq:.1f} solves/s one world after another, {conc:.1f} with the worlds concurrent, "
         f"{grouped[0]:.1f} as one group (merged step), {grouped[1]:.1f} as one group (a stream per world)")
     return {"workload": "12 months x worlds 512x1024, 768x1536, 1024x2048, 1024x2048 (seeds 1-4, land 20/27/33/40%)",
@@ -285,12 +286,12 @@ def batched_worlds(acc, tol: float):
             "ms_per_world_one_at_a_time": per, "cycles_min_max": cycles, "max_relative_residual": worst}
 
 
-def batched_combined(acc, tol: float):
+def batched_combined(acc, tol: float, autotune: bool = True):
     """64 instances of uneven size: 16 each at 256x512, 512x1024, 1024x2048 and 2048x4096 (months
     0..11 then 0..3), one batched plan of 16 per size."""
     log("bench: combined 64 instances")
     specs = [(256, 5, 0.3), (512, 6, 0.3), (1024, 7, 0.3), (2048, 8, 0.3)]
-    seq, conc, grouped, per, cycles, worst = uneven_batches(acc, specs, 16, tol)
+    seq, conc, grouped, per, cycles, worst = uneven_batches(acc, specs, 16, tol, autotune)
     log(f"bench: combined 64 instances {seq:.1f} solves/s one size after another, {conc:.1f} concurrent, "
         f"{grouped[0]:.1f} as one group (merged step), {grouped[1]:.1f} as one group (a stream per size)")
     return {"workload": "64 instances: 16 monthly forcings each at 256x512, 512x1024, 1024x2048, 2048x4096",
@@ -864,8 +865,8 @@ def main() -> int:
         if proc.returncode != 0:
             raise RuntimeError(f"the learned leg failed with status {proc.returncode}")
         learned = json.loads(proc.stdout.strip().splitlines()[-1])
-    worlds = batched_worlds(acc, args.tol) if args.worlds else None
-    combined = batched_combined(acc, args.tol) if args.combined else None
+    worlds = batched_worlds(acc, args.tol, not args.no_autotune) if args.worlds else None
+    combined = batched_combined(acc, args.tol, not args.no_autotune) if args.combined else None
 
     err = None
     vs_ref = None
