@@ -58,7 +58,7 @@ extern "C" {
  * (cg_ocean_plan_set_coarse_net): down to the network's grid as usual, then the network's correction
  * for that grid's forcing in place of the grids below it, then up as usual; every step one captured
  * hipGraph, batched plans included. The learned steps stop once the residual is under tol, or as a
- * fallback once it falls by less than a factor 0.8 a step (cg_ocean_plan_net_fallbacks counts those).
+ * fallback once it falls by less than a factor 0.6 a step (cg_ocean_plan_net_fallbacks counts those).
  * Plain multigrid then judges every answer: each instance goes on from it with the plain cycle, at
  * least three steps and until CG_METHOD_REFINED's residual and error rule holds with half the error
  * bound (an answer that is not
