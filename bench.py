@@ -601,6 +601,7 @@ def live_traffic(rows0: int, rows: str):
         except subprocess.TimeoutExpired:
             os.killpg(proc.pid, signal.SIGKILL)
             proc.wait()
+            shutil.rmtree(d, ignore_errors=True)
             return None, None, f"the {counter} pass timed out"
         disp = pmc_dispatches(d, counter)
         shutil.rmtree(d, ignore_errors=True)
