@@ -131,8 +131,10 @@ struct cg_ctx {
     // is at most error_tol (0: the residual alone decides).
     double error_tol = 1e-10;
     // Passes per level of the fused binary32 cycle, 4 bits per level from level 0 up (2, 4 or 6;
-    // levels past the eighth and the bottom's run 2).
-    long level_passes = 0x22222222;
+    // levels past the eighth and the bottom's run 2). Four on the three grids below the finest: at
+    // 1024 x 2048 the refinement takes 15 steps instead of 19 for 4-5 us more a step (823 against
+    // 750 solves/s); six passes cost far more than they save.
+    long level_passes = 0x22224442;
     size_t lds_limit = 0;       // dynamic LDS a workgroup may use (k_bottom)
     bool bottom_attr_set = false;
     std::map<uint64_t, Stencil *> stencils;
@@ -295,31 +297,42 @@ hipError_t tile_dispatch(cg_ctx *ctx, const LevelArgs &L, const float *f, const 
     }
 }
 
-// Launches of k_relax_tile_jobs: the legs of several plans (one mode, tile height and two passes for
-// all) in one launch, their job table in device memory. Tile heights 2, 4, 6 and 8.
+// Launches of k_relax_tile_jobs: the legs of several plans (one mode, tile height and pass count for
+// all) in one launch, their job table in device memory. Tile heights 2, 4, 6 and 8; two or four
+// passes (the outer step's leg: two).
 constexpr int kJobRows[4] = {2, 4, 6, 8};
 
-template <int TH, int MODE>
+template <int TH, int MODE, int NH>
 hipError_t launch_jobs_th(hipStream_t s, const TileJob *jobs, int njobs, unsigned blocks) {
     constexpr bool zero = MODE == TILE_DOWN;
-    hipLaunchKernelGGL((k_relax_tile_jobs<4, TH, zero, MODE>), dim3(blocks), dim3(256), 0, s, jobs, njobs);
+    hipLaunchKernelGGL((k_relax_tile_jobs<NH, TH, zero, MODE>), dim3(blocks), dim3(256), 0, s, jobs, njobs);
     return hipGetLastError();
 }
 
-template <int MODE>
+template <int MODE, int NH>
 hipError_t launch_jobs_mode(hipStream_t s, int rows, const TileJob *jobs, int njobs, unsigned blocks) {
     switch (rows) {
-        case 2: return launch_jobs_th<2, MODE>(s, jobs, njobs, blocks);
-        case 4: return launch_jobs_th<4, MODE>(s, jobs, njobs, blocks);
-        case 6: return launch_jobs_th<6, MODE>(s, jobs, njobs, blocks);
-        default: return launch_jobs_th<8, MODE>(s, jobs, njobs, blocks);
+        case 2: return launch_jobs_th<2, MODE, NH>(s, jobs, njobs, blocks);
+        case 4: return launch_jobs_th<4, MODE, NH>(s, jobs, njobs, blocks);
+        case 6: return launch_jobs_th<6, MODE, NH>(s, jobs, njobs, blocks);
+        default: return launch_jobs_th<8, MODE, NH>(s, jobs, njobs, blocks);
     }
 }
 
-hipError_t launch_jobs(hipStream_t s, int mode, int rows, const TileJob *jobs, int njobs, unsigned blocks) {
-    if (mode == TILE_DOWN) return launch_jobs_mode<TILE_DOWN>(s, rows, jobs, njobs, blocks);
-    if (mode == TILE_UP_OUTER) return launch_jobs_mode<TILE_UP_OUTER>(s, rows, jobs, njobs, blocks);
-    return launch_jobs_mode<TILE_UP>(s, rows, jobs, njobs, blocks);
+hipError_t launch_jobs(hipStream_t s, int mode, int nh, int rows, const TileJob *jobs, int njobs, unsigned blocks) {
+    if (mode == TILE_UP_OUTER) return launch_jobs_mode<TILE_UP_OUTER, 4>(s, rows, jobs, njobs, blocks);
+    if (mode == TILE_DOWN)
+        return nh == 8 ? launch_jobs_mode<TILE_DOWN, 8>(s, rows, jobs, njobs, blocks)
+                       : launch_jobs_mode<TILE_DOWN, 4>(s, rows, jobs, njobs, blocks);
+    return nh == 8 ? launch_jobs_mode<TILE_UP, 8>(s, rows, jobs, njobs, blocks)
+                   : launch_jobs_mode<TILE_UP, 4>(s, rows, jobs, njobs, blocks);
+}
+
+// The interior width of a tile of the given mode and half-sweeps (TileGeom's IW).
+int jobs_tile_width(int mode, int nh) {
+    if (mode == TILE_UP_OUTER) return TileGeom<4, TILE_UP_OUTER>::IW;
+    if (mode == TILE_DOWN) return nh == 8 ? TileGeom<8, TILE_DOWN>::IW : TileGeom<4, TILE_DOWN>::IW;
+    return nh == 8 ? TileGeom<8, TILE_UP>::IW : TileGeom<4, TILE_UP>::IW;
 }
 
 // Runs `passes` passes on level values held in a ping-pong pair x[0], x[1] with x[*cur] current;
@@ -2160,15 +2173,16 @@ int fgmres_solve(cg_plan *p, double tol_asked, bool warm, cg_solution *info, con
 // by its gate.
 // ------------------------------------------------------------------------------------------------
 
-// Every plan fuses the outer step into its finest up leg, ends its cycle at the direct bottom and
-// runs two passes a leg, and the learned correction is off.
+// Every plan fuses the outer step into its finest up leg (two passes there), ends its cycle at the
+// direct bottom and runs two or four passes a leg on its other grids, and the learned correction is
+// off.
 bool group_mergeable(const cg_group *gr) {
     const cg_ctx *ctx = gr->ctx;
     if (!ctx->group_merged || !ctx->use_graph || ctx->time_kernels) return false;
     for (const cg_plan *p : gr->plans) {
         if (!outer_fused(p) || !direct_bottom(p) || p->net_on) return false;
-        for (int l = 0; l < p->bottom; ++l)
-            if (level_half_sweeps(ctx, l) != 4) return false;
+        for (int l = 1; l < p->bottom; ++l)
+            if (level_half_sweeps(ctx, l) != 4 && level_half_sweeps(ctx, l) != 8) return false;
     }
     return true;
 }
@@ -2261,12 +2275,11 @@ int merged_for(cg_group *gr, unsigned mask, cg_group::Merged **out) {
         hipError_t e = hipSuccess;
         size_t t = (size_t)par * 2 * maxB;
         // One launch of the given legs (each plan's instances, its own grid and ladder).
-        auto emit = [&](const std::vector<LegSpec> &specs, int rows, int mode) {
+        auto emit = [&](const std::vector<LegSpec> &specs, int rows, int mode, int nh) {
             TileJob *tab = host.data() + t * P;
             const TileJob *dtab = m.d_jobs + t * P;
             ++t;
-            const int iw = mode == TILE_DOWN ? TileGeom<4, TILE_DOWN>::IW
-                                             : (mode == TILE_UP ? TileGeom<4, TILE_UP>::IW : TileGeom<4, TILE_UP_OUTER>::IW);
+            const int iw = jobs_tile_width(mode, nh);
             int first = 0, n = 0;
             for (const LegSpec &sp : specs) {
                 const long waves = (long)((sp.L.across + iw - 1) / iw) * ((sp.L.down + rows - 1) / rows);
@@ -2282,7 +2295,7 @@ int merged_for(cg_group *gr, unsigned mask, cg_group::Merged **out) {
                 jb.count = per_inst * sp.K;
                 first += jb.count;
             }
-            if (n > 0 && e == hipSuccess) e = launch_jobs(ctx->stream, mode, rows, dtab, n, (unsigned)first);
+            if (n > 0 && e == hipSuccess) e = launch_jobs(ctx->stream, mode, nh, rows, dtab, n, (unsigned)first);
         };
         for (int l = 0; l < maxB; ++l) {
             std::vector<LegSpec> specs;
@@ -2292,7 +2305,7 @@ int merged_for(cg_group *gr, unsigned mask, cg_group::Merged **out) {
                     specs.push_back(down_leg_spec(p, l, l == 0 ? level0_forcing(p, p->x64cur) : nullptr, rows));
                     p->lv[l].cur = 1 - p->lv[l].cur;
                 }
-            emit(specs, rows, TILE_DOWN);
+            emit(specs, rows, TILE_DOWN, level_half_sweeps(ctx, l));
         }
         {
             // Every plan's direct bottom in one launch, four instances a workgroup row.
@@ -2321,7 +2334,7 @@ int merged_for(cg_group *gr, unsigned mask, cg_group::Merged **out) {
                                                 rows));
                     p->lv[l].cur = 1 - p->lv[l].cur;
                 }
-            emit(specs, rows, l == 0 ? TILE_UP_OUTER : TILE_UP);
+            emit(specs, rows, l == 0 ? TILE_UP_OUTER : TILE_UP, level_half_sweeps(ctx, l));
         }
         const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
         for (size_t k = 0; k < P; ++k) {
@@ -3596,6 +3609,18 @@ double cg_ocean_plan_time_kernel(cg_plan *p, int kind, int reps) {
     }
     PlanLevel &L = p->lv[0];
     const int nh = level_half_sweeps(ctx, 0), th = plan_tile_rows(p, 0);
+    // The fused kinds' binary64 output goes to scratch: a batched plan's instances keep their answers
+    // in the parity where each stopped, so neither parity is free of answers.
+    double *x64_out = p->r64;
+    struct Tmp {
+        double *q = nullptr;
+        ~Tmp() { dfree(q); }
+    } tmp;
+    if (kind == 3 && instances > 1) {
+        int rc = dalloc(ctx, &tmp.q, (size_t)L.st->cells * instances);
+        if (rc) return -1.0;
+        x64_out = tmp.q;
+    }
     auto launch = [&]() -> hipError_t {
         if (kind == 4) {
             bottom_launch(p, nullptr, nullptr);
@@ -3624,11 +3649,11 @@ double cg_ocean_plan_time_kernel(cg_plan *p, int kind, int reps) {
         ex.cd = C.st->down;
         ex.beyond = p->pole_is_wall ? -1.0f : 1.0f;
         if (kind == 3) {
-            // The fused finest up leg and outer step: reads parity q, writes parity 1 - q (not the
-            // answer's), its maxima into the scratch slot groups 3 (out) and 2 (cleared).
+            // The fused finest up leg and outer step: reads parity q, writes the binary64 answer to
+            // scratch, its maxima into the scratch slot groups 3 (out) and 2 (cleared).
             const int q = p->x64cur;
             ex.x64 = p->x64[q];
-            ex.x64n = p->x64[1 - q];
+            ex.x64n = x64_out;
             ex.f0 = p->f0;
             ex.fcn = level0_forcing(p, 1 - q);
             ex.red.out = p->d_u64 + 3 * kMaxSlots;

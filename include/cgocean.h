@@ -78,8 +78,10 @@ extern "C" {
 #define CG_OPT_FGMRES_RESTART 4  /* restart length m of CG_METHOD_FGMRES (1..64), default 20 */
 #define CG_OPT_FUSED 7           /* 1 (default): fused cycle (one launch per leg per grid, small grids in
                                     one workgroup); 0: separate smoothing/residual/transfer kernels */
-#define CG_OPT_LEVEL_PASSES 8    /* passes per grid of the binary32 cycle, one hex digit per grid from the
-                                    finest (lowest digit) up, each 2, 4 or 6; default 0x22222222 */
+#define CG_OPT_LEVEL_PASSES 8    /* passes per grid of the fused binary32 cycle (the unfused one runs two
+                                    on every grid), one hex digit per grid from the
+                                    finest (lowest digit) up, each 2, 4 or 6; default 0x22224442 (two on
+                                    the finest grid, four on the three below it, two below those) */
 #define CG_OPT_FUSE_OUTER 9      /* 1 (default): CG_METHOD_REFINED runs the binary64 outer step inside the
                                     finest grid's up-leg launch (fused cycle, two passes there); 0: as a
                                     launch of its own. Both give the same bits. */

@@ -128,16 +128,21 @@ def test_stommel_basin_on_the_device(acc, aspect):
 def test_fused_and_unfused_cycles_reach_the_oracle(acc, rows):
     """The fused cycle (one down and one up launch per grid, the bottom grids in one workgroup) and
     the unfused one (separate smoothing, residual, restriction and prolongation kernels) both reach
-    the binary64 oracle; they differ only in the coarsest grid's summation order."""
+    the binary64 oracle; with two passes on every grid (the unfused cycle's only schedule) they
+    differ only in the coarsest grid's summation order, so their step counts agree."""
+    from cartogenesis_amd import CG_OPT_LEVEL_PASSES
     p = circulation_problem(rows, seed=5)
     ref64 = oracle.solve64([to_oracle(s) for s in p.levels], 1e-13)[0]
     out = {}
+    default = acc.get_option(CG_OPT_LEVEL_PASSES)
     for fused in (1, 0):
         acc.set_option(7, fused)
+        acc.set_option(CG_OPT_LEVEL_PASSES, 0x22222222)
         try:
             sol = acc.solve_levels(p.levels, None, 1e-11, True, CG_METHOD_REFINED)
         finally:
             acc.set_option(7, 1)
+            acc.set_option(CG_OPT_LEVEL_PASSES, default)
         out[fused] = sol
         assert sol.relative_residual < 1e-11
         assert rel_l2(sol.values64, ref64) <= PARITY_L2
@@ -213,12 +218,14 @@ def test_level_pass_schedules_reach_the_oracle(acc, schedule):
     from cartogenesis_amd import CG_OPT_LEVEL_PASSES
     p = circulation_problem(256, seed=0)
     ref64 = oracle.solve64([to_oracle(s) for s in p.levels], 1e-13)[0]
+    default = acc.get_option(CG_OPT_LEVEL_PASSES)
     acc.set_option(CG_OPT_LEVEL_PASSES, schedule)
     try:
         sol = acc.solve_levels(p.levels, None, 1e-12, True, CG_METHOD_REFINED)
-    finally:
         acc.set_option(CG_OPT_LEVEL_PASSES, 0x22222222)
-    base = acc.solve_levels(p.levels, None, 1e-12, True, CG_METHOD_REFINED)
+        base = acc.solve_levels(p.levels, None, 1e-12, True, CG_METHOD_REFINED)
+    finally:
+        acc.set_option(CG_OPT_LEVEL_PASSES, default)
     print(f"OCEAN schedule {schedule:#x}: {sol.cycles} cycles (V(2,2): {base.cycles})")
     assert sol.relative_residual < 1e-12
     assert rel_l2(sol.values64, ref64) <= PARITY_L2
@@ -252,11 +259,14 @@ def test_direct_bottom_and_cycled_bottom_reach_the_oracle(acc, rows):
     inverse (one chip-wide launch), or cycles that grid and the coarser ones in one workgroup. Both
     preconditioners leave the binary64 fixed point unchanged: each reaches the binary64 oracle, and
     the exact bottom solve needs no more cycles than the cycled one (one more allowed for the
-    rounding of the last step), fused and unfused alike."""
-    from cartogenesis_amd import CG_OPT_DIRECT_BOTTOM, CG_OPT_FUSED
+    rounding of the last step), fused and unfused alike. The fused cycle runs two passes on every
+    grid here, the unfused cycle's only schedule, so the two cycle counts are comparable."""
+    from cartogenesis_amd import CG_OPT_DIRECT_BOTTOM, CG_OPT_FUSED, CG_OPT_LEVEL_PASSES
     p = circulation_problem(rows, seed=6)
     ref64 = oracle.solve64([to_oracle(s) for s in p.levels], 1e-13)[0]
     cycles = {}
+    default = acc.get_option(CG_OPT_LEVEL_PASSES)
+    acc.set_option(CG_OPT_LEVEL_PASSES, 0x22222222)
     try:
         for fused in (1, 0):
             for direct in (1, 0):
@@ -270,6 +280,7 @@ def test_direct_bottom_and_cycled_bottom_reach_the_oracle(acc, rows):
     finally:
         acc.set_option(CG_OPT_FUSED, 1)
         acc.set_option(CG_OPT_DIRECT_BOTTOM, 1)
+        acc.set_option(CG_OPT_LEVEL_PASSES, default)
     for fused in (1, 0):
         assert cycles[fused, 1] <= cycles[fused, 0] + 1
     # The fused and unfused cycles share the direct bottom launch.
